@@ -1,0 +1,419 @@
+"""Pins the CPU oracle against the reference's own known-answer tests (test/*.cpp, transcribed in
+tests/golden/toy_example.json), against closed forms and invariants, and against the
+independent cross-check values in tests/golden/survey_crosschecks.json."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+ULP4 = 4  # EXPECT_FLOAT_EQ in googletest: within 4 float ULPs
+
+
+def float_eq(a, b, ulps=ULP4):
+    fa, fb = np.float32(a), np.float32(b)
+    return abs(int(fa.view(np.int32)) - int(fb.view(np.int32))) <= ulps
+
+
+def toy_graph(toy):
+    return O.Graph.from_edges(toy["num_vertices"], toy["edges"])
+
+
+def hlogf(x):
+    return O.logf(np.float32(x))
+
+
+def cell32(value, din, dout):
+    """FP32 cell entropy value * logf(value / (din * dout)) with host logf."""
+    value, din, dout = np.float32(value), np.float32(din), np.float32(dout)
+    if value == 0:
+        return np.float32(0.0)
+    return np.float32(value * np.float32(hlogf(np.float32(value / np.float32(din * dout)))))
+
+
+def mdl_from_assignment(toy_or_n, edges, assignment, B, exact_log=False):
+    """MDL from scratch in numpy: the block matrix is counted directly from the edge list."""
+    n = len(assignment)
+    M = np.zeros((B, B), np.int64)
+    seen = set()
+    for u, v in edges:
+        if (u, v) in seen:
+            continue
+        seen.add((u, v))
+        M[assignment[u], assignment[v]] += 1
+    E = len(seen)
+    dout = M.sum(1)
+    din = M.sum(0)
+    lp = 0.0
+    for i in range(B):
+        for j in range(B):
+            if M[i, j]:
+                ratio = M[i, j] / (float(dout[i]) * float(din[j]))
+                lp += M[i, j] * (math.log(ratio) if exact_log else float(hlogf(np.float32(ratio))))
+    x = B ** 2 / E
+    h = (1 + x) * math.log(1 + x) - x * math.log(x)
+    return E * h + n * math.log(B) - lp
+
+
+# ----------------------------------------------------------------------------- Philox
+def test_philox_known_answers():
+    # Random123 known-answer vectors for philox4x32-10 (kat_vectors).
+    assert list(O.philox(0, 0, 0, 0, 0)) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(O.philox(0xFFFFFFFFFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF, 0xFFFFFFFF)) == \
+        [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    key = 0xA4093822 | (0x299F31D0 << 32)
+    assert list(O.philox(key, 0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344)) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+# ----------------------------------------------------------------------------- graph + build
+def test_toy_graph_setup(toy):
+    g = toy_graph(toy)
+    assert g.n == 11 and g.E == toy["num_edges"]  # entropy_test.cpp:43-48
+
+
+def test_block_degrees_known_answer(toy):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    _, dout, din, deg = bm.state()
+    assert list(dout) == toy["block_degrees_out"]
+    assert list(din) == toy["block_degrees_in"]
+    assert list(deg) == toy["block_degrees"]
+
+
+def test_build_matches_direct_count_and_invariants():
+    rng = np.random.default_rng(5)
+    for trial in range(20):
+        n = int(rng.integers(2, 40))
+        m = int(rng.integers(0, 4 * n))
+        edges = [(int(a), int(b)) for a, b in rng.integers(0, n, size=(m, 2))]
+        g = O.Graph.from_edges(n, edges)
+        B = int(rng.integers(1, n + 1))
+        a = rng.integers(0, B, n).astype(np.int32)
+        bm = O.Blockmodel(g, B, a)
+        M = bm.dense()
+        ref = np.zeros((B, B), np.int64)
+        for u, v in set(edges):
+            ref[a[u], a[v]] += 1
+        assert (M == ref).all()
+        _, dout, din, deg = bm.state()
+        assert (dout == ref.sum(1)).all() and (din == ref.sum(0)).all()
+        assert (deg == dout + din - np.diag(ref)).all()
+        assert M.sum() == g.E
+
+
+# ----------------------------------------------------------------------------- MDL
+def test_mdl_known_answer(toy, crosschecks):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    mdl = bm.mdl()
+    assert float_eq(mdl, toy["entropy"])  # entropy_test.cpp:50-53
+    # The reference computes the log in FP32 (utils.hpp:145-151); an exact log gives ENTROPY.
+    exact = mdl_from_assignment(None, toy["edges"], toy["assignment"], 3, exact_log=True)
+    assert abs(exact - toy["entropy"]) < 1e-7
+    assert abs(mdl - crosschecks["toy_mdl_reference_rule"]) < 1e-7
+    assert abs(mdl - exact) > 5e-7  # the FP32 log rule is visible at this precision
+    assert mdl == pytest.approx(mdl_from_assignment(None, toy["edges"], toy["assignment"], 3), rel=1e-14)
+
+
+def test_null_models(toy):
+    nm = toy["null_models"]
+    assert float_eq(O.null_mdl_v1(50), nm["v1_E50"])
+    assert float_eq(O.null_mdl_v2(10, 50), nm["v2_N10_E50"])
+    assert float_eq(O.null_mdl_v1(int(5e12)), nm["v1_E5e12"])
+    assert float_eq(O.null_mdl_v2(int(1e12), int(5e12)), nm["v2_N1e12_E5e12"])
+
+
+# ----------------------------------------------------------------------------- vertex Delta
+def test_vertex_delta_known_answer(toy):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    mv = toy["move"]
+    d, sw = bm.vertex_delta(mv["vertex"], mv["proposal"])
+    assert len(d) == 6
+    for r, c, v in mv["delta"]:
+        assert d[(r, c)] == v
+    for r, c in mv["delta_absent"]:
+        assert (r, c) not in d
+    cx = toy["complex"]
+    bm6 = O.Blockmodel(g, cx["B"], cx["assignment"])
+    d6, _ = bm6.vertex_delta(cx["vertex"], cx["proposal"])
+    assert len(d6) == 6
+    for r, c, v in cx["delta"]:
+        assert d6[(r, c)] == v
+
+
+def test_vertex_delta_sums_to_zero(toy):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    for v in range(11):
+        for s in range(3):
+            if s == toy["assignment"][v]:
+                continue
+            d, _ = bm.vertex_delta(v, s)
+            assert sum(d.values()) == 0  # finetune_test.cpp:105-126
+
+
+# ----------------------------------------------------------------------------- delta MDL
+def _all_moves(toy):
+    yield 3, toy["assignment"]
+    yield 6, toy["complex"]["assignment"]
+
+
+def test_delta_mdl_equals_mdl_difference_for_known_move(toy, crosschecks):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    mv = toy["move"]
+    dS, H, _ = bm.eval_moves([mv["vertex"]], [mv["proposal"]], [0.5])
+    a2 = list(toy["assignment"])
+    a2[mv["vertex"]] = mv["proposal"]
+    after = O.Blockmodel(g, 3, a2).mdl()
+    assert float_eq(dS[0], after - bm.mdl())  # entropy_test.cpp:86-98
+    ref = crosschecks["moves"][0]
+    assert dS[0] == pytest.approx(ref["dS"], abs=2e-6)
+    assert H[0] == pytest.approx(ref["H"], abs=2e-7)
+
+
+def test_delta_mdl_closed_form_with_net_zero_double_count(toy, crosschecks):
+    """Reference delta MDL = exact MDL difference + the entries that a net-zero Delta key
+    contributes a second time in the row/column loops (entropy.cpp:317-401)."""
+    g = toy_graph(toy)
+    fired = 0
+    for B, assign in _all_moves(toy):
+        bm = O.Blockmodel(g, B, assign)
+        before = bm.mdl()
+        M = bm.dense()
+        _, dout, din, _ = bm.state()
+        for v in range(11):
+            r = assign[v]
+            for s in range(B):
+                if s == r:
+                    continue
+                dS, _, _ = bm.eval_moves([v], [s], [0.5])
+                a2 = list(assign)
+                a2[v] = s
+                exact = O.Blockmodel(g, B, a2).mdl() - before
+                d, self_w = bm.vertex_delta(v, s)
+                out_v = [w for (x, w) in set(map(tuple, toy["edges"])) if x == v]
+                k_out = len(out_v)
+                k_in = len([x for (x, w) in set(map(tuple, toy["edges"])) if w == v and x != v])
+                din2, dout2 = din.astype(np.int64).copy(), dout.astype(np.int64).copy()
+                din2[r] -= k_in + self_w
+                din2[s] += k_in + self_w
+                dout2[r] -= k_out
+                dout2[s] += k_out
+                extra = 0.0
+                for (i, j), dv in d.items():
+                    if dv != 0 or M[i, j] == 0:
+                        continue
+                    in_rows = i in (r, s)
+                    in_cols = j in (r, s) and i not in (r, s)
+                    if in_rows or in_cols:
+                        extra += float(cell32(M[i, j], din[j], dout[i])) - float(cell32(M[i, j], din2[j], dout2[i]))
+                        fired += 1
+                assert dS[0] == pytest.approx(exact + extra, abs=2e-5), (B, v, s)
+    assert fired > 0
+    for c in crosschecks["double_count_moves"]:
+        assign = toy["assignment"] if c["B"] == 3 else toy["complex"]["assignment"]
+        bm = O.Blockmodel(g, c["B"], assign)
+        dS, _, _ = bm.eval_moves([c["v"]], [c["s"]], [0.5])
+        assert dS[0] == pytest.approx(c["dS"], abs=2e-6)
+
+
+# ----------------------------------------------------------------------------- Hastings
+def test_hastings_cross_checks(toy, crosschecks):
+    g = toy_graph(toy)
+    for c in crosschecks["moves"]:
+        assign = toy["assignment"] if c["B"] == 3 else toy["complex"]["assignment"]
+        bm = O.Blockmodel(g, c["B"], assign)
+        dS, H, _ = bm.eval_moves([c["v"]], [c["s"]], [0.5])
+        assert H[0] == pytest.approx(c["H"], abs=2e-7)
+        assert dS[0] == pytest.approx(c["dS"], abs=2e-6)
+
+
+def test_hastings_reverse_move_is_reciprocal(toy):
+    """With exact block degrees and no self-loop, H(r->s) * H(s->r after the move) = 1."""
+    g = toy_graph(toy)
+    selfs = toy["self_edges"]
+    checked = 0
+    for B, assign in _all_moves(toy):
+        bm = O.Blockmodel(g, B, assign)
+        for v in range(11):
+            if selfs[v]:
+                continue
+            r = assign[v]
+            for s in range(B):
+                if s == r:
+                    continue
+                _, H, _ = bm.eval_moves([v], [s], [0.5])
+                a2 = list(assign)
+                a2[v] = s
+                _, H2, _ = O.Blockmodel(g, B, a2).eval_moves([v], [r], [0.5])
+                assert H[0] * H2[0] == pytest.approx(1.0, rel=1e-12)
+                checked += 1
+    assert checked > 50
+
+
+def test_accept_rule(toy):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    dS, H, _ = bm.eval_moves([7], [0], [0.0])
+    p = min(1.0, math.exp(-3.0 * dS[0]) * H[0])
+    _, _, acc_lo = bm.eval_moves([7, 7], [0, 0], [p * 0.999999, p])
+    _, _, acc_hi = bm.eval_moves([7], [0], [p * 1.000001])
+    assert list(acc_lo) == [1, 1] and list(acc_hi) == [0]
+
+
+# ----------------------------------------------------------------------------- move_vertex
+def test_move_vertex_matches_rebuild_and_degree_drift(toy, crosschecks):
+    g = toy_graph(toy)
+    cx = toy["complex"]
+    bm = O.Blockmodel(g, 6, cx["assignment"])
+    bm.apply_moves([cx["vertex"]], [cx["proposal"]])
+    a2 = list(cx["assignment"])
+    a2[cx["vertex"]] = cx["proposal"]
+    fresh = O.Blockmodel(g, 6, a2)
+    assert (bm.dense() == fresh.dense()).all()  # blockmodel_test.cpp:110-123
+    _, dout, din, deg = bm.state()
+    _, fdout, fdin, fdeg = fresh.state()
+    assert list(fdout) == cx["after_out"] and list(fdin) == cx["after_in"] and list(fdeg) == cx["after_total"]
+    assert (dout == fdout).all() and (din == fdin).all()
+    # blockmodel.cpp:348-359 recomputes deg from the updated diagonal plus the Delta diagonal.
+    q9 = crosschecks["q9_drift"]
+    assert deg[3] == q9["reference_deg3"]
+    d, _ = O.Blockmodel(g, 6, cx["assignment"]).vertex_delta(cx["vertex"], cx["proposal"])
+    for x in range(6):
+        drift = d.get((x, x), 0) if x in (cx["current"], cx["proposal"]) else 0
+        assert deg[x] == fdeg[x] - drift
+
+
+def test_apply_moves_matrix_equals_rebuild_random():
+    rng = np.random.default_rng(11)
+    for trial in range(10):
+        n = 60
+        edges = [(int(a), int(b)) for a, b in rng.integers(0, n, size=(300, 2))]
+        g = O.Graph.from_edges(n, edges)
+        B = 7
+        a = rng.integers(0, B, n).astype(np.int32)
+        bm = O.Blockmodel(g, B, a)
+        vs = rng.permutation(n)[:25].astype(np.int32)
+        ss = rng.integers(0, B, 25).astype(np.int32)
+        bm.apply_moves(vs, ss)
+        a2 = a.copy()
+        a2[vs] = ss
+        fresh = O.Blockmodel(g, B, a2)
+        assert (bm.dense() == fresh.dense()).all()
+        _, dout, din, _ = bm.state()
+        _, fdout, fdin, _ = fresh.state()
+        assert (dout == fdout).all() and (din == fdin).all()
+
+
+# ----------------------------------------------------------------------------- merges
+def test_merge_delta_known_answer(toy, crosschecks):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    mg = toy["merge"]
+    d = bm.merge_delta(mg["from"], mg["to"])
+    assert len(d) == 6
+    for r, c, v in mg["delta"]:
+        assert d[(r, c)] == v
+    for r, c in mg["delta_absent"]:
+        assert (r, c) not in d
+    merged = O.Blockmodel(g, 3, mg["merged_assignment"])
+    _, dout, din, deg = merged.state()
+    assert list(dout) == mg["new_out"] and list(din) == mg["new_in"] and list(deg) == mg["new_total"]
+    dS = bm.merge_delta_mdl(mg["from"], mg["to"])
+    assert float_eq(dS, merged.mdl() - bm.mdl())  # entropy_test.cpp:238-244
+    assert dS == pytest.approx(crosschecks["merge_0_to_1_dS"], abs=2e-6)
+
+
+def test_merge_delta_mdl_is_exact_for_all_pairs(toy):
+    g = toy_graph(toy)
+    for B, assign in _all_moves(toy):
+        bm = O.Blockmodel(g, B, assign)
+        before = bm.mdl()
+        for r, s in itertools.permutations(range(B), 2):
+            merged = [s if b == r else b for b in assign]
+            after = O.Blockmodel(g, B, merged).mdl()
+            assert bm.merge_delta_mdl(r, s) == pytest.approx(after - before, abs=2e-5)
+
+
+def test_carry_out_best_merges_hand_case():
+    # 8 vertices in 6 blocks; sorted order by dS: 4, 1, 0, 5, 2, 3 (ties broken as std::sort).
+    n = 8
+    edges = [(i, (i + 1) % n) for i in range(n)]
+    g = O.Graph.from_edges(n, edges)
+    a = np.array([0, 0, 1, 2, 3, 4, 5, 5], np.int32)
+    bm = O.Blockmodel(g, 6, a)
+    bm.num_to_merge = 3
+    dS = np.array([0.3, 0.2, 0.5, 0.9, 0.1, 0.4])
+    best = np.array([1, 0, 3, 2, 1, 4], np.int32)
+    bm.carry_out(dS, best)
+    # 4 -> 1 (map[4]=1); 1 -> map[0]=0 (blocks 1,4 -> 0); 0 -> map[best[0]=1]=0: skip;
+    # 5 -> map[4]=0: merge 5 -> 0. Three merges; survivors {0, 2, 3} -> {0, 1, 2}.
+    aa, _, _, _ = bm.state()
+    assert bm.B == 3
+    assert list(aa) == [0, 0, 0, 1, 2, 0, 0, 0]
+
+
+# ----------------------------------------------------------------------------- proposals
+def test_vertex_proposal_distribution(toy):
+    """s ~ sum over neighbour slots (1/k) (M[t,s] + M[s,t]) / (dout[t] + din[t])."""
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 6, toy["complex"]["assignment"])
+    M = bm.dense().astype(float)
+    a, dout, din, _ = bm.state()
+    edges = sorted(set(map(tuple, toy["edges"])))
+    for v in (3, 7, 10):
+        nbrs = [w for (x, w) in edges if x == v] + [x for (x, w) in edges if w == v and x != v]
+        p = np.zeros(6)
+        for w in nbrs:
+            t = a[w]
+            p += (M[t, :] + M[:, t]) / (dout[t] + din[t]) / len(nbrs)
+        n = 20000
+        s, _ = bm.propose(np.full(n, v, np.int32), seed=123, call=0)
+        # identical vertex => identical draw; vary the call id instead
+        s = np.array([bm.propose([v], seed=123, call=c)[0][0] for c in range(n)])
+        counts = np.bincount(s, minlength=6)
+        expected = p * n
+        mask = expected > 0
+        assert counts[~mask].sum() == 0
+        chi2 = ((counts[mask] - expected[mask]) ** 2 / expected[mask]).sum()
+        assert chi2 < 30.0, (v, counts, expected)
+
+
+def test_merge_proposals_are_in_support(toy):
+    g = toy_graph(toy)
+    bm = O.Blockmodel(g, 6, toy["complex"]["assignment"])
+    dS, best = bm.merge_proposals(seed=7, call=0)
+    M = bm.dense()
+    for r in range(6):
+        assert best[r] != r and 0 <= best[r] < 6
+        assert dS[r] == pytest.approx(bm.merge_delta_mdl(r, int(best[r])), abs=1e-12)
+
+
+# ----------------------------------------------------------------------------- driver
+def test_sbp_end_to_end_small(toy):
+    rng = np.random.default_rng(3)
+    n, K = 120, 3
+    truth = rng.integers(0, K, n)
+    edges = []
+    for u in range(n):
+        for _ in range(8):
+            if rng.random() < 0.85:
+                cand = np.flatnonzero(truth == truth[u])
+            else:
+                cand = np.flatnonzero(truth != truth[u])
+            v = int(rng.choice(cand))
+            if v != u:
+                edges.append((u, v))
+    g = O.Graph.from_edges(n, edges)
+    for algo in (0, 1, 2):
+        a, run = O.sbp(g, algo, 1, True, 42)
+        a2, run2 = O.sbp(g, algo, 1, True, 42)
+        assert (a == a2).all() and run.B == run2.B  # deterministic given the seed
+        assert 1 <= run.B <= n
+        initial = O.Blockmodel(g, n).mdl()
+        assert run.mdl < initial

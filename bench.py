@@ -1,0 +1,173 @@
+"""Benchmark: stochastic block partitioning on one MI355X.
+
+A step is one complete partitioning run (sbp::stochastic_block_partition: block merges, MCMC
+sweeps, golden-section search over the number of blocks) of a synthetic DC-SBM graph that is
+already resident in HBM. value = vertex-move evaluations per second over the timed steps
+(proposals with s != r whose delta MDL, Hastings correction and accept were computed);
+ms_per_step = time to partition. The roofline object covers the dominant kernel, the
+vertex-move evaluation kernel for degree <= 127 (one wavefront per vertex), with its
+algorithmic bytes counted on the device in an untimed run of the same seed and its time taken
+from HIP events on the library's stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "vertex-move evals/s + time-to-partition at equal MDL, learned vs plain, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "sbpx_amd", os.path.join(ROOT, "hybrid-stochastic-block-partitioning_amd", "__init__.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["sbpx_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(arrays, seconds, seed):
+    """The CPU oracle (C++/OpenMP restatement of the reference) on the host cores: one merge phase
+    from the identity partition, then asynchronous Gibbs sweeps (batches=1) for about `seconds`."""
+    import oracle_lib as O
+
+    n = len(arrays[0]) - 1
+    g = O.Graph(n, *arrays)
+    bm = O.Blockmodel(g, n)
+    t0 = time.perf_counter()
+    bm.merge_blocks(True, seed, 0)
+    t_merge = time.perf_counter() - t0
+    evals = 0
+    sweeps = 0
+    t1 = time.perf_counter()
+    while time.perf_counter() - t1 < seconds and sweeps < 100:
+        st = bm.async_sweep(1, seed, sweeps)
+        evals += st.evals
+        sweeps += 1
+    t_sweeps = time.perf_counter() - t1
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": evals / t_sweeps,
+        "unit": "vertex-move evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle async-Gibbs sweeps on the same graph after one merge phase from the identity "
+                   f"partition (B={n}->{bm.B}, merge {t_merge:.1f} s untimed); {sweeps} sweeps, {evals} evals "
+                   f"in {t_sweeps:.1f} s, OpenMP threads={threads}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--algorithm", default="async_gibbs")
+    ap.add_argument("--batches", type=int, default=1)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described, not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    m = load_pkg()
+    cfg = m.CONFIGS[args.config]
+    out_ptr, out_idx, in_ptr, in_idx, truth = m.generate_dcsbm(**cfg, seed=args.seed)
+    arrays = (out_ptr, out_idx, in_ptr, in_idx)
+    E = int(out_ptr[-1])
+    print(f"# graph {args.config}: N={cfg['num_vertices']} E={E}", file=sys.stderr, flush=True)
+    g = m.ResidentGraph(*arrays)
+    kw = dict(algorithm=args.algorithm, batches=args.batches, approximate=True, seed=args.seed)
+    for w in range(args.warmup):
+        _, st = g.partition(**kw)
+        print(f"# warmup {w}: {st.seconds:.3f} s, B={st.num_blocks}", file=sys.stderr, flush=True)
+    times, evals, attempts, sweeps = [], 0, 0, 0
+    ms_bin0, launches_bin0 = 0.0, 0
+    last = None
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        a, st = g.partition(**kw)
+        times.append(time.perf_counter() - t0)
+        evals += st.evals
+        attempts += st.attempts
+        sweeps += st.sweeps
+        ms_bin0 += st.eval_ms[0]
+        launches_bin0 += st.eval_launches[0]
+        last = (a, st)
+        print(f"# step {k}: {times[-1]:.3f} s, B={st.num_blocks}, mdl={st.mdl:.1f}, evals={st.evals}",
+              file=sys.stderr, flush=True)
+    total = time.perf_counter() - t_all
+    # untimed counting run of the same seed: algorithmic bytes of the evaluation kernels
+    _, cst = g.partition(count_bytes=True, **kw)
+    bytes_bin0_per_run = cst.eval_bytes[0]
+    ms_bin0_per_run = ms_bin0 / args.steps
+    launches_per_run = launches_bin0 / args.steps
+    achieved = bytes_bin0_per_run / (ms_bin0_per_run / 1000.0) / 1e9 if ms_bin0_per_run > 0 else 0.0
+    a, st = last
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "sbpx_eval", os.path.join(ROOT, "hybrid-stochastic-block-partitioning_amd", "evaluation.py"))
+    ev = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ev)
+    nmi = ev.nmi(truth, a)
+    modularity = ev.modularity(out_ptr, out_idx, in_ptr, a)
+    modularity_planted = ev.modularity(out_ptr, out_idx, in_ptr, truth)
+    out = {
+        "metric": METRIC,
+        "value": evals / total,
+        "unit": "vertex-move evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * float(np.mean(times)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 state, f32 cells, f64 sums",
+        "data": "synthetic DC-SBM (seeded generator)",
+        "config": {"workload": f"{args.config} plain {args.algorithm} SBP, batches={args.batches}, --approximate",
+                   "num_vertices": cfg["num_vertices"], "num_edges": E, "num_blocks_planted": cfg["num_blocks"],
+                   "p_in": cfg["p_in"], "dirichlet": cfg["dirichlet"], "deg": [cfg["deg_min"], cfg["deg_max"]]},
+        "time_to_partition_s": float(np.mean(times)),
+        "attempts_per_s": attempts / total,
+        "sweeps_per_s": sweeps / total,
+        "final_mdl": st.mdl,
+        "final_num_blocks": st.num_blocks,
+        "nmi_vs_planted": nmi,
+        "modularity": modularity,
+        "modularity_planted": modularity_planted,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "eval_small_kernel (vertex-move evaluation, degree <= 127)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_run": bytes_bin0_per_run,
+            "ms_per_run": ms_bin0_per_run,
+            "launches_per_run": launches_per_run,
+            "avg_launch_us": 1000.0 * ms_bin0_per_run / max(1.0, launches_per_run),
+        },
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(arrays, args.cpu_seconds, args.seed)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

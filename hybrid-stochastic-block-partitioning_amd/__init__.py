@@ -1,0 +1,314 @@
+"""Python access to the MI355X stochastic-block-partitioning library (libsbpx.so, C ABI in
+include/sbpx.h) and to the C++ driver (libsbp_host.so). The compute path is the HIP library;
+this module only moves host arrays in and out. It raises if the libraries are not built."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsbpx.so")
+HOST_LIB_PATH = os.path.join(HERE, "libsbp_host.so")
+
+ALGO_MH, ALGO_ASYNC_GIBBS, ALGO_HYBRID = 0, 1, 2
+MERGE_APPROXIMATE, MERGE_ADVANCED = 0, 1
+ALGORITHMS = {"metropolis_hastings": ALGO_MH, "async_gibbs": ALGO_ASYNC_GIBBS, "hybrid_mcmc": ALGO_HYBRID}
+
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+# Synthetic DC-SBM shapes (num_vertices, num_blocks, p_in, dirichlet, deg_min, deg_max, exponent).
+CONFIGS = {
+    "C1": dict(num_vertices=1000, num_blocks=8, p_in=0.8, dirichlet=20.0, deg_min=10, deg_max=100, exponent=2.1),
+    "C2": dict(num_vertices=50000, num_blocks=44, p_in=0.6, dirichlet=1.0, deg_min=10, deg_max=100, exponent=2.1),
+    "C4": dict(num_vertices=1000000, num_blocks=200, p_in=0.6, dirichlet=1.0, deg_min=10, deg_max=200, exponent=2.1),
+    "C5": dict(num_vertices=5000000, num_blocks=400, p_in=0.55, dirichlet=1.0, deg_min=5, deg_max=500, exponent=2.1),
+}
+
+
+class SweepStats(C.Structure):
+    _fields_ = [("attempts", C.c_int64), ("evals", C.c_int64), ("moves", C.c_int64), ("dS_accepted", C.c_double),
+                ("eval_ms", C.c_double * 3), ("eval_bytes", C.c_double * 3), ("eval_launches", C.c_int32 * 3),
+                ("reserved", C.c_int32)]
+
+
+class DcsbmParams(C.Structure):
+    _fields_ = [("num_vertices", C.c_int32), ("num_blocks", C.c_int32), ("p_in", C.c_double),
+                ("dirichlet", C.c_double), ("deg_min", C.c_int32), ("deg_max", C.c_int32),
+                ("exponent", C.c_double), ("seed", C.c_uint64)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("num_blocks", C.c_int32), ("outer_iterations", C.c_int32), ("sweeps", C.c_int64),
+                ("attempts", C.c_int64), ("evals", C.c_int64), ("moves", C.c_int64), ("seconds", C.c_double),
+                ("mcmc_seconds", C.c_double), ("eval_ms", C.c_double * 3), ("eval_bytes", C.c_double * 3),
+                ("eval_launches", C.c_int64 * 3), ("mdl", C.c_double)]
+
+
+_lib = None
+_host = None
+
+
+def load():
+    """The HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with build.py (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.sbpx_create.argtypes = [C.POINTER(vp), C.c_int]
+    L.sbpx_destroy.argtypes = [vp]
+    L.sbpx_destroy.restype = None
+    L.sbpx_last_error.argtypes = [vp]
+    L.sbpx_last_error.restype = C.c_char_p
+    L.sbpx_set_option.argtypes = [vp, C.c_int, C.c_int64]
+    L.sbpx_graph_upload.argtypes = [vp, C.c_int32, C.c_int64, i32p, i32p, i32p, i32p]
+    L.sbpx_graph_split.argtypes = [vp, vp, C.POINTER(C.c_int32), vp]
+    L.sbpx_blockmodel_build.argtypes = [vp, C.c_int32, i32p]
+    L.sbpx_blockmodel_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+    L.sbpx_blockmodel_download.argtypes = [vp, vp, vp, vp, vp]
+    L.sbpx_blockmodel_coo.argtypes = [vp, C.c_int, i32p, i32p, i32p]
+    L.sbpx_mdl.argtypes = [vp, C.POINTER(C.c_double)]
+    L.sbpx_log_posterior.argtypes = [vp, C.POINTER(C.c_double)]
+    L.sbpx_eval_moves.argtypes = [vp, C.c_int32, i32p, i32p, f64p, f64p, f64p, u8p]
+    L.sbpx_propose.argtypes = [vp, C.c_int32, i32p, C.c_uint64, C.c_uint32, C.c_int, i32p, f64p]
+    L.sbpx_apply_moves.argtypes = [vp, C.c_int32, i32p, i32p]
+    L.sbpx_sweep.argtypes = [vp, C.c_int, C.c_int32, C.c_uint64, C.c_uint32, C.POINTER(SweepStats)]
+    L.sbpx_merge_proposals.argtypes = [vp, C.c_uint64, C.c_uint32, f64p, i32p]
+    L.sbpx_carry_out_merges.argtypes = [vp, C.c_int32, C.c_int, f64p, i32p]
+    L.sbpx_merge_blocks.argtypes = [vp, C.c_int32, C.c_int, C.c_uint64, C.c_uint32]
+    L.sbpx_selftest_logf.argtypes = [vp, C.c_uint32, C.c_uint32, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]
+    L.sbpx_generate_dcsbm.argtypes = [C.POINTER(DcsbmParams), C.POINTER(C.c_int64), vp, vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def load_host():
+    """The C++ driver library (reference API on top of the C ABI)."""
+    global _host
+    if _host is not None:
+        return _host
+    load()
+    if not os.path.exists(HOST_LIB_PATH):
+        raise RuntimeError(f"{HOST_LIB_PATH} is missing: build it with build.py")
+    H = C.CDLL(HOST_LIB_PATH)
+    H.sbp_run.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int,
+                          C.c_int, i32p, C.POINTER(RunStats), C.c_char_p, C.c_int]
+    H.sbp_graph_new.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_char_p, C.c_int]
+    H.sbp_graph_new.restype = C.c_void_p
+    H.sbp_graph_free.argtypes = [C.c_void_p]
+    H.sbp_graph_free.restype = None
+    H.sbp_partition.argtypes = [C.c_void_p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int, i32p,
+                                C.POINTER(RunStats), C.c_char_p, C.c_int]
+    _host = H
+    return H
+
+
+EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option", "sbpx_graph_upload",
+            "sbpx_graph_split", "sbpx_blockmodel_build", "sbpx_blockmodel_info", "sbpx_blockmodel_download",
+            "sbpx_blockmodel_coo", "sbpx_mdl", "sbpx_log_posterior", "sbpx_eval_moves", "sbpx_propose",
+            "sbpx_apply_moves", "sbpx_sweep", "sbpx_merge_proposals", "sbpx_carry_out_merges",
+            "sbpx_merge_blocks", "sbpx_selftest_logf", "sbpx_generate_dcsbm"]
+
+
+class SbpxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"sbpx status {status}: {msg}")
+        self.status = status
+
+
+def generate_dcsbm(num_vertices, num_blocks, p_in, dirichlet, deg_min, deg_max, exponent, seed=0):
+    """Synthetic directed DC-SBM graph: (out_ptr, out_idx, in_ptr, in_idx, truth)."""
+    L = load()
+    p = DcsbmParams(num_vertices, num_blocks, p_in, dirichlet, deg_min, deg_max, exponent, seed)
+    E = C.c_int64()
+    truth = np.zeros(num_vertices, np.int32)
+    st = L.sbpx_generate_dcsbm(C.byref(p), C.byref(E), None, None, None, None, truth.ctypes.data)
+    if st != 0:
+        raise SbpxError(st, "bad generator parameters")
+    out_ptr = np.zeros(num_vertices + 1, np.int32)
+    in_ptr = np.zeros(num_vertices + 1, np.int32)
+    out_idx = np.zeros(E.value, np.int32)
+    in_idx = np.zeros(E.value, np.int32)
+    st = L.sbpx_generate_dcsbm(C.byref(p), C.byref(E), out_ptr.ctypes.data, out_idx.ctypes.data, in_ptr.ctypes.data,
+                               in_idx.ctypes.data, truth.ctypes.data)
+    if st != 0:
+        raise SbpxError(st, "generator failed")
+    return out_ptr, out_idx, in_ptr, in_idx, truth
+
+
+class Context:
+    """One GPU context holding a graph and its current Blockmodel."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        h = C.c_void_p()
+        st = self.L.sbpx_create(C.byref(h), device)
+        if st != 0:
+            raise SbpxError(st, "sbpx_create failed (no GPU?)")
+        self.h = h
+        self.N = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbpx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st):
+        if st != 0:
+            raise SbpxError(st, self.L.sbpx_last_error(self.h).decode())
+
+    def set_option(self, option, value):
+        self._chk(self.L.sbpx_set_option(self.h, option, value))
+
+    def upload_graph(self, out_ptr, out_idx, in_ptr, in_idx):
+        out_ptr, out_idx, in_ptr, in_idx = (np.ascontiguousarray(x, np.int32) for x in (out_ptr, out_idx, in_ptr, in_idx))
+        self.N = len(out_ptr) - 1
+        self._chk(self.L.sbpx_graph_upload(self.h, self.N, len(out_idx), out_ptr, out_idx, in_ptr, in_idx))
+
+    def split(self):
+        nh = C.c_int32()
+        self._chk(self.L.sbpx_graph_split(self.h, None, C.byref(nh), None))
+        high = np.zeros(nh.value, np.int32)
+        low = np.zeros(self.N - nh.value, np.int32)
+        self._chk(self.L.sbpx_graph_split(self.h, high.ctypes.data, C.byref(nh), low.ctypes.data))
+        return high, low
+
+    def build(self, B, assignment):
+        self._chk(self.L.sbpx_blockmodel_build(self.h, int(B), np.ascontiguousarray(assignment, np.int32)))
+
+    def info(self):
+        B = C.c_int32()
+        nnz = C.c_int64()
+        self._chk(self.L.sbpx_blockmodel_info(self.h, C.byref(B), C.byref(nnz)))
+        return B.value, nnz.value
+
+    def state(self):
+        B, _ = self.info()
+        a = np.zeros(self.N, np.int32)
+        dout = np.zeros(B, np.int32)
+        din = np.zeros(B, np.int32)
+        deg = np.zeros(B, np.int32)
+        self._chk(self.L.sbpx_blockmodel_download(self.h, a.ctypes.data, dout.ctypes.data, din.ctypes.data,
+                                                  deg.ctypes.data))
+        return a, dout, din, deg
+
+    def coo(self, by_column=False):
+        _, nnz = self.info()
+        r = np.zeros(nnz, np.int32)
+        c = np.zeros(nnz, np.int32)
+        v = np.zeros(nnz, np.int32)
+        self._chk(self.L.sbpx_blockmodel_coo(self.h, int(by_column), r, c, v))
+        return r, c, v
+
+    def mdl(self):
+        out = C.c_double()
+        self._chk(self.L.sbpx_mdl(self.h, C.byref(out)))
+        return out.value
+
+    def log_posterior(self):
+        out = C.c_double()
+        self._chk(self.L.sbpx_log_posterior(self.h, C.byref(out)))
+        return out.value
+
+    def eval_moves(self, v, s, u):
+        v = np.ascontiguousarray(v, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        u = np.ascontiguousarray(u, np.float64)
+        n = len(v)
+        dS = np.zeros(n)
+        H = np.zeros(n)
+        acc = np.zeros(n, np.uint8)
+        self._chk(self.L.sbpx_eval_moves(self.h, n, v, s, u, dS, H, acc))
+        return dS, H, acc
+
+    def propose(self, v, seed, call, tag=0):
+        v = np.ascontiguousarray(v, np.int32)
+        s = np.zeros(len(v), np.int32)
+        u = np.zeros(len(v))
+        self._chk(self.L.sbpx_propose(self.h, len(v), v, seed, call, tag, s, u))
+        return s, u
+
+    def apply_moves(self, v, s):
+        self._chk(self.L.sbpx_apply_moves(self.h, len(v), np.ascontiguousarray(v, np.int32),
+                                          np.ascontiguousarray(s, np.int32)))
+
+    def sweep(self, algo, batches, seed, call):
+        st = SweepStats()
+        self._chk(self.L.sbpx_sweep(self.h, algo, batches, seed, call, C.byref(st)))
+        return st
+
+    def merge_proposals(self, seed, call):
+        B, _ = self.info()
+        dS = np.zeros(B)
+        best = np.zeros(B, np.int32)
+        self._chk(self.L.sbpx_merge_proposals(self.h, seed, call, dS, best))
+        return dS, best
+
+    def carry_out_merges(self, num_to_merge, dS, best, mode=MERGE_APPROXIMATE):
+        self._chk(self.L.sbpx_carry_out_merges(self.h, int(num_to_merge), mode, np.ascontiguousarray(dS, np.float64),
+                                               np.ascontiguousarray(best, np.int32)))
+
+    def selftest_logf(self, lo, n):
+        out = np.zeros(n, np.float32)
+        self._chk(self.L.sbpx_selftest_logf(self.h, lo, n, out))
+        return out
+
+    def merge_blocks(self, num_to_merge, seed, call, mode=MERGE_APPROXIMATE):
+        self._chk(self.L.sbpx_merge_blocks(self.h, int(num_to_merge), mode, seed, call))
+
+
+def run_sbp(out_ptr, out_idx, in_ptr, in_idx, algorithm="async_gibbs", batches=1, approximate=True, seed=0,
+            device=0, count_bytes=False):
+    """sbp::stochastic_block_partition through the C++ driver; returns (assignment, RunStats)."""
+    H = load_host()
+    N = len(out_ptr) - 1
+    a = np.zeros(N, np.int32)
+    st = RunStats()
+    err = C.create_string_buffer(512)
+    algo = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    rc = H.sbp_run(N, np.ascontiguousarray(out_ptr, np.int32), np.ascontiguousarray(out_idx, np.int32),
+                   np.ascontiguousarray(in_ptr, np.int32), np.ascontiguousarray(in_idx, np.int32), algo, batches,
+                   int(approximate), seed, device, int(count_bytes), a, C.byref(st), err, 512)
+    if rc != 0:
+        raise RuntimeError("sbp_run: " + err.value.decode())
+    return a, st
+
+
+class ResidentGraph:
+    """A graph held on one GPU by the C++ driver, for repeated partitioning runs."""
+
+    def __init__(self, out_ptr, out_idx, in_ptr, in_idx, device=0):
+        self.H = load_host()
+        self.N = len(out_ptr) - 1
+        err = C.create_string_buffer(512)
+        self.h = self.H.sbp_graph_new(self.N, np.ascontiguousarray(out_ptr, np.int32),
+                                      np.ascontiguousarray(out_idx, np.int32), np.ascontiguousarray(in_ptr, np.int32),
+                                      np.ascontiguousarray(in_idx, np.int32), device, err, 512)
+        if not self.h:
+            raise RuntimeError("sbp_graph_new: " + err.value.decode())
+
+    def partition(self, algorithm="async_gibbs", batches=1, approximate=True, seed=0, count_bytes=False):
+        a = np.zeros(self.N, np.int32)
+        st = RunStats()
+        err = C.create_string_buffer(512)
+        algo = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        rc = self.H.sbp_partition(self.h, algo, batches, int(approximate), seed, int(count_bytes), a, C.byref(st),
+                                  err, 512)
+        if rc != 0:
+            raise RuntimeError("sbp_partition: " + err.value.decode())
+        return a, st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.H.sbp_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

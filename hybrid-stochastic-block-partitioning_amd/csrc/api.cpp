@@ -1,0 +1,848 @@
+// C ABI (include/sbpx.h) over the device kernels.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "internal.hpp"
+
+using namespace sbpx;
+
+struct sbpx_ctx {
+    Ctx c;
+    bool count_bytes = false;
+    std::vector<hipEvent_t> events;
+};
+
+namespace {
+
+template <typename F>
+sbpx_status guarded(sbpx_ctx* ctx, F&& f) {
+    if (!ctx) return SBPX_ERR_ARG;
+    try {
+        f();
+        return SBPX_OK;
+    } catch (const ArgError& e) {
+        ctx->c.err = e.what();
+        return SBPX_ERR_ARG;
+    } catch (const NumericError& e) {
+        ctx->c.err = e.what();
+        return SBPX_ERR_NUMERIC;
+    } catch (const StateError& e) {
+        ctx->c.err = e.what();
+        return SBPX_ERR_STATE;
+    } catch (const HipError& e) {
+        ctx->c.err = e.what();
+        return SBPX_ERR_HIP;
+    } catch (const std::exception& e) {
+        ctx->c.err = e.what();
+        return SBPX_ERR_UNSUPPORTED;
+    }
+}
+
+template <typename T>
+void dalloc(T** p, size_t n) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    SBPX_HIP(hipMalloc((void**)p, std::max<size_t>(1, n) * sizeof(T)));
+}
+template <typename T>
+void dfree(T** p) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+}
+
+void free_graph_buffers(Ctx& c) {
+    dfree(&c.g.out_ptr);
+    dfree(&c.g.out_idx);
+    dfree(&c.g.in_ptr);
+    dfree(&c.g.in_idx);
+    dfree(&c.g.edge_src);
+    DevBlockmodel& b = c.bm;
+    dfree(&b.assign);
+    dfree(&b.row_ptr);
+    dfree(&b.row_col);
+    dfree(&b.row_val);
+    dfree(&b.row_cum);
+    dfree(&b.coo_row);
+    dfree(&b.col_ptr);
+    dfree(&b.col_row);
+    dfree(&b.col_val);
+    dfree(&b.col_cum);
+    dfree(&b.dout);
+    dfree(&b.din);
+    dfree(&b.deg);
+    dfree(&b.diag);
+    dfree(&b.drift);
+    dfree(&c.keys0);
+    dfree(&c.keys1);
+    dfree(&c.vals0);
+    dfree(&c.vals1);
+    dfree(&c.flags);
+    dfree(&c.pos);
+    dfree(&c.hist);
+    dfree(&c.scan_tmp);
+    dfree(&c.tmp_b);
+    dfree(&c.ev_s);
+    dfree(&c.ev_acc);
+    dfree(&c.ev_dS);
+    dfree(&c.ev_H);
+    dfree(&c.ev_u);
+    dfree(&c.moved_epoch);
+    dfree(&c.new_block);
+    dfree(&c.last_touch);
+    dfree(&c.m_dS);
+    dfree(&c.m_best);
+    dfree(&c.m_map);
+    dfree(&c.m_count);
+    dfree(&c.list_buf);
+    dfree(&c.aux_buf);
+    dfree(&c.plan.d_lists);
+    dfree(&c.plan.d_huge_offsets);
+    dfree(&c.table_pool);
+    c.table_pool_slots = 0;
+    MhState& h = c.mh;
+    for (int** p : {&h.rbase, &h.cbase, &h.rkeys, &h.rvals, &h.ckeys, &h.cvals, &h.rocc, &h.cocc, &h.rneed,
+                    &h.cneed, &h.tables, &h.status, &h.d_all, &h.d_high})
+        dfree(p);
+    h = MhState();
+    c.plan = SweepPlan();
+    c.bm = DevBlockmodel();
+}
+
+int bin_of(int degree) {
+    if (degree <= BIN_SMALL_MAXDEG) return 0;
+    if (degree <= BIN_LARGE_MAXDEG) return 1;
+    return 2;
+}
+int table_cap(int degree) {
+    int cap = 16;
+    while (cap < 2 * (degree + 2)) cap <<= 1;
+    return cap;
+}
+
+void ensure_pool(Ctx& c, size_t slots) {
+    if (slots <= c.table_pool_slots) return;
+    dalloc(&c.table_pool, slots);
+    c.table_pool_slots = slots;
+}
+
+// Orders `vertices` into bins [small | large | huge]; returns counts and, for huge vertices,
+// table offsets (7 * cap ints each) starting at 0.
+void bin_vertices(const Ctx& c, const int* vertices, int n, std::vector<int>& out, int counts[NUM_BINS],
+                  std::vector<int>& huge_off, std::vector<int>* perm = nullptr) {
+    out.clear();
+    out.reserve(n);
+    if (perm) perm->clear();
+    huge_off.assign(1, 0);
+    for (int b = 0; b < NUM_BINS; ++b) {
+        counts[b] = 0;
+        for (int i = 0; i < n; ++i) {
+            int v = vertices[i];
+            if (bin_of(c.g.degree[v]) != b) continue;
+            out.push_back(v);
+            if (perm) perm->push_back(i);
+            counts[b]++;
+            if (b == 2) huge_off.push_back(huge_off.back() + 7 * table_cap(c.g.degree[v]));
+        }
+    }
+}
+
+void require_graph(Ctx& c) {
+    if (c.g.N <= 0) throw StateError("no graph uploaded");
+}
+void require_bm(Ctx& c) {
+    require_graph(c);
+    if (!c.bm.valid) throw StateError("no blockmodel built");
+}
+
+double model_term(long N, long E, int B) {
+    // entropy.cpp:584-589 without the log posterior
+    double x = pow(B, 2) / E;
+    double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    return (E * h) + (N * log(B));
+}
+
+// Evaluates a binned vertex list (each wave one vertex) with outputs at the vertex index. Each
+// bin's launch is bracketed by events when timed.
+void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, const int* d_huge_off,
+                      uint64_t seed, uint32_t call, uint32_t tag, long long* bytes, bool timed) {
+    Ctx& c = ctx->c;
+    for (int b = 0; b < NUM_BINS; ++b) {
+        if (bl.count[b] == 0) continue;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timed) {
+            size_t k = c.timed.size() * 2;
+            while (ctx->events.size() < k + 2) {
+                hipEvent_t e;
+                SBPX_HIP(hipEventCreate(&e));
+                ctx->events.push_back(e);
+            }
+            e0 = ctx->events[k];
+            e1 = ctx->events[k + 1];
+            SBPX_HIP(hipEventRecord(e0, c.stream));
+        }
+        EvalLaunch L{};
+        L.list = d_list + bl.offset[b];
+        L.n = bl.count[b];
+        L.bin = b;
+        L.seed = seed;
+        L.call = call;
+        L.tag = tag;
+        L.out_by_vertex = 1;
+        L.huge_offsets = d_huge_off;
+        L.bytes = bytes ? bytes + b : nullptr;
+        launch_eval(c, L);
+        if (timed) {
+            SBPX_HIP(hipEventRecord(e1, c.stream));
+            c.timed.push_back(b);
+        }
+    }
+}
+
+void build_plan(Ctx& c, int algo, int batches) {
+    if (c.plan.algo == algo && c.plan.batches == batches) return;
+    SweepPlan& p = c.plan;
+    dfree(&p.d_lists);
+    dfree(&p.d_huge_offsets);
+    p.batches_async.clear();
+    p.algo = algo;
+    p.batches = batches;
+    const int N = c.g.N;
+    std::vector<int> base;
+    if (algo == SBPX_ALGO_HYBRID) {
+        base = c.g.low;
+    } else {
+        base.resize(N);
+        std::iota(base.begin(), base.end(), 0);
+    }
+    std::vector<int> lists;
+    std::vector<int> huge_all;
+    size_t pool_need = 0;
+    const int n = (int)base.size();
+    if (n > 0 && algo != SBPX_ALGO_METROPOLIS_HASTINGS) {
+        // finetune.cpp:43-47, 352-366: batch_size = ceil(n / batches), n / batch_size batches
+        double num_batches = batches;
+        int batch_size = int(ceil(n / num_batches));
+        int nb = n / batch_size;
+        for (int b = 0; b < nb; ++b) {
+            int start = b * batch_size;
+            int end = std::min(n, (b + 1) * batch_size);
+            BinnedList bl;
+            std::vector<int> ordered, hoff;
+            bin_vertices(c, base.data() + start, end - start, ordered, bl.count, hoff);
+            int off = (int)lists.size();
+            bl.offset[0] = off;
+            bl.offset[1] = off + bl.count[0];
+            bl.offset[2] = off + bl.count[0] + bl.count[1];
+            bl.total = end - start;
+            bl.huge_base = (int)huge_all.size();
+            lists.insert(lists.end(), ordered.begin(), ordered.end());
+            huge_all.insert(huge_all.end(), hoff.begin(), hoff.end());
+            pool_need = std::max(pool_need, (size_t)hoff.back());
+            p.batches_async.push_back(bl);
+        }
+    }
+    dalloc(&p.d_lists, lists.size());
+    if (!lists.empty())
+        SBPX_HIP(hipMemcpy(p.d_lists, lists.data(), sizeof(int) * lists.size(), hipMemcpyHostToDevice));
+    dalloc(&p.d_huge_offsets, huge_all.size());
+    if (!huge_all.empty())
+        SBPX_HIP(hipMemcpy(p.d_huge_offsets, huge_all.data(), sizeof(int) * huge_all.size(),
+                           hipMemcpyHostToDevice));
+    ensure_pool(c, pool_need);
+}
+
+}  // namespace
+
+extern "C" {
+
+sbpx_status sbpx_create(sbpx_ctx** out, int device) {
+    if (!out) return SBPX_ERR_ARG;
+    *out = nullptr;
+    sbpx_ctx* ctx = new sbpx_ctx();
+    sbpx_status st = guarded(ctx, [&] {
+        int n = 0;
+        SBPX_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw ArgError("no such device");
+        ctx->c.device = device;
+        SBPX_HIP(hipSetDevice(device));
+        SBPX_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        dalloc(&ctx->c.dev_err, 4);
+        SBPX_HIP(hipMemset(ctx->c.dev_err, 0, 4 * sizeof(int)));
+        dalloc(&ctx->c.counters, 8);
+        dalloc(&ctx->c.red, 4096);
+    });
+    if (st != SBPX_OK) {
+        static thread_local std::string last;
+        last = ctx->c.err;
+        delete ctx;
+        return st;
+    }
+    *out = ctx;
+    return SBPX_OK;
+}
+
+void sbpx_destroy(sbpx_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+    free_graph_buffers(ctx->c);
+    dfree(&ctx->c.dev_err);
+    dfree(&ctx->c.counters);
+    dfree(&ctx->c.red);
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+}
+
+const char* sbpx_last_error(const sbpx_ctx* ctx) { return ctx ? ctx->c.err.c_str() : "null context"; }
+
+sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
+    return guarded(ctx, [&] {
+        if (option == 1) ctx->count_bytes = value != 0;
+        else throw ArgError("unknown option");
+    });
+}
+
+sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t N, int64_t E, const int32_t* out_ptr, const int32_t* out_idx,
+                              const int32_t* in_ptr, const int32_t* in_idx) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (N <= 0 || E < 0 || E >= INT32_MAX / 2) throw ArgError("bad graph size");
+        if (!out_ptr || !in_ptr || (E > 0 && (!out_idx || !in_idx))) throw ArgError("null graph array");
+        if (out_ptr[0] != 0 || in_ptr[0] != 0 || out_ptr[N] != E || in_ptr[N] != E)
+            throw ArgError("CSR/CSC pointers do not span num_edges");
+        SBPX_HIP(hipSetDevice(c.device));
+        free_graph_buffers(c);
+        DevGraph& g = c.g;
+        g.N = N;
+        g.E = (int)E;
+        g.h_out_ptr.assign(out_ptr, out_ptr + N + 1);
+        g.h_in_ptr.assign(in_ptr, in_ptr + N + 1);
+        g.h_out_idx.assign(out_idx, out_idx + E);
+        g.h_in_idx.assign(in_idx, in_idx + E);
+        g.self.assign(N, 0);
+        std::vector<int> edge_src(E);
+        std::vector<int> in_self(N, 0);
+        for (int v = 0; v < N; ++v) {
+            if (out_ptr[v + 1] < out_ptr[v] || in_ptr[v + 1] < in_ptr[v]) throw ArgError("pointers not monotone");
+            for (int j = out_ptr[v]; j < out_ptr[v + 1]; ++j) {
+                int w = out_idx[j];
+                if (w < 0 || w >= N) throw ArgError("edge endpoint out of range");
+                if (w == v) g.self[v] = 1;
+                edge_src[j] = v;
+            }
+            for (int j = in_ptr[v]; j < in_ptr[v + 1]; ++j) {
+                int w = in_idx[j];
+                if (w < 0 || w >= N) throw ArgError("edge endpoint out of range");
+                if (w == v) in_self[v] = 1;
+            }
+            if (in_self[v] != g.self[v]) throw ArgError("self-loop present in only one of CSR/CSC");
+        }
+        // degree = |out| + |in| - self (graph.cpp:15-22); top 15% after std::sort with '>'
+        // are the high-degree vertices (graph.cpp:215-228)
+        g.degree.resize(N);
+        g.max_deg = 0;
+        for (int v = 0; v < N; ++v) {
+            g.degree[v] = (out_ptr[v + 1] - out_ptr[v]) + (in_ptr[v + 1] - in_ptr[v]) - g.self[v];
+            g.max_deg = std::max(g.max_deg, g.degree[v]);
+        }
+        std::vector<int> idx(N);
+        std::iota(idx.begin(), idx.end(), 0);
+        const std::vector<int>& vd = g.degree;
+        std::sort(idx.data(), idx.data() + idx.size(), [&vd](size_t i1, size_t i2) { return vd[i1] > vd[i2]; });
+        g.high.clear();
+        g.low.clear();
+        for (int index = 0; index < N; ++index) {
+            if (index < 0.15 * N) g.high.push_back(idx[index]);
+            else g.low.push_back(idx[index]);
+        }
+        // device buffers
+        dalloc(&g.out_ptr, N + 1);
+        dalloc(&g.in_ptr, N + 1);
+        dalloc(&g.out_idx, E);
+        dalloc(&g.in_idx, E);
+        dalloc(&g.edge_src, E);
+        SBPX_HIP(hipMemcpy(g.out_ptr, out_ptr, sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(g.in_ptr, in_ptr, sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+        if (E > 0) {
+            SBPX_HIP(hipMemcpy(g.out_idx, out_idx, sizeof(int) * E, hipMemcpyHostToDevice));
+            SBPX_HIP(hipMemcpy(g.in_idx, in_idx, sizeof(int) * E, hipMemcpyHostToDevice));
+            SBPX_HIP(hipMemcpy(g.edge_src, edge_src.data(), sizeof(int) * E, hipMemcpyHostToDevice));
+        }
+        DevBlockmodel& b = c.bm;
+        const size_t NB = (size_t)N + 1, EE = (size_t)E + 1;
+        dalloc(&b.assign, N);
+        dalloc(&b.row_ptr, NB);
+        dalloc(&b.col_ptr, NB);
+        dalloc(&b.row_col, EE);
+        dalloc(&b.row_val, EE);
+        dalloc(&b.row_cum, EE);
+        dalloc(&b.coo_row, EE);
+        dalloc(&b.col_row, EE);
+        dalloc(&b.col_val, EE);
+        dalloc(&b.col_cum, EE);
+        dalloc(&b.dout, NB);
+        dalloc(&b.din, NB);
+        dalloc(&b.deg, NB);
+        dalloc(&b.diag, NB);
+        dalloc(&b.drift, NB);
+        SBPX_HIP(hipMemset(b.drift, 0, sizeof(int) * NB));
+        dalloc((uint64_t**)&c.keys0, EE);
+        dalloc((uint64_t**)&c.keys1, EE);
+        dalloc(&c.vals0, EE);
+        dalloc(&c.vals1, EE);
+        dalloc(&c.flags, EE);
+        dalloc(&c.pos, EE);
+        dalloc(&c.hist, radix_hist_ints(E));
+        dalloc(&c.scan_tmp, scan_tmp_ints(std::max<long>(E, 256L * ((E + 4095) / 4096) + 1) + NB));
+        dalloc(&c.tmp_b, NB);
+        dalloc(&c.ev_s, NB);
+        dalloc(&c.ev_acc, NB);
+        dalloc(&c.ev_dS, NB);
+        dalloc(&c.ev_H, NB);
+        dalloc(&c.ev_u, NB);
+        dalloc(&c.moved_epoch, NB);
+        dalloc(&c.new_block, NB);
+        dalloc(&c.last_touch, NB);
+        SBPX_HIP(hipMemset(c.moved_epoch, 0, sizeof(int) * NB));
+        SBPX_HIP(hipMemset(c.last_touch, 0, sizeof(unsigned long long) * NB));
+        c.epoch = 0;
+        dalloc(&c.m_dS, NB);
+        dalloc(&c.m_best, NB);
+        dalloc(&c.m_map, NB);
+        dalloc(&c.m_count, NB);
+        dalloc(&c.list_buf, NB);
+        dalloc(&c.aux_buf, 2 * NB);
+        c.plan = SweepPlan();
+        MhState& h = c.mh;
+        dalloc(&h.rbase, NB + 1);
+        dalloc(&h.cbase, NB + 1);
+        dalloc(&h.rocc, NB);
+        dalloc(&h.cocc, NB);
+        dalloc(&h.rneed, NB);
+        dalloc(&h.cneed, NB);
+        dalloc(&h.status, 2);
+        dalloc(&h.d_all, NB);
+        dalloc(&h.d_high, NB);
+        {
+            std::vector<int> all(N);
+            std::iota(all.begin(), all.end(), 0);
+            SBPX_HIP(hipMemcpy(h.d_all, all.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+            if (!g.high.empty())
+                SBPX_HIP(hipMemcpy(h.d_high, g.high.data(), sizeof(int) * g.high.size(), hipMemcpyHostToDevice));
+            h.max_deg_high = 0;
+            for (int v : g.high) h.max_deg_high = std::max(h.max_deg_high, g.degree[v]);
+        }
+        SBPX_HIP(hipDeviceSynchronize());
+    });
+}
+
+sbpx_status sbpx_graph_split(sbpx_ctx* ctx, int32_t* high, int32_t* num_high, int32_t* low) {
+    return guarded(ctx, [&] {
+        require_graph(ctx->c);
+        const DevGraph& g = ctx->c.g;
+        if (num_high) *num_high = (int32_t)g.high.size();
+        if (high) std::copy(g.high.begin(), g.high.end(), high);
+        if (low) std::copy(g.low.begin(), g.low.end(), low);
+    });
+}
+
+sbpx_status sbpx_blockmodel_build(sbpx_ctx* ctx, int32_t B, const int32_t* assignment) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_graph(c);
+        if (B < 1 || B > c.g.N) throw ArgError("num_blocks must be in [1, N]");
+        if (!assignment) throw ArgError("null assignment");
+        for (int v = 0; v < c.g.N; ++v)
+            if (assignment[v] < 0 || assignment[v] >= B) throw ArgError("block id out of range [0, B)");
+        SBPX_HIP(hipSetDevice(c.device));
+        SBPX_HIP(hipMemcpyAsync(c.bm.assign, assignment, sizeof(int) * c.g.N, hipMemcpyHostToDevice, c.stream));
+        c.bm.B = B;
+        build_blockmodel(c);
+        check_device_error(c);
+    });
+}
+
+sbpx_status sbpx_blockmodel_info(sbpx_ctx* ctx, int32_t* B, int64_t* nnz) {
+    return guarded(ctx, [&] {
+        require_bm(ctx->c);
+        if (B) *B = ctx->c.bm.B;
+        if (nnz) *nnz = ctx->c.bm.nnz;
+    });
+}
+
+sbpx_status sbpx_blockmodel_download(sbpx_ctx* ctx, int32_t* assignment, int32_t* dout, int32_t* din,
+                                     int32_t* deg) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        const int B = c.bm.B;
+        if (assignment)
+            SBPX_HIP(hipMemcpyAsync(assignment, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToHost, c.stream));
+        if (dout) SBPX_HIP(hipMemcpyAsync(dout, c.bm.dout, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+        if (din) SBPX_HIP(hipMemcpyAsync(din, c.bm.din, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+        if (deg) SBPX_HIP(hipMemcpyAsync(deg, c.bm.deg, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+sbpx_status sbpx_blockmodel_coo(sbpx_ctx* ctx, int by_column, int32_t* rows, int32_t* cols, int32_t* vals) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        const int nnz = c.bm.nnz, B = c.bm.B;
+        std::vector<int> ptr(B + 1), idx(nnz), val(nnz);
+        int* dptr = by_column ? c.bm.col_ptr : c.bm.row_ptr;
+        int* didx = by_column ? c.bm.col_row : c.bm.row_col;
+        int* dval = by_column ? c.bm.col_val : c.bm.row_val;
+        SBPX_HIP(hipMemcpyAsync(ptr.data(), dptr, sizeof(int) * (B + 1), hipMemcpyDeviceToHost, c.stream));
+        if (nnz) {
+            SBPX_HIP(hipMemcpyAsync(idx.data(), didx, sizeof(int) * nnz, hipMemcpyDeviceToHost, c.stream));
+            SBPX_HIP(hipMemcpyAsync(val.data(), dval, sizeof(int) * nnz, hipMemcpyDeviceToHost, c.stream));
+        }
+        c.sync();
+        for (int x = 0; x < B; ++x) {
+            for (int j = ptr[x]; j < ptr[x + 1]; ++j) {
+                if (by_column) {
+                    rows[j] = idx[j];
+                    cols[j] = x;
+                } else {
+                    rows[j] = x;
+                    cols[j] = idx[j];
+                }
+                vals[j] = val[j];
+            }
+        }
+    });
+}
+
+sbpx_status sbpx_log_posterior(sbpx_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        require_bm(ctx->c);
+        *out = log_posterior(ctx->c);
+    });
+}
+
+sbpx_status sbpx_mdl(sbpx_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        double lp = log_posterior(c);
+        *out = model_term(c.g.N, c.g.E, c.bm.B) - lp;
+    });
+}
+
+static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
+                             double* dS, double* H, uint8_t* acc, uint64_t seed, uint32_t call, int tag,
+                             int32_t* s_out, double* u_out);
+
+// Requests longer than the per-vertex buffers (N entries) run in chunks.
+static void eval_fixed(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u, double* dS,
+                       double* H, uint8_t* acc, uint64_t seed, uint32_t call, int tag, int32_t* s_out,
+                       double* u_out) {
+    Ctx& c = ctx->c;
+    require_bm(c);
+    if (n < 0) throw ArgError("negative count");
+    const int chunk = c.g.N;
+    for (int off = 0; off < n; off += chunk) {
+        int m = std::min(chunk, n - off);
+        eval_fixed_chunk(ctx, m, v + off, s ? s + off : nullptr, u ? u + off : nullptr, dS ? dS + off : nullptr,
+                         H ? H + off : nullptr, acc ? acc + off : nullptr, seed, call, tag,
+                         s_out ? s_out + off : nullptr, u_out ? u_out + off : nullptr);
+    }
+}
+
+static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
+                             double* dS, double* H, uint8_t* acc, uint64_t seed, uint32_t call, int tag,
+                             int32_t* s_out, double* u_out) {
+    Ctx& c = ctx->c;
+    if (n == 0) return;
+    for (int i = 0; i < n; ++i)
+        if (v[i] < 0 || v[i] >= c.g.N) throw ArgError("vertex out of range");
+    if (s)
+        for (int i = 0; i < n; ++i)
+            if (s[i] < 0 || s[i] >= c.bm.B) throw ArgError("block id out of range [0, B)");
+    std::vector<int> ordered, hoff, perm;
+    int counts[NUM_BINS];
+    bin_vertices(c, v, n, ordered, counts, hoff, &perm);
+    std::vector<int> sv(n);
+    std::vector<double> uv(n);
+    if (s) {
+        for (int i = 0; i < n; ++i) {
+            sv[i] = s[perm[i]];
+            uv[i] = u[perm[i]];
+        }
+    }
+    int* d_list = c.list_buf;
+    int* d_s = c.aux_buf;
+    double* d_u = (double*)(c.m_dS);
+    SBPX_HIP(hipMemcpyAsync(d_list, ordered.data(), sizeof(int) * n, hipMemcpyHostToDevice, c.stream));
+    if (s) {
+        SBPX_HIP(hipMemcpyAsync(d_s, sv.data(), sizeof(int) * n, hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemcpyAsync(d_u, uv.data(), sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    }
+    ensure_pool(c, (size_t)hoff.back());
+    int* d_hoff = c.aux_buf + (c.g.N + 1);
+    SBPX_HIP(hipMemcpyAsync(d_hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c.stream));
+    int off = 0;
+    for (int b = 0; b < NUM_BINS; ++b) {
+        if (counts[b]) {
+            EvalLaunch L{};
+            L.list = d_list + off;
+            L.n = counts[b];
+            L.bin = b;
+            L.given_s = s ? d_s + off : nullptr;
+            L.given_u = s ? d_u + off : nullptr;
+            L.seed = seed;
+            L.call = call;
+            L.tag = tag ? TAG_MH : TAG_ASYNC;
+            L.out_by_vertex = 0;
+            L.out_offset = off;
+            L.propose_only = s ? 0 : 1;
+            L.out_u = c.ev_u;
+            L.huge_offsets = d_hoff;
+            launch_eval(c, L);
+        }
+        off += counts[b];
+    }
+    std::vector<int> rs(n);
+    std::vector<uint8_t> racc(n);
+    std::vector<double> rdS(n), rH(n), ru(n);
+    SBPX_HIP(hipMemcpyAsync(rs.data(), c.ev_s, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+    SBPX_HIP(hipMemcpyAsync(racc.data(), c.ev_acc, n, hipMemcpyDeviceToHost, c.stream));
+    SBPX_HIP(hipMemcpyAsync(rdS.data(), c.ev_dS, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    SBPX_HIP(hipMemcpyAsync(rH.data(), c.ev_H, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    SBPX_HIP(hipMemcpyAsync(ru.data(), c.ev_u, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    check_device_error(c);
+    for (int i = 0; i < n; ++i) {
+        int o = perm[i];
+        if (dS) dS[o] = rdS[i];
+        if (H) H[o] = rH[i];
+        if (acc) acc[o] = racc[i] == 2 ? 1 : 0;
+        if (s_out) s_out[o] = rs[i];
+        if (u_out) u_out[o] = ru[i];
+    }
+}
+
+sbpx_status sbpx_eval_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
+                            double* dS, double* H, uint8_t* accept) {
+    return guarded(ctx, [&] {
+        if (n > 0 && (!v || !s || !u)) throw ArgError("null proposal array");
+        eval_fixed(ctx, n, v, s, u, dS, H, accept, 0, 0, 0, nullptr, nullptr);
+    });
+}
+
+sbpx_status sbpx_propose(sbpx_ctx* ctx, int32_t n, const int32_t* v, uint64_t seed, uint32_t call, int tag,
+                         int32_t* s, double* u) {
+    return guarded(ctx, [&] {
+        if (n > 0 && !v) throw ArgError("null vertex array");
+        eval_fixed(ctx, n, v, nullptr, nullptr, nullptr, nullptr, nullptr, seed, call, tag, s, u);
+    });
+}
+
+sbpx_status sbpx_apply_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        if (n < 0) throw ArgError("negative count");
+        if (n == 0) return;
+        std::vector<char> seen(c.g.N, 0);
+        for (int i = 0; i < n; ++i) {
+            if (v[i] < 0 || v[i] >= c.g.N) throw ArgError("vertex out of range");
+            if (s[i] < 0 || s[i] >= c.bm.B) throw ArgError("block id out of range [0, B)");
+            if (seen[v[i]]) throw ArgError("vertex listed twice");
+            seen[v[i]] = 1;
+        }
+        std::vector<int> a(c.g.N);
+        SBPX_HIP(hipMemcpyAsync(a.data(), c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        std::vector<uint8_t> stat(n);
+        for (int i = 0; i < n; ++i) stat[i] = (a[v[i]] != s[i]) ? 2 : 0;
+        // vertex-indexed outputs: scatter s and status through a small kernel-free path
+        std::vector<int> sv(c.g.N, 0);
+        std::vector<uint8_t> st(c.g.N, 0);
+        for (int i = 0; i < n; ++i) {
+            sv[v[i]] = s[i];
+            st[v[i]] = stat[i];
+        }
+        SBPX_HIP(hipMemcpyAsync(c.ev_s, sv.data(), sizeof(int) * c.g.N, hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemcpyAsync(c.ev_acc, st.data(), c.g.N, hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemsetAsync(c.ev_dS, 0, sizeof(double) * c.g.N, c.stream));
+        SBPX_HIP(hipMemcpyAsync(c.list_buf, v, sizeof(int) * n, hipMemcpyHostToDevice, c.stream));
+        apply_accepted(c, c.list_buf, n);
+        c.sync();
+        check_device_error(c);
+    });
+}
+
+sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, uint32_t call,
+                       sbpx_sweep_stats* stats) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        if (batches < 1) throw ArgError("batches must be >= 1");
+        if (algo != SBPX_ALGO_ASYNC_GIBBS && algo != SBPX_ALGO_HYBRID && algo != SBPX_ALGO_METROPOLIS_HASTINGS)
+            throw ArgError("unknown algorithm");
+        build_plan(c, algo, batches);
+        SBPX_HIP(hipMemsetAsync(c.counters, 0, sizeof(long long) * 8, c.stream));
+        SBPX_HIP(hipMemsetAsync(c.red + 2048, 0, sizeof(double), c.stream));
+        c.timed.clear();
+        long long* bytes = ctx->count_bytes ? c.counters + 4 : nullptr;
+        if (algo == SBPX_ALGO_METROPOLIS_HASTINGS)
+            mh_sequential(c, c.mh.d_all, c.g.N, c.g.max_deg, seed, call, TAG_MH);
+        else if (algo == SBPX_ALGO_HYBRID)
+            mh_sequential(c, c.mh.d_high, (int)c.g.high.size(), c.mh.max_deg_high, seed, call, TAG_MH);
+        for (const BinnedList& bl : c.plan.batches_async) {
+            const int* d_list = c.plan.d_lists + bl.offset[0];
+            eval_vertex_list(ctx, c.plan.d_lists, bl, c.plan.d_huge_offsets + bl.huge_base, seed, call, TAG_ASYNC,
+                             bytes, true);
+            apply_accepted(c, d_list, bl.total);
+        }
+        long long cnt[8];
+        double dsum = 0.0;
+        SBPX_HIP(hipMemcpyAsync(cnt, c.counters, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(&dsum, c.red + 2048, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        check_device_error(c);
+        if (stats) {
+            *stats = sbpx_sweep_stats{};
+            for (size_t i = 0; i < c.timed.size(); ++i) {
+                float t = 0.0f;
+                SBPX_HIP(hipEventElapsedTime(&t, ctx->events[2 * i], ctx->events[2 * i + 1]));
+                stats->eval_ms[c.timed[i]] += t;
+                stats->eval_launches[c.timed[i]] += 1;
+            }
+            stats->attempts = cnt[0];
+            stats->evals = cnt[1];
+            stats->moves = cnt[2];
+            stats->dS_accepted = dsum;
+            for (int b = 0; b < NUM_BINS; ++b) stats->eval_bytes[b] = ctx->count_bytes ? (double)cnt[4 + b] : 0.0;
+        }
+    });
+}
+
+sbpx_status sbpx_merge_proposals(sbpx_ctx* ctx, uint64_t seed, uint32_t call, double* dS, int32_t* best) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        launch_merge_proposals(c, seed, call);
+        if (dS) SBPX_HIP(hipMemcpyAsync(dS, c.m_dS, sizeof(double) * c.bm.B, hipMemcpyDeviceToHost, c.stream));
+        if (best) SBPX_HIP(hipMemcpyAsync(best, c.m_best, sizeof(int) * c.bm.B, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        check_device_error(c);
+    });
+}
+
+static void carry_out(Ctx& c, int num_to_merge, int mode, const double* dS, const int32_t* best) {
+    const int B = c.bm.B;
+    if (num_to_merge < 0 || num_to_merge >= B) throw ArgError("num_to_merge must be in [0, B)");
+    if (mode != SBPX_MERGE_APPROXIMATE) throw std::runtime_error("advanced merge selection not built yet");
+    // utils::sort_indices (utils.hpp:94-101): std::sort ascending, same comparator, so ties
+    // fall in the same order as the reference's.
+    std::vector<int> order(B);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<double> d(dS, dS + B);
+    std::sort(order.data(), order.data() + order.size(), [&d](size_t i1, size_t i2) { return d[i1] < d[i2]; });
+    // Blockmodel::carry_out_best_merges (blockmodel.cpp:50-78): block_map always names the
+    // block a block has been merged into; a union-find with the merge target as root keeps
+    // the same mapping in near-linear time.
+    std::vector<int> parent(B);
+    std::iota(parent.begin(), parent.end(), 0);
+    auto find = [&parent](int x) {
+        int root = x;
+        while (parent[root] != root) root = parent[root];
+        while (parent[x] != root) {
+            int nx = parent[x];
+            parent[x] = root;
+            x = nx;
+        }
+        return root;
+    };
+    int num_merged = 0;
+    size_t counter = 0;
+    while (num_merged < num_to_merge) {
+        if (counter >= order.size()) throw StateError("ran out of merge candidates");
+        int from = order[counter];
+        int b = best[from];
+        if (b < 0 || b >= B) throw StateError("block without a merge proposal");
+        int to = find(b);
+        counter++;
+        if (to != from) {
+            parent[from] = to;
+            num_merged++;
+        }
+    }
+    // build_mapping (blockmodel.cpp:29-41): surviving labels of non-empty blocks, ascending.
+    launch_block_counts(c);
+    std::vector<int> count(B);
+    SBPX_HIP(hipMemcpyAsync(count.data(), c.m_count, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    std::vector<char> used(B, 0);
+    for (int x = 0; x < B; ++x)
+        if (count[x] > 0) used[find(x)] = 1;
+    std::vector<int> rank(B, -1);
+    int next = 0;
+    for (int x = 0; x < B; ++x)
+        if (used[x]) rank[x] = next++;
+    std::vector<int> map(B);
+    for (int x = 0; x < B; ++x) map[x] = rank[find(x)];
+    const int newB = B - num_to_merge;
+    if (next > newB) throw StateError("more surviving labels than blocks after the merge");
+    SBPX_HIP(hipMemcpyAsync(c.m_map, map.data(), sizeof(int) * B, hipMemcpyHostToDevice, c.stream));
+    launch_relabel(c);
+    c.bm.B = newB;
+    build_blockmodel(c);
+    check_device_error(c);
+}
+
+sbpx_status sbpx_carry_out_merges(sbpx_ctx* ctx, int32_t num_to_merge, int mode, const double* dS,
+                                  const int32_t* best) {
+    return guarded(ctx, [&] {
+        require_bm(ctx->c);
+        if (!dS || !best) throw ArgError("null merge arrays");
+        carry_out(ctx->c, num_to_merge, mode, dS, best);
+    });
+}
+
+sbpx_status sbpx_merge_blocks(sbpx_ctx* ctx, int32_t num_to_merge, int mode, uint64_t seed, uint32_t call) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        launch_merge_proposals(c, seed, call);
+        c.h_dS.resize(c.bm.B);
+        c.h_best.resize(c.bm.B);
+        SBPX_HIP(hipMemcpyAsync(c.h_dS.data(), c.m_dS, sizeof(double) * c.bm.B, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(c.h_best.data(), c.m_best, sizeof(int) * c.bm.B, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        check_device_error(c);
+        carry_out(c, num_to_merge, mode, c.h_dS.data(), c.h_best.data());
+    });
+}
+
+sbpx_status sbpx_selftest_logf(sbpx_ctx* ctx, uint32_t lo, uint32_t n, float* out) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (!out) throw ArgError("null output");
+        if (n == 0) return;
+        float* d = nullptr;
+        SBPX_HIP(hipMalloc((void**)&d, sizeof(float) * n));
+        try {
+            launch_logf_table(c, lo, n, d);
+            SBPX_HIP(hipMemcpyAsync(out, d, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+        } catch (...) {
+            (void)hipFree(d);
+            throw;
+        }
+        SBPX_HIP(hipFree(d));
+    });
+}
+
+}  // extern "C"

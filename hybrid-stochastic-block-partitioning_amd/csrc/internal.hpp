@@ -1,0 +1,199 @@
+// Internal state of an sbpx context: the device-resident graph and blockmodel, scratch
+// buffers, and the launchers each translation unit provides.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "../../include/sbpx.h"
+
+namespace sbpx {
+
+#define SBPX_HIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) throw HipError(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct ArgError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NumericError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Device error word bits raised by kernels.
+enum : int {
+    DEV_ERR_ZERO_DEGREE = 1,   // nonzero cell with a zero block degree (blockmodel.cpp:276-281)
+    DEV_ERR_NAN = 2,           // NaN/Inf delta MDL (entropy.cpp:323-340)
+    DEV_ERR_TABLE = 4,         // neighbour-block table overflow
+    DEV_ERR_BLOCK_RANGE = 8,   // block id out of range
+};
+
+// Degree bins for the vertex-move kernels: neighbour-block table capacity per bin.
+constexpr int BIN_SMALL_MAXDEG = 127;   // 256-slot LDS table, 4 wavefronts per workgroup
+constexpr int BIN_LARGE_MAXDEG = 2047;  // 4096-slot LDS table, 1 wavefront per workgroup
+constexpr int NUM_BINS = 3;             // small, large, huge (table in global memory)
+
+struct DevGraph {
+    int N = 0;
+    int E = 0;
+    int max_deg = 0;
+    int *out_ptr = nullptr, *out_idx = nullptr, *in_ptr = nullptr, *in_idx = nullptr;
+    int* edge_src = nullptr;  // source vertex of each CSR edge
+    std::vector<int> h_out_ptr, h_out_idx, h_in_ptr, h_in_idx;
+    std::vector<int> degree;             // |out| + |in| - self (graph.cpp:15-22)
+    std::vector<char> self;
+    std::vector<int> high, low;          // degree split (graph.cpp:215-228)
+};
+
+struct DevBlockmodel {
+    int B = 0;
+    int nnz = 0;
+    bool valid = false;
+    int* assign = nullptr;  // [N]
+    // CSR of the block matrix (rows sorted by column) and CSC (columns sorted by row);
+    // *_cum are inclusive prefix sums of the values inside each row / column.
+    int *row_ptr = nullptr, *row_col = nullptr, *row_val = nullptr, *row_cum = nullptr, *coo_row = nullptr;
+    int *col_ptr = nullptr, *col_row = nullptr, *col_val = nullptr, *col_cum = nullptr;
+    int *dout = nullptr, *din = nullptr, *deg = nullptr, *diag = nullptr, *drift = nullptr;
+};
+
+// A vertex list split by degree bin, for one batch.
+struct BinnedList {
+    int offset[NUM_BINS] = {0, 0, 0};  // into the device list buffer
+    int count[NUM_BINS] = {0, 0, 0};
+    int total = 0;
+    int huge_base = 0;  // index of this batch's first huge-table offset
+};
+
+struct SweepPlan {
+    int algo = -1, batches = 0;
+    int* d_lists = nullptr;  // concatenated binned vertex lists
+    std::vector<BinnedList> batches_async;
+    int huge_pool_slots = 0;
+    int* d_huge_offsets = nullptr;
+    int* d_high = nullptr;   // high-degree vertices in sorted order (Metropolis-Hastings queue)
+    int n_high = 0;
+};
+
+// Mutable block matrix of a Metropolis-Hastings pass: per-row open-addressing tables.
+struct MhState {
+    int *rbase = nullptr, *cbase = nullptr;  // [N + 2] slot offsets
+    int *rkeys = nullptr, *rvals = nullptr, *ckeys = nullptr, *cvals = nullptr;
+    size_t slots = 0;
+    int *rocc = nullptr, *cocc = nullptr, *rneed = nullptr, *cneed = nullptr;  // [N + 1]
+    int* tables = nullptr;  // neighbour-block tables, one per wavefront
+    size_t table_slots = 0;
+    int* status = nullptr;  // [2]
+    int* d_all = nullptr;   // 0..N-1
+    int* d_high = nullptr;  // high-degree vertices in degree order
+    int max_deg_high = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    DevGraph g;
+    DevBlockmodel bm;
+    SweepPlan plan;
+    MhState mh;
+
+    // scratch
+    void* keys0 = nullptr;
+    void* keys1 = nullptr;
+    int* vals0 = nullptr;
+    int* vals1 = nullptr;
+    int* flags = nullptr;   // E + 1
+    int* pos = nullptr;     // E + 1
+    int* hist = nullptr;
+    int* scan_tmp = nullptr;
+    int* tmp_b = nullptr;   // N + 1
+    double* red = nullptr;  // reduction partials
+    int* dev_err = nullptr;
+    long long* counters = nullptr;  // [8]: attempts, evals, moves
+
+    // per-vertex move buffers
+    int* ev_s = nullptr;
+    uint8_t* ev_acc = nullptr;
+    double* ev_dS = nullptr;
+    double* ev_H = nullptr;
+    double* ev_u = nullptr;
+    int* list_buf = nullptr;      // [N + 1] staging for caller-given vertex lists
+    int* aux_buf = nullptr;       // [2 (N + 1)]
+    int* moved_epoch = nullptr;   // [N]
+    int* new_block = nullptr;     // [N]
+    unsigned long long* last_touch = nullptr;  // [N]
+    int epoch = 0;
+    int* table_pool = nullptr;    // neighbour-block tables of huge-degree vertices
+    size_t table_pool_slots = 0;
+
+    // merge buffers
+    double* m_dS = nullptr;
+    int* m_best = nullptr;
+    int* m_map = nullptr;
+    int* m_count = nullptr;
+
+    // host mirrors
+    std::vector<double> h_dS;
+    std::vector<int> h_best;
+
+    // timing of the last sweep's eval kernels (ms) and algorithmic bytes
+    std::vector<int> timed;  // degree bin of each timed evaluation launch in this sweep
+
+    void sync() { SBPX_HIP(hipStreamSynchronize(stream)); }
+};
+
+// sort_scan.hip
+size_t scan_tmp_ints(long n);
+size_t radix_hist_ints(long n);
+void scan_exclusive(const int* in, int* out, int n, int* tmp, hipStream_t st);
+template <typename K>
+int radix_sort(K* keys, K* keys_alt, int* vals, int* vals_alt, long n, const int* shifts, int npass,
+               int* hist, int* scan_tmp, hipStream_t st);
+
+// blockmodel.hip
+void build_blockmodel(Ctx& c);               // from c.bm.assign, resets drift
+void rebuild_keep_drift(Ctx& c);             // from c.bm.assign, keeps drift
+double log_posterior(Ctx& c);                // sum of M * logf(M / (dout din)) in FP64
+void check_device_error(Ctx& c);
+void launch_logf_table(Ctx& c, uint32_t lo, uint32_t n, float* out);
+
+// sweep.hip
+struct EvalLaunch {
+    const int* list;   // vertex ids
+    int n;
+    int bin;
+    const int* given_s;     // optional fixed proposals
+    const double* given_u;  // optional fixed uniforms
+    uint64_t seed;
+    uint32_t call;
+    uint32_t tag;
+    int out_by_vertex;      // 1: outputs at index v; 0: at out_offset + position
+    int out_offset;
+    int propose_only;       // 1: stop after drawing the proposal
+    double* out_u;          // optional: the accept uniform used
+    const int* huge_offsets;
+    long long* bytes;       // optional algorithmic byte counter
+};
+void launch_eval(Ctx& c, const EvalLaunch& L);
+void apply_accepted(Ctx& c, const int* list, int n);
+void finish_degrees(Ctx& c);
+void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag);
+
+// merge.hip
+void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call);
+void launch_block_counts(Ctx& c);
+void launch_relabel(Ctx& c);
+
+}  // namespace sbpx

@@ -1,0 +1,225 @@
+// Block-merge proposals: one wavefront per block r evaluates its 10 proposals in order
+// (block_merge.cpp:278-301, 335-368). A proposal draws t in proportion to row r + column r and
+// s in proportion to row t + column t without index r (common.cpp:131-182, block_merge=true),
+// both by wave-parallel searches of the prefix sums. Repeated proposals score DBL_MAX. The delta
+// MDL of merging r into s follows entropy::block_merge_delta_mdl (entropy.cpp:127-180):
+//   old cells of row r and column r (their new value is 0 and contributes nothing),
+//   cells (s, c) and (c, s) that gain row/column r's counts, the diagonal (s, s),
+//   and the untouched cells of row s and column s, whose degrees change.
+#include <cfloat>
+
+#include "internal.hpp"
+
+namespace sbpx {
+
+struct MergeParams {
+    const int* __restrict__ row_ptr;
+    const int* __restrict__ row_col;
+    const int* __restrict__ row_val;
+    const int* __restrict__ row_cum;
+    const int* __restrict__ col_ptr;
+    const int* __restrict__ col_row;
+    const int* __restrict__ col_val;
+    const int* __restrict__ col_cum;
+    const int* __restrict__ dout;
+    const int* __restrict__ din;
+    int B;
+    uint64_t seed;
+    uint32_t call;
+    double* __restrict__ out_dS;
+    int* __restrict__ out_best;
+};
+
+__device__ __forceinline__ float fm(int x) { return (float)x; }
+
+__device__ double merge_delta_mdl_wave(const MergeParams& P, int r, int s, int k_out, int k_in) {
+    const int lane = lane_id();
+    const int r0 = P.row_ptr[r], r1 = P.row_ptr[r + 1];
+    const int c0 = P.col_ptr[r], c1 = P.col_ptr[r + 1];
+    const int s0 = P.row_ptr[s], s1 = P.row_ptr[s + 1];
+    const int cs0 = P.col_ptr[s], cs1 = P.col_ptr[s + 1];
+    const int dout_r = P.dout[r], din_r = P.din[r], dout_s = P.dout[s], din_s = P.din[s];
+    const int ndout_s = dout_s + k_out, ndin_s = din_s + k_in;
+    double acc = 0.0;
+    int m_rr = 0, m_rs = 0, m_sr = 0;
+    // row r: old (r, c); keys (s, c) for c not in {r, s}
+    for (int j = r0 + lane; j < r1; j += WAVE) {
+        int c = P.row_col[j], m = P.row_val[j];
+        int dc = P.din[c];
+        acc += (double)cell_entropy(fm(m), fm(dc), fm(dout_r));
+        if (c == r) {
+            m_rr = m;
+        } else if (c == s) {
+            m_rs = m;
+        } else {
+            int msc = lookup_sorted(P.row_col, P.row_val, s0, s1, c);
+            acc += (double)cell_entropy(fm(msc), fm(dc), fm(dout_s));
+            acc -= (double)cell_entropy(fm(msc + m), fm(dc), fm(ndout_s));
+        }
+    }
+    // column r without (r, r): old (i, r); keys (i, s) for i not in {r, s}
+    for (int j = c0 + lane; j < c1; j += WAVE) {
+        int i = P.col_row[j];
+        if (i == r) continue;
+        int m = P.col_val[j];
+        int di = P.dout[i];
+        acc += (double)cell_entropy(fm(m), fm(din_r), fm(di));
+        if (i == s) {
+            m_sr = m;
+        } else {
+            int mis = lookup_sorted(P.col_row, P.col_val, cs0, cs1, i);
+            acc += (double)cell_entropy(fm(mis), fm(din_s), fm(di));
+            acc -= (double)cell_entropy(fm(mis + m), fm(ndin_s), fm(di));
+        }
+    }
+    m_rr = wave_max(m_rr);
+    m_rs = wave_max(m_rs);
+    m_sr = wave_max(m_sr);
+    const int d_ss = m_rr + m_rs + m_sr;
+    // row s: untouched cells with the new degrees
+    int m_ss = 0;
+    for (int j = s0 + lane; j < s1; j += WAVE) {
+        int c = P.row_col[j], m = P.row_val[j];
+        if (c == s) m_ss = m;
+        bool touched;
+        if (c == s) touched = d_ss > 0;
+        else if (c == r) touched = true;
+        else touched = lookup_sorted(P.row_col, P.row_val, r0, r1, c) != 0;
+        if (!touched) {
+            int dc = P.din[c];
+            int ndc = (c == s) ? ndin_s : dc;
+            acc += (double)cell_entropy(fm(m), fm(dc), fm(dout_s));
+            acc -= (double)cell_entropy(fm(m), fm(ndc), fm(ndout_s));
+        }
+    }
+    // column s, rows not in {r, s}: untouched cells
+    for (int j = cs0 + lane; j < cs1; j += WAVE) {
+        int i = P.col_row[j];
+        if (i == r || i == s) continue;
+        if (lookup_sorted(P.col_row, P.col_val, c0, c1, i) != 0) continue;
+        int m = P.col_val[j];
+        int di = P.dout[i];
+        acc += (double)cell_entropy(fm(m), fm(din_s), fm(di));
+        acc -= (double)cell_entropy(fm(m), fm(ndin_s), fm(di));
+    }
+    m_ss = wave_max(m_ss);
+    if (lane == 0 && d_ss > 0) {
+        acc += (double)cell_entropy(fm(m_ss), fm(din_s), fm(dout_s));
+        acc -= (double)cell_entropy(fm(m_ss + d_ss), fm(ndin_s), fm(ndout_s));
+    }
+    return wave_sum(acc);
+}
+
+constexpr int MERGE_WAVES = 4;
+constexpr int NUM_PROPOSALS = 10;  // NUM_AGG_PROPOSALS_PER_BLOCK (block_merge.hpp:24)
+
+__global__ void __launch_bounds__(MERGE_WAVES * WAVE) merge_proposals_kernel(MergeParams P) {
+    const int r = blockIdx.x * MERGE_WAVES + threadIdx.x / WAVE;
+    if (r >= P.B) return;
+    const int lane = lane_id();
+    const int B = P.B;
+    const int k_out = P.dout[r], k_in = P.din[r];
+    const int k = k_out + k_in;
+    int past[NUM_PROPOSALS];
+    double best_dS = DBL_MAX;
+    int best = -1;
+    for (int i = 0; i < NUM_PROPOSALS; ++i) {
+        Draw4 dr = philox4x32_10(P.seed, (uint32_t)r, P.call, (uint32_t)i, TAG_MERGE);
+        int s;
+        if (k == 0) {
+            s = random_block(r, B, dr.x2);
+        } else {
+            int target = (int)mulhi32(dr.x0, (uint32_t)k);
+            int t;
+            if (target < k_out) {
+                int p = wave_upper_bound(P.row_cum, P.row_ptr[r], P.row_ptr[r + 1], target);
+                t = P.row_col[p];
+            } else {
+                int p = wave_upper_bound(P.col_cum, P.col_ptr[r], P.col_ptr[r + 1], target - k_out);
+                t = P.col_row[p];
+            }
+            const int t0 = P.row_ptr[t], t1 = P.row_ptr[t + 1];
+            const int u0 = P.col_ptr[t], u1 = P.col_ptr[t + 1];
+            const int pr = lower_bound_lane(P.row_col, t0, t1, r);
+            const int pc = lower_bound_lane(P.col_row, u0, u1, r);
+            const int Mtr = (pr < t1 && P.row_col[pr] == r) ? P.row_val[pr] : 0;
+            const int Mrt = (pc < u1 && P.col_row[pc] == r) ? P.col_val[pc] : 0;
+            const int rowpart = P.dout[t] - Mtr;
+            const int total = rowpart + P.din[t] - Mrt;
+            if (total == 0) {
+                s = random_block(r, B, dr.x2);
+            } else {
+                int target2 = (int)mulhi32(dr.x1, (uint32_t)total);
+                if (target2 < rowpart) {
+                    int before = pr > t0 ? P.row_cum[pr - 1] : 0;
+                    int tt = target2 < before ? target2 : target2 + Mtr;
+                    int p = wave_upper_bound(P.row_cum, t0, t1, tt);
+                    s = P.row_col[p];
+                } else {
+                    int target3 = target2 - rowpart;
+                    int before = pc > u0 ? P.col_cum[pc - 1] : 0;
+                    int tt = target3 < before ? target3 : target3 + Mrt;
+                    int p = wave_upper_bound(P.col_cum, u0, u1, tt);
+                    s = P.col_row[p];
+                }
+            }
+        }
+        bool repeat = false;
+        for (int q = 0; q < i; ++q) repeat |= past[q] == s;
+        past[i] = s;
+        double dS = repeat ? DBL_MAX : merge_delta_mdl_wave(P, r, s, k_out, k_in);
+        if (dS < best_dS) {
+            best_dS = dS;
+            best = s;
+        }
+    }
+    if (lane == 0) {
+        P.out_dS[r] = best_dS;
+        P.out_best[r] = best;
+    }
+}
+
+void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call) {
+    MergeParams P{};
+    P.row_ptr = c.bm.row_ptr;
+    P.row_col = c.bm.row_col;
+    P.row_val = c.bm.row_val;
+    P.row_cum = c.bm.row_cum;
+    P.col_ptr = c.bm.col_ptr;
+    P.col_row = c.bm.col_row;
+    P.col_val = c.bm.col_val;
+    P.col_cum = c.bm.col_cum;
+    P.dout = c.bm.dout;
+    P.din = c.bm.din;
+    P.B = c.bm.B;
+    P.seed = seed;
+    P.call = call;
+    P.out_dS = c.m_dS;
+    P.out_best = c.m_best;
+    int nb = (c.bm.B + MERGE_WAVES - 1) / MERGE_WAVES;
+    merge_proposals_kernel<<<nb, MERGE_WAVES * WAVE, 0, c.stream>>>(P);
+    SBPX_HIP(hipGetLastError());
+}
+
+__global__ void block_counts_kernel(const int* __restrict__ assign, int N, int* __restrict__ count) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) atomicAdd(&count[assign[v]], 1);
+}
+
+__global__ void relabel_kernel(int* __restrict__ assign, int N, const int* __restrict__ map) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < N) assign[v] = map[assign[v]];
+}
+
+void launch_block_counts(Ctx& c) {
+    SBPX_HIP(hipMemsetAsync(c.m_count, 0, sizeof(int) * c.bm.B, c.stream));
+    block_counts_kernel<<<(c.g.N + 255) / 256, 256, 0, c.stream>>>(c.bm.assign, c.g.N, c.m_count);
+    SBPX_HIP(hipGetLastError());
+}
+
+void launch_relabel(Ctx& c) {
+    relabel_kernel<<<(c.g.N + 255) / 256, 256, 0, c.stream>>>(c.bm.assign, c.g.N, c.m_map);
+    SBPX_HIP(hipGetLastError());
+}
+
+}  // namespace sbpx

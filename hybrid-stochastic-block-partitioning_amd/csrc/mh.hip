@@ -1,0 +1,362 @@
+// Sequential Metropolis-Hastings (finetune.cpp:446-461, 488-501, used by metropolis_hastings
+// :412-444 and by hybrid_mcmc for the high-degree vertices :358-363): each accepted move changes
+// the block matrix before the next vertex is proposed.
+//
+// The block matrix is held, for the duration of the pass, as a per-row open-addressing table in
+// both orientations (HashView), built from the CSR snapshot. One workgroup of 16 wavefronts runs
+// the pass: in each round the wavefronts evaluate the next 16 vertices of the queue against the
+// current state; the first accepted move in queue order is committed and the round restarts
+// after it. Every vertex before it was rejected and so left the state unchanged, which makes the
+// outcome identical to evaluating one vertex at a time (proposals and uniforms are keyed by
+// vertex, not by evaluation order). A commit first checks that every row and column it may
+// extend has room; if not, the kernel stops before touching anything, the host rebuilds the
+// tables with more room for those rows, and the pass resumes at the same vertex.
+#include <algorithm>
+
+#include "eval_core.hpp"
+#include "internal.hpp"
+
+namespace sbpx {
+
+constexpr int MH_WAVES = 16;
+
+// ------------------------------------------------------------------------------------------
+// Table build: cap(x) = next power of two >= 2 (nnz(x) + need(x)) + 32, filled from the CSR.
+__global__ void hash_caps_kernel(const int* __restrict__ ptr, const int* __restrict__ need, int B,
+                                 int* __restrict__ caps) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= B) return;
+    int want = 2 * (ptr[x + 1] - ptr[x] + need[x]) + 32;
+    int cap = 32;
+    while (cap < want) cap <<= 1;
+    caps[x] = cap;
+}
+
+// One wavefront per row: clear the row's slots, then insert its CSR entries.
+__global__ void hash_fill_kernel(const int* __restrict__ ptr, const int* __restrict__ idx, const int* __restrict__ val,
+                                 int B, const int* __restrict__ base, int* keys, int* vals, int* occ) {
+    const int x = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (x >= B) return;
+    const int lane = lane_id();
+    const int b0 = base[x], cap = base[x + 1] - b0;
+    for (int j = lane; j < cap; j += WAVE) {
+        keys[b0 + j] = -1;
+        vals[b0 + j] = 0;
+    }
+    fence_wave();
+    for (int j = ptr[x] + lane; j < ptr[x + 1]; j += WAVE) {
+        int key = idx[j];
+        int slot = (int)(hash_block(key) & (uint32_t)(cap - 1));
+        for (;;) {
+            int old = atomicCAS(&keys[b0 + slot], -1, key);
+            if (old == -1) break;
+            slot = (slot + 1) & (cap - 1);
+        }
+        vals[b0 + slot] = val[j];
+    }
+    if (lane == 0) occ[x] = ptr[x + 1] - ptr[x];
+}
+
+// ------------------------------------------------------------------------------------------
+struct MhParams {
+    GraphView G;
+    // mutable state, read and written through the same (non-restrict) pointers
+    int* assign;
+    int* dout;
+    int* din;
+    int* deg;
+    int* drift;
+    int B;
+    int* rbase;
+    int* rkeys;
+    int* rvals;
+    int* rocc;
+    int* rneed;
+    int* cbase;
+    int* ckeys;
+    int* cvals;
+    int* cocc;
+    int* cneed;
+    const int* __restrict__ list;
+    int n;
+    int start;
+    uint64_t seed;
+    uint32_t call;
+    uint32_t tag;
+    int* tables;
+    int tcap;
+    int* status;  // [0] next queue position, [1] 1 if stopped for room
+    long long* counters;  // attempts, evals, moves
+    double* dS_sum;
+    int* err;
+};
+
+__device__ __forceinline__ int room(const int* base, const int* occ, int x) {
+    int cap = base[x + 1] - base[x];
+    return (cap * 3) / 4 - occ[x];
+}
+
+// Adds d to cell (x, key) of one orientation, claiming a slot when the key is new.
+__device__ __forceinline__ void upsert(int* base, int* keys, int* vals, int* occ, int x, int key, int d) {
+    const int b0 = base[x], cap = base[x + 1] - b0;
+    int slot = (int)(hash_block(key) & (uint32_t)(cap - 1));
+    for (int probe = 0; probe < cap; ++probe) {
+        int k = keys[b0 + slot];
+        if (k == key) {
+            atomicAdd(&vals[b0 + slot], d);
+            return;
+        }
+        if (k == -1) {
+            int old = atomicCAS(&keys[b0 + slot], -1, key);
+            if (old == -1) {
+                atomicAdd(&occ[x], 1);
+                atomicAdd(&vals[b0 + slot], d);
+                return;
+            }
+            if (old == key) {
+                atomicAdd(&vals[b0 + slot], d);
+                return;
+            }
+        }
+        slot = (slot + 1) & (cap - 1);
+    }
+}
+
+__device__ __forceinline__ void cell_add(const MhParams& P, int row, int col, int d) {
+    upsert(P.rbase, P.rkeys, P.rvals, P.rocc, row, col, d);
+    upsert(P.cbase, P.ckeys, P.cvals, P.cocc, col, row, d);
+}
+
+// Room check for a commit: row s and column s may gain one cell per neighbour block plus the
+// four cells among {r, s}; rows/columns r one or two; every other neighbour block one cell in
+// its row and one in its column. Records the room a failing row/column needs.
+__device__ bool commit_fits(const MhParams& P, const MoveResult& R, const int* tab, int cap) {
+    const int lane = lane_id();
+    const int r = R.r, s = R.s;
+    const int big = R.nuniq + 4;
+    int bad = 0;
+    if (lane == 0) {
+        if (room(P.rbase, P.rocc, s) < big) { atomicMax(&P.rneed[s], big); bad = 1; }
+        if (room(P.cbase, P.cocc, s) < big) { atomicMax(&P.cneed[s], big); bad = 1; }
+        if (room(P.rbase, P.rocc, r) < 4) { atomicMax(&P.rneed[r], 4); bad = 1; }
+        if (room(P.cbase, P.cocc, r) < 4) { atomicMax(&P.cneed[r], 4); bad = 1; }
+    }
+    for (int j = lane; j < cap; j += WAVE) {
+        int b = tab[j];
+        if (b < 0 || b == r || b == s) continue;
+        if (room(P.rbase, P.rocc, b) < 2) { atomicMax(&P.rneed[b], 2); bad = 1; }
+        if (room(P.cbase, P.cocc, b) < 2) { atomicMax(&P.cneed[b], 2); bad = 1; }
+    }
+    return wave_or(bad) == 0;
+}
+
+// Applies the accepted move: block matrix cells from (o_b, i_b, self), block degrees with the
+// update rule of Blockmodel::move_vertex (blockmodel.cpp:344-360), the assignment.
+__device__ void commit_move(const MhParams& P, const MoveResult& R, const int* tab, int cap) {
+    const int lane = lane_id();
+    const int r = R.r, s = R.s;
+    const int* to = tab + cap;
+    const int* ti = tab + 2 * cap;
+    for (int j = lane; j < cap; j += WAVE) {
+        int b = tab[j];
+        if (b < 0 || b == r || b == s) continue;
+        int o = to[j], i = ti[j];
+        if (o) {
+            cell_add(P, s, b, o);
+            cell_add(P, r, b, -o);
+        }
+        if (i) {
+            cell_add(P, b, s, i);
+            cell_add(P, b, r, -i);
+        }
+    }
+    if (lane == 0) {
+        if (R.d_ss) cell_add(P, s, s, R.d_ss);
+        if (R.d_rr) cell_add(P, r, r, R.d_rr);
+        if (R.d_sr) cell_add(P, s, r, R.d_sr);
+        if (R.d_rs) cell_add(P, r, s, R.d_rs);
+        P.assign[R.v] = s;
+        const int nout_r = P.dout[r] - R.k_out, nout_s = P.dout[s] + R.k_out;
+        const int nin_r = P.din[r] - (R.k_in + R.self), nin_s = P.din[s] + (R.k_in + R.self);
+        P.dout[r] = nout_r;
+        P.dout[s] = nout_s;
+        P.din[r] = nin_r;
+        P.din[s] = nin_s;
+        // self edges read after the update, plus the Delta diagonal (blockmodel.cpp:348-359)
+        P.deg[r] = nout_r + nin_r - ((R.m_rr + R.d_rr) + R.d_rr);
+        P.deg[s] = nout_s + nin_s - ((R.m_ss + R.d_ss) + R.d_ss);
+        P.drift[r] = R.d_rr;
+        P.drift[s] = R.d_ss;
+    }
+}
+
+__global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
+    __shared__ int st[MH_WAVES];
+    __shared__ double sdS[MH_WAVES];
+    __shared__ int s_flag;
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    int* tab = P.tables + (size_t)wave * 7 * P.tcap;
+    const GraphView G = P.G;
+    const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
+    const HashView M{P.rbase, P.rkeys, P.rvals, P.cbase, P.ckeys, P.cvals, P.B};
+    long long att = 0, evs = 0, mvs = 0;
+    double dsum = 0.0;
+    int pos = P.start;
+    int stopped = 0;
+    while (pos < P.n) {
+        const int idx = pos + wave;
+        MoveResult R{};
+        const bool active = idx < P.n;
+        if (active) R = eval_move<HashView>(G, S, M, P.list[idx], tab, P.tcap, P.seed, P.call, P.tag, -1, 0.0, false);
+        if (lane == 0) {
+            st[wave] = active ? R.status : 3;
+            sdS[wave] = active ? R.dS : 0.0;
+        }
+        if (threadIdx.x == 0) s_flag = 0;
+        __syncthreads();
+        const int W = min(MH_WAVES, P.n - pos);
+        int j = -1;
+        for (int w = 0; w < W; ++w) {
+            if (st[w] == 2 || st[w] < 0) {
+                j = w;
+                break;
+            }
+        }
+        if (j >= 0 && st[j] < 0) {
+            if (threadIdx.x == 0) atomicOr(P.err, DEV_ERR_TABLE);
+            stopped = 2;
+            pos += j;
+            break;
+        }
+        if (wave == j) {
+            bool ok = commit_fits(P, R, tab, P.tcap);
+            if (ok) commit_move(P, R, tab, P.tcap);
+            if (lane == 0) s_flag = ok ? 1 : 2;
+        }
+        __syncthreads();
+        const int flag = s_flag;
+        const int done = (j >= 0) ? j : W;  // vertices before the accepted one were rejected or idle
+        if (threadIdx.x == 0) {
+            for (int w = 0; w < done; ++w) {
+                att += st[w] != 3;
+                evs += st[w] == 1;
+            }
+            if (j >= 0 && flag == 1) {
+                att++;
+                evs++;
+                mvs++;
+                dsum += sdS[j];
+            }
+        }
+        if (flag == 2) {
+            stopped = 1;
+            pos += j;
+            break;
+        }
+        pos += (j >= 0) ? j + 1 : W;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        P.status[0] = pos;
+        P.status[1] = stopped;
+        atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
+        atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
+        atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)mvs);
+        P.dS_sum[0] += dsum;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+static void build_tables(Ctx& c) {
+    const int B = c.bm.B;
+    hipStream_t st = c.stream;
+    MhState& h = c.mh;
+    const int tb = 256;
+    // capacities and bases, rows then columns
+    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.row_ptr, h.rneed, B, c.tmp_b);
+    scan_exclusive(c.tmp_b, h.rbase, B, c.scan_tmp, st);
+    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.col_ptr, h.cneed, B, c.tmp_b);
+    scan_exclusive(c.tmp_b, h.cbase, B, c.scan_tmp, st);
+    int tot[2] = {0, 0};
+    SBPX_HIP(hipMemcpyAsync(&tot[0], h.rbase + B, sizeof(int), hipMemcpyDeviceToHost, st));
+    SBPX_HIP(hipMemcpyAsync(&tot[1], h.cbase + B, sizeof(int), hipMemcpyDeviceToHost, st));
+    SBPX_HIP(hipStreamSynchronize(st));
+    const size_t need = (size_t)std::max(tot[0], tot[1]);
+    if (need > h.slots) {
+        for (int** p : {&h.rkeys, &h.rvals, &h.ckeys, &h.cvals}) {
+            if (*p) SBPX_HIP(hipFree(*p));
+            SBPX_HIP(hipMalloc((void**)p, sizeof(int) * need));
+        }
+        h.slots = need;
+    }
+    const int wpb = tb / WAVE;
+    hash_fill_kernel<<<(B + wpb - 1) / wpb, tb, 0, st>>>(c.bm.row_ptr, c.bm.row_col, c.bm.row_val, B, h.rbase,
+                                                          h.rkeys, h.rvals, h.rocc);
+    hash_fill_kernel<<<(B + wpb - 1) / wpb, tb, 0, st>>>(c.bm.col_ptr, c.bm.col_row, c.bm.col_val, B, h.cbase,
+                                                          h.ckeys, h.cvals, h.cocc);
+    SBPX_HIP(hipGetLastError());
+}
+
+void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag) {
+    if (n <= 0) return;
+    MhState& h = c.mh;
+    const int B = c.bm.B;
+    int tcap = 16;
+    while (tcap < 2 * (max_deg + 2)) tcap <<= 1;
+    const size_t tslots = (size_t)MH_WAVES * 7 * tcap;
+    if (tslots > h.table_slots) {
+        if (h.tables) SBPX_HIP(hipFree(h.tables));
+        SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
+        h.table_slots = tslots;
+    }
+    SBPX_HIP(hipMemsetAsync(h.rneed, 0, sizeof(int) * B, c.stream));
+    SBPX_HIP(hipMemsetAsync(h.cneed, 0, sizeof(int) * B, c.stream));
+    int pos = 0;
+    int rounds = 0;
+    while (pos < n) {
+        if (rounds++ > 64) throw StateError("Metropolis-Hastings pass keeps running out of table room");
+        build_tables(c);
+        MhParams P{};
+        P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx};
+        P.assign = c.bm.assign;
+        P.dout = c.bm.dout;
+        P.din = c.bm.din;
+        P.deg = c.bm.deg;
+        P.drift = c.bm.drift;
+        P.B = B;
+        P.rbase = h.rbase;
+        P.rkeys = h.rkeys;
+        P.rvals = h.rvals;
+        P.rocc = h.rocc;
+        P.rneed = h.rneed;
+        P.cbase = h.cbase;
+        P.ckeys = h.ckeys;
+        P.cvals = h.cvals;
+        P.cocc = h.cocc;
+        P.cneed = h.cneed;
+        P.list = d_list;
+        P.n = n;
+        P.start = pos;
+        P.seed = seed;
+        P.call = call;
+        P.tag = tag;
+        P.tables = h.tables;
+        P.tcap = tcap;
+        P.status = h.status;
+        P.counters = c.counters;
+        P.dS_sum = c.red + 2048;
+        P.err = c.dev_err;
+        mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
+        SBPX_HIP(hipGetLastError());
+        int stat[2];
+        SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipStreamSynchronize(c.stream));
+        check_device_error(c);
+        pos = stat[0];
+        // the matrix changed: bring the CSR snapshot up to date (dout/din exact, deg keeps drift)
+        rebuild_keep_drift(c);
+        if (stat[1] == 0 && pos < n) throw StateError("Metropolis-Hastings pass stopped early");
+    }
+}
+
+}  // namespace sbpx

@@ -1,0 +1,267 @@
+// Batched vertex-move evaluation (one wavefront per vertex, asynchronous Gibbs semantics:
+// every vertex of a batch is evaluated against the same snapshot) and batch application. The
+// evaluation itself is eval_move in eval_core.hpp, read through the CSR snapshot view.
+#include <algorithm>
+#include <climits>
+
+#include "eval_core.hpp"
+#include "internal.hpp"
+
+namespace sbpx {
+
+struct EvalParams {
+    GraphView G;
+    BlockState S;
+    CsrView M;
+    const int* __restrict__ list;
+    int n;
+    const int* __restrict__ given_s;
+    const double* __restrict__ given_u;
+    uint64_t seed;
+    uint32_t call;
+    uint32_t tag;
+    int out_by_vertex;  // 1: outputs at index v; 0: at out_offset + position
+    int out_offset;
+    int propose_only;
+    int* __restrict__ out_s;
+    uint8_t* __restrict__ out_stat;  // 0 no move proposed, 1 evaluated and rejected, 2 accepted
+    double* __restrict__ out_dS;
+    double* __restrict__ out_H;
+    double* __restrict__ out_u;
+    long long* __restrict__ bytes;
+    int* __restrict__ err;
+    int* __restrict__ pool;
+    const int* __restrict__ pool_off;
+};
+
+__device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
+    const int v = P.list[idx];
+    const int gs = P.given_s ? P.given_s[idx] : -1;
+    const double gu = P.given_u ? P.given_u[idx] : 0.0;
+    MoveResult R = eval_move<CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0);
+    if (lane_id() != 0) return;
+    if (R.status < 0) {
+        atomicOr(P.err, DEV_ERR_TABLE);
+        return;
+    }
+    if (R.status >= 1 && (__builtin_isnan(R.dS) || __builtin_isinf(R.dS))) atomicOr(P.err, DEV_ERR_NAN);
+    const int o = P.out_by_vertex ? v : P.out_offset + idx;
+    P.out_s[o] = R.s;
+    if (P.out_stat) P.out_stat[o] = (uint8_t)R.status;
+    if (P.out_dS) P.out_dS[o] = R.dS;
+    if (P.out_H) P.out_H[o] = R.H;
+    if (P.out_u) P.out_u[o] = R.u;
+    if (P.bytes) {
+        // algorithmic bytes per evaluation: CSR/CSC pointers, own block, neighbour ids and blocks,
+        // proposal rows (row t + column t), delta-MDL rows (rows/columns r and s), degree
+        // lookups and the Hastings cells, result record; an attempt with s == r stops after
+        // the proposal.
+        long long k = R.k_out + R.k_in;
+        long long nnz_t = R.t >= 0 ? (long long)P.M.nnz_row(R.t) + P.M.nnz_col(R.t) : 0;
+        long long b = 16 + 4 + 8 * k + 8 * nnz_t;
+        if (R.status >= 1) b += 8 * R.nnz_rs + 12LL * (R.nuniq + 2) + 8LL * R.nuniq + 16;
+        atomicAdd((unsigned long long*)P.bytes, (unsigned long long)b);
+    }
+}
+
+constexpr int SMALL_CAP = 256;
+constexpr int SMALL_WAVES = 4;
+constexpr int LARGE_CAP = 4096;
+
+__global__ void __launch_bounds__(SMALL_WAVES * WAVE) eval_small_kernel(EvalParams P) {
+    __shared__ int tab[SMALL_WAVES][7 * SMALL_CAP];
+    const int wave = threadIdx.x / WAVE;
+    const int idx = blockIdx.x * SMALL_WAVES + wave;
+    if (idx >= P.n) return;
+    eval_vertex(P, tab[wave], SMALL_CAP, idx);
+}
+
+__global__ void __launch_bounds__(WAVE) eval_large_kernel(EvalParams P) {
+    __shared__ int tab[7 * LARGE_CAP];
+    const int idx = blockIdx.x;
+    if (idx >= P.n) return;
+    eval_vertex(P, tab, LARGE_CAP, idx);
+}
+
+__global__ void __launch_bounds__(WAVE) eval_huge_kernel(EvalParams P) {
+    const int idx = blockIdx.x;
+    if (idx >= P.n) return;
+    const int base = P.pool_off[idx];
+    const int cap = (P.pool_off[idx + 1] - base) / 7;
+    eval_vertex(P, P.pool + base, cap, idx);
+}
+
+static EvalParams make_params(Ctx& c) {
+    EvalParams P{};
+    P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx};
+    P.S = BlockState{c.bm.assign, c.bm.dout, c.bm.din, c.bm.deg, c.bm.B};
+    P.M = CsrView{c.bm.row_ptr, c.bm.row_col, c.bm.row_val, c.bm.row_cum,
+                  c.bm.col_ptr, c.bm.col_row, c.bm.col_val, c.bm.col_cum};
+    P.err = c.dev_err;
+    P.pool = c.table_pool;
+    return P;
+}
+
+void launch_eval(Ctx& c, const EvalLaunch& L) {
+    if (L.n <= 0) return;
+    EvalParams P = make_params(c);
+    P.list = L.list;
+    P.n = L.n;
+    P.given_s = L.given_s;
+    P.given_u = L.given_u;
+    P.seed = L.seed;
+    P.call = L.call;
+    P.tag = L.tag;
+    P.out_by_vertex = L.out_by_vertex;
+    P.out_offset = L.out_offset;
+    P.propose_only = L.propose_only;
+    P.out_s = c.ev_s;
+    P.out_stat = c.ev_acc;
+    P.out_dS = c.ev_dS;
+    P.out_H = c.ev_H;
+    P.out_u = L.out_u;
+    P.bytes = L.bytes;
+    P.pool_off = L.huge_offsets;
+    if (L.bin == 0) {
+        eval_small_kernel<<<(L.n + SMALL_WAVES - 1) / SMALL_WAVES, SMALL_WAVES * WAVE, 0, c.stream>>>(P);
+    } else if (L.bin == 1) {
+        eval_large_kernel<<<L.n, WAVE, 0, c.stream>>>(P);
+    } else {
+        eval_huge_kernel<<<L.n, WAVE, 0, c.stream>>>(P);
+    }
+    SBPX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------------------------------
+// Batch application (finetune.cpp:58-71): every accepted move takes effect; the block matrix
+// and dout/din are rebuilt; for each block x the total degree keeps the drift that
+// Blockmodel::move_vertex (blockmodel.cpp:344-360) leaves when the moves are applied one by
+// one in ascending vertex id: deg[x] = exact - Delta_last[x,x], where Delta_last is the Delta of
+// the last move (highest vertex id) touching x, evaluated with the blocks neighbours have at
+// that point (moved earlier in the batch => new block).
+constexpr int APPLY_TB = 256;
+
+__global__ void mark_moves_kernel(const int* __restrict__ list, int n, const int* __restrict__ out_s,
+                                  const uint8_t* __restrict__ out_stat, const int* __restrict__ assign, int epoch,
+                                  int* __restrict__ moved_epoch, int* __restrict__ new_block,
+                                  unsigned long long* __restrict__ last_touch, long long* __restrict__ counters) {
+    __shared__ long long cnt[3];
+    if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    long long a = 0, e = 0, m = 0;
+    if (i < n) {
+        int v = list[i];
+        int st = out_stat[v];
+        a = 1;
+        e = st >= 1;
+        m = st == 2;
+        if (st == 2) {
+            int s = out_s[v], r = assign[v];
+            moved_epoch[v] = epoch;
+            new_block[v] = s;
+            unsigned long long key = ((unsigned long long)(unsigned)epoch << 32) | (unsigned)v;
+            atomicMax(&last_touch[r], key);
+            atomicMax(&last_touch[s], key);
+        }
+    }
+    a = wave_sum_ll(a);
+    e = wave_sum_ll(e);
+    m = wave_sum_ll(m);
+    if (lane_id() == 0) {
+        atomicAdd((unsigned long long*)&cnt[0], (unsigned long long)a);
+        atomicAdd((unsigned long long*)&cnt[1], (unsigned long long)e);
+        atomicAdd((unsigned long long*)&cnt[2], (unsigned long long)m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) atomicAdd((unsigned long long*)&counters[threadIdx.x], (unsigned long long)cnt[threadIdx.x]);
+}
+
+// One wavefront per batch vertex; only the last mover of a block computes its drift.
+__global__ void drift_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
+                             const int* __restrict__ out_ptr, const int* __restrict__ out_idx,
+                             const int* __restrict__ in_ptr, const int* __restrict__ in_idx,
+                             const int* __restrict__ assign, int epoch, const int* __restrict__ moved_epoch,
+                             const int* __restrict__ new_block, const unsigned long long* __restrict__ last_touch,
+                             int* __restrict__ drift) {
+    const int idx = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (idx >= n) return;
+    const int v = list[idx];
+    if (out_stat[v] != 2) return;
+    const int lane = lane_id();
+    const int r = assign[v], s = new_block[v];
+    const unsigned long long key = ((unsigned long long)(unsigned)epoch << 32) | (unsigned)v;
+    const bool do_r = last_touch[r] == key, do_s = last_touch[s] == key;
+    if (!do_r && !do_s) return;
+    int cr = 0, cs = 0, self = 0;
+    for (int j = out_ptr[v] + lane; j < out_ptr[v + 1]; j += WAVE) {
+        int w = out_idx[j];
+        if (w == v) {
+            self = 1;
+            continue;
+        }
+        int b = (w < v && moved_epoch[w] == epoch) ? new_block[w] : assign[w];
+        cr += b == r;
+        cs += b == s;
+    }
+    for (int j = in_ptr[v] + lane; j < in_ptr[v + 1]; j += WAVE) {
+        int w = in_idx[j];
+        if (w == v) continue;
+        int b = (w < v && moved_epoch[w] == epoch) ? new_block[w] : assign[w];
+        cr += b == r;
+        cs += b == s;
+    }
+    cr = (int)wave_sum_ll(cr);
+    cs = (int)wave_sum_ll(cs);
+    self = wave_or(self);
+    if (lane == 0) {
+        if (do_r) drift[r] = -(cr + self);
+        if (do_s) drift[s] = cs + self;
+    }
+}
+
+__global__ void commit_moves_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
+                                    const int* __restrict__ new_block, int* __restrict__ assign) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int v = list[i];
+    if (out_stat[v] == 2) assign[v] = new_block[v];
+}
+
+// Sums dS over the accepted moves of the batch (vertex-indexed outputs).
+__global__ void sum_accepted_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
+                                    const double* __restrict__ out_dS, double* __restrict__ out) {
+    __shared__ double lds[APPLY_TB / WAVE];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        int v = list[i];
+        if (out_stat[v] == 2) acc += out_dS[v];
+    }
+    acc = wave_sum(acc);
+    if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < APPLY_TB / WAVE; ++w) s += lds[w];
+        out[0] += s;
+    }
+}
+
+// list: the batch's vertices (any order); outputs are vertex-indexed.
+void apply_accepted(Ctx& c, const int* list, int n) {
+    if (n <= 0) return;
+    c.epoch++;
+    const int nb = (n + APPLY_TB - 1) / APPLY_TB;
+    mark_moves_kernel<<<nb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_s, c.ev_acc, c.bm.assign, c.epoch, c.moved_epoch,
+                                                     c.new_block, c.last_touch, c.counters);
+    sum_accepted_kernel<<<1, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.ev_dS, c.red + 2048);
+    const int wpb = APPLY_TB / WAVE;
+    drift_kernel<<<(n + wpb - 1) / wpb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.g.out_ptr, c.g.out_idx, c.g.in_ptr,
+                                                                c.g.in_idx, c.bm.assign, c.epoch, c.moved_epoch,
+                                                                c.new_block, c.last_touch, c.bm.drift);
+    commit_moves_kernel<<<nb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.new_block, c.bm.assign);
+    SBPX_HIP(hipGetLastError());
+    rebuild_keep_drift(c);
+}
+
+}  // namespace sbpx

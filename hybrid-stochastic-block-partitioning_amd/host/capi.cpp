@@ -1,0 +1,97 @@
+// Flat entry point over the C++ driver, for Python callers (bench.py, tests).
+#include <chrono>
+#include <cstring>
+#include <string>
+
+#include "device.hpp"
+#include "sbp_host.hpp"
+
+extern "C" {
+
+typedef struct sbp_run_stats {
+    int32_t num_blocks;
+    int32_t outer_iterations;
+    int64_t sweeps, attempts, evals, moves;
+    double seconds;       // wall time of stochastic_block_partition, graph already resident
+    double mcmc_seconds;  // wall time inside the finetune phases
+    double eval_ms[3];    // device time of the vertex-move evaluation kernels, per degree bin
+    double eval_bytes[3]; // their algorithmic bytes (when counting is on)
+    int64_t eval_launches[3];
+    double mdl;           // final MDL (float, as Blockmodel caches it)
+} sbp_run_stats;
+
+static void set_err(char* err, int errlen, const char* msg) {
+    if (err && errlen > 0) {
+        std::strncpy(err, msg, errlen - 1);
+        err[errlen - 1] = 0;
+    }
+}
+
+// A graph resident on one GPU, for repeated partitioning runs.
+void* sbp_graph_new(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, const int32_t* in_ptr,
+                    const int32_t* in_idx, int device, char* err, int errlen) {
+    try {
+        int64_t E = out_ptr[N];
+        return new Graph(N, std::vector<int32_t>(out_ptr, out_ptr + N + 1), std::vector<int32_t>(out_idx, out_idx + E),
+                         std::vector<int32_t>(in_ptr, in_ptr + N + 1), std::vector<int32_t>(in_idx, in_idx + E),
+                         device);
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+
+void sbp_graph_free(void* g) { delete (Graph*)g; }
+
+// sbp::stochastic_block_partition on a resident graph. algorithm: 0 = metropolis_hastings,
+// 1 = async_gibbs, 2 = hybrid_mcmc. Returns 0 on success, otherwise writes a message into err.
+int sbp_partition(void* gp, int algorithm, int32_t batches, int approximate, uint64_t seed, int count_bytes,
+                  int32_t* assignment, sbp_run_stats* stats, char* err, int errlen) {
+    try {
+        Graph& graph = *(Graph*)gp;
+        const int N = graph.num_vertices();
+        sbpx_set_option(graph.device()->ctx, 1, count_bytes ? 1 : 0);
+        Args a;
+        a.algorithm = algorithm == 0 ? "metropolis_hastings" : (algorithm == 1 ? "async_gibbs" : "hybrid_mcmc");
+        a.batches = batches;
+        a.approximate = approximate != 0;
+        a.seed = seed;
+        auto t0 = std::chrono::steady_clock::now();
+        Blockmodel bm = sbp::stochastic_block_partition(graph, a);
+        double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::vector<int> assign = bm.block_assignment();
+        if (assignment) std::memcpy(assignment, assign.data(), sizeof(int32_t) * N);
+        if (stats) {
+            stats->num_blocks = bm.getNum_blocks();
+            stats->outer_iterations = (int)sbp::get_intermediates().size() - 2;
+            stats->sweeps = finetune::totals.sweeps;
+            stats->attempts = finetune::totals.attempts;
+            stats->evals = finetune::totals.evals;
+            stats->moves = finetune::totals.moves;
+            stats->seconds = secs;
+            stats->mcmc_seconds = finetune::MCMC_time;
+            for (int b = 0; b < 3; ++b) {
+                stats->eval_ms[b] = finetune::totals.eval_ms[b];
+                stats->eval_bytes[b] = finetune::totals.eval_bytes[b];
+                stats->eval_launches[b] = finetune::totals.eval_launches[b];
+            }
+            stats->mdl = bm.getOverall_entropy();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return 1;
+    }
+}
+
+int sbp_run(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, const int32_t* in_ptr, const int32_t* in_idx,
+            int algorithm, int32_t batches, int approximate, uint64_t seed, int device, int count_bytes,
+            int32_t* assignment, sbp_run_stats* stats, char* err, int errlen) {
+    void* g = sbp_graph_new(N, out_ptr, out_idx, in_ptr, in_idx, device, err, errlen);
+    if (!g) return 1;
+    int rc = sbp_partition(g, algorithm, batches, approximate, seed, count_bytes, assignment, stats, err, errlen);
+    sbp_graph_free(g);
+    return rc;
+}
+
+}  // extern "C"

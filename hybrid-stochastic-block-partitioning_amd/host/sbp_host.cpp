@@ -1,0 +1,346 @@
+// The reference's driver, finetune and block-merge control flow on top of the sbpx C ABI.
+// Control flow follows src/sbp.cpp:61-112, src/finetune.cpp:31-89, 342-444, 144-162,
+// src/block_merge.cpp:278-311 and src/blockmodel/blockmodel_triplet.cpp:3-100; each sweep,
+// merge phase and MDL runs on the device.
+#include "sbp_host.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <numeric>
+#include <stdexcept>
+
+#include "device.hpp"
+
+Args args;
+
+namespace sbpx_host {
+
+static void check(sbpx_ctx* ctx, sbpx_status st, const char* what) {
+    if (st != SBPX_OK) throw std::runtime_error(std::string(what) + ": " + sbpx_last_error(ctx));
+}
+
+}  // namespace sbpx_host
+
+using sbpx_host::check;
+using sbpx_host::Device;
+
+// ---------------------------------------------------------------------------- Graph
+Graph::Graph(int num_vertices, std::vector<int32_t> out_ptr, std::vector<int32_t> out_idx, std::vector<int32_t> in_ptr,
+             std::vector<int32_t> in_idx, int device) {
+    num_vertices_ = num_vertices;
+    num_edges_ = (int)out_idx.size();
+    dev_ = std::make_shared<Device>();
+    check(nullptr, sbpx_create(&dev_->ctx, device), "sbpx_create");
+    dev_->N = num_vertices_;
+    dev_->E = num_edges_;
+    check(dev_->ctx,
+          sbpx_graph_upload(dev_->ctx, num_vertices, num_edges_, out_ptr.data(), out_idx.data(), in_ptr.data(),
+                            in_idx.data()),
+          "sbpx_graph_upload");
+    int nh = 0;
+    check(dev_->ctx, sbpx_graph_split(dev_->ctx, nullptr, &nh, nullptr), "sbpx_graph_split");
+    high_.resize(nh);
+    low_.resize(num_vertices - nh);
+    check(dev_->ctx, sbpx_graph_split(dev_->ctx, high_.data(), &nh, low_.data()), "sbpx_graph_split");
+}
+
+// ---------------------------------------------------------------------------- Blockmodel
+Blockmodel::Blockmodel(int num_blocks_, const Graph& graph, float rate) : empty(false) {
+    std::vector<int> identity(graph.num_vertices());
+    std::iota(identity.begin(), identity.end(), 0);
+    *this = Blockmodel(num_blocks_, graph, rate, identity);
+}
+
+// Blockmodel(num_blocks, graph, rate, assignment) (blockmodel.hpp:63-73).
+Blockmodel::Blockmodel(int num_blocks_, const Graph& graph, float rate, std::vector<int>& block_assignment)
+    : empty(false) {
+    dev_ = graph.device();
+    id_ = dev_->next_id++;
+    num_blocks = num_blocks_;
+    block_reduction_rate = rate;
+    num_blocks_to_merge = (int)(num_blocks * block_reduction_rate);
+    check(dev_->ctx, sbpx_blockmodel_build(dev_->ctx, num_blocks, block_assignment.data()), "sbpx_blockmodel_build");
+    dev_->live_id = id_;
+}
+
+sbpx_ctx* Blockmodel::ctx() const { return dev_->ctx; }
+void Blockmodel::mark_live() const { dev_->live_id = id_; }
+
+void Blockmodel::make_live() const {
+    if (empty || !dev_) throw std::runtime_error("empty blockmodel");
+    if (dev_->live_id == id_) return;
+    if ((int)snapshot_.size() != dev_->N) throw std::runtime_error("blockmodel state was overwritten");
+    check(dev_->ctx, sbpx_blockmodel_build(dev_->ctx, num_blocks, snapshot_.data()), "sbpx_blockmodel_build");
+    dev_->live_id = id_;
+}
+
+// Blockmodel::copy (blockmodel.cpp:95-105): a deep copy with num_blocks_to_merge = 0.
+Blockmodel Blockmodel::copy() {
+    Blockmodel c;
+    c.empty = empty;
+    if (empty) return c;
+    c.dev_ = dev_;
+    c.id_ = dev_->next_id++;
+    c.num_blocks = num_blocks;
+    c.num_blocks_to_merge = 0;
+    c.block_reduction_rate = block_reduction_rate;
+    c.overall_entropy = overall_entropy;
+    c.snapshot_ = block_assignment();
+    return c;
+}
+
+std::vector<int> Blockmodel::block_assignment() const {
+    if (dev_->live_id != id_) return snapshot_;
+    std::vector<int> a(dev_->N);
+    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, a.data(), nullptr, nullptr, nullptr), "download");
+    return a;
+}
+std::vector<int> Blockmodel::degrees() const {
+    make_live();
+    std::vector<int> d(num_blocks);
+    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, nullptr, nullptr, nullptr, d.data()), "download");
+    return d;
+}
+std::vector<int> Blockmodel::degrees_in() const {
+    make_live();
+    std::vector<int> d(num_blocks);
+    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, nullptr, nullptr, d.data(), nullptr), "download");
+    return d;
+}
+std::vector<int> Blockmodel::degrees_out() const {
+    make_live();
+    std::vector<int> d(num_blocks);
+    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, nullptr, d.data(), nullptr, nullptr), "download");
+    return d;
+}
+
+// ---------------------------------------------------------------------------- triplet
+bool BlockmodelTriplet::golden_ratio_not_reached() { return get(2).empty; }
+
+bool BlockmodelTriplet::is_done() {
+    if ((!get(0).empty && get(0).getNum_blocks() - get(2).getNum_blocks() == 2) ||
+        (get(0).empty && get(1).getNum_blocks() - get(2).getNum_blocks() == 1)) {
+        optimal_num_blocks_found = true;
+    }
+    return optimal_num_blocks_found;
+}
+
+int BlockmodelTriplet::lower_difference() { return get(1).getNum_blocks() - get(2).getNum_blocks(); }
+int BlockmodelTriplet::upper_difference() { return get(0).getNum_blocks() - get(1).getNum_blocks(); }
+
+void BlockmodelTriplet::update(Blockmodel& blockmodel) {
+    int index;
+    if (blockmodels[1].empty) {
+        index = 1;
+    } else {
+        if (blockmodel.getOverall_entropy() <= blockmodels[1].getOverall_entropy()) {
+            int old_index = (get(1).getNum_blocks() > blockmodel.getNum_blocks()) ? 0 : 2;
+            blockmodels[old_index] = blockmodels[1].copy();
+            index = 1;
+        } else {
+            index = (blockmodels[1].getNum_blocks() > blockmodel.getNum_blocks()) ? 2 : 0;
+        }
+    }
+    blockmodels[index] = blockmodel.copy();
+}
+
+Blockmodel BlockmodelTriplet::get_next_blockmodel(Blockmodel& old_blockmodel) {
+    old_blockmodel.setNum_blocks_to_merge(0);
+    update(old_blockmodel);
+    if (golden_ratio_not_reached()) {
+        Blockmodel blockmodel = get(1).copy();
+        blockmodel.setNum_blocks_to_merge(int(blockmodel.getNum_blocks() * BLOCK_REDUCTION_RATE));
+        if (blockmodel.getNum_blocks_to_merge() == 0) optimal_num_blocks_found = true;
+        return blockmodel;
+    }
+    if (is_done()) return get(1).copy();
+    int index = 1;
+    if (get(0).empty && get(1).getNum_blocks() > get(2).getNum_blocks()) index = 1;
+    else if (upper_difference() >= lower_difference()) index = 0;
+    else index = 1;
+    int next_num_blocks_to_try = get(index + 1).getNum_blocks();
+    next_num_blocks_to_try += int((get(index).getNum_blocks() - get(index + 1).getNum_blocks()) * 0.618);
+    Blockmodel blockmodel = get(index).copy();
+    blockmodel.setNum_blocks_to_merge(blockmodel.getNum_blocks() - next_num_blocks_to_try);
+    return blockmodel;
+}
+
+// ---------------------------------------------------------------------------- entropy
+namespace entropy {
+double mdl(const Blockmodel& blockmodel, int num_vertices, int num_edges) {
+    (void)num_vertices;
+    (void)num_edges;
+    blockmodel.make_live();
+    double out = 0.0;
+    check(blockmodel.ctx(), sbpx_mdl(blockmodel.ctx(), &out), "sbpx_mdl");
+    return out;
+}
+double null_mdl_v1(long num_edges) {
+    double log_posterior_p = num_edges * log(1.0 / num_edges);
+    double x = 1.0 / num_edges;
+    double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    return (num_edges * h) - log_posterior_p;
+}
+double null_mdl_v2(long num_vertices, long num_edges) {
+    double log_posterior_p = num_edges * log(1.0 / num_edges);
+    double x = pow(num_vertices, 2) / num_edges;
+    double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    return (num_edges * h) + (num_vertices * log(num_vertices)) - log_posterior_p;
+}
+double normalize_mdl_v1(double mdl, long num_edges) { return mdl / null_mdl_v1(num_edges); }
+double normalize_mdl_v2(double mdl, long num_vertices, long num_edges) {
+    return mdl / null_mdl_v2(num_vertices, num_edges);
+}
+}  // namespace entropy
+
+// ---------------------------------------------------------------------------- finetune
+namespace finetune {
+int MCMC_iterations = 0;
+double MCMC_time = 0.0;
+Totals totals;
+
+bool early_stop(int iteration, BlockmodelTriplet& blockmodels, double initial_entropy,
+                std::vector<double>& delta_entropies) {
+    size_t last_index = delta_entropies.size() - 1;
+    if (delta_entropies[last_index] == 0.0) return true;
+    if (iteration < 3) return false;
+    double average = delta_entropies[last_index] + delta_entropies[last_index - 1] + delta_entropies[last_index - 2];
+    average /= -3.0;
+    double threshold = blockmodels.get(2).empty ? 5e-4 * initial_entropy : 1e-4 * initial_entropy;
+    return average < threshold;
+}
+
+static void add_totals(const sbpx_sweep_stats& st) {
+    totals.sweeps++;
+    totals.attempts += st.attempts;
+    totals.evals += st.evals;
+    totals.moves += st.moves;
+    for (int b = 0; b < 3; ++b) {
+        totals.eval_ms[b] += st.eval_ms[b];
+        totals.eval_bytes[b] += st.eval_bytes[b];
+        totals.eval_launches[b] += st.eval_launches[b];
+    }
+}
+
+// asynchronous_gibbs (finetune.cpp:31-89) and hybrid_mcmc (finetune.cpp:342-410): the entropy
+// change is measured from the phase's start (last_iteration_entropy is never advanced).
+static Blockmodel& batched_mcmc(int algo, Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels) {
+    if (blockmodel.getNum_blocks() == 1) return blockmodel;
+    blockmodel.make_live();
+    Device& dev = *graph.device();
+    std::vector<double> delta_entropies;
+    blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
+    double initial_entropy = blockmodel.getOverall_entropy();
+    double last_iteration_entropy = initial_entropy;
+    for (int iteration = 0; iteration < MAX_NUM_ITERATIONS; ++iteration) {
+        sbpx_sweep_stats st{};
+        check(dev.ctx, sbpx_sweep(dev.ctx, algo, args.batches, args.seed, dev.sweep_call++, &st), "sbpx_sweep");
+        add_totals(st);
+        double entropy = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
+        delta_entropies.push_back(entropy - last_iteration_entropy);
+        MCMC_iterations++;
+        if (early_stop(iteration, blockmodels, initial_entropy, delta_entropies)) break;
+    }
+    blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
+    return blockmodel;
+}
+
+Blockmodel& asynchronous_gibbs(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels) {
+    return batched_mcmc(SBPX_ALGO_ASYNC_GIBBS, blockmodel, graph, blockmodels);
+}
+
+Blockmodel& hybrid_mcmc(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels) {
+    return batched_mcmc(SBPX_ALGO_HYBRID, blockmodel, graph, blockmodels);
+}
+
+// metropolis_hastings (finetune.cpp:412-444): per-iteration sums of accepted delta MDL, measured
+// against the float entropy cached at the start.
+Blockmodel& metropolis_hastings(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels) {
+    if (blockmodel.getNum_blocks() == 1) return blockmodel;
+    blockmodel.make_live();
+    Device& dev = *graph.device();
+    std::vector<double> delta_entropies;
+    blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
+    for (int iteration = 0; iteration < MAX_NUM_ITERATIONS; ++iteration) {
+        sbpx_sweep_stats st{};
+        check(dev.ctx,
+              sbpx_sweep(dev.ctx, SBPX_ALGO_METROPOLIS_HASTINGS, args.batches, args.seed, dev.sweep_call++, &st),
+              "sbpx_sweep");
+        add_totals(st);
+        delta_entropies.push_back(st.dS_accepted);
+        MCMC_iterations++;
+        if (early_stop(iteration, blockmodels, blockmodel.getOverall_entropy(), delta_entropies)) break;
+    }
+    blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
+    return blockmodel;
+}
+}  // namespace finetune
+
+// ---------------------------------------------------------------------------- block merge
+namespace block_merge {
+Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edges) {
+    (void)num_edges;
+    blockmodel.make_live();
+    Device& dev = *graph.device();
+    int mode = args.approximate ? SBPX_MERGE_APPROXIMATE : SBPX_MERGE_ADVANCED;
+    check(dev.ctx,
+          sbpx_merge_blocks(dev.ctx, blockmodel.getNum_blocks_to_merge(), mode, args.seed, dev.merge_call++),
+          "sbpx_merge_blocks");
+    int32_t B = 0;
+    check(dev.ctx, sbpx_blockmodel_info(dev.ctx, &B, nullptr), "sbpx_blockmodel_info");
+    blockmodel.setNum_blocks(B);
+    blockmodel.mark_live();
+    return blockmodel;
+}
+}  // namespace block_merge
+
+// ---------------------------------------------------------------------------- driver
+namespace sbp {
+static std::vector<Intermediate> intermediate_results;
+std::vector<Intermediate> get_intermediates() { return intermediate_results; }
+
+static void add_intermediate(float iteration, const Blockmodel& blockmodel, double mdl) {
+    intermediate_results.push_back(Intermediate{iteration, mdl, blockmodel.getNum_blocks(),
+                                                finetune::MCMC_iterations, finetune::MCMC_time});
+}
+
+bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks) {
+    if (min_num_blocks > 0) {
+        if ((blockmodel.getNum_blocks() <= min_num_blocks) || !blockmodel_triplet.get(2).empty) return true;
+    }
+    return blockmodel_triplet.optimal_num_blocks_found;
+}
+
+// sbp::stochastic_block_partition (sbp.cpp:61-98).
+Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
+    args = a;
+    intermediate_results.clear();
+    finetune::MCMC_iterations = 0;
+    finetune::MCMC_time = 0.0;
+    finetune::totals = finetune::Totals();
+    Device& dev = *graph.device();
+    dev.sweep_call = 0;
+    dev.merge_call = 0;
+    Blockmodel blockmodel(graph.num_vertices(), graph, float(BLOCK_REDUCTION_RATE));
+    double initial_mdl = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
+    add_intermediate(0, blockmodel, initial_mdl);
+    BlockmodelTriplet blockmodel_triplet = BlockmodelTriplet();
+    float iteration = 0;
+    while (!done_blockmodeling(blockmodel, blockmodel_triplet)) {
+        blockmodel = block_merge::merge_blocks(blockmodel, graph, graph.num_edges());
+        auto start = std::chrono::steady_clock::now();
+        if (args.algorithm == "async_gibbs" && iteration < float(args.asynciterations))
+            blockmodel = finetune::asynchronous_gibbs(blockmodel, graph, blockmodel_triplet);
+        else if (args.algorithm == "hybrid_mcmc")
+            blockmodel = finetune::hybrid_mcmc(blockmodel, graph, blockmodel_triplet);
+        else
+            blockmodel = finetune::metropolis_hastings(blockmodel, graph, blockmodel_triplet);
+        finetune::MCMC_time +=
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+        add_intermediate(++iteration, blockmodel, blockmodel.getOverall_entropy());
+        blockmodel = blockmodel_triplet.get_next_blockmodel(blockmodel);
+    }
+    add_intermediate(-1, blockmodel, blockmodel.getOverall_entropy());
+    return blockmodel;
+}
+}  // namespace sbp

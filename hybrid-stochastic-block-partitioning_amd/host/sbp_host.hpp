@@ -1,0 +1,144 @@
+// The reference's C++ API for the stochastic-block-partitioning path, on top of the sbpx C ABI.
+// Names, argument meaning and control flow follow the reference (include/sbp.hpp,
+// include/finetune.hpp, include/block_merge.hpp, include/entropy.hpp,
+// include/blockmodel/blockmodel.hpp, include/blockmodel/blockmodel_triplet.hpp); the data live
+// on one GPU. Errors raise std::runtime_error with the library's message.
+#pragma once
+#include <climits>
+#include <cstdint>
+#include <limits>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sbpx.h"
+
+/// Hot-path flags of include/args.hpp:13-130 plus the seed and device this build adds.
+struct Args {
+    std::string algorithm = "async_gibbs";  // async_gibbs | metropolis_hastings | hybrid_mcmc
+    bool approximate = false;
+    int asynciterations = INT_MAX;
+    int batches = 10;
+    bool transpose = true;  // the device keeps both orientations; kept for flag parity
+    int threads = 0;        // unused on the device path
+    uint64_t seed = 0;
+    int device = 0;
+};
+
+namespace sbpx_host {
+struct Device;  // one context per graph
+}
+
+class Graph {
+public:
+    Graph() = default;
+    Graph(int num_vertices, std::vector<int32_t> out_ptr, std::vector<int32_t> out_idx, std::vector<int32_t> in_ptr,
+          std::vector<int32_t> in_idx, int device = 0);
+    int num_vertices() const { return num_vertices_; }
+    int num_edges() const { return num_edges_; }
+    const std::vector<int>& high_degree_vertices() const { return high_; }
+    const std::vector<int>& low_degree_vertices() const { return low_; }
+    std::shared_ptr<sbpx_host::Device> device() const { return dev_; }
+
+private:
+    int num_vertices_ = 0, num_edges_ = 0;
+    std::vector<int> high_, low_;
+    std::shared_ptr<sbpx_host::Device> dev_;
+};
+
+static const double BLOCK_REDUCTION_RATE = 0.5;
+
+class Blockmodel {
+public:
+    Blockmodel() : empty(true) {}
+    Blockmodel(int num_blocks, const Graph& graph, float block_reduction_rate);
+    Blockmodel(int num_blocks, const Graph& graph, float block_reduction_rate, std::vector<int>& block_assignment);
+    Blockmodel copy();
+    std::vector<int> block_assignment() const;
+    std::vector<int> degrees() const;
+    std::vector<int> degrees_in() const;
+    std::vector<int> degrees_out() const;
+    int getNum_blocks() const { return num_blocks; }
+    void setNum_blocks(int n) { num_blocks = n; }
+    int& getNum_blocks_to_merge() { return num_blocks_to_merge; }
+    void setNum_blocks_to_merge(int n) { num_blocks_to_merge = n; }
+    float& getOverall_entropy() { return overall_entropy; }
+    void setOverall_entropy(double e) { overall_entropy = (float)e; }
+    float& getBlock_reduction_rate() { return block_reduction_rate; }
+    bool empty;
+
+    // device binding
+    void make_live() const;
+    sbpx_ctx* ctx() const;
+    void mark_live() const;
+
+private:
+    std::shared_ptr<sbpx_host::Device> dev_;
+    long id_ = -1;
+    int num_blocks = 0;
+    int num_blocks_to_merge = 0;
+    float block_reduction_rate = 0.5f;
+    float overall_entropy = std::numeric_limits<float>::max();
+    mutable std::vector<int> snapshot_;  // assignment, when another blockmodel holds the device
+};
+
+class BlockmodelTriplet {
+public:
+    BlockmodelTriplet() : optimal_num_blocks_found(false) {}
+    bool optimal_num_blocks_found;
+    void update(Blockmodel& blockmodel);
+    Blockmodel& get(int i) { return blockmodels[i]; }
+    bool golden_ratio_not_reached();
+    bool is_done();
+    Blockmodel get_next_blockmodel(Blockmodel& old_blockmodel);
+
+protected:
+    Blockmodel blockmodels[3];
+    int lower_difference();
+    int upper_difference();
+};
+
+namespace entropy {
+double mdl(const Blockmodel& blockmodel, int num_vertices, int num_edges);
+double null_mdl_v1(long num_edges);
+double null_mdl_v2(long num_vertices, long num_edges);
+double normalize_mdl_v1(double mdl, long num_edges);
+double normalize_mdl_v2(double mdl, long num_vertices, long num_edges);
+}  // namespace entropy
+
+namespace finetune {
+extern int MCMC_iterations;
+extern double MCMC_time;
+static const int MAX_NUM_ITERATIONS = 100;
+struct Totals {
+    long long sweeps = 0, attempts = 0, evals = 0, moves = 0;
+    double eval_ms[3] = {0, 0, 0}, eval_bytes[3] = {0, 0, 0};
+    long long eval_launches[3] = {0, 0, 0};
+};
+extern Totals totals;
+Blockmodel& asynchronous_gibbs(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
+Blockmodel& hybrid_mcmc(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
+Blockmodel& metropolis_hastings(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
+bool early_stop(int iteration, BlockmodelTriplet& blockmodels, double initial_entropy,
+                std::vector<double>& delta_entropies);
+}  // namespace finetune
+
+namespace block_merge {
+static const int NUM_AGG_PROPOSALS_PER_BLOCK = 10;
+Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edges);
+}  // namespace block_merge
+
+namespace sbp {
+struct Intermediate {
+    float iteration;
+    double mdl;
+    int num_blocks;
+    int mcmc_iterations;
+    double mcmc_time;
+};
+std::vector<Intermediate> get_intermediates();
+Blockmodel stochastic_block_partition(Graph& graph, Args& args);
+bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks = 0);
+}  // namespace sbp
+
+extern Args args;

@@ -1,0 +1,142 @@
+/*
+ * sbpx — C ABI of the MI355X stochastic-block-partitioning hot path.
+ *
+ * One context owns one GPU's copy of a graph (int32 CSR + CSC) and of a Blockmodel (block
+ * assignment, block matrix as CSR + CSC, block degrees). Host buffers are copied in and out;
+ * the library never keeps caller pointers. Calls on one context are serialised on the
+ * context's stream; distinct contexts are independent. Every entry point returns SBPX_OK or an
+ * error code, with a message in sbpx_last_error().
+ *
+ * The reference exposes this path as C++ free functions and classes, not as a C interface.
+ * Each entry point below names the reference function whose work it replaces (file:line in
+ * vtsynergy/Hybrid-Stochastic-Block-Partitioning). The C++ layer in
+ * hybrid-stochastic-block-partitioning_amd/host restores the reference's signatures on top.
+ *
+ * Randomness: proposals and accept uniforms come from Philox4x32-10 keyed by `seed`, with the
+ * counter (vertex or block, `call`, proposal index, tag). A call id names one sweep or one
+ * merge phase; the caller advances it.
+ */
+#ifndef SBPX_H
+#define SBPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sbpx_ctx sbpx_ctx;
+typedef int sbpx_status;
+
+#define SBPX_OK 0
+#define SBPX_ERR_ARG 1         /* bad argument: block id >= B, edge endpoint >= N, ... */
+#define SBPX_ERR_HIP 2         /* HIP runtime failure or out of device memory */
+#define SBPX_ERR_STATE 3       /* call out of order (no graph, no blockmodel) */
+#define SBPX_ERR_NUMERIC 4     /* zero block degree under a nonzero cell, NaN delta MDL */
+#define SBPX_ERR_UNSUPPORTED 5 /* configuration outside what this build handles */
+
+#define SBPX_ALGO_METROPOLIS_HASTINGS 0
+#define SBPX_ALGO_ASYNC_GIBBS 1
+#define SBPX_ALGO_HYBRID 2
+
+#define SBPX_MERGE_APPROXIMATE 0 /* Blockmodel::carry_out_best_merges */
+#define SBPX_MERGE_ADVANCED 1    /* block_merge::carry_out_best_merges_advanced */
+
+typedef struct sbpx_sweep_stats {
+    int64_t attempts;     /* proposals drawn */
+    int64_t evals;        /* proposals with s != r: delta MDL, Hastings and accept computed */
+    int64_t moves;        /* accepted moves */
+    double dS_accepted;   /* sum of delta MDL over accepted moves */
+    /* Per degree bin of the vertex-move evaluation kernels (0: degree <= 127, 1: <= 2047,
+     * 2: larger): device time, algorithmic bytes (when byte counting is on), launches. */
+    double eval_ms[3];
+    double eval_bytes[3];
+    int32_t eval_launches[3];
+    int32_t reserved;
+} sbpx_sweep_stats;
+
+/* Context. Replaces the process-global state of the reference (args, static RNG engines). */
+sbpx_status sbpx_create(sbpx_ctx** ctx, int device);
+void sbpx_destroy(sbpx_ctx* ctx);
+const char* sbpx_last_error(const sbpx_ctx* ctx);
+/* option 1 = count algorithmic bytes in the evaluation kernels (0/1). */
+sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
+
+/* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as
+ * given. Replaces Graph (graph.hpp:15-113) and its degree split (graph.cpp:15-22, 215-228). */
+sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t num_vertices, int64_t num_edges,
+                              const int32_t* out_ptr, const int32_t* out_idx,
+                              const int32_t* in_ptr, const int32_t* in_idx);
+/* High-degree (top 15%) and low-degree vertex lists, as Graph::sort_vertices orders them. */
+sbpx_status sbpx_graph_split(sbpx_ctx* ctx, int32_t* high, int32_t* num_high, int32_t* low);
+
+/* Blockmodel(num_blocks, graph, rate, assignment) + initialize_edge_counts
+ * (blockmodel.hpp:63-73, blockmodel.cpp:187-255). Resets the total-degree drift. */
+sbpx_status sbpx_blockmodel_build(sbpx_ctx* ctx, int32_t num_blocks, const int32_t* assignment);
+sbpx_status sbpx_blockmodel_info(sbpx_ctx* ctx, int32_t* num_blocks, int64_t* nnz);
+/* Any pointer may be NULL. deg includes the drift of Blockmodel::move_vertex
+ * (blockmodel.cpp:344-360). */
+sbpx_status sbpx_blockmodel_download(sbpx_ctx* ctx, int32_t* assignment, int32_t* dout,
+                                     int32_t* din, int32_t* deg);
+/* Nonzero cells, row-major sorted (from the CSR) or column-major sorted (from the CSC). */
+sbpx_status sbpx_blockmodel_coo(sbpx_ctx* ctx, int by_column, int32_t* rows, int32_t* cols,
+                                int32_t* vals);
+
+/* entropy::mdl (entropy.cpp:584-589) and Blockmodel::log_posterior_probability
+ * (blockmodel.cpp:267-285). */
+sbpx_status sbpx_mdl(sbpx_ctx* ctx, double* mdl);
+sbpx_status sbpx_log_posterior(sbpx_ctx* ctx, double* log_posterior);
+
+/* Evaluate fixed proposals (v[i] -> s[i], uniform u[i]) against the current state without
+ * applying them: entropy::delta_mdl (entropy.cpp:296-401), entropy::hastings_correction
+ * (entropy.cpp:495-557), finetune::accept (finetune.cpp:19-29). */
+sbpx_status sbpx_eval_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s,
+                            const double* u, double* dS, double* H, uint8_t* accept);
+/* The proposal each vertex draws on the current state (common::propose_new_block,
+ * common.cpp:131-192); tag 0 = asynchronous Gibbs stream, 1 = Metropolis-Hastings stream. */
+sbpx_status sbpx_propose(sbpx_ctx* ctx, int32_t n, const int32_t* v, uint64_t seed, uint32_t call,
+                         int tag, int32_t* s, double* u);
+/* Apply moves as one asynchronous batch does (finetune.cpp:58-71): all moves take effect, the
+ * block matrix and dout/din equal a rebuild, and deg carries the drift of applying them one by
+ * one in ascending vertex id. Each vertex may appear at most once. */
+sbpx_status sbpx_apply_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s);
+/* One MCMC sweep (one iteration of the loops in finetune.cpp:41-84, 356-405, 420-439). */
+sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, uint32_t call,
+                       sbpx_sweep_stats* stats);
+
+/* block_merge::merge_blocks proposal phase (block_merge.cpp:278-301, 335-368): best merge per
+ * block and its delta MDL (DBL_MAX marks a block whose proposals all repeated). */
+sbpx_status sbpx_merge_proposals(sbpx_ctx* ctx, uint64_t seed, uint32_t call, double* dS,
+                                 int32_t* best);
+/* Carry out num_to_merge merges from (dS, best) and rebuild (blockmodel.cpp:50-78,
+ * block_merge.cpp:123-217, :309). */
+sbpx_status sbpx_carry_out_merges(sbpx_ctx* ctx, int32_t num_to_merge, int mode, const double* dS,
+                                  const int32_t* best);
+/* Both steps: block_merge::merge_blocks (block_merge.cpp:278-311). */
+sbpx_status sbpx_merge_blocks(sbpx_ctx* ctx, int32_t num_to_merge, int mode, uint64_t seed,
+                              uint32_t call);
+
+/* Self-test of the device's FP32 log (the function the objective uses, utils.hpp:145-151):
+ * out[i] = log of the float whose bit pattern is lo + i, for i < n. */
+sbpx_status sbpx_selftest_logf(sbpx_ctx* ctx, uint32_t lo, uint32_t n, float* out);
+
+/* Synthetic degree-corrected SBM graph (host only; no GPU needed). Returns CSR/CSC sized by
+ * the caller after a first call with NULL outputs reports N and E. */
+typedef struct sbpx_dcsbm_params {
+    int32_t num_vertices;
+    int32_t num_blocks;
+    double p_in;          /* probability that an edge stays inside its source block */
+    double dirichlet;     /* concentration of the block-size Dirichlet */
+    int32_t deg_min, deg_max;
+    double exponent;      /* truncated power law P(k) ~ k^-exponent on [deg_min, deg_max] */
+    uint64_t seed;
+} sbpx_dcsbm_params;
+sbpx_status sbpx_generate_dcsbm(const sbpx_dcsbm_params* p, int64_t* num_edges, int32_t* out_ptr,
+                                int32_t* out_idx, int32_t* in_ptr, int32_t* in_idx,
+                                int32_t* truth);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBPX_H */

@@ -1,0 +1,303 @@
+"""GPU parity: the MI355X library, called through its C ABI, against the CPU oracle on the same
+seeded inputs. Integer state (block matrix, degrees, drift, assignments, proposals, accept
+decisions) must be bit-exact; a vertex-move delta MDL must agree to 1e-6 relative (FP32 cells,
+FP64 sums in a different order); the global MDL to 1e-9 relative."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from pkg import sbpx
+
+pytestmark = pytest.mark.gpu
+
+DS_RTOL = 1e-6
+MDL_RTOL = 1e-9
+
+
+def ctx_for(graph_arrays):
+    m = sbpx()
+    c = m.Context(0)
+    c.upload_graph(*graph_arrays)
+    return c
+
+
+def toy_arrays(toy):
+    return O.csr_from_edges(toy["num_vertices"], toy["edges"])
+
+
+def gen_arrays(n, K, p_in=0.8, alpha=20.0, dmin=10, dmax=100, seed=0):
+    m = sbpx()
+    out_ptr, out_idx, in_ptr, in_idx, truth = m.generate_dcsbm(n, K, p_in, alpha, dmin, dmax, 2.1, seed=seed)
+    return (out_ptr, out_idx, in_ptr, in_idx), truth
+
+
+def assert_same_state(c, bm, check_deg=True):
+    a, dout, din, deg = c.state()
+    oa, odout, odin, odeg = bm.state()
+    assert (a == oa).all(), "assignment"
+    assert (dout == odout).all(), "dout"
+    assert (din == odin).all(), "din"
+    if check_deg:
+        assert (deg == odeg).all(), ("deg", np.flatnonzero(deg != odeg)[:10])
+    r, col, v = c.coo()
+    orow, ocol, ov = bm.coo()
+    assert len(r) == len(orow), "nnz"
+    assert (r == orow).all() and (col == ocol).all() and (v == ov).all(), "CSR"
+    r2, c2, v2 = c.coo(by_column=True)
+    order = np.lexsort((orow, ocol))
+    assert (r2 == orow[order]).all() and (c2 == ocol[order]).all() and (v2 == ov[order]).all(), "CSC"
+
+
+def assignments(n, rng):
+    yield n, np.arange(n, dtype=np.int32)
+    for B in (n // 2, 37, 8, 1):
+        B = max(1, min(B, n))
+        a = rng.integers(0, B, n).astype(np.int32)
+        yield B, a
+
+
+# ----------------------------------------------------------------------------- logf
+def test_device_logf_matches_host_logf_on_every_positive_float():
+    c = sbpx().Context(0)
+    L = O.lib()
+    chunk = 1 << 26
+    lo, hi = 1, 0x7F800001  # all positive subnormals, normals and +inf
+    bad = 0
+    start = lo
+    while start < hi:
+        n = min(chunk, hi - start)
+        dev = c.selftest_logf(start, n)
+        bad += L.oracle_logf_check(start, start + n, dev)
+        start += n
+    assert bad == 0
+
+
+# ----------------------------------------------------------------------------- build + MDL
+def test_build_and_mdl_toy(toy):
+    c = ctx_for(toy_arrays(toy))
+    g = O.Graph.from_edges(toy["num_vertices"], toy["edges"])
+    for B, a in ((3, toy["assignment"]), (6, toy["complex"]["assignment"]), (11, list(range(11)))):
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        assert_same_state(c, bm)
+        assert c.mdl() == pytest.approx(bm.mdl(), rel=MDL_RTOL)
+    c.build(3, toy["assignment"])
+    _, dout, din, deg = c.state()
+    assert list(dout) == toy["block_degrees_out"] and list(din) == toy["block_degrees_in"]
+    assert list(deg) == toy["block_degrees"]
+    assert abs(c.mdl() - toy["entropy"]) < 1e-5
+
+
+def test_build_and_mdl_generated_graphs():
+    rng = np.random.default_rng(0)
+    for seed, (n, K) in enumerate(((1000, 8), (3000, 20))):
+        arrays, truth = gen_arrays(n, K, seed=seed)
+        c = ctx_for(arrays)
+        g = O.Graph(n, *arrays)
+        for B, a in list(assignments(n, rng)) + [(K, truth)]:
+            c.build(B, a)
+            bm = O.Blockmodel(g, B, a)
+            assert_same_state(c, bm)
+            assert c.mdl() == pytest.approx(bm.mdl(), rel=MDL_RTOL)
+
+
+def test_build_rejects_bad_block_ids(toy):
+    c = ctx_for(toy_arrays(toy))
+    m = sbpx()
+    with pytest.raises(m.SbpxError):
+        c.build(3, [0, 1, 2, 3, 0, 0, 0, 0, 0, 0, 0])
+    with pytest.raises(m.SbpxError):
+        c.build(0, [0] * 11)
+
+
+# ----------------------------------------------------------------------------- moves
+def _random_proposals(n, B, a, rng, count):
+    v = rng.permutation(n)[:count].astype(np.int32)
+    s = rng.integers(0, B, count).astype(np.int32)
+    s = np.where(s == a[v], (s + 1) % B, s).astype(np.int32)
+    u = rng.random(count)
+    return v, s, u
+
+
+def test_eval_moves_toy_every_move(toy):
+    c = ctx_for(toy_arrays(toy))
+    g = O.Graph.from_edges(toy["num_vertices"], toy["edges"])
+    for B, a in ((3, toy["assignment"]), (6, toy["complex"]["assignment"])):
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        vs, ss = zip(*[(v, s) for v in range(11) for s in range(B) if s != a[v]])
+        u = np.linspace(0.01, 0.99, len(vs))
+        dS, H, acc = c.eval_moves(vs, ss, u)
+        odS, oH, oacc = bm.eval_moves(vs, ss, u)
+        assert np.allclose(dS, odS, rtol=DS_RTOL, atol=1e-9)
+        assert np.allclose(H, oH, rtol=1e-12)
+        assert (acc == oacc).all()
+
+
+def test_eval_moves_generated():
+    rng = np.random.default_rng(1)
+    n = 2000
+    arrays, truth = gen_arrays(n, 12, seed=3)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    for B, a in list(assignments(n, rng))[:-1] + [(12, truth)]:
+        if B < 2:
+            continue
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        v, s, u = _random_proposals(n, B, a, rng, 400)
+        dS, H, acc = c.eval_moves(v, s, u)
+        odS, oH, oacc = bm.eval_moves(v, s, u)
+        assert np.allclose(dS, odS, rtol=DS_RTOL, atol=1e-9), np.max(np.abs(dS - odS))
+        assert np.allclose(H, oH, rtol=1e-12)
+        assert (acc == oacc).sum() >= len(acc) - 1  # a decision may flip only at u == p to 1e-15
+
+
+def test_proposals_match_oracle_sampler():
+    rng = np.random.default_rng(2)
+    n = 1500
+    arrays, truth = gen_arrays(n, 10, seed=4)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    for B, a in list(assignments(n, rng))[:-1] + [(10, truth)]:
+        if B < 2:
+            continue
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        v = np.arange(n, dtype=np.int32)
+        for tag in (0, 1):
+            s, u = c.propose(v, seed=99, call=7, tag=tag)
+            os_, ou = bm.propose(v, seed=99, call=7, tag=tag)
+            assert (s == os_).all()
+            assert (u == ou).all()
+
+
+def test_apply_moves_matches_sequential_application_with_degree_drift():
+    rng = np.random.default_rng(3)
+    n = 1200
+    arrays, truth = gen_arrays(n, 9, seed=5)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    for B, a in list(assignments(n, rng))[:-1] + [(9, truth)]:
+        if B < 2:
+            continue
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        v, s, _ = _random_proposals(n, B, a, rng, 300)
+        c.apply_moves(v, s)
+        order = np.argsort(v)
+        bm.apply_moves(v[order], s[order])
+        assert_same_state(c, bm)
+        # a second batch on top keeps the earlier drift where untouched
+        v2, s2, _ = _random_proposals(n, B, bm.state()[0], rng, 50)
+        c.apply_moves(v2, s2)
+        o2 = np.argsort(v2)
+        bm.apply_moves(v2[o2], s2[o2])
+        assert_same_state(c, bm)
+
+
+# ----------------------------------------------------------------------------- sweeps
+@pytest.mark.parametrize("batches", [1, 3, 10])
+def test_async_gibbs_sweeps_match_oracle(batches):
+    n = 1000
+    arrays, truth = gen_arrays(n, 8, seed=0)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    rng = np.random.default_rng(4)
+    for B, a in ((n, np.arange(n, dtype=np.int32)), (64, rng.integers(0, 64, n).astype(np.int32)), (8, truth)):
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        for call in range(3):
+            st = c.sweep(sbpx().ALGO_ASYNC_GIBBS, batches, 1234, call)
+            ost = bm.async_sweep(batches, 1234, call)
+            assert (st.attempts, st.evals, st.moves) == (ost.attempts, ost.evals, ost.moves)
+            assert_same_state(c, bm)
+            assert c.mdl() == pytest.approx(bm.mdl(), rel=MDL_RTOL)
+
+
+# ----------------------------------------------------------------------------- merges
+def test_merge_proposals_match_oracle():
+    rng = np.random.default_rng(5)
+    n = 1000
+    arrays, truth = gen_arrays(n, 8, seed=1)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    for B, a in ((n, np.arange(n, dtype=np.int32)), (200, rng.integers(0, 200, n).astype(np.int32)),
+                 (16, rng.integers(0, 16, n).astype(np.int32))):
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        dS, best = c.merge_proposals(77, 3)
+        odS, obest = bm.merge_proposals(77, 3)
+        assert (best == obest).all()
+        assert np.allclose(dS, odS, rtol=DS_RTOL, atol=1e-9)
+
+
+def test_merge_blocks_matches_oracle():
+    n = 1000
+    arrays, truth = gen_arrays(n, 8, seed=2)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    a = np.arange(n, dtype=np.int32)
+    c.build(n, a)
+    bm = O.Blockmodel(g, n, a)
+    B = n
+    for call in range(4):
+        k = B // 2
+        c.merge_blocks(k, 5, call)
+        bm.num_to_merge = k
+        bm.merge_blocks(True, 5, call)
+        B -= k
+        assert c.info()[0] == bm.B == B
+        assert_same_state(c, bm)
+
+
+# ----------------------------------------------------------------------------- driver
+def test_sbp_async_gibbs_end_to_end_matches_oracle():
+    m = sbpx()
+    n = 1000
+    arrays, truth = gen_arrays(n, 8, seed=0)
+    g = O.Graph(n, *arrays)
+    a, st = m.run_sbp(*arrays, algorithm="async_gibbs", batches=1, approximate=True, seed=11)
+    oa, ost = O.sbp(g, 1, 1, True, 11)
+    assert st.num_blocks == ost.B
+    assert st.mdl == pytest.approx(ost.mdl, rel=5e-3)
+    print("device", st.num_blocks, st.mdl, "oracle", ost.B, ost.mdl, "identical:", bool((a == oa).all()))
+
+
+# ----------------------------------------------------------------------------- sequential MH
+@pytest.mark.parametrize("algo", ["hybrid", "mh"])
+def test_sequential_sweeps_match_oracle(algo):
+    m = sbpx()
+    n = 1000
+    arrays, truth = gen_arrays(n, 8, seed=6)
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    rng = np.random.default_rng(7)
+    code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
+    for B, a in ((n, np.arange(n, dtype=np.int32)), (128, rng.integers(0, 128, n).astype(np.int32)), (8, truth)):
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        for call in range(3):
+            st = c.sweep(code, 2, 4321, call)
+            if algo == "hybrid":
+                o1 = bm.mh_pass(4321, call)
+                o2 = bm.async_sweep(2, 4321, call, hybrid_low=True)
+                ost = (o1.attempts + o2.attempts, o1.evals + o2.evals, o1.moves + o2.moves)
+            else:
+                o1 = bm.mh_pass(4321, call, order=np.arange(n, dtype=np.int32))
+                ost = (o1.attempts, o1.evals, o1.moves)
+                assert st.dS_accepted == pytest.approx(o1.dS_sum, rel=1e-9, abs=1e-9)
+            assert (st.attempts, st.evals, st.moves) == ost
+            assert_same_state(c, bm)
+
+
+def test_sbp_hybrid_end_to_end_close_to_oracle():
+    m = sbpx()
+    n = 1000
+    arrays, truth = gen_arrays(n, 8, seed=0)
+    g = O.Graph(n, *arrays)
+    a, st = m.run_sbp(*arrays, algorithm="hybrid_mcmc", batches=1, approximate=True, seed=5)
+    oa, ost = O.sbp(g, 2, 1, True, 5)
+    assert st.num_blocks == ost.B
+    assert st.mdl == pytest.approx(ost.mdl, rel=5e-3)
+    print("device", st.num_blocks, st.mdl, "oracle", ost.B, ost.mdl, "identical:", bool((a == oa).all()))

@@ -106,8 +106,8 @@ void free_graph_buffers(Ctx& c) {
     dfree(&c.table_pool);
     c.table_pool_slots = 0;
     MhState& h = c.mh;
-    for (int** p : {&h.rbase, &h.cbase, &h.rkeys, &h.rvals, &h.ckeys, &h.cvals, &h.rocc, &h.cocc, &h.rneed,
-                    &h.cneed, &h.tables, &h.status, &h.d_all, &h.d_high})
+    for (int** p : {&h.rbase, &h.cbase, &h.rkeys, &h.rvals, &h.ckeys, &h.cvals, &h.rocc, &h.cocc, &h.rcap,
+                    &h.ccap, &h.tables, &h.status, &h.d_all, &h.d_high})
         dfree(p);
     h = MhState();
     c.plan = SweepPlan();
@@ -425,9 +425,9 @@ sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t N, int64_t E, const int32_t
         dalloc(&h.cbase, NB + 1);
         dalloc(&h.rocc, NB);
         dalloc(&h.cocc, NB);
-        dalloc(&h.rneed, NB);
-        dalloc(&h.cneed, NB);
-        dalloc(&h.status, 2);
+        dalloc(&h.rcap, NB);
+        dalloc(&h.ccap, NB);
+        dalloc(&h.status, 4);
         dalloc(&h.d_all, NB);
         dalloc(&h.d_high, NB);
         {

@@ -46,6 +46,9 @@ struct CsrView {
     __device__ int cval(int j) const { return col_val[j]; }
     __device__ int nnz_row(int x) const { return row_ptr[x + 1] - row_ptr[x]; }
     __device__ int nnz_col(int x) const { return col_ptr[x + 1] - col_ptr[x]; }
+    // M[x, c] and M[c, x] (one lane)
+    __device__ int rlookup(int x, int c) const { return lookup_sorted(row_col, row_val, row_ptr[x], row_ptr[x + 1], c); }
+    __device__ int clookup(int x, int c) const { return lookup_sorted(col_row, col_val, col_ptr[x], col_ptr[x + 1], c); }
     // Block at unit `target` of row t (target < dout[t]) or of column t (target - dout[t]).
     __device__ int sample(int t, int target, int dout_t) const {
         if (target < dout_t) return row_col[wave_upper_bound(row_cum, row_ptr[t], row_ptr[t + 1], target)];
@@ -91,28 +94,44 @@ __device__ __forceinline__ int wave_weighted_select(const int* keys, const int* 
 }
 
 // No __restrict__ here: the Metropolis-Hastings kernel writes these arrays between evaluations.
+// Row x occupies slots [base[x], base[x] + cap[x]) of the key/value pools (cap a power of two).
 struct HashView {
     const int* rbase;
+    const int* rcap;
     const int* rkeys;
     const int* rvals;
     const int* cbase;
+    const int* ccap;
     const int* ckeys;
     const int* cvals;
     int B;
     static constexpr bool kSlots = true;  // skip key < 0 and value 0
     __device__ int rbeg(int x) const { return rbase[x]; }
-    __device__ int rend(int x) const { return rbase[x + 1]; }
+    __device__ int rend(int x) const { return rbase[x] + rcap[x]; }
     __device__ int rkey(int j) const { return rkeys[j]; }
     __device__ int rval(int j) const { return rvals[j]; }
     __device__ int cbeg(int x) const { return cbase[x]; }
-    __device__ int cend(int x) const { return cbase[x + 1]; }
+    __device__ int cend(int x) const { return cbase[x] + ccap[x]; }
     __device__ int ckey(int j) const { return ckeys[j]; }
     __device__ int cval(int j) const { return cvals[j]; }
-    __device__ int nnz_row(int x) const { return rbase[x + 1] - rbase[x]; }
-    __device__ int nnz_col(int x) const { return cbase[x + 1] - cbase[x]; }
+    __device__ int nnz_row(int x) const { return rcap[x]; }
+    __device__ int nnz_col(int x) const { return ccap[x]; }
+    static __device__ int probe(const int* base, const int* capa, const int* keys, const int* vals, int x, int c) {
+        const int b0 = base[x], cap = capa[x];
+        int slot = (int)(hash_block(c) & (uint32_t)(cap - 1));
+        for (int p = 0; p < cap; ++p) {
+            int k = keys[b0 + slot];
+            if (k == c) return vals[b0 + slot];
+            if (k == -1) return 0;
+            slot = (slot + 1) & (cap - 1);
+        }
+        return 0;
+    }
+    __device__ int rlookup(int x, int c) const { return probe(rbase, rcap, rkeys, rvals, x, c); }
+    __device__ int clookup(int x, int c) const { return probe(cbase, ccap, ckeys, cvals, x, c); }
     __device__ int sample(int t, int target, int dout_t) const {
-        if (target < dout_t) return wave_weighted_select(rkeys, rvals, rbase[t], rbase[t + 1], target, B);
-        return wave_weighted_select(ckeys, cvals, cbase[t], cbase[t + 1], target - dout_t, B);
+        if (target < dout_t) return wave_weighted_select(rkeys, rvals, rbase[t], rbase[t] + rcap[t], target, B);
+        return wave_weighted_select(ckeys, cvals, cbase[t], cbase[t] + ccap[t], target - dout_t, B);
     }
 };
 

@@ -88,13 +88,15 @@ struct SweepPlan {
 
 // Mutable block matrix of a Metropolis-Hastings pass: per-row open-addressing tables.
 struct MhState {
-    int *rbase = nullptr, *cbase = nullptr;  // [N + 2] slot offsets
+    int *rbase = nullptr, *cbase = nullptr;  // [N + 1] first slot of each row / column
+    int *rcap = nullptr, *ccap = nullptr;    // [N] slots of each row / column (powers of two)
     int *rkeys = nullptr, *rvals = nullptr, *ckeys = nullptr, *cvals = nullptr;
-    size_t slots = 0;
-    int *rocc = nullptr, *cocc = nullptr, *rneed = nullptr, *cneed = nullptr;  // [N + 1]
+    size_t slots = 0;      // pool size of each of the four arrays
+    size_t headroom = 0;   // slots beyond the initial layout, for rows that outgrow their segment
+    int *rocc = nullptr, *cocc = nullptr;    // [N] keys held by each row / column
     int* tables = nullptr;  // neighbour-block tables, one per wavefront
     size_t table_slots = 0;
-    int* status = nullptr;  // [2]
+    int* status = nullptr;  // [4]: next queue position, stop reason, row pool top, column pool top
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
     int max_deg_high = 0;

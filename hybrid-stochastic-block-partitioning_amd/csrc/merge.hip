@@ -8,6 +8,7 @@
 //   and the untouched cells of row s and column s, whose degrees change.
 #include <cfloat>
 
+#include "eval_core.hpp"
 #include "internal.hpp"
 
 namespace sbpx {
@@ -32,42 +33,49 @@ struct MergeParams {
 
 __device__ __forceinline__ float fm(int x) { return (float)x; }
 
-__device__ double merge_delta_mdl_wave(const MergeParams& P, int r, int s, int k_out, int k_in) {
+// Delta MDL of merging r into s, read through a view (CSR snapshot or mutable hash tables); the
+// whole wavefront takes part. k_out / k_in: the row / column sums of r.
+template <class View>
+__device__ double merge_delta_mdl_wave(const View& M, const int* dout, const int* din, int r, int s, int k_out,
+                                       int k_in) {
     const int lane = lane_id();
-    const int r0 = P.row_ptr[r], r1 = P.row_ptr[r + 1];
-    const int c0 = P.col_ptr[r], c1 = P.col_ptr[r + 1];
-    const int s0 = P.row_ptr[s], s1 = P.row_ptr[s + 1];
-    const int cs0 = P.col_ptr[s], cs1 = P.col_ptr[s + 1];
-    const int dout_r = P.dout[r], din_r = P.din[r], dout_s = P.dout[s], din_s = P.din[s];
+    const int r0 = M.rbeg(r), r1 = M.rend(r);
+    const int c0 = M.cbeg(r), c1 = M.cend(r);
+    const int s0 = M.rbeg(s), s1 = M.rend(s);
+    const int cs0 = M.cbeg(s), cs1 = M.cend(s);
+    const int dout_r = dout[r], din_r = din[r], dout_s = dout[s], din_s = din[s];
     const int ndout_s = dout_s + k_out, ndin_s = din_s + k_in;
     double acc = 0.0;
     int m_rr = 0, m_rs = 0, m_sr = 0;
     // row r: old (r, c); keys (s, c) for c not in {r, s}
     for (int j = r0 + lane; j < r1; j += WAVE) {
-        int c = P.row_col[j], m = P.row_val[j];
-        int dc = P.din[c];
+        int c = M.rkey(j), m = M.rval(j);
+        if (View::kSlots && (c < 0 || m == 0)) continue;
+        int dc = din[c];
         acc += (double)cell_entropy(fm(m), fm(dc), fm(dout_r));
         if (c == r) {
             m_rr = m;
         } else if (c == s) {
             m_rs = m;
         } else {
-            int msc = lookup_sorted(P.row_col, P.row_val, s0, s1, c);
+            int msc = M.rlookup(s, c);
             acc += (double)cell_entropy(fm(msc), fm(dc), fm(dout_s));
             acc -= (double)cell_entropy(fm(msc + m), fm(dc), fm(ndout_s));
         }
     }
     // column r without (r, r): old (i, r); keys (i, s) for i not in {r, s}
     for (int j = c0 + lane; j < c1; j += WAVE) {
-        int i = P.col_row[j];
+        int i = M.ckey(j);
+        if (View::kSlots && i < 0) continue;
         if (i == r) continue;
-        int m = P.col_val[j];
-        int di = P.dout[i];
+        int m = M.cval(j);
+        if (View::kSlots && m == 0) continue;
+        int di = dout[i];
         acc += (double)cell_entropy(fm(m), fm(din_r), fm(di));
         if (i == s) {
             m_sr = m;
         } else {
-            int mis = lookup_sorted(P.col_row, P.col_val, cs0, cs1, i);
+            int mis = M.clookup(s, i);
             acc += (double)cell_entropy(fm(mis), fm(din_s), fm(di));
             acc -= (double)cell_entropy(fm(mis + m), fm(ndin_s), fm(di));
         }
@@ -79,14 +87,15 @@ __device__ double merge_delta_mdl_wave(const MergeParams& P, int r, int s, int k
     // row s: untouched cells with the new degrees
     int m_ss = 0;
     for (int j = s0 + lane; j < s1; j += WAVE) {
-        int c = P.row_col[j], m = P.row_val[j];
+        int c = M.rkey(j), m = M.rval(j);
+        if (View::kSlots && (c < 0 || m == 0)) continue;
         if (c == s) m_ss = m;
         bool touched;
         if (c == s) touched = d_ss > 0;
         else if (c == r) touched = true;
-        else touched = lookup_sorted(P.row_col, P.row_val, r0, r1, c) != 0;
+        else touched = M.rlookup(r, c) != 0;
         if (!touched) {
-            int dc = P.din[c];
+            int dc = din[c];
             int ndc = (c == s) ? ndin_s : dc;
             acc += (double)cell_entropy(fm(m), fm(dc), fm(dout_s));
             acc -= (double)cell_entropy(fm(m), fm(ndc), fm(ndout_s));
@@ -94,11 +103,13 @@ __device__ double merge_delta_mdl_wave(const MergeParams& P, int r, int s, int k
     }
     // column s, rows not in {r, s}: untouched cells
     for (int j = cs0 + lane; j < cs1; j += WAVE) {
-        int i = P.col_row[j];
+        int i = M.ckey(j);
+        if (View::kSlots && i < 0) continue;
         if (i == r || i == s) continue;
-        if (lookup_sorted(P.col_row, P.col_val, c0, c1, i) != 0) continue;
-        int m = P.col_val[j];
-        int di = P.dout[i];
+        int m = M.cval(j);
+        if (View::kSlots && m == 0) continue;
+        if (M.clookup(r, i) != 0) continue;
+        int di = dout[i];
         acc += (double)cell_entropy(fm(m), fm(din_s), fm(di));
         acc -= (double)cell_entropy(fm(m), fm(ndin_s), fm(di));
     }
@@ -167,7 +178,8 @@ __global__ void __launch_bounds__(MERGE_WAVES * WAVE) merge_proposals_kernel(Mer
         bool repeat = false;
         for (int q = 0; q < i; ++q) repeat |= past[q] == s;
         past[i] = s;
-        double dS = repeat ? DBL_MAX : merge_delta_mdl_wave(P, r, s, k_out, k_in);
+        const CsrView M{P.row_ptr, P.row_col, P.row_val, P.row_cum, P.col_ptr, P.col_row, P.col_val, P.col_cum};
+        double dS = repeat ? DBL_MAX : merge_delta_mdl_wave(M, P.dout, P.din, r, s, k_out, k_in);
         if (dS < best_dS) {
             best_dS = dS;
             best = s;

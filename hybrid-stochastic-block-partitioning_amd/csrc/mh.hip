@@ -8,10 +8,13 @@
 // current state; the first accepted move in queue order is committed and the round restarts
 // after it. Every vertex before it was rejected and so left the state unchanged, which makes the
 // outcome identical to evaluating one vertex at a time (proposals and uniforms are keyed by
-// vertex, not by evaluation order). A commit first checks that every row and column it may
-// extend has room; if not, the kernel stops before touching anything, the host rebuilds the
-// tables with more room for those rows, and the pass resumes at the same vertex.
+// vertex, not by evaluation order). Before a commit, every row and column it may extend is given
+// room: a row that is too full moves to a larger segment taken from the free end of the pool
+// (the other wavefronts wait at the barrier meanwhile). Only when the pool itself runs out does
+// the kernel stop; the host then rebuilds the tables with a larger pool and the pass resumes at
+// the same vertex.
 #include <algorithm>
+#include <cstdint>
 
 #include "eval_core.hpp"
 #include "internal.hpp"
@@ -21,15 +24,18 @@ namespace sbpx {
 constexpr int MH_WAVES = 16;
 
 // ------------------------------------------------------------------------------------------
-// Table build: cap(x) = next power of two >= 2 (nnz(x) + need(x)) + 32, filled from the CSR.
-__global__ void hash_caps_kernel(const int* __restrict__ ptr, const int* __restrict__ need, int B,
-                                 int* __restrict__ caps) {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= B) return;
-    int want = 2 * (ptr[x + 1] - ptr[x] + need[x]) + 32;
+// Table build: cap(x) = next power of two >= 2 nnz(x) + 32, filled from the CSR.
+__device__ __forceinline__ int table_cap(int keys) {
+    int want = 2 * keys + 32;
     int cap = 32;
     while (cap < want) cap <<= 1;
-    caps[x] = cap;
+    return cap;
+}
+
+__global__ void hash_caps_kernel(const int* __restrict__ ptr, int B, int* __restrict__ caps) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= B) return;
+    caps[x] = table_cap(ptr[x + 1] - ptr[x]);
 }
 
 // One wavefront per row: clear the row's slots, then insert its CSR entries.
@@ -68,15 +74,16 @@ struct MhParams {
     int* drift;
     int B;
     int* rbase;
+    int* rcap;
     int* rkeys;
     int* rvals;
     int* rocc;
-    int* rneed;
     int* cbase;
+    int* ccap;
     int* ckeys;
     int* cvals;
     int* cocc;
-    int* cneed;
+    int pool;  // slots in each key/value array
     const int* __restrict__ list;
     int n;
     int start;
@@ -85,20 +92,18 @@ struct MhParams {
     uint32_t tag;
     int* tables;
     int tcap;
-    int* status;  // [0] next queue position, [1] 1 if stopped for room
+    int* status;  // [0] next queue position, [1] 1 if the pool ran out, [2] / [3] row / column pool top
     long long* counters;  // attempts, evals, moves
     double* dS_sum;
     int* err;
 };
 
-__device__ __forceinline__ int room(const int* base, const int* occ, int x) {
-    int cap = base[x + 1] - base[x];
-    return (cap * 3) / 4 - occ[x];
-}
+__device__ __forceinline__ int room(const int* capa, const int* occ, int x) { return (capa[x] * 3) / 4 - occ[x]; }
 
 // Adds d to cell (x, key) of one orientation, claiming a slot when the key is new.
-__device__ __forceinline__ void upsert(int* base, int* keys, int* vals, int* occ, int x, int key, int d) {
-    const int b0 = base[x], cap = base[x + 1] - b0;
+__device__ __forceinline__ void upsert(const int* base, const int* capa, int* keys, int* vals, int* occ, int x, int key,
+                                       int d) {
+    const int b0 = base[x], cap = capa[x];
     int slot = (int)(hash_block(key) & (uint32_t)(cap - 1));
     for (int probe = 0; probe < cap; ++probe) {
         int k = keys[b0 + slot];
@@ -123,31 +128,89 @@ __device__ __forceinline__ void upsert(int* base, int* keys, int* vals, int* occ
 }
 
 __device__ __forceinline__ void cell_add(const MhParams& P, int row, int col, int d) {
-    upsert(P.rbase, P.rkeys, P.rvals, P.rocc, row, col, d);
-    upsert(P.cbase, P.ckeys, P.cvals, P.cocc, col, row, d);
+    upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, row, col, d);
+    upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, col, row, d);
 }
 
-// Room check for a commit: row s and column s may gain one cell per neighbour block plus the
-// four cells among {r, s}; rows/columns r one or two; every other neighbour block one cell in
-// its row and one in its column. Records the room a failing row/column needs.
-__device__ bool commit_fits(const MhParams& P, const MoveResult& R, const int* tab, int cap) {
+// Moves row x of one orientation to a new segment from the pool's free end, sized for its live
+// keys plus `extra`, dropping keys whose count fell to zero (lookups read 0 for them either
+// way). The whole wavefront takes part. Returns false, leaving the row where it was, when the
+// pool has no segment of that size left.
+__device__ bool relocate(int* base, int* capa, int* keys, int* vals, int* occ, int* top, int pool, int x, int extra) {
+    const int lane = lane_id();
+    const int b0 = base[x], ocap = capa[x];
+    const int ncap = table_cap(occ[x] + extra);
+    int nb = 0;
+    if (lane == 0) nb = atomicAdd(top, ncap);
+    nb = __shfl(nb, 0, WAVE);
+    if (nb > pool - ncap) return false;
+    for (int j = lane; j < ncap; j += WAVE) {
+        keys[nb + j] = -1;
+        vals[nb + j] = 0;
+    }
+    fence_wave();
+    int live = 0;
+    for (int j = lane; j < ocap; j += WAVE) {
+        const int k = keys[b0 + j], m = vals[b0 + j];
+        if (k < 0 || m == 0) continue;
+        int slot = (int)(hash_block(k) & (uint32_t)(ncap - 1));
+        for (;;) {
+            int old = atomicCAS(&keys[nb + slot], -1, k);
+            if (old == -1) break;
+            slot = (slot + 1) & (ncap - 1);
+        }
+        vals[nb + slot] = m;
+        ++live;
+    }
+    live = (int)wave_sum_ll(live);
+    fence_wave();
+    if (lane == 0) {
+        base[x] = nb;
+        capa[x] = ncap;
+        occ[x] = live;
+    }
+    fence_wave();
+    return true;
+}
+
+// Gives every row and column a commit may extend the room it needs: row s and column s one cell
+// per neighbour block plus the four cells among {r, s}; rows/columns r four; every other
+// neighbour block one cell in its row and one in its column (two kept, for slack). Returns false
+// when the pool ran out.
+__device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab, int cap) {
     const int lane = lane_id();
     const int r = R.r, s = R.s;
     const int big = R.nuniq + 4;
-    int bad = 0;
-    if (lane == 0) {
-        if (room(P.rbase, P.rocc, s) < big) { atomicMax(&P.rneed[s], big); bad = 1; }
-        if (room(P.cbase, P.cocc, s) < big) { atomicMax(&P.cneed[s], big); bad = 1; }
-        if (room(P.rbase, P.rocc, r) < 4) { atomicMax(&P.rneed[r], 4); bad = 1; }
-        if (room(P.cbase, P.cocc, r) < 4) { atomicMax(&P.cneed[r], 4); bad = 1; }
+    int* rtop = P.status + 2;
+    int* ctop = P.status + 3;
+    if (room(P.rcap, P.rocc, s) < big && !relocate(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, rtop, P.pool, s, big))
+        return false;
+    if (room(P.ccap, P.cocc, s) < big && !relocate(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, ctop, P.pool, s, big))
+        return false;
+    if (room(P.rcap, P.rocc, r) < 4 && !relocate(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, rtop, P.pool, r, 4))
+        return false;
+    if (room(P.ccap, P.cocc, r) < 4 && !relocate(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, ctop, P.pool, r, 4))
+        return false;
+    for (int j0 = 0; j0 < cap; j0 += WAVE) {
+        const int j = j0 + lane;
+        const int b = j < cap ? tab[j] : -1;
+        const bool mine = b >= 0 && b != r && b != s;
+        unsigned long long mr = __ballot(mine && room(P.rcap, P.rocc, b) < 2);
+        unsigned long long mc = __ballot(mine && room(P.ccap, P.cocc, b) < 2);
+        while (mr) {
+            const int l = first_lane(mr);
+            mr &= mr - 1;
+            const int x = __shfl(b, l, WAVE);
+            if (!relocate(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, rtop, P.pool, x, 2)) return false;
+        }
+        while (mc) {
+            const int l = first_lane(mc);
+            mc &= mc - 1;
+            const int x = __shfl(b, l, WAVE);
+            if (!relocate(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, ctop, P.pool, x, 2)) return false;
+        }
     }
-    for (int j = lane; j < cap; j += WAVE) {
-        int b = tab[j];
-        if (b < 0 || b == r || b == s) continue;
-        if (room(P.rbase, P.rocc, b) < 2) { atomicMax(&P.rneed[b], 2); bad = 1; }
-        if (room(P.cbase, P.cocc, b) < 2) { atomicMax(&P.cneed[b], 2); bad = 1; }
-    }
-    return wave_or(bad) == 0;
+    return true;
 }
 
 // Applies the accepted move: block matrix cells from (o_b, i_b, self), block degrees with the
@@ -198,7 +261,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     int* tab = P.tables + (size_t)wave * 7 * P.tcap;
     const GraphView G = P.G;
     const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
-    const HashView M{P.rbase, P.rkeys, P.rvals, P.cbase, P.ckeys, P.cvals, P.B};
+    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B};
     long long att = 0, evs = 0, mvs = 0;
     double dsum = 0.0;
     int pos = P.start;
@@ -229,7 +292,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
             break;
         }
         if (wave == j) {
-            bool ok = commit_fits(P, R, tab, P.tcap);
+            bool ok = make_room(P, R, tab, P.tcap);
             if (ok) commit_move(P, R, tab, P.tcap);
             if (lane == 0) s_flag = ok ? 1 : 2;
         }
@@ -273,15 +336,19 @@ static void build_tables(Ctx& c) {
     MhState& h = c.mh;
     const int tb = 256;
     // capacities and bases, rows then columns
-    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.row_ptr, h.rneed, B, c.tmp_b);
-    scan_exclusive(c.tmp_b, h.rbase, B, c.scan_tmp, st);
-    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.col_ptr, h.cneed, B, c.tmp_b);
-    scan_exclusive(c.tmp_b, h.cbase, B, c.scan_tmp, st);
+    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.row_ptr, B, h.rcap);
+    scan_exclusive(h.rcap, h.rbase, B, c.scan_tmp, st);
+    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.col_ptr, B, h.ccap);
+    scan_exclusive(h.ccap, h.cbase, B, c.scan_tmp, st);
     int tot[2] = {0, 0};
     SBPX_HIP(hipMemcpyAsync(&tot[0], h.rbase + B, sizeof(int), hipMemcpyDeviceToHost, st));
     SBPX_HIP(hipMemcpyAsync(&tot[1], h.cbase + B, sizeof(int), hipMemcpyDeviceToHost, st));
     SBPX_HIP(hipStreamSynchronize(st));
-    const size_t need = (size_t)std::max(tot[0], tot[1]);
+    const size_t used = (size_t)std::max(tot[0], tot[1]);
+    h.headroom = std::max(h.headroom, std::max(used, (size_t)1 << 20));
+    // pool tops stay below 2^31 even after a failed claim
+    const size_t need = std::min(used + h.headroom, (size_t)(INT32_MAX / 2));
+    if (need < used + 64) throw StateError("block matrix too large for the Metropolis-Hastings tables");
     if (need > h.slots) {
         for (int** p : {&h.rkeys, &h.rvals, &h.ckeys, &h.cvals}) {
             if (*p) SBPX_HIP(hipFree(*p));
@@ -295,6 +362,9 @@ static void build_tables(Ctx& c) {
     hash_fill_kernel<<<(B + wpb - 1) / wpb, tb, 0, st>>>(c.bm.col_ptr, c.bm.col_row, c.bm.col_val, B, h.cbase,
                                                           h.ckeys, h.cvals, h.cocc);
     SBPX_HIP(hipGetLastError());
+    // free ends of the pools
+    SBPX_HIP(hipMemcpyAsync(h.status + 2, h.rbase + B, sizeof(int), hipMemcpyDeviceToDevice, st));
+    SBPX_HIP(hipMemcpyAsync(h.status + 3, h.cbase + B, sizeof(int), hipMemcpyDeviceToDevice, st));
 }
 
 void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag) {
@@ -309,12 +379,11 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
         h.table_slots = tslots;
     }
-    SBPX_HIP(hipMemsetAsync(h.rneed, 0, sizeof(int) * B, c.stream));
-    SBPX_HIP(hipMemsetAsync(h.cneed, 0, sizeof(int) * B, c.stream));
     int pos = 0;
     int rounds = 0;
     while (pos < n) {
-        if (rounds++ > 64) throw StateError("Metropolis-Hastings pass keeps running out of table room");
+        // each restart doubles the pool headroom, so the number of restarts is logarithmic
+        if (rounds++ > 24) throw StateError("Metropolis-Hastings pass keeps running out of table room");
         build_tables(c);
         MhParams P{};
         P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx};
@@ -325,15 +394,16 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.drift = c.bm.drift;
         P.B = B;
         P.rbase = h.rbase;
+        P.rcap = h.rcap;
         P.rkeys = h.rkeys;
         P.rvals = h.rvals;
         P.rocc = h.rocc;
-        P.rneed = h.rneed;
         P.cbase = h.cbase;
+        P.ccap = h.ccap;
         P.ckeys = h.ckeys;
         P.cvals = h.cvals;
         P.cocc = h.cocc;
-        P.cneed = h.cneed;
+        P.pool = (int)h.slots;
         P.list = d_list;
         P.n = n;
         P.start = pos;
@@ -356,6 +426,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // the matrix changed: bring the CSR snapshot up to date (dout/din exact, deg keeps drift)
         rebuild_keep_drift(c);
         if (stat[1] == 0 && pos < n) throw StateError("Metropolis-Hastings pass stopped early");
+        if (stat[1] == 1) h.headroom *= 2;
     }
 }
 

@@ -5,6 +5,8 @@
 #include "sbp_host.hpp"
 
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
@@ -338,6 +340,9 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
         finetune::MCMC_time +=
             std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
         add_intermediate(++iteration, blockmodel, blockmodel.getOverall_entropy());
+        if (std::getenv("SBP_TRACE"))
+            std::fprintf(stderr, "sbp iter %.0f: B=%d mdl=%.3f mcmc_iterations=%d\n", iteration,
+                         blockmodel.getNum_blocks(), (double)blockmodel.getOverall_entropy(), finetune::MCMC_iterations);
         blockmodel = blockmodel_triplet.get_next_blockmodel(blockmodel);
     }
     add_intermediate(-1, blockmodel, blockmodel.getOverall_entropy());

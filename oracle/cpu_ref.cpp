@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <limits>
 #include <map>
@@ -855,6 +857,9 @@ RunResult stochastic_block_partition(const Graph& g, int algo, int batches, bool
         res.evals += f.stats.evals;
         res.moves += f.stats.moves;
         res.outer_iterations++;
+        if (getenv("ORACLE_TRACE"))
+            fprintf(stderr, "oracle iter %d: B=%d mdl=%.3f sweeps=%d moves=%ld\n", res.outer_iterations, bm.B,
+                    (double)bm.overall_entropy, f.sweeps, f.stats.moves);
         bm = triplet.next(bm);
     }
     res.bm = bm;

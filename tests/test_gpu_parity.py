@@ -301,3 +301,67 @@ def test_sbp_hybrid_end_to_end_close_to_oracle():
     assert st.num_blocks == ost.B
     assert st.mdl == pytest.approx(ost.mdl, rel=5e-3)
     print("device", st.num_blocks, st.mdl, "oracle", ost.B, ost.mdl, "identical:", bool((a == oa).all()))
+
+
+# ----------------------------------------------------------------------------- all degree bins
+def heavy_arrays(seed=0):
+    """Graph whose vertices fall in every degree bin: a few hubs with in-degree > 2047 and a
+    few hundred with degree in (127, 2047]."""
+    rng = np.random.default_rng(seed)
+    n = 3000
+    edges = set()
+    for v in range(n):
+        for w in rng.integers(0, n, 6):
+            if w != v:
+                edges.add((v, int(w)))
+    for hub in (0, 1):
+        for v in rng.permutation(n)[:2300]:
+            if v != hub:
+                edges.add((int(v), hub))
+    for mid in range(2, 300):
+        for v in rng.permutation(n)[: int(rng.integers(130, 400))]:
+            if v != mid:
+                edges.add((mid, int(v)) if v % 2 else (int(v), mid))
+    for v in (5, 17):
+        edges.add((v, v))  # self-loops on binned vertices
+    return O.csr_from_edges(n, sorted(edges))
+
+
+def test_every_degree_bin_matches_oracle():
+    m = sbpx()
+    arrays = heavy_arrays()
+    n = len(arrays[0]) - 1
+    deg = np.diff(arrays[0]) + np.diff(arrays[2])
+    assert (deg > 2047).sum() >= 2 and ((deg > 127) & (deg <= 2047)).sum() >= 100
+    c = ctx_for(arrays)
+    g = O.Graph(n, *arrays)
+    rng = np.random.default_rng(8)
+    for B, a in ((n, np.arange(n, dtype=np.int32)), (400, rng.integers(0, 400, n).astype(np.int32)),
+                 (20, rng.integers(0, 20, n).astype(np.int32))):
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        v = np.arange(n, dtype=np.int32)
+        s, u = c.propose(v, seed=3, call=1)
+        os_, ou = bm.propose(v, seed=3, call=1)
+        assert (s == os_).all() and (u == ou).all()
+        vv, ss, uu = _random_proposals(n, B, a, rng, 600)
+        hubs = np.array([0, 1, 2, 3, 5, 17], np.int32)
+        vv = np.concatenate([hubs, vv[~np.isin(vv, hubs)]])[:600].astype(np.int32)
+        ss = rng.integers(0, B, len(vv)).astype(np.int32)
+        ss = np.where(ss == a[vv], (ss + 1) % B, ss).astype(np.int32)
+        uu = rng.random(len(vv))
+        dS, H, acc = c.eval_moves(vv, ss, uu)
+        odS, oH, oacc = bm.eval_moves(vv, ss, uu)
+        assert np.allclose(dS, odS, rtol=DS_RTOL, atol=1e-9), np.max(np.abs(dS - odS))
+        assert np.allclose(H, oH, rtol=1e-12)
+        for call in range(2):
+            st = c.sweep(m.ALGO_ASYNC_GIBBS, 1, 9, call)
+            ost = bm.async_sweep(1, 9, call)
+            assert (st.attempts, st.evals, st.moves) == (ost.attempts, ost.evals, ost.moves)
+            assert_same_state(c, bm)
+        st = c.sweep(m.ALGO_HYBRID, 1, 9, 5)
+        o1 = bm.mh_pass(9, 5)
+        o2 = bm.async_sweep(1, 9, 5, hybrid_low=True)
+        assert (st.attempts, st.evals, st.moves) == (o1.attempts + o2.attempts, o1.evals + o2.evals,
+                                                    o1.moves + o2.moves)
+        assert_same_state(c, bm)

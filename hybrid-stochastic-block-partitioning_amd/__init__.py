@@ -29,8 +29,7 @@ CONFIGS = {
 
 class SweepStats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("evals", C.c_int64), ("moves", C.c_int64), ("dS_accepted", C.c_double),
-                ("eval_ms", C.c_double * 3), ("eval_bytes", C.c_double * 3), ("eval_launches", C.c_int32 * 3),
-                ("reserved", C.c_int32)]
+                ("eval_ms", C.c_double * 4), ("eval_bytes", C.c_double * 4), ("eval_launches", C.c_int32 * 4)]
 
 
 class DcsbmParams(C.Structure):
@@ -42,8 +41,8 @@ class DcsbmParams(C.Structure):
 class RunStats(C.Structure):
     _fields_ = [("num_blocks", C.c_int32), ("outer_iterations", C.c_int32), ("sweeps", C.c_int64),
                 ("attempts", C.c_int64), ("evals", C.c_int64), ("moves", C.c_int64), ("seconds", C.c_double),
-                ("mcmc_seconds", C.c_double), ("eval_ms", C.c_double * 3), ("eval_bytes", C.c_double * 3),
-                ("eval_launches", C.c_int64 * 3), ("mdl", C.c_double)]
+                ("mcmc_seconds", C.c_double), ("eval_ms", C.c_double * 4), ("eval_bytes", C.c_double * 4),
+                ("eval_launches", C.c_int64 * 4), ("mdl", C.c_double)]
 
 
 _lib = None

@@ -13,7 +13,6 @@ using namespace sbpx;
 struct sbpx_ctx {
     Ctx c;
     bool count_bytes = false;
-    std::vector<hipEvent_t> events;
 };
 
 namespace {
@@ -62,6 +61,7 @@ void free_graph_buffers(Ctx& c) {
     dfree(&c.g.in_ptr);
     dfree(&c.g.in_idx);
     dfree(&c.g.edge_src);
+    dfree(&c.g.self_flag);
     DevBlockmodel& b = c.bm;
     dfree(&b.assign);
     dfree(&b.row_ptr);
@@ -107,7 +107,7 @@ void free_graph_buffers(Ctx& c) {
     c.table_pool_slots = 0;
     MhState& h = c.mh;
     for (int** p : {&h.rbase, &h.cbase, &h.rkeys, &h.rvals, &h.ckeys, &h.cvals, &h.rocc, &h.cocc, &h.rcap,
-                    &h.ccap, &h.tables, &h.status, &h.d_all, &h.d_high})
+                    &h.ccap, &h.tables, &h.status, &h.touch, &h.d_all, &h.d_high})
         dfree(p);
     h = MhState();
     c.plan = SweepPlan();
@@ -174,18 +174,7 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
     Ctx& c = ctx->c;
     for (int b = 0; b < NUM_BINS; ++b) {
         if (bl.count[b] == 0) continue;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timed) {
-            size_t k = c.timed.size() * 2;
-            while (ctx->events.size() < k + 2) {
-                hipEvent_t e;
-                SBPX_HIP(hipEventCreate(&e));
-                ctx->events.push_back(e);
-            }
-            e0 = ctx->events[k];
-            e1 = ctx->events[k + 1];
-            SBPX_HIP(hipEventRecord(e0, c.stream));
-        }
+        if (timed) c.timer_begin();
         EvalLaunch L{};
         L.list = d_list + bl.offset[b];
         L.n = bl.count[b];
@@ -197,10 +186,7 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
         L.huge_offsets = d_huge_off;
         L.bytes = bytes ? bytes + b : nullptr;
         launch_eval(c, L);
-        if (timed) {
-            SBPX_HIP(hipEventRecord(e1, c.stream));
-            c.timed.push_back(b);
-        }
+        if (timed) c.timer_end(b);
     }
 }
 
@@ -295,7 +281,7 @@ void sbpx_destroy(sbpx_ctx* ctx) {
     dfree(&ctx->c.dev_err);
     dfree(&ctx->c.counters);
     dfree(&ctx->c.red);
-    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->c.events) (void)hipEventDestroy(e);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }
@@ -368,6 +354,8 @@ sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t N, int64_t E, const int32_t
         dalloc(&g.out_idx, E);
         dalloc(&g.in_idx, E);
         dalloc(&g.edge_src, E);
+        dalloc(&g.self_flag, N);
+        SBPX_HIP(hipMemcpy(g.self_flag, g.self.data(), N, hipMemcpyHostToDevice));
         SBPX_HIP(hipMemcpy(g.out_ptr, out_ptr, sizeof(int) * (N + 1), hipMemcpyHostToDevice));
         SBPX_HIP(hipMemcpy(g.in_ptr, in_ptr, sizeof(int) * (N + 1), hipMemcpyHostToDevice));
         if (E > 0) {
@@ -427,7 +415,10 @@ sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t N, int64_t E, const int32_t
         dalloc(&h.cocc, NB);
         dalloc(&h.rcap, NB);
         dalloc(&h.ccap, NB);
-        dalloc(&h.status, 4);
+        dalloc(&h.status, 8);
+        dalloc(&h.touch, N);
+        SBPX_HIP(hipMemset(h.status, 0, sizeof(int) * 8));
+        SBPX_HIP(hipMemset(h.touch, 0, sizeof(int) * std::max(N, 1)));
         dalloc(&h.d_all, NB);
         dalloc(&h.d_high, NB);
         {
@@ -695,10 +686,12 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
         SBPX_HIP(hipMemsetAsync(c.red + 2048, 0, sizeof(double), c.stream));
         c.timed.clear();
         long long* bytes = ctx->count_bytes ? c.counters + 4 : nullptr;
+        long long* mh_bytes = ctx->count_bytes ? c.counters + 4 + KERNEL_MH : nullptr;
         if (algo == SBPX_ALGO_METROPOLIS_HASTINGS)
-            mh_sequential(c, c.mh.d_all, c.g.N, c.g.max_deg, seed, call, TAG_MH);
+            mh_sequential(c, c.mh.d_all, c.g.N, c.g.max_deg, seed, call, TAG_MH, mh_bytes, true);
         else if (algo == SBPX_ALGO_HYBRID)
-            mh_sequential(c, c.mh.d_high, (int)c.g.high.size(), c.mh.max_deg_high, seed, call, TAG_MH);
+            mh_sequential(c, c.mh.d_high, (int)c.g.high.size(), c.mh.max_deg_high, seed, call, TAG_MH, mh_bytes,
+                          true);
         for (const BinnedList& bl : c.plan.batches_async) {
             const int* d_list = c.plan.d_lists + bl.offset[0];
             eval_vertex_list(ctx, c.plan.d_lists, bl, c.plan.d_huge_offsets + bl.huge_base, seed, call, TAG_ASYNC,
@@ -715,7 +708,7 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
             *stats = sbpx_sweep_stats{};
             for (size_t i = 0; i < c.timed.size(); ++i) {
                 float t = 0.0f;
-                SBPX_HIP(hipEventElapsedTime(&t, ctx->events[2 * i], ctx->events[2 * i + 1]));
+                SBPX_HIP(hipEventElapsedTime(&t, c.events[2 * i], c.events[2 * i + 1]));
                 stats->eval_ms[c.timed[i]] += t;
                 stats->eval_launches[c.timed[i]] += 1;
             }
@@ -723,7 +716,8 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
             stats->evals = cnt[1];
             stats->moves = cnt[2];
             stats->dS_accepted = dsum;
-            for (int b = 0; b < NUM_BINS; ++b) stats->eval_bytes[b] = ctx->count_bytes ? (double)cnt[4 + b] : 0.0;
+            for (int b = 0; b < NUM_KERNEL_CLASSES; ++b)
+                stats->eval_bytes[b] = ctx->count_bytes ? (double)cnt[4 + b] : 0.0;
         }
     });
 }

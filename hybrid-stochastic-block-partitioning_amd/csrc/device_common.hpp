@@ -105,6 +105,12 @@ __device__ __forceinline__ float cell_entropy(float value, float din, float dout
 // Wavefront primitives.
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
 
+// Orders this wavefront's earlier memory writes before its later reads (LDS and global).
+__device__ __forceinline__ void fence_wave() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);

@@ -93,6 +93,66 @@ __device__ __forceinline__ int wave_weighted_select(const int* keys, const int* 
     return a;
 }
 
+// The same selection by a radix descent over 8-bit digits of the block index, most significant
+// first: each pass sums, per digit, the weights of the keys that share the digits chosen so far,
+// in an LDS histogram of 256 counters owned by this wavefront, and picks the digit where the
+// running sum passes the target. ceil(bits(B - 1) / 8) passes over the slots instead of
+// log2(B) for the bisection.
+__device__ __forceinline__ int wave_weighted_select_radix(const int* keys, const int* vals, int lo, int hi,
+                                                          int target, int B, int* hist) {
+    const int lane = lane_id();
+    int nbits = 1;
+    while (nbits < 31 && (1 << nbits) < B) ++nbits;
+    int shift = ((nbits + 7) / 8 - 1) * 8;  // lowest bit of the top digit
+    int prefix = 0;                         // digits chosen so far
+    long long below = 0;                    // weight of the keys below the chosen prefix
+    for (;;) {
+        for (int d = lane; d < 256; d += WAVE) hist[d] = 0;
+        fence_wave();
+        const int up = shift + 8;
+        for (int j = lo + lane; j < hi; j += WAVE) {
+            const int k = keys[j];
+            if (k < 0) continue;
+            const int w = vals[j];
+            if (w == 0) continue;
+            const int hi_part = up >= 31 ? 0 : (k >> up);
+            if (hi_part == prefix) atomicAdd(&hist[(k >> shift) & 255], w);
+        }
+        fence_wave();
+        long long h[4];
+        long long local = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            h[q] = hist[lane * 4 + q];
+            local += h[q];
+        }
+        long long incl = local;
+#pragma unroll
+        for (int off = 1; off < WAVE; off <<= 1) {
+            long long y = __shfl_up(incl, off, WAVE);
+            if (lane >= off) incl += y;
+        }
+        long long run = below + (incl - local);
+        int digit = INT_MAX;
+        long long before = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (digit == INT_MAX && run + h[q] > target) {
+                digit = lane * 4 + q;
+                before = run;
+            }
+            run += h[q];
+        }
+        const int d = wave_min(digit);
+        if (d == INT_MAX) return B - 1;  // target beyond the total weight: not reached for valid input
+        below = __shfl(before, d / 4, WAVE);
+        prefix = (prefix << 8) | d;
+        if (shift == 0) return prefix;
+        shift -= 8;
+        fence_wave();
+    }
+}
+
 // No __restrict__ here: the Metropolis-Hastings kernel writes these arrays between evaluations.
 // Row x occupies slots [base[x], base[x] + cap[x]) of the key/value pools (cap a power of two).
 struct HashView {
@@ -105,6 +165,7 @@ struct HashView {
     const int* ckeys;
     const int* cvals;
     int B;
+    int* hist = nullptr;  // 256 LDS counters of this wavefront for the radix selection, or null
     static constexpr bool kSlots = true;  // skip key < 0 and value 0
     __device__ int rbeg(int x) const { return rbase[x]; }
     __device__ int rend(int x) const { return rbase[x] + rcap[x]; }
@@ -130,8 +191,14 @@ struct HashView {
     __device__ int rlookup(int x, int c) const { return probe(rbase, rcap, rkeys, rvals, x, c); }
     __device__ int clookup(int x, int c) const { return probe(cbase, ccap, ckeys, cvals, x, c); }
     __device__ int sample(int t, int target, int dout_t) const {
-        if (target < dout_t) return wave_weighted_select(rkeys, rvals, rbase[t], rbase[t] + rcap[t], target, B);
-        return wave_weighted_select(ckeys, cvals, cbase[t], cbase[t] + ccap[t], target - dout_t, B);
+        const bool row = target < dout_t;
+        const int* keys = row ? rkeys : ckeys;
+        const int* vals = row ? rvals : cvals;
+        const int lo = row ? rbase[t] : cbase[t];
+        const int hi = lo + (row ? rcap[t] : ccap[t]);
+        const int tg = row ? target : target - dout_t;
+        if (hist != nullptr && hi - lo > 4 * WAVE) return wave_weighted_select_radix(keys, vals, lo, hi, tg, B, hist);
+        return wave_weighted_select(keys, vals, lo, hi, tg, B);
     }
 };
 
@@ -140,6 +207,7 @@ struct GraphView {
     const int* __restrict__ out_idx;
     const int* __restrict__ in_ptr;
     const int* __restrict__ in_idx;
+    const uint8_t* __restrict__ self;  // 1 where the vertex has a self-loop
 };
 
 // No __restrict__: mutable during a Metropolis-Hastings pass.
@@ -162,11 +230,6 @@ struct MoveResult {
     int nuniq;
     long long nnz_rs;  // nonzeros read in rows r, s and columns r, s
 };
-
-__device__ __forceinline__ void fence_wave() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
 
 // Insert (or find) block b; returns its slot, or -1 when the table is full.
 __device__ __forceinline__ int tab_insert(int* key, int cap, int b) {
@@ -193,6 +256,12 @@ __device__ __forceinline__ float fi(int x) { return (float)x; }
 
 // Table layout: 7 arrays of cap ints: key, o, i, M[s,b], M[b,s], M[r,b], M[b,r].
 // given_s >= 0 fixes the proposal (with uniform given_u); propose_only stops after step 2.
+//
+// Latency is the bound (each step waits on a gather), so the steps are arranged to keep the
+// chain of dependent loads short: the proposal's random draws come first and the lane that
+// gathers the drawn neighbour slot records its block during step 1; the four scans of step 3
+// run as one loop over the concatenated ranges, two slots per lane per trip, with the keys,
+// then the degrees, loaded for both slots before either is used.
 template <class View>
 __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockState& S, const View& M, int v,
                                                 int* tab, int cap, uint64_t seed, uint32_t call, uint32_t tag,
@@ -220,45 +289,60 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
         tmrb[j] = 0;
         tmbr[j] = 0;
     }
+    const int ob = G.out_ptr[v], dO = G.out_ptr[v + 1] - ob;
+    const int ib = G.in_ptr[v], dI = G.in_ptr[v + 1] - ib;
+    const int self = G.self[v];
+    const int k_out = dO;
+    const int k_in = dI - self;
+    const int k = k_out + k_in;
+    R.k_out = k_out;
+    R.k_in = k_in;
+    R.self = self;
+
+    // proposal draws (common.cpp:131-182): neighbour slot ps among the out list (self included)
+    // and the in list (self excluded)
+    Draw4 dr{0u, 0u, 0u, 0u};
+    double u = given_u;
+    int pick0 = -1, pick1 = -1;  // combined-list positions whose block may be t
+    if (given_s < 0) {
+        dr = philox4x32_10(seed, (uint32_t)v, call, 0u, tag);
+        u = unit53(dr.x2, dr.x3);
+        if (k > 0) {
+            const int ps = (int)mulhi32(dr.x0, (uint32_t)k);
+            if (ps < k_out) {
+                pick0 = ps;
+            } else {
+                pick0 = dO + (ps - k_out);  // in-list position when the self-loop comes later
+                pick1 = pick0 + 1;          // ... and when it comes at or before it
+            }
+        }
+    }
     fence_wave();
 
-    // 1. neighbour blocks
-    const int ob = G.out_ptr[v], oe = G.out_ptr[v + 1];
-    const int ib = G.in_ptr[v], ie = G.in_ptr[v + 1];
-    int self = 0, self_pos = INT_MAX, full = 0;
-    for (int j = ob + lane; j < oe; j += WAVE) {
-        int w = G.out_idx[j];
+    // 1. neighbour blocks over the concatenated out and in lists
+    int self_pos = INT_MAX, full = 0, cand0 = -1, cand1 = -1;
+    for (int j = lane; j < dO + dI; j += WAVE) {
+        const bool is_out = j < dO;
+        const int w = is_out ? G.out_idx[ob + j] : G.in_idx[ib + (j - dO)];
+        const int bw = S.assign[w];
+        if (j == pick0) cand0 = bw;
+        if (j == pick1) cand1 = bw;
         if (w == v) {
-            self = 1;
+            if (!is_out) self_pos = j - dO;
         } else {
-            int slot = tab_insert(tkey, cap, S.assign[w]);
+            int slot = tab_insert(tkey, cap, bw);
             if (slot < 0) full = 1;
-            else atomicAdd(&to[slot], 1);
+            else atomicAdd(is_out ? &to[slot] : &ti[slot], 1);
         }
     }
-    for (int j = ib + lane; j < ie; j += WAVE) {
-        int w = G.in_idx[j];
-        if (w == v) {
-            self_pos = j - ib;
-        } else {
-            int slot = tab_insert(tkey, cap, S.assign[w]);
-            if (slot < 0) full = 1;
-            else atomicAdd(&ti[slot], 1);
-        }
-    }
-    self = wave_or(self);
     self_pos = wave_min(self_pos);
+    cand0 = wave_max(cand0);
+    cand1 = wave_max(cand1);
     if (self && lane == 0) {
         if (tab_insert(tkey, cap, r) < 0) full = 1;
     }
     full = wave_or(full);
     fence_wave();
-    const int k_out = oe - ob;
-    const int k_in = (ie - ib) - (self_pos != INT_MAX ? 1 : 0);
-    const int k = k_out + k_in;
-    R.k_out = k_out;
-    R.k_in = k_in;
-    R.self = self;
     if (full) {
         R.status = -1;
         return R;
@@ -266,32 +350,18 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
 
     // 2. proposal
     int s;
-    double u;
     if (given_s >= 0) {
         s = given_s;
-        u = given_u;
+    } else if (k == 0) {
+        s = random_block(r, B, dr.x0);
     } else {
-        Draw4 dr = philox4x32_10(seed, (uint32_t)v, call, 0u, tag);
-        u = unit53(dr.x2, dr.x3);
-        if (k == 0) {
-            s = random_block(r, B, dr.x0);
-        } else {
-            int slot = (int)mulhi32(dr.x0, (uint32_t)k);
-            int w;
-            if (slot < k_out) {
-                w = G.out_idx[ob + slot];
-            } else {
-                int jj = slot - k_out;
-                if (jj >= self_pos) jj++;
-                w = G.in_idx[ib + jj];
-            }
-            const int t = S.assign[w];
-            R.t = t;
-            const int trow = S.dout[t];
-            const int total = trow + S.din[t];
-            const int target = (int)mulhi32(dr.x1, (uint32_t)total);
-            s = M.sample(t, target, trow);
-        }
+        int t = cand0;
+        if (pick1 >= 0 && pick0 - dO >= self_pos) t = cand1;  // in list: skip over the self-loop
+        R.t = t;
+        const int trow = S.dout[t];
+        const int total = trow + S.din[t];
+        const int target = (int)mulhi32(dr.x1, (uint32_t)total);
+        s = M.sample(t, target, trow);
     }
     R.s = s;
     R.u = u;
@@ -302,7 +372,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
         return R;
     }
 
-    // 3. one pass over rows r, s and columns r, s
+    // 3. one pass over rows s, r and columns s, r
     const int slot_r = tab_find(tkey, cap, r), slot_s = tab_find(tkey, cap, s);
     const int o_r = slot_r >= 0 ? to[slot_r] : 0, i_r = slot_r >= 0 ? ti[slot_r] : 0;
     const int o_s = slot_s >= 0 ? to[slot_s] : 0, i_s = slot_s >= 0 ? ti[slot_s] : 0;
@@ -316,96 +386,100 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     int m_ss = 0, m_rr = 0, m_sr = 0, m_rs = 0;
     double acc = 0.0;
 
-    const int rs0 = M.rbeg(s), rs1 = M.rend(s);
-    for (int j = rs0 + lane; j < rs1; j += WAVE) {
-        int c = M.rkey(j), m = M.rval(j);
-        if (View::kSlots && (c < 0 || m == 0)) continue;
-        bool unchanged;
-        if (c == s) {
-            m_ss = m;
-            unchanged = !t_ss;
-        } else if (c == r) {
-            m_sr = m;
-            unchanged = !t_sr || d_sr == 0;
-        } else {
-            int sl = tab_find(tkey, cap, c);
-            if (sl >= 0) {
-                tmsb[sl] = m;
-                unchanged = to[sl] == 0;
-            } else {
-                unchanged = true;
+    const int b0 = M.rbeg(s), L0 = M.rend(s) - b0;  // row s
+    const int b1 = M.rbeg(r), L1 = M.rend(r) - b1;  // row r
+    const int b2 = M.cbeg(s), L2 = M.cend(s) - b2;  // column s
+    const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;  // column r
+    const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
+    long long live = 0;
+    constexpr int U = 2;
+    for (int base = 0; base < T; base += U * WAVE) {
+        int key[U], val[U], seg[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int i = base + q * WAVE + lane;
+            seg[q] = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : i < T ? 3 : -1;
+            key[q] = -1;
+            val[q] = 0;
+            if (seg[q] == 0) {
+                key[q] = M.rkey(b0 + i);
+                val[q] = M.rval(b0 + i);
+            } else if (seg[q] == 1) {
+                key[q] = M.rkey(b1 + (i - L0));
+                val[q] = M.rval(b1 + (i - L0));
+            } else if (seg[q] == 2) {
+                key[q] = M.ckey(b2 + (i - T1));
+                val[q] = M.cval(b2 + (i - T1));
+            } else if (seg[q] == 3) {
+                key[q] = M.ckey(b3 + (i - T2));
+                val[q] = M.cval(b3 + (i - T2));
             }
         }
-        if (unchanged) {
-            int dc = S.din[c];
-            int ndc = c == r ? ndin_r : (c == s ? ndin_s : dc);
-            acc += (double)cell_entropy(fi(m), fi(dc), fi(dout_s));
-            acc -= (double)cell_entropy(fi(m), fi(ndc), fi(ndout_s));
-        }
-    }
-    const int rr0 = M.rbeg(r), rr1 = M.rend(r);
-    for (int j = rr0 + lane; j < rr1; j += WAVE) {
-        int c = M.rkey(j), m = M.rval(j);
-        if (View::kSlots && (c < 0 || m == 0)) continue;
-        bool unchanged;
-        if (c == r) {
-            m_rr = m;
-            unchanged = !t_rr;
-        } else if (c == s) {
-            m_rs = m;
-            unchanged = !t_rs || d_rs == 0;
-        } else {
-            int sl = tab_find(tkey, cap, c);
-            if (sl >= 0) {
-                tmrb[sl] = m;
-                unchanged = to[sl] == 0;
-            } else {
-                unchanged = true;
+        int dsel[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            dsel[q] = -1;
+            const int c = key[q], m = val[q];
+            if (seg[q] < 0 || (View::kSlots && (c < 0 || m == 0))) continue;
+            ++live;
+            bool unchanged = true;
+            if (seg[q] == 0) {  // row s: cell (s, c)
+                if (c == s) {
+                    m_ss = m;
+                    unchanged = !t_ss;
+                } else if (c == r) {
+                    m_sr = m;
+                    unchanged = !t_sr || d_sr == 0;
+                } else {
+                    const int sl = tab_find(tkey, cap, c);
+                    if (sl >= 0) {
+                        tmsb[sl] = m;
+                        unchanged = to[sl] == 0;
+                    }
+                }
+            } else if (seg[q] == 1) {  // row r: cell (r, c)
+                if (c == r) {
+                    m_rr = m;
+                    unchanged = !t_rr;
+                } else if (c == s) {
+                    m_rs = m;
+                    unchanged = !t_rs || d_rs == 0;
+                } else {
+                    const int sl = tab_find(tkey, cap, c);
+                    if (sl >= 0) {
+                        tmrb[sl] = m;
+                        unchanged = to[sl] == 0;
+                    }
+                }
+            } else {  // column s or r: cell (c, s) or (c, r), rows r and s left to the rows
+                if (c == r || c == s) continue;
+                const int sl = tab_find(tkey, cap, c);
+                if (sl >= 0) {
+                    (seg[q] == 2 ? tmbs : tmbr)[sl] = m;
+                    unchanged = ti[sl] == 0;
+                }
             }
+            if (unchanged) dsel[q] = c;
         }
-        if (unchanged) {
-            int dc = S.din[c];
-            int ndc = c == r ? ndin_r : (c == s ? ndin_s : dc);
-            acc += (double)cell_entropy(fi(m), fi(dc), fi(dout_r));
-            acc -= (double)cell_entropy(fi(m), fi(ndc), fi(ndout_r));
-        }
-    }
-    const int cs0 = M.cbeg(s), cs1 = M.cend(s);
-    for (int j = cs0 + lane; j < cs1; j += WAVE) {
-        int i = M.ckey(j);
-        if (View::kSlots && i < 0) continue;
-        if (i == r || i == s) continue;
-        int m = M.cval(j);
-        if (View::kSlots && m == 0) continue;
-        int sl = tab_find(tkey, cap, i);
-        bool unchanged = true;
-        if (sl >= 0) {
-            tmbs[sl] = m;
-            unchanged = ti[sl] == 0;
-        }
-        if (unchanged) {
-            int di = S.dout[i];
-            acc += (double)cell_entropy(fi(m), fi(din_s), fi(di));
-            acc -= (double)cell_entropy(fi(m), fi(ndin_s), fi(di));
-        }
-    }
-    const int cr0 = M.cbeg(r), cr1 = M.cend(r);
-    for (int j = cr0 + lane; j < cr1; j += WAVE) {
-        int i = M.ckey(j);
-        if (View::kSlots && i < 0) continue;
-        if (i == r || i == s) continue;
-        int m = M.cval(j);
-        if (View::kSlots && m == 0) continue;
-        int sl = tab_find(tkey, cap, i);
-        bool unchanged = true;
-        if (sl >= 0) {
-            tmbr[sl] = m;
-            unchanged = ti[sl] == 0;
-        }
-        if (unchanged) {
-            int di = S.dout[i];
-            acc += (double)cell_entropy(fi(m), fi(din_r), fi(di));
-            acc -= (double)cell_entropy(fi(m), fi(ndin_r), fi(di));
+        int dv[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) dv[q] = dsel[q] >= 0 ? (seg[q] < 2 ? S.din[dsel[q]] : S.dout[dsel[q]]) : 0;
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            if (dsel[q] < 0) continue;
+            const int c = dsel[q];
+            const float m = fi(val[q]);
+            const int d = dv[q];
+            if (seg[q] < 2) {  // unchanged cell of row x in {s, r}: din[c] may change, dout[x] does
+                const int xo = seg[q] == 0 ? dout_s : dout_r, nxo = seg[q] == 0 ? ndout_s : ndout_r;
+                const int nd = c == r ? ndin_r : (c == s ? ndin_s : d);
+                acc += (double)cell_entropy(m, fi(d), fi(xo));
+                acc -= (double)cell_entropy(m, fi(nd), fi(nxo));
+            } else {  // unchanged cell of column x: din[x] changes
+                const int xi = seg[q] == 2 ? din_s : din_r, nxi = seg[q] == 2 ? ndin_s : ndin_r;
+                acc += (double)cell_entropy(m, fi(xi), fi(d));
+                acc -= (double)cell_entropy(m, fi(nxi), fi(d));
+            }
         }
     }
     m_ss = wave_max(m_ss);
@@ -417,44 +491,62 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     // 4. Delta keys and Hastings sums
     double pf = 0.0, pb = 0.0;
     int nuniq = 0;
-    for (int j = lane; j < cap; j += WAVE) {
-        int b = tkey[j];
-        if (b < 0) continue;
-        nuniq++;
-        int ob_ = to[j], ib_ = ti[j];
-        int Msb, Mbs, Mrb, Mbr, Drb, Dbr, nd;
-        if (b == r) {
-            Msb = m_sr; Mbs = m_rs; Mrb = m_rr; Mbr = m_rr; Drb = d_rr; Dbr = d_rr;
-            nd = (dout_r - k_out) + (din_r - k_in) - (m_rr + d_rr);
-        } else if (b == s) {
-            Msb = m_ss; Mbs = m_ss; Mrb = m_rs; Mbr = m_sr; Drb = d_rs; Dbr = d_sr;
-            nd = (dout_s + k_out) + (din_s + k_in) - (m_ss + d_ss);
-        } else {
-            Msb = tmsb[j]; Mbs = tmbs[j]; Mrb = tmrb[j]; Mbr = tmbr[j]; Drb = -ob_; Dbr = -ib_;
-            nd = S.deg[b];
-            const int db = S.din[b], ub = S.dout[b];
-            if (ob_ > 0) {
-                acc += (double)cell_entropy(fi(Msb), fi(db), fi(dout_s));
-                acc -= (double)cell_entropy(fi(Msb + ob_), fi(db), fi(ndout_s));
-                acc += (double)cell_entropy(fi(Mrb), fi(db), fi(dout_r));
-                acc -= (double)cell_entropy(fi(Mrb - ob_), fi(db), fi(ndout_r));
-            }
-            if (ib_ > 0) {
-                acc += (double)cell_entropy(fi(Mbs), fi(din_s), fi(ub));
-                acc -= (double)cell_entropy(fi(Mbs + ib_), fi(ndin_s), fi(ub));
-                acc += (double)cell_entropy(fi(Mbr), fi(din_r), fi(ub));
-                acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
-            }
+    for (int j0 = 0; j0 < cap; j0 += U * WAVE) {
+        int bk[U], degb[U], dinb[U], doutb[U];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int j = j0 + q * WAVE + lane;
+            bk[q] = j < cap ? tkey[j] : -1;
         }
-        int nb = ob_ + ib_ + (b == r ? self : 0);
-        if (nb == 0) continue;
-        double count = (double)nb;
-        double pw = (double)(Msb + Mbs) + 1.0;
-        double bd = (double)(S.deg[b] + B);
-        double bw = (double)(Mrb + Drb + Mbr + Dbr) + 1.0;
-        double pd = (double)(nd + B);
-        pf += count * pw / bd;
-        pb += count * bw / pd;
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int b = bk[q];
+            const bool other = b >= 0 && b != r && b != s;
+            degb[q] = b >= 0 ? S.deg[b] : 0;
+            dinb[q] = other ? S.din[b] : 0;
+            doutb[q] = other ? S.dout[b] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const int b = bk[q];
+            if (b < 0) continue;
+            const int j = j0 + q * WAVE + lane;
+            nuniq++;
+            const int ob_ = to[j], ib_ = ti[j];
+            int Msb, Mbs, Mrb, Mbr, Drb, Dbr, nd;
+            if (b == r) {
+                Msb = m_sr; Mbs = m_rs; Mrb = m_rr; Mbr = m_rr; Drb = d_rr; Dbr = d_rr;
+                nd = (dout_r - k_out) + (din_r - k_in) - (m_rr + d_rr);
+            } else if (b == s) {
+                Msb = m_ss; Mbs = m_ss; Mrb = m_rs; Mbr = m_sr; Drb = d_rs; Dbr = d_sr;
+                nd = (dout_s + k_out) + (din_s + k_in) - (m_ss + d_ss);
+            } else {
+                Msb = tmsb[j]; Mbs = tmbs[j]; Mrb = tmrb[j]; Mbr = tmbr[j]; Drb = -ob_; Dbr = -ib_;
+                nd = degb[q];
+                const int db = dinb[q], ub = doutb[q];
+                if (ob_ > 0) {
+                    acc += (double)cell_entropy(fi(Msb), fi(db), fi(dout_s));
+                    acc -= (double)cell_entropy(fi(Msb + ob_), fi(db), fi(ndout_s));
+                    acc += (double)cell_entropy(fi(Mrb), fi(db), fi(dout_r));
+                    acc -= (double)cell_entropy(fi(Mrb - ob_), fi(db), fi(ndout_r));
+                }
+                if (ib_ > 0) {
+                    acc += (double)cell_entropy(fi(Mbs), fi(din_s), fi(ub));
+                    acc -= (double)cell_entropy(fi(Mbs + ib_), fi(ndin_s), fi(ub));
+                    acc += (double)cell_entropy(fi(Mbr), fi(din_r), fi(ub));
+                    acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
+                }
+            }
+            const int nb = ob_ + ib_ + (b == r ? self : 0);
+            if (nb == 0) continue;
+            const double count = (double)nb;
+            const double pw = (double)(Msb + Mbs) + 1.0;
+            const double bd = (double)(degb[q] + B);
+            const double bw = (double)(Mrb + Drb + Mbr + Dbr) + 1.0;
+            const double pd = (double)(nd + B);
+            pf += count * pw / bd;
+            pb += count * bw / pd;
+        }
     }
     if (lane == 0) {
         if (t_ss) {
@@ -492,7 +584,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     R.d_rr = d_rr;
     R.d_sr = d_sr;
     R.d_rs = d_rs;
-    R.nnz_rs = (long long)(rr1 - rr0) + (rs1 - rs0) + (cr1 - cr0) + (cs1 - cs0);
+    R.nnz_rs = View::kSlots ? wave_sum_ll(live) : (long long)T;
     return R;
 }
 

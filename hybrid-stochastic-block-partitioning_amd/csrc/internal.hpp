@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,8 @@ enum : int {
 constexpr int BIN_SMALL_MAXDEG = 127;   // 256-slot LDS table, 4 wavefronts per workgroup
 constexpr int BIN_LARGE_MAXDEG = 2047;  // 4096-slot LDS table, 1 wavefront per workgroup
 constexpr int NUM_BINS = 3;             // small, large, huge (table in global memory)
+constexpr int KERNEL_MH = 3;            // timing / byte class of the sequential Metropolis-Hastings kernel
+constexpr int NUM_KERNEL_CLASSES = 4;
 
 struct DevGraph {
     int N = 0;
@@ -50,6 +53,7 @@ struct DevGraph {
     int max_deg = 0;
     int *out_ptr = nullptr, *out_idx = nullptr, *in_ptr = nullptr, *in_idx = nullptr;
     int* edge_src = nullptr;  // source vertex of each CSR edge
+    uint8_t* self_flag = nullptr;  // [N] 1 where the vertex has a self-loop
     std::vector<int> h_out_ptr, h_out_idx, h_in_ptr, h_in_idx;
     std::vector<int> degree;             // |out| + |in| - self (graph.cpp:15-22)
     std::vector<char> self;
@@ -96,7 +100,9 @@ struct MhState {
     int *rocc = nullptr, *cocc = nullptr;    // [N] keys held by each row / column
     int* tables = nullptr;  // neighbour-block tables, one per wavefront
     size_t table_slots = 0;
-    int* status = nullptr;  // [4]: next queue position, stop reason, row pool top, column pool top
+    int* status = nullptr;  // [8]: next queue position, stop reason, row pool top, column pool top,
+                            // commit serial
+    int* touch = nullptr;   // [N] serial of the last commit that moved a neighbour of each vertex
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
     int max_deg_high = 0;
@@ -151,7 +157,22 @@ struct Ctx {
     std::vector<int> h_best;
 
     // timing of the last sweep's eval kernels (ms) and algorithmic bytes
-    std::vector<int> timed;  // degree bin of each timed evaluation launch in this sweep
+    std::vector<int> timed;  // kernel class of each timed launch in this sweep (KERNEL_* below)
+    std::vector<hipEvent_t> events;  // two per timed launch, reused across sweeps
+    // Brackets one launch on the stream with a pair of events, recorded under kernel class k.
+    void timer_begin() {
+        const size_t need = timed.size() * 2 + 2;
+        while (events.size() < need) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+            events.push_back(e);
+        }
+        (void)hipEventRecord(events[timed.size() * 2], stream);
+    }
+    void timer_end(int k) {
+        (void)hipEventRecord(events[timed.size() * 2 + 1], stream);
+        timed.push_back(k);
+    }
 
     void sync() { SBPX_HIP(hipStreamSynchronize(stream)); }
 };
@@ -191,7 +212,8 @@ struct EvalLaunch {
 void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
 void finish_degrees(Ctx& c);
-void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag);
+void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,
+                   long long* bytes, bool timed);
 
 // merge.hip
 void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call);

@@ -92,8 +92,11 @@ struct MhParams {
     uint32_t tag;
     int* tables;
     int tcap;
-    int* status;  // [0] next queue position, [1] 1 if the pool ran out, [2] / [3] row / column pool top
+    int* status;  // [0] next queue position, [1] 1 if the pool ran out, [2] / [3] row / column pool
+                  // top, [4] commits so far (the touch serial)
     long long* counters;  // attempts, evals, moves
+    long long* bytes;     // algorithmic bytes of the evaluations that stand, or null
+    int* touch;           // [N] serial of the last commit that moved a neighbour of the vertex
     double* dS_sum;
     int* err;
 };
@@ -215,9 +218,16 @@ __device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab
 
 // Applies the accepted move: block matrix cells from (o_b, i_b, self), block degrees with the
 // update rule of Blockmodel::move_vertex (blockmodel.cpp:344-360), the assignment.
-__device__ void commit_move(const MhParams& P, const MoveResult& R, const int* tab, int cap) {
+__device__ void commit_move(const MhParams& P, const MoveResult& R, const int* tab, int cap, int serial) {
     const int lane = lane_id();
     const int r = R.r, s = R.s;
+    {  // stamp the neighbours, whose neighbour-block tables this move changes
+        const int v = R.v;
+        const int ob = P.G.out_ptr[v], oe = P.G.out_ptr[v + 1];
+        const int ib = P.G.in_ptr[v], ie = P.G.in_ptr[v + 1];
+        for (int j = ob + lane; j < oe; j += WAVE) P.touch[P.G.out_idx[j]] = serial;
+        for (int j = ib + lane; j < ie; j += WAVE) P.touch[P.G.in_idx[j]] = serial;
+    }
     const int* to = tab + cap;
     const int* ti = tab + 2 * cap;
     for (int j = lane; j < cap; j += WAVE) {
@@ -253,75 +263,167 @@ __device__ void commit_move(const MhParams& P, const MoveResult& R, const int* t
     }
 }
 
+constexpr int MH_LDS_CAP = 256;  // neighbour-block table slots per wavefront held in LDS
+
+// Table slots for vertex v: a power of two at least twice its neighbour count plus two.
+__device__ __forceinline__ int vertex_table_cap(const GraphView& G, int v) {
+    const int d = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
+    int cap = 16;
+    while (cap < 2 * (d + 2)) cap <<= 1;
+    return cap;
+}
+
+// Algorithmic bytes of an evaluation that stands: the formula of the batched kernels, with rows
+// and columns of the hash tables counted by the keys they hold.
+__device__ __forceinline__ long long eval_bytes(const MhParams& P, const MoveResult& R) {
+    long long b = 16 + 4 + 8LL * (R.k_out + R.k_in);
+    if (R.t >= 0) b += 8LL * (P.rocc[R.t] + P.cocc[R.t]);
+    if (R.status >= 1) b += 8 * R.nnz_rs + 12LL * (R.nuniq + 2) + 8LL * R.nuniq + 16;
+    return b;
+}
+
+// Whether wavefront w's evaluation (result R, own table tab/cap) still holds after the commit
+// that moved a vertex from block cr to block cs with neighbour-block table ctab/ccap and stamped
+// its neighbours with `serial`. The commit changed the assignment of one vertex, the cells of
+// rows and columns cr and cs (which include every cell (b, cr), (b, cs), (cr, b), (cs, b) for
+// its neighbour blocks b) and the degrees of cr and cs. The evaluation read the blocks of its
+// neighbours, rows and columns r, s and t, and the degrees of t, of its neighbour blocks and of
+// the blocks keyed in rows and columns r and s. It holds when none of these was touched; rows
+// r, s and t were then neither changed nor moved, so a new evaluation would read the same
+// values in the same order.
+__device__ bool still_holds(const MhParams& P, const HashView& M, const MoveResult& R, const int* tab, int cap,
+                            const int* ctab, int ccap, int cr, int cs, int serial) {
+    const int lane = lane_id();
+    int bad = 0;
+    if (lane == 0) {
+        bad = P.touch[R.v] == serial;  // a neighbour moved
+    } else if (lane <= 3) {            // r, s or t is one of the blocks whose cells changed
+        const int y = lane == 1 ? R.r : (lane == 2 ? R.s : R.t);
+        if (y >= 0) bad = y == cr || y == cs || tab_find(ctab, ccap, y) >= 0;
+    } else if (lane <= 11) {  // cr or cs keyed in rows / columns r and s
+        const int q = lane - 4;
+        const int x = (q & 1) ? cs : cr;
+        const int row = (q & 2) ? R.s : R.r;
+        bad = ((q & 4) ? M.clookup(row, x) : M.rlookup(row, x)) != 0;
+    } else if (lane <= 13) {  // cr or cs a neighbour block
+        const int x = lane == 12 ? cr : cs;
+        bad = tab_find(tab, cap, x) >= 0;
+    }
+    return wave_or(bad) == 0;
+}
+
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
-    __shared__ int st[MH_WAVES];
+    __shared__ int st[MH_WAVES];    // status of each wavefront's evaluation, 3 = idle
+    __shared__ int holds[MH_WAVES]; // 1 while that evaluation still holds
     __shared__ double sdS[MH_WAVES];
-    __shared__ int s_flag;
+    __shared__ int s_flag, s_cr, s_cs, s_ccap;
+    __shared__ int* s_ctab;
+    __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
+    __shared__ int lds_hist[MH_WAVES][256];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
-    int* tab = P.tables + (size_t)wave * 7 * P.tcap;
+    int* const gtab = P.tables + (size_t)wave * 7 * P.tcap;
     const GraphView G = P.G;
     const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
-    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B};
-    long long att = 0, evs = 0, mvs = 0;
+    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[wave]};
+    long long att = 0, evs = 0, mvs = 0, bytes_local = 0;
     double dsum = 0.0;
     int pos = P.start;
     int stopped = 0;
-    while (pos < P.n) {
+    int serial = P.status[4];
+    while (pos < P.n && stopped == 0) {
+        // a window of up to 16 queue entries, each evaluated against the current state
         const int idx = pos + wave;
         MoveResult R{};
         const bool active = idx < P.n;
-        if (active) R = eval_move<HashView>(G, S, M, P.list[idx], tab, P.tcap, P.seed, P.call, P.tag, -1, 0.0, false);
+        int cap = 0;
+        int* tab = gtab;
+        if (active) {
+            const int v = P.list[idx];
+            cap = vertex_table_cap(G, v);
+            if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
+            R = eval_move<HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+        }
         if (lane == 0) {
             st[wave] = active ? R.status : 3;
+            holds[wave] = 1;
             sdS[wave] = active ? R.dS : 0.0;
         }
-        if (threadIdx.x == 0) s_flag = 0;
         __syncthreads();
         const int W = min(MH_WAVES, P.n - pos);
-        int j = -1;
-        for (int w = 0; w < W; ++w) {
-            if (st[w] == 2 || st[w] < 0) {
-                j = w;
+        // walk the window in queue order: rejected entries are final, an accepted one is committed
+        // and the later entries are checked against the commit
+        int cur = 0;
+        for (;;) {
+            int j = -1;
+            for (int w = cur; w < W; ++w) {
+                if (!holds[w] || st[w] == 2 || st[w] < 0) {
+                    j = w;
+                    break;
+                }
+            }
+            const int end = j >= 0 ? j : W;
+            if (P.bytes != nullptr && lane == 0 && wave >= cur && wave < end && st[wave] != 3)
+                bytes_local += eval_bytes(P, R);
+            if (threadIdx.x == 0) {
+                for (int w = cur; w < end; ++w) {
+                    att += st[w] != 3;
+                    evs += st[w] == 1;
+                }
+            }
+            if (j < 0) {
+                pos += W;
                 break;
             }
-        }
-        if (j >= 0 && st[j] < 0) {
-            if (threadIdx.x == 0) atomicOr(P.err, DEV_ERR_TABLE);
-            stopped = 2;
-            pos += j;
-            break;
-        }
-        if (wave == j) {
-            bool ok = make_room(P, R, tab, P.tcap);
-            if (ok) commit_move(P, R, tab, P.tcap);
-            if (lane == 0) s_flag = ok ? 1 : 2;
-        }
-        __syncthreads();
-        const int flag = s_flag;
-        const int done = (j >= 0) ? j : W;  // vertices before the accepted one were rejected or idle
-        if (threadIdx.x == 0) {
-            for (int w = 0; w < done; ++w) {
-                att += st[w] != 3;
-                evs += st[w] == 1;
+            if (!holds[j]) {  // evaluate again from here
+                pos += j;
+                break;
             }
-            if (j >= 0 && flag == 1) {
+            if (st[j] < 0) {
+                if (threadIdx.x == 0) atomicOr(P.err, DEV_ERR_TABLE);
+                stopped = 2;
+                pos += j;
+                break;
+            }
+            if (wave == j) {
+                const bool ok = make_room(P, R, tab, cap);
+                if (ok) commit_move(P, R, tab, cap, serial + 1);
+                if (lane == 0) {
+                    s_flag = ok ? 1 : 2;
+                    s_cr = R.r;
+                    s_cs = R.s;
+                    s_ctab = tab;
+                    s_ccap = cap;
+                }
+            }
+            __syncthreads();
+            if (s_flag == 2) {  // the pool is full: the host rebuilds it and resumes here
+                stopped = 1;
+                pos += j;
+                break;
+            }
+            ++serial;
+            if (wave == j && lane == 0 && P.bytes != nullptr) bytes_local += eval_bytes(P, R);
+            if (threadIdx.x == 0) {
                 att++;
                 evs++;
                 mvs++;
                 dsum += sdS[j];
             }
+            if (wave > j && wave < W && holds[wave] && st[wave] != 3) {
+                const bool h = still_holds(P, M, R, tab, cap, s_ctab, s_ccap, s_cr, s_cs, serial);
+                if (lane == 0 && !h) holds[wave] = 0;
+            }
+            __syncthreads();
+            cur = j + 1;
         }
-        if (flag == 2) {
-            stopped = 1;
-            pos += j;
-            break;
-        }
-        pos += (j >= 0) ? j + 1 : W;
         __syncthreads();
     }
+    if (P.bytes != nullptr && lane == 0 && bytes_local > 0)
+        atomicAdd((unsigned long long*)P.bytes, (unsigned long long)bytes_local);
     if (threadIdx.x == 0) {
         P.status[0] = pos;
         P.status[1] = stopped;
+        P.status[4] = serial;
         atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
         atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
         atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)mvs);
@@ -367,12 +469,14 @@ static void build_tables(Ctx& c) {
     SBPX_HIP(hipMemcpyAsync(h.status + 3, h.cbase + B, sizeof(int), hipMemcpyDeviceToDevice, st));
 }
 
-void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag) {
+void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,
+                   long long* bytes, bool timed) {
     if (n <= 0) return;
     MhState& h = c.mh;
     const int B = c.bm.B;
+    // max_deg counts a self-loop once; vertex_table_cap counts it in both lists
     int tcap = 16;
-    while (tcap < 2 * (max_deg + 2)) tcap <<= 1;
+    while (tcap < 2 * (max_deg + 3)) tcap <<= 1;
     const size_t tslots = (size_t)MH_WAVES * 7 * tcap;
     if (tslots > h.table_slots) {
         if (h.tables) SBPX_HIP(hipFree(h.tables));
@@ -386,7 +490,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         if (rounds++ > 24) throw StateError("Metropolis-Hastings pass keeps running out of table room");
         build_tables(c);
         MhParams P{};
-        P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx};
+        P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
         P.assign = c.bm.assign;
         P.dout = c.bm.dout;
         P.din = c.bm.din;
@@ -414,15 +518,23 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.tcap = tcap;
         P.status = h.status;
         P.counters = c.counters;
+        P.bytes = bytes;
+        P.touch = h.touch;
         P.dS_sum = c.red + 2048;
         P.err = c.dev_err;
+        if (timed) c.timer_begin();
         mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
         SBPX_HIP(hipGetLastError());
-        int stat[2];
+        if (timed) c.timer_end(KERNEL_MH);
+        int stat[5];
         SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
         SBPX_HIP(hipStreamSynchronize(c.stream));
         check_device_error(c);
         pos = stat[0];
+        if (stat[4] > (1 << 30)) {  // restart the commit serials well before they wrap
+            SBPX_HIP(hipMemsetAsync(h.touch, 0, sizeof(int) * c.g.N, c.stream));
+            SBPX_HIP(hipMemsetAsync(h.status + 4, 0, sizeof(int), c.stream));
+        }
         // the matrix changed: bring the CSR snapshot up to date (dout/din exact, deg keeps drift)
         rebuild_keep_drift(c);
         if (stat[1] == 0 && pos < n) throw StateError("Metropolis-Hastings pass stopped early");

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(WAVE) eval_huge_kernel(EvalParams P) {
 
 static EvalParams make_params(Ctx& c) {
     EvalParams P{};
-    P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx};
+    P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
     P.S = BlockState{c.bm.assign, c.bm.dout, c.bm.din, c.bm.deg, c.bm.B};
     P.M = CsrView{c.bm.row_ptr, c.bm.row_col, c.bm.row_val, c.bm.row_cum,
                   c.bm.col_ptr, c.bm.col_row, c.bm.col_val, c.bm.col_cum};

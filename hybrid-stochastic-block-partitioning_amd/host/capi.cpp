@@ -14,9 +14,9 @@ typedef struct sbp_run_stats {
     int64_t sweeps, attempts, evals, moves;
     double seconds;       // wall time of stochastic_block_partition, graph already resident
     double mcmc_seconds;  // wall time inside the finetune phases
-    double eval_ms[3];    // device time of the vertex-move evaluation kernels, per degree bin
-    double eval_bytes[3]; // their algorithmic bytes (when counting is on)
-    int64_t eval_launches[3];
+    double eval_ms[4];    // device time of the vertex-move evaluation kernels: degree bins 0-2, sequential MH
+    double eval_bytes[4]; // their algorithmic bytes (when counting is on)
+    int64_t eval_launches[4];
     double mdl;           // final MDL (float, as Blockmodel caches it)
 } sbp_run_stats;
 
@@ -70,7 +70,7 @@ int sbp_partition(void* gp, int algorithm, int32_t batches, int approximate, uin
             stats->moves = finetune::totals.moves;
             stats->seconds = secs;
             stats->mcmc_seconds = finetune::MCMC_time;
-            for (int b = 0; b < 3; ++b) {
+            for (int b = 0; b < 4; ++b) {
                 stats->eval_ms[b] = finetune::totals.eval_ms[b];
                 stats->eval_bytes[b] = finetune::totals.eval_bytes[b];
                 stats->eval_launches[b] = finetune::totals.eval_launches[b];

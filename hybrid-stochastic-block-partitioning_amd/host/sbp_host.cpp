@@ -217,7 +217,7 @@ static void add_totals(const sbpx_sweep_stats& st) {
     totals.attempts += st.attempts;
     totals.evals += st.evals;
     totals.moves += st.moves;
-    for (int b = 0; b < 3; ++b) {
+    for (int b = 0; b < 4; ++b) {
         totals.eval_ms[b] += st.eval_ms[b];
         totals.eval_bytes[b] += st.eval_bytes[b];
         totals.eval_launches[b] += st.eval_launches[b];

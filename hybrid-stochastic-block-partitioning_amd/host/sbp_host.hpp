@@ -112,8 +112,8 @@ extern double MCMC_time;
 static const int MAX_NUM_ITERATIONS = 100;
 struct Totals {
     long long sweeps = 0, attempts = 0, evals = 0, moves = 0;
-    double eval_ms[3] = {0, 0, 0}, eval_bytes[3] = {0, 0, 0};
-    long long eval_launches[3] = {0, 0, 0};
+    double eval_ms[4] = {0, 0, 0, 0}, eval_bytes[4] = {0, 0, 0, 0};
+    long long eval_launches[4] = {0, 0, 0, 0};
 };
 extern Totals totals;
 Blockmodel& asynchronous_gibbs(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);

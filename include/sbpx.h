@@ -47,12 +47,12 @@ typedef struct sbpx_sweep_stats {
     int64_t evals;        /* proposals with s != r: delta MDL, Hastings and accept computed */
     int64_t moves;        /* accepted moves */
     double dS_accepted;   /* sum of delta MDL over accepted moves */
-    /* Per degree bin of the vertex-move evaluation kernels (0: degree <= 127, 1: <= 2047,
-     * 2: larger): device time, algorithmic bytes (when byte counting is on), launches. */
-    double eval_ms[3];
-    double eval_bytes[3];
-    int32_t eval_launches[3];
-    int32_t reserved;
+    /* Per vertex-move evaluation kernel: 0-2 the batched kernels by degree bin (0: degree <= 127,
+     * 1: <= 2047, 2: larger), 3 the sequential Metropolis-Hastings kernel. Device time,
+     * algorithmic bytes of the evaluations that count (when byte counting is on), launches. */
+    double eval_ms[4];
+    double eval_bytes[4];
+    int32_t eval_launches[4];
 } sbpx_sweep_stats;
 
 /* Context. Replaces the process-global state of the reference (args, static RNG engines). */

@@ -274,7 +274,12 @@ def test_sequential_sweeps_match_oracle(algo):
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(7)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
-    for B, a in ((n, np.arange(n, dtype=np.int32)), (128, rng.integers(0, 128, n).astype(np.int32)), (8, truth)):
+    # a giant block 0 next to ~600 small ones: its row and column hold more than 256 slots, so
+    # proposals drawn from it take the multi-pass radix selection
+    giant = rng.integers(1, 600, n).astype(np.int32)
+    giant[rng.random(n) < 0.4] = 0
+    for B, a in ((n, np.arange(n, dtype=np.int32)), (128, rng.integers(0, 128, n).astype(np.int32)), (8, truth),
+                 (600, giant)):
         c.build(B, a)
         bm = O.Blockmodel(g, B, a)
         for call in range(3):

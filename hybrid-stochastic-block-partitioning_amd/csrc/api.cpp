@@ -290,8 +290,15 @@ const char* sbpx_last_error(const sbpx_ctx* ctx) { return ctx ? ctx->c.err.c_str
 
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
     return guarded(ctx, [&] {
-        if (option == 1) ctx->count_bytes = value != 0;
-        else throw ArgError("unknown option");
+        if (option == 1) {
+            ctx->count_bytes = value != 0;
+        } else if (option == 2) {
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+                throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");
+            ctx->c.mh_group = (int)value;
+        } else {
+            throw ArgError("unknown option");
+        }
     });
 }
 
@@ -415,9 +422,9 @@ sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t N, int64_t E, const int32_t
         dalloc(&h.cocc, NB);
         dalloc(&h.rcap, NB);
         dalloc(&h.ccap, NB);
-        dalloc(&h.status, 8);
+        dalloc(&h.status, 12);
         dalloc(&h.touch, N);
-        SBPX_HIP(hipMemset(h.status, 0, sizeof(int) * 8));
+        SBPX_HIP(hipMemset(h.status, 0, sizeof(int) * 12));
         SBPX_HIP(hipMemset(h.touch, 0, sizeof(int) * std::max(N, 1)));
         dalloc(&h.d_all, NB);
         dalloc(&h.d_high, NB);
@@ -737,16 +744,7 @@ sbpx_status sbpx_merge_proposals(sbpx_ctx* ctx, uint64_t seed, uint32_t call, do
 static void carry_out(Ctx& c, int num_to_merge, int mode, const double* dS, const int32_t* best) {
     const int B = c.bm.B;
     if (num_to_merge < 0 || num_to_merge >= B) throw ArgError("num_to_merge must be in [0, B)");
-    if (mode != SBPX_MERGE_APPROXIMATE) throw std::runtime_error("advanced merge selection not built yet");
-    // utils::sort_indices (utils.hpp:94-101): std::sort ascending, same comparator, so ties
-    // fall in the same order as the reference's.
-    std::vector<int> order(B);
-    std::iota(order.begin(), order.end(), 0);
-    std::vector<double> d(dS, dS + B);
-    std::sort(order.data(), order.data() + order.size(), [&d](size_t i1, size_t i2) { return d[i1] < d[i2]; });
-    // Blockmodel::carry_out_best_merges (blockmodel.cpp:50-78): block_map always names the
-    // block a block has been merged into; a union-find with the merge target as root keeps
-    // the same mapping in near-linear time.
+    if (mode != SBPX_MERGE_APPROXIMATE && mode != SBPX_MERGE_ADVANCED) throw ArgError("unknown merge mode");
     std::vector<int> parent(B);
     std::iota(parent.begin(), parent.end(), 0);
     auto find = [&parent](int x) {
@@ -759,18 +757,34 @@ static void carry_out(Ctx& c, int num_to_merge, int mode, const double* dS, cons
         }
         return root;
     };
-    int num_merged = 0;
-    size_t counter = 0;
-    while (num_merged < num_to_merge) {
-        if (counter >= order.size()) throw StateError("ran out of merge candidates");
-        int from = order[counter];
-        int b = best[from];
-        if (b < 0 || b >= B) throw StateError("block without a merge proposal");
-        int to = find(b);
-        counter++;
-        if (to != from) {
-            parent[from] = to;
-            num_merged++;
+    if (mode == SBPX_MERGE_ADVANCED) {
+        for (int x = 0; x < B; ++x)
+            if (best[x] >= B) throw ArgError("proposed block out of range [0, B)");
+        merge_heap_select(c, num_to_merge, dS, best, parent);
+    } else {
+        // utils::sort_indices (utils.hpp:94-101): std::sort ascending, same comparator, so ties
+        // fall in the same order as the reference's.
+        std::vector<int> order(B);
+        std::iota(order.begin(), order.end(), 0);
+        std::vector<double> d(dS, dS + B);
+        std::sort(order.data(), order.data() + order.size(),
+                  [&d](size_t i1, size_t i2) { return d[i1] < d[i2]; });
+        // Blockmodel::carry_out_best_merges (blockmodel.cpp:50-78): block_map always names the
+        // block a block has been merged into; a union-find with the merge target as root keeps
+        // the same mapping in near-linear time.
+        int num_merged = 0;
+        size_t counter = 0;
+        while (num_merged < num_to_merge) {
+            if (counter >= order.size()) throw StateError("ran out of merge candidates");
+            int from = order[counter];
+            int b = best[from];
+            if (b < 0 || b >= B) throw StateError("block without a merge proposal");
+            int to = find(b);
+            counter++;
+            if (to != from) {
+                parent[from] = to;
+                num_merged++;
+            }
         }
     }
     // build_mapping (blockmodel.cpp:29-41): surviving labels of non-empty blocks, ascending.

@@ -285,6 +285,7 @@ void check_device_error(Ctx& c) {
         if (e & DEV_ERR_ZERO_DEGREE) throw NumericError("nonzero block-matrix cell with a zero block degree");
         if (e & DEV_ERR_NAN) throw NumericError("NaN or Inf in a delta MDL");
         if (e & DEV_ERR_TABLE) throw StateError("neighbour-block table overflow");
+        if (e & DEV_ERR_NO_PROPOSAL) throw StateError("block without a merge proposal");
         throw StateError("device error");
     }
 }

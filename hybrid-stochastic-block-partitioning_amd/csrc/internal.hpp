@@ -38,6 +38,7 @@ enum : int {
     DEV_ERR_NAN = 2,           // NaN/Inf delta MDL (entropy.cpp:323-340)
     DEV_ERR_TABLE = 4,         // neighbour-block table overflow
     DEV_ERR_BLOCK_RANGE = 8,   // block id out of range
+    DEV_ERR_NO_PROPOSAL = 16,  // a merge candidate without a proposed block
 };
 
 // Degree bins for the vertex-move kernels: neighbour-block table capacity per bin.
@@ -100,9 +101,12 @@ struct MhState {
     int *rocc = nullptr, *cocc = nullptr;    // [N] keys held by each row / column
     int* tables = nullptr;  // neighbour-block tables, one per wavefront
     size_t table_slots = 0;
-    int* status = nullptr;  // [8]: next queue position, stop reason, row pool top, column pool top,
-                            // commit serial
+    int* status = nullptr;  // [12]: next queue position, stop reason, row pool top, column pool top,
+                            // commit serial, longest row, pending merge (2), heap length
     int* touch = nullptr;   // [N] serial of the last commit that moved a neighbour of each vertex
+    int max_cap = 0;        // slots of the longest row or column when the tables were built
+    double mean_scan = -1.0;  // mean row/column slots scanned per evaluation in the last chunk
+    int last_B = -1;          // number of blocks when mean_scan was measured
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
     int max_deg_high = 0;
@@ -116,6 +120,7 @@ struct Ctx {
     DevBlockmodel bm;
     SweepPlan plan;
     MhState mh;
+    int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
 
     // scratch
     void* keys0 = nullptr;
@@ -212,6 +217,8 @@ struct EvalLaunch {
 void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
 void finish_degrees(Ctx& c);
+// Heap-ordered merge selection on the device; parent: the union-find of merged blocks (B).
+void merge_heap_select(Ctx& c, int num_to_merge, const double* dS, const int32_t* best, std::vector<int>& parent);
 void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,
                    long long* bytes, bool timed);
 

@@ -14,14 +14,17 @@
 // the kernel stop; the host then rebuilds the tables with a larger pool and the pass resumes at
 // the same vertex.
 #include <algorithm>
+#include <numeric>
 #include <cstdint>
 
 #include "eval_core.hpp"
 #include "internal.hpp"
+#include "merge_core.hpp"
 
 namespace sbpx {
 
 constexpr int MH_WAVES = 16;
+constexpr int MH_CHUNK = 1024;  // queue entries per launch
 
 // ------------------------------------------------------------------------------------------
 // Table build: cap(x) = next power of two >= 2 nnz(x) + 32, filled from the CSR.
@@ -32,10 +35,12 @@ __device__ __forceinline__ int table_cap(int keys) {
     return cap;
 }
 
-__global__ void hash_caps_kernel(const int* __restrict__ ptr, int B, int* __restrict__ caps) {
+__global__ void hash_caps_kernel(const int* __restrict__ ptr, int B, int* __restrict__ caps, int* max_cap) {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= B) return;
-    caps[x] = table_cap(ptr[x + 1] - ptr[x]);
+    const int cap = table_cap(ptr[x + 1] - ptr[x]);
+    caps[x] = cap;
+    atomicMax(max_cap, cap);
 }
 
 // One wavefront per row: clear the row's slots, then insert its CSR entries.
@@ -97,6 +102,8 @@ struct MhParams {
     long long* counters;  // attempts, evals, moves
     long long* bytes;     // algorithmic bytes of the evaluations that stand, or null
     int* touch;           // [N] serial of the last commit that moved a neighbour of the vertex
+    double* scan;         // [2] slots scanned in rows/columns r and s, and evaluations, that stand
+    int stop_at;          // queue position where this launch ends
     double* dS_sum;
     int* err;
 };
@@ -326,11 +333,11 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
     const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[wave]};
     long long att = 0, evs = 0, mvs = 0, bytes_local = 0;
-    double dsum = 0.0;
+    double dsum = 0.0, scan_local = 0.0, scan_count = 0.0;
     int pos = P.start;
     int stopped = 0;
     int serial = P.status[4];
-    while (pos < P.n && stopped == 0) {
+    while (pos < P.stop_at && stopped == 0) {
         // a window of up to 16 queue entries, each evaluated against the current state
         const int idx = pos + wave;
         MoveResult R{};
@@ -362,8 +369,13 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
                 }
             }
             const int end = j >= 0 ? j : W;
-            if (P.bytes != nullptr && lane == 0 && wave >= cur && wave < end && st[wave] != 3)
-                bytes_local += eval_bytes(P, R);
+            if (lane == 0 && wave >= cur && wave < end && st[wave] != 3) {
+                if (P.bytes != nullptr) bytes_local += eval_bytes(P, R);
+                if (R.status >= 1) {
+                    scan_local += (double)R.nnz_rs;
+                    scan_count += 1.0;
+                }
+            }
             if (threadIdx.x == 0) {
                 for (int w = cur; w < end; ++w) {
                     att += st[w] != 3;
@@ -402,7 +414,11 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
                 break;
             }
             ++serial;
-            if (wave == j && lane == 0 && P.bytes != nullptr) bytes_local += eval_bytes(P, R);
+            if (wave == j && lane == 0) {
+                if (P.bytes != nullptr) bytes_local += eval_bytes(P, R);
+                scan_local += (double)R.nnz_rs;
+                scan_count += 1.0;
+            }
             if (threadIdx.x == 0) {
                 att++;
                 evs++;
@@ -420,6 +436,560 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     }
     if (P.bytes != nullptr && lane == 0 && bytes_local > 0)
         atomicAdd((unsigned long long*)P.bytes, (unsigned long long)bytes_local);
+    if (lane == 0 && scan_count > 0.0) {
+        atomicAdd(&P.scan[0], scan_local);
+        atomicAdd(&P.scan[1], scan_count);
+    }
+    if (threadIdx.x == 0) {
+        P.status[0] = pos;
+        P.status[1] = stopped;
+        P.status[4] = serial;
+        atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
+        atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
+        atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)mvs);
+        P.dS_sum[0] += dsum;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Group variant: G wavefronts share one vertex's evaluation, so a vertex whose rows hold
+// thousands of slots is scanned by up to 1024 lanes, and 16 / G vertices are evaluated side by
+// side. The table lives in the group's LDS slice (or its global slice), partial results meet in
+// LDS, and every group runs the same sequence of workgroup barriers (idle groups included), so
+// the barriers never depend on the data. Results equal those of eval_move: the same cells are
+// visited and the same terms formed; only the order of the FP64 sums differs.
+constexpr int RED_D = 4;   // per-wavefront double partials: dS, pf, pb
+constexpr int RED_I = 12;  // per-wavefront int partials
+
+template <int G>
+__device__ MoveResult eval_group(const MhParams& P, const GraphView& Gr, const BlockState& S, const HashView& M,
+                                 int v, int* tab, int cap, int* hist, double (*redd)[RED_D], int (*redi)[RED_I]) {
+    constexpr int GS = G * WAVE;
+    const int lane = lane_id();
+    const int wave = threadIdx.x / WAVE;
+    const int wig = wave % G;
+    const int w0 = wave - wig;
+    const int gl = wig * WAVE + lane;
+    const bool act = v >= 0;
+    const int B = S.B;
+    MoveResult R{};
+    R.v = v;
+    R.t = -1;
+    R.status = 3;
+    const int r = act ? S.assign[v] : 0;
+    R.r = r;
+    int* tkey = tab;
+    int* to = tab + cap;
+    int* ti = tab + 2 * cap;
+    int* tmsb = tab + 3 * cap;
+    int* tmbs = tab + 4 * cap;
+    int* tmrb = tab + 5 * cap;
+    int* tmbr = tab + 6 * cap;
+    if (act) {
+        for (int j = gl; j < cap; j += GS) {
+            tkey[j] = -1;
+            to[j] = 0;
+            ti[j] = 0;
+            tmsb[j] = 0;
+            tmbs[j] = 0;
+            tmrb[j] = 0;
+            tmbr[j] = 0;
+        }
+    }
+    const int ob = act ? Gr.out_ptr[v] : 0, dO = act ? Gr.out_ptr[v + 1] - ob : 0;
+    const int ib = act ? Gr.in_ptr[v] : 0, dI = act ? Gr.in_ptr[v + 1] - ib : 0;
+    const int self = act ? Gr.self[v] : 0;
+    const int k_out = dO, k_in = dI - self, k = k_out + k_in;
+    R.k_out = k_out;
+    R.k_in = k_in;
+    R.self = self;
+    Draw4 dr{0u, 0u, 0u, 0u};
+    double u = 0.0;
+    int pick0 = -1, pick1 = -1;
+    if (act) {
+        dr = philox4x32_10(P.seed, (uint32_t)v, P.call, 0u, P.tag);
+        u = unit53(dr.x2, dr.x3);
+        if (k > 0) {
+            const int ps = (int)mulhi32(dr.x0, (uint32_t)k);
+            if (ps < k_out) {
+                pick0 = ps;
+            } else {
+                pick0 = dO + (ps - k_out);
+                pick1 = pick0 + 1;
+            }
+        }
+    }
+    __syncthreads();  // table cleared
+
+    // 1. neighbour blocks
+    int self_pos = INT_MAX, full = 0, cand0 = -1, cand1 = -1;
+    for (int j = gl; j < dO + dI; j += GS) {
+        const bool is_out = j < dO;
+        const int w = is_out ? Gr.out_idx[ob + j] : Gr.in_idx[ib + (j - dO)];
+        const int bw = S.assign[w];
+        if (j == pick0) cand0 = bw;
+        if (j == pick1) cand1 = bw;
+        if (w == v) {
+            if (!is_out) self_pos = j - dO;
+        } else {
+            int slot = tab_insert(tkey, cap, bw);
+            if (slot < 0) full = 1;
+            else atomicAdd(is_out ? &to[slot] : &ti[slot], 1);
+        }
+    }
+    if (act && self && gl == 0 && tab_insert(tkey, cap, r) < 0) full = 1;
+    self_pos = wave_min(self_pos);
+    cand0 = wave_max(cand0);
+    cand1 = wave_max(cand1);
+    full = wave_or(full);
+    if (lane == 0) {
+        redi[wave][0] = self_pos;
+        redi[wave][1] = cand0;
+        redi[wave][2] = cand1;
+        redi[wave][3] = full;
+    }
+    __syncthreads();  // table built, partials visible
+    for (int i = 0; i < G; ++i) {
+        self_pos = min(self_pos, redi[w0 + i][0]);
+        cand0 = max(cand0, redi[w0 + i][1]);
+        cand1 = max(cand1, redi[w0 + i][2]);
+        full |= redi[w0 + i][3];
+    }
+
+    // 2. proposal: s drawn from row t + column t by a group-wide radix descent over the block index
+    int s = r;
+    bool sample = false;
+    const int* skeys = nullptr;
+    const int* svals = nullptr;
+    int slo = 0, shi = 0, starget = 0;
+    if (act && !full) {
+        if (k == 0) {
+            s = random_block(r, B, dr.x0);
+        } else {
+            int t = cand0;
+            if (pick1 >= 0 && pick0 - dO >= self_pos) t = cand1;
+            R.t = t;
+            const int trow = S.dout[t];
+            const int total = trow + S.din[t];
+            const int target = (int)mulhi32(dr.x1, (uint32_t)total);
+            const bool row = target < trow;
+            skeys = row ? M.rkeys : M.ckeys;
+            svals = row ? M.rvals : M.cvals;
+            slo = row ? M.rbase[t] : M.cbase[t];
+            shi = slo + (row ? M.rcap[t] : M.ccap[t]);
+            starget = row ? target : target - trow;
+            sample = true;
+        }
+    }
+    {
+        int nbits = 1;
+        while (nbits < 31 && (1 << nbits) < B) ++nbits;
+        int shift = ((nbits + 7) / 8 - 1) * 8;
+        int prefix = 0;
+        long long below = 0;
+        for (;;) {
+            for (int j = gl; j < 256; j += GS) hist[j] = 0;
+            __syncthreads();
+            if (sample) {
+                const int up = shift + 8;
+                for (int j = slo + gl; j < shi; j += GS) {
+                    const int kk = skeys[j];
+                    if (kk < 0) continue;
+                    const int w = svals[j];
+                    if (w == 0) continue;
+                    const int hp = up >= 31 ? 0 : (kk >> up);
+                    if (hp == prefix) atomicAdd(&hist[(kk >> shift) & 255], w);
+                }
+            }
+            __syncthreads();
+            if (sample) {
+                long long h[4];
+                long long local = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    h[q] = hist[lane * 4 + q];
+                    local += h[q];
+                }
+                long long incl = local;
+#pragma unroll
+                for (int off = 1; off < WAVE; off <<= 1) {
+                    long long y = __shfl_up(incl, off, WAVE);
+                    if (lane >= off) incl += y;
+                }
+                long long run = below + (incl - local);
+                int digit = INT_MAX;
+                long long before = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (digit == INT_MAX && run + h[q] > starget) {
+                        digit = lane * 4 + q;
+                        before = run;
+                    }
+                    run += h[q];
+                }
+                const int d = wave_min(digit);
+                below = __shfl(before, d == INT_MAX ? 0 : d / 4, WAVE);
+                prefix = (prefix << 8) | (d == INT_MAX ? 255 : d);
+            }
+            __syncthreads();  // histogram read before the next pass clears it
+            if (shift == 0) break;
+            shift -= 8;
+        }
+        if (sample) s = prefix < B ? prefix : B - 1;
+    }
+    R.s = s;
+    R.u = u;
+    R.H = 1.0;
+    R.dS = 0.0;
+    const bool ev = act && !full && s != r;
+
+    // 3. rows s, r and columns s, r
+    int o_r = 0, i_r = 0, o_s = 0, i_s = 0;
+    if (ev) {
+        const int slot_r = tab_find(tkey, cap, r), slot_s = tab_find(tkey, cap, s);
+        o_r = slot_r >= 0 ? to[slot_r] : 0;
+        i_r = slot_r >= 0 ? ti[slot_r] : 0;
+        o_s = slot_s >= 0 ? to[slot_s] : 0;
+        i_s = slot_s >= 0 ? ti[slot_s] : 0;
+    }
+    const int dout_r = ev ? S.dout[r] : 0, dout_s = ev ? S.dout[s] : 0;
+    const int din_r = ev ? S.din[r] : 0, din_s = ev ? S.din[s] : 0;
+    const int ndout_r = dout_r - k_out, ndout_s = dout_s + k_out;
+    const int ndin_r = din_r - (k_in + self), ndin_s = din_s + (k_in + self);
+    const bool t_ss = o_s > 0 || i_s > 0 || self, t_rr = o_r > 0 || i_r > 0 || self;
+    const bool t_sr = o_r > 0 || i_s > 0, t_rs = o_s > 0 || i_r > 0;
+    const int d_ss = o_s + i_s + self, d_rr = -(o_r + i_r + self);
+    const int d_sr = o_r - i_s, d_rs = i_r - o_s;
+    int m_ss = 0, m_rr = 0, m_sr = 0, m_rs = 0;
+    double acc = 0.0;
+    long long live = 0;
+    if (ev) {
+        const int b0 = M.rbeg(s), L0 = M.rend(s) - b0;
+        const int b1 = M.rbeg(r), L1 = M.rend(r) - b1;
+        const int b2 = M.cbeg(s), L2 = M.cend(s) - b2;
+        const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;
+        const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
+        for (int i = gl; i < T; i += GS) {
+            const int seg = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : 3;
+            int c, m;
+            if (seg == 0) {
+                c = M.rkeys[b0 + i];
+                m = M.rvals[b0 + i];
+            } else if (seg == 1) {
+                c = M.rkeys[b1 + (i - L0)];
+                m = M.rvals[b1 + (i - L0)];
+            } else if (seg == 2) {
+                c = M.ckeys[b2 + (i - T1)];
+                m = M.cvals[b2 + (i - T1)];
+            } else {
+                c = M.ckeys[b3 + (i - T2)];
+                m = M.cvals[b3 + (i - T2)];
+            }
+            if (c < 0 || m == 0) continue;
+            ++live;
+            bool unchanged = true;
+            if (seg == 0) {
+                if (c == s) {
+                    m_ss = m;
+                    unchanged = !t_ss;
+                } else if (c == r) {
+                    m_sr = m;
+                    unchanged = !t_sr || d_sr == 0;
+                } else {
+                    const int sl = tab_find(tkey, cap, c);
+                    if (sl >= 0) {
+                        tmsb[sl] = m;
+                        unchanged = to[sl] == 0;
+                    }
+                }
+            } else if (seg == 1) {
+                if (c == r) {
+                    m_rr = m;
+                    unchanged = !t_rr;
+                } else if (c == s) {
+                    m_rs = m;
+                    unchanged = !t_rs || d_rs == 0;
+                } else {
+                    const int sl = tab_find(tkey, cap, c);
+                    if (sl >= 0) {
+                        tmrb[sl] = m;
+                        unchanged = to[sl] == 0;
+                    }
+                }
+            } else {
+                if (c == r || c == s) continue;
+                const int sl = tab_find(tkey, cap, c);
+                if (sl >= 0) {
+                    (seg == 2 ? tmbs : tmbr)[sl] = m;
+                    unchanged = ti[sl] == 0;
+                }
+            }
+            if (!unchanged) continue;
+            const float mf = fi(m);
+            if (seg < 2) {
+                const int d = S.din[c];
+                const int xo = seg == 0 ? dout_s : dout_r, nxo = seg == 0 ? ndout_s : ndout_r;
+                const int nd = c == r ? ndin_r : (c == s ? ndin_s : d);
+                acc += (double)cell_entropy(mf, fi(d), fi(xo));
+                acc -= (double)cell_entropy(mf, fi(nd), fi(nxo));
+            } else {
+                const int d = S.dout[c];
+                const int xi = seg == 2 ? din_s : din_r, nxi = seg == 2 ? ndin_s : ndin_r;
+                acc += (double)cell_entropy(mf, fi(xi), fi(d));
+                acc -= (double)cell_entropy(mf, fi(nxi), fi(d));
+            }
+        }
+    }
+    m_ss = wave_max(m_ss);
+    m_rr = wave_max(m_rr);
+    m_sr = wave_max(m_sr);
+    m_rs = wave_max(m_rs);
+    live = wave_sum_ll(live);
+    if (lane == 0) {
+        redi[wave][4] = m_ss;
+        redi[wave][5] = m_rr;
+        redi[wave][6] = m_sr;
+        redi[wave][7] = m_rs;
+        redi[wave][8] = (int)live;
+    }
+    __syncthreads();  // cell records in the table and partials visible
+    live = 0;
+    for (int i = 0; i < G; ++i) {
+        m_ss = max(m_ss, redi[w0 + i][4]);
+        m_rr = max(m_rr, redi[w0 + i][5]);
+        m_sr = max(m_sr, redi[w0 + i][6]);
+        m_rs = max(m_rs, redi[w0 + i][7]);
+        live += redi[w0 + i][8];
+    }
+
+    // 4. Delta keys and Hastings sums
+    double pf = 0.0, pb = 0.0;
+    int nuniq = 0;
+    if (ev) {
+        for (int j = gl; j < cap; j += GS) {
+            const int b = tkey[j];
+            if (b < 0) continue;
+            nuniq++;
+            const int ob_ = to[j], ib_ = ti[j];
+            const int degb = S.deg[b];
+            int Msb, Mbs, Mrb, Mbr, Drb, Dbr, nd;
+            if (b == r) {
+                Msb = m_sr; Mbs = m_rs; Mrb = m_rr; Mbr = m_rr; Drb = d_rr; Dbr = d_rr;
+                nd = (dout_r - k_out) + (din_r - k_in) - (m_rr + d_rr);
+            } else if (b == s) {
+                Msb = m_ss; Mbs = m_ss; Mrb = m_rs; Mbr = m_sr; Drb = d_rs; Dbr = d_sr;
+                nd = (dout_s + k_out) + (din_s + k_in) - (m_ss + d_ss);
+            } else {
+                Msb = tmsb[j]; Mbs = tmbs[j]; Mrb = tmrb[j]; Mbr = tmbr[j]; Drb = -ob_; Dbr = -ib_;
+                nd = degb;
+                const int db = S.din[b], ub = S.dout[b];
+                if (ob_ > 0) {
+                    acc += (double)cell_entropy(fi(Msb), fi(db), fi(dout_s));
+                    acc -= (double)cell_entropy(fi(Msb + ob_), fi(db), fi(ndout_s));
+                    acc += (double)cell_entropy(fi(Mrb), fi(db), fi(dout_r));
+                    acc -= (double)cell_entropy(fi(Mrb - ob_), fi(db), fi(ndout_r));
+                }
+                if (ib_ > 0) {
+                    acc += (double)cell_entropy(fi(Mbs), fi(din_s), fi(ub));
+                    acc -= (double)cell_entropy(fi(Mbs + ib_), fi(ndin_s), fi(ub));
+                    acc += (double)cell_entropy(fi(Mbr), fi(din_r), fi(ub));
+                    acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
+                }
+            }
+            const int nb = ob_ + ib_ + (b == r ? self : 0);
+            if (nb == 0) continue;
+            const double count = (double)nb;
+            pf += count * ((double)(Msb + Mbs) + 1.0) / (double)(degb + B);
+            pb += count * ((double)(Mrb + Drb + Mbr + Dbr) + 1.0) / (double)(nd + B);
+        }
+        if (gl == 0) {
+            if (t_ss) {
+                acc += (double)cell_entropy(fi(m_ss), fi(din_s), fi(dout_s));
+                acc -= (double)cell_entropy(fi(m_ss + d_ss), fi(ndin_s), fi(ndout_s));
+            }
+            if (t_rr) {
+                acc += (double)cell_entropy(fi(m_rr), fi(din_r), fi(dout_r));
+                acc -= (double)cell_entropy(fi(m_rr + d_rr), fi(ndin_r), fi(ndout_r));
+            }
+            if (t_sr) {
+                acc += (double)cell_entropy(fi(m_sr), fi(din_r), fi(dout_s));
+                acc -= (double)cell_entropy(fi(m_sr + d_sr), fi(ndin_r), fi(ndout_s));
+            }
+            if (t_rs) {
+                acc += (double)cell_entropy(fi(m_rs), fi(din_s), fi(dout_r));
+                acc -= (double)cell_entropy(fi(m_rs + d_rs), fi(ndin_s), fi(ndout_r));
+            }
+        }
+    }
+    acc = wave_sum(acc);
+    pf = wave_sum(pf);
+    pb = wave_sum(pb);
+    nuniq = (int)wave_sum_ll(nuniq);
+    if (lane == 0) {
+        redd[wave][0] = acc;
+        redd[wave][1] = pf;
+        redd[wave][2] = pb;
+        redi[wave][9] = nuniq;
+    }
+    __syncthreads();
+    double dS = 0.0;
+    pf = 0.0;
+    pb = 0.0;
+    nuniq = 0;
+    for (int i = 0; i < G; ++i) {
+        dS += redd[w0 + i][0];
+        pf += redd[w0 + i][1];
+        pb += redd[w0 + i][2];
+        nuniq += redi[w0 + i][9];
+    }
+    R.nuniq = nuniq;
+    R.m_ss = m_ss;
+    R.m_rr = m_rr;
+    R.m_sr = m_sr;
+    R.m_rs = m_rs;
+    R.d_ss = d_ss;
+    R.d_rr = d_rr;
+    R.d_sr = d_sr;
+    R.d_rs = d_rs;
+    R.nnz_rs = live;
+    if (!act) return R;
+    if (full) {
+        R.status = -1;
+        return R;
+    }
+    if (!ev) {
+        R.status = 0;
+        return R;
+    }
+    const double H = (k == 0) ? 1.0 : pb / pf;
+    double a = exp(-3.0 * dS) * H;
+    a = (a >= 1.0) ? 1.0 : a;
+    R.dS = dS;
+    R.H = H;
+    R.status = (u <= a) ? 2 : 1;
+    return R;
+}
+
+template <int G>
+__global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
+    constexpr int NG = MH_WAVES / G;
+    constexpr int LCAP = MH_WAVES * MH_LDS_CAP / NG;  // LDS table slots per group
+    __shared__ int st[NG];
+    __shared__ int holds[NG];
+    __shared__ double sdS[NG];
+    __shared__ int s_flag, s_cr, s_cs, s_ccap;
+    __shared__ int* s_ctab;
+    __shared__ int lds_tab[NG][7 * LCAP];
+    __shared__ int lds_hist[NG][256];
+    __shared__ double redd[MH_WAVES][RED_D];
+    __shared__ int redi[MH_WAVES][RED_I];
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    const int g = wave / G, wig = wave % G;
+    const bool leader = wig == 0;  // the group's first wavefront commits and checks
+    int* const gtab = P.tables + (size_t)g * 7 * P.tcap;
+    const GraphView Gr = P.G;
+    const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
+    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, nullptr};
+    long long att = 0, evs = 0, mvs = 0, bytes_local = 0;
+    double dsum = 0.0, scan_local = 0.0, scan_count = 0.0;
+    int pos = P.start;
+    int stopped = 0;
+    int serial = P.status[4];
+    while (pos < P.stop_at && stopped == 0) {
+        const int idx = pos + g;
+        const bool active = idx < P.n;
+        const int v = active ? P.list[idx] : -1;
+        const int cap = active ? vertex_table_cap(Gr, v) : 16;
+        int* tab = cap <= LCAP ? lds_tab[g] : gtab;
+        MoveResult R = eval_group<G>(P, Gr, S, M, v, tab, cap, lds_hist[g], redd, redi);
+        if (leader && lane == 0) {
+            st[g] = active ? R.status : 3;
+            holds[g] = 1;
+            sdS[g] = active ? R.dS : 0.0;
+        }
+        __syncthreads();
+        const int W = min(NG, P.n - pos);
+        int cur = 0;
+        for (;;) {
+            int j = -1;
+            for (int w = cur; w < W; ++w) {
+                if (!holds[w] || st[w] == 2 || st[w] < 0) {
+                    j = w;
+                    break;
+                }
+            }
+            const int end = j >= 0 ? j : W;
+            if (leader && lane == 0 && g >= cur && g < end && st[g] != 3) {
+                if (P.bytes != nullptr) bytes_local += eval_bytes(P, R);
+                if (R.status >= 1) {
+                    scan_local += (double)R.nnz_rs;
+                    scan_count += 1.0;
+                }
+            }
+            if (threadIdx.x == 0) {
+                for (int w = cur; w < end; ++w) {
+                    att += st[w] != 3;
+                    evs += st[w] == 1;
+                }
+            }
+            if (j < 0) {
+                pos += W;
+                break;
+            }
+            if (!holds[j]) {
+                pos += j;
+                break;
+            }
+            if (st[j] < 0) {
+                if (threadIdx.x == 0) atomicOr(P.err, DEV_ERR_TABLE);
+                stopped = 2;
+                pos += j;
+                break;
+            }
+            if (g == j && leader) {
+                const bool ok = make_room(P, R, tab, cap);
+                if (ok) commit_move(P, R, tab, cap, serial + 1);
+                if (lane == 0) {
+                    s_flag = ok ? 1 : 2;
+                    s_cr = R.r;
+                    s_cs = R.s;
+                    s_ctab = tab;
+                    s_ccap = cap;
+                }
+            }
+            __syncthreads();
+            if (s_flag == 2) {
+                stopped = 1;
+                pos += j;
+                break;
+            }
+            ++serial;
+            if (g == j && leader && lane == 0) {
+                if (P.bytes != nullptr) bytes_local += eval_bytes(P, R);
+                scan_local += (double)R.nnz_rs;
+                scan_count += 1.0;
+            }
+            if (threadIdx.x == 0) {
+                att++;
+                evs++;
+                mvs++;
+                dsum += sdS[j];
+            }
+            if (leader && g > j && g < W && holds[g] && st[g] != 3) {
+                const bool h = still_holds(P, M, R, tab, cap, s_ctab, s_ccap, s_cr, s_cs, serial);
+                if (lane == 0 && !h) holds[g] = 0;
+            }
+            __syncthreads();
+            cur = j + 1;
+        }
+        __syncthreads();
+    }
+    if (P.bytes != nullptr && lane == 0 && bytes_local > 0)
+        atomicAdd((unsigned long long*)P.bytes, (unsigned long long)bytes_local);
+    if (lane == 0 && scan_count > 0.0) {
+        atomicAdd(&P.scan[0], scan_local);
+        atomicAdd(&P.scan[1], scan_count);
+    }
     if (threadIdx.x == 0) {
         P.status[0] = pos;
         P.status[1] = stopped;
@@ -438,13 +1008,15 @@ static void build_tables(Ctx& c) {
     MhState& h = c.mh;
     const int tb = 256;
     // capacities and bases, rows then columns
-    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.row_ptr, B, h.rcap);
+    SBPX_HIP(hipMemsetAsync(h.status + 5, 0, sizeof(int), st));
+    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.row_ptr, B, h.rcap, h.status + 5);
     scan_exclusive(h.rcap, h.rbase, B, c.scan_tmp, st);
-    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.col_ptr, B, h.ccap);
+    hash_caps_kernel<<<(B + tb - 1) / tb, tb, 0, st>>>(c.bm.col_ptr, B, h.ccap, h.status + 5);
     scan_exclusive(h.ccap, h.cbase, B, c.scan_tmp, st);
     int tot[2] = {0, 0};
     SBPX_HIP(hipMemcpyAsync(&tot[0], h.rbase + B, sizeof(int), hipMemcpyDeviceToHost, st));
     SBPX_HIP(hipMemcpyAsync(&tot[1], h.cbase + B, sizeof(int), hipMemcpyDeviceToHost, st));
+    SBPX_HIP(hipMemcpyAsync(&h.max_cap, h.status + 5, sizeof(int), hipMemcpyDeviceToHost, st));
     SBPX_HIP(hipStreamSynchronize(st));
     const size_t used = (size_t)std::max(tot[0], tot[1]);
     h.headroom = std::max(h.headroom, std::max(used, (size_t)1 << 20));
@@ -469,6 +1041,311 @@ static void build_tables(Ctx& c) {
     SBPX_HIP(hipMemcpyAsync(h.status + 3, h.cbase + B, sizeof(int), hipMemcpyDeviceToDevice, st));
 }
 
+// ------------------------------------------------------------------------------------------
+// Heap-ordered merge selection (carry_out_best_merges_advanced, block_merge.cpp:123-217): a
+// min-heap of (delta MDL, block, proposed block); the top merge is re-scored on the current
+// matrix and goes back into the heap when it got worse than the next one, otherwise it is
+// applied at once. One wavefront runs the whole loop: lane 0 keeps the heap and the union-find
+// of merged blocks, the wavefront scores merges and folds row and column r into s in the hash
+// tables. The heap steps are those of libstdc++'s std::push_heap / std::pop_heap (used by the
+// reference through std::priority_queue), so merges with equal delta MDL leave the heap in the
+// same order.
+struct MergeEntry {
+    double dS;
+    int r;
+    int s;
+};
+
+// priority_queue's comparator: a greater delta MDL sorts first, so the top is the minimum
+__device__ __forceinline__ bool heap_before(const MergeEntry& a, const MergeEntry& b) { return a.dS > b.dS; }
+
+__device__ void heap_sift_up(MergeEntry* h, int hole, int top, MergeEntry value) {
+    int parent = (hole - 1) / 2;
+    while (hole > top && heap_before(h[parent], value)) {
+        h[hole] = h[parent];
+        hole = parent;
+        parent = (hole - 1) / 2;
+    }
+    h[hole] = value;
+}
+
+// Removes the top of a heap of len entries (len >= 1); the heap then holds len - 1.
+__device__ void heap_pop(MergeEntry* h, int len) {
+    if (len <= 1) return;
+    const int n = len - 1;  // the last entry is re-inserted from the hole left at the top
+    MergeEntry value = h[n];
+    h[n] = h[0];
+    int hole = 0, child = 0;
+    while (child < (n - 1) / 2) {
+        child = 2 * (child + 1);
+        if (heap_before(h[child], h[child - 1])) child--;
+        h[hole] = h[child];
+        hole = child;
+    }
+    if ((n & 1) == 0 && child == (n - 2) / 2) {
+        child = 2 * (child + 1);
+        h[hole] = h[child - 1];
+        hole = child - 1;
+    }
+    heap_sift_up(h, hole, 0, value);
+}
+
+__device__ int uf_find(int* parent, int x) {
+    int root = x;
+    while (parent[root] != root) root = parent[root];
+    while (parent[x] != root) {
+        const int nx = parent[x];
+        parent[x] = root;
+        x = nx;
+    }
+    return root;
+}
+
+// Room for folding row and column r into s: row and column s may gain every key of row and
+// column r; each block keyed in row r gains key s in its column, each block keyed in column r
+// gains key s in its row.
+__device__ bool merge_room(const MhParams& P, int r, int s) {
+    const int lane = lane_id();
+    int* rtop = P.status + 2;
+    int* ctop = P.status + 3;
+    const int big = P.rocc[r] + P.cocc[r] + 4;
+    if (room(P.rcap, P.rocc, s) < big && !relocate(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, rtop, P.pool, s, big))
+        return false;
+    if (room(P.ccap, P.cocc, s) < big && !relocate(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, ctop, P.pool, s, big))
+        return false;
+    for (int pass = 0; pass < 2; ++pass) {
+        const int b0 = pass == 0 ? P.rbase[r] : P.cbase[r];
+        const int n = pass == 0 ? P.rcap[r] : P.ccap[r];
+        const int* keys = pass == 0 ? P.rkeys : P.ckeys;
+        const int* vals = pass == 0 ? P.rvals : P.cvals;
+        for (int j0 = 0; j0 < n; j0 += WAVE) {
+            const int j = j0 + lane;
+            int x = -1;
+            if (j < n) {
+                const int kk = keys[b0 + j];
+                if (kk >= 0 && vals[b0 + j] != 0 && kk != r && kk != s) x = kk;
+            }
+            // a key of row r needs room in its column; a key of column r in its row
+            unsigned long long need = pass == 0 ? __ballot(x >= 0 && room(P.ccap, P.cocc, x) < 2)
+                                                : __ballot(x >= 0 && room(P.rcap, P.rocc, x) < 2);
+            while (need) {
+                const int l = first_lane(need);
+                need &= need - 1;
+                const int y = __shfl(x, l, WAVE);
+                const bool ok = pass == 0 ? relocate(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, ctop, P.pool, y, 2)
+                                          : relocate(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, rtop, P.pool, y, 2);
+                if (!ok) return false;
+            }
+        }
+    }
+    return true;
+}
+
+// Folds row and column r into s (block_merge::blockmodel_delta, block_merge.cpp:13-37, applied as
+// DictTransposeMatrix::update_edge_counts): cell (r, c) moves to (s, c), with (r, r) and (r, s)
+// landing on (s, s); then cell (i, r) moves to (i, s). Degrees as block_merge.cpp:199-205.
+__device__ void merge_apply(const MhParams& P, int r, int s, int k_out, int k_in) {
+    const int lane = lane_id();
+    {
+        const int b0 = P.rbase[r], n = P.rcap[r];
+        for (int j = lane; j < n; j += WAVE) {
+            const int c = P.rkeys[b0 + j], m = P.rvals[b0 + j];
+            if (c < 0 || m == 0) continue;
+            cell_add(P, s, c == r ? s : c, m);
+            cell_add(P, r, c, -m);
+        }
+    }
+    fence_wave();
+    {
+        const int b0 = P.cbase[r], n = P.ccap[r];
+        for (int j = lane; j < n; j += WAVE) {
+            const int i = P.ckeys[b0 + j], m = P.cvals[b0 + j];
+            if (i < 0 || m == 0) continue;
+            cell_add(P, i, s, m);
+            cell_add(P, i, r, -m);
+        }
+    }
+    fence_wave();
+    if (lane == 0) {
+        P.dout[r] = 0;
+        P.dout[s] += k_out;
+        P.din[r] = 0;
+        P.din[s] += k_in;
+    }
+    fence_wave();
+}
+
+// status: [0] merges done, [1] 1 when the table pool ran out before a merge (its fields in
+// [6] = r, [7] = proposed block, and the heap top already removed), [2]/[3] pool tops,
+// [8] entries left in the heap; err gets DEV_ERR_NO_PROPOSAL for a block without a proposal.
+__global__ void __launch_bounds__(WAVE) merge_heap_kernel(MhParams P, MergeEntry* heap, int len, int* parent,
+                                                         int num_merged, int num_to_merge, int pending_r,
+                                                         int pending_s, double* merged_dS) {
+    const int lane = lane_id();
+    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, nullptr};
+    int stopped = 0;
+    double dsum = 0.0;
+    // a merge that was accepted when the pool ran out is applied first, without a new score
+    if (pending_r >= 0) {
+        const int r = pending_r;
+        int s = 0;
+        if (lane == 0) s = uf_find(parent, pending_s);
+        s = __shfl(s, 0, WAVE);
+        const int k_out = P.dout[r], k_in = P.din[r];
+        if (merge_room(P, r, s)) {
+            merge_apply(P, r, s, k_out, k_in);
+            if (lane == 0) parent[r] = s;
+            fence_wave();
+            num_merged++;
+        } else {
+            stopped = 1;
+        }
+    }
+    while (!stopped && num_merged < num_to_merge && len > 0) {
+        MergeEntry top{0.0, 0, 0};
+        if (lane == 0) {
+            top = heap[0];
+            heap_pop(heap, len);
+        }
+        len--;
+        top.dS = __shfl(top.dS, 0, WAVE);
+        top.r = __shfl(top.r, 0, WAVE);
+        top.s = __shfl(top.s, 0, WAVE);
+        fence_wave();
+        const int r = top.r;
+        if (top.s < 0) {
+            if (lane == 0) atomicOr(P.err, DEV_ERR_NO_PROPOSAL);
+            break;
+        }
+        int s = 0;
+        if (lane == 0) s = uf_find(parent, top.s);
+        s = __shfl(s, 0, WAVE);
+        if (r == s) continue;
+        const int k_out = P.dout[r], k_in = P.din[r];
+        const double actual = merge_delta_mdl_wave<HashView>(M, P.dout, P.din, r, s, k_out, k_in);
+        if (len > 0) {
+            double next = 0.0;
+            if (lane == 0) next = heap[0].dS;
+            next = __shfl(next, 0, WAVE);
+            if (actual > next) {  // worse than anticipated: back into the heap with its new score
+                if (lane == 0) {
+                    heap[len] = MergeEntry{actual, top.r, top.s};
+                    heap_sift_up(heap, len, 0, MergeEntry{actual, top.r, top.s});
+                }
+                len++;
+                fence_wave();
+                continue;
+            }
+        }
+        if (!merge_room(P, r, s)) {
+            if (lane == 0) {
+                P.status[6] = r;
+                P.status[7] = top.s;
+            }
+            stopped = 1;
+            break;
+        }
+        merge_apply(P, r, s, k_out, k_in);
+        if (lane == 0) parent[r] = s;
+        fence_wave();
+        dsum += actual;
+        num_merged++;
+    }
+    if (lane == 0) {
+        P.status[0] = num_merged;
+        P.status[1] = stopped;
+        P.status[8] = len;
+        merged_dS[0] += dsum;
+    }
+}
+
+void merge_heap_select(Ctx& c, int num_to_merge, const double* dS, const int32_t* best, std::vector<int>& parent) {
+    const int B = c.bm.B;
+    MhState& h = c.mh;
+    // the initial heap, built by successive pushes as std::priority_queue does
+    std::vector<MergeEntry> heap;
+    heap.reserve(B);
+    auto before = [](const MergeEntry& a, const MergeEntry& b) { return a.dS > b.dS; };
+    for (int i = 0; i < B; ++i) {
+        heap.push_back(MergeEntry{dS[i], i, best[i]});
+        std::push_heap(heap.begin(), heap.end(), before);
+    }
+    MergeEntry* d_heap = nullptr;
+    int* d_parent = nullptr;
+    SBPX_HIP(hipMalloc((void**)&d_heap, sizeof(MergeEntry) * std::max(B, 1)));
+    SBPX_HIP(hipMalloc((void**)&d_parent, sizeof(int) * std::max(B, 1)));
+    std::vector<int> ident(B);
+    std::iota(ident.begin(), ident.end(), 0);
+    try {
+        SBPX_HIP(hipMemcpyAsync(d_heap, heap.data(), sizeof(MergeEntry) * B, hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemcpyAsync(d_parent, ident.data(), sizeof(int) * B, hipMemcpyHostToDevice, c.stream));
+        int len = B, merged = 0, pending_r = -1, pending_s = -1, rounds = 0;
+        double* merged_dS = c.red + 2056;
+        SBPX_HIP(hipMemsetAsync(merged_dS, 0, sizeof(double), c.stream));
+        for (;;) {
+            if (rounds++ > 24) throw StateError("merge selection keeps running out of table room");
+            build_tables(c);
+            MhParams P{};
+            P.assign = c.bm.assign;
+            P.dout = c.bm.dout;
+            P.din = c.bm.din;
+            P.deg = c.bm.deg;
+            P.drift = c.bm.drift;
+            P.B = B;
+            P.rbase = h.rbase;
+            P.rcap = h.rcap;
+            P.rkeys = h.rkeys;
+            P.rvals = h.rvals;
+            P.rocc = h.rocc;
+            P.cbase = h.cbase;
+            P.ccap = h.ccap;
+            P.ckeys = h.ckeys;
+            P.cvals = h.cvals;
+            P.cocc = h.cocc;
+            P.pool = (int)h.slots;
+            P.status = h.status;
+            P.err = c.dev_err;
+            merge_heap_kernel<<<1, WAVE, 0, c.stream>>>(P, d_heap, len, d_parent, merged, num_to_merge, pending_r,
+                                                       pending_s, merged_dS);
+            SBPX_HIP(hipGetLastError());
+            int stat[9];
+            SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
+            SBPX_HIP(hipStreamSynchronize(c.stream));
+            check_device_error(c);
+            merged = stat[0];
+            len = stat[8];
+            if (stat[1] == 0) break;
+            // The pool ran out: bring the matrix snapshot up to date with the merges so far
+            // (assignment through the union-find; merged-away blocks become empty), rebuild the
+            // tables with more headroom and resume with the accepted merge.
+            pending_r = stat[6];
+            pending_s = stat[7];
+            parent.resize(B);
+            SBPX_HIP(hipMemcpyAsync(parent.data(), d_parent, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+            SBPX_HIP(hipStreamSynchronize(c.stream));
+            std::vector<int> map(B);
+            for (int x = 0; x < B; ++x) {
+                int y = x;
+                while (parent[y] != y) y = parent[y];
+                map[x] = y;
+            }
+            SBPX_HIP(hipMemcpyAsync(c.m_map, map.data(), sizeof(int) * B, hipMemcpyHostToDevice, c.stream));
+            launch_relabel(c);
+            build_blockmodel(c);
+            h.headroom *= 2;
+        }
+        parent.resize(B);
+        SBPX_HIP(hipMemcpyAsync(parent.data(), d_parent, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipStreamSynchronize(c.stream));
+    } catch (...) {
+        (void)hipFree(d_heap);
+        (void)hipFree(d_parent);
+        throw;
+    }
+    SBPX_HIP(hipFree(d_heap));
+    SBPX_HIP(hipFree(d_parent));
+}
+
 void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,
                    long long* bytes, bool timed) {
     if (n <= 0) return;
@@ -485,6 +1362,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     }
     int pos = 0;
     int rounds = 0;
+    double* scan = c.red + 2052;  // [2] scanned slots and evaluations of the current chunk
     while (pos < n) {
         // each restart doubles the pool headroom, so the number of restarts is logarithmic
         if (rounds++ > 24) throw StateError("Metropolis-Hastings pass keeps running out of table room");
@@ -510,7 +1388,6 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.pool = (int)h.slots;
         P.list = d_list;
         P.n = n;
-        P.start = pos;
         P.seed = seed;
         P.call = call;
         P.tag = tag;
@@ -520,25 +1397,59 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.counters = c.counters;
         P.bytes = bytes;
         P.touch = h.touch;
+        P.scan = scan;
         P.dS_sum = c.red + 2048;
         P.err = c.dev_err;
-        if (timed) c.timer_begin();
-        mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
-        SBPX_HIP(hipGetLastError());
-        if (timed) c.timer_end(KERNEL_MH);
-        int stat[5];
-        SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
-        SBPX_HIP(hipStreamSynchronize(c.stream));
-        check_device_error(c);
-        pos = stat[0];
-        if (stat[4] > (1 << 30)) {  // restart the commit serials well before they wrap
-            SBPX_HIP(hipMemsetAsync(h.touch, 0, sizeof(int) * c.g.N, c.stream));
-            SBPX_HIP(hipMemsetAsync(h.status + 4, 0, sizeof(int), c.stream));
+        bool out_of_room = false;
+        // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
+        // mean number of row/column slots the previous chunk's evaluations scanned (long scans
+        // gain from more lanes per vertex, short ones from more vertices in flight). The first
+        // chunk at a new number of blocks goes by the longest row instead.
+        while (pos < n) {
+            int G = c.mh_group;
+            if (G == 0) {
+                if (h.last_B != B || h.mean_scan < 0.0)
+                    G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : 1);
+                else
+                    G = h.mean_scan >= 4000.0 ? 16 : (h.mean_scan >= 1500.0 ? 4 : 1);
+            }
+            P.start = pos;
+            P.stop_at = std::min(n, pos + MH_CHUNK);
+            SBPX_HIP(hipMemsetAsync(scan, 0, 2 * sizeof(double), c.stream));
+            if (timed) c.timer_begin();
+            switch (G) {
+                case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                case 2: mh_group_kernel<2><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                case 4: mh_group_kernel<4><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                case 8: mh_group_kernel<8><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                default: mh_group_kernel<16><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+            }
+            SBPX_HIP(hipGetLastError());
+            if (timed) c.timer_end(KERNEL_MH);
+            int stat[5];
+            double sc[2];
+            SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
+            SBPX_HIP(hipMemcpyAsync(sc, scan, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
+            SBPX_HIP(hipStreamSynchronize(c.stream));
+            check_device_error(c);
+            pos = stat[0];
+            if (sc[1] > 0.0) {
+                h.mean_scan = sc[0] / sc[1];
+                h.last_B = B;
+            }
+            if (stat[4] > (1 << 30)) {  // restart the commit serials well before they wrap
+                SBPX_HIP(hipMemsetAsync(h.touch, 0, sizeof(int) * c.g.N, c.stream));
+                SBPX_HIP(hipMemsetAsync(h.status + 4, 0, sizeof(int), c.stream));
+            }
+            if (stat[1] == 1) {
+                out_of_room = true;
+                break;
+            }
+            if (pos < P.stop_at) throw StateError("Metropolis-Hastings pass stopped early");
         }
         // the matrix changed: bring the CSR snapshot up to date (dout/din exact, deg keeps drift)
         rebuild_keep_drift(c);
-        if (stat[1] == 0 && pos < n) throw StateError("Metropolis-Hastings pass stopped early");
-        if (stat[1] == 1) h.headroom *= 2;
+        if (out_of_room) h.headroom *= 2;
     }
 }
 

@@ -59,7 +59,9 @@ typedef struct sbpx_sweep_stats {
 sbpx_status sbpx_create(sbpx_ctx** ctx, int device);
 void sbpx_destroy(sbpx_ctx* ctx);
 const char* sbpx_last_error(const sbpx_ctx* ctx);
-/* option 1 = count algorithmic bytes in the evaluation kernels (0/1). */
+/* option 1 = count algorithmic bytes in the evaluation kernels (0/1).
+ * option 2 = wavefronts that share one vertex's evaluation in the sequential Metropolis-Hastings
+ *            kernel: 1, 2, 4, 8 or 16, or 0 to choose by the longest block-matrix row (default). */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

@@ -232,7 +232,11 @@ def test_merge_proposals_match_oracle():
         assert np.allclose(dS, odS, rtol=DS_RTOL, atol=1e-9)
 
 
-def test_merge_blocks_matches_oracle():
+@pytest.mark.parametrize("approximate", [True, False])
+def test_merge_blocks_matches_oracle(approximate):
+    """Merge phases from the identity partition: proposals, then the sorted greedy selection
+    (--approximate) or the heap-ordered selection with re-scoring on the current matrix."""
+    m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=2)
     c = ctx_for(arrays)
@@ -241,13 +245,37 @@ def test_merge_blocks_matches_oracle():
     c.build(n, a)
     bm = O.Blockmodel(g, n, a)
     B = n
+    mode = m.MERGE_APPROXIMATE if approximate else m.MERGE_ADVANCED
     for call in range(4):
         k = B // 2
-        c.merge_blocks(k, 5, call)
+        c.merge_blocks(k, 5, call, mode=mode)
         bm.num_to_merge = k
-        bm.merge_blocks(True, 5, call)
+        bm.merge_blocks(approximate, 5, call)
         B -= k
         assert c.info()[0] == bm.B == B
+        assert_same_state(c, bm)
+
+
+def test_heap_merge_selection_with_tied_scores():
+    """Heap-ordered selection from fixed scores with many exact ties (scores rounded to whole
+    numbers): the heap must pop tied entries in the order std::priority_queue does."""
+    m = sbpx()
+    n = 1500
+    arrays, truth = gen_arrays(n, 10, seed=8)
+    g = O.Graph(n, *arrays)
+    rng = np.random.default_rng(9)
+    for B in (n, 300):
+        a = np.arange(n, dtype=np.int32) if B == n else rng.integers(0, B, n).astype(np.int32)
+        c = ctx_for(arrays)
+        c.build(B, a)
+        bm = O.Blockmodel(g, B, a)
+        dS, best = bm.merge_proposals(3, 1)
+        dS = np.round(dS)
+        k = B // 3
+        c.carry_out_merges(k, dS, best, mode=m.MERGE_ADVANCED)
+        bm.num_to_merge = k
+        bm.carry_out(dS, best, advanced=True)
+        assert c.info()[0] == bm.B
         assert_same_state(c, bm)
 
 
@@ -265,12 +293,16 @@ def test_sbp_async_gibbs_end_to_end_matches_oracle():
 
 
 # ----------------------------------------------------------------------------- sequential MH
-@pytest.mark.parametrize("algo", ["hybrid", "mh"])
-def test_sequential_sweeps_match_oracle(algo):
+@pytest.mark.parametrize("algo,group", [("hybrid", 0), ("mh", 0), ("mh", 1), ("mh", 2), ("hybrid", 4), ("mh", 8),
+                                        ("mh", 16)])
+def test_sequential_sweeps_match_oracle(algo, group):
+    """Sequential Metropolis-Hastings (alone, or on the high-degree vertices of the hybrid) equals
+    the oracle's one-vertex-at-a-time pass for every number of wavefronts per vertex."""
     m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=6)
     c = ctx_for(arrays)
+    c.set_option(2, group)
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(7)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
@@ -296,13 +328,14 @@ def test_sequential_sweeps_match_oracle(algo):
             assert_same_state(c, bm)
 
 
-def test_sbp_hybrid_end_to_end_close_to_oracle():
+@pytest.mark.parametrize("approximate", [True, False])
+def test_sbp_hybrid_end_to_end_close_to_oracle(approximate):
     m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=0)
     g = O.Graph(n, *arrays)
-    a, st = m.run_sbp(*arrays, algorithm="hybrid_mcmc", batches=1, approximate=True, seed=5)
-    oa, ost = O.sbp(g, 2, 1, True, 5)
+    a, st = m.run_sbp(*arrays, algorithm="hybrid_mcmc", batches=1, approximate=approximate, seed=5)
+    oa, ost = O.sbp(g, 2, 1, approximate, 5)
     assert st.num_blocks == ost.B
     assert st.mdl == pytest.approx(ost.mdl, rel=5e-3)
     print("device", st.num_blocks, st.mdl, "oracle", ost.B, ost.mdl, "identical:", bool((a == oa).all()))

@@ -44,6 +44,19 @@ KERNELS = ["eval_small_kernel (batched vertex-move evaluation, degree <= 127)",
            "mh_kernel (sequential Metropolis-Hastings evaluation and commit)"]
 
 
+def measured_traffic():
+    """HBM bytes per launch per kernel, from the PMC summary kept in profiles/ (FETCH_SIZE and
+    WRITE_SIZE runs scaled by a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    out = {}
+    if os.path.exists(path):
+        for line in open(path):
+            parts = line.split()
+            if len(parts) == 2 and not line.startswith("#"):
+                out[parts[0]] = float(parts[1])
+    return out
+
+
 def cpu_baseline(arrays, seconds, seed, algorithm):
     """The CPU oracle (C++/OpenMP restatement of the reference) on the host cores: one merge phase
     from the identity partition, then sweeps of the same algorithm (batches=1) for about
@@ -94,7 +107,21 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true", help="skip the untimed byte-counting run")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device 10 times (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 4 * n, "launches": 10}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described, not built",
                           "n_gpus": args.gpus}))
@@ -129,14 +156,19 @@ def main():
               file=sys.stderr, flush=True)
     total = time.perf_counter() - t_all
     # untimed counting run of the same seed: algorithmic bytes of the evaluation kernels
-    _, cst = g.partition(count_bytes=True, **kw)
-    kernel_bytes = np.array(cst.eval_bytes[:4])
+    if args.no_count:
+        kernel_bytes = np.zeros(4)
+    else:
+        _, cst = g.partition(count_bytes=True, **kw)
+        kernel_bytes = np.array(cst.eval_bytes[:4])
     ms_per_run = kernel_ms / args.steps
     launches_per_run = kernel_launches / args.steps
     gbs = np.where(ms_per_run > 0, kernel_bytes / np.maximum(ms_per_run, 1e-12) / 1e6, 0.0)
     dom = int(np.argmax(ms_per_run))
     achieved = float(gbs[dom])
     a, st = last
+    traffic = measured_traffic()
+    dom_name = KERNELS[dom].split(" ")[0]
     import importlib.util
 
     spec = importlib.util.spec_from_file_location(
@@ -178,7 +210,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic.get(dom_name),
+            "traffic_source": "profiles/hbm_traffic.txt (rocprofv3 PMC, same workload)" if dom_name in traffic else None,
             "bytes_per_run": float(kernel_bytes[dom]),
             "ms_per_run": float(ms_per_run[dom]),
             "launches_per_run": float(launches_per_run[dom]),
@@ -187,7 +220,8 @@ def main():
         "eval_kernels": [
             {"kernel": KERNELS[i].split(" ")[0], "ms_per_run": float(ms_per_run[i]),
              "launches_per_run": float(launches_per_run[i]), "bytes_per_run": float(kernel_bytes[i]),
-             "achieved_GBs": float(gbs[i])}
+             "achieved_GBs": float(gbs[i]), "frac": float(gbs[i]) / HBM_PEAK_GBS,
+             "traffic": traffic.get(KERNELS[i].split(" ")[0])}
             for i in range(4) if launches_per_run[i] > 0],
     }
     if not args.no_cpu:

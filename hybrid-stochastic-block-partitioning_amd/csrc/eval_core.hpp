@@ -262,7 +262,8 @@ __device__ __forceinline__ float fi(int x) { return (float)x; }
 // gathers the drawn neighbour slot records its block during step 1; the four scans of step 3
 // run as one loop over the concatenated ranges, two slots per lane per trip, with the keys,
 // then the degrees, loaded for both slots before either is used.
-template <class View>
+// U: slots per lane per trip of the row scans and the Hastings pass (loads in flight per lane).
+template <class View, int U = 2>
 __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockState& S, const View& M, int v,
                                                 int* tab, int cap, uint64_t seed, uint32_t call, uint32_t tag,
                                                 int given_s, double given_u, bool propose_only) {
@@ -392,7 +393,6 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;  // column r
     const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
     long long live = 0;
-    constexpr int U = 2;
     for (int base = 0; base < T; base += U * WAVE) {
         int key[U], val[U], seg[U];
 #pragma unroll

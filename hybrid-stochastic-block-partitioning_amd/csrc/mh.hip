@@ -25,6 +25,9 @@ namespace sbpx {
 
 constexpr int MH_WAVES = 16;
 constexpr int MH_CHUNK = 1024;  // queue entries per launch
+#ifndef MH_UNROLL
+#define MH_UNROLL 2  // slots per lane per trip of the row scans (4 spills at 128 VGPRs)
+#endif
 
 // ------------------------------------------------------------------------------------------
 // Table build: cap(x) = next power of two >= 2 nnz(x) + 32, filled from the CSR.
@@ -348,7 +351,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
             const int v = P.list[idx];
             cap = vertex_table_cap(G, v);
             if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
-            R = eval_move<HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+            R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
         }
         if (lane == 0) {
             st[wave] = active ? R.status : 3;

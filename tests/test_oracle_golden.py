@@ -358,6 +358,60 @@ def test_carry_out_best_merges_hand_case():
     assert list(aa) == [0, 0, 0, 1, 2, 0, 0, 0]
 
 
+def _replay_heap_merges(g, B, assign, dS, best, k):
+    """block_merge.cpp:123-217 replayed in plain Python: the minimum-score entry is found by a
+    scan (scores are made distinct, so any priority queue pops the same order), each merge is
+    re-scored on a blockmodel rebuilt from the current assignment, block_map is rewritten as the
+    reference does, and labels are compacted in ascending order at the end."""
+    assign = np.array(assign, np.int32)
+    block_map = list(range(B))
+    queue = [(float(dS[i]), i, int(best[i])) for i in range(B)]
+    merged = 0
+    rescored = 0
+    while merged < k and queue:
+        i = min(range(len(queue)), key=lambda j: queue[j][0])
+        _, r, b = queue.pop(i)
+        to = block_map[b]
+        if r == to:
+            continue
+        actual = O.Blockmodel(g, B, assign).merge_delta_mdl(r, to)
+        if queue and actual > min(q[0] for q in queue):
+            queue.append((actual, r, b))
+            rescored += 1
+            continue
+        block_map = [to if x == r else x for x in block_map]
+        assign[assign == r] = to
+        merged += 1
+    labels = np.unique(assign)
+    return np.searchsorted(labels, assign).astype(np.int32), merged, rescored
+
+
+def test_carry_out_best_merges_advanced_replays_the_heap_loop():
+    rng = np.random.default_rng(12)
+    total_rescored = 0
+    for trial in range(6):
+        n = 240
+        edges = [(int(a), int(b)) for a, b in rng.integers(0, n, size=(1500, 2))]
+        g = O.Graph.from_edges(n, edges)
+        B = 48
+        a = rng.integers(0, B, n).astype(np.int32)
+        bm = O.Blockmodel(g, B, a)
+        dS, best = bm.merge_proposals(7, trial)
+        dS = dS + rng.random(B) * 1e-3  # distinct scores
+        k = B // 3
+        want, merged, rescored = _replay_heap_merges(g, B, a, dS, best, k)
+        total_rescored += rescored
+        assert merged == k
+        bm.num_to_merge = k
+        bm.carry_out(dS, best, advanced=True)
+        got, dout, din, _ = bm.state()
+        assert bm.B == B - k
+        assert (got == want).all()
+        fresh = O.Blockmodel(g, B - k, want)
+        assert (bm.dense() == fresh.dense()).all()
+    assert total_rescored > 0  # the re-push branch was exercised
+
+
 # ----------------------------------------------------------------------------- proposals
 def test_vertex_proposal_distribution(toy):
     """s ~ sum over neighbour slots (1/k) (M[t,s] + M[s,t]) / (dout[t] + din[t])."""

@@ -115,9 +115,9 @@ void free_graph_buffers(Ctx& c) {
 }
 
 int bin_of(int degree) {
-    if (degree <= BIN_SMALL_MAXDEG) return 0;
-    if (degree <= BIN_LARGE_MAXDEG) return 1;
-    return 2;
+    for (int b = 0; b < NUM_LDS_BINS; ++b)
+        if (degree <= BIN_MAXDEG[b]) return b;
+    return NUM_LDS_BINS;
 }
 int table_cap(int degree) {
     int cap = 16;
@@ -147,7 +147,7 @@ void bin_vertices(const Ctx& c, const int* vertices, int n, std::vector<int>& ou
             out.push_back(v);
             if (perm) perm->push_back(i);
             counts[b]++;
-            if (b == 2) huge_off.push_back(huge_off.back() + 7 * table_cap(c.g.degree[v]));
+            if (b == NUM_LDS_BINS) huge_off.push_back(huge_off.back() + 7 * table_cap(c.g.degree[v] + 1));
         }
     }
 }
@@ -179,14 +179,15 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
         L.list = d_list + bl.offset[b];
         L.n = bl.count[b];
         L.bin = b;
+        L.variant = c.pick_variant(b);
         L.seed = seed;
         L.call = call;
         L.tag = tag;
         L.out_by_vertex = 1;
         L.huge_offsets = d_huge_off;
-        L.bytes = bytes ? bytes + b : nullptr;
+        L.bytes = bytes ? bytes + bin_class(b) : nullptr;
         launch_eval(c, L);
-        if (timed) c.timer_end(b);
+        if (timed) c.timer_end(bin_class(b), b, L.variant, L.n);
     }
 }
 
@@ -221,10 +222,11 @@ void build_plan(Ctx& c, int algo, int batches) {
             BinnedList bl;
             std::vector<int> ordered, hoff;
             bin_vertices(c, base.data() + start, end - start, ordered, bl.count, hoff);
-            int off = (int)lists.size();
-            bl.offset[0] = off;
-            bl.offset[1] = off + bl.count[0];
-            bl.offset[2] = off + bl.count[0] + bl.count[1];
+            int off = (int)lists.size();  // the bins follow one another
+            for (int k = 0; k < NUM_BINS; ++k) {
+                bl.offset[k] = off;
+                off += bl.count[k];
+            }
             bl.total = end - start;
             bl.huge_base = (int)huge_all.size();
             lists.insert(lists.end(), ordered.begin(), ordered.end());
@@ -292,6 +294,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
     return guarded(ctx, [&] {
         if (option == 1) {
             ctx->count_bytes = value != 0;
+        } else if (option == 3) {
+            if (value < 0 || value > 2) throw ArgError("evaluation kernel variant must be 0 (tuned), 1 or 2");
+            ctx->c.eval_variant = (int)value;
         } else if (option == 2) {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");
@@ -692,6 +697,7 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
         SBPX_HIP(hipMemsetAsync(c.counters, 0, sizeof(long long) * 8, c.stream));
         SBPX_HIP(hipMemsetAsync(c.red + 2048, 0, sizeof(double), c.stream));
         c.timed.clear();
+        c.timed_bins.clear();
         long long* bytes = ctx->count_bytes ? c.counters + 4 : nullptr;
         long long* mh_bytes = ctx->count_bytes ? c.counters + 4 + KERNEL_MH : nullptr;
         if (algo == SBPX_ALGO_METROPOLIS_HASTINGS)
@@ -711,12 +717,16 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
         SBPX_HIP(hipMemcpyAsync(&dsum, c.red + 2048, sizeof(double), hipMemcpyDeviceToHost, c.stream));
         c.sync();
         check_device_error(c);
+        std::vector<float> launch_ms(c.timed.size(), 0.0f);
+        for (size_t i = 0; i < c.timed.size(); ++i) {
+            SBPX_HIP(hipEventElapsedTime(&launch_ms[i], c.events[2 * i], c.events[2 * i + 1]));
+            const Ctx::TimedBin& tb = c.timed_bins[i];
+            c.record_variant(tb.bin, tb.variant, tb.n, launch_ms[i]);
+        }
         if (stats) {
             *stats = sbpx_sweep_stats{};
             for (size_t i = 0; i < c.timed.size(); ++i) {
-                float t = 0.0f;
-                SBPX_HIP(hipEventElapsedTime(&t, c.events[2 * i], c.events[2 * i + 1]));
-                stats->eval_ms[c.timed[i]] += t;
+                stats->eval_ms[c.timed[i]] += launch_ms[i];
                 stats->eval_launches[c.timed[i]] += 1;
             }
             stats->attempts = cnt[0];

@@ -42,9 +42,16 @@ enum : int {
 };
 
 // Degree bins for the vertex-move kernels: neighbour-block table capacity per bin.
-constexpr int BIN_SMALL_MAXDEG = 127;   // 256-slot LDS table, 4 wavefronts per workgroup
-constexpr int BIN_LARGE_MAXDEG = 2047;  // 4096-slot LDS table, 1 wavefront per workgroup
-constexpr int NUM_BINS = 3;             // small, large, huge (table in global memory)
+// Degree bins of the batched evaluation: bin b < NUM_BINS - 1 takes degrees up to
+// BIN_MAXDEG[b] with a BIN_CAP[b]-slot neighbour-block table per wavefront in LDS; the last bin
+// keeps each vertex's table in global memory. Smaller tables leave room for more wavefronts.
+constexpr int NUM_LDS_BINS = 5;
+constexpr int BIN_MAXDEG[NUM_LDS_BINS] = {30, 62, 126, 254, 510};
+constexpr int BIN_CAP[NUM_LDS_BINS] = {64, 128, 256, 512, 1024};
+constexpr int NUM_BINS = NUM_LDS_BINS + 1;
+// timing / byte classes reported per sweep: 0 = LDS bins with tables up to 256 slots,
+// 1 = LDS bins with 512-1024 slots, 2 = tables in global memory, 3 = sequential MH kernel
+inline int bin_class(int b) { return b < 3 ? 0 : (b < NUM_LDS_BINS ? 1 : 2); }
 constexpr int KERNEL_MH = 3;            // timing / byte class of the sequential Metropolis-Hastings kernel
 constexpr int NUM_KERNEL_CLASSES = 4;
 
@@ -75,8 +82,8 @@ struct DevBlockmodel {
 
 // A vertex list split by degree bin, for one batch.
 struct BinnedList {
-    int offset[NUM_BINS] = {0, 0, 0};  // into the device list buffer
-    int count[NUM_BINS] = {0, 0, 0};
+    int offset[NUM_BINS] = {};  // into the device list buffer
+    int count[NUM_BINS] = {};
     int total = 0;
     int huge_base = 0;  // index of this batch's first huge-table offset
 };
@@ -163,6 +170,10 @@ struct Ctx {
 
     // timing of the last sweep's eval kernels (ms) and algorithmic bytes
     std::vector<int> timed;  // kernel class of each timed launch in this sweep (KERNEL_* below)
+    struct TimedBin {
+        int bin, variant, n;  // batched-evaluation launch: degree bin, kernel variant, vertices
+    };
+    std::vector<TimedBin> timed_bins;  // parallel to `timed`; bin -1 for other launches
     std::vector<hipEvent_t> events;  // two per timed launch, reused across sweeps
     // Brackets one launch on the stream with a pair of events, recorded under kernel class k.
     void timer_begin() {
@@ -174,9 +185,52 @@ struct Ctx {
         }
         (void)hipEventRecord(events[timed.size() * 2], stream);
     }
-    void timer_end(int k) {
+    void timer_end(int k, int bin = -1, int variant = 0, int n = 0) {
         (void)hipEventRecord(events[timed.size() * 2 + 1], stream);
         timed.push_back(k);
+        timed_bins.push_back(TimedBin{bin, variant, n});
+    }
+    // Online choice of the kernel variant per degree bin: while a bin's variants have fewer than
+    // TUNE_TRIALS timed launches each they take turns; then the one with the lower time per
+    // vertex is kept. The choice is made again after every change of the number of blocks.
+    static constexpr int NUM_VARIANTS = 2;
+    static constexpr int TUNE_TRIALS = 2;
+    struct BinTuner {
+        double ms[NUM_VARIANTS] = {0, 0};
+        double vertices[NUM_VARIANTS] = {0, 0};
+        int trials[NUM_VARIANTS] = {0, 0};
+        int next = 0;
+    };
+    BinTuner tuner[8];
+    int tuner_B = -1;
+    int eval_variant = 0;  // 0 = tuned, 1 = default kernels, 2 = at least 8 wavefronts per SIMD
+    int pick_variant(int bin) {
+        if (bin >= 3) return 0;  // one variant only
+        if (eval_variant) return eval_variant - 1;
+        if (tuner_B != bm.B) {
+            for (auto& t : tuner) t = BinTuner();
+            tuner_B = bm.B;
+        }
+        BinTuner& t = tuner[bin];
+        for (int v = 0; v < NUM_VARIANTS; ++v)
+            if (t.trials[v] < TUNE_TRIALS) {
+                int w = (t.next + v) % NUM_VARIANTS;
+                if (t.trials[w] < TUNE_TRIALS) {
+                    t.next = (w + 1) % NUM_VARIANTS;
+                    return w;
+                }
+            }
+        int best = 0;
+        for (int v = 1; v < NUM_VARIANTS; ++v)
+            if (t.ms[v] * t.vertices[best] < t.ms[best] * t.vertices[v]) best = v;
+        return best;
+    }
+    void record_variant(int bin, int variant, int n, double ms) {
+        if (bin < 0 || bin >= 3 || n <= 0) return;
+        BinTuner& t = tuner[bin];
+        t.ms[variant] += ms;
+        t.vertices[variant] += n;
+        t.trials[variant] += 1;
     }
 
     void sync() { SBPX_HIP(hipStreamSynchronize(stream)); }
@@ -202,6 +256,7 @@ struct EvalLaunch {
     const int* list;   // vertex ids
     int n;
     int bin;
+    int variant;            // kernel variant of the bin (0 default, 1 at least 8 wavefronts per SIMD)
     const int* given_s;     // optional fixed proposals
     const double* given_u;  // optional fixed uniforms
     uint64_t seed;

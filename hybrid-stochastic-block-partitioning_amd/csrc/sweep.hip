@@ -64,31 +64,34 @@ __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int c
     }
 }
 
-constexpr int SMALL_CAP = 256;
-constexpr int SMALL_WAVES = 4;
-constexpr int LARGE_CAP = 4096;
-
-__global__ void __launch_bounds__(SMALL_WAVES * WAVE) eval_small_kernel(EvalParams P) {
-    __shared__ int tab[SMALL_WAVES][7 * SMALL_CAP];
+// One wavefront per vertex, its neighbour-block table in LDS (CAP slots); WAVES wavefronts per
+// workgroup share nothing but the launch. OCC > 0 asks the compiler for at least OCC
+// wavefronts per SIMD (registers capped accordingly, at the price of some spills).
+template <int CAP, int WAVES, int OCC>
+__global__ void __launch_bounds__(WAVES * WAVE, OCC) eval_bin_kernel(EvalParams P) {
+    __shared__ int tab[WAVES][7 * CAP];
     const int wave = threadIdx.x / WAVE;
-    const int idx = blockIdx.x * SMALL_WAVES + wave;
+    const int idx = blockIdx.x * WAVES + wave;
     if (idx >= P.n) return;
-    eval_vertex(P, tab[wave], SMALL_CAP, idx);
+    eval_vertex(P, tab[wave], CAP, idx);
 }
 
-__global__ void __launch_bounds__(WAVE) eval_large_kernel(EvalParams P) {
-    __shared__ int tab[7 * LARGE_CAP];
-    const int idx = blockIdx.x;
-    if (idx >= P.n) return;
-    eval_vertex(P, tab, LARGE_CAP, idx);
-}
-
-__global__ void __launch_bounds__(WAVE) eval_huge_kernel(EvalParams P) {
-    const int idx = blockIdx.x;
+// Vertices of the highest degrees: tables in global memory at per-vertex offsets.
+constexpr int GLOBAL_WAVES = 4;
+__global__ void __launch_bounds__(GLOBAL_WAVES * WAVE) eval_global_kernel(EvalParams P) {
+    const int idx = blockIdx.x * GLOBAL_WAVES + threadIdx.x / WAVE;
     if (idx >= P.n) return;
     const int base = P.pool_off[idx];
     const int cap = (P.pool_off[idx + 1] - base) / 7;
     eval_vertex(P, P.pool + base, cap, idx);
+}
+
+template <int CAP, int WAVES>
+static void launch_bin(const EvalParams& P, int n, int variant, hipStream_t st) {
+    if (variant == 1)
+        eval_bin_kernel<CAP, WAVES, 8><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
+    else
+        eval_bin_kernel<CAP, WAVES, 1><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
 }
 
 static EvalParams make_params(Ctx& c) {
@@ -122,12 +125,18 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     P.out_u = L.out_u;
     P.bytes = L.bytes;
     P.pool_off = L.huge_offsets;
-    if (L.bin == 0) {
-        eval_small_kernel<<<(L.n + SMALL_WAVES - 1) / SMALL_WAVES, SMALL_WAVES * WAVE, 0, c.stream>>>(P);
-    } else if (L.bin == 1) {
-        eval_large_kernel<<<L.n, WAVE, 0, c.stream>>>(P);
-    } else {
-        eval_huge_kernel<<<L.n, WAVE, 0, c.stream>>>(P);
+    static_assert(BIN_CAP[0] == 64 && BIN_CAP[1] == 128 && BIN_CAP[2] == 256 && BIN_CAP[3] == 512 &&
+                      BIN_CAP[4] == 1024,
+                  "bin table sizes");
+    switch (L.bin) {
+        case 0: launch_bin<64, 8>(P, L.n, L.variant, c.stream); break;
+        case 1: launch_bin<128, 4>(P, L.n, L.variant, c.stream); break;
+        case 2: launch_bin<256, 4>(P, L.n, L.variant, c.stream); break;
+        case 3: launch_bin<512, 2>(P, L.n, 0, c.stream); break;
+        case 4: launch_bin<1024, 1>(P, L.n, 0, c.stream); break;
+        default:
+            eval_global_kernel<<<(L.n + GLOBAL_WAVES - 1) / GLOBAL_WAVES, GLOBAL_WAVES * WAVE, 0, c.stream>>>(P);
+            break;
     }
     SBPX_HIP(hipGetLastError());
 }

@@ -38,10 +38,10 @@ def load_pkg():
     return mod
 
 
-KERNELS = ["eval_small_kernel (batched vertex-move evaluation, degree <= 127)",
-           "eval_large_kernel (batched vertex-move evaluation, degree <= 2047)",
-           "eval_huge_kernel (batched vertex-move evaluation, larger degree)",
-           "mh_kernel (sequential Metropolis-Hastings evaluation and commit)"]
+KERNELS = ["eval_bin_kernel<64..256> (batched vertex-move evaluation, degree <= 126, LDS tables)",
+           "eval_bin_kernel<512..1024> (batched vertex-move evaluation, degree 127-510, LDS tables)",
+           "eval_global_kernel (batched vertex-move evaluation, degree > 510, tables in HBM)",
+           "mh_kernel (sequential Metropolis-Hastings evaluation and commit, all group sizes)"]
 
 
 def measured_traffic():

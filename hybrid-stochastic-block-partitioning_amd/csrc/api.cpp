@@ -117,7 +117,7 @@ void free_graph_buffers(Ctx& c) {
 int bin_of(int degree) {
     for (int b = 0; b < NUM_LDS_BINS; ++b)
         if (degree <= BIN_MAXDEG[b]) return b;
-    return NUM_LDS_BINS;
+    return degree <= GROUP_MAXDEG ? NUM_LDS_BINS : NUM_LDS_BINS + 1;
 }
 int table_cap(int degree) {
     int cap = 16;
@@ -131,7 +131,7 @@ void ensure_pool(Ctx& c, size_t slots) {
     c.table_pool_slots = slots;
 }
 
-// Orders `vertices` into bins [small | large | huge]; returns counts and, for huge vertices,
+// Orders `vertices` by degree bin (ascending); returns counts and, for the last bin,
 // table offsets (7 * cap ints each) starting at 0.
 void bin_vertices(const Ctx& c, const int* vertices, int n, std::vector<int>& out, int counts[NUM_BINS],
                   std::vector<int>& huge_off, std::vector<int>* perm = nullptr) {
@@ -147,7 +147,7 @@ void bin_vertices(const Ctx& c, const int* vertices, int n, std::vector<int>& ou
             out.push_back(v);
             if (perm) perm->push_back(i);
             counts[b]++;
-            if (b == NUM_LDS_BINS) huge_off.push_back(huge_off.back() + 7 * table_cap(c.g.degree[v] + 1));
+            if (b == NUM_BINS - 1) huge_off.push_back(huge_off.back() + 7 * table_cap(c.g.degree[v] + 1));
         }
     }
 }

@@ -42,15 +42,18 @@ enum : int {
 };
 
 // Degree bins for the vertex-move kernels: neighbour-block table capacity per bin.
-// Degree bins of the batched evaluation: bin b < NUM_BINS - 1 takes degrees up to
-// BIN_MAXDEG[b] with a BIN_CAP[b]-slot neighbour-block table per wavefront in LDS; the last bin
-// keeps each vertex's table in global memory. Smaller tables leave room for more wavefronts.
+// Degree bins of the batched evaluation: bin b < NUM_LDS_BINS takes degrees up to
+// BIN_MAXDEG[b], one wavefront per vertex with a BIN_CAP[b]-slot neighbour-block table in LDS
+// (smaller tables leave room for more wavefronts); bin NUM_LDS_BINS (degree <= GROUP_MAXDEG)
+// gives each vertex 4 wavefronts and a 4096-slot LDS table; the last bin 16 wavefronts and a
+// table in global memory.
 constexpr int NUM_LDS_BINS = 5;
 constexpr int BIN_MAXDEG[NUM_LDS_BINS] = {30, 62, 126, 254, 510};
 constexpr int BIN_CAP[NUM_LDS_BINS] = {64, 128, 256, 512, 1024};
-constexpr int NUM_BINS = NUM_LDS_BINS + 1;
+constexpr int GROUP_MAXDEG = 2047;
+constexpr int NUM_BINS = NUM_LDS_BINS + 2;
 // timing / byte classes reported per sweep: 0 = LDS bins with tables up to 256 slots,
-// 1 = LDS bins with 512-1024 slots, 2 = tables in global memory, 3 = sequential MH kernel
+// 1 = LDS bins with 512-1024 slots, 2 = group bins (degree > 510), 3 = sequential MH kernel
 inline int bin_class(int b) { return b < 3 ? 0 : (b < NUM_LDS_BINS ? 1 : 2); }
 constexpr int KERNEL_MH = 3;            // timing / byte class of the sequential Metropolis-Hastings kernel
 constexpr int NUM_KERNEL_CLASSES = 4;
@@ -190,7 +193,9 @@ struct Ctx {
         timed.push_back(k);
         timed_bins.push_back(TimedBin{bin, variant, n});
     }
-    // Online choice of the kernel variant per degree bin: while a bin's variants have fewer than
+    // Online choice of the kernel variant per degree bin (bins 0-2: default registers or at least 8
+    // wavefronts per SIMD; bins 3-4: one wavefront per vertex or 4 sharing each vertex, which
+    // shortens the slowest vertex when the bin holds few): while a bin's variants have fewer than
     // TUNE_TRIALS timed launches each they take turns; then the one with the lower time per
     // vertex is kept. The choice is made again after every change of the number of blocks.
     static constexpr int NUM_VARIANTS = 2;
@@ -203,9 +208,9 @@ struct Ctx {
     };
     BinTuner tuner[8];
     int tuner_B = -1;
-    int eval_variant = 0;  // 0 = tuned, 1 = default kernels, 2 = at least 8 wavefronts per SIMD
+    int eval_variant = 0;  // 0 = tuned, 1 = default kernels, 2 = the alternative of every bin
     int pick_variant(int bin) {
-        if (bin >= 3) return 0;  // one variant only
+        if (bin >= NUM_LDS_BINS) return 0;  // one variant only
         if (eval_variant) return eval_variant - 1;
         if (tuner_B != bm.B) {
             for (auto& t : tuner) t = BinTuner();
@@ -226,7 +231,7 @@ struct Ctx {
         return best;
     }
     void record_variant(int bin, int variant, int n, double ms) {
-        if (bin < 0 || bin >= 3 || n <= 0) return;
+        if (bin < 0 || bin >= NUM_LDS_BINS || n <= 0) return;
         BinTuner& t = tuner[bin];
         t.ms[variant] += ms;
         t.vertices[variant] += n;
@@ -256,7 +261,8 @@ struct EvalLaunch {
     const int* list;   // vertex ids
     int n;
     int bin;
-    int variant;            // kernel variant of the bin (0 default, 1 at least 8 wavefronts per SIMD)
+    int variant;            // kernel variant of the bin: bins 0-2: 1 = at least 8 wavefronts per SIMD;
+                            // bins 3-4: 1 = 4 wavefronts share each vertex
     const int* given_s;     // optional fixed proposals
     const double* given_u;  // optional fixed uniforms
     uint64_t seed;

@@ -19,6 +19,7 @@
 
 #include "eval_core.hpp"
 #include "internal.hpp"
+#include "eval_group.hpp"
 #include "merge_core.hpp"
 
 namespace sbpx {
@@ -454,425 +455,6 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     }
 }
 
-// ------------------------------------------------------------------------------------------
-// Group variant: G wavefronts share one vertex's evaluation, so a vertex whose rows hold
-// thousands of slots is scanned by up to 1024 lanes, and 16 / G vertices are evaluated side by
-// side. The table lives in the group's LDS slice (or its global slice), partial results meet in
-// LDS, and every group runs the same sequence of workgroup barriers (idle groups included), so
-// the barriers never depend on the data. Results equal those of eval_move: the same cells are
-// visited and the same terms formed; only the order of the FP64 sums differs.
-constexpr int RED_D = 4;   // per-wavefront double partials: dS, pf, pb
-constexpr int RED_I = 12;  // per-wavefront int partials
-
-template <int G>
-__device__ MoveResult eval_group(const MhParams& P, const GraphView& Gr, const BlockState& S, const HashView& M,
-                                 int v, int* tab, int cap, int* hist, double (*redd)[RED_D], int (*redi)[RED_I]) {
-    constexpr int GS = G * WAVE;
-    const int lane = lane_id();
-    const int wave = threadIdx.x / WAVE;
-    const int wig = wave % G;
-    const int w0 = wave - wig;
-    const int gl = wig * WAVE + lane;
-    const bool act = v >= 0;
-    const int B = S.B;
-    MoveResult R{};
-    R.v = v;
-    R.t = -1;
-    R.status = 3;
-    const int r = act ? S.assign[v] : 0;
-    R.r = r;
-    int* tkey = tab;
-    int* to = tab + cap;
-    int* ti = tab + 2 * cap;
-    int* tmsb = tab + 3 * cap;
-    int* tmbs = tab + 4 * cap;
-    int* tmrb = tab + 5 * cap;
-    int* tmbr = tab + 6 * cap;
-    if (act) {
-        for (int j = gl; j < cap; j += GS) {
-            tkey[j] = -1;
-            to[j] = 0;
-            ti[j] = 0;
-            tmsb[j] = 0;
-            tmbs[j] = 0;
-            tmrb[j] = 0;
-            tmbr[j] = 0;
-        }
-    }
-    const int ob = act ? Gr.out_ptr[v] : 0, dO = act ? Gr.out_ptr[v + 1] - ob : 0;
-    const int ib = act ? Gr.in_ptr[v] : 0, dI = act ? Gr.in_ptr[v + 1] - ib : 0;
-    const int self = act ? Gr.self[v] : 0;
-    const int k_out = dO, k_in = dI - self, k = k_out + k_in;
-    R.k_out = k_out;
-    R.k_in = k_in;
-    R.self = self;
-    Draw4 dr{0u, 0u, 0u, 0u};
-    double u = 0.0;
-    int pick0 = -1, pick1 = -1;
-    if (act) {
-        dr = philox4x32_10(P.seed, (uint32_t)v, P.call, 0u, P.tag);
-        u = unit53(dr.x2, dr.x3);
-        if (k > 0) {
-            const int ps = (int)mulhi32(dr.x0, (uint32_t)k);
-            if (ps < k_out) {
-                pick0 = ps;
-            } else {
-                pick0 = dO + (ps - k_out);
-                pick1 = pick0 + 1;
-            }
-        }
-    }
-    __syncthreads();  // table cleared
-
-    // 1. neighbour blocks
-    int self_pos = INT_MAX, full = 0, cand0 = -1, cand1 = -1;
-    for (int j = gl; j < dO + dI; j += GS) {
-        const bool is_out = j < dO;
-        const int w = is_out ? Gr.out_idx[ob + j] : Gr.in_idx[ib + (j - dO)];
-        const int bw = S.assign[w];
-        if (j == pick0) cand0 = bw;
-        if (j == pick1) cand1 = bw;
-        if (w == v) {
-            if (!is_out) self_pos = j - dO;
-        } else {
-            int slot = tab_insert(tkey, cap, bw);
-            if (slot < 0) full = 1;
-            else atomicAdd(is_out ? &to[slot] : &ti[slot], 1);
-        }
-    }
-    if (act && self && gl == 0 && tab_insert(tkey, cap, r) < 0) full = 1;
-    self_pos = wave_min(self_pos);
-    cand0 = wave_max(cand0);
-    cand1 = wave_max(cand1);
-    full = wave_or(full);
-    if (lane == 0) {
-        redi[wave][0] = self_pos;
-        redi[wave][1] = cand0;
-        redi[wave][2] = cand1;
-        redi[wave][3] = full;
-    }
-    __syncthreads();  // table built, partials visible
-    for (int i = 0; i < G; ++i) {
-        self_pos = min(self_pos, redi[w0 + i][0]);
-        cand0 = max(cand0, redi[w0 + i][1]);
-        cand1 = max(cand1, redi[w0 + i][2]);
-        full |= redi[w0 + i][3];
-    }
-
-    // 2. proposal: s drawn from row t + column t by a group-wide radix descent over the block index
-    int s = r;
-    bool sample = false;
-    const int* skeys = nullptr;
-    const int* svals = nullptr;
-    int slo = 0, shi = 0, starget = 0;
-    if (act && !full) {
-        if (k == 0) {
-            s = random_block(r, B, dr.x0);
-        } else {
-            int t = cand0;
-            if (pick1 >= 0 && pick0 - dO >= self_pos) t = cand1;
-            R.t = t;
-            const int trow = S.dout[t];
-            const int total = trow + S.din[t];
-            const int target = (int)mulhi32(dr.x1, (uint32_t)total);
-            const bool row = target < trow;
-            skeys = row ? M.rkeys : M.ckeys;
-            svals = row ? M.rvals : M.cvals;
-            slo = row ? M.rbase[t] : M.cbase[t];
-            shi = slo + (row ? M.rcap[t] : M.ccap[t]);
-            starget = row ? target : target - trow;
-            sample = true;
-        }
-    }
-    {
-        int nbits = 1;
-        while (nbits < 31 && (1 << nbits) < B) ++nbits;
-        int shift = ((nbits + 7) / 8 - 1) * 8;
-        int prefix = 0;
-        long long below = 0;
-        for (;;) {
-            for (int j = gl; j < 256; j += GS) hist[j] = 0;
-            __syncthreads();
-            if (sample) {
-                const int up = shift + 8;
-                for (int j = slo + gl; j < shi; j += GS) {
-                    const int kk = skeys[j];
-                    if (kk < 0) continue;
-                    const int w = svals[j];
-                    if (w == 0) continue;
-                    const int hp = up >= 31 ? 0 : (kk >> up);
-                    if (hp == prefix) atomicAdd(&hist[(kk >> shift) & 255], w);
-                }
-            }
-            __syncthreads();
-            if (sample) {
-                long long h[4];
-                long long local = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    h[q] = hist[lane * 4 + q];
-                    local += h[q];
-                }
-                long long incl = local;
-#pragma unroll
-                for (int off = 1; off < WAVE; off <<= 1) {
-                    long long y = __shfl_up(incl, off, WAVE);
-                    if (lane >= off) incl += y;
-                }
-                long long run = below + (incl - local);
-                int digit = INT_MAX;
-                long long before = 0;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    if (digit == INT_MAX && run + h[q] > starget) {
-                        digit = lane * 4 + q;
-                        before = run;
-                    }
-                    run += h[q];
-                }
-                const int d = wave_min(digit);
-                below = __shfl(before, d == INT_MAX ? 0 : d / 4, WAVE);
-                prefix = (prefix << 8) | (d == INT_MAX ? 255 : d);
-            }
-            __syncthreads();  // histogram read before the next pass clears it
-            if (shift == 0) break;
-            shift -= 8;
-        }
-        if (sample) s = prefix < B ? prefix : B - 1;
-    }
-    R.s = s;
-    R.u = u;
-    R.H = 1.0;
-    R.dS = 0.0;
-    const bool ev = act && !full && s != r;
-
-    // 3. rows s, r and columns s, r
-    int o_r = 0, i_r = 0, o_s = 0, i_s = 0;
-    if (ev) {
-        const int slot_r = tab_find(tkey, cap, r), slot_s = tab_find(tkey, cap, s);
-        o_r = slot_r >= 0 ? to[slot_r] : 0;
-        i_r = slot_r >= 0 ? ti[slot_r] : 0;
-        o_s = slot_s >= 0 ? to[slot_s] : 0;
-        i_s = slot_s >= 0 ? ti[slot_s] : 0;
-    }
-    const int dout_r = ev ? S.dout[r] : 0, dout_s = ev ? S.dout[s] : 0;
-    const int din_r = ev ? S.din[r] : 0, din_s = ev ? S.din[s] : 0;
-    const int ndout_r = dout_r - k_out, ndout_s = dout_s + k_out;
-    const int ndin_r = din_r - (k_in + self), ndin_s = din_s + (k_in + self);
-    const bool t_ss = o_s > 0 || i_s > 0 || self, t_rr = o_r > 0 || i_r > 0 || self;
-    const bool t_sr = o_r > 0 || i_s > 0, t_rs = o_s > 0 || i_r > 0;
-    const int d_ss = o_s + i_s + self, d_rr = -(o_r + i_r + self);
-    const int d_sr = o_r - i_s, d_rs = i_r - o_s;
-    int m_ss = 0, m_rr = 0, m_sr = 0, m_rs = 0;
-    double acc = 0.0;
-    long long live = 0;
-    if (ev) {
-        const int b0 = M.rbeg(s), L0 = M.rend(s) - b0;
-        const int b1 = M.rbeg(r), L1 = M.rend(r) - b1;
-        const int b2 = M.cbeg(s), L2 = M.cend(s) - b2;
-        const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;
-        const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
-        for (int i = gl; i < T; i += GS) {
-            const int seg = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : 3;
-            int c, m;
-            if (seg == 0) {
-                c = M.rkeys[b0 + i];
-                m = M.rvals[b0 + i];
-            } else if (seg == 1) {
-                c = M.rkeys[b1 + (i - L0)];
-                m = M.rvals[b1 + (i - L0)];
-            } else if (seg == 2) {
-                c = M.ckeys[b2 + (i - T1)];
-                m = M.cvals[b2 + (i - T1)];
-            } else {
-                c = M.ckeys[b3 + (i - T2)];
-                m = M.cvals[b3 + (i - T2)];
-            }
-            if (c < 0 || m == 0) continue;
-            ++live;
-            bool unchanged = true;
-            if (seg == 0) {
-                if (c == s) {
-                    m_ss = m;
-                    unchanged = !t_ss;
-                } else if (c == r) {
-                    m_sr = m;
-                    unchanged = !t_sr || d_sr == 0;
-                } else {
-                    const int sl = tab_find(tkey, cap, c);
-                    if (sl >= 0) {
-                        tmsb[sl] = m;
-                        unchanged = to[sl] == 0;
-                    }
-                }
-            } else if (seg == 1) {
-                if (c == r) {
-                    m_rr = m;
-                    unchanged = !t_rr;
-                } else if (c == s) {
-                    m_rs = m;
-                    unchanged = !t_rs || d_rs == 0;
-                } else {
-                    const int sl = tab_find(tkey, cap, c);
-                    if (sl >= 0) {
-                        tmrb[sl] = m;
-                        unchanged = to[sl] == 0;
-                    }
-                }
-            } else {
-                if (c == r || c == s) continue;
-                const int sl = tab_find(tkey, cap, c);
-                if (sl >= 0) {
-                    (seg == 2 ? tmbs : tmbr)[sl] = m;
-                    unchanged = ti[sl] == 0;
-                }
-            }
-            if (!unchanged) continue;
-            const float mf = fi(m);
-            if (seg < 2) {
-                const int d = S.din[c];
-                const int xo = seg == 0 ? dout_s : dout_r, nxo = seg == 0 ? ndout_s : ndout_r;
-                const int nd = c == r ? ndin_r : (c == s ? ndin_s : d);
-                acc += (double)cell_entropy(mf, fi(d), fi(xo));
-                acc -= (double)cell_entropy(mf, fi(nd), fi(nxo));
-            } else {
-                const int d = S.dout[c];
-                const int xi = seg == 2 ? din_s : din_r, nxi = seg == 2 ? ndin_s : ndin_r;
-                acc += (double)cell_entropy(mf, fi(xi), fi(d));
-                acc -= (double)cell_entropy(mf, fi(nxi), fi(d));
-            }
-        }
-    }
-    m_ss = wave_max(m_ss);
-    m_rr = wave_max(m_rr);
-    m_sr = wave_max(m_sr);
-    m_rs = wave_max(m_rs);
-    live = wave_sum_ll(live);
-    if (lane == 0) {
-        redi[wave][4] = m_ss;
-        redi[wave][5] = m_rr;
-        redi[wave][6] = m_sr;
-        redi[wave][7] = m_rs;
-        redi[wave][8] = (int)live;
-    }
-    __syncthreads();  // cell records in the table and partials visible
-    live = 0;
-    for (int i = 0; i < G; ++i) {
-        m_ss = max(m_ss, redi[w0 + i][4]);
-        m_rr = max(m_rr, redi[w0 + i][5]);
-        m_sr = max(m_sr, redi[w0 + i][6]);
-        m_rs = max(m_rs, redi[w0 + i][7]);
-        live += redi[w0 + i][8];
-    }
-
-    // 4. Delta keys and Hastings sums
-    double pf = 0.0, pb = 0.0;
-    int nuniq = 0;
-    if (ev) {
-        for (int j = gl; j < cap; j += GS) {
-            const int b = tkey[j];
-            if (b < 0) continue;
-            nuniq++;
-            const int ob_ = to[j], ib_ = ti[j];
-            const int degb = S.deg[b];
-            int Msb, Mbs, Mrb, Mbr, Drb, Dbr, nd;
-            if (b == r) {
-                Msb = m_sr; Mbs = m_rs; Mrb = m_rr; Mbr = m_rr; Drb = d_rr; Dbr = d_rr;
-                nd = (dout_r - k_out) + (din_r - k_in) - (m_rr + d_rr);
-            } else if (b == s) {
-                Msb = m_ss; Mbs = m_ss; Mrb = m_rs; Mbr = m_sr; Drb = d_rs; Dbr = d_sr;
-                nd = (dout_s + k_out) + (din_s + k_in) - (m_ss + d_ss);
-            } else {
-                Msb = tmsb[j]; Mbs = tmbs[j]; Mrb = tmrb[j]; Mbr = tmbr[j]; Drb = -ob_; Dbr = -ib_;
-                nd = degb;
-                const int db = S.din[b], ub = S.dout[b];
-                if (ob_ > 0) {
-                    acc += (double)cell_entropy(fi(Msb), fi(db), fi(dout_s));
-                    acc -= (double)cell_entropy(fi(Msb + ob_), fi(db), fi(ndout_s));
-                    acc += (double)cell_entropy(fi(Mrb), fi(db), fi(dout_r));
-                    acc -= (double)cell_entropy(fi(Mrb - ob_), fi(db), fi(ndout_r));
-                }
-                if (ib_ > 0) {
-                    acc += (double)cell_entropy(fi(Mbs), fi(din_s), fi(ub));
-                    acc -= (double)cell_entropy(fi(Mbs + ib_), fi(ndin_s), fi(ub));
-                    acc += (double)cell_entropy(fi(Mbr), fi(din_r), fi(ub));
-                    acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
-                }
-            }
-            const int nb = ob_ + ib_ + (b == r ? self : 0);
-            if (nb == 0) continue;
-            const double count = (double)nb;
-            pf += count * ((double)(Msb + Mbs) + 1.0) / (double)(degb + B);
-            pb += count * ((double)(Mrb + Drb + Mbr + Dbr) + 1.0) / (double)(nd + B);
-        }
-        if (gl == 0) {
-            if (t_ss) {
-                acc += (double)cell_entropy(fi(m_ss), fi(din_s), fi(dout_s));
-                acc -= (double)cell_entropy(fi(m_ss + d_ss), fi(ndin_s), fi(ndout_s));
-            }
-            if (t_rr) {
-                acc += (double)cell_entropy(fi(m_rr), fi(din_r), fi(dout_r));
-                acc -= (double)cell_entropy(fi(m_rr + d_rr), fi(ndin_r), fi(ndout_r));
-            }
-            if (t_sr) {
-                acc += (double)cell_entropy(fi(m_sr), fi(din_r), fi(dout_s));
-                acc -= (double)cell_entropy(fi(m_sr + d_sr), fi(ndin_r), fi(ndout_s));
-            }
-            if (t_rs) {
-                acc += (double)cell_entropy(fi(m_rs), fi(din_s), fi(dout_r));
-                acc -= (double)cell_entropy(fi(m_rs + d_rs), fi(ndin_s), fi(ndout_r));
-            }
-        }
-    }
-    acc = wave_sum(acc);
-    pf = wave_sum(pf);
-    pb = wave_sum(pb);
-    nuniq = (int)wave_sum_ll(nuniq);
-    if (lane == 0) {
-        redd[wave][0] = acc;
-        redd[wave][1] = pf;
-        redd[wave][2] = pb;
-        redi[wave][9] = nuniq;
-    }
-    __syncthreads();
-    double dS = 0.0;
-    pf = 0.0;
-    pb = 0.0;
-    nuniq = 0;
-    for (int i = 0; i < G; ++i) {
-        dS += redd[w0 + i][0];
-        pf += redd[w0 + i][1];
-        pb += redd[w0 + i][2];
-        nuniq += redi[w0 + i][9];
-    }
-    R.nuniq = nuniq;
-    R.m_ss = m_ss;
-    R.m_rr = m_rr;
-    R.m_sr = m_sr;
-    R.m_rs = m_rs;
-    R.d_ss = d_ss;
-    R.d_rr = d_rr;
-    R.d_sr = d_sr;
-    R.d_rs = d_rs;
-    R.nnz_rs = live;
-    if (!act) return R;
-    if (full) {
-        R.status = -1;
-        return R;
-    }
-    if (!ev) {
-        R.status = 0;
-        return R;
-    }
-    const double H = (k == 0) ? 1.0 : pb / pf;
-    double a = exp(-3.0 * dS) * H;
-    a = (a >= 1.0) ? 1.0 : a;
-    R.dS = dS;
-    R.H = H;
-    R.status = (u <= a) ? 2 : 1;
-    return R;
-}
-
 template <int G>
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     constexpr int NG = MH_WAVES / G;
@@ -904,7 +486,8 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
         const int v = active ? P.list[idx] : -1;
         const int cap = active ? vertex_table_cap(Gr, v) : 16;
         int* tab = cap <= LCAP ? lds_tab[g] : gtab;
-        MoveResult R = eval_group<G>(P, Gr, S, M, v, tab, cap, lds_hist[g], redd, redi);
+        MoveResult R = eval_group<G, HashView>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false,
+                                                lds_hist[g], redd, redi);
         if (leader && lane == 0) {
             st[g] = active ? R.status : 3;
             holds[g] = 1;

@@ -5,6 +5,7 @@
 #include <climits>
 
 #include "eval_core.hpp"
+#include "eval_group.hpp"
 #include "internal.hpp"
 
 namespace sbpx {
@@ -34,12 +35,8 @@ struct EvalParams {
     const int* __restrict__ pool_off;
 };
 
-__device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
-    const int v = P.list[idx];
-    const int gs = P.given_s ? P.given_s[idx] : -1;
-    const double gu = P.given_u ? P.given_u[idx] : 0.0;
-    MoveResult R = eval_move<CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0);
-    if (lane_id() != 0) return;
+// Stores one evaluation's outputs (one lane).
+__device__ void write_result(const EvalParams& P, const MoveResult& R, int v, int idx) {
     if (R.status < 0) {
         atomicOr(P.err, DEV_ERR_TABLE);
         return;
@@ -64,6 +61,14 @@ __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int c
     }
 }
 
+__device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
+    const int v = P.list[idx];
+    const int gs = P.given_s ? P.given_s[idx] : -1;
+    const double gu = P.given_u ? P.given_u[idx] : 0.0;
+    MoveResult R = eval_move<CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0);
+    if (lane_id() == 0) write_result(P, R, v, idx);
+}
+
 // One wavefront per vertex, its neighbour-block table in LDS (CAP slots); WAVES wavefronts per
 // workgroup share nothing but the launch. OCC > 0 asks the compiler for at least OCC
 // wavefronts per SIMD (registers capped accordingly, at the price of some spills).
@@ -76,14 +81,29 @@ __global__ void __launch_bounds__(WAVES * WAVE, OCC) eval_bin_kernel(EvalParams 
     eval_vertex(P, tab[wave], CAP, idx);
 }
 
-// Vertices of the highest degrees: tables in global memory at per-vertex offsets.
-constexpr int GLOBAL_WAVES = 4;
-__global__ void __launch_bounds__(GLOBAL_WAVES * WAVE) eval_global_kernel(EvalParams P) {
-    const int idx = blockIdx.x * GLOBAL_WAVES + threadIdx.x / WAVE;
+// Vertices of the highest degrees: G wavefronts (the whole workgroup) share one vertex, its
+// table in LDS (CAP slots) or, with CAP = 0, in global memory at a per-vertex offset.
+template <int G, int CAP>
+__global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
+    __shared__ int tab_lds[CAP > 0 ? 7 * CAP : 1];
+    __shared__ int hist[256];
+    __shared__ double redd[G][RED_D];
+    __shared__ int redi[G][RED_I];
+    const int idx = blockIdx.x;
     if (idx >= P.n) return;
-    const int base = P.pool_off[idx];
-    const int cap = (P.pool_off[idx + 1] - base) / 7;
-    eval_vertex(P, P.pool + base, cap, idx);
+    const int v = P.list[idx];
+    int* tab = tab_lds;
+    int cap = CAP;
+    if constexpr (CAP == 0) {
+        const int base = P.pool_off[idx];
+        cap = (P.pool_off[idx + 1] - base) / 7;
+        tab = P.pool + base;
+    }
+    const int gs = P.given_s ? P.given_s[idx] : -1;
+    const double gu = P.given_u ? P.given_u[idx] : 0.0;
+    MoveResult R = eval_group<G, CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu,
+                                          P.propose_only != 0, hist, redd, redi);
+    if (threadIdx.x == 0) write_result(P, R, v, idx);
 }
 
 template <int CAP, int WAVES>
@@ -132,11 +152,16 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
         case 0: launch_bin<64, 8>(P, L.n, L.variant, c.stream); break;
         case 1: launch_bin<128, 4>(P, L.n, L.variant, c.stream); break;
         case 2: launch_bin<256, 4>(P, L.n, L.variant, c.stream); break;
-        case 3: launch_bin<512, 2>(P, L.n, 0, c.stream); break;
-        case 4: launch_bin<1024, 1>(P, L.n, 0, c.stream); break;
-        default:
-            eval_global_kernel<<<(L.n + GLOBAL_WAVES - 1) / GLOBAL_WAVES, GLOBAL_WAVES * WAVE, 0, c.stream>>>(P);
+        case 3:
+            if (L.variant == 1) eval_group_kernel<4, 512><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
+            else launch_bin<512, 2>(P, L.n, 0, c.stream);
             break;
+        case 4:
+            if (L.variant == 1) eval_group_kernel<4, 1024><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
+            else launch_bin<1024, 1>(P, L.n, 0, c.stream);
+            break;
+        case 5: eval_group_kernel<4, 4096><<<L.n, 4 * WAVE, 0, c.stream>>>(P); break;
+        default: eval_group_kernel<16, 0><<<L.n, 16 * WAVE, 0, c.stream>>>(P); break;
     }
     SBPX_HIP(hipGetLastError());
 }
@@ -237,12 +262,14 @@ __global__ void commit_moves_kernel(const int* __restrict__ list, int n, const u
     if (out_stat[v] == 2) assign[v] = new_block[v];
 }
 
-// Sums dS over the accepted moves of the batch (vertex-indexed outputs).
+// Sums dS over the accepted moves of the batch (vertex-indexed outputs): SUM_BLOCKS partial
+// sums over a grid-stride split, then one workgroup adds them in order (deterministic).
+constexpr int SUM_BLOCKS = 256;
 __global__ void sum_accepted_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
-                                    const double* __restrict__ out_dS, double* __restrict__ out) {
+                                    const double* __restrict__ out_dS, double* __restrict__ partial) {
     __shared__ double lds[APPLY_TB / WAVE];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int v = list[i];
         if (out_stat[v] == 2) acc += out_dS[v];
     }
@@ -252,8 +279,15 @@ __global__ void sum_accepted_kernel(const int* __restrict__ list, int n, const u
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < APPLY_TB / WAVE; ++w) s += lds[w];
-        out[0] += s;
+        partial[blockIdx.x] = s;
     }
+}
+
+__global__ void add_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += WAVE) acc += partial[i];
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) out[0] += acc;
 }
 
 // list: the batch's vertices (any order); outputs are vertex-indexed.
@@ -263,7 +297,9 @@ void apply_accepted(Ctx& c, const int* list, int n) {
     const int nb = (n + APPLY_TB - 1) / APPLY_TB;
     mark_moves_kernel<<<nb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_s, c.ev_acc, c.bm.assign, c.epoch, c.moved_epoch,
                                                      c.new_block, c.last_touch, c.counters);
-    sum_accepted_kernel<<<1, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.ev_dS, c.red + 2048);
+    const int sb = std::min(SUM_BLOCKS, nb);
+    sum_accepted_kernel<<<sb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.ev_dS, c.red + 2560);
+    add_partials_kernel<<<1, WAVE, 0, c.stream>>>(c.red + 2560, sb, c.red + 2048);
     const int wpb = APPLY_TB / WAVE;
     drift_kernel<<<(n + wpb - 1) / wpb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.g.out_ptr, c.g.out_idx, c.g.in_ptr,
                                                                 c.g.in_idx, c.bm.assign, c.epoch, c.moved_epoch,

@@ -62,10 +62,11 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
 /* option 1 = count algorithmic bytes in the evaluation kernels (0/1).
  * option 2 = wavefronts that share one vertex's evaluation in the sequential Metropolis-Hastings
  *            kernel: 1, 2, 4, 8 or 16, or 0 to choose by the longest block-matrix row (default).
- * option 3 = kernel variant of the batched evaluation for degrees <= 126: 1 default registers,
- *            2 at least 8 wavefronts per SIMD, or 0 to time both and keep the faster per degree
- *            bin after every change of the number of blocks (default). Results do not depend on
- *            options 2 and 3. */
+ * option 3 = kernel variants of the batched evaluation per degree bin: 1 the first (default
+ *            registers, one wavefront per vertex), 2 the second (at least 8 wavefronts per SIMD
+ *            for degrees <= 126; 4 wavefronts sharing a vertex for degrees 127-510), or 0 to time
+ *            both and keep the faster per bin after every change of the number of blocks
+ *            (default). Results do not depend on options 2 and 3. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

@@ -343,13 +343,14 @@ def test_sbp_hybrid_end_to_end_close_to_oracle(approximate):
 
 # ----------------------------------------------------------------------------- all degree bins
 def heavy_arrays(seed=0):
-    """Graph whose vertices fall in every degree bin: a few hubs with in-degree > 2047 and a
-    few hundred with degree in (127, 2047]."""
+    """Graph whose vertices fall in every degree bin: two hubs with in-degree > 2047, ten with
+    degree in (510, 2047], a few hundred in (126, 510], and a thousand low-degree vertices
+    (mostly <= 30) that the hubs and mids do not reach."""
     rng = np.random.default_rng(seed)
     n = 3000
     edges = set()
     for v in range(n):
-        for w in rng.integers(0, n, 6):
+        for w in rng.integers(0, n, 6 if v < 2000 else 2):
             if w != v:
                 edges.add((v, int(w)))
     for hub in (0, 1):
@@ -357,21 +358,31 @@ def heavy_arrays(seed=0):
             if v != hub:
                 edges.add((int(v), hub))
     for mid in range(2, 300):
-        for v in rng.permutation(n)[: int(rng.integers(130, 400))]:
+        for v in rng.permutation(2000)[: int(rng.integers(130, 400))]:
             if v != mid:
                 edges.add((mid, int(v)) if v % 2 else (int(v), mid))
+    for big in range(300, 310):
+        for v in rng.permutation(2000)[: int(rng.integers(600, 1500))]:
+            if v != big:
+                edges.add((big, int(v)) if v % 3 else (int(v), big))
     for v in (5, 17):
         edges.add((v, v))  # self-loops on binned vertices
     return O.csr_from_edges(n, sorted(edges))
 
 
-def test_every_degree_bin_matches_oracle():
+@pytest.mark.parametrize("variant", [1, 2, 0])
+def test_every_degree_bin_matches_oracle(variant):
+    """Every degree bin and kernel variant (one wavefront per vertex with 64..1024-slot LDS
+    tables, more wavefronts per SIMD, 4 or 16 wavefronts sharing a vertex, tables in global
+    memory) against the oracle, through proposals, fixed-proposal evaluation and sweeps."""
     m = sbpx()
     arrays = heavy_arrays()
     n = len(arrays[0]) - 1
     deg = np.diff(arrays[0]) + np.diff(arrays[2])
-    assert (deg > 2047).sum() >= 2 and ((deg > 127) & (deg <= 2047)).sum() >= 100
+    assert (deg > 2047).sum() >= 2 and ((deg > 510) & (deg <= 2047)).sum() >= 5
+    assert ((deg > 126) & (deg <= 510)).sum() >= 50 and (deg <= 30).sum() >= 100
     c = ctx_for(arrays)
+    c.set_option(3, variant)
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(8)
     for B, a in ((n, np.arange(n, dtype=np.int32)), (400, rng.integers(0, 400, n).astype(np.int32)),

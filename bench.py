@@ -40,7 +40,7 @@ def load_pkg():
 
 KERNELS = ["eval_bin_kernel<64..256> (batched vertex-move evaluation, degree <= 126, LDS tables)",
            "eval_bin_kernel<512..1024> (batched vertex-move evaluation, degree 127-510, LDS tables)",
-           "eval_global_kernel (batched vertex-move evaluation, degree > 510, tables in HBM)",
+           "eval_group_kernel (batched vertex-move evaluation, degree > 510, 4-16 wavefronts per vertex)",
            "mh_kernel (sequential Metropolis-Hastings evaluation and commit, all group sizes)"]
 
 

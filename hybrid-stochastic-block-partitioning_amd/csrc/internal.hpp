@@ -193,8 +193,8 @@ struct Ctx {
         timed.push_back(k);
         timed_bins.push_back(TimedBin{bin, variant, n});
     }
-    // Online choice of the kernel variant per degree bin (bins 0-2: default registers or at least 8
-    // wavefronts per SIMD; bins 3-4: one wavefront per vertex or 4 sharing each vertex, which
+    // Online choice of the kernel variant per degree bin (bins 0-2: two or four row slots per lane
+    // per trip of the scans; bins 3-4: one wavefront per vertex or 4 sharing each vertex, which
     // shortens the slowest vertex when the bin holds few): while a bin's variants have fewer than
     // TUNE_TRIALS timed launches each they take turns; then the one with the lower time per
     // vertex is kept. The choice is made again after every change of the number of blocks.
@@ -208,7 +208,7 @@ struct Ctx {
     };
     BinTuner tuner[8];
     int tuner_B = -1;
-    int eval_variant = 0;  // 0 = tuned, 1 = default kernels, 2 = the alternative of every bin
+    int eval_variant = 0;  // 0 = tuned, 1 = first variant of every bin, 2 = the second
     int pick_variant(int bin) {
         if (bin >= NUM_LDS_BINS) return 0;  // one variant only
         if (eval_variant) return eval_variant - 1;
@@ -261,7 +261,7 @@ struct EvalLaunch {
     const int* list;   // vertex ids
     int n;
     int bin;
-    int variant;            // kernel variant of the bin: bins 0-2: 1 = at least 8 wavefronts per SIMD;
+    int variant;            // kernel variant of the bin: bins 0-2: 1 = four row slots per lane per trip;
                             // bins 3-4: 1 = 4 wavefronts share each vertex
     const int* given_s;     // optional fixed proposals
     const double* given_u;  // optional fixed uniforms

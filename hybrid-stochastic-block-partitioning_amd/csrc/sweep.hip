@@ -61,24 +61,25 @@ __device__ void write_result(const EvalParams& P, const MoveResult& R, int v, in
     }
 }
 
+template <int U = 2>
 __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
     const int v = P.list[idx];
     const int gs = P.given_s ? P.given_s[idx] : -1;
     const double gu = P.given_u ? P.given_u[idx] : 0.0;
-    MoveResult R = eval_move<CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0);
+    MoveResult R =
+        eval_move<CsrView, U>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0);
     if (lane_id() == 0) write_result(P, R, v, idx);
 }
 
 // One wavefront per vertex, its neighbour-block table in LDS (CAP slots); WAVES wavefronts per
-// workgroup share nothing but the launch. OCC > 0 asks the compiler for at least OCC
-// wavefronts per SIMD (registers capped accordingly, at the price of some spills).
-template <int CAP, int WAVES, int OCC>
-__global__ void __launch_bounds__(WAVES * WAVE, OCC) eval_bin_kernel(EvalParams P) {
+// workgroup share nothing but the launch. U: row slots per lane per trip of the scans.
+template <int CAP, int WAVES, int U>
+__global__ void __launch_bounds__(WAVES * WAVE) eval_bin_kernel(EvalParams P) {
     __shared__ int tab[WAVES][7 * CAP];
     const int wave = threadIdx.x / WAVE;
     const int idx = blockIdx.x * WAVES + wave;
     if (idx >= P.n) return;
-    eval_vertex(P, tab[wave], CAP, idx);
+    eval_vertex<U>(P, tab[wave], CAP, idx);
 }
 
 // Vertices of the highest degrees: G wavefronts (the whole workgroup) share one vertex, its
@@ -109,9 +110,9 @@ __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
 template <int CAP, int WAVES>
 static void launch_bin(const EvalParams& P, int n, int variant, hipStream_t st) {
     if (variant == 1)
-        eval_bin_kernel<CAP, WAVES, 8><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
+        eval_bin_kernel<CAP, WAVES, 4><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
     else
-        eval_bin_kernel<CAP, WAVES, 1><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
+        eval_bin_kernel<CAP, WAVES, 2><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
 }
 
 static EvalParams make_params(Ctx& c) {

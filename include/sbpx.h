@@ -63,7 +63,7 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  * option 2 = wavefronts that share one vertex's evaluation in the sequential Metropolis-Hastings
  *            kernel: 1, 2, 4, 8 or 16, or 0 to choose by the longest block-matrix row (default).
  * option 3 = kernel variants of the batched evaluation per degree bin: 1 the first (default
- *            registers, one wavefront per vertex), 2 the second (at least 8 wavefronts per SIMD
+ *            unroll, one wavefront per vertex), 2 the second (four row slots per lane per trip
  *            for degrees <= 126; 4 wavefronts sharing a vertex for degrees 127-510), or 0 to time
  *            both and keep the faster per bin after every change of the number of blocks
  *            (default). Results do not depend on options 2 and 3. */

@@ -12,6 +12,8 @@ HOST_LIB_PATH = os.path.join(HERE, "libsbp_host.so")
 
 ALGO_MH, ALGO_ASYNC_GIBBS, ALGO_HYBRID = 0, 1, 2
 MERGE_APPROXIMATE, MERGE_ADVANCED = 0, 1
+LEARNED_PARAMS = 90656  # W1[256][64], b1[256], W2[256][256], b2[256], W3[32][256], b3[32]
+OPT_COUNT_BYTES, OPT_MH_GROUP, OPT_EVAL_VARIANT, OPT_PROPOSAL = 1, 2, 3, 4
 ALGORITHMS = {"metropolis_hastings": ALGO_MH, "async_gibbs": ALGO_ASYNC_GIBBS, "hybrid_mcmc": ALGO_HYBRID}
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -81,6 +83,15 @@ def load():
     L.sbpx_merge_blocks.argtypes = [vp, C.c_int32, C.c_int, C.c_uint64, C.c_uint32]
     L.sbpx_selftest_logf.argtypes = [vp, C.c_uint32, C.c_uint32, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]
     L.sbpx_generate_dcsbm.argtypes = [C.POINTER(DcsbmParams), C.POINTER(C.c_int64), vp, vp, vp, vp, vp]
+    f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.sbpx_learned_init.argtypes = [vp, C.c_uint64]
+    L.sbpx_learned_params.argtypes = [vp, vp, vp]
+    L.sbpx_learned_log.argtypes = [vp, C.c_int32]
+    L.sbpx_learned_log_count.argtypes = [vp, C.POINTER(C.c_int32)]
+    L.sbpx_learned_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]
+    L.sbpx_learned_forward.argtypes = [vp, C.c_int32, f32p, f32p]
+    L.sbpx_learned_gradients.argtypes = [vp, C.c_int32, f32p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                         i32p, f32p, f32p]
     _lib = L
     return L
 
@@ -100,6 +111,9 @@ def load_host():
     H.sbp_graph_new.restype = C.c_void_p
     H.sbp_graph_free.argtypes = [C.c_void_p]
     H.sbp_graph_free.restype = None
+    H.sbp_graph_option.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_char_p, C.c_int]
+    H.sbp_graph_learned.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS"),
+                                    C.c_char_p, C.c_int]
     H.sbp_partition.argtypes = [C.c_void_p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int, i32p,
                                 C.POINTER(RunStats), C.c_char_p, C.c_int]
     _host = H
@@ -110,7 +124,9 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_graph_split", "sbpx_blockmodel_build", "sbpx_blockmodel_info", "sbpx_blockmodel_download",
             "sbpx_blockmodel_coo", "sbpx_mdl", "sbpx_log_posterior", "sbpx_eval_moves", "sbpx_propose",
             "sbpx_apply_moves", "sbpx_sweep", "sbpx_merge_proposals", "sbpx_carry_out_merges",
-            "sbpx_merge_blocks", "sbpx_selftest_logf", "sbpx_generate_dcsbm"]
+            "sbpx_merge_blocks", "sbpx_selftest_logf", "sbpx_generate_dcsbm", "sbpx_learned_init",
+            "sbpx_learned_params", "sbpx_learned_log", "sbpx_learned_log_count", "sbpx_learned_train",
+            "sbpx_learned_forward", "sbpx_learned_gradients"]
 
 
 class SbpxError(RuntimeError):
@@ -262,6 +278,46 @@ class Context:
     def merge_blocks(self, num_to_merge, seed, call, mode=MERGE_APPROXIMATE):
         self._chk(self.L.sbpx_merge_blocks(self.h, int(num_to_merge), mode, seed, call))
 
+    # proposal network
+    def learned_init(self, seed=0):
+        self._chk(self.L.sbpx_learned_init(self.h, seed))
+
+    def learned_params(self, new=None):
+        """Returns the flat parameters; with `new`, installs them afterwards."""
+        out = np.zeros(LEARNED_PARAMS, np.float32)
+        src = None if new is None else np.ascontiguousarray(new, np.float32)
+        if src is not None and src.size != LEARNED_PARAMS:
+            raise ValueError("expected %d parameters" % LEARNED_PARAMS)
+        self._chk(self.L.sbpx_learned_params(self.h, out.ctypes.data, None if src is None else src.ctypes.data))
+        return out
+
+    def learned_log(self, capacity):
+        self._chk(self.L.sbpx_learned_log(self.h, int(capacity)))
+
+    def learned_log_count(self):
+        n = C.c_int32()
+        self._chk(self.L.sbpx_learned_log_count(self.h, C.byref(n)))
+        return n.value
+
+    def learned_train(self, steps, batch=4096, lr=1e-3, seed=0):
+        loss = C.c_float()
+        self._chk(self.L.sbpx_learned_train(self.h, int(steps), int(batch), float(lr), seed, C.byref(loss)))
+        return loss.value
+
+    def learned_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        z = np.zeros((x.shape[0], 32), np.float32)
+        self._chk(self.L.sbpx_learned_forward(self.h, x.shape[0], x, z))
+        return z
+
+    def learned_gradients(self, x, mask, y):
+        x = np.ascontiguousarray(x, np.float32)
+        loss = np.zeros(x.shape[0], np.float32)
+        grads = np.zeros(LEARNED_PARAMS, np.float32)
+        self._chk(self.L.sbpx_learned_gradients(self.h, x.shape[0], x, np.ascontiguousarray(mask, np.uint32),
+                                                np.ascontiguousarray(y, np.int32), loss, grads))
+        return loss, grads
+
 
 def run_sbp(out_ptr, out_idx, in_ptr, in_idx, algorithm="async_gibbs", batches=1, approximate=True, seed=0,
             device=0, count_bytes=False):
@@ -293,7 +349,23 @@ class ResidentGraph:
         if not self.h:
             raise RuntimeError("sbp_graph_new: " + err.value.decode())
 
-    def partition(self, algorithm="async_gibbs", batches=1, approximate=True, seed=0, count_bytes=False):
+    def set_option(self, option, value):
+        err = C.create_string_buffer(512)
+        if self.H.sbp_graph_option(self.h, option, int(value), err, 512) != 0:
+            raise RuntimeError("sbp_graph_option: " + err.value.decode())
+
+    def load_learned(self, params):
+        """Installs proposal-network parameters (flat, LEARNED_PARAMS floats) for learned runs."""
+        err = C.create_string_buffer(512)
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != LEARNED_PARAMS:
+            raise ValueError("expected %d parameters" % LEARNED_PARAMS)
+        if self.H.sbp_graph_learned(self.h, p, err, 512) != 0:
+            raise RuntimeError("sbp_graph_learned: " + err.value.decode())
+
+    def partition(self, algorithm="async_gibbs", batches=1, approximate=True, seed=0, count_bytes=False,
+                  learned=False):
+        self.set_option(OPT_PROPOSAL, 1 if learned else 0)
         a = np.zeros(self.N, np.int32)
         st = RunStats()
         err = C.create_string_buffer(512)

@@ -14,7 +14,8 @@ HOST_LIB = os.path.join(HERE, "libsbp_host.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SBPX_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "merge.hip", "api.cpp", "generator.cpp"]
+DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "merge.hip", "mlp.hip", "learned.hip", "api.cpp",
+                  "generator.cpp"]
 HOST_SOURCES = ["sbp_host.cpp", "capi.cpp"]
 
 # No FMA contraction (the objective's FP32 cell chain and FP64 sums must match the reference's

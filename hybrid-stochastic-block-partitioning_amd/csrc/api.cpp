@@ -114,6 +114,117 @@ void free_graph_buffers(Ctx& c) {
     c.bm = DevBlockmodel();
 }
 
+// ------------------------------------------------------------------------------------------
+// Proposal network buffers.
+void free_mlp(Ctx& c) {
+    MlpState& m = c.mlp;
+    for (int l = 0; l < 3; ++l) {
+        for (float** p : {&m.w[l], &m.b[l], &m.gw[l], &m.gb[l], &m.m1w[l], &m.m2w[l], &m.m1b[l], &m.m2b[l]}) dfree(p);
+        dfree(&m.w16[l]);
+    }
+    for (__bf16** p : {&m.h1, &m.h2, &m.dz, &m.dh1, &m.dh2, &m.ones, &m.ds_x, &m.x, &m.x2, &m.sh1, &m.sh2, &m.tx})
+        dfree(p);
+    for (float** p : {&m.z, &m.loss, &m.sz, &m.sz2, &m.qf, &m.qb}) dfree(p);
+    dfree(&m.ds_mask);
+    dfree(&m.tmask);
+    for (int** p : {&m.ds_y, &m.ds_count, &m.cand, &m.cnt, &m.ps, &m.ty, &m.tidx}) dfree(p);
+    dfree(&m.pu);
+    m = MlpState();
+}
+
+// Allocates the weights, Adam state and training buffers; initialises the weights (He normal
+// for the hidden layers, scaled down for the scores; zero biases) from a seeded host generator.
+void init_mlp(Ctx& c, uint64_t seed) {
+    MlpState& m = c.mlp;
+    if (!m.ready) {
+        for (int l = 0; l < 3; ++l) {
+            const long n = mlp_weight_count(l), nb = mlp_bias_count(l);
+            for (float** p : {&m.w[l], &m.gw[l], &m.m1w[l], &m.m2w[l]}) dalloc(p, n);
+            for (float** p : {&m.b[l], &m.gb[l], &m.m1b[l], &m.m2b[l]}) dalloc(p, nb);
+            dalloc(&m.w16[l], n);
+        }
+        for (__bf16** p : {&m.h1, &m.h2, &m.dh1, &m.dh2}) dalloc(p, (size_t)MLP_BATCH * MLP_H);
+        dalloc(&m.dz, (size_t)MLP_BATCH * MLP_OUT);
+        dalloc(&m.ones, (size_t)MLP_BATCH);
+        dalloc(&m.z, (size_t)MLP_BATCH * MLP_OUT);
+        dalloc(&m.loss, (size_t)MLP_BATCH);
+        dalloc(&m.ds_count, 1);
+        dalloc(&m.tx, (size_t)MLP_BATCH * MLP_IN);
+        dalloc(&m.tmask, (size_t)MLP_BATCH);
+        dalloc(&m.ty, (size_t)MLP_BATCH);
+        dalloc(&m.tidx, (size_t)MLP_BATCH);
+        SBPX_HIP(hipMemset(m.ds_count, 0, sizeof(int)));
+        std::vector<float> one(MLP_BATCH, 1.0f);
+        float* tmp = nullptr;
+        dalloc(&tmp, MLP_BATCH);
+        SBPX_HIP(hipMemcpy(tmp, one.data(), sizeof(float) * MLP_BATCH, hipMemcpyHostToDevice));
+        mlp_convert_to_bf16(c, tmp, MLP_BATCH, m.ones);
+        c.sync();
+        dfree(&tmp);
+        m.ready = true;
+    }
+    // xoshiro256** stream from the seed, Box-Muller normals
+    uint64_t st[4];
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+    for (auto& x : st) {
+        z += 0x9E3779B97F4A7C15ull;
+        uint64_t y = z;
+        y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
+        y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
+        x = y ^ (y >> 31);
+    }
+    auto next = [&st]() {
+        auto rotl = [](uint64_t x, int k) { return (x << k) | (x >> (64 - k)); };
+        const uint64_t result = rotl(st[1] * 5, 7) * 9;
+        const uint64_t t = st[1] << 17;
+        st[2] ^= st[0];
+        st[3] ^= st[1];
+        st[1] ^= st[2];
+        st[0] ^= st[3];
+        st[2] ^= t;
+        st[3] = rotl(st[3], 45);
+        return result;
+    };
+    auto unit = [&next]() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+    for (int l = 0; l < 3; ++l) {
+        const long n = mlp_weight_count(l), nb = mlp_bias_count(l);
+        const int fan_in = l == 0 ? MLP_IN : MLP_H;
+        const double sd = (l == 2 ? 0.5 : 1.0) * std::sqrt(2.0 / fan_in);
+        std::vector<float> w(n);
+        for (long i = 0; i < n; ++i) w[i] = (float)(sd * std::sqrt(-2.0 * std::log(unit())) * std::cos(6.283185307179586 * unit()));
+        std::vector<float> zero(nb, 0.0f), zw(n, 0.0f);
+        SBPX_HIP(hipMemcpy(m.w[l], w.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(m.b[l], zero.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(m.m1w[l], zw.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(m.m2w[l], zw.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(m.m1b[l], zero.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(m.m2b[l], zero.data(), sizeof(float) * nb, hipMemcpyHostToDevice));
+    }
+    m.adam_t = 0;
+    mlp_sync_bf16(c);
+    c.sync();
+}
+
+// Per-vertex buffers of the learned sweeps ([N] rows).
+void ensure_mlp_rows(Ctx& c) {
+    MlpState& m = c.mlp;
+    const size_t N = (size_t)std::max(c.g.N, 1);
+    if (m.rows >= (int)N) return;
+    dalloc(&m.x, N * MLP_IN);
+    dalloc(&m.x2, N * MLP_IN);
+    dalloc(&m.sh1, N * MLP_H);
+    dalloc(&m.sh2, N * MLP_H);
+    dalloc(&m.sz, N * MLP_OUT);
+    dalloc(&m.sz2, N * MLP_OUT);
+    dalloc(&m.cand, N * MLP_CAND);
+    dalloc(&m.cnt, N * MLP_CAND);
+    dalloc(&m.qf, N);
+    dalloc(&m.qb, N);
+    dalloc(&m.ps, N);
+    dalloc(&m.pu, N);
+    m.rows = (int)N;
+}
+
 int bin_of(int degree) {
     for (int b = 0; b < NUM_LDS_BINS; ++b)
         if (degree <= BIN_MAXDEG[b]) return b;
@@ -170,8 +281,32 @@ double model_term(long N, long E, int B) {
 // Evaluates a binned vertex list (each wave one vertex) with outputs at the vertex index. Each
 // bin's launch is bracketed by events when timed.
 void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, const int* d_huge_off,
-                      uint64_t seed, uint32_t call, uint32_t tag, long long* bytes, bool timed) {
+                      uint64_t seed, uint32_t call, uint32_t tag, long long* bytes, bool timed, bool learned = false) {
     Ctx& c = ctx->c;
+    // the vertices the learned proposal covers: bins 0-2, contiguous in the list
+    const int* low = d_list + bl.offset[0];
+    const int nlow = bl.count[0] + bl.count[1] + bl.count[2];
+    if (learned && nlow > 0) {
+        // plain proposals and accept uniforms (vertex-indexed), then the mixture's draws
+        for (int b = 0; b < 3; ++b) {
+            if (bl.count[b] == 0) continue;
+            EvalLaunch L{};
+            L.list = d_list + bl.offset[b];
+            L.n = bl.count[b];
+            L.bin = b;
+            L.seed = seed;
+            L.call = call;
+            L.tag = tag;
+            L.out_by_vertex = 1;
+            L.propose_only = 1;
+            L.out_u = c.ev_u;
+            launch_eval(c, L);
+        }
+        learned_features(c, low, nlow);
+        learned_mixture(c, low, nlow, seed, call);
+    } else if (c.mlp.logging && nlow > 0) {
+        learned_features(c, low, nlow);  // same snapshot as the evaluation
+    }
     for (int b = 0; b < NUM_BINS; ++b) {
         if (bl.count[b] == 0) continue;
         if (timed) c.timer_begin();
@@ -180,6 +315,7 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
         L.n = bl.count[b];
         L.bin = b;
         L.variant = c.pick_variant(b);
+        L.learned = learned && b < 3;
         L.seed = seed;
         L.call = call;
         L.tag = tag;
@@ -280,6 +416,7 @@ void sbpx_destroy(sbpx_ctx* ctx) {
     (void)hipSetDevice(ctx->c.device);
     if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
     free_graph_buffers(ctx->c);
+    free_mlp(ctx->c);
     dfree(&ctx->c.dev_err);
     dfree(&ctx->c.counters);
     dfree(&ctx->c.red);
@@ -294,6 +431,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
     return guarded(ctx, [&] {
         if (option == 1) {
             ctx->count_bytes = value != 0;
+        } else if (option == 4) {
+            if (value != 0 && value != 1) throw ArgError("proposal must be 0 (plain) or 1 (learned mixture)");
+            ctx->c.proposal = (int)value;
         } else if (option == 3) {
             if (value < 0 || value > 2) throw ArgError("evaluation kernel variant must be 0 (tuned), 1 or 2");
             ctx->c.eval_variant = (int)value;
@@ -705,10 +845,14 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
         else if (algo == SBPX_ALGO_HYBRID)
             mh_sequential(c, c.mh.d_high, (int)c.g.high.size(), c.mh.max_deg_high, seed, call, TAG_MH, mh_bytes,
                           true);
+        const bool learned = c.proposal == 1;
+        if (learned && !c.mlp.ready) throw StateError("learned proposal selected but no network initialised");
+        if ((learned || c.mlp.logging) && !c.mlp.rows) ensure_mlp_rows(c);
         for (const BinnedList& bl : c.plan.batches_async) {
             const int* d_list = c.plan.d_lists + bl.offset[0];
             eval_vertex_list(ctx, c.plan.d_lists, bl, c.plan.d_huge_offsets + bl.huge_base, seed, call, TAG_ASYNC,
-                             bytes, true);
+                             bytes, true, learned);
+            if (c.mlp.logging && !learned) learned_log(c, d_list, bl.count[0] + bl.count[1] + bl.count[2]);
             apply_accepted(c, d_list, bl.total);
         }
         long long cnt[8];
@@ -860,6 +1004,172 @@ sbpx_status sbpx_selftest_logf(sbpx_ctx* ctx, uint32_t lo, uint32_t n, float* ou
             throw;
         }
         SBPX_HIP(hipFree(d));
+    });
+}
+
+// ------------------------------------------------------------------------------------------
+// Proposal network (no counterpart in the reference; see include/sbpx.h).
+sbpx_status sbpx_learned_init(sbpx_ctx* ctx, uint64_t seed) {
+    return guarded(ctx, [&] { init_mlp(ctx->c, seed); });
+}
+
+sbpx_status sbpx_learned_params(sbpx_ctx* ctx, float* out, const float* in) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (!c.mlp.ready) throw StateError("no network initialised");
+        long off = 0;
+        for (int l = 0; l < 3; ++l) {
+            const long n = mlp_weight_count(l), nb = mlp_bias_count(l);
+            if (out) {
+                SBPX_HIP(hipMemcpy(out + off, c.mlp.w[l], sizeof(float) * n, hipMemcpyDeviceToHost));
+                SBPX_HIP(hipMemcpy(out + off + n, c.mlp.b[l], sizeof(float) * nb, hipMemcpyDeviceToHost));
+            }
+            if (in) {
+                SBPX_HIP(hipMemcpy(c.mlp.w[l], in + off, sizeof(float) * n, hipMemcpyHostToDevice));
+                SBPX_HIP(hipMemcpy(c.mlp.b[l], in + off + n, sizeof(float) * nb, hipMemcpyHostToDevice));
+            }
+            off += n + nb;
+        }
+        if (in) {
+            mlp_sync_bf16(c);
+            c.sync();
+        }
+    });
+}
+
+sbpx_status sbpx_learned_log(sbpx_ctx* ctx, int32_t capacity) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        MlpState& m = c.mlp;
+        if (!m.ready) throw StateError("no network initialised");
+        if (capacity < 0) throw ArgError("capacity must be >= 0");
+        if (capacity == 0) {
+            m.logging = false;
+            return;
+        }
+        if (capacity != m.ds_cap) {
+            dalloc(&m.ds_x, (size_t)capacity * MLP_IN);
+            dalloc(&m.ds_mask, (size_t)capacity);
+            dalloc(&m.ds_y, (size_t)capacity);
+            m.ds_cap = capacity;
+        }
+        SBPX_HIP(hipMemset(m.ds_count, 0, sizeof(int)));
+        m.logging = true;
+    });
+}
+
+sbpx_status sbpx_learned_log_count(sbpx_ctx* ctx, int32_t* count) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (!count) throw ArgError("null output");
+        int n = 0;
+        if (c.mlp.ready) {
+            SBPX_HIP(hipMemcpyAsync(&n, c.mlp.ds_count, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+        }
+        *count = std::min(n, c.mlp.ds_cap);
+    });
+}
+
+sbpx_status sbpx_learned_train(sbpx_ctx* ctx, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                               float* mean_loss) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        MlpState& m = c.mlp;
+        if (!m.ready) throw StateError("no network initialised");
+        if (batch <= 0 || batch > MLP_BATCH) throw ArgError("batch must be in [1, 8192]");
+        int n = 0;
+        SBPX_HIP(hipMemcpyAsync(&n, m.ds_count, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        n = std::min(n, m.ds_cap);
+        if (n <= 0) throw StateError("no logged samples to train on");
+        uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+        std::vector<int> idx(batch);
+        float* grads[3] = {m.gw[0], m.gw[1], m.gw[2]};
+        float* gbias[3] = {m.gb[0], m.gb[1], m.gb[2]};
+        for (int t = 0; t < steps; ++t) {
+            for (int i = 0; i < batch; ++i) {  // splitmix64 draws, uniform rows with replacement
+                x += 0x9E3779B97F4A7C15ull;
+                uint64_t z = x;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                z ^= z >> 31;
+                idx[i] = (int)((unsigned __int128)z * (uint64_t)n >> 64);
+            }
+            SBPX_HIP(hipMemcpyAsync(m.tidx, idx.data(), sizeof(int) * batch, hipMemcpyHostToDevice, c.stream));
+            mlp_gather(c, m.tidx, batch);
+            mlp_backward(c, m.tx, m.tmask, m.ty, batch, m.loss, grads, gbias);
+            mlp_adam_step(c, lr);
+            c.sync();  // idx is reused by the next step's copy
+        }
+        if (mean_loss) {
+            std::vector<float> l(batch);
+            SBPX_HIP(hipMemcpyAsync(l.data(), m.loss, sizeof(float) * batch, hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+            double sum = 0.0;
+            for (float v : l) sum += v;
+            *mean_loss = (float)(sum / batch);
+        }
+    });
+}
+
+sbpx_status sbpx_learned_forward(sbpx_ctx* ctx, int32_t n, const float* x, float* z) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        MlpState& m = c.mlp;
+        if (!m.ready) throw StateError("no network initialised");
+        if (n <= 0 || n > MLP_BATCH) throw ArgError("n must be in [1, 8192]");
+        if (!x || !z) throw ArgError("null array");
+        float* tmp = nullptr;
+        dalloc(&tmp, (size_t)n * MLP_IN);
+        try {
+            SBPX_HIP(hipMemcpy(tmp, x, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice));
+            mlp_convert_to_bf16(c, tmp, (long)n * MLP_IN, m.tx);
+            mlp_forward(c, m.tx, n, m.h1, m.h2, m.z);
+            SBPX_HIP(hipMemcpyAsync(z, m.z, sizeof(float) * n * MLP_OUT, hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+        } catch (...) {
+            dfree(&tmp);
+            throw;
+        }
+        dfree(&tmp);
+    });
+}
+
+sbpx_status sbpx_learned_gradients(sbpx_ctx* ctx, int32_t n, const float* x, const uint32_t* mask, const int32_t* y,
+                                   float* loss, float* grads) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        MlpState& m = c.mlp;
+        if (!m.ready) throw StateError("no network initialised");
+        if (n <= 0 || n > MLP_BATCH) throw ArgError("n must be in [1, 8192]");
+        if (!x || !mask || !y || !grads) throw ArgError("null array");
+        for (int i = 0; i < n; ++i)
+            if (y[i] < 0 || y[i] >= MLP_CAND || !(mask[i] >> y[i] & 1u)) throw ArgError("label outside its mask");
+        float* tmp = nullptr;
+        dalloc(&tmp, (size_t)n * MLP_IN);
+        try {
+            SBPX_HIP(hipMemcpy(tmp, x, sizeof(float) * n * MLP_IN, hipMemcpyHostToDevice));
+            mlp_convert_to_bf16(c, tmp, (long)n * MLP_IN, m.tx);
+            SBPX_HIP(hipMemcpy(m.tmask, mask, sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+            SBPX_HIP(hipMemcpy(m.ty, y, sizeof(int) * n, hipMemcpyHostToDevice));
+            float* gw[3] = {m.gw[0], m.gw[1], m.gw[2]};
+            float* gb[3] = {m.gb[0], m.gb[1], m.gb[2]};
+            mlp_backward(c, m.tx, m.tmask, m.ty, n, m.loss, gw, gb);
+            long off = 0;
+            for (int l = 0; l < 3; ++l) {
+                const long nw = mlp_weight_count(l), nb = mlp_bias_count(l);
+                SBPX_HIP(hipMemcpyAsync(grads + off, m.gw[l], sizeof(float) * nw, hipMemcpyDeviceToHost, c.stream));
+                SBPX_HIP(hipMemcpyAsync(grads + off + nw, m.gb[l], sizeof(float) * nb, hipMemcpyDeviceToHost, c.stream));
+                off += nw + nb;
+            }
+            if (loss) SBPX_HIP(hipMemcpyAsync(loss, m.loss, sizeof(float) * n, hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+        } catch (...) {
+            dfree(&tmp);
+            throw;
+        }
+        dfree(&tmp);
     });
 }
 

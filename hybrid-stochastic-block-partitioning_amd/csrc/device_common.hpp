@@ -35,6 +35,7 @@ __device__ __forceinline__ Draw4 philox4x32_10(uint64_t seed, uint32_t c0, uint3
 constexpr uint32_t TAG_ASYNC = 0x4D430000u;
 constexpr uint32_t TAG_MH = 0x4D480000u;
 constexpr uint32_t TAG_MERGE = 0x4D520000u;
+constexpr uint32_t TAG_LEARN = 0x4D4C0000u;  // learned-mixture branch and categorical draws
 
 __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) { return __umulhi(a, b); }
 

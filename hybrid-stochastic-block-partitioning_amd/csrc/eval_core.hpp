@@ -263,10 +263,13 @@ __device__ __forceinline__ float fi(int x) { return (float)x; }
 // run as one loop over the concatenated ranges, two slots per lane per trip, with the keys,
 // then the degrees, loaded for both slots before either is used.
 // U: slots per lane per trip of the row scans and the Hastings pass (loads in flight per lane).
+// lw >= 0: the proposal was the learned mixture w * plain + (1 - w) * net with w = lw; lqf / lqb
+// are the net's forward and reverse probabilities (negative: no learned mass in that state).
 template <class View, int U = 2>
 __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockState& S, const View& M, int v,
                                                 int* tab, int cap, uint64_t seed, uint32_t call, uint32_t tag,
-                                                int given_s, double given_u, bool propose_only) {
+                                                int given_s, double given_u, bool propose_only, double lw = -1.0,
+                                                double lqf = 0.0, double lqb = 0.0) {
     const int lane = lane_id();
     MoveResult R{};
     R.v = v;
@@ -570,7 +573,14 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     pf = wave_sum(pf);
     pb = wave_sum(pb);
     R.nuniq = (int)wave_sum_ll(nuniq);
-    const double H = (k == 0) ? 1.0 : pb / pf;
+    double H = (k == 0) ? 1.0 : pb / pf;
+    if (lw >= 0.0 && k > 0) {
+        // mixture proposal: plain part by the closed form per unit of degree, learned part as given
+        const double plain_f = pf / (double)k, plain_b = pb / (double)k;
+        const double fwd = lw * plain_f + (1.0 - lw) * (lqf >= 0.0 ? lqf : plain_f);
+        const double bwd = lw * plain_b + (1.0 - lw) * (lqb >= 0.0 ? lqb : plain_b);
+        H = bwd / fwd;
+    }
     double a = exp(-3.0 * dS) * H;
     a = (a >= 1.0) ? 1.0 : a;
     R.dS = dS;

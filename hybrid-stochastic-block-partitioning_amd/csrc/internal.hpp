@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "device_common.hpp"
+#include "mlp.hpp"
 #include "../../include/sbpx.h"
 
 namespace sbpx {
@@ -51,6 +52,10 @@ constexpr int NUM_LDS_BINS = 5;
 constexpr int BIN_MAXDEG[NUM_LDS_BINS] = {30, 62, 126, 254, 510};
 constexpr int BIN_CAP[NUM_LDS_BINS] = {64, 128, 256, 512, 1024};
 constexpr int GROUP_MAXDEG = 2047;
+// The learned proposal covers the vertices of the first three bins (degree <= 126); the mixture
+// keeps the plain proposal with this weight.
+constexpr int LEARN_MAXDEG = 126;
+constexpr float LEARN_W_PLAIN = 0.2f;
 constexpr int NUM_BINS = NUM_LDS_BINS + 2;
 // timing / byte classes reported per sweep: 0 = LDS bins with tables up to 256 slots,
 // 1 = LDS bins with 512-1024 slots, 2 = group bins (degree > 510), 3 = sequential MH kernel
@@ -130,6 +135,8 @@ struct Ctx {
     DevBlockmodel bm;
     SweepPlan plan;
     MhState mh;
+    MlpState mlp;
+    int proposal = 0;  // 0 plain, 1 learned mixture for the batched low-degree vertices
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
 
     // scratch
@@ -261,6 +268,7 @@ struct EvalLaunch {
     const int* list;   // vertex ids
     int n;
     int bin;
+    int learned;            // 1: proposals from the learned mixture (bins 0-2 only)
     int variant;            // kernel variant of the bin: bins 0-2: 1 = four row slots per lane per trip;
                             // bins 3-4: 1 = 4 wavefronts share each vertex
     const int* given_s;     // optional fixed proposals
@@ -278,6 +286,19 @@ struct EvalLaunch {
 void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
 void finish_degrees(Ctx& c);
+// learned.hip / mlp.hip
+void learned_features(Ctx& c, const int* list, int n);
+void learned_log(Ctx& c, const int* list, int n);
+void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call);
+void mlp_forward(Ctx& c, const __bf16* X, int V, __bf16* H1, __bf16* H2, float* Z);
+void mlp_backward(Ctx& c, const __bf16* X, const uint32_t* cmask, const int* y, int V, float* loss,
+                  float* grads[3], float* gbias[3]);
+void mlp_adam_step(Ctx& c, float lr);
+void mlp_gather(Ctx& c, const int* idx, int n);
+void mlp_sync_bf16(Ctx& c);
+void mlp_convert_to_bf16(Ctx& c, const float* x, long n, __bf16* y);
+void mlp_convert_to_f32(Ctx& c, const __bf16* x, long n, float* y);
+
 // Heap-ordered merge selection on the device; parent: the union-find of merged blocks (B).
 void merge_heap_select(Ctx& c, int num_to_merge, const double* dS, const int32_t* best, std::vector<int>& parent);
 void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,

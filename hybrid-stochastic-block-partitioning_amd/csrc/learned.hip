@@ -1,0 +1,329 @@
+// Learned proposal for the batched (asynchronous Gibbs) evaluation. For a vertex v in block r
+// with degree <= LEARN_MAXDEG the network scores the 16 blocks its neighbours fall in most often
+// (ties: smaller block id); the proposal is the mixture
+//   q(s | v) = w * q_plain(s | v) + (1 - w) * q_net(s | v),   w = LEARN_W_PLAIN,
+// where q_plain is the reference's proposal (common.cpp:131-182) and q_net a softmax over the
+// candidates other than r. The Hastings ratio uses the mixture in both directions: the forward
+// q_net(s) from the scores before the move, the reverse q_net'(r) from the scores of the
+// features after the move (the candidates do not change when v moves; the degrees of r and s,
+// and the own block, do). q_plain enters through the reference's closed form (entropy.cpp:
+// 495-557, divided by the vertex degree). The chain therefore keeps the reference's target:
+// every move is still judged by its delta MDL. A vertex whose neighbours all sit in its own
+// block has no learned mass and uses the plain proposal alone.
+//
+// Features (64, BF16): for candidate i < 16: count_i / k, log1p(din[b_i]) / log1p(E),
+// log1p(dout[b_i]) / log1p(E); then log1p(k_out) / log(4097), log1p(k_in) / log(4097),
+// log1p(din[own]) / log1p(E), log1p(dout[own]) / log1p(E), count_own / k (0 unless the own block
+// is a candidate), [own block is a candidate], log(B) / log(N), 1, and eight zeros.
+#include <cmath>
+
+#include "eval_core.hpp"
+#include "internal.hpp"
+#include "mlp.hpp"
+
+namespace sbpx {
+
+constexpr int FEAT_WAVES = 4;
+constexpr int FEAT_CAP = 256;  // neighbour-block table per wavefront (degree <= LEARN_MAXDEG)
+
+__device__ __forceinline__ __bf16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    const unsigned short h = (unsigned short)(u >> 16);
+    return __builtin_bit_cast(__bf16, h);
+}
+
+struct FeatParams {
+    GraphView G;
+    BlockState S;
+    const int* list;  // vertices
+    int n;
+    float inv_logE;   // 1 / log1p(E)
+    float log_ratio;  // log(B) / log(N)
+    __bf16* x;        // [n][64]
+    int* cand;        // [n][16]
+    int* cnt;         // [n][16]
+};
+
+// One lane writes feature j (< 64) of a vertex whose own block is `own`; din_of / dout_of give
+// the block degrees to use (before or after the move).
+template <class DinOf, class DoutOf>
+__device__ __forceinline__ float feature(int j, const int* cand, const int* cnt, int k_out, int k_in, int own,
+                                         DinOf din_of, DoutOf dout_of, float inv_logE, float log_ratio) {
+    const int k = k_out + k_in;
+    const float invk = k > 0 ? 1.0f / (float)k : 0.0f;
+    if (j < 16) return cand[j] >= 0 ? (float)cnt[j] * invk : 0.0f;
+    if (j < 32) return cand[j - 16] >= 0 ? log1pf((float)din_of(cand[j - 16])) * inv_logE : 0.0f;
+    if (j < 48) return cand[j - 32] >= 0 ? log1pf((float)dout_of(cand[j - 32])) * inv_logE : 0.0f;
+    const float inv_logD = 1.0f / logf(4097.0f);
+    int own_i = -1;
+    for (int i = 0; i < 16; ++i)
+        if (cand[i] == own) own_i = i;
+    switch (j - 48) {
+        case 0: return log1pf((float)k_out) * inv_logD;
+        case 1: return log1pf((float)k_in) * inv_logD;
+        case 2: return log1pf((float)din_of(own)) * inv_logE;
+        case 3: return log1pf((float)dout_of(own)) * inv_logE;
+        case 4: return own_i >= 0 ? (float)cnt[own_i] * invk : 0.0f;
+        case 5: return own_i >= 0 ? 1.0f : 0.0f;
+        case 6: return log_ratio;
+        case 7: return 1.0f;
+        default: return 0.0f;
+    }
+}
+
+__global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams P) {
+    __shared__ int tkey[FEAT_WAVES][FEAT_CAP];
+    __shared__ int tcnt[FEAT_WAVES][FEAT_CAP];
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    const int i = blockIdx.x * FEAT_WAVES + wave;
+    if (i >= P.n) return;
+    int* key = tkey[wave];
+    int* cntt = tcnt[wave];
+    for (int j = lane; j < FEAT_CAP; j += WAVE) {
+        key[j] = -1;
+        cntt[j] = 0;
+    }
+    fence_wave();
+    const int v = P.list[i];
+    const int r = P.S.assign[v];
+    const int ob = P.G.out_ptr[v], dO = P.G.out_ptr[v + 1] - ob;
+    const int ib = P.G.in_ptr[v], dI = P.G.in_ptr[v + 1] - ib;
+    const int self = P.G.self[v];
+    for (int j = lane; j < dO + dI; j += WAVE) {
+        const int w = j < dO ? P.G.out_idx[ob + j] : P.G.in_idx[ib + (j - dO)];
+        if (w == v) continue;
+        const int slot = tab_insert(key, FEAT_CAP, P.S.assign[w]);
+        if (slot >= 0) atomicAdd(&cntt[slot], 1);
+    }
+    fence_wave();
+    // top 16 by (count desc, block asc)
+    int* cand = P.cand + (long)i * MLP_CAND;
+    int* cnt = P.cnt + (long)i * MLP_CAND;
+    for (int t = 0; t < MLP_CAND; ++t) {
+        long long best = -1;
+        for (int j = lane; j < FEAT_CAP; j += WAVE) {
+            const int kk = key[j], c = cntt[j];
+            if (kk >= 0 && c > 0) {
+                const long long score = ((long long)c << 32) | (long long)(0x7FFFFFFF - kk);
+                best = score > best ? score : best;
+            }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const long long o = __shfl_xor(best, off, WAVE);
+            best = o > best ? o : best;
+        }
+        int b = -1, c = 0;
+        if (best >= 0) {
+            b = 0x7FFFFFFF - (int)(best & 0xFFFFFFFF);
+            c = (int)(best >> 32);
+            for (int j = lane; j < FEAT_CAP; j += WAVE)
+                if (key[j] == b) cntt[j] = 0;  // taken
+        }
+        if (lane == 0) {
+            cand[t] = b;
+            cnt[t] = c;
+        }
+        fence_wave();
+    }
+    const int k_out = dO, k_in = dI - self;
+    auto din_of = [&](int b) { return P.S.din[b]; };
+    auto dout_of = [&](int b) { return P.S.dout[b]; };
+    P.x[(long)i * MLP_IN + lane] =
+        f2bf(feature(lane, cand, cnt, k_out, k_in, r, din_of, dout_of, P.inv_logE, P.log_ratio));
+}
+
+// Softmax probability of candidate `pick` among the candidates other than `own` (-1 when no
+// candidate qualifies; 0 when pick is not one of them).
+__device__ float cand_prob(const float* z, const int* cand, int own, int pick) {
+    float mx = -1e30f;
+    int any = 0;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            mx = fmaxf(mx, z[j]);
+            any = 1;
+        }
+    if (!any) return -1.0f;
+    float den = 0.0f, num = 0.0f;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            const float e = expf(z[j] - mx);
+            den += e;
+            if (cand[j] == pick) num = e;
+        }
+    return num / den;
+}
+
+struct MixParams {
+    GraphView G;
+    BlockState S;
+    const int* list;
+    int n;
+    uint64_t seed;
+    uint32_t call;
+    float w_plain;
+    const float* z;       // [n][32] scores before the move
+    const int* cand;
+    const int* cnt;
+    const int* s_plain;   // [N] plain proposal (vertex-indexed)
+    int* ps;              // [N] mixture proposal
+    float* qf;            // [N]
+    __bf16* x2;           // [n][64] features after the move
+    float inv_logE;
+    float log_ratio;
+};
+
+// One wavefront per vertex: draws the mixture's proposal, its learned forward probability, and
+// the features of the moved vertex.
+__global__ void __launch_bounds__(FEAT_WAVES * WAVE) mixture_kernel(MixParams P) {
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    const int i = blockIdx.x * FEAT_WAVES + wave;
+    if (i >= P.n) return;
+    const int v = P.list[i];
+    const int r = P.S.assign[v];
+    const int* cand = P.cand + (long)i * MLP_CAND;
+    const int* cnt = P.cnt + (long)i * MLP_CAND;
+    const float* z = P.z + (long)i * MLP_OUT;
+    const Draw4 dr = philox4x32_10(P.seed, (uint32_t)v, P.call, 0u, TAG_LEARN);
+    int s = P.s_plain[v];
+    float qf = cand_prob(z, cand, r, -1);  // -1: no learned mass
+    if (qf >= 0.0f) {
+        if (unit53(dr.x0, dr.x1) >= (double)P.w_plain) {
+            // categorical draw over the softmax, candidates in slot order
+            float mx = -1e30f;
+            for (int j = 0; j < MLP_CAND; ++j)
+                if (cand[j] >= 0 && cand[j] != r) mx = fmaxf(mx, z[j]);
+            float den = 0.0f;
+            for (int j = 0; j < MLP_CAND; ++j)
+                if (cand[j] >= 0 && cand[j] != r) den += expf(z[j] - mx);
+            const float target = (float)unit53(dr.x2, dr.x3) * den;
+            float run = 0.0f;
+            int pick = -1;
+            for (int j = 0; j < MLP_CAND; ++j)
+                if (cand[j] >= 0 && cand[j] != r) {
+                    run += expf(z[j] - mx);
+                    pick = cand[j];
+                    if (run > target) break;
+                }
+            s = pick;
+        }
+        qf = cand_prob(z, cand, r, s);
+    }
+    if (lane == 0) {
+        P.ps[v] = s;
+        P.qf[v] = qf;
+    }
+    const int ob = P.G.out_ptr[v], dO = P.G.out_ptr[v + 1] - ob;
+    const int dI = P.G.in_ptr[v + 1] - P.G.in_ptr[v];
+    const int self = P.G.self[v];
+    const int k_out = dO, k_in = dI - self;
+    // block degrees after v moves r -> s (Blockmodel::move_vertex, blockmodel.cpp:344-360)
+    auto din_of = [&](int b) {
+        int d = P.S.din[b];
+        if (s != r) d += b == s ? (k_in + self) : (b == r ? -(k_in + self) : 0);
+        return d;
+    };
+    auto dout_of = [&](int b) {
+        int d = P.S.dout[b];
+        if (s != r) d += b == s ? k_out : (b == r ? -k_out : 0);
+        return d;
+    };
+    P.x2[(long)i * MLP_IN + lane] =
+        f2bf(feature(lane, cand, cnt, k_out, k_in, s, din_of, dout_of, P.inv_logE, P.log_ratio));
+}
+
+// Learned reverse probability q_net'(r | v in s) from the scores after the move.
+__global__ void reverse_kernel(const int* __restrict__ list, int n, const int* __restrict__ assign,
+                               const float* __restrict__ z2, const int* __restrict__ cand, const int* __restrict__ ps,
+                               float* __restrict__ qb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = list[i];
+    qb[v] = cand_prob(z2 + (long)i * MLP_OUT, cand + (long)i * MLP_CAND, ps[v], assign[v]);
+}
+
+// Appends the accepted moves whose target is a candidate to the training set.
+__global__ void log_moves_kernel(const int* __restrict__ list, int n, const int* __restrict__ assign,
+                                 const uint8_t* __restrict__ stat, const int* __restrict__ out_s,
+                                 const __bf16* __restrict__ x, const int* __restrict__ cand, __bf16* ds_x,
+                                 uint32_t* ds_mask, int* ds_y, int* count, int cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = list[i];
+    if (stat[v] != 2) return;
+    const int s = out_s[v], r = assign[v];
+    const int* c = cand + (long)i * MLP_CAND;
+    int label = -1;
+    uint32_t mask = 0;
+    for (int j = 0; j < MLP_CAND; ++j) {
+        if (c[j] >= 0 && c[j] != r) mask |= 1u << j;
+        if (c[j] == s) label = j;
+    }
+    if (label < 0 || !(mask >> label & 1u)) return;
+    const int slot = atomicAdd(count, 1);
+    if (slot >= cap) return;
+    for (int j = 0; j < MLP_IN; ++j) ds_x[(long)slot * MLP_IN + j] = x[(long)i * MLP_IN + j];
+    ds_mask[slot] = mask;
+    ds_y[slot] = label;
+}
+
+// ------------------------------------------------------------------------------------------
+static FeatParams feat_params(Ctx& c, const int* list, int n) {
+    FeatParams P{};
+    P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
+    P.S = BlockState{c.bm.assign, c.bm.dout, c.bm.din, c.bm.deg, c.bm.B};
+    P.list = list;
+    P.n = n;
+    P.inv_logE = 1.0f / log1pf((float)c.g.E);
+    P.log_ratio = c.g.N > 1 ? (float)(log((double)c.bm.B) / log((double)c.g.N)) : 0.0f;
+    P.x = c.mlp.x;
+    P.cand = c.mlp.cand;
+    P.cnt = c.mlp.cnt;
+    return P;
+}
+
+void learned_features(Ctx& c, const int* list, int n) {
+    if (n <= 0) return;
+    FeatParams P = feat_params(c, list, n);
+    features_kernel<<<(n + FEAT_WAVES - 1) / FEAT_WAVES, FEAT_WAVES * WAVE, 0, c.stream>>>(P);
+    SBPX_HIP(hipGetLastError());
+}
+
+void learned_log(Ctx& c, const int* list, int n) {
+    MlpState& m = c.mlp;
+    if (n <= 0 || !m.logging) return;
+    log_moves_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(list, n, c.bm.assign, c.ev_acc, c.ev_s, m.x, m.cand,
+                                                           m.ds_x, m.ds_mask, m.ds_y, m.ds_count, m.ds_cap);
+    SBPX_HIP(hipGetLastError());
+}
+
+void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call) {
+    if (n <= 0) return;
+    MlpState& m = c.mlp;
+    mlp_forward(c, m.x, n, m.sh1, m.sh2, m.sz);
+    MixParams P{};
+    P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
+    P.S = BlockState{c.bm.assign, c.bm.dout, c.bm.din, c.bm.deg, c.bm.B};
+    P.list = list;
+    P.n = n;
+    P.seed = seed;
+    P.call = call;
+    P.w_plain = LEARN_W_PLAIN;
+    P.z = m.sz;
+    P.cand = m.cand;
+    P.cnt = m.cnt;
+    P.s_plain = c.ev_s;
+    P.ps = m.ps;
+    P.qf = m.qf;
+    P.x2 = m.x2;
+    P.inv_logE = 1.0f / log1pf((float)c.g.E);
+    P.log_ratio = c.g.N > 1 ? (float)(log((double)c.bm.B) / log((double)c.g.N)) : 0.0f;
+    mixture_kernel<<<(n + FEAT_WAVES - 1) / FEAT_WAVES, FEAT_WAVES * WAVE, 0, c.stream>>>(P);
+    SBPX_HIP(hipGetLastError());
+    mlp_forward(c, m.x2, n, m.sh1, m.sh2, m.sz2);
+    reverse_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(list, n, c.bm.assign, m.sz2, m.cand, m.ps, m.qb);
+    SBPX_HIP(hipGetLastError());
+}
+
+}  // namespace sbpx

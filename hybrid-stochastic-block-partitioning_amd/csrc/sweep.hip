@@ -33,6 +33,12 @@ struct EvalParams {
     int* __restrict__ err;
     int* __restrict__ pool;
     const int* __restrict__ pool_off;
+    // learned mixture (vertex-indexed): proposals, uniforms, net probabilities; lw < 0: plain
+    const int* __restrict__ lps;
+    const double* __restrict__ lpu;
+    const float* __restrict__ lqf;
+    const float* __restrict__ lqb;
+    float lw;
 };
 
 // Stores one evaluation's outputs (one lane).
@@ -64,10 +70,17 @@ __device__ void write_result(const EvalParams& P, const MoveResult& R, int v, in
 template <int U = 2>
 __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
     const int v = P.list[idx];
-    const int gs = P.given_s ? P.given_s[idx] : -1;
-    const double gu = P.given_u ? P.given_u[idx] : 0.0;
-    MoveResult R =
-        eval_move<CsrView, U>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0);
+    int gs = P.given_s ? P.given_s[idx] : -1;
+    double gu = P.given_u ? P.given_u[idx] : 0.0;
+    double lqf = 0.0, lqb = 0.0;
+    if (P.lw >= 0.0f) {
+        gs = P.lps[v];
+        gu = P.lpu[v];
+        lqf = P.lqf[v];
+        lqb = P.lqb[v];
+    }
+    MoveResult R = eval_move<CsrView, U>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu,
+                                         P.propose_only != 0, (double)P.lw, lqf, lqb);
     if (lane_id() == 0) write_result(P, R, v, idx);
 }
 
@@ -123,6 +136,7 @@ static EvalParams make_params(Ctx& c) {
                   c.bm.col_ptr, c.bm.col_row, c.bm.col_val, c.bm.col_cum};
     P.err = c.dev_err;
     P.pool = c.table_pool;
+    P.lw = -1.0f;
     return P;
 }
 
@@ -146,6 +160,14 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     P.out_u = L.out_u;
     P.bytes = L.bytes;
     P.pool_off = L.huge_offsets;
+    if (L.learned) {
+        if (L.bin >= 3) throw StateError("learned proposals are evaluated by the low-degree bins only");
+        P.lps = c.mlp.ps;
+        P.lpu = c.ev_u;
+        P.lqf = c.mlp.qf;
+        P.lqb = c.mlp.qb;
+        P.lw = LEARN_W_PLAIN;
+    }
     static_assert(BIN_CAP[0] == 64 && BIN_CAP[1] == 128 && BIN_CAP[2] == 256 && BIN_CAP[3] == 512 &&
                       BIN_CAP[4] == 1024,
                   "bin table sizes");

@@ -43,6 +43,26 @@ void* sbp_graph_new(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, c
 
 void sbp_graph_free(void* g) { delete (Graph*)g; }
 
+// A device option of the graph's context (sbpx_set_option), e.g. 4 = proposal (0 plain, 1 learned).
+int sbp_graph_option(void* gp, int option, int64_t value, char* err, int errlen) {
+    sbpx_ctx* ctx = ((Graph*)gp)->device()->ctx;
+    if (sbpx_set_option(ctx, option, value) != SBPX_OK) {
+        set_err(err, errlen, sbpx_last_error(ctx));
+        return 1;
+    }
+    return 0;
+}
+
+// Installs the proposal network's parameters (SBPX_LEARNED_PARAMS floats) on the graph's context.
+int sbp_graph_learned(void* gp, const float* params, char* err, int errlen) {
+    sbpx_ctx* ctx = ((Graph*)gp)->device()->ctx;
+    if (sbpx_learned_init(ctx, 0) != SBPX_OK || sbpx_learned_params(ctx, nullptr, params) != SBPX_OK) {
+        set_err(err, errlen, sbpx_last_error(ctx));
+        return 1;
+    }
+    return 0;
+}
+
 // sbp::stochastic_block_partition on a resident graph. algorithm: 0 = metropolis_hastings,
 // 1 = async_gibbs, 2 = hybrid_mcmc. Returns 0 on success, otherwise writes a message into err.
 int sbp_partition(void* gp, int algorithm, int32_t batches, int approximate, uint64_t seed, int count_bytes,

@@ -66,7 +66,9 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            unroll, one wavefront per vertex), 2 the second (four row slots per lane per trip
  *            for degrees <= 126; 4 wavefronts sharing a vertex for degrees 127-510), or 0 to time
  *            both and keep the faster per bin after every change of the number of blocks
- *            (default). Results do not depend on options 2 and 3. */
+ *            (default). Results do not depend on options 2 and 3.
+ * option 4 = proposal of the batched evaluation: 0 the reference's, 1 the learned mixture
+ *            (see sbpx_learned_init). */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as
@@ -122,6 +124,37 @@ sbpx_status sbpx_carry_out_merges(sbpx_ctx* ctx, int32_t num_to_merge, int mode,
 /* Both steps: block_merge::merge_blocks (block_merge.cpp:278-311). */
 sbpx_status sbpx_merge_blocks(sbpx_ctx* ctx, int32_t num_to_merge, int mode, uint64_t seed,
                               uint32_t call);
+
+/* ---------------------------------------------------------------------------------------
+ * Learned proposal. No counterpart in the reference: a proposal network for the batched
+ * (asynchronous Gibbs) evaluation, mixed with the reference's proposal and corrected in the
+ * Hastings ratio, so the chain's target and MDL stay the reference's. Network: 64 features ->
+ * 256 -> 256 -> 32 scores (ReLU), BF16 weights on the matrix cores, FP32 accumulation; it scores
+ * the 16 blocks a vertex's neighbours fall in most often. Selected per context with option 4
+ * (0 plain proposal, 1 mixture 0.2 * plain + 0.8 * network) for vertices of degree <= 126.
+ * Parameters, flat, in this order: W1[256][64], b1[256], W2[256][256], b2[256], W3[32][256],
+ * b3[32] (SBPX_LEARNED_PARAMS floats; weights output-major). */
+#define SBPX_LEARNED_PARAMS 90656
+/* (Re)initialise the network: He-normal weights from the seed, zero biases, fresh Adam state. */
+sbpx_status sbpx_learned_init(sbpx_ctx* ctx, uint64_t seed);
+/* Copy the parameters out (out != NULL), then in (in != NULL). */
+sbpx_status sbpx_learned_params(sbpx_ctx* ctx, float* out, const float* in);
+/* Log training samples from the following plain sweeps: for every accepted move of a vertex of
+ * degree <= 126 whose target is one of its candidates, its features and the candidate index.
+ * capacity > 0 starts a fresh log of up to that many rows; 0 stops logging (rows are kept). */
+sbpx_status sbpx_learned_log(sbpx_ctx* ctx, int32_t capacity);
+sbpx_status sbpx_learned_log_count(sbpx_ctx* ctx, int32_t* count);
+/* Adam steps on the logged rows: `steps` minibatches of `batch` (<= 8192) rows drawn uniformly
+ * with replacement from the seed; softmax cross-entropy over each row's candidates. Reports the
+ * mean loss of the last minibatch. */
+sbpx_status sbpx_learned_train(sbpx_ctx* ctx, int32_t steps, int32_t batch, float lr, uint64_t seed,
+                               float* mean_loss);
+/* The network on n (<= 8192) rows of 64 features (rounded to BF16): z[n][32] scores. */
+sbpx_status sbpx_learned_forward(sbpx_ctx* ctx, int32_t n, const float* x, float* z);
+/* Mean softmax cross-entropy of n rows (candidate bit masks, labels in their masks): per-row
+ * losses and the gradient of every parameter, in the flat order above. */
+sbpx_status sbpx_learned_gradients(sbpx_ctx* ctx, int32_t n, const float* x, const uint32_t* mask,
+                                   const int32_t* y, float* loss, float* grads);
 
 /* Self-test of the device's FP32 log (the function the objective uses, utils.hpp:145-151):
  * out[i] = log of the float whose bit pattern is lo + i, for i < n. */

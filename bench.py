@@ -44,6 +44,39 @@ KERNELS = ["eval_bin_kernel<64..256> (batched vertex-move evaluation, degree <= 
            "mh_kernel (sequential Metropolis-Hastings evaluation and commit, all group sizes)"]
 
 
+def train_learned(m, cfg, seeds, log_sweeps=8):
+    """Trains the proposal network (untimed) on accepted moves logged from plain asynchronous Gibbs
+    sweeps on other graphs of the same family (one merge phase at a time, `log_sweeps` sweeps
+    each, down to about twice the planted block count)."""
+    ctx = m.Context(0)
+    ctx.learned_init(1)
+    ctx.learned_log(4_000_000)
+    t0 = time.perf_counter()
+    for seed in seeds:
+        op, oi, ip, ii, _ = m.generate_dcsbm(**cfg, seed=seed)
+        n = len(op) - 1
+        ctx.upload_graph(op, oi, ip, ii)
+        ctx.split()
+        ctx.build(n, np.arange(n, dtype=np.int32))
+        B, call = n, 0
+        while B > 2 * cfg["num_blocks"]:
+            ctx.merge_blocks(B // 2, seed, call)
+            B -= B // 2
+            for i in range(log_sweeps):
+                ctx.sweep(m.ALGO_ASYNC_GIBBS, 1, seed, 1000 * call + i)
+            call += 1
+    ctx.learned_log(0)
+    rows = ctx.learned_log_count()
+    t1 = time.perf_counter()
+    first = ctx.learned_train(50, 4096, 1e-3, 7)
+    last = ctx.learned_train(550, 4096, 1e-3, 8)
+    t2 = time.perf_counter()
+    params = ctx.learned_params()
+    ctx.close()
+    return params, {"graphs": [int(s) for s in seeds], "rows": rows, "steps": 600, "batch": 4096,
+                    "loss_first": first, "loss_last": last, "log_s": t1 - t0, "train_s": t2 - t1}
+
+
 def measured_traffic():
     """HBM bytes per launch per kernel, from the PMC summary kept in profiles/ (FETCH_SIZE and
     WRITE_SIZE runs scaled by a copy kernel of known size)."""
@@ -98,8 +131,8 @@ def cpu_baseline(arrays, seconds, seed, algorithm):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", default="C2")
     ap.add_argument("--algorithm", default="hybrid_mcmc", choices=["hybrid_mcmc", "async_gibbs",
                                                                    "metropolis_hastings"])
@@ -108,6 +141,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the untimed byte-counting run")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -224,6 +258,28 @@ def main():
              "traffic": traffic.get(KERNELS[i].split(" ")[0])}
             for i in range(4) if launches_per_run[i] > 0],
     }
+    if not args.no_learned and args.algorithm != "metropolis_hastings":
+        params, info = train_learned(m, cfg, seeds=(args.seed + 1, args.seed + 2))
+        g.load_learned(params)
+        t0 = time.perf_counter()
+        la, lst = g.partition(learned=True, **kw)
+        tl = time.perf_counter() - t0
+        plain_t = float(np.mean(times))
+        # time at equal MDL: when the learned run's outer loop first reached the plain run's final MDL
+        lt, lmdl, _ = g.trajectory()
+        hit = np.flatnonzero(lmdl <= st.mdl)
+        t_equal = float(lt[hit[0]]) if hit.size else None
+        out["learned"] = {
+            "proposal": "0.2 * plain + 0.8 * network (Hastings-corrected), vertices of degree <= 126 in the "
+                        "batched sweeps; the sequential Metropolis-Hastings part stays plain",
+            "time_to_partition_s": tl, "speedup_vs_plain": plain_t / tl,
+            "time_to_plain_final_mdl_s": t_equal,
+            "speedup_at_equal_mdl": (plain_t / t_equal) if t_equal else None,
+            "evals_per_s": lst.evals / tl, "sweeps": int(lst.sweeps),
+            "final_mdl": lst.mdl, "mdl_vs_plain": lst.mdl / st.mdl, "final_num_blocks": int(lst.num_blocks),
+            "nmi_vs_planted": ev.nmi(truth, la), "modularity": ev.modularity(out_ptr, out_idx, in_ptr, la),
+            "training": info}
+        print(f"# learned: {tl:.3f} s, B={lst.num_blocks}, mdl={lst.mdl:.1f}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(arrays, args.cpu_seconds, args.seed, args.algorithm)
     print(json.dumps(out), flush=True)

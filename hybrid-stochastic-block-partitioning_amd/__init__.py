@@ -111,6 +111,7 @@ def load_host():
     H.sbp_graph_new.restype = C.c_void_p
     H.sbp_graph_free.argtypes = [C.c_void_p]
     H.sbp_graph_free.restype = None
+    H.sbp_last_trajectory.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
     H.sbp_graph_option.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_char_p, C.c_int]
     H.sbp_graph_learned.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS"),
                                     C.c_char_p, C.c_int]
@@ -375,6 +376,13 @@ class ResidentGraph:
         if rc != 0:
             raise RuntimeError("sbp_partition: " + err.value.decode())
         return a, st
+
+    def trajectory(self):
+        """Outer iterations of the last run: (seconds since its start, MDL, number of blocks)."""
+        n = self.H.sbp_last_trajectory(None, None, None, 0)
+        t, mdl, B = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
+        self.H.sbp_last_trajectory(t.ctypes.data, mdl.ctypes.data, B.ctypes.data, n)
+        return t, mdl, B
 
     def close(self):
         if getattr(self, "h", None):

@@ -43,6 +43,19 @@ void* sbp_graph_new(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, c
 
 void sbp_graph_free(void* g) { delete (Graph*)g; }
 
+// The last run's outer iterations (sbp::intermediate_results): wall seconds since the start,
+// MDL and number of blocks after each merge + MCMC phase. Returns how many there are; fills at
+// most cap entries.
+int sbp_last_trajectory(double* seconds, double* mdl, int32_t* num_blocks, int cap) {
+    const std::vector<sbp::Intermediate> r = sbp::get_intermediates();
+    for (int i = 0; i < (int)r.size() && i < cap; ++i) {
+        if (seconds) seconds[i] = r[i].seconds;
+        if (mdl) mdl[i] = r[i].mdl;
+        if (num_blocks) num_blocks[i] = r[i].num_blocks;
+    }
+    return (int)r.size();
+}
+
 // A device option of the graph's context (sbpx_set_option), e.g. 4 = proposal (0 plain, 1 learned).
 int sbp_graph_option(void* gp, int option, int64_t value, char* err, int errlen) {
     sbpx_ctx* ctx = ((Graph*)gp)->device()->ctx;

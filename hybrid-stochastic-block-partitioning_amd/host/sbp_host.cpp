@@ -299,11 +299,13 @@ Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edg
 // ---------------------------------------------------------------------------- driver
 namespace sbp {
 static std::vector<Intermediate> intermediate_results;
+static std::chrono::steady_clock::time_point partition_start;
 std::vector<Intermediate> get_intermediates() { return intermediate_results; }
 
 static void add_intermediate(float iteration, const Blockmodel& blockmodel, double mdl) {
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - partition_start).count();
     intermediate_results.push_back(Intermediate{iteration, mdl, blockmodel.getNum_blocks(),
-                                                finetune::MCMC_iterations, finetune::MCMC_time});
+                                                finetune::MCMC_iterations, finetune::MCMC_time, t});
 }
 
 bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks) {
@@ -316,6 +318,7 @@ bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_tr
 // sbp::stochastic_block_partition (sbp.cpp:61-98).
 Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
     args = a;
+    partition_start = std::chrono::steady_clock::now();
     intermediate_results.clear();
     finetune::MCMC_iterations = 0;
     finetune::MCMC_time = 0.0;

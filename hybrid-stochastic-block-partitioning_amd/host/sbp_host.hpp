@@ -135,6 +135,7 @@ struct Intermediate {
     int num_blocks;
     int mcmc_iterations;
     double mcmc_time;
+    double seconds;  // wall time since stochastic_block_partition began
 };
 std::vector<Intermediate> get_intermediates();
 Blockmodel stochastic_block_partition(Graph& graph, Args& args);

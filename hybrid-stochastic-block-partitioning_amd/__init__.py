@@ -107,7 +107,15 @@ def load_host():
     H = C.CDLL(HOST_LIB_PATH)
     H.sbp_run.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int,
                           C.c_int, i32p, C.POINTER(RunStats), C.c_char_p, C.c_int]
-    H.sbp_graph_new.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_char_p, C.c_int]
+    H.sbp_graph_new.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_int64, C.c_char_p, C.c_int]
+    H.sbp_file_load.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int]
+    H.sbp_file_load.restype = C.c_void_p
+    H.sbp_file_info.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+    H.sbp_file_info.restype = None
+    H.sbp_file_arrays.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+    H.sbp_file_arrays.restype = None
+    H.sbp_file_free.argtypes = [C.c_void_p]
+    H.sbp_file_free.restype = None
     H.sbp_graph_new.restype = C.c_void_p
     H.sbp_graph_free.argtypes = [C.c_void_p]
     H.sbp_graph_free.restype = None
@@ -337,16 +345,40 @@ def run_sbp(out_ptr, out_idx, in_ptr, in_idx, algorithm="async_gibbs", batches=1
     return a, st
 
 
+def load_graph(base_path, undirected=False):
+    """Reads base_path.tsv (else .mtx) and base_path_truePartition.tsv as the reference's
+    Graph::load does (graph.cpp:24-62). Returns dict(out_ptr, out_idx, in_ptr, in_idx, truth,
+    num_edges, undirected); num_edges is the count the reference's MDL uses."""
+    H = load_host()
+    err = C.create_string_buffer(512)
+    h = H.sbp_file_load(str(base_path).encode(), int(undirected), err, 512)
+    if not h:
+        raise RuntimeError("sbp_file_load: " + err.value.decode())
+    try:
+        n, stored, E, und, tl = C.c_int32(), C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        H.sbp_file_info(h, C.byref(n), C.byref(stored), C.byref(E), C.byref(und), C.byref(tl))
+        op = np.zeros(n.value + 1, np.int32)
+        ip = np.zeros(n.value + 1, np.int32)
+        oi = np.zeros(stored.value, np.int32)
+        ii = np.zeros(stored.value, np.int32)
+        truth = np.zeros(tl.value, np.int32)
+        H.sbp_file_arrays(h, op.ctypes.data, oi.ctypes.data, ip.ctypes.data, ii.ctypes.data, truth.ctypes.data)
+    finally:
+        H.sbp_file_free(h)
+    return dict(out_ptr=op, out_idx=oi, in_ptr=ip, in_idx=ii, truth=truth, num_edges=E.value,
+                undirected=bool(und.value))
+
+
 class ResidentGraph:
     """A graph held on one GPU by the C++ driver, for repeated partitioning runs."""
 
-    def __init__(self, out_ptr, out_idx, in_ptr, in_idx, device=0):
+    def __init__(self, out_ptr, out_idx, in_ptr, in_idx, device=0, num_edges=-1):
         self.H = load_host()
         self.N = len(out_ptr) - 1
         err = C.create_string_buffer(512)
         self.h = self.H.sbp_graph_new(self.N, np.ascontiguousarray(out_ptr, np.int32),
                                       np.ascontiguousarray(out_idx, np.int32), np.ascontiguousarray(in_ptr, np.int32),
-                                      np.ascontiguousarray(in_idx, np.int32), device, err, 512)
+                                      np.ascontiguousarray(in_idx, np.int32), device, int(num_edges), err, 512)
         if not self.h:
             raise RuntimeError("sbp_graph_new: " + err.value.decode())
 

@@ -16,7 +16,7 @@ ARCH = os.environ.get("SBPX_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "merge.hip", "mlp.hip", "learned.hip", "api.cpp",
                   "generator.cpp"]
-HOST_SOURCES = ["sbp_host.cpp", "capi.cpp"]
+HOST_SOURCES = ["sbp_host.cpp", "graph_io.cpp", "capi.cpp"]
 
 # No FMA contraction (the objective's FP32 cell chain and FP64 sums must match the reference's
 # uncontracted arithmetic); packed FP32 instructions off.

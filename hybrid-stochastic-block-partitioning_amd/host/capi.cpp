@@ -1,4 +1,5 @@
 // Flat entry point over the C++ driver, for Python callers (bench.py, tests).
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -28,13 +29,14 @@ static void set_err(char* err, int errlen, const char* msg) {
 }
 
 // A graph resident on one GPU, for repeated partitioning runs.
+// num_edges < 0: the stored edges count in the MDL; otherwise that count (see Graph).
 void* sbp_graph_new(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, const int32_t* in_ptr,
-                    const int32_t* in_idx, int device, char* err, int errlen) {
+                    const int32_t* in_idx, int device, int64_t num_edges, char* err, int errlen) {
     try {
         int64_t E = out_ptr[N];
         return new Graph(N, std::vector<int32_t>(out_ptr, out_ptr + N + 1), std::vector<int32_t>(out_idx, out_idx + E),
                          std::vector<int32_t>(in_ptr, in_ptr + N + 1), std::vector<int32_t>(in_idx, in_idx + E),
-                         device);
+                         device, (long)num_edges);
     } catch (const std::exception& e) {
         set_err(err, errlen, e.what());
         return nullptr;
@@ -42,6 +44,34 @@ void* sbp_graph_new(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, c
 }
 
 void sbp_graph_free(void* g) { delete (Graph*)g; }
+
+// Graph files as the reference reads them (graph_io.cpp): load, sizes, arrays, free.
+void* sbp_file_load(const char* base_path, int undirected, char* err, int errlen) {
+    try {
+        return new graph_io::LoadedGraph(graph_io::load(base_path, undirected != 0));
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+void sbp_file_info(void* h, int32_t* num_vertices, int64_t* stored_edges, int64_t* num_edges, int32_t* undirected,
+                   int32_t* truth_len) {
+    const graph_io::LoadedGraph& g = *(graph_io::LoadedGraph*)h;
+    if (num_vertices) *num_vertices = g.num_vertices;
+    if (stored_edges) *stored_edges = (int64_t)g.out_idx.size();
+    if (num_edges) *num_edges = g.num_edges;
+    if (undirected) *undirected = g.undirected ? 1 : 0;
+    if (truth_len) *truth_len = (int32_t)g.truth.size();
+}
+void sbp_file_arrays(void* h, int32_t* out_ptr, int32_t* out_idx, int32_t* in_ptr, int32_t* in_idx, int32_t* truth) {
+    const graph_io::LoadedGraph& g = *(graph_io::LoadedGraph*)h;
+    std::copy(g.out_ptr.begin(), g.out_ptr.end(), out_ptr);
+    std::copy(g.out_idx.begin(), g.out_idx.end(), out_idx);
+    std::copy(g.in_ptr.begin(), g.in_ptr.end(), in_ptr);
+    std::copy(g.in_idx.begin(), g.in_idx.end(), in_idx);
+    if (truth) std::copy(g.truth.begin(), g.truth.end(), truth);
+}
+void sbp_file_free(void* h) { delete (graph_io::LoadedGraph*)h; }
 
 // The last run's outer iterations (sbp::intermediate_results): wall seconds since the start,
 // MDL and number of blocks after each merge + MCMC phase. Returns how many there are; fills at
@@ -120,7 +150,7 @@ int sbp_partition(void* gp, int algorithm, int32_t batches, int approximate, uin
 int sbp_run(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, const int32_t* in_ptr, const int32_t* in_idx,
             int algorithm, int32_t batches, int approximate, uint64_t seed, int device, int count_bytes,
             int32_t* assignment, sbp_run_stats* stats, char* err, int errlen) {
-    void* g = sbp_graph_new(N, out_ptr, out_idx, in_ptr, in_idx, device, err, errlen);
+    void* g = sbp_graph_new(N, out_ptr, out_idx, in_ptr, in_idx, device, -1, err, errlen);
     if (!g) return 1;
     int rc = sbp_partition(g, algorithm, batches, approximate, seed, count_bytes, assignment, stats, err, errlen);
     sbp_graph_free(g);

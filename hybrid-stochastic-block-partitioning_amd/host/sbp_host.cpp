@@ -28,16 +28,16 @@ using sbpx_host::Device;
 
 // ---------------------------------------------------------------------------- Graph
 Graph::Graph(int num_vertices, std::vector<int32_t> out_ptr, std::vector<int32_t> out_idx, std::vector<int32_t> in_ptr,
-             std::vector<int32_t> in_idx, int device) {
+             std::vector<int32_t> in_idx, int device, long num_edges) {
     num_vertices_ = num_vertices;
-    num_edges_ = (int)out_idx.size();
+    num_edges_ = num_edges < 0 ? (int)out_idx.size() : (int)num_edges;
     dev_ = std::make_shared<Device>();
     check(nullptr, sbpx_create(&dev_->ctx, device), "sbpx_create");
     dev_->N = num_vertices_;
     dev_->E = num_edges_;
     check(dev_->ctx,
-          sbpx_graph_upload(dev_->ctx, num_vertices, num_edges_, out_ptr.data(), out_idx.data(), in_ptr.data(),
-                            in_idx.data()),
+          sbpx_graph_upload(dev_->ctx, num_vertices, (int64_t)out_idx.size(), out_ptr.data(), out_idx.data(),
+                            in_ptr.data(), in_idx.data()),
           "sbpx_graph_upload");
     int nh = 0;
     check(dev_->ctx, sbpx_graph_split(dev_->ctx, nullptr, &nh, nullptr), "sbpx_graph_split");
@@ -169,13 +169,16 @@ Blockmodel BlockmodelTriplet::get_next_blockmodel(Blockmodel& old_blockmodel) {
 
 // ---------------------------------------------------------------------------- entropy
 namespace entropy {
+// entropy::mdl (entropy.cpp:584-589): the model term for (num_vertices, num_edges, B) on the host,
+// minus the device's log posterior.
 double mdl(const Blockmodel& blockmodel, int num_vertices, int num_edges) {
-    (void)num_vertices;
-    (void)num_edges;
     blockmodel.make_live();
-    double out = 0.0;
-    check(blockmodel.ctx(), sbpx_mdl(blockmodel.ctx(), &out), "sbpx_mdl");
-    return out;
+    double log_posterior = 0.0;
+    check(blockmodel.ctx(), sbpx_log_posterior(blockmodel.ctx(), &log_posterior), "sbpx_log_posterior");
+    const double B = blockmodel.getNum_blocks();
+    const double x = pow(B, 2) / num_edges;
+    const double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    return (num_edges * h) + (num_vertices * log(B)) - log_posterior;
 }
 double null_mdl_v1(long num_edges) {
     double log_posterior_p = num_edges * log(1.0 / num_edges);

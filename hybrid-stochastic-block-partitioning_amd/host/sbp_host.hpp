@@ -29,11 +29,28 @@ namespace sbpx_host {
 struct Device;  // one context per graph
 }
 
+namespace graph_io {
+/// A graph read as the reference reads it (see graph_io.cpp).
+struct LoadedGraph {
+    int num_vertices = 0;
+    long num_edges = 0;  // the count the reference's MDL uses
+    bool undirected = false;
+    std::vector<int32_t> out_ptr, out_idx, in_ptr, in_idx;
+    std::vector<int> truth;  // planted blocks (0-based; -1 where the file names none)
+};
+/// Reads base_path + ".tsv" (else ".mtx") and base_path + "_truePartition.tsv" (graph.cpp:24-62).
+LoadedGraph load(const std::string& base_path, bool undirected);
+}  // namespace graph_io
+
 class Graph {
 public:
     Graph() = default;
+    /// num_edges < 0: the number of stored edges; otherwise the count the MDL uses (as a loaded
+    /// MatrixMarket or undirected graph reports it).
     Graph(int num_vertices, std::vector<int32_t> out_ptr, std::vector<int32_t> out_idx, std::vector<int32_t> in_ptr,
-          std::vector<int32_t> in_idx, int device = 0);
+          std::vector<int32_t> in_idx, int device = 0, long num_edges = -1);
+    /// Graph::load (graph.cpp:24-62) from a path without extension.
+    static Graph load(const std::string& base_path, bool undirected = false, int device = 0);
     int num_vertices() const { return num_vertices_; }
     int num_edges() const { return num_edges_; }
     const std::vector<int>& high_degree_vertices() const { return high_; }

@@ -120,6 +120,8 @@ def load_host():
     H.sbp_graph_free.argtypes = [C.c_void_p]
     H.sbp_graph_free.restype = None
     H.sbp_last_trajectory.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    H.sbp_partition_checkpointed.argtypes = [C.c_void_p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int, C.c_char_p,
+                                             C.c_int, C.c_int32, i32p, C.POINTER(RunStats), C.c_char_p, C.c_int]
     H.sbp_graph_option.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_char_p, C.c_int]
     H.sbp_graph_learned.argtypes = [C.c_void_p, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS"),
                                     C.c_char_p, C.c_int]
@@ -397,14 +399,17 @@ class ResidentGraph:
             raise RuntimeError("sbp_graph_learned: " + err.value.decode())
 
     def partition(self, algorithm="async_gibbs", batches=1, approximate=True, seed=0, count_bytes=False,
-                  learned=False):
+                  learned=False, checkpoint=None, resume=False, max_iterations=0):
+        """sbp::stochastic_block_partition; with `checkpoint`, the driver state is written after every
+        outer iteration, `resume` continues from it and `max_iterations` > 0 stops early."""
         self.set_option(OPT_PROPOSAL, 1 if learned else 0)
         a = np.zeros(self.N, np.int32)
         st = RunStats()
         err = C.create_string_buffer(512)
         algo = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
-        rc = self.H.sbp_partition(self.h, algo, batches, int(approximate), seed, int(count_bytes), a, C.byref(st),
-                                  err, 512)
+        ck = None if checkpoint is None else str(checkpoint).encode()
+        rc = self.H.sbp_partition_checkpointed(self.h, algo, batches, int(approximate), seed, int(count_bytes), ck,
+                                               int(resume), int(max_iterations), a, C.byref(st), err, 512)
         if rc != 0:
             raise RuntimeError("sbp_partition: " + err.value.decode())
         return a, st

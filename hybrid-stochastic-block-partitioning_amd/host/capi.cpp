@@ -106,10 +106,24 @@ int sbp_graph_learned(void* gp, const float* params, char* err, int errlen) {
     return 0;
 }
 
+int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int approximate, uint64_t seed,
+                               int count_bytes, const char* checkpoint, int resume, int32_t max_iterations,
+                               int32_t* assignment, sbp_run_stats* stats, char* err, int errlen);
+
 // sbp::stochastic_block_partition on a resident graph. algorithm: 0 = metropolis_hastings,
 // 1 = async_gibbs, 2 = hybrid_mcmc. Returns 0 on success, otherwise writes a message into err.
 int sbp_partition(void* gp, int algorithm, int32_t batches, int approximate, uint64_t seed, int count_bytes,
                   int32_t* assignment, sbp_run_stats* stats, char* err, int errlen) {
+    return sbp_partition_checkpointed(gp, algorithm, batches, approximate, seed, count_bytes, nullptr, 0, 0, assignment,
+                                      stats, err, errlen);
+}
+
+// As sbp_partition, writing the driver state to `checkpoint` after every outer iteration (when
+// not null), continuing from it when `resume` is set, and stopping after `max_iterations` outer
+// iterations when that is positive.
+int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int approximate, uint64_t seed,
+                               int count_bytes, const char* checkpoint, int resume, int32_t max_iterations,
+                               int32_t* assignment, sbp_run_stats* stats, char* err, int errlen) {
     try {
         Graph& graph = *(Graph*)gp;
         const int N = graph.num_vertices();
@@ -119,6 +133,9 @@ int sbp_partition(void* gp, int algorithm, int32_t batches, int approximate, uin
         a.batches = batches;
         a.approximate = approximate != 0;
         a.seed = seed;
+        if (checkpoint) a.checkpoint = checkpoint;
+        a.resume = resume != 0;
+        a.max_iterations = max_iterations;
         auto t0 = std::chrono::steady_clock::now();
         Blockmodel bm = sbp::stochastic_block_partition(graph, a);
         double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -167,6 +167,62 @@ Blockmodel BlockmodelTriplet::get_next_blockmodel(Blockmodel& old_blockmodel) {
     return blockmodel;
 }
 
+// ---------------------------------------------------------------------------- checkpoints
+namespace {
+template <class T>
+void put(std::FILE* f, const T& v) {
+    if (std::fwrite(&v, sizeof(T), 1, f) != 1) throw std::runtime_error("checkpoint write failed");
+}
+template <class T>
+T take(std::FILE* f) {
+    T v;
+    if (std::fread(&v, sizeof(T), 1, f) != 1) throw std::runtime_error("checkpoint truncated");
+    return v;
+}
+const uint64_t CHECKPOINT_MAGIC = 0x3150434B50425353ull;  // "SSBPKCP1"
+}  // namespace
+
+void Blockmodel::write(std::FILE* f) const {
+    put<int32_t>(f, empty ? 1 : 0);
+    if (empty) return;
+    put<int32_t>(f, num_blocks);
+    put<int32_t>(f, num_blocks_to_merge);
+    put<float>(f, block_reduction_rate);
+    put<float>(f, overall_entropy);
+    const std::vector<int> a = block_assignment();
+    put<int64_t>(f, (int64_t)a.size());
+    if (!a.empty() && std::fwrite(a.data(), sizeof(int), a.size(), f) != a.size())
+        throw std::runtime_error("checkpoint write failed");
+}
+
+Blockmodel Blockmodel::read(std::FILE* f, const Graph& graph) {
+    Blockmodel b;
+    if (take<int32_t>(f)) return b;
+    b.empty = false;
+    b.dev_ = graph.device();
+    b.id_ = b.dev_->next_id++;
+    b.num_blocks = take<int32_t>(f);
+    b.num_blocks_to_merge = take<int32_t>(f);
+    b.block_reduction_rate = take<float>(f);
+    b.overall_entropy = take<float>(f);
+    const int64_t n = take<int64_t>(f);
+    if (n != graph.num_vertices()) throw std::runtime_error("checkpoint is for another graph");
+    b.snapshot_.resize(n);
+    if (n > 0 && std::fread(b.snapshot_.data(), sizeof(int), n, f) != (size_t)n)
+        throw std::runtime_error("checkpoint truncated");
+    return b;
+}
+
+void BlockmodelTriplet::write(std::FILE* f) const {
+    put<int32_t>(f, optimal_num_blocks_found ? 1 : 0);
+    for (const Blockmodel& b : blockmodels) b.write(f);
+}
+
+void BlockmodelTriplet::read(std::FILE* f, const Graph& graph) {
+    optimal_num_blocks_found = take<int32_t>(f) != 0;
+    for (Blockmodel& b : blockmodels) b = Blockmodel::read(f, graph);
+}
+
 // ---------------------------------------------------------------------------- entropy
 namespace entropy {
 // entropy::mdl (entropy.cpp:584-589): the model term for (num_vertices, num_edges, B) on the host,
@@ -329,12 +385,35 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
     Device& dev = *graph.device();
     dev.sweep_call = 0;
     dev.merge_call = 0;
-    Blockmodel blockmodel(graph.num_vertices(), graph, float(BLOCK_REDUCTION_RATE));
-    double initial_mdl = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
-    add_intermediate(0, blockmodel, initial_mdl);
+    Blockmodel blockmodel;
     BlockmodelTriplet blockmodel_triplet = BlockmodelTriplet();
     float iteration = 0;
+    std::FILE* cf = (args.resume && !args.checkpoint.empty()) ? std::fopen(args.checkpoint.c_str(), "rb") : nullptr;
+    if (cf) {  // continue a checkpointed run where it stopped
+        try {
+            if (take<uint64_t>(cf) != CHECKPOINT_MAGIC) throw std::runtime_error("not a checkpoint file");
+            if (take<int64_t>(cf) != graph.num_vertices() || take<int64_t>(cf) != graph.num_edges())
+                throw std::runtime_error("checkpoint is for another graph");
+            iteration = take<float>(cf);
+            dev.sweep_call = take<uint32_t>(cf);
+            dev.merge_call = take<uint32_t>(cf);
+            finetune::MCMC_iterations = take<int32_t>(cf);
+            finetune::MCMC_time = take<double>(cf);
+            blockmodel_triplet.read(cf, graph);
+            blockmodel = Blockmodel::read(cf, graph);
+        } catch (...) {
+            std::fclose(cf);
+            throw;
+        }
+        std::fclose(cf);
+    } else {
+        blockmodel = Blockmodel(graph.num_vertices(), graph, float(BLOCK_REDUCTION_RATE));
+        double initial_mdl = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
+        add_intermediate(0, blockmodel, initial_mdl);
+    }
+    int iterations_this_call = 0;
     while (!done_blockmodeling(blockmodel, blockmodel_triplet)) {
+        if (args.max_iterations > 0 && iterations_this_call++ >= args.max_iterations) break;
         blockmodel = block_merge::merge_blocks(blockmodel, graph, graph.num_edges());
         auto start = std::chrono::steady_clock::now();
         if (args.algorithm == "async_gibbs" && iteration < float(args.asynciterations))
@@ -350,6 +429,28 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
             std::fprintf(stderr, "sbp iter %.0f: B=%d mdl=%.3f mcmc_iterations=%d\n", iteration,
                          blockmodel.getNum_blocks(), (double)blockmodel.getOverall_entropy(), finetune::MCMC_iterations);
         blockmodel = blockmodel_triplet.get_next_blockmodel(blockmodel);
+        if (!args.checkpoint.empty()) {  // written to a temporary file, then renamed over the old one
+            const std::string tmp = args.checkpoint + ".tmp";
+            std::FILE* f = std::fopen(tmp.c_str(), "wb");
+            if (!f) throw std::runtime_error("cannot write " + tmp);
+            try {
+                put<uint64_t>(f, CHECKPOINT_MAGIC);
+                put<int64_t>(f, graph.num_vertices());
+                put<int64_t>(f, graph.num_edges());
+                put<float>(f, iteration);
+                put<uint32_t>(f, dev.sweep_call);
+                put<uint32_t>(f, dev.merge_call);
+                put<int32_t>(f, finetune::MCMC_iterations);
+                put<double>(f, finetune::MCMC_time);
+                blockmodel_triplet.write(f);
+                blockmodel.write(f);
+            } catch (...) {
+                std::fclose(f);
+                throw;
+            }
+            if (std::fclose(f) != 0 || std::rename(tmp.c_str(), args.checkpoint.c_str()) != 0)
+                throw std::runtime_error("cannot write " + args.checkpoint);
+        }
     }
     add_intermediate(-1, blockmodel, blockmodel.getOverall_entropy());
     return blockmodel;

@@ -5,6 +5,7 @@
 // on one GPU. Errors raise std::runtime_error with the library's message.
 #pragma once
 #include <climits>
+#include <cstdio>
 #include <cstdint>
 #include <limits>
 #include <memory>
@@ -23,6 +24,12 @@ struct Args {
     int threads = 0;        // unused on the device path
     uint64_t seed = 0;
     int device = 0;
+    // checkpointing of the outer loop (no counterpart in the reference): after every outer
+    // iteration the driver state is written to `checkpoint` (when set); with `resume` a run
+    // continues from that file; `max_iterations` > 0 stops after that many outer iterations.
+    std::string checkpoint;
+    bool resume = false;
+    int max_iterations = 0;
 };
 
 namespace sbpx_host {
@@ -88,6 +95,9 @@ public:
     void make_live() const;
     sbpx_ctx* ctx() const;
     void mark_live() const;
+    // checkpoint I/O of the fields and the assignment (the device state is rebuilt when used)
+    void write(std::FILE* f) const;
+    static Blockmodel read(std::FILE* f, const Graph& graph);
 
 private:
     std::shared_ptr<sbpx_host::Device> dev_;
@@ -108,6 +118,9 @@ public:
     bool golden_ratio_not_reached();
     bool is_done();
     Blockmodel get_next_blockmodel(Blockmodel& old_blockmodel);
+
+    void write(std::FILE* f) const;
+    void read(std::FILE* f, const Graph& graph);
 
 protected:
     Blockmodel blockmodels[3];

@@ -92,6 +92,17 @@ def load():
     L.sbpx_learned_forward.argtypes = [vp, C.c_int32, f32p, f32p]
     L.sbpx_learned_gradients.argtypes = [vp, C.c_int32, f32p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                          i32p, f32p, f32p]
+    L.sbpx_batch_define.argtypes = [vp, C.c_int32, i32p]
+    L.sbpx_batch_info.argtypes = [vp, C.POINTER(C.c_int32), vp]
+    L.sbpx_batch_build.argtypes = [vp, C.c_int32, i32p, i32p, i32p]
+    L.sbpx_batch_download.argtypes = [vp, i32p]
+    L.sbpx_batch_log_posterior.argtypes = [vp, f64p]
+    L.sbpx_batch_sweep.argtypes = [vp, C.c_int, C.c_int32, C.c_int32, i32p,
+                                   np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
+                                   np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), C.POINTER(SweepStats)]
+    L.sbpx_batch_merge.argtypes = [vp, C.c_int32, i32p, i32p, C.c_int,
+                                   np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
+                                   np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
     _lib = L
     return L
 
@@ -127,6 +138,9 @@ def load_host():
                                     C.c_char_p, C.c_int]
     H.sbp_partition.argtypes = [C.c_void_p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int, i32p,
                                 C.POINTER(RunStats), C.c_char_p, C.c_int]
+    H.sbp_partition_batch.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_void_p, C.c_int32,
+                                      np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"), C.c_int, C.c_int,
+                                      i32p, i32p, f64p, i32p, f64p, C.c_char_p, C.c_int]
     _host = H
     return H
 
@@ -137,7 +151,9 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_apply_moves", "sbpx_sweep", "sbpx_merge_proposals", "sbpx_carry_out_merges",
             "sbpx_merge_blocks", "sbpx_selftest_logf", "sbpx_generate_dcsbm", "sbpx_learned_init",
             "sbpx_learned_params", "sbpx_learned_log", "sbpx_learned_log_count", "sbpx_learned_train",
-            "sbpx_learned_forward", "sbpx_learned_gradients"]
+            "sbpx_learned_forward", "sbpx_learned_gradients", "sbpx_batch_define", "sbpx_batch_info",
+            "sbpx_batch_build", "sbpx_batch_download", "sbpx_batch_log_posterior", "sbpx_batch_sweep",
+            "sbpx_batch_merge"]
 
 
 class SbpxError(RuntimeError):
@@ -289,6 +305,45 @@ class Context:
     def merge_blocks(self, num_to_merge, seed, call, mode=MERGE_APPROXIMATE):
         self._chk(self.L.sbpx_merge_blocks(self.h, int(num_to_merge), mode, seed, call))
 
+    # a batch of graphs in this context (the uploaded graph is their union)
+    def batch_define(self, vertex_offsets):
+        offs = np.ascontiguousarray(vertex_offsets, np.int32)
+        self._chk(self.L.sbpx_batch_define(self.h, len(offs) - 1, offs))
+
+    def batch_info(self):
+        G = C.c_int32()
+        self._chk(self.L.sbpx_batch_info(self.h, C.byref(G), None))
+        nb = np.zeros(G.value, np.int32)
+        self._chk(self.L.sbpx_batch_info(self.h, None, nb.ctypes.data))
+        return nb
+
+    def batch_build(self, graphs, num_blocks, assignment):
+        g = np.ascontiguousarray(graphs, np.int32)
+        self._chk(self.L.sbpx_batch_build(self.h, len(g), g, np.ascontiguousarray(num_blocks, np.int32),
+                                          np.ascontiguousarray(assignment, np.int32)))
+
+    def batch_download(self):
+        a = np.zeros(self.N, np.int32)
+        self._chk(self.L.sbpx_batch_download(self.h, a))
+        return a
+
+    def batch_log_posterior(self):
+        out = np.zeros(len(self.batch_info()))
+        self._chk(self.L.sbpx_batch_log_posterior(self.h, out))
+        return out
+
+    def batch_sweep(self, graphs, seeds, calls, batches=1, algo=ALGO_ASYNC_GIBBS):
+        g = np.ascontiguousarray(graphs, np.int32)
+        st = SweepStats()
+        self._chk(self.L.sbpx_batch_sweep(self.h, algo, batches, len(g), g, np.ascontiguousarray(seeds, np.uint64),
+                                          np.ascontiguousarray(calls, np.uint32), C.byref(st)))
+        return st
+
+    def batch_merge(self, graphs, num_to_merge, seeds, calls, mode=MERGE_APPROXIMATE):
+        g = np.ascontiguousarray(graphs, np.int32)
+        self._chk(self.L.sbpx_batch_merge(self.h, len(g), g, np.ascontiguousarray(num_to_merge, np.int32), mode,
+                                          np.ascontiguousarray(seeds, np.uint64), np.ascontiguousarray(calls, np.uint32)))
+
     # proposal network
     def learned_init(self, seed=0):
         self._chk(self.L.sbpx_learned_init(self.h, seed))
@@ -369,6 +424,48 @@ def load_graph(base_path, undirected=False):
         H.sbp_file_free(h)
     return dict(out_ptr=op, out_idx=oi, in_ptr=ip, in_idx=ii, truth=truth, num_edges=E.value,
                 undirected=bool(und.value))
+
+
+def union_graphs(graphs):
+    """Joins graphs given as (out_ptr, out_idx, in_ptr, in_idx) into one graph without edges between
+    them: returns (vertex_offsets, out_ptr, out_idx, in_ptr, in_idx)."""
+    offs, ops, ois, ips, iis = [0], [np.zeros(1, np.int64)], [], [np.zeros(1, np.int64)], []
+    eo = 0
+    for op, oi, ip, ii in graphs:
+        n, v0 = len(op) - 1, offs[-1]
+        ops.append(np.asarray(op[1:], np.int64) + eo)
+        ips.append(np.asarray(ip[1:], np.int64) + eo)
+        ois.append(np.asarray(oi, np.int64) + v0)
+        iis.append(np.asarray(ii, np.int64) + v0)
+        eo += int(op[-1])
+        offs.append(v0 + n)
+    cat = lambda xs: np.ascontiguousarray(np.concatenate(xs), np.int32)
+    return np.array(offs, np.int32), cat(ops), cat(ois), cat(ips), cat(iis)
+
+
+def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eval_variant=0):
+    """sbp::partition_batch: each graph of the list (out_ptr, out_idx, in_ptr, in_idx) partitioned by
+    async_gibbs with approximate merges, all on one GPU at once; graph g runs as it would alone with
+    seed seeds[g] (eval_variant as OPT_EVAL_VARIANT). Returns (list of (assignment, num_blocks, mdl,
+    mcmc_iterations), totals) with
+    totals = dict(seconds, rounds, sweeps, evals)."""
+    H = load_host()
+    G = len(graphs)
+    offs, op, oi, ip, ii = union_graphs(graphs)
+    seeds = np.ascontiguousarray(np.zeros(G) if seeds is None else seeds, np.uint64)
+    ne = None if num_edges is None else np.ascontiguousarray(num_edges, np.int64)
+    a = np.zeros(int(offs[-1]), np.int32)
+    B = np.zeros(G, np.int32)
+    mdl = np.zeros(G)
+    its = np.zeros(G, np.int32)
+    tot = np.zeros(4)
+    err = C.create_string_buffer(512)
+    rc = H.sbp_partition_batch(G, offs, op, oi, ip, ii, None if ne is None else ne.ctypes.data, batches, seeds, device,
+                               int(eval_variant), a, B, mdl, its, tot, err, 512)
+    if rc != 0:
+        raise RuntimeError("sbp_partition_batch: " + err.value.decode())
+    out = [(a[offs[g]:offs[g + 1]].copy(), int(B[g]), float(mdl[g]), int(its[g])) for g in range(G)]
+    return out, dict(seconds=tot[0], rounds=int(tot[1]), sweeps=int(tot[2]), evals=int(tot[3]))
 
 
 class ResidentGraph:

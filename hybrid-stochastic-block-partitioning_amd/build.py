@@ -16,7 +16,7 @@ ARCH = os.environ.get("SBPX_ARCH", "gfx950")
 
 DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "merge.hip", "mlp.hip", "learned.hip", "api.cpp",
                   "generator.cpp"]
-HOST_SOURCES = ["sbp_host.cpp", "graph_io.cpp", "capi.cpp"]
+HOST_SOURCES = ["sbp_host.cpp", "graph_io.cpp", "batch.cpp", "capi.cpp"]
 
 # No FMA contraction (the objective's FP32 cell chain and FP64 sums must match the reference's
 # uncontracted arithmetic); packed FP32 instructions off.
@@ -42,7 +42,7 @@ def build_host():
     if not srcs:
         return
     tmp = HOST_LIB + ".tmp"
-    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", f"-I{os.path.join(ROOT, 'include')}",
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-ffp-contract=off", f"-I{os.path.join(ROOT, 'include')}",
          f"-I{HOST}"] + srcs + ["-o", tmp, f"-L{HERE}", "-lsbpx", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, HOST_LIB)
 

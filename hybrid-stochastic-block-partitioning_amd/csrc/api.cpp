@@ -55,7 +55,10 @@ void dfree(T** p) {
     *p = nullptr;
 }
 
+void free_batch(Ctx& c);
+
 void free_graph_buffers(Ctx& c) {
+    free_batch(c);
     dfree(&c.g.out_ptr);
     dfree(&c.g.out_idx);
     dfree(&c.g.in_ptr);
@@ -268,7 +271,22 @@ void require_graph(Ctx& c) {
 }
 void require_bm(Ctx& c) {
     require_graph(c);
+    if (c.batch.G > 0) throw StateError("the context holds a batch of graphs: use the sbpx_batch_* calls");
     if (!c.bm.valid) throw StateError("no blockmodel built");
+}
+void require_batch(Ctx& c) {
+    require_graph(c);
+    if (c.batch.G <= 0) throw StateError("no batch of graphs defined");
+}
+
+void free_batch(Ctx& c) {
+    DevBatch& b = c.batch;
+    for (int** p : {&b.d_vgraph, &b.d_bgraph, &b.d_vo, &b.d_bo, &b.d_nb, &b.d_req, &b.d_plan, &b.d_plan_huge})
+        dfree(p);
+    dfree(&b.d_seed);
+    dfree(&b.d_call);
+    dfree(&b.d_active);
+    b = DevBatch();
 }
 
 double model_term(long N, long E, int B) {
@@ -324,6 +342,36 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
         L.bytes = bytes ? bytes + bin_class(b) : nullptr;
         launch_eval(c, L);
         if (timed) c.timer_end(bin_class(b), b, L.variant, L.n);
+    }
+}
+
+// Reads back a sweep's counters and event timings into stats and feeds the variant tuner.
+void sweep_stats(sbpx_ctx* ctx, sbpx_sweep_stats* stats) {
+    Ctx& c = ctx->c;
+    long long cnt[8];
+    double dsum = 0.0;
+    SBPX_HIP(hipMemcpyAsync(cnt, c.counters, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
+    SBPX_HIP(hipMemcpyAsync(&dsum, c.red + 2048, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    check_device_error(c);
+    std::vector<float> launch_ms(c.timed.size(), 0.0f);
+    for (size_t i = 0; i < c.timed.size(); ++i) {
+        SBPX_HIP(hipEventElapsedTime(&launch_ms[i], c.events[2 * i], c.events[2 * i + 1]));
+        const Ctx::TimedBin& tb = c.timed_bins[i];
+        c.record_variant(tb.bin, tb.variant, tb.n, launch_ms[i]);
+    }
+    if (stats) {
+        *stats = sbpx_sweep_stats{};
+        for (size_t i = 0; i < c.timed.size(); ++i) {
+            stats->eval_ms[c.timed[i]] += launch_ms[i];
+            stats->eval_launches[c.timed[i]] += 1;
+        }
+        stats->attempts = cnt[0];
+        stats->evals = cnt[1];
+        stats->moves = cnt[2];
+        stats->dS_accepted = dsum;
+        for (int b = 0; b < NUM_KERNEL_CLASSES; ++b)
+            stats->eval_bytes[b] = ctx->count_bytes ? (double)cnt[4 + b] : 0.0;
     }
 }
 
@@ -420,6 +468,8 @@ void sbpx_destroy(sbpx_ctx* ctx) {
     dfree(&ctx->c.dev_err);
     dfree(&ctx->c.counters);
     dfree(&ctx->c.red);
+    dfree(&ctx->c.lp_part);
+    dfree(&ctx->c.lp_seg);
     for (hipEvent_t e : ctx->c.events) (void)hipEventDestroy(e);
     if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
@@ -600,6 +650,7 @@ sbpx_status sbpx_blockmodel_build(sbpx_ctx* ctx, int32_t B, const int32_t* assig
     return guarded(ctx, [&] {
         Ctx& c = ctx->c;
         require_graph(c);
+        if (c.batch.G > 0) throw StateError("the context holds a batch of graphs: use the sbpx_batch_* calls");
         if (B < 1 || B > c.g.N) throw ArgError("num_blocks must be in [1, N]");
         if (!assignment) throw ArgError("null assignment");
         for (int v = 0; v < c.g.N; ++v)
@@ -855,31 +906,7 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
             if (c.mlp.logging && !learned) learned_log(c, d_list, bl.count[0] + bl.count[1] + bl.count[2]);
             apply_accepted(c, d_list, bl.total);
         }
-        long long cnt[8];
-        double dsum = 0.0;
-        SBPX_HIP(hipMemcpyAsync(cnt, c.counters, sizeof(cnt), hipMemcpyDeviceToHost, c.stream));
-        SBPX_HIP(hipMemcpyAsync(&dsum, c.red + 2048, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-        c.sync();
-        check_device_error(c);
-        std::vector<float> launch_ms(c.timed.size(), 0.0f);
-        for (size_t i = 0; i < c.timed.size(); ++i) {
-            SBPX_HIP(hipEventElapsedTime(&launch_ms[i], c.events[2 * i], c.events[2 * i + 1]));
-            const Ctx::TimedBin& tb = c.timed_bins[i];
-            c.record_variant(tb.bin, tb.variant, tb.n, launch_ms[i]);
-        }
-        if (stats) {
-            *stats = sbpx_sweep_stats{};
-            for (size_t i = 0; i < c.timed.size(); ++i) {
-                stats->eval_ms[c.timed[i]] += launch_ms[i];
-                stats->eval_launches[c.timed[i]] += 1;
-            }
-            stats->attempts = cnt[0];
-            stats->evals = cnt[1];
-            stats->moves = cnt[2];
-            stats->dS_accepted = dsum;
-            for (int b = 0; b < NUM_KERNEL_CLASSES; ++b)
-                stats->eval_bytes[b] = ctx->count_bytes ? (double)cnt[4 + b] : 0.0;
-        }
+        sweep_stats(ctx, stats);
     });
 }
 
@@ -895,71 +922,80 @@ sbpx_status sbpx_merge_proposals(sbpx_ctx* ctx, uint64_t seed, uint32_t call, do
     });
 }
 
+static int uf_root(std::vector<int>& parent, int x) {
+    int root = x;
+    while (parent[root] != root) root = parent[root];
+    while (parent[x] != root) {
+        int nx = parent[x];
+        parent[x] = root;
+        x = nx;
+    }
+    return root;
+}
+
+// Blockmodel::carry_out_best_merges (blockmodel.cpp:50-78) over blocks [0, B) of one graph:
+// candidates by ascending dS (utils::sort_indices, utils.hpp:94-101: std::sort with the same
+// comparator, so ties fall in the same order as the reference's). block_map always names the
+// block a block has been merged into; a union-find with the merge target as root keeps the same
+// mapping in near-linear time.
+static void select_merges_sorted(int B, int num_to_merge, const double* dS, const int32_t* best,
+                                 std::vector<int>& parent) {
+    std::vector<int> order(B);
+    std::iota(order.begin(), order.end(), 0);
+    std::vector<double> d(dS, dS + B);
+    std::sort(order.data(), order.data() + order.size(), [&d](size_t i1, size_t i2) { return d[i1] < d[i2]; });
+    int num_merged = 0;
+    size_t counter = 0;
+    while (num_merged < num_to_merge) {
+        if (counter >= order.size()) throw StateError("ran out of merge candidates");
+        int from = order[counter];
+        int b = best[from];
+        if (b < 0 || b >= B) throw StateError("block without a merge proposal");
+        int to = uf_root(parent, b);
+        counter++;
+        if (to != from) {
+            parent[from] = to;
+            num_merged++;
+        }
+    }
+}
+
+// build_mapping (blockmodel.cpp:29-41): surviving labels of non-empty blocks, ascending, from
+// `base` on; map[x] for the blocks [0, B) of one graph.
+static void merge_mapping(int B, int num_to_merge, const int* count, std::vector<int>& parent, int base, int* map) {
+    std::vector<char> used(B, 0);
+    for (int x = 0; x < B; ++x)
+        if (count[x] > 0) used[uf_root(parent, x)] = 1;
+    std::vector<int> rank(B, -1);
+    int next = 0;
+    for (int x = 0; x < B; ++x)
+        if (used[x]) rank[x] = next++;
+    if (next > B - num_to_merge) throw StateError("more surviving labels than blocks after the merge");
+    for (int x = 0; x < B; ++x) map[x] = base + rank[uf_root(parent, x)];
+}
+
 static void carry_out(Ctx& c, int num_to_merge, int mode, const double* dS, const int32_t* best) {
     const int B = c.bm.B;
     if (num_to_merge < 0 || num_to_merge >= B) throw ArgError("num_to_merge must be in [0, B)");
     if (mode != SBPX_MERGE_APPROXIMATE && mode != SBPX_MERGE_ADVANCED) throw ArgError("unknown merge mode");
     std::vector<int> parent(B);
     std::iota(parent.begin(), parent.end(), 0);
-    auto find = [&parent](int x) {
-        int root = x;
-        while (parent[root] != root) root = parent[root];
-        while (parent[x] != root) {
-            int nx = parent[x];
-            parent[x] = root;
-            x = nx;
-        }
-        return root;
-    };
     if (mode == SBPX_MERGE_ADVANCED) {
         for (int x = 0; x < B; ++x)
             if (best[x] >= B) throw ArgError("proposed block out of range [0, B)");
         merge_heap_select(c, num_to_merge, dS, best, parent);
     } else {
-        // utils::sort_indices (utils.hpp:94-101): std::sort ascending, same comparator, so ties
-        // fall in the same order as the reference's.
-        std::vector<int> order(B);
-        std::iota(order.begin(), order.end(), 0);
-        std::vector<double> d(dS, dS + B);
-        std::sort(order.data(), order.data() + order.size(),
-                  [&d](size_t i1, size_t i2) { return d[i1] < d[i2]; });
-        // Blockmodel::carry_out_best_merges (blockmodel.cpp:50-78): block_map always names the
-        // block a block has been merged into; a union-find with the merge target as root keeps
-        // the same mapping in near-linear time.
-        int num_merged = 0;
-        size_t counter = 0;
-        while (num_merged < num_to_merge) {
-            if (counter >= order.size()) throw StateError("ran out of merge candidates");
-            int from = order[counter];
-            int b = best[from];
-            if (b < 0 || b >= B) throw StateError("block without a merge proposal");
-            int to = find(b);
-            counter++;
-            if (to != from) {
-                parent[from] = to;
-                num_merged++;
-            }
-        }
+        select_merges_sorted(B, num_to_merge, dS, best, parent);
     }
-    // build_mapping (blockmodel.cpp:29-41): surviving labels of non-empty blocks, ascending.
     launch_block_counts(c);
     std::vector<int> count(B);
     SBPX_HIP(hipMemcpyAsync(count.data(), c.m_count, sizeof(int) * B, hipMemcpyDeviceToHost, c.stream));
     c.sync();
-    std::vector<char> used(B, 0);
-    for (int x = 0; x < B; ++x)
-        if (count[x] > 0) used[find(x)] = 1;
-    std::vector<int> rank(B, -1);
-    int next = 0;
-    for (int x = 0; x < B; ++x)
-        if (used[x]) rank[x] = next++;
     std::vector<int> map(B);
-    for (int x = 0; x < B; ++x) map[x] = rank[find(x)];
-    const int newB = B - num_to_merge;
-    if (next > newB) throw StateError("more surviving labels than blocks after the merge");
+    merge_mapping(B, num_to_merge, count.data(), parent, 0, map.data());
     SBPX_HIP(hipMemcpyAsync(c.m_map, map.data(), sizeof(int) * B, hipMemcpyHostToDevice, c.stream));
     launch_relabel(c);
-    c.bm.B = newB;
+    c.bm.B = B - num_to_merge;
     build_blockmodel(c);
     check_device_error(c);
 }
@@ -1170,6 +1206,348 @@ sbpx_status sbpx_learned_gradients(sbpx_ctx* ctx, int32_t n, const float* x, con
             throw;
         }
         dfree(&tmp);
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// A batch of independent graphs held as one disjoint union on the device (DevBatch): every
+// operation takes the graphs it applies to, runs over the union in one set of launches, and
+// leaves the other graphs' state as it was.
+namespace {
+
+// Neighbour-block table size the batched evaluation gives a vertex of this degree.
+int eval_table_cap(int degree) {
+    const int b = bin_of(degree);
+    if (b < NUM_LDS_BINS) return BIN_CAP[b];
+    if (b == NUM_LDS_BINS) return 4096;
+    return table_cap(degree + 1);
+}
+
+// Marks the listed graphs active (each at most once) and uploads their seeds and calls.
+void batch_select(Ctx& c, int n, const int32_t* graphs, const uint64_t* seeds, const uint32_t* calls,
+                  std::vector<char>& active) {
+    DevBatch& b = c.batch;
+    if (n < 0 || n > b.G) throw ArgError("number of graphs out of range");
+    if (n > 0 && !graphs) throw ArgError("null graph list");
+    active.assign(b.G, 0);
+    for (int i = 0; i < n; ++i) {
+        if (graphs[i] < 0 || graphs[i] >= b.G) throw ArgError("graph index out of range");
+        if (active[graphs[i]]) throw ArgError("graph listed twice");
+        active[graphs[i]] = 1;
+    }
+    b.h_active.assign(active.begin(), active.end());
+    SBPX_HIP(hipMemcpyAsync(b.d_active, b.h_active.data(), b.G, hipMemcpyHostToDevice, c.stream));
+    if (seeds && calls) {
+        for (int i = 0; i < n; ++i) {
+            b.h_seed[graphs[i]] = seeds[i];
+            b.h_call[graphs[i]] = calls[i];
+        }
+        SBPX_HIP(hipMemcpyAsync(b.d_seed, b.h_seed.data(), sizeof(uint64_t) * b.G, hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemcpyAsync(b.d_call, b.h_call.data(), sizeof(uint32_t) * b.G, hipMemcpyHostToDevice, c.stream));
+    }
+}
+
+void batch_upload_nb(Ctx& c) {
+    DevBatch& b = c.batch;
+    SBPX_HIP(hipMemcpyAsync(b.d_nb, b.nb.data(), sizeof(int) * b.G, hipMemcpyHostToDevice, c.stream));
+}
+
+// The reference's batches of every graph (finetune.cpp:43-47, 352-366, as in build_plan), split
+// by degree bin.
+void batch_lists(Ctx& c, int batches) {
+    DevBatch& b = c.batch;
+    if (b.lists_batches == batches) return;
+    b.lists.assign(b.G, {});
+    for (int g = 0; g < b.G; ++g) {
+        const int n = b.vo[g + 1] - b.vo[g];
+        double num_batches = batches;
+        int batch_size = int(ceil(n / num_batches));
+        int nbat = n / batch_size;
+        for (int k = 0; k < nbat; ++k) {
+            std::array<std::vector<int>, NUM_BINS> bins;
+            for (int v = k * batch_size; v < std::min(n, (k + 1) * batch_size); ++v) {
+                const int u = b.vo[g] + v;
+                bins[bin_of(c.g.degree[u])].push_back(u);
+            }
+            b.lists[g].push_back(std::move(bins));
+        }
+    }
+    b.lists_batches = batches;
+    b.plan_active.clear();
+}
+
+// Union batch k: batch k of every active graph that has one, binned; kept while the same graphs
+// sweep with the same number of batches.
+void batch_plan(Ctx& c, const std::vector<char>& active, int batches) {
+    DevBatch& b = c.batch;
+    if (b.plan_active == active && b.plan_batches == batches) return;
+    size_t K = 0;
+    for (int g = 0; g < b.G; ++g)
+        if (active[g]) K = std::max(K, b.lists[g].size());
+    std::vector<int> all, huge_all;
+    size_t pool_need = 0;
+    b.plan.clear();
+    for (size_t k = 0; k < K; ++k) {
+        BinnedList bl;
+        const int first = (int)all.size();
+        std::vector<int> hoff(1, 0);
+        for (int bin = 0; bin < NUM_BINS; ++bin) {
+            bl.offset[bin] = (int)all.size();
+            for (int g = 0; g < b.G; ++g) {
+                if (!active[g] || k >= b.lists[g].size()) continue;
+                for (int u : b.lists[g][k][bin]) {
+                    all.push_back(u);
+                    if (bin == NUM_BINS - 1) hoff.push_back(hoff.back() + 7 * table_cap(c.g.degree[u] + 1));
+                }
+            }
+            bl.count[bin] = (int)all.size() - bl.offset[bin];
+        }
+        bl.total = (int)all.size() - first;
+        bl.huge_base = (int)huge_all.size();
+        huge_all.insert(huge_all.end(), hoff.begin(), hoff.end());
+        pool_need = std::max(pool_need, (size_t)hoff.back());
+        b.plan.push_back(bl);
+    }
+    if (all.size() > b.plan_slots) {
+        dalloc(&b.d_plan, all.size());
+        b.plan_slots = all.size();
+    }
+    if (huge_all.size() > b.plan_huge_slots) {
+        dalloc(&b.d_plan_huge, huge_all.size());
+        b.plan_huge_slots = huge_all.size();
+    }
+    if (!all.empty())
+        SBPX_HIP(hipMemcpy(b.d_plan, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice));
+    if (!huge_all.empty())
+        SBPX_HIP(hipMemcpy(b.d_plan_huge, huge_all.data(), sizeof(int) * huge_all.size(), hipMemcpyHostToDevice));
+    ensure_pool(c, pool_need);
+    b.plan_active = active;
+    b.plan_batches = batches;
+}
+
+}  // namespace
+
+extern "C" {
+
+sbpx_status sbpx_batch_define(sbpx_ctx* ctx, int32_t num_graphs, const int32_t* vertex_offsets) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_graph(c);
+        const int N = c.g.N, G = num_graphs;
+        if (G < 1 || G > N) throw ArgError("number of graphs must be in [1, N]");
+        if (!vertex_offsets) throw ArgError("null vertex offsets");
+        if (vertex_offsets[0] != 0 || vertex_offsets[G] != N) throw ArgError("vertex offsets must run from 0 to N");
+        for (int g = 0; g < G; ++g)
+            if (vertex_offsets[g + 1] <= vertex_offsets[g]) throw ArgError("every graph needs a vertex");
+        std::vector<int> vgraph(N);
+        for (int g = 0; g < G; ++g)
+            for (int v = vertex_offsets[g]; v < vertex_offsets[g + 1]; ++v) vgraph[v] = g;
+        long align = 64;
+        for (int v = 0; v < N; ++v) {
+            for (int j = c.g.h_out_ptr[v]; j < c.g.h_out_ptr[v + 1]; ++j)
+                if (vgraph[c.g.h_out_idx[j]] != vgraph[v]) throw ArgError("an edge joins two graphs of the batch");
+            align = std::max<long>(align, eval_table_cap(c.g.degree[v]));
+        }
+        std::vector<int> bo(G + 1, 0);
+        long top = 0;
+        for (int g = 0; g < G; ++g) {
+            bo[g] = (int)top;
+            top = (top + (vertex_offsets[g + 1] - vertex_offsets[g]) + align - 1) / align * align;
+            if (top > (1L << 30)) throw ArgError("the batch needs more than 2^30 block ids");
+        }
+        bo[G] = (int)top;
+        SBPX_HIP(hipSetDevice(c.device));
+        SBPX_HIP(hipStreamSynchronize(c.stream));
+        free_batch(c);
+        DevBatch& b = c.batch;
+        b.G = G;
+        b.align = (int)align;
+        b.range = bo[G];
+        b.vo.assign(vertex_offsets, vertex_offsets + G + 1);
+        b.bo = bo;
+        b.nb.resize(G);
+        for (int g = 0; g < G; ++g) b.nb[g] = b.vo[g + 1] - b.vo[g];
+        b.h_seed.assign(G, 0);
+        b.h_call.assign(G, 0);
+        // block-indexed buffers over the union's block ids
+        const size_t NBK = (size_t)std::max(N, b.range) + 1;
+        DevBlockmodel& m = c.bm;
+        for (int** p : {&m.row_ptr, &m.col_ptr, &m.dout, &m.din, &m.deg, &m.diag, &m.drift, &c.m_best, &c.m_map,
+                        &c.m_count, &c.tmp_b})
+            dalloc(p, NBK);
+        dalloc(&c.m_dS, NBK);
+        dalloc(&c.last_touch, NBK);
+        const long E = c.g.E;
+        dalloc(&c.scan_tmp, scan_tmp_ints(std::max<long>(E, 256L * ((E + 4095) / 4096) + 1) + NBK));
+        SBPX_HIP(hipMemset(m.drift, 0, sizeof(int) * NBK));
+        SBPX_HIP(hipMemset(c.last_touch, 0, sizeof(unsigned long long) * NBK));
+        dalloc(&b.d_vgraph, N);
+        dalloc(&b.d_bgraph, b.range);
+        dalloc(&b.d_vo, G + 1);
+        dalloc(&b.d_bo, G + 1);
+        dalloc(&b.d_nb, G);
+        dalloc(&b.d_seed, G);
+        dalloc(&b.d_call, G);
+        dalloc(&b.d_active, G);
+        dalloc(&b.d_req, 2 * (size_t)G + 2);
+        std::vector<int> bgraph(b.range), assign(N);
+        for (int g = 0; g < G; ++g) {
+            for (int x = bo[g]; x < bo[g + 1]; ++x) bgraph[x] = g;
+            for (int v = b.vo[g]; v < b.vo[g + 1]; ++v) assign[v] = bo[g] + (v - b.vo[g]);
+        }
+        SBPX_HIP(hipMemcpy(b.d_vgraph, vgraph.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(b.d_bgraph, bgraph.data(), sizeof(int) * b.range, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(b.d_vo, b.vo.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(b.d_bo, b.bo.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemcpy(b.d_nb, b.nb.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+        SBPX_HIP(hipMemset(b.d_seed, 0, sizeof(uint64_t) * G));
+        SBPX_HIP(hipMemset(b.d_call, 0, sizeof(uint32_t) * G));
+        SBPX_HIP(hipMemset(b.d_active, 0, G));
+        SBPX_HIP(hipMemcpy(m.assign, assign.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+        c.plan = SweepPlan();
+        m.B = b.range;
+        build_blockmodel(c);
+        check_device_error(c);
+    });
+}
+
+sbpx_status sbpx_batch_info(sbpx_ctx* ctx, int32_t* num_graphs, int32_t* num_blocks) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (num_graphs) *num_graphs = c.batch.G;
+        if (num_blocks) std::copy(c.batch.nb.begin(), c.batch.nb.end(), num_blocks);
+    });
+}
+
+sbpx_status sbpx_batch_build(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, const int32_t* num_blocks,
+                             const int32_t* assignment) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        DevBatch& b = c.batch;
+        std::vector<char> active;
+        batch_select(c, n, graphs, nullptr, nullptr, active);
+        if (n == 0) return;
+        if (!num_blocks || !assignment) throw ArgError("null block counts or assignment");
+        std::vector<int> req(2 * (size_t)n + 1);
+        req[0] = 0;
+        for (int i = 0; i < n; ++i) {
+            const int g = graphs[i], Ng = b.vo[g + 1] - b.vo[g], B = num_blocks[i];
+            if (B < 1 || B > Ng) throw ArgError("num_blocks must be in [1, N] of its graph");
+            const int32_t* a = assignment + req[i];
+            for (int v = 0; v < Ng; ++v)
+                if (a[v] < 0 || a[v] >= B) throw ArgError("block id out of range [0, B)");
+            req[i + 1] = req[i] + Ng;
+            req[n + 1 + i] = g;
+        }
+        const int total = req[n];
+        SBPX_HIP(hipMemcpyAsync(b.d_req, req.data(), sizeof(int) * req.size(), hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemcpyAsync(c.list_buf, assignment, sizeof(int) * total, hipMemcpyHostToDevice, c.stream));
+        launch_batch_scatter(c, n, total);
+        for (int i = 0; i < n; ++i) b.nb[graphs[i]] = num_blocks[i];
+        batch_upload_nb(c);
+        launch_zero_drift_active(c);
+        rebuild_keep_drift(c);
+        check_device_error(c);
+    });
+}
+
+sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t* assignment) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (!assignment) throw ArgError("null assignment");
+        const DevBatch& b = c.batch;
+        SBPX_HIP(hipMemcpyAsync(assignment, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (int g = 0; g < b.G; ++g)
+            for (int v = b.vo[g]; v < b.vo[g + 1]; ++v) assignment[v] -= b.bo[g];
+    });
+}
+
+sbpx_status sbpx_batch_log_posterior(sbpx_ctx* ctx, double* out) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (!out) throw ArgError("null output");
+        log_posterior_segments(c, out);
+    });
+}
+
+sbpx_status sbpx_batch_sweep(sbpx_ctx* ctx, int algo, int32_t batches, int32_t n, const int32_t* graphs,
+                             const uint64_t* seeds, const uint32_t* calls, sbpx_sweep_stats* stats) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (algo != SBPX_ALGO_ASYNC_GIBBS) throw ArgError("a batch of graphs runs asynchronous Gibbs sweeps");
+        if (batches < 1) throw ArgError("batches must be >= 1");
+        if (c.proposal != 0 || c.mlp.logging) throw StateError("the learned proposal runs on one graph at a time");
+        if (n > 0 && (!seeds || !calls)) throw ArgError("null seeds or calls");
+        std::vector<char> active;
+        batch_select(c, n, graphs, seeds, calls, active);
+        batch_lists(c, batches);
+        batch_plan(c, active, batches);
+        SBPX_HIP(hipMemsetAsync(c.counters, 0, sizeof(long long) * 8, c.stream));
+        SBPX_HIP(hipMemsetAsync(c.red + 2048, 0, sizeof(double), c.stream));
+        c.timed.clear();
+        c.timed_bins.clear();
+        long long* bytes = ctx->count_bytes ? c.counters + 4 : nullptr;
+        const DevBatch& b = c.batch;
+        for (const BinnedList& bl : b.plan) {
+            eval_vertex_list(ctx, b.d_plan, bl, b.d_plan_huge + bl.huge_base, 0, 0, TAG_ASYNC, bytes, true, false);
+            apply_accepted(c, b.d_plan + bl.offset[0], bl.total);
+        }
+        sweep_stats(ctx, stats);
+    });
+}
+
+sbpx_status sbpx_batch_merge(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, const int32_t* num_to_merge, int mode,
+                             const uint64_t* seeds, const uint32_t* calls) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (mode != SBPX_MERGE_APPROXIMATE) throw ArgError("a batch of graphs merges in the approximate mode");
+        if (n > 0 && (!num_to_merge || !seeds || !calls)) throw ArgError("null merge counts, seeds or calls");
+        DevBatch& b = c.batch;
+        std::vector<char> active;
+        batch_select(c, n, graphs, seeds, calls, active);
+        if (n == 0) return;
+        int lo = INT32_MAX, hi = 0;
+        for (int i = 0; i < n; ++i) {
+            const int g = graphs[i];
+            if (num_to_merge[i] < 0 || num_to_merge[i] >= b.nb[g]) throw ArgError("num_to_merge must be in [0, B)");
+            lo = std::min(lo, b.bo[g]);
+            hi = std::max(hi, b.bo[g] + b.nb[g]);
+        }
+        launch_merge_proposals(c, 0, 0);
+        launch_block_counts(c);
+        const int span = hi - lo;
+        std::vector<double> dS(span);
+        std::vector<int> best(span), count(span), map(span);
+        SBPX_HIP(hipMemcpyAsync(dS.data(), c.m_dS + lo, sizeof(double) * span, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(best.data(), c.m_best + lo, sizeof(int) * span, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(count.data(), c.m_count + lo, sizeof(int) * span, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        check_device_error(c);
+        std::iota(map.begin(), map.end(), lo);
+        for (int i = 0; i < n; ++i) {
+            const int g = graphs[i], B = b.nb[g], o = b.bo[g] - lo;
+            std::vector<int> best_local(B), parent(B);
+            for (int x = 0; x < B; ++x) best_local[x] = best[o + x] >= 0 ? best[o + x] - b.bo[g] : -1;
+            std::iota(parent.begin(), parent.end(), 0);
+            select_merges_sorted(B, num_to_merge[i], dS.data() + o, best_local.data(), parent);
+            merge_mapping(B, num_to_merge[i], count.data() + o, parent, b.bo[g], map.data() + o);
+        }
+        SBPX_HIP(hipMemcpyAsync(c.m_map, map.data(), sizeof(int) * span, hipMemcpyHostToDevice, c.stream));
+        launch_batch_relabel(c, lo, hi);
+        for (int i = 0; i < n; ++i) b.nb[graphs[i]] -= num_to_merge[i];
+        batch_upload_nb(c);
+        launch_zero_drift_active(c);
+        rebuild_keep_drift(c);
+        check_device_error(c);
     });
 }
 

@@ -212,15 +212,63 @@ void build_blockmodel(Ctx& c) {
 }
 
 // ------------------------------------------------------------------------------------------
-constexpr int LP_BLOCKS = 1024;
+// The log posterior is summed per graph (a one-graph context has one): the cells of graph g,
+// [row_ptr[bo[g]], row_ptr[bo[g] + nb[g]]), fall in chunks of LP_CHUNK from its first cell; one
+// workgroup sums a chunk in a fixed order, then one wavefront per graph adds its chunks in order.
+// The result depends on the graph's own cells only, not on where they sit in a union.
+constexpr int LP_CHUNK = 2048;
 
-__global__ void log_posterior_kernel(const int* __restrict__ coo_row, const int* __restrict__ row_col,
-                                     const int* __restrict__ row_val, const int* __restrict__ dout,
-                                     const int* __restrict__ din, int nnz, double* __restrict__ partial,
-                                     int* __restrict__ err) {
+// seg = lo[G], hi[G], chunk offsets[G + 1]; one workgroup.
+__global__ void lp_segments_kernel(const int* __restrict__ row_ptr, int G, const int* __restrict__ bo,
+                                   const int* __restrict__ nb, int nnz, int* __restrict__ seg) {
+    __shared__ int lds[TB];
+    const int per = (G + TB - 1) / TB;
+    const int g0 = threadIdx.x * per, g1 = min(G, g0 + per);
+    int local = 0;
+    for (int g = g0; g < g1; ++g) {
+        const int lo = bo ? row_ptr[bo[g]] : 0;
+        const int hi = bo ? row_ptr[bo[g] + nb[g]] : nnz;
+        seg[g] = lo;
+        seg[G + g] = hi;
+        local += (hi - lo + LP_CHUNK - 1) / LP_CHUNK;
+    }
+    lds[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < TB; ++t) {
+            const int x = lds[t];
+            lds[t] = acc;
+            acc += x;
+        }
+        seg[3 * G] = acc;
+    }
+    __syncthreads();
+    int off = lds[threadIdx.x];
+    for (int g = g0; g < g1; ++g) {
+        seg[2 * G + g] = off;
+        off += (seg[G + g] - seg[g] + LP_CHUNK - 1) / LP_CHUNK;
+    }
+}
+
+__global__ void __launch_bounds__(TB) lp_chunk_kernel(const int* __restrict__ coo_row, const int* __restrict__ row_col,
+                                                      const int* __restrict__ row_val, const int* __restrict__ dout,
+                                                      const int* __restrict__ din, const int* __restrict__ seg, int G,
+                                                      double* __restrict__ part, int* __restrict__ err) {
     __shared__ double lds[TB / WAVE];
+    const int* choff = seg + 2 * G;
+    const int w = blockIdx.x;
+    if (w >= choff[G]) return;
+    int g = 0, top = G;  // the last g with choff[g] <= w
+    while (top - g > 1) {
+        const int mid = (g + top) >> 1;
+        if (choff[mid] <= w) g = mid;
+        else top = mid;
+    }
+    const int lo = seg[g] + (w - choff[g]) * LP_CHUNK;
+    const int end = min(seg[G + g], lo + LP_CHUNK);
     double acc = 0.0;
-    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nnz; j += gridDim.x * blockDim.x) {
+    for (int j = lo + (int)threadIdx.x; j < end; j += TB) {
         double value = (double)row_val[j];
         double d_out = (double)dout[coo_row[j]];
         double d_in = (double)din[row_col[j]];
@@ -233,35 +281,60 @@ __global__ void log_posterior_kernel(const int* __restrict__ coo_row, const int*
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0.0;
-        for (int w = 0; w < TB / WAVE; ++w) s += lds[w];
-        partial[blockIdx.x] = s;
+        for (int q = 0; q < TB / WAVE; ++q) s += lds[q];
+        part[w] = s;
     }
 }
 
-__global__ void sum_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
-    __shared__ double lds[TB / WAVE];
+__global__ void lp_graph_sum_kernel(const double* __restrict__ part, const int* __restrict__ seg, int G,
+                                    double* __restrict__ out) {
+    const int g = blockIdx.x * (TB / WAVE) + threadIdx.x / WAVE;
+    if (g >= G) return;
+    const int* choff = seg + 2 * G;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    for (int i = choff[g] + lane_id(); i < choff[g + 1]; i += WAVE) acc += part[i];
     acc = wave_sum(acc);
-    if (lane_id() == 0) lds[threadIdx.x / WAVE] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        for (int w = 0; w < TB / WAVE; ++w) s += lds[w];
-        out[0] = s;
+    if (lane_id() == 0) out[g] = acc;
+}
+
+void log_posterior_segments(Ctx& c, double* out) {
+    const int G = c.batch.G > 0 ? c.batch.G : 1;
+    const int nnz = c.bm.nnz;
+    const size_t chunks = (size_t)nnz / LP_CHUNK + G + 1;
+    if (c.lp_part_slots < chunks + G) {
+        SBPX_HIP(hipFree(c.lp_part));
+        SBPX_HIP(hipFree(c.lp_seg));
+        c.lp_part = nullptr;
+        c.lp_seg = nullptr;
+        c.lp_part_slots = 0;
+        const size_t slots = chunks + G + 1024;
+        SBPX_HIP(hipMalloc((void**)&c.lp_part, sizeof(double) * slots));
+        SBPX_HIP(hipMalloc((void**)&c.lp_seg, sizeof(int) * (3 * (size_t)G + 1)));
+        c.lp_part_slots = slots;
+        c.lp_seg_graphs = G;
+    } else if (c.lp_seg_graphs < G) {
+        SBPX_HIP(hipFree(c.lp_seg));
+        c.lp_seg = nullptr;
+        SBPX_HIP(hipMalloc((void**)&c.lp_seg, sizeof(int) * (3 * (size_t)G + 1)));
+        c.lp_seg_graphs = G;
     }
+    const bool batch = c.batch.G > 0;
+    lp_segments_kernel<<<1, TB, 0, c.stream>>>(c.bm.row_ptr, G, batch ? c.batch.d_bo : nullptr,
+                                               batch ? c.batch.d_nb : nullptr, nnz, c.lp_seg);
+    lp_chunk_kernel<<<(int)chunks, TB, 0, c.stream>>>(c.bm.coo_row, c.bm.row_col, c.bm.row_val, c.bm.dout,
+                                                      c.bm.din, c.lp_seg, G, c.lp_part, c.dev_err);
+    double* d_out = c.lp_part + chunks;
+    lp_graph_sum_kernel<<<(G + TB / WAVE - 1) / (TB / WAVE), TB, 0, c.stream>>>(c.lp_part, c.lp_seg, G, d_out);
+    SBPX_HIP(hipGetLastError());
+    SBPX_HIP(hipMemcpyAsync(out, d_out, sizeof(double) * G, hipMemcpyDeviceToHost, c.stream));
+    c.sync();
+    check_device_error(c);
 }
 
 double log_posterior(Ctx& c) {
-    const int nnz = c.bm.nnz;
-    int nb = std::min(LP_BLOCKS, blocks_for(nnz));
-    log_posterior_kernel<<<nb, TB, 0, c.stream>>>(c.bm.coo_row, c.bm.row_col, c.bm.row_val, c.bm.dout, c.bm.din,
-                                                  nnz, c.red, c.dev_err);
-    sum_partials_kernel<<<1, TB, 0, c.stream>>>(c.red, nb, c.red + LP_BLOCKS);
+    if (c.batch.G > 0) throw StateError("the context holds a batch of graphs");
     double out = 0.0;
-    SBPX_HIP(hipMemcpyAsync(&out, c.red + LP_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    c.sync();
-    check_device_error(c);
+    log_posterior_segments(c, &out);
     return out;
 }
 

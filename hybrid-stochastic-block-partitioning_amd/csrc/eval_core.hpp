@@ -217,7 +217,28 @@ struct BlockState {
     const int* din;
     const int* deg;
     int B;
+    // Several graphs at once (null for one graph): graph g owns the vertices [gvo[g], gvo[g + 1])
+    // and the block ids from gbo[g], of which its first gB[g] are in use; its randomness is keyed
+    // by its own seed and call and by vertex ids local to it.
+    const int* vgraph = nullptr;  // graph of each vertex
+    const int* gvo = nullptr;
+    const int* gbo = nullptr;
+    const int* gB = nullptr;
+    const uint64_t* gseed = nullptr;
+    const uint32_t* gcall = nullptr;
 };
+
+// The graph-local view of vertex v: block count, first block id, local vertex id, seed, call.
+struct LocalKey {
+    int B, base, id;
+    uint64_t seed;
+    uint32_t call;
+};
+__device__ __forceinline__ LocalKey local_key(const BlockState& S, int v, uint64_t seed, uint32_t call) {
+    if (S.vgraph == nullptr) return LocalKey{S.B, 0, v, seed, call};
+    const int g = S.vgraph[v];
+    return LocalKey{S.gB[g], S.gbo[g], v - S.gvo[g], S.gseed[g], S.gcall[g]};
+}
 
 // Wave-uniform result of one evaluation; the neighbour table stays valid for a commit.
 struct MoveResult {
@@ -276,7 +297,8 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     const int r = S.assign[v];
     R.r = r;
     R.t = -1;
-    const int B = S.B;
+    const LocalKey lk = local_key(S, v, seed, call);
+    const int B = lk.B;  // blocks of this vertex's graph (Hastings smoothing, random block)
     int* tkey = tab;
     int* to = tab + cap;
     int* ti = tab + 2 * cap;
@@ -309,7 +331,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     double u = given_u;
     int pick0 = -1, pick1 = -1;  // combined-list positions whose block may be t
     if (given_s < 0) {
-        dr = philox4x32_10(seed, (uint32_t)v, call, 0u, tag);
+        dr = philox4x32_10(lk.seed, (uint32_t)lk.id, lk.call, 0u, tag);
         u = unit53(dr.x2, dr.x3);
         if (k > 0) {
             const int ps = (int)mulhi32(dr.x0, (uint32_t)k);
@@ -357,7 +379,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     if (given_s >= 0) {
         s = given_s;
     } else if (k == 0) {
-        s = random_block(r, B, dr.x0);
+        s = lk.base + random_block(r - lk.base, B, dr.x0);
     } else {
         int t = cand0;
         if (pick1 >= 0 && pick0 - dO >= self_pos) t = cand1;  // in list: skip over the self-loop

@@ -26,7 +26,9 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
     const int w0 = wave - wig;
     const int gl = wig * WAVE + lane;
     const bool act = v >= 0;
-    const int B = S.B;
+    const LocalKey lk = act ? local_key(S, v, seed, call) : LocalKey{S.B, 0, 0, seed, call};
+    const int B = lk.B;     // blocks of this vertex's graph
+    const int Bids = S.B;   // range of block ids (radix descent)
     MoveResult R{};
     R.v = v;
     R.t = -1;
@@ -62,7 +64,7 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
     double u = given_u;
     int pick0 = -1, pick1 = -1;
     if (act && given_s < 0) {
-        dr = philox4x32_10(seed, (uint32_t)v, call, 0u, tag);
+        dr = philox4x32_10(lk.seed, (uint32_t)lk.id, lk.call, 0u, tag);
         u = unit53(dr.x2, dr.x3);
         if (k > 0) {
             const int ps = (int)mulhi32(dr.x0, (uint32_t)k);
@@ -120,7 +122,7 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         if (given_s >= 0) {
             s = given_s;
         } else if (k == 0) {
-            s = random_block(r, B, dr.x0);
+            s = lk.base + random_block(r - lk.base, B, dr.x0);
         } else {
             int t = cand0;
             if (pick1 >= 0 && pick0 - dO >= self_pos) t = cand1;
@@ -140,7 +142,7 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         const int shi = sample ? slo + (row ? M.rcap[samp_t] : M.ccap[samp_t]) : 0;
         const int starget = row ? samp_target : samp_target - samp_trow;
         int nbits = 1;
-        while (nbits < 31 && (1 << nbits) < B) ++nbits;
+        while (nbits < 31 && (1 << nbits) < Bids) ++nbits;
         int shift = ((nbits + 7) / 8 - 1) * 8;
         int prefix = 0;
         long long below = 0;
@@ -192,7 +194,7 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
             if (shift == 0) break;
             shift -= 8;
         }
-        if (sample) s = prefix < B ? prefix : B - 1;
+        if (sample) s = prefix < Bids ? prefix : Bids - 1;
     } else {
         if (sample && wig == 0) {
             const int ss = M.sample(samp_t, samp_target, samp_trow);

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -127,6 +128,39 @@ struct MhState {
     int max_deg_high = 0;
 };
 
+// Several independent graphs held as one disjoint union. Graph g owns the vertices
+// [vo[g], vo[g + 1]) and the block ids [bo[g], bo[g + 1]), of which the first nb[g] are in use.
+// Every bo[g] is a multiple of `align`, the largest neighbour-block table any vertex uses, so the
+// table slot of a block (hash_block(b) & (cap - 1)) is the same as for the graph alone; with the
+// randomness keyed by graph-local ids and the MDL summed per graph, each graph follows the run it
+// would have on its own.
+struct DevBatch {
+    int G = 0;  // 0: the context holds one graph
+    int align = 0;
+    int range = 0;  // block ids in use by the union: bo[G]
+    std::vector<int> vo, bo, nb;
+    int *d_vgraph = nullptr;  // [N] graph of each vertex
+    int *d_bgraph = nullptr;  // [range] graph of each block id
+    int *d_vo = nullptr, *d_bo = nullptr, *d_nb = nullptr;
+    uint64_t* d_seed = nullptr;
+    uint32_t* d_call = nullptr;
+    uint8_t* d_active = nullptr;  // [G] graphs taking part in the current operation
+    int* d_req = nullptr;         // [2 G + 2] request staging
+    std::vector<uint8_t> h_active;
+    std::vector<uint64_t> h_seed;
+    std::vector<uint32_t> h_call;
+    // the reference's batches of every graph, split by degree bin: lists[g][k][bin] (union ids)
+    std::vector<std::vector<std::array<std::vector<int>, NUM_BINS>>> lists;
+    int lists_batches = 0;
+    // the union plan of the last sweep and the graphs it covered
+    std::vector<char> plan_active;
+    int plan_batches = 0;
+    std::vector<BinnedList> plan;
+    int* d_plan = nullptr;
+    int* d_plan_huge = nullptr;
+    size_t plan_slots = 0, plan_huge_slots = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -136,6 +170,7 @@ struct Ctx {
     SweepPlan plan;
     MhState mh;
     MlpState mlp;
+    DevBatch batch;
     int proposal = 0;  // 0 plain, 1 learned mixture for the batched low-degree vertices
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
 
@@ -150,6 +185,10 @@ struct Ctx {
     int* scan_tmp = nullptr;
     int* tmp_b = nullptr;   // N + 1
     double* red = nullptr;  // reduction partials
+    double* lp_part = nullptr;  // log posterior chunk partials and per-graph results
+    size_t lp_part_slots = 0;
+    int* lp_seg = nullptr;      // log posterior segments: [3 G + 1]
+    int lp_seg_graphs = 0;
     int* dev_err = nullptr;
     long long* counters = nullptr;  // [8]: attempts, evals, moves
 
@@ -260,6 +299,10 @@ int radix_sort(K* keys, K* keys_alt, int* vals, int* vals_alt, long n, const int
 void build_blockmodel(Ctx& c);               // from c.bm.assign, resets drift
 void rebuild_keep_drift(Ctx& c);             // from c.bm.assign, keeps drift
 double log_posterior(Ctx& c);                // sum of M * logf(M / (dout din)) in FP64
+// Per-graph log posteriors of a union (out[G]); a one-graph context gives out[0]. Each graph's
+// cells are summed in fixed chunks from its first cell, so the value does not depend on the
+// other graphs.
+void log_posterior_segments(Ctx& c, double* out);
 void check_device_error(Ctx& c);
 void launch_logf_table(Ctx& c, uint32_t lo, uint32_t n, float* out);
 
@@ -305,7 +348,10 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                    long long* bytes, bool timed);
 
 // merge.hip
-void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call);
+void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call);  // union: per-graph seeds and calls
+void launch_batch_relabel(Ctx& c, int lo, int hi);  // assign[v] = map[assign[v] - lo] where in [lo, hi)
+void launch_zero_drift_active(Ctx& c);              // drift = 0 on the blocks of active graphs
+void launch_batch_scatter(Ctx& c, int n, int total);  // assignments of n requests into the union
 void launch_block_counts(Ctx& c);
 void launch_relabel(Ctx& c);
 

@@ -30,6 +30,14 @@ struct MergeParams {
     uint32_t call;
     double* __restrict__ out_dS;
     int* __restrict__ out_best;
+    // a union of graphs (null for one graph): graph of each block id, its first block id, blocks
+    // in use, seed and call, and whether it takes part in this merge phase
+    const int* __restrict__ bgraph;
+    const int* __restrict__ gbo;
+    const int* __restrict__ gB;
+    const uint64_t* __restrict__ gseed;
+    const uint32_t* __restrict__ gcall;
+    const uint8_t* __restrict__ active;
 };
 
 constexpr int MERGE_WAVES = 4;
@@ -39,17 +47,29 @@ __global__ void __launch_bounds__(MERGE_WAVES * WAVE) merge_proposals_kernel(Mer
     const int r = blockIdx.x * MERGE_WAVES + threadIdx.x / WAVE;
     if (r >= P.B) return;
     const int lane = lane_id();
-    const int B = P.B;
+    int B = P.B, base = 0;  // blocks of r's graph and its first block id
+    uint64_t seed = P.seed;
+    uint32_t call = P.call;
+    if (P.bgraph != nullptr) {
+        const int g = P.bgraph[r];
+        if (!P.active[g]) return;
+        base = P.gbo[g];
+        B = P.gB[g];
+        if (r - base >= B) return;
+        seed = P.gseed[g];
+        call = P.gcall[g];
+    }
+    const int rl = r - base;  // graph-local id: keys the randomness
     const int k_out = P.dout[r], k_in = P.din[r];
     const int k = k_out + k_in;
     int past[NUM_PROPOSALS];
     double best_dS = DBL_MAX;
     int best = -1;
     for (int i = 0; i < NUM_PROPOSALS; ++i) {
-        Draw4 dr = philox4x32_10(P.seed, (uint32_t)r, P.call, (uint32_t)i, TAG_MERGE);
+        Draw4 dr = philox4x32_10(seed, (uint32_t)rl, call, (uint32_t)i, TAG_MERGE);
         int s;
         if (k == 0) {
-            s = random_block(r, B, dr.x2);
+            s = base + random_block(rl, B, dr.x2);
         } else {
             int target = (int)mulhi32(dr.x0, (uint32_t)k);
             int t;
@@ -69,7 +89,7 @@ __global__ void __launch_bounds__(MERGE_WAVES * WAVE) merge_proposals_kernel(Mer
             const int rowpart = P.dout[t] - Mtr;
             const int total = rowpart + P.din[t] - Mrt;
             if (total == 0) {
-                s = random_block(r, B, dr.x2);
+                s = base + random_block(rl, B, dr.x2);
             } else {
                 int target2 = (int)mulhi32(dr.x1, (uint32_t)total);
                 if (target2 < rowpart) {
@@ -119,6 +139,15 @@ void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call) {
     P.call = call;
     P.out_dS = c.m_dS;
     P.out_best = c.m_best;
+    if (c.batch.G > 0) {
+        const DevBatch& b = c.batch;
+        P.bgraph = b.d_bgraph;
+        P.gbo = b.d_bo;
+        P.gB = b.d_nb;
+        P.gseed = b.d_seed;
+        P.gcall = b.d_call;
+        P.active = b.d_active;
+    }
     int nb = (c.bm.B + MERGE_WAVES - 1) / MERGE_WAVES;
     merge_proposals_kernel<<<nb, MERGE_WAVES * WAVE, 0, c.stream>>>(P);
     SBPX_HIP(hipGetLastError());
@@ -132,6 +161,55 @@ __global__ void block_counts_kernel(const int* __restrict__ assign, int N, int* 
 __global__ void relabel_kernel(int* __restrict__ assign, int N, const int* __restrict__ map) {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < N) assign[v] = map[assign[v]];
+}
+
+__global__ void batch_relabel_kernel(int* __restrict__ assign, int N, const int* __restrict__ map, int lo, int hi) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const int a = assign[v];
+    if (a >= lo && a < hi) assign[v] = map[a - lo];
+}
+
+__global__ void zero_drift_active_kernel(int* __restrict__ drift, int range, const int* __restrict__ bgraph,
+                                         const uint8_t* __restrict__ active) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < range && active[bgraph[x]]) drift[x] = 0;
+}
+
+// Request i (graph req[n + i]) holds the assignment values [pre[i], pre[i + 1]) of the staged
+// list, for the vertices from vo[graph] on, in graph-local block ids.
+__global__ void batch_scatter_kernel(const int* __restrict__ vals, int total, const int* __restrict__ req, int n,
+                                     const int* __restrict__ vo, const int* __restrict__ bo,
+                                     int* __restrict__ assign) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= total) return;
+    int i = 0, hi = n;  // the last i with pre[i] <= p (req[0..n] = pre, pre[0] = 0, pre[n] = total)
+    while (hi - i > 1) {
+        const int mid = (i + hi) >> 1;
+        if (req[mid] <= p) i = mid;
+        else hi = mid;
+    }
+    const int g = req[n + 1 + i];
+    assign[vo[g] + (p - req[i])] = bo[g] + vals[p];
+}
+
+void launch_batch_relabel(Ctx& c, int lo, int hi) {
+    batch_relabel_kernel<<<(c.g.N + 255) / 256, 256, 0, c.stream>>>(c.bm.assign, c.g.N, c.m_map, lo, hi);
+    SBPX_HIP(hipGetLastError());
+}
+
+void launch_zero_drift_active(Ctx& c) {
+    const int range = c.batch.range;
+    zero_drift_active_kernel<<<(range + 255) / 256, 256, 0, c.stream>>>(c.bm.drift, range, c.batch.d_bgraph,
+                                                                       c.batch.d_active);
+    SBPX_HIP(hipGetLastError());
+}
+
+void launch_batch_scatter(Ctx& c, int n, int total) {
+    if (total <= 0) return;
+    batch_scatter_kernel<<<(total + 255) / 256, 256, 0, c.stream>>>(c.list_buf, total, c.batch.d_req, n,
+                                                                   c.batch.d_vo, c.batch.d_bo, c.bm.assign);
+    SBPX_HIP(hipGetLastError());
 }
 
 void launch_block_counts(Ctx& c) {

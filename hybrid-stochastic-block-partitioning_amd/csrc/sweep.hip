@@ -132,6 +132,15 @@ static EvalParams make_params(Ctx& c) {
     EvalParams P{};
     P.G = GraphView{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
     P.S = BlockState{c.bm.assign, c.bm.dout, c.bm.din, c.bm.deg, c.bm.B};
+    if (c.batch.G > 0) {
+        const DevBatch& b = c.batch;
+        P.S.vgraph = b.d_vgraph;
+        P.S.gvo = b.d_vo;
+        P.S.gbo = b.d_bo;
+        P.S.gB = b.d_nb;
+        P.S.gseed = b.d_seed;
+        P.S.gcall = b.d_call;
+    }
     P.M = CsrView{c.bm.row_ptr, c.bm.row_col, c.bm.row_val, c.bm.row_cum,
                   c.bm.col_ptr, c.bm.col_row, c.bm.col_val, c.bm.col_cum};
     P.err = c.dev_err;

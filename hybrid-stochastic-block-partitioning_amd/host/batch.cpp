@@ -1,0 +1,337 @@
+// Many graphs partitioned side by side on one GPU. Each graph runs the reference's driver
+// unchanged (sbp::stochastic_block_partition) in a thread of its own; its device steps go to a
+// coordinator instead of a context of its own. When every running driver waits on a step, the
+// last to arrive executes the round: one batched call (sbpx_batch_*) per kind of step over the
+// shared context, which holds all graphs as one disjoint union. Each graph therefore follows the
+// same sequence of steps, with the same seeds and calls, as it would alone, while a round costs
+// one set of launches for all graphs.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <tuple>
+
+#include "device.hpp"
+#include "sbp_host.hpp"
+
+namespace sbpx_host {
+
+namespace {
+
+struct Request {
+    enum Kind { BUILD, DOWNLOAD, LOGP, SWEEP, MERGE } kind;
+    int slot = -1;
+    int num_blocks = 0;
+    const int32_t* assignment = nullptr;
+    int32_t* out_assignment = nullptr;
+    int algo = 0, batches = 0, num_to_merge = 0, mode = 0;
+    uint64_t seed = 0;
+    uint32_t call = 0;
+    sbpx_sweep_stats* stats = nullptr;
+    double value = 0.0;
+    int blocks_left = 0;
+    std::string error;
+    bool done = false;
+};
+
+}  // namespace
+
+struct Batch {
+    sbpx_ctx* ctx = nullptr;
+    std::vector<int> vertex_offsets;
+    std::mutex mu;
+    std::condition_variable cv;
+    int live = 0;  // drivers still running
+    std::vector<Request*> pending;
+    long rounds = 0;
+    sbpx_sweep_stats totals{};  // sweep statistics over all graphs
+    long long sweeps = 0;
+    std::vector<double> lp;
+    std::vector<int32_t> assign;
+};
+
+namespace {
+
+void fail(sbpx_ctx* ctx, const std::vector<Request*>& rs, const char* what) {
+    const std::string msg = std::string(what) + ": " + sbpx_last_error(ctx);
+    for (Request* r : rs) r->error = msg;
+}
+
+// Executes one round: every pending request, one batched call per kind (and per parameters).
+// Each graph has at most one request in a round and every call changes only the state of the
+// graphs it lists, so the order of the calls does not matter.
+void execute(Batch& b, std::vector<Request*>& reqs) {
+    std::map<int, std::vector<Request*>> by_kind;
+    for (Request* r : reqs) by_kind[r->kind].push_back(r);
+    sbpx_ctx* ctx = b.ctx;
+    for (auto& kv : by_kind) {
+        std::vector<Request*>& rs = kv.second;
+        const int n = (int)rs.size();
+        std::vector<int32_t> graphs(n);
+        for (int i = 0; i < n; ++i) graphs[i] = rs[i]->slot;
+        switch (kv.first) {
+            case Request::BUILD: {
+                std::vector<int32_t> blocks(n), concat;
+                for (int i = 0; i < n; ++i) {
+                    const int g = rs[i]->slot, Ng = b.vertex_offsets[g + 1] - b.vertex_offsets[g];
+                    blocks[i] = rs[i]->num_blocks;
+                    concat.insert(concat.end(), rs[i]->assignment, rs[i]->assignment + Ng);
+                }
+                if (sbpx_batch_build(ctx, n, graphs.data(), blocks.data(), concat.data()) != SBPX_OK)
+                    fail(ctx, rs, "sbpx_batch_build");
+                break;
+            }
+            case Request::DOWNLOAD: {
+                b.assign.resize(b.vertex_offsets.back());
+                if (sbpx_batch_download(ctx, b.assign.data()) != SBPX_OK) {
+                    fail(ctx, rs, "sbpx_batch_download");
+                    break;
+                }
+                for (Request* r : rs) {
+                    const int lo = b.vertex_offsets[r->slot], hi = b.vertex_offsets[r->slot + 1];
+                    std::copy(b.assign.begin() + lo, b.assign.begin() + hi, r->out_assignment);
+                }
+                break;
+            }
+            case Request::LOGP: {
+                b.lp.resize(b.vertex_offsets.size() - 1);
+                if (sbpx_batch_log_posterior(ctx, b.lp.data()) != SBPX_OK) {
+                    fail(ctx, rs, "sbpx_batch_log_posterior");
+                    break;
+                }
+                for (Request* r : rs) r->value = b.lp[r->slot];
+                break;
+            }
+            case Request::SWEEP: {
+                std::map<std::pair<int, int>, std::vector<Request*>> groups;
+                for (Request* r : rs) groups[{r->algo, r->batches}].push_back(r);
+                for (auto& gv : groups) {
+                    std::vector<Request*>& q = gv.second;
+                    const int m = (int)q.size();
+                    std::vector<int32_t> gs(m);
+                    std::vector<uint64_t> seeds(m);
+                    std::vector<uint32_t> calls(m);
+                    for (int i = 0; i < m; ++i) {
+                        gs[i] = q[i]->slot;
+                        seeds[i] = q[i]->seed;
+                        calls[i] = q[i]->call;
+                    }
+                    sbpx_sweep_stats st{};
+                    if (sbpx_batch_sweep(ctx, gv.first.first, gv.first.second, m, gs.data(), seeds.data(), calls.data(),
+                                         &st) != SBPX_OK) {
+                        fail(ctx, q, "sbpx_batch_sweep");
+                        continue;
+                    }
+                    b.sweeps += 1;
+                    b.totals.attempts += st.attempts;
+                    b.totals.evals += st.evals;
+                    b.totals.moves += st.moves;
+                    for (int k = 0; k < 4; ++k) {
+                        b.totals.eval_ms[k] += st.eval_ms[k];
+                        b.totals.eval_bytes[k] += st.eval_bytes[k];
+                        b.totals.eval_launches[k] += st.eval_launches[k];
+                    }
+                    for (Request* r : q)
+                        if (r->stats) *r->stats = sbpx_sweep_stats{};  // the batch keeps the totals
+                }
+                break;
+            }
+            case Request::MERGE: {
+                std::map<int, std::vector<Request*>> groups;
+                for (Request* r : rs) groups[r->mode].push_back(r);
+                for (auto& gv : groups) {
+                    std::vector<Request*>& q = gv.second;
+                    const int m = (int)q.size();
+                    std::vector<int32_t> gs(m), k(m);
+                    std::vector<uint64_t> seeds(m);
+                    std::vector<uint32_t> calls(m);
+                    for (int i = 0; i < m; ++i) {
+                        gs[i] = q[i]->slot;
+                        k[i] = q[i]->num_to_merge;
+                        seeds[i] = q[i]->seed;
+                        calls[i] = q[i]->call;
+                    }
+                    if (sbpx_batch_merge(ctx, m, gs.data(), k.data(), gv.first, seeds.data(), calls.data()) != SBPX_OK) {
+                        fail(ctx, q, "sbpx_batch_merge");
+                        continue;
+                    }
+                    std::vector<int32_t> nb(b.vertex_offsets.size() - 1);
+                    if (sbpx_batch_info(ctx, nullptr, nb.data()) != SBPX_OK) {
+                        fail(ctx, q, "sbpx_batch_info");
+                        continue;
+                    }
+                    for (Request* r : q) r->blocks_left = nb[r->slot];
+                }
+                break;
+            }
+        }
+    }
+}
+
+// Called with the lock held when every running driver waits: runs the round without the lock
+// (no driver can act meanwhile) and wakes them.
+void run_round(Batch& b, std::unique_lock<std::mutex>& lk) {
+    std::vector<Request*> reqs;
+    reqs.swap(b.pending);
+    lk.unlock();
+    try {
+        execute(b, reqs);
+    } catch (const std::exception& e) {
+        for (Request* r : reqs)
+            if (r->error.empty()) r->error = e.what();
+    }
+    lk.lock();
+    b.rounds++;
+    for (Request* r : reqs) r->done = true;
+    b.cv.notify_all();
+}
+
+void submit(Batch& b, Request& r) {
+    {
+        std::unique_lock<std::mutex> lk(b.mu);
+        b.pending.push_back(&r);
+        if ((int)b.pending.size() == b.live) run_round(b, lk);
+        else b.cv.wait(lk, [&r] { return r.done; });
+    }
+    if (!r.error.empty()) throw std::runtime_error(r.error);
+}
+
+void leave(Batch& b) {
+    std::unique_lock<std::mutex> lk(b.mu);
+    b.live--;
+    if (b.live > 0 && (int)b.pending.size() == b.live) run_round(b, lk);
+}
+
+}  // namespace
+
+void batch_build(Batch& b, int slot, int num_blocks, const int32_t* assignment) {
+    Request r{Request::BUILD};
+    r.slot = slot;
+    r.num_blocks = num_blocks;
+    r.assignment = assignment;
+    submit(b, r);
+}
+
+void batch_download(Batch& b, int slot, int32_t* assignment) {
+    Request r{Request::DOWNLOAD};
+    r.slot = slot;
+    r.out_assignment = assignment;
+    submit(b, r);
+}
+
+double batch_log_posterior(Batch& b, int slot) {
+    Request r{Request::LOGP};
+    r.slot = slot;
+    submit(b, r);
+    return r.value;
+}
+
+void batch_sweep(Batch& b, int slot, int algo, int batches, uint64_t seed, uint32_t call, sbpx_sweep_stats* stats) {
+    Request r{Request::SWEEP};
+    r.slot = slot;
+    r.algo = algo;
+    r.batches = batches;
+    r.seed = seed;
+    r.call = call;
+    r.stats = stats;
+    submit(b, r);
+}
+
+int batch_merge(Batch& b, int slot, int num_to_merge, int mode, uint64_t seed, uint32_t call) {
+    Request r{Request::MERGE};
+    r.slot = slot;
+    r.num_to_merge = num_to_merge;
+    r.mode = mode;
+    r.seed = seed;
+    r.call = call;
+    submit(b, r);
+    return r.blocks_left;
+}
+
+}  // namespace sbpx_host
+
+namespace sbp {
+
+BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::vector<int32_t>& out_ptr,
+                         const std::vector<int32_t>& out_idx, const std::vector<int32_t>& in_ptr,
+                         const std::vector<int32_t>& in_idx, const std::vector<long>& num_edges,
+                         const std::vector<Args>& graph_args, int device, int eval_variant) {
+    using sbpx_host::Batch;
+    using sbpx_host::Device;
+    const int G = (int)vertex_offsets.size() - 1;
+    if (G < 1) throw std::runtime_error("partition_batch: no graphs");
+    if ((int)graph_args.size() != G || (int)num_edges.size() != G)
+        throw std::runtime_error("partition_batch: one Args and one edge count per graph");
+    for (const Args& a : graph_args) {
+        if (a.algorithm != "async_gibbs")
+            throw std::runtime_error("partition_batch: a batch of graphs runs asynchronous Gibbs");
+        if (!a.approximate) throw std::runtime_error("partition_batch: a batch of graphs merges in the approximate mode");
+        if (!a.checkpoint.empty()) throw std::runtime_error("partition_batch: no checkpoints for a batch of graphs");
+    }
+    const int N = vertex_offsets.back();
+    Batch batch;
+    batch.vertex_offsets = vertex_offsets;
+    auto check = [&](sbpx_status st, const char* what) {
+        if (st != SBPX_OK) throw std::runtime_error(std::string(what) + ": " + sbpx_last_error(batch.ctx));
+    };
+    if (sbpx_create(&batch.ctx, device) != SBPX_OK) throw std::runtime_error("sbpx_create failed");
+    struct CtxGuard {
+        sbpx_ctx*& c;
+        ~CtxGuard() {
+            if (c) sbpx_destroy(c);
+        }
+    } guard{batch.ctx};
+    check(sbpx_graph_upload(batch.ctx, N, (int64_t)out_idx.size(), out_ptr.data(), out_idx.data(), in_ptr.data(),
+                            in_idx.data()),
+          "sbpx_graph_upload");
+    check(sbpx_batch_define(batch.ctx, G, vertex_offsets.data()), "sbpx_batch_define");
+    check(sbpx_set_option(batch.ctx, 3, eval_variant), "sbpx_set_option");
+
+    BatchRun run;
+    run.graphs.resize(G);
+    std::vector<std::string> errors(G);
+    batch.live = G;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<std::thread> threads;
+    threads.reserve(G);
+    for (int g = 0; g < G; ++g) {
+        threads.emplace_back([&, g] {
+            try {
+                auto dev = std::make_shared<Device>();
+                dev->batch = &batch;
+                dev->slot = g;
+                dev->N = vertex_offsets[g + 1] - vertex_offsets[g];
+                long E = num_edges[g] >= 0 ? num_edges[g]
+                                           : (long)(out_ptr[vertex_offsets[g + 1]] - out_ptr[vertex_offsets[g]]);
+                dev->E = (int)E;
+                Graph graph(dev->N, E, dev);
+                Args a = graph_args[g];
+                Blockmodel bm = stochastic_block_partition(graph, a);
+                BatchRun::Graph& out = run.graphs[g];
+                out.assignment = bm.block_assignment();
+                out.num_blocks = bm.getNum_blocks();
+                out.mdl = bm.getOverall_entropy();
+                out.intermediates = get_intermediates();
+                out.mcmc_iterations = finetune::MCMC_iterations;
+            } catch (const std::exception& e) {
+                errors[g] = e.what();
+            }
+            sbpx_host::leave(batch);
+        });
+    }
+    for (std::thread& t : threads) t.join();
+    run.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int g = 0; g < G; ++g)
+        if (!errors[g].empty()) throw std::runtime_error("graph " + std::to_string(g) + ": " + errors[g]);
+    run.rounds = batch.rounds;
+    run.sweeps = batch.sweeps;
+    run.totals = batch.totals;
+    return run;
+}
+
+}  // namespace sbp

@@ -164,6 +164,51 @@ int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int app
     }
 }
 
+// sbp::partition_batch: G graphs given as one union (vertex_offsets[G + 1]), each with its seed
+// and edge count (num_edges may be null: stored edges), async_gibbs with approximate merges and
+// `batches` batches; eval_variant as option 3 of sbpx_set_option. Writes graph-local assignments (N entries), blocks and MDL per graph, and
+// totals[4] = {wall seconds, rounds, batched sweeps, vertex-move evaluations}.
+int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t* out_ptr, const int32_t* out_idx,
+                        const int32_t* in_ptr, const int32_t* in_idx, const int64_t* num_edges, int32_t batches,
+                        const uint64_t* seeds, int device, int eval_variant, int32_t* assignment, int32_t* num_blocks,
+                        double* mdl, int32_t* mcmc_iterations, double* totals, char* err, int errlen) {
+    try {
+        const int N = vertex_offsets[G];
+        const int64_t E = out_ptr[N];
+        std::vector<long> ne(G, -1);
+        if (num_edges)
+            for (int g = 0; g < G; ++g) ne[g] = (long)num_edges[g];
+        std::vector<Args> args(G);
+        for (int g = 0; g < G; ++g) {
+            args[g].algorithm = "async_gibbs";
+            args[g].batches = batches;
+            args[g].approximate = true;
+            args[g].seed = seeds[g];
+        }
+        sbp::BatchRun run = sbp::partition_batch(
+            std::vector<int32_t>(vertex_offsets, vertex_offsets + G + 1), std::vector<int32_t>(out_ptr, out_ptr + N + 1),
+            std::vector<int32_t>(out_idx, out_idx + E), std::vector<int32_t>(in_ptr, in_ptr + N + 1),
+            std::vector<int32_t>(in_idx, in_idx + E), ne, args, device, eval_variant);
+        for (int g = 0; g < G; ++g) {
+            const sbp::BatchRun::Graph& r = run.graphs[g];
+            if (assignment) std::copy(r.assignment.begin(), r.assignment.end(), assignment + vertex_offsets[g]);
+            if (num_blocks) num_blocks[g] = r.num_blocks;
+            if (mdl) mdl[g] = r.mdl;
+            if (mcmc_iterations) mcmc_iterations[g] = r.mcmc_iterations;
+        }
+        if (totals) {
+            totals[0] = run.seconds;
+            totals[1] = (double)run.rounds;
+            totals[2] = (double)run.sweeps;
+            totals[3] = (double)run.totals.evals;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return 1;
+    }
+}
+
 int sbp_run(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, const int32_t* in_ptr, const int32_t* in_idx,
             int algorithm, int32_t batches, int approximate, uint64_t seed, int device, int count_bytes,
             int32_t* assignment, sbp_run_stats* stats, char* err, int errlen) {

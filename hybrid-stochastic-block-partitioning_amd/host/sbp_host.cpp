@@ -13,12 +13,54 @@
 
 #include "device.hpp"
 
-Args args;
+thread_local Args args;
 
 namespace sbpx_host {
 
 static void check(sbpx_ctx* ctx, sbpx_status st, const char* what) {
     if (st != SBPX_OK) throw std::runtime_error(std::string(what) + ": " + sbpx_last_error(ctx));
+}
+
+// Requests of a batch member go to its coordinator (batch.cpp).
+void batch_build(Batch& b, int slot, int num_blocks, const int32_t* assignment);
+void batch_download(Batch& b, int slot, int32_t* assignment);
+double batch_log_posterior(Batch& b, int slot);
+void batch_sweep(Batch& b, int slot, int algo, int batches, uint64_t seed, uint32_t call, sbpx_sweep_stats* stats);
+int batch_merge(Batch& b, int slot, int num_to_merge, int mode, uint64_t seed, uint32_t call);
+
+sbpx_ctx* Device::own_ctx(const char* what) const {
+    if (!ctx) throw std::runtime_error(std::string(what) + ": not available for a graph of a batch");
+    return ctx;
+}
+
+void Device::build(int num_blocks, const int32_t* assignment) {
+    if (batch) return batch_build(*batch, slot, num_blocks, assignment);
+    check(ctx, sbpx_blockmodel_build(ctx, num_blocks, assignment), "sbpx_blockmodel_build");
+}
+
+void Device::download(int32_t* assignment) {
+    if (batch) return batch_download(*batch, slot, assignment);
+    check(ctx, sbpx_blockmodel_download(ctx, assignment, nullptr, nullptr, nullptr), "download");
+}
+
+double Device::log_posterior() {
+    if (batch) return batch_log_posterior(*batch, slot);
+    double lp = 0.0;
+    check(ctx, sbpx_log_posterior(ctx, &lp), "sbpx_log_posterior");
+    return lp;
+}
+
+void Device::sweep(int algo, int batches, uint64_t seed, uint32_t call, sbpx_sweep_stats* stats) {
+    if (batch) return batch_sweep(*batch, slot, algo, batches, seed, call, stats);
+    check(ctx, sbpx_sweep(ctx, algo, batches, seed, call, stats), "sbpx_sweep");
+}
+
+int Device::merge(int num_to_merge, int mode, uint64_t seed, uint32_t call) {
+    if (batch) return batch_merge(*batch, slot, num_to_merge, mode, seed, call);
+    check(ctx, sbpx_merge_blocks(ctx, num_to_merge, mode, seed, call), "sbpx_merge_blocks");
+    int32_t B = 0;
+    check(ctx, sbpx_blockmodel_info(ctx, &B, nullptr), "sbpx_blockmodel_info");
+    return B;
 }
 
 }  // namespace sbpx_host
@@ -61,18 +103,18 @@ Blockmodel::Blockmodel(int num_blocks_, const Graph& graph, float rate, std::vec
     num_blocks = num_blocks_;
     block_reduction_rate = rate;
     num_blocks_to_merge = (int)(num_blocks * block_reduction_rate);
-    check(dev_->ctx, sbpx_blockmodel_build(dev_->ctx, num_blocks, block_assignment.data()), "sbpx_blockmodel_build");
+    dev_->build(num_blocks, block_assignment.data());
     dev_->live_id = id_;
 }
 
-sbpx_ctx* Blockmodel::ctx() const { return dev_->ctx; }
+sbpx_ctx* Blockmodel::ctx() const { return dev_->own_ctx("Blockmodel::ctx"); }
 void Blockmodel::mark_live() const { dev_->live_id = id_; }
 
 void Blockmodel::make_live() const {
     if (empty || !dev_) throw std::runtime_error("empty blockmodel");
     if (dev_->live_id == id_) return;
     if ((int)snapshot_.size() != dev_->N) throw std::runtime_error("blockmodel state was overwritten");
-    check(dev_->ctx, sbpx_blockmodel_build(dev_->ctx, num_blocks, snapshot_.data()), "sbpx_blockmodel_build");
+    dev_->build(num_blocks, snapshot_.data());
     dev_->live_id = id_;
 }
 
@@ -94,25 +136,28 @@ Blockmodel Blockmodel::copy() {
 std::vector<int> Blockmodel::block_assignment() const {
     if (dev_->live_id != id_) return snapshot_;
     std::vector<int> a(dev_->N);
-    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, a.data(), nullptr, nullptr, nullptr), "download");
+    dev_->download(a.data());
     return a;
 }
 std::vector<int> Blockmodel::degrees() const {
     make_live();
     std::vector<int> d(num_blocks);
-    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, nullptr, nullptr, nullptr, d.data()), "download");
+    sbpx_ctx* c = dev_->own_ctx("Blockmodel::degrees");
+    check(c, sbpx_blockmodel_download(c, nullptr, nullptr, nullptr, d.data()), "download");
     return d;
 }
 std::vector<int> Blockmodel::degrees_in() const {
     make_live();
     std::vector<int> d(num_blocks);
-    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, nullptr, nullptr, d.data(), nullptr), "download");
+    sbpx_ctx* c = dev_->own_ctx("Blockmodel::degrees_in");
+    check(c, sbpx_blockmodel_download(c, nullptr, nullptr, d.data(), nullptr), "download");
     return d;
 }
 std::vector<int> Blockmodel::degrees_out() const {
     make_live();
     std::vector<int> d(num_blocks);
-    check(dev_->ctx, sbpx_blockmodel_download(dev_->ctx, nullptr, d.data(), nullptr, nullptr), "download");
+    sbpx_ctx* c = dev_->own_ctx("Blockmodel::degrees_out");
+    check(c, sbpx_blockmodel_download(c, nullptr, d.data(), nullptr, nullptr), "download");
     return d;
 }
 
@@ -229,8 +274,7 @@ namespace entropy {
 // minus the device's log posterior.
 double mdl(const Blockmodel& blockmodel, int num_vertices, int num_edges) {
     blockmodel.make_live();
-    double log_posterior = 0.0;
-    check(blockmodel.ctx(), sbpx_log_posterior(blockmodel.ctx(), &log_posterior), "sbpx_log_posterior");
+    const double log_posterior = blockmodel.device().log_posterior();
     const double B = blockmodel.getNum_blocks();
     const double x = pow(B, 2) / num_edges;
     const double h = ((1 + x) * log(1 + x)) - (x * log(x));
@@ -256,9 +300,9 @@ double normalize_mdl_v2(double mdl, long num_vertices, long num_edges) {
 
 // ---------------------------------------------------------------------------- finetune
 namespace finetune {
-int MCMC_iterations = 0;
-double MCMC_time = 0.0;
-Totals totals;
+thread_local int MCMC_iterations = 0;
+thread_local double MCMC_time = 0.0;
+thread_local Totals totals;
 
 bool early_stop(int iteration, BlockmodelTriplet& blockmodels, double initial_entropy,
                 std::vector<double>& delta_entropies) {
@@ -295,7 +339,7 @@ static Blockmodel& batched_mcmc(int algo, Blockmodel& blockmodel, const Graph& g
     double last_iteration_entropy = initial_entropy;
     for (int iteration = 0; iteration < MAX_NUM_ITERATIONS; ++iteration) {
         sbpx_sweep_stats st{};
-        check(dev.ctx, sbpx_sweep(dev.ctx, algo, args.batches, args.seed, dev.sweep_call++, &st), "sbpx_sweep");
+        dev.sweep(algo, args.batches, args.seed, dev.sweep_call++, &st);
         add_totals(st);
         double entropy = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
         delta_entropies.push_back(entropy - last_iteration_entropy);
@@ -324,9 +368,7 @@ Blockmodel& metropolis_hastings(Blockmodel& blockmodel, const Graph& graph, Bloc
     blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
     for (int iteration = 0; iteration < MAX_NUM_ITERATIONS; ++iteration) {
         sbpx_sweep_stats st{};
-        check(dev.ctx,
-              sbpx_sweep(dev.ctx, SBPX_ALGO_METROPOLIS_HASTINGS, args.batches, args.seed, dev.sweep_call++, &st),
-              "sbpx_sweep");
+        dev.sweep(SBPX_ALGO_METROPOLIS_HASTINGS, args.batches, args.seed, dev.sweep_call++, &st);
         add_totals(st);
         delta_entropies.push_back(st.dS_accepted);
         MCMC_iterations++;
@@ -344,11 +386,7 @@ Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edg
     blockmodel.make_live();
     Device& dev = *graph.device();
     int mode = args.approximate ? SBPX_MERGE_APPROXIMATE : SBPX_MERGE_ADVANCED;
-    check(dev.ctx,
-          sbpx_merge_blocks(dev.ctx, blockmodel.getNum_blocks_to_merge(), mode, args.seed, dev.merge_call++),
-          "sbpx_merge_blocks");
-    int32_t B = 0;
-    check(dev.ctx, sbpx_blockmodel_info(dev.ctx, &B, nullptr), "sbpx_blockmodel_info");
+    const int B = dev.merge(blockmodel.getNum_blocks_to_merge(), mode, args.seed, dev.merge_call++);
     blockmodel.setNum_blocks(B);
     blockmodel.mark_live();
     return blockmodel;
@@ -357,8 +395,8 @@ Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edg
 
 // ---------------------------------------------------------------------------- driver
 namespace sbp {
-static std::vector<Intermediate> intermediate_results;
-static std::chrono::steady_clock::time_point partition_start;
+static thread_local std::vector<Intermediate> intermediate_results;
+static thread_local std::chrono::steady_clock::time_point partition_start;
 std::vector<Intermediate> get_intermediates() { return intermediate_results; }
 
 static void add_intermediate(float iteration, const Blockmodel& blockmodel, double mdl) {

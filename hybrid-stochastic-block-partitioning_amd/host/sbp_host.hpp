@@ -58,6 +58,9 @@ public:
           std::vector<int32_t> in_idx, int device = 0, long num_edges = -1);
     /// Graph::load (graph.cpp:24-62) from a path without extension.
     static Graph load(const std::string& base_path, bool undirected = false, int device = 0);
+    /// A graph of a batch: its device steps run through the batch's shared context.
+    Graph(int num_vertices, long num_edges, std::shared_ptr<sbpx_host::Device> dev)
+        : num_vertices_(num_vertices), num_edges_((int)num_edges), dev_(std::move(dev)) {}
     int num_vertices() const { return num_vertices_; }
     int num_edges() const { return num_edges_; }
     const std::vector<int>& high_degree_vertices() const { return high_; }
@@ -93,7 +96,8 @@ public:
 
     // device binding
     void make_live() const;
-    sbpx_ctx* ctx() const;
+    sbpx_ctx* ctx() const;  // the graph's own context (throws for a graph of a batch)
+    sbpx_host::Device& device() const { return *dev_; }
     void mark_live() const;
     // checkpoint I/O of the fields and the assignment (the device state is rebuilt when used)
     void write(std::FILE* f) const;
@@ -137,15 +141,15 @@ double normalize_mdl_v2(double mdl, long num_vertices, long num_edges);
 }  // namespace entropy
 
 namespace finetune {
-extern int MCMC_iterations;
-extern double MCMC_time;
+extern thread_local int MCMC_iterations;
+extern thread_local double MCMC_time;
 static const int MAX_NUM_ITERATIONS = 100;
 struct Totals {
     long long sweeps = 0, attempts = 0, evals = 0, moves = 0;
     double eval_ms[4] = {0, 0, 0, 0}, eval_bytes[4] = {0, 0, 0, 0};
     long long eval_launches[4] = {0, 0, 0, 0};
 };
-extern Totals totals;
+extern thread_local Totals totals;
 Blockmodel& asynchronous_gibbs(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
 Blockmodel& hybrid_mcmc(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
 Blockmodel& metropolis_hastings(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
@@ -169,7 +173,32 @@ struct Intermediate {
 };
 std::vector<Intermediate> get_intermediates();
 Blockmodel stochastic_block_partition(Graph& graph, Args& args);
+
+/// Many independent graphs partitioned side by side on one GPU (batch.cpp). The graphs are given
+/// as one union: graph g is the vertex range [vertex_offsets[g], vertex_offsets[g + 1]) and no edge
+/// joins two graphs. Each runs stochastic_block_partition with its own Args (async_gibbs with
+/// approximate merges) and ends as it would alone; num_edges[g] < 0 means its stored edges.
+/// eval_variant: the evaluation kernel variant (option 3 of sbpx_set_option; 0 = tuned).
+struct BatchRun {
+    struct Graph {
+        std::vector<int> assignment;  // graph-local block ids
+        int num_blocks = 0;
+        double mdl = 0.0;             // as the driver caches it (float)
+        int mcmc_iterations = 0;
+        std::vector<Intermediate> intermediates;
+    };
+    std::vector<Graph> graphs;
+    double seconds = 0.0;  // wall time of the drivers (graphs resident)
+    long rounds = 0;       // rounds of batched device steps
+    long long sweeps = 0;  // batched sweeps
+    sbpx_sweep_stats totals{};
+};
+BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::vector<int32_t>& out_ptr,
+                         const std::vector<int32_t>& out_idx, const std::vector<int32_t>& in_ptr,
+                         const std::vector<int32_t>& in_idx, const std::vector<long>& num_edges,
+                         const std::vector<Args>& graph_args, int device = 0, int eval_variant = 0);
 bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks = 0);
 }  // namespace sbp
 
-extern Args args;
+// The driver's globals are per thread, so that the graphs of a batch run their drivers side by side.
+extern thread_local Args args;

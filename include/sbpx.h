@@ -156,6 +156,45 @@ sbpx_status sbpx_learned_forward(sbpx_ctx* ctx, int32_t n, const float* x, float
 sbpx_status sbpx_learned_gradients(sbpx_ctx* ctx, int32_t n, const float* x, const uint32_t* mask,
                                    const int32_t* y, float* loss, float* grads);
 
+/* ---- A batch of independent graphs in one context ------------------------------------
+ * The reference partitions one graph per run; these calls hold many graphs as one disjoint
+ * union on the device and run each step of the driver for any subset of them in one set of
+ * launches. Every graph follows exactly the run it would have alone with the same seed and
+ * calls: randomness is keyed by graph-local ids, block ids keep the graph's own table slots,
+ * and the log posterior is summed per graph. Graph g is the vertex range
+ * [vertex_offsets[g], vertex_offsets[g + 1]) of the uploaded graph; block ids in these calls
+ * are graph-local. While a batch is defined, the one-graph blockmodel, sweep and merge calls
+ * return SBPX_ERR_STATE. */
+
+/* Defines the batch over the uploaded graph (no edge may join two graphs) and builds every
+ * graph's blockmodel with one block per vertex (Blockmodel(N, graph, rate), blockmodel.hpp:63-66). */
+sbpx_status sbpx_batch_define(sbpx_ctx* ctx, int32_t num_graphs, const int32_t* vertex_offsets);
+
+/* Number of graphs and the blocks in use per graph (num_blocks[num_graphs]); either may be NULL. */
+sbpx_status sbpx_batch_info(sbpx_ctx* ctx, int32_t* num_graphs, int32_t* num_blocks);
+
+/* Rebuilds the blockmodels of the n listed graphs from their assignments (Blockmodel(num_blocks,
+ * graph, rate, assignment), blockmodel.hpp:67-73); `assignment` holds the graphs' vertex
+ * assignments one after the other, in the order of `graphs`. */
+sbpx_status sbpx_batch_build(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, const int32_t* num_blocks,
+                             const int32_t* assignment);
+
+/* Every vertex's graph-local block (N entries). */
+sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t* assignment);
+
+/* Log posterior of every graph (out[num_graphs]; Blockmodel::log_posterior_p, blockmodel.cpp:267-285). */
+sbpx_status sbpx_batch_log_posterior(sbpx_ctx* ctx, double* out);
+
+/* One asynchronous Gibbs sweep (finetune.cpp:31-89, one iteration) of each listed graph, with
+ * its own seed and call; stats are totals over the graphs. */
+sbpx_status sbpx_batch_sweep(sbpx_ctx* ctx, int algo, int32_t batches, int32_t n, const int32_t* graphs,
+                             const uint64_t* seeds, const uint32_t* calls, sbpx_sweep_stats* stats);
+
+/* One merge phase (block_merge::merge_blocks, block_merge.cpp:278-311) of each listed graph,
+ * merging num_to_merge[i] blocks of graphs[i]; SBPX_MERGE_APPROXIMATE only. */
+sbpx_status sbpx_batch_merge(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, const int32_t* num_to_merge, int mode,
+                             const uint64_t* seeds, const uint32_t* calls);
+
 /* Self-test of the device's FP32 log (the function the objective uses, utils.hpp:145-151):
  * out[i] = log of the float whose bit pattern is lo + i, for i < n. */
 sbpx_status sbpx_selftest_logf(sbpx_ctx* ctx, uint32_t lo, uint32_t n, float* out);

@@ -1,0 +1,132 @@
+"""GPU tests of many graphs partitioned side by side in one context (the batched form): every
+batched device step leaves each listed graph exactly as the same step leaves it in a context of
+its own (assignment, block count and log posterior bit for bit) and the graphs it does not list
+untouched; whole batched runs end exactly as the runs of the graphs alone with the same seeds."""
+import numpy as np
+import pytest
+
+from pkg import sbpx
+from test_gpu_parity import heavy_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _graphs(sizes, seed0=0):
+    m = sbpx()
+    out = []
+    for i, n in enumerate(sizes):
+        K = max(2, int(round(1.5 * n ** 0.4)))
+        op, oi, ip, ii, _ = m.generate_dcsbm(n, K, 0.55 + 0.05 * (i % 6), 1.0 if i % 2 else 20.0, 5, 100, 2.1,
+                                            seed=seed0 + i)
+        out.append((op, oi, ip, ii))
+    return out
+
+
+def _singles(m, graphs):
+    cs = []
+    for g in graphs:
+        c = m.Context(0)
+        c.upload_graph(*g)
+        n = len(g[0]) - 1
+        c.build(n, np.arange(n, dtype=np.int32))
+        c.set_option(m.OPT_EVAL_VARIANT, 1)
+        cs.append(c)
+    return cs
+
+
+def _check(cb, offs, singles):
+    a = cb.batch_download()
+    nb = cb.batch_info()
+    lp = cb.batch_log_posterior()
+    for g, c in enumerate(singles):
+        B, _ = c.info()
+        assert nb[g] == B, g
+        assert (a[offs[g]:offs[g + 1]] == c.state()[0]).all(), g
+        assert lp[g] == c.log_posterior(), g
+
+
+def test_batched_steps_match_separate_contexts():
+    m = sbpx()
+    graphs = _graphs([400, 1100, 250]) + [heavy_arrays(3)] + _graphs([800], seed0=9)
+    offs, op, oi, ip, ii = m.union_graphs(graphs)
+    cb = m.Context(0)
+    cb.upload_graph(op, oi, ip, ii)
+    cb.batch_define(offs)
+    cb.set_option(m.OPT_EVAL_VARIANT, 1)
+    singles = _singles(m, graphs)
+    _check(cb, offs, singles)
+    G = len(graphs)
+    seeds = np.arange(G, dtype=np.uint64) * 7 + 1
+
+    def merge(gs, frac, call):
+        k = [int(cb.batch_info()[g] * frac) for g in gs]
+        cb.batch_merge(gs, k, seeds[gs], np.full(len(gs), call, np.uint32))
+        for g, kk in zip(gs, k):
+            singles[g].merge_blocks(kk, int(seeds[g]), call)
+
+    def sweep(gs, call, batches=1):
+        cb.batch_sweep(gs, seeds[gs], np.full(len(gs), call, np.uint32), batches=batches)
+        for g in gs:
+            singles[g].sweep(m.ALGO_ASYNC_GIBBS, batches, int(seeds[g]), call)
+
+    merge([0, 1, 3, 4], 0.5, 0)
+    _check(cb, offs, singles)
+    sweep([1, 2, 3], 0)
+    _check(cb, offs, singles)
+    merge([2], 0.5, 1)
+    sweep(list(range(G)), 1)
+    _check(cb, offs, singles)
+    sweep([0, 3, 4], 2, batches=3)
+    sweep([0, 1, 2, 3, 4], 3, batches=3)
+    _check(cb, offs, singles)
+    merge([0, 1, 2, 3, 4], 0.6, 2)
+    for call in range(4, 8):
+        sweep([g for g in range(G) if (g + call) % 3], call)
+    _check(cb, offs, singles)
+    # a graph rebuilt from an assignment of its own
+    a = singles[1].state()[0]
+    B = int(a.max()) + 1
+    perm = np.random.default_rng(0).permutation(B).astype(np.int32)
+    cb.batch_build([1], [B], perm[a])
+    singles[1].build(B, perm[a])
+    sweep([1, 4], 9)
+    _check(cb, offs, singles)
+
+
+def test_batch_rejects_what_it_does_not_run():
+    m = sbpx()
+    graphs = _graphs([300, 200])
+    offs, op, oi, ip, ii = m.union_graphs(graphs)
+    cb = m.Context(0)
+    cb.upload_graph(op, oi, ip, ii)
+    with pytest.raises(m.SbpxError):
+        cb.batch_define(np.array([0, 250, 500], np.int32))  # an edge would join two graphs
+    cb.batch_define(offs)
+    seeds, calls = np.ones(2, np.uint64), np.zeros(2, np.uint32)
+    with pytest.raises(m.SbpxError):
+        cb.sweep(m.ALGO_ASYNC_GIBBS, 1, 0, 0)
+    with pytest.raises(m.SbpxError):
+        cb.batch_sweep([0, 1], seeds, calls, algo=m.ALGO_HYBRID)
+    with pytest.raises(m.SbpxError):
+        cb.batch_merge([0, 1], [10, 10], seeds, calls, mode=m.MERGE_ADVANCED)
+    with pytest.raises(m.SbpxError):
+        cb.batch_merge([0, 0], [10, 10], seeds, calls)
+    with pytest.raises(m.SbpxError):
+        cb.batch_merge([1], [200], seeds[:1], calls[:1])
+
+
+def test_batched_runs_end_as_runs_alone():
+    m = sbpx()
+    graphs = _graphs([1500, 300, 700, 2200, 450, 1000], seed0=20)
+    seeds = [11, 12, 13, 14, 15, 16]
+    runs, totals = m.partition_batch(graphs, batches=1, seeds=seeds, eval_variant=1)
+    assert totals["rounds"] > 0 and totals["sweeps"] > 0
+    for g, (arrays, seed) in enumerate(zip(graphs, seeds)):
+        rg = m.ResidentGraph(*arrays)
+        rg.set_option(m.OPT_EVAL_VARIANT, 1)
+        a, st = rg.partition(algorithm="async_gibbs", batches=1, approximate=True, seed=seed)
+        assignment, B, mdl, _ = runs[g]
+        assert B == st.num_blocks, g
+        assert (assignment == a).all(), g
+        assert mdl == st.mdl, g
+        rg.close()

@@ -128,6 +128,128 @@ def cpu_baseline(arrays, seconds, seed, algorithm):
     }
 
 
+def bench_batch(args, m):
+    """The batched workload: `args.graphs` DC-SBM graphs of uneven size partitioned side by side in
+    one context (async_gibbs, approximate merges). value = graphs partitioned per second with the
+    graphs resident; beside it the same graphs partitioned one at a time on the GPU (a sample),
+    the per-graph MDL of the batched run against those runs and against the CPU oracle's runs."""
+    import oracle_lib as O
+
+    params = m.c3_graph_params(args.graphs, args.seed)
+    graphs, truths = [], []
+    for p in params:
+        op, oi, ip, ii, truth = m.generate_dcsbm(**p)
+        graphs.append((op, oi, ip, ii))
+        truths.append(truth)
+    sizes = np.array([len(g[0]) - 1 for g in graphs])
+    edges = np.array([int(g[0][-1]) for g in graphs])
+    seeds = [args.seed * 7919 + g for g in range(len(graphs))]
+    print(f"# batch: {len(graphs)} graphs, N={sizes.sum()} E={edges.sum()}", file=sys.stderr, flush=True)
+    for w in range(args.warmup):
+        _, wt = m.partition_batch(graphs[:16], batches=args.batches, seeds=seeds[:16])
+        print(f"# warmup {w}: {wt['seconds']:.2f} s for 16 graphs", file=sys.stderr, flush=True)
+    times, runs, tot = [], None, None
+    for k in range(args.steps):
+        runs, tot = m.partition_batch(graphs, batches=args.batches, seeds=seeds)
+        times.append(tot["seconds"])
+        print(f"# step {k}: {tot['seconds']:.2f} s, {tot['rounds']} rounds, {tot['sweeps']} batched sweeps",
+              file=sys.stderr, flush=True)
+    t_batch = float(np.mean(times))
+    if args.no_count:
+        kernel_bytes = np.zeros(4)
+    else:
+        _, ctot = m.partition_batch(graphs, batches=args.batches, seeds=seeds, count_bytes=True)
+        kernel_bytes = np.array(ctot["eval_bytes"])
+    ms = np.array(tot["eval_ms"])
+    launches = np.array(tot["eval_launches"])
+    gbs = np.where(ms > 0, kernel_bytes / np.maximum(ms, 1e-12) / 1e6, 0.0)
+    dom = int(np.argmax(ms))
+    traffic = measured_traffic()
+    dom_name = KERNELS[dom].split(" ")[0]
+    # the same graphs one at a time on the GPU: a sample, timed, and the per-graph parity
+    rng = np.random.default_rng(args.seed)
+    sample = sorted(rng.choice(len(graphs), size=min(args.sample, len(graphs)), replace=False).tolist())
+    t_alone, same, rel = 0.0, 0, []
+    for g in sample:
+        rg = m.ResidentGraph(*graphs[g])
+        t0 = time.perf_counter()
+        a, st = rg.partition(algorithm="async_gibbs", batches=args.batches, approximate=True, seed=seeds[g])
+        t_alone += time.perf_counter() - t0
+        rg.close()
+        same += int((a == runs[g][0]).all() and st.mdl == runs[g][2])
+        rel.append(abs(runs[g][2] / st.mdl - 1.0))
+    alone_rate = sizes[sample].sum() / t_alone if sample else None  # vertices per second, one graph at a time
+    projected_alone = sizes.sum() / alone_rate if sample else None
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "sbpx_eval", os.path.join(ROOT, "hybrid-stochastic-block-partitioning_amd", "evaluation.py"))
+    ev = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(ev)
+    nmis = [ev.nmi(truths[g], runs[g][0]) for g in range(len(graphs))]
+    out = {
+        "metric": "graphs partitioned/s, batched plain SBP (per-graph MDL parity), 1 GPU",
+        "value": len(graphs) / t_batch,
+        "unit": "graphs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * t_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 state, f32 cells, f64 sums",
+        "data": "synthetic DC-SBM (seeded generator)",
+        "config": {"workload": f"C3 batched plain async_gibbs SBP, {len(graphs)} graphs, batches={args.batches}, "
+                               "--approximate",
+                   "seed": args.seed, "num_graphs": len(graphs), "total_vertices": int(sizes.sum()),
+                   "total_edges": int(edges.sum()), "vertices_per_graph": [int(sizes.min()), int(sizes.max())],
+                   "deg": [5, 100]},
+        "rounds": tot["rounds"],
+        "batched_sweeps": tot["sweeps"],
+        "evals_per_s": tot["evals"] / t_batch,
+        "alone_on_gpu": {"graphs": len(sample), "seconds": t_alone, "vertices_per_s": alone_rate,
+                         "projected_seconds_all": projected_alone,
+                         "batched_speedup": projected_alone / t_batch if sample else None,
+                         "identical_results": same, "max_mdl_rel_diff": float(max(rel)) if rel else None},
+        "nmi_vs_planted": {"mean": float(np.mean(nmis)), "min": float(np.min(nmis))},
+        "roofline": {
+            "bound": "hbm", "kernel": KERNELS[dom], "achieved": float(gbs[dom]), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": float(gbs[dom]) / HBM_PEAK_GBS, "traffic": None,
+            "bytes_per_run": float(kernel_bytes[dom]), "ms_per_run": float(ms[dom]),
+            "launches_per_run": float(launches[dom]),
+            "avg_launch_us": 1000.0 * float(ms[dom]) / max(1.0, float(launches[dom]))},
+        "eval_kernels": [
+            {"kernel": KERNELS[i].split(" ")[0], "ms_per_run": float(ms[i]), "launches_per_run": float(launches[i]),
+             "bytes_per_run": float(kernel_bytes[i]), "achieved_GBs": float(gbs[i]),
+             "frac": float(gbs[i]) / HBM_PEAK_GBS}
+            for i in range(4) if launches[i] > 0],
+    }
+    if dom_name in traffic:
+        out["roofline"]["traffic_note"] = "PMC traffic in profiles/ is for the one-graph workload"
+    if not args.no_cpu and sample:
+        # the CPU oracle's whole runs on graphs of the batch, in the sample's order, for about cpu_seconds
+        t_cpu, done, verts, ratios = 0.0, 0, 0, []
+        for g in sample:
+            og = O.Graph(int(sizes[g]), *graphs[g])
+            t0 = time.perf_counter()
+            _, orun = O.sbp(og, 1, args.batches, True, seeds[g])
+            t_cpu += time.perf_counter() - t0
+            done += 1
+            verts += int(sizes[g])
+            ratios.append(runs[g][2] / orun.mdl)
+            if t_cpu > args.cpu_seconds:
+                break
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {
+            "value": (verts / t_cpu) / float(sizes.mean()), "unit": "graphs/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle async_gibbs SBP (approximate merges) run whole on {done} graphs of the batch "
+                       f"({verts} vertices) in {t_cpu:.1f} s; graphs/s scaled to the batch's mean graph size "
+                       f"({sizes.mean():.0f} vertices); OpenMP threads={threads}"),
+            "mdl_batched_over_oracle": [float(r) for r in ratios]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +264,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the untimed byte-counting run")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
+    ap.add_argument("--graphs", type=int, default=512, help="graphs of the batched workload (--config C3)")
+    ap.add_argument("--sample", type=int, default=12, help="C3: graphs also run alone (GPU) and by the oracle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device 10 times (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -161,6 +285,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     m = load_pkg()
+    if args.config == "C3":
+        return bench_batch(args, m)
     cfg = m.CONFIGS[args.config]
     out_ptr, out_idx, in_ptr, in_idx, truth = m.generate_dcsbm(**cfg, seed=args.seed)
     arrays = (out_ptr, out_idx, in_ptr, in_idx)

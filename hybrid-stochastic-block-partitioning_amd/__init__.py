@@ -29,6 +29,21 @@ CONFIGS = {
 }
 
 
+def c3_graph_params(count=512, seed=0):
+    """Generator parameters of the batched workload: `count` DC-SBM graphs with N log-uniform in
+    [1000, 20000], K = round(1.5 N^0.4), p_in uniform in [0.55, 0.85], Dirichlet concentration 1
+    or 20, out-degrees on [5, 100] with exponent 2.1; graph g is generated with seed
+    seed * 1000003 + g."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for g in range(count):
+        n = int(round(np.exp(rng.uniform(np.log(1000.0), np.log(20000.0)))))
+        out.append(dict(num_vertices=n, num_blocks=int(round(1.5 * n ** 0.4)), p_in=float(rng.uniform(0.55, 0.85)),
+                        dirichlet=float(rng.choice([1.0, 20.0])), deg_min=5, deg_max=100, exponent=2.1,
+                        seed=seed * 1000003 + g))
+    return out
+
+
 class SweepStats(C.Structure):
     _fields_ = [("attempts", C.c_int64), ("evals", C.c_int64), ("moves", C.c_int64), ("dS_accepted", C.c_double),
                 ("eval_ms", C.c_double * 4), ("eval_bytes", C.c_double * 4), ("eval_launches", C.c_int32 * 4)]
@@ -95,7 +110,7 @@ def load():
     L.sbpx_batch_define.argtypes = [vp, C.c_int32, i32p]
     L.sbpx_batch_info.argtypes = [vp, C.POINTER(C.c_int32), vp]
     L.sbpx_batch_build.argtypes = [vp, C.c_int32, i32p, i32p, i32p]
-    L.sbpx_batch_download.argtypes = [vp, i32p]
+    L.sbpx_batch_download.argtypes = [vp, C.c_int32, i32p, i32p]
     L.sbpx_batch_log_posterior.argtypes = [vp, f64p]
     L.sbpx_batch_sweep.argtypes = [vp, C.c_int, C.c_int32, C.c_int32, i32p,
                                    np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
@@ -140,7 +155,7 @@ def load_host():
                                 C.POINTER(RunStats), C.c_char_p, C.c_int]
     H.sbp_partition_batch.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_void_p, C.c_int32,
                                       np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"), C.c_int, C.c_int,
-                                      i32p, i32p, f64p, i32p, f64p, C.c_char_p, C.c_int]
+                                      C.c_int, i32p, i32p, f64p, i32p, f64p, C.c_char_p, C.c_int]
     _host = H
     return H
 
@@ -309,6 +324,7 @@ class Context:
     def batch_define(self, vertex_offsets):
         offs = np.ascontiguousarray(vertex_offsets, np.int32)
         self._chk(self.L.sbpx_batch_define(self.h, len(offs) - 1, offs))
+        self._batch_sizes = np.diff(offs)
 
     def batch_info(self):
         G = C.c_int32()
@@ -322,9 +338,12 @@ class Context:
         self._chk(self.L.sbpx_batch_build(self.h, len(g), g, np.ascontiguousarray(num_blocks, np.int32),
                                           np.ascontiguousarray(assignment, np.int32)))
 
-    def batch_download(self):
-        a = np.zeros(self.N, np.int32)
-        self._chk(self.L.sbpx_batch_download(self.h, a))
+    def batch_download(self, graphs=None):
+        """Graph-local assignments of the listed graphs (default all), one after the other."""
+        g = np.arange(len(self.batch_info()), dtype=np.int32) if graphs is None else np.ascontiguousarray(graphs, np.int32)
+        n = int(sum(self._batch_sizes[x] for x in g))
+        a = np.zeros(n, np.int32)
+        self._chk(self.L.sbpx_batch_download(self.h, len(g), g, a))
         return a
 
     def batch_log_posterior(self):
@@ -443,12 +462,12 @@ def union_graphs(graphs):
     return np.array(offs, np.int32), cat(ops), cat(ois), cat(ips), cat(iis)
 
 
-def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eval_variant=0):
+def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eval_variant=0, count_bytes=False):
     """sbp::partition_batch: each graph of the list (out_ptr, out_idx, in_ptr, in_idx) partitioned by
     async_gibbs with approximate merges, all on one GPU at once; graph g runs as it would alone with
-    seed seeds[g] (eval_variant as OPT_EVAL_VARIANT). Returns (list of (assignment, num_blocks, mdl,
-    mcmc_iterations), totals) with
-    totals = dict(seconds, rounds, sweeps, evals)."""
+    seed seeds[g] (eval_variant as OPT_EVAL_VARIANT; count_bytes counts the evaluation kernels'
+    algorithmic bytes). Returns (list of (assignment, num_blocks, mdl, mcmc_iterations), totals) with
+    totals = dict(seconds, rounds, sweeps, evals, eval_ms[4], eval_bytes[4], eval_launches[4])."""
     H = load_host()
     G = len(graphs)
     offs, op, oi, ip, ii = union_graphs(graphs)
@@ -458,14 +477,15 @@ def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eva
     B = np.zeros(G, np.int32)
     mdl = np.zeros(G)
     its = np.zeros(G, np.int32)
-    tot = np.zeros(4)
+    tot = np.zeros(16)
     err = C.create_string_buffer(512)
     rc = H.sbp_partition_batch(G, offs, op, oi, ip, ii, None if ne is None else ne.ctypes.data, batches, seeds, device,
-                               int(eval_variant), a, B, mdl, its, tot, err, 512)
+                               int(eval_variant), int(count_bytes), a, B, mdl, its, tot, err, 512)
     if rc != 0:
         raise RuntimeError("sbp_partition_batch: " + err.value.decode())
     out = [(a[offs[g]:offs[g + 1]].copy(), int(B[g]), float(mdl[g]), int(its[g])) for g in range(G)]
-    return out, dict(seconds=tot[0], rounds=int(tot[1]), sweeps=int(tot[2]), evals=int(tot[3]))
+    return out, dict(seconds=tot[0], rounds=int(tot[1]), sweeps=int(tot[2]), evals=int(tot[3]),
+                     eval_ms=tot[4:8].copy(), eval_bytes=tot[8:12].copy(), eval_launches=tot[12:16].copy())
 
 
 class ResidentGraph:

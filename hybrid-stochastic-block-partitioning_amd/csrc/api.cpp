@@ -1455,16 +1455,26 @@ sbpx_status sbpx_batch_build(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, co
     });
 }
 
-sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t* assignment) {
+sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, int32_t* assignment) {
     return guarded(ctx, [&] {
         Ctx& c = ctx->c;
         require_batch(c);
-        if (!assignment) throw ArgError("null assignment");
         const DevBatch& b = c.batch;
-        SBPX_HIP(hipMemcpyAsync(assignment, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToHost, c.stream));
+        if (n < 0 || n > b.G) throw ArgError("number of graphs out of range");
+        if (n > 0 && (!graphs || !assignment)) throw ArgError("null graph list or assignment");
+        std::vector<size_t> at(n + 1, 0);
+        for (int i = 0; i < n; ++i) {
+            if (graphs[i] < 0 || graphs[i] >= b.G) throw ArgError("graph index out of range");
+            at[i + 1] = at[i] + (b.vo[graphs[i] + 1] - b.vo[graphs[i]]);
+        }
+        for (int i = 0; i < n; ++i) {
+            const int g = graphs[i];
+            SBPX_HIP(hipMemcpyAsync(assignment + at[i], c.bm.assign + b.vo[g], sizeof(int) * (at[i + 1] - at[i]),
+                                    hipMemcpyDeviceToHost, c.stream));
+        }
         c.sync();
-        for (int g = 0; g < b.G; ++g)
-            for (int v = b.vo[g]; v < b.vo[g + 1]; ++v) assignment[v] -= b.bo[g];
+        for (int i = 0; i < n; ++i)
+            for (size_t j = at[i]; j < at[i + 1]; ++j) assignment[j] -= b.bo[graphs[i]];
     });
 }
 

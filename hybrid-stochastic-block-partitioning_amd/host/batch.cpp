@@ -51,7 +51,11 @@ struct Batch {
     long rounds = 0;
     sbpx_sweep_stats totals{};  // sweep statistics over all graphs
     long long sweeps = 0;
+    // log posterior of every graph after the last round that changed a state; a graph's entry
+    // stays valid until that graph asks for a step that changes its state (no other graph's step
+    // touches it), so its MDL queries need no round of their own
     std::vector<double> lp;
+    std::vector<char> lp_valid;
     std::vector<int32_t> assign;
 };
 
@@ -69,12 +73,24 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
     std::map<int, std::vector<Request*>> by_kind;
     for (Request* r : reqs) by_kind[r->kind].push_back(r);
     sbpx_ctx* ctx = b.ctx;
-    for (auto& kv : by_kind) {
-        std::vector<Request*>& rs = kv.second;
+    // steps that change states first, then the log posteriors of all graphs, then the reads
+    for (int kind : {Request::BUILD, Request::MERGE, Request::SWEEP, Request::LOGP, Request::DOWNLOAD}) {
+        if (kind == Request::LOGP) {
+            const bool changed = by_kind.count(Request::BUILD) || by_kind.count(Request::MERGE) ||
+                                 by_kind.count(Request::SWEEP);
+            if (changed || by_kind.count(Request::LOGP)) {
+                b.lp.resize(b.vertex_offsets.size() - 1);
+                const bool ok = sbpx_batch_log_posterior(ctx, b.lp.data()) == SBPX_OK;
+                b.lp_valid.assign(b.lp.size(), ok ? 1 : 0);
+                if (!ok && by_kind.count(Request::LOGP)) fail(ctx, by_kind[Request::LOGP], "sbpx_batch_log_posterior");
+            }
+        }
+        if (!by_kind.count(kind)) continue;
+        std::vector<Request*>& rs = by_kind[kind];
         const int n = (int)rs.size();
         std::vector<int32_t> graphs(n);
         for (int i = 0; i < n; ++i) graphs[i] = rs[i]->slot;
-        switch (kv.first) {
+        switch (kind) {
             case Request::BUILD: {
                 std::vector<int32_t> blocks(n), concat;
                 for (int i = 0; i < n; ++i) {
@@ -87,24 +103,24 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
                 break;
             }
             case Request::DOWNLOAD: {
-                b.assign.resize(b.vertex_offsets.back());
-                if (sbpx_batch_download(ctx, b.assign.data()) != SBPX_OK) {
+                size_t total = 0;
+                for (Request* r : rs) total += b.vertex_offsets[r->slot + 1] - b.vertex_offsets[r->slot];
+                b.assign.resize(total);
+                if (sbpx_batch_download(ctx, n, graphs.data(), b.assign.data()) != SBPX_OK) {
                     fail(ctx, rs, "sbpx_batch_download");
                     break;
                 }
+                size_t at = 0;
                 for (Request* r : rs) {
-                    const int lo = b.vertex_offsets[r->slot], hi = b.vertex_offsets[r->slot + 1];
-                    std::copy(b.assign.begin() + lo, b.assign.begin() + hi, r->out_assignment);
+                    const size_t len = b.vertex_offsets[r->slot + 1] - b.vertex_offsets[r->slot];
+                    std::copy(b.assign.begin() + at, b.assign.begin() + at + len, r->out_assignment);
+                    at += len;
                 }
                 break;
             }
             case Request::LOGP: {
-                b.lp.resize(b.vertex_offsets.size() - 1);
-                if (sbpx_batch_log_posterior(ctx, b.lp.data()) != SBPX_OK) {
-                    fail(ctx, rs, "sbpx_batch_log_posterior");
-                    break;
-                }
-                for (Request* r : rs) r->value = b.lp[r->slot];
+                for (Request* r : rs)
+                    if (r->error.empty()) r->value = b.lp[r->slot];
                 break;
             }
             case Request::SWEEP: {
@@ -194,6 +210,13 @@ void run_round(Batch& b, std::unique_lock<std::mutex>& lk) {
 void submit(Batch& b, Request& r) {
     {
         std::unique_lock<std::mutex> lk(b.mu);
+        if (r.kind == Request::LOGP && r.slot < (int)b.lp_valid.size() && b.lp_valid[r.slot]) {
+            r.value = b.lp[r.slot];
+            return;
+        }
+        if ((r.kind == Request::BUILD || r.kind == Request::MERGE || r.kind == Request::SWEEP) &&
+            r.slot < (int)b.lp_valid.size())
+            b.lp_valid[r.slot] = 0;
         b.pending.push_back(&r);
         if ((int)b.pending.size() == b.live) run_round(b, lk);
         else b.cv.wait(lk, [&r] { return r.done; });
@@ -260,7 +283,7 @@ namespace sbp {
 BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::vector<int32_t>& out_ptr,
                          const std::vector<int32_t>& out_idx, const std::vector<int32_t>& in_ptr,
                          const std::vector<int32_t>& in_idx, const std::vector<long>& num_edges,
-                         const std::vector<Args>& graph_args, int device, int eval_variant) {
+                         const std::vector<Args>& graph_args, int device, int eval_variant, bool count_bytes) {
     using sbpx_host::Batch;
     using sbpx_host::Device;
     const int G = (int)vertex_offsets.size() - 1;
@@ -291,6 +314,7 @@ BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::
           "sbpx_graph_upload");
     check(sbpx_batch_define(batch.ctx, G, vertex_offsets.data()), "sbpx_batch_define");
     check(sbpx_set_option(batch.ctx, 3, eval_variant), "sbpx_set_option");
+    check(sbpx_set_option(batch.ctx, 1, count_bytes ? 1 : 0), "sbpx_set_option");
 
     BatchRun run;
     run.graphs.resize(G);

@@ -166,11 +166,14 @@ int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int app
 
 // sbp::partition_batch: G graphs given as one union (vertex_offsets[G + 1]), each with its seed
 // and edge count (num_edges may be null: stored edges), async_gibbs with approximate merges and
-// `batches` batches; eval_variant as option 3 of sbpx_set_option. Writes graph-local assignments (N entries), blocks and MDL per graph, and
-// totals[4] = {wall seconds, rounds, batched sweeps, vertex-move evaluations}.
+// `batches` batches; eval_variant and count_bytes as options 3 and 1 of sbpx_set_option. Writes
+// graph-local assignments (N entries), blocks and MDL per graph, and totals[16] = {wall seconds,
+// rounds, batched sweeps, vertex-move evaluations, eval_ms[4], eval_bytes[4], eval_launches[4]}
+// (per kernel class, as in sbpx_sweep_stats).
 int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t* out_ptr, const int32_t* out_idx,
                         const int32_t* in_ptr, const int32_t* in_idx, const int64_t* num_edges, int32_t batches,
-                        const uint64_t* seeds, int device, int eval_variant, int32_t* assignment, int32_t* num_blocks,
+                        const uint64_t* seeds, int device, int eval_variant, int count_bytes, int32_t* assignment,
+                        int32_t* num_blocks,
                         double* mdl, int32_t* mcmc_iterations, double* totals, char* err, int errlen) {
     try {
         const int N = vertex_offsets[G];
@@ -188,7 +191,7 @@ int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t*
         sbp::BatchRun run = sbp::partition_batch(
             std::vector<int32_t>(vertex_offsets, vertex_offsets + G + 1), std::vector<int32_t>(out_ptr, out_ptr + N + 1),
             std::vector<int32_t>(out_idx, out_idx + E), std::vector<int32_t>(in_ptr, in_ptr + N + 1),
-            std::vector<int32_t>(in_idx, in_idx + E), ne, args, device, eval_variant);
+            std::vector<int32_t>(in_idx, in_idx + E), ne, args, device, eval_variant, count_bytes != 0);
         for (int g = 0; g < G; ++g) {
             const sbp::BatchRun::Graph& r = run.graphs[g];
             if (assignment) std::copy(r.assignment.begin(), r.assignment.end(), assignment + vertex_offsets[g]);
@@ -201,6 +204,11 @@ int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t*
             totals[1] = (double)run.rounds;
             totals[2] = (double)run.sweeps;
             totals[3] = (double)run.totals.evals;
+            for (int k = 0; k < 4; ++k) {
+                totals[4 + k] = run.totals.eval_ms[k];
+                totals[8 + k] = run.totals.eval_bytes[k];
+                totals[12 + k] = (double)run.totals.eval_launches[k];
+            }
         }
         return 0;
     } catch (const std::exception& e) {

@@ -178,7 +178,8 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& args);
 /// as one union: graph g is the vertex range [vertex_offsets[g], vertex_offsets[g + 1]) and no edge
 /// joins two graphs. Each runs stochastic_block_partition with its own Args (async_gibbs with
 /// approximate merges) and ends as it would alone; num_edges[g] < 0 means its stored edges.
-/// eval_variant: the evaluation kernel variant (option 3 of sbpx_set_option; 0 = tuned).
+/// eval_variant: the evaluation kernel variant (option 3 of sbpx_set_option; 0 = tuned);
+/// count_bytes: count the evaluation kernels' algorithmic bytes (option 1).
 struct BatchRun {
     struct Graph {
         std::vector<int> assignment;  // graph-local block ids
@@ -196,7 +197,8 @@ struct BatchRun {
 BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::vector<int32_t>& out_ptr,
                          const std::vector<int32_t>& out_idx, const std::vector<int32_t>& in_ptr,
                          const std::vector<int32_t>& in_idx, const std::vector<long>& num_edges,
-                         const std::vector<Args>& graph_args, int device = 0, int eval_variant = 0);
+                         const std::vector<Args>& graph_args, int device = 0, int eval_variant = 0,
+                         bool count_bytes = false);
 bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks = 0);
 }  // namespace sbp
 

@@ -179,8 +179,8 @@ sbpx_status sbpx_batch_info(sbpx_ctx* ctx, int32_t* num_graphs, int32_t* num_blo
 sbpx_status sbpx_batch_build(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, const int32_t* num_blocks,
                              const int32_t* assignment);
 
-/* Every vertex's graph-local block (N entries). */
-sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t* assignment);
+/* The graph-local vertex assignments of the n listed graphs, one after the other. */
+sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, int32_t* assignment);
 
 /* Log posterior of every graph (out[num_graphs]; Blockmodel::log_posterior_p, blockmodel.cpp:267-285). */
 sbpx_status sbpx_batch_log_posterior(sbpx_ctx* ctx, double* out);

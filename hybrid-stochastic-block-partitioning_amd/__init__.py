@@ -146,6 +146,7 @@ def load_host():
     H.sbp_graph_free.argtypes = [C.c_void_p]
     H.sbp_graph_free.restype = None
     H.sbp_last_trajectory.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    H.sbp_last_intermediates.argtypes = [C.c_void_p, C.c_int]
     H.sbp_partition_checkpointed.argtypes = [C.c_void_p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int, C.c_char_p,
                                              C.c_int, C.c_int32, i32p, C.POINTER(RunStats), C.c_char_p, C.c_int]
     H.sbp_graph_option.argtypes = [C.c_void_p, C.c_int, C.c_int64, C.c_char_p, C.c_int]
@@ -160,6 +161,9 @@ def load_host():
     return H
 
 
+INTERMEDIATE_FIELDS = ("iteration", "mdl", "normalized_mdl_v1", "normalized_mdl_v2", "modularity", "interblock_edges",
+                       "block_size_variation", "num_blocks", "mcmc_iterations", "mcmc_time", "seconds")
+
 EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option", "sbpx_graph_upload",
             "sbpx_graph_split", "sbpx_blockmodel_build", "sbpx_blockmodel_info", "sbpx_blockmodel_download",
             "sbpx_blockmodel_coo", "sbpx_mdl", "sbpx_log_posterior", "sbpx_eval_moves", "sbpx_propose",
@@ -168,7 +172,7 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_learned_params", "sbpx_learned_log", "sbpx_learned_log_count", "sbpx_learned_train",
             "sbpx_learned_forward", "sbpx_learned_gradients", "sbpx_batch_define", "sbpx_batch_info",
             "sbpx_batch_build", "sbpx_batch_download", "sbpx_batch_log_posterior", "sbpx_batch_sweep",
-            "sbpx_batch_merge"]
+            "sbpx_batch_merge", "sbpx_blockmodel_trace", "sbpx_batch_trace"]
 
 
 class SbpxError(RuntimeError):
@@ -537,6 +541,15 @@ class ResidentGraph:
         t, mdl, B = np.zeros(n), np.zeros(n), np.zeros(n, np.int32)
         self.H.sbp_last_trajectory(t.ctypes.data, mdl.ctypes.data, B.ctypes.data, n)
         return t, mdl, B
+
+    def intermediates(self):
+        """sbp::get_intermediates of the last run: one dict per outer iteration (iteration, mdl,
+        normalized_mdl_v1, normalized_mdl_v2, modularity, interblock_edges, block_size_variation,
+        num_blocks, mcmc_iterations, mcmc_time, seconds)."""
+        n = self.H.sbp_last_intermediates(None, 0)
+        rows = np.zeros((n, 11))
+        self.H.sbp_last_intermediates(rows.ctypes.data, n)
+        return [dict(zip(INTERMEDIATE_FIELDS, r)) for r in rows]
 
     def close(self):
         if getattr(self, "h", None):

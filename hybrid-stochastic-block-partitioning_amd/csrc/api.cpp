@@ -716,6 +716,18 @@ sbpx_status sbpx_blockmodel_coo(sbpx_ctx* ctx, int by_column, int32_t* rows, int
     });
 }
 
+sbpx_status sbpx_blockmodel_trace(sbpx_ctx* ctx, int64_t* trace, int64_t* edges) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        std::vector<int> diag(c.bm.B);
+        SBPX_HIP(hipMemcpyAsync(diag.data(), c.bm.diag, sizeof(int) * c.bm.B, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        if (trace) *trace = std::accumulate(diag.begin(), diag.end(), (int64_t)0);
+        if (edges) *edges = c.g.E;
+    });
+}
+
 sbpx_status sbpx_log_posterior(sbpx_ctx* ctx, double* out) {
     return guarded(ctx, [&] {
         require_bm(ctx->c);
@@ -1475,6 +1487,20 @@ sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t n, const int32_t* graphs,
         c.sync();
         for (int i = 0; i < n; ++i)
             for (size_t j = at[i]; j < at[i + 1]; ++j) assignment[j] -= b.bo[graphs[i]];
+    });
+}
+
+sbpx_status sbpx_batch_trace(sbpx_ctx* ctx, int64_t* trace) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (!trace) throw ArgError("null output");
+        const DevBatch& b = c.batch;
+        std::vector<int> diag(b.range);
+        SBPX_HIP(hipMemcpyAsync(diag.data(), c.bm.diag, sizeof(int) * b.range, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        for (int g = 0; g < b.G; ++g)
+            trace[g] = std::accumulate(diag.begin() + b.bo[g], diag.begin() + b.bo[g] + b.nb[g], (int64_t)0);
     });
 }
 

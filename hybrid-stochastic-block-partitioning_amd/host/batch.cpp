@@ -24,7 +24,7 @@ namespace sbpx_host {
 namespace {
 
 struct Request {
-    enum Kind { BUILD, DOWNLOAD, LOGP, SWEEP, MERGE } kind;
+    enum Kind { BUILD, DOWNLOAD, LOGP, SWEEP, MERGE, TRACE } kind;
     int slot = -1;
     int num_blocks = 0;
     const int32_t* assignment = nullptr;
@@ -74,7 +74,8 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
     for (Request* r : reqs) by_kind[r->kind].push_back(r);
     sbpx_ctx* ctx = b.ctx;
     // steps that change states first, then the log posteriors of all graphs, then the reads
-    for (int kind : {Request::BUILD, Request::MERGE, Request::SWEEP, Request::LOGP, Request::DOWNLOAD}) {
+    for (int kind : {Request::BUILD, Request::MERGE, Request::SWEEP, Request::LOGP, Request::DOWNLOAD,
+                     Request::TRACE}) {
         if (kind == Request::LOGP) {
             const bool changed = by_kind.count(Request::BUILD) || by_kind.count(Request::MERGE) ||
                                  by_kind.count(Request::SWEEP);
@@ -121,6 +122,15 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
             case Request::LOGP: {
                 for (Request* r : rs)
                     if (r->error.empty()) r->value = b.lp[r->slot];
+                break;
+            }
+            case Request::TRACE: {
+                std::vector<int64_t> tr(b.vertex_offsets.size() - 1);
+                if (sbpx_batch_trace(ctx, tr.data()) != SBPX_OK) {
+                    fail(ctx, rs, "sbpx_batch_trace");
+                    break;
+                }
+                for (Request* r : rs) r->value = (double)tr[r->slot];
                 break;
             }
             case Request::SWEEP: {
@@ -276,6 +286,13 @@ int batch_merge(Batch& b, int slot, int num_to_merge, int mode, uint64_t seed, u
     return r.blocks_left;
 }
 
+long batch_trace(Batch& b, int slot) {
+    Request r{Request::TRACE};
+    r.slot = slot;
+    submit(b, r);
+    return (long)r.value;
+}
+
 }  // namespace sbpx_host
 
 namespace sbp {
@@ -333,6 +350,7 @@ BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::
                 long E = num_edges[g] >= 0 ? num_edges[g]
                                            : (long)(out_ptr[vertex_offsets[g + 1]] - out_ptr[vertex_offsets[g]]);
                 dev->E = (int)E;
+                dev->stored_edges = (long)(out_ptr[vertex_offsets[g + 1]] - out_ptr[vertex_offsets[g]]);
                 Graph graph(dev->N, E, dev);
                 Args a = graph_args[g];
                 Blockmodel bm = stochastic_block_partition(graph, a);

@@ -86,6 +86,21 @@ int sbp_last_trajectory(double* seconds, double* mdl, int32_t* num_blocks, int c
     return (int)r.size();
 }
 
+// The last run's intermediates on this thread, 11 doubles per outer iteration: iteration, mdl,
+// normalized_mdl_v1, normalized_mdl_v2, modularity (-1), interblock_edges, block_size_variation,
+// num_blocks, mcmc_iterations, mcmc_time, seconds. Returns how many there are; fills at most cap.
+int sbp_last_intermediates(double* rows, int cap) {
+    const std::vector<sbp::Intermediate> r = sbp::get_intermediates();
+    for (int i = 0; i < (int)r.size() && i < cap; ++i) {
+        const sbp::Intermediate& x = r[i];
+        const double row[11] = {x.iteration, x.mdl, x.normalized_mdl_v1, x.normalized_mdl_v2, x.modularity,
+                                x.interblock_edges, x.block_size_variation, (double)x.num_blocks,
+                                (double)x.mcmc_iterations, x.mcmc_time, x.seconds};
+        std::copy(row, row + 11, rows + 11 * i);
+    }
+    return (int)r.size();
+}
+
 // A device option of the graph's context (sbpx_set_option), e.g. 4 = proposal (0 plain, 1 learned).
 int sbp_graph_option(void* gp, int option, int64_t value, char* err, int errlen) {
     sbpx_ctx* ctx = ((Graph*)gp)->device()->ctx;

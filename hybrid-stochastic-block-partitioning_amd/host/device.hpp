@@ -19,6 +19,7 @@ struct Device {
     uint32_t sweep_call = 0;  // call id of the next MCMC sweep (randomness stream)
     uint32_t merge_call = 0;  // call id of the next merge phase
     int N = 0, E = 0;
+    long stored_edges = 0;  // edges held (E may be the count a loaded file reports)
 
     // The device steps of the driver; each throws std::runtime_error with the library's message.
     void build(int num_blocks, const int32_t* assignment);
@@ -26,6 +27,7 @@ struct Device {
     double log_posterior();
     void sweep(int algo, int batches, uint64_t seed, uint32_t call, sbpx_sweep_stats* stats);
     int merge(int num_to_merge, int mode, uint64_t seed, uint32_t call);  // returns the number of blocks left
+    long trace();  // sum of M[b, b]
     sbpx_ctx* own_ctx(const char* what) const;  // this graph's context; throws for a batch member
 
     ~Device() {

@@ -27,6 +27,7 @@ void batch_download(Batch& b, int slot, int32_t* assignment);
 double batch_log_posterior(Batch& b, int slot);
 void batch_sweep(Batch& b, int slot, int algo, int batches, uint64_t seed, uint32_t call, sbpx_sweep_stats* stats);
 int batch_merge(Batch& b, int slot, int num_to_merge, int mode, uint64_t seed, uint32_t call);
+long batch_trace(Batch& b, int slot);
 
 sbpx_ctx* Device::own_ctx(const char* what) const {
     if (!ctx) throw std::runtime_error(std::string(what) + ": not available for a graph of a batch");
@@ -55,6 +56,13 @@ void Device::sweep(int algo, int batches, uint64_t seed, uint32_t call, sbpx_swe
     check(ctx, sbpx_sweep(ctx, algo, batches, seed, call, stats), "sbpx_sweep");
 }
 
+long Device::trace() {
+    if (batch) return batch_trace(*batch, slot);
+    int64_t t = 0;
+    check(ctx, sbpx_blockmodel_trace(ctx, &t, nullptr), "sbpx_blockmodel_trace");
+    return (long)t;
+}
+
 int Device::merge(int num_to_merge, int mode, uint64_t seed, uint32_t call) {
     if (batch) return batch_merge(*batch, slot, num_to_merge, mode, seed, call);
     check(ctx, sbpx_merge_blocks(ctx, num_to_merge, mode, seed, call), "sbpx_merge_blocks");
@@ -77,6 +85,7 @@ Graph::Graph(int num_vertices, std::vector<int32_t> out_ptr, std::vector<int32_t
     check(nullptr, sbpx_create(&dev_->ctx, device), "sbpx_create");
     dev_->N = num_vertices_;
     dev_->E = num_edges_;
+    dev_->stored_edges = (long)out_idx.size();
     check(dev_->ctx,
           sbpx_graph_upload(dev_->ctx, num_vertices, (int64_t)out_idx.size(), out_ptr.data(), out_idx.data(),
                             in_ptr.data(), in_idx.data()),
@@ -159,6 +168,33 @@ std::vector<int> Blockmodel::degrees_out() const {
     sbpx_ctx* c = dev_->own_ctx("Blockmodel::degrees_out");
     check(c, sbpx_blockmodel_download(c, nullptr, d.data(), nullptr, nullptr), "download");
     return d;
+}
+
+// Blockmodel::block_size_variation (blockmodel.cpp:9-27): variance of the block sizes over
+// (max - min)(max - mean) / 4, with the reference's initial min/max and float rounding.
+double Blockmodel::block_size_variation() const {
+    std::vector<int> block_sizes(num_blocks, 0);
+    for (int block : block_assignment()) block_sizes[block]++;
+    double total = 0;
+    for (int x : block_sizes) total += x;
+    double mean = total / double(num_blocks);
+    double min = std::numeric_limits<double>::max(), max = std::numeric_limits<double>::min(), variance = 0;
+    for (int block_size : block_sizes) {
+        if (block_size < min) min = block_size;
+        if (block_size > max) max = block_size;
+        variance += double(block_size - mean) * double(block_size - mean);
+    }
+    variance /= double(num_blocks);
+    double max_variance = (double(max - min) * double(max - mean)) / 4.0;
+    return float(variance / max_variance);
+}
+
+// Blockmodel::interblock_edges (blockmodel.cpp:257-261): the share of edges between blocks.
+double Blockmodel::interblock_edges() const {
+    make_live();
+    double num_edges = (double)dev_->stored_edges;
+    double interblock = num_edges - double(dev_->trace());
+    return interblock / num_edges;
 }
 
 // ---------------------------------------------------------------------------- triplet
@@ -399,10 +435,24 @@ static thread_local std::vector<Intermediate> intermediate_results;
 static thread_local std::chrono::steady_clock::time_point partition_start;
 std::vector<Intermediate> get_intermediates() { return intermediate_results; }
 
-static void add_intermediate(float iteration, const Blockmodel& blockmodel, double mdl) {
+// sbp::add_intermediate (sbp.cpp:33-59). Modularity stays -1 here: the reference computes it only
+// for the final row, with an O(N^2) loop over the graph, which the caller reproduces in closed
+// form from the final assignment.
+static void add_intermediate(float iteration, const Graph& graph, const Blockmodel& blockmodel, double mdl) {
     const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - partition_start).count();
-    intermediate_results.push_back(Intermediate{iteration, mdl, blockmodel.getNum_blocks(),
-                                                finetune::MCMC_iterations, finetune::MCMC_time, t});
+    Intermediate r{};
+    r.iteration = iteration;
+    r.mdl = mdl;
+    r.normalized_mdl_v1 = entropy::normalize_mdl_v1(mdl, graph.num_edges());
+    r.normalized_mdl_v2 = entropy::normalize_mdl_v2(mdl, graph.num_vertices(), graph.num_edges());
+    r.modularity = -1;
+    r.interblock_edges = blockmodel.interblock_edges();
+    r.block_size_variation = blockmodel.block_size_variation();
+    r.num_blocks = blockmodel.getNum_blocks();
+    r.mcmc_iterations = finetune::MCMC_iterations;
+    r.mcmc_time = finetune::MCMC_time;
+    r.seconds = t;
+    intermediate_results.push_back(r);
 }
 
 bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks) {
@@ -447,7 +497,7 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
     } else {
         blockmodel = Blockmodel(graph.num_vertices(), graph, float(BLOCK_REDUCTION_RATE));
         double initial_mdl = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
-        add_intermediate(0, blockmodel, initial_mdl);
+        add_intermediate(0, graph, blockmodel, initial_mdl);
     }
     int iterations_this_call = 0;
     while (!done_blockmodeling(blockmodel, blockmodel_triplet)) {
@@ -462,7 +512,7 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
             blockmodel = finetune::metropolis_hastings(blockmodel, graph, blockmodel_triplet);
         finetune::MCMC_time +=
             std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-        add_intermediate(++iteration, blockmodel, blockmodel.getOverall_entropy());
+        add_intermediate(++iteration, graph, blockmodel, blockmodel.getOverall_entropy());
         if (std::getenv("SBP_TRACE"))
             std::fprintf(stderr, "sbp iter %.0f: B=%d mdl=%.3f mcmc_iterations=%d\n", iteration,
                          blockmodel.getNum_blocks(), (double)blockmodel.getOverall_entropy(), finetune::MCMC_iterations);
@@ -490,7 +540,7 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
                 throw std::runtime_error("cannot write " + args.checkpoint);
         }
     }
-    add_intermediate(-1, blockmodel, blockmodel.getOverall_entropy());
+    add_intermediate(-1, graph, blockmodel, blockmodel.getOverall_entropy());
     return blockmodel;
 }
 }  // namespace sbp

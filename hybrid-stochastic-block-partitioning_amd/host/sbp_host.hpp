@@ -92,6 +92,8 @@ public:
     float& getOverall_entropy() { return overall_entropy; }
     void setOverall_entropy(double e) { overall_entropy = (float)e; }
     float& getBlock_reduction_rate() { return block_reduction_rate; }
+    double block_size_variation() const;
+    double interblock_edges() const;
     bool empty;
 
     // device binding
@@ -163,9 +165,15 @@ Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edg
 }  // namespace block_merge
 
 namespace sbp {
+/// sbp::Intermediate (sbp.hpp:18-28) plus the number of blocks and the wall time.
 struct Intermediate {
     float iteration;
     double mdl;
+    double normalized_mdl_v1;
+    double normalized_mdl_v2;
+    double modularity;  // -1 (computed by the caller for the final row)
+    double interblock_edges;
+    double block_size_variation;
     int num_blocks;
     int mcmc_iterations;
     double mcmc_time;

@@ -91,6 +91,11 @@ sbpx_status sbpx_blockmodel_download(sbpx_ctx* ctx, int32_t* assignment, int32_t
 sbpx_status sbpx_blockmodel_coo(sbpx_ctx* ctx, int by_column, int32_t* rows, int32_t* cols,
                                 int32_t* vals);
 
+/* Trace of the block matrix (sum of M[b, b]: edges inside a block) and the number of stored
+ * edges (sum of the in-degrees), the two terms of Blockmodel::interblock_edges
+ * (blockmodel.cpp:257-261). */
+sbpx_status sbpx_blockmodel_trace(sbpx_ctx* ctx, int64_t* trace, int64_t* edges);
+
 /* entropy::mdl (entropy.cpp:584-589) and Blockmodel::log_posterior_probability
  * (blockmodel.cpp:267-285). */
 sbpx_status sbpx_mdl(sbpx_ctx* ctx, double* mdl);
@@ -181,6 +186,9 @@ sbpx_status sbpx_batch_build(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, co
 
 /* The graph-local vertex assignments of the n listed graphs, one after the other. */
 sbpx_status sbpx_batch_download(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, int32_t* assignment);
+
+/* Trace of every graph's block matrix (trace[num_graphs]), as sbpx_blockmodel_trace. */
+sbpx_status sbpx_batch_trace(sbpx_ctx* ctx, int64_t* trace);
 
 /* Log posterior of every graph (out[num_graphs]; Blockmodel::log_posterior_p, blockmodel.cpp:267-285). */
 sbpx_status sbpx_batch_log_posterior(sbpx_ctx* ctx, double* out);

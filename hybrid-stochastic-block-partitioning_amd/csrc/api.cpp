@@ -76,6 +76,8 @@ void free_graph_buffers(Ctx& c) {
     dfree(&b.col_row);
     dfree(&b.col_val);
     dfree(&b.col_cum);
+    dfree(&b.row_h);
+    dfree(&b.col_h);
     dfree(&b.dout);
     dfree(&b.din);
     dfree(&b.deg);
@@ -485,7 +487,7 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
             if (value != 0 && value != 1) throw ArgError("proposal must be 0 (plain) or 1 (learned mixture)");
             ctx->c.proposal = (int)value;
         } else if (option == 3) {
-            if (value < 0 || value > 2) throw ArgError("evaluation kernel variant must be 0 (tuned), 1 or 2");
+            if (value < 0 || value > 3) throw ArgError("evaluation kernel variant must be 0 (tuned), 1, 2 or 3");
             ctx->c.eval_variant = (int)value;
         } else if (option == 2) {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
@@ -577,6 +579,8 @@ sbpx_status sbpx_graph_upload(sbpx_ctx* ctx, int32_t N, int64_t E, const int32_t
         dalloc(&b.col_row, EE);
         dalloc(&b.col_val, EE);
         dalloc(&b.col_cum, EE);
+        dalloc(&b.row_h, EE);
+        dalloc(&b.col_h, EE);
         dalloc(&b.dout, NB);
         dalloc(&b.din, NB);
         dalloc(&b.deg, NB);
@@ -803,6 +807,7 @@ static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const i
             L.list = d_list + off;
             L.n = counts[b];
             L.bin = b;
+            L.variant = c.eval_variant ? c.pick_variant(b) : 0;  // a fixed variant applies here too
             L.given_s = s ? d_s + off : nullptr;
             L.given_u = s ? d_u + off : nullptr;
             L.seed = seed;

@@ -119,6 +119,19 @@ __global__ void decode_csc_kernel(const K* __restrict__ keys, const int* __restr
     col_val[j] = vals[j];
 }
 
+// Current entropy of every nonzero, the "before" term of each cell an evaluation reads: CSR
+// entry j is cell (coo_row[j], row_col[j]); CSC entry j is cell (col_row[j], col_of[j]).
+__global__ void cell_entropy_kernel(const int* __restrict__ coo_row, const int* __restrict__ row_col,
+                                    const int* __restrict__ row_val, const int* __restrict__ col_of,
+                                    const int* __restrict__ col_row, const int* __restrict__ col_val,
+                                    const int* __restrict__ dout, const int* __restrict__ din, int nnz,
+                                    float* __restrict__ row_h, float* __restrict__ col_h) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nnz) return;
+    row_h[j] = cell_entropy((float)row_val[j], (float)din[row_col[j]], (float)dout[coo_row[j]]);
+    col_h[j] = cell_entropy((float)col_val[j], (float)din[col_of[j]], (float)dout[col_row[j]]);
+}
+
 __global__ void degrees_kernel(const int* __restrict__ dout, const int* __restrict__ din,
                                const int* __restrict__ diag, const int* __restrict__ drift, int B,
                                int* __restrict__ deg) {
@@ -192,6 +205,8 @@ static void build_impl(Ctx& c) {
         scan_exclusive(bm.col_val, c.pos, nnz, c.scan_tmp, st);
         segment_cum_kernel<<<blocks_for(nnz), TB, 0, st>>>(col_of, bm.col_ptr, c.pos, nnz, bm.col_cum);
         segment_total_kernel<<<blocks_for(B), TB, 0, st>>>(bm.col_ptr, c.pos, B, bm.din);
+        cell_entropy_kernel<<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, bm.row_val, col_of, bm.col_row,
+                                                            bm.col_val, bm.dout, bm.din, nnz, bm.row_h, bm.col_h);
     } else {
         SBPX_HIP(hipMemsetAsync(bm.col_ptr, 0, sizeof(int) * (B + 1), st));
         SBPX_HIP(hipMemsetAsync(bm.din, 0, sizeof(int) * B, st));

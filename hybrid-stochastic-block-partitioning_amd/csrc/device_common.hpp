@@ -160,6 +160,70 @@ __device__ __forceinline__ int wave_upper_bound(const int* __restrict__ cum, int
     return lo + first_lane(m);
 }
 
+// Lane groups: W consecutive lanes (W a power of two dividing 64) evaluate one vertex together.
+// Shuffles stay inside the group (offsets below W) and ballots are read at the group's bits, so
+// groups of one wavefront may follow different paths; every lane of a group follows the same.
+template <int W>
+__device__ __forceinline__ int group_lane() { return threadIdx.x & (W - 1); }
+template <int W>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    return v;
+}
+template <int W>
+__device__ __forceinline__ long long group_sum_ll(long long v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    return v;
+}
+template <int W>
+__device__ __forceinline__ int group_max(int v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, W));
+    return v;
+}
+template <int W>
+__device__ __forceinline__ int group_min(int v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, W));
+    return v;
+}
+template <int W>
+__device__ __forceinline__ int group_or(int v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v |= __shfl_xor(v, off, W);
+    return v;
+}
+// The group's bits of a wavefront ballot.
+template <int W>
+__device__ __forceinline__ unsigned long long group_ballot(bool p) {
+    const unsigned long long m = __ballot(p);
+    if constexpr (W == WAVE) return m;
+    const int base = (threadIdx.x & (WAVE - 1)) & ~(W - 1);
+    return (m >> base) & ((1ull << W) - 1ull);
+}
+
+// wave_upper_bound for a group of W lanes: each round narrows the range W-fold.
+template <int W>
+__device__ __forceinline__ int group_upper_bound(const int* __restrict__ cum, int lo, int hi, int target) {
+    if constexpr (W == WAVE) return wave_upper_bound(cum, lo, hi, target);
+    const int lane = group_lane<W>();
+    while (hi - lo > W) {
+        int len = hi - lo;
+        int step = (len + W - 1) / W;
+        int end = min((lane + 1) * step, len);
+        bool gt = (lane * step < len) && (cum[lo + end - 1] > target);
+        int L = first_lane(group_ballot<W>(gt));
+        int nlo = lo + L * step;
+        hi = min(lo + (L + 1) * step, hi);
+        lo = nlo;
+    }
+    int p = lo + lane;
+    bool gt = (p < hi) && (cum[p] > target);
+    return lo + first_lane(group_ballot<W>(gt));
+}
+
 // Lower bound of key in the sorted array keys[lo, hi) (one lane).
 __device__ __forceinline__ int lower_bound_lane(const int* __restrict__ keys, int lo, int hi, int key) {
     while (lo < hi) {

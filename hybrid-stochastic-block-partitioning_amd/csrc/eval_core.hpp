@@ -35,7 +35,14 @@ struct CsrView {
     const int* __restrict__ col_row;
     const int* __restrict__ col_val;
     const int* __restrict__ col_cum;
+    // Current entropy of every nonzero, cell_entropy(M, din[col], dout[row]), computed when the
+    // snapshot is built (CSR and CSC order): the "before" term of every cell an evaluation reads.
+    const float* __restrict__ row_h = nullptr;
+    const float* __restrict__ col_h = nullptr;
     static constexpr bool kSlots = false;  // every entry in range is a nonzero
+    static constexpr bool kHasH = true;
+    __device__ float rh(int j) const { return row_h[j]; }
+    __device__ float ch(int j) const { return col_h[j]; }
     __device__ int rbeg(int x) const { return row_ptr[x]; }
     __device__ int rend(int x) const { return row_ptr[x + 1]; }
     __device__ int rkey(int j) const { return row_col[j]; }
@@ -49,10 +56,12 @@ struct CsrView {
     // M[x, c] and M[c, x] (one lane)
     __device__ int rlookup(int x, int c) const { return lookup_sorted(row_col, row_val, row_ptr[x], row_ptr[x + 1], c); }
     __device__ int clookup(int x, int c) const { return lookup_sorted(col_row, col_val, col_ptr[x], col_ptr[x + 1], c); }
-    // Block at unit `target` of row t (target < dout[t]) or of column t (target - dout[t]).
+    // Block at unit `target` of row t (target < dout[t]) or of column t (target - dout[t]), found
+    // by a group of W lanes.
+    template <int W = WAVE>
     __device__ int sample(int t, int target, int dout_t) const {
-        if (target < dout_t) return row_col[wave_upper_bound(row_cum, row_ptr[t], row_ptr[t + 1], target)];
-        return col_row[wave_upper_bound(col_cum, col_ptr[t], col_ptr[t + 1], target - dout_t)];
+        if (target < dout_t) return row_col[group_upper_bound<W>(row_cum, row_ptr[t], row_ptr[t + 1], target)];
+        return col_row[group_upper_bound<W>(col_cum, col_ptr[t], col_ptr[t + 1], target - dout_t)];
     }
 };
 
@@ -167,6 +176,9 @@ struct HashView {
     int B;
     int* hist = nullptr;  // 256 LDS counters of this wavefront for the radix selection, or null
     static constexpr bool kSlots = true;  // skip key < 0 and value 0
+    static constexpr bool kHasH = false;  // cells change as moves commit: entropies are computed
+    __device__ float rh(int) const { return 0.0f; }
+    __device__ float ch(int) const { return 0.0f; }
     __device__ int rbeg(int x) const { return rbase[x]; }
     __device__ int rend(int x) const { return rbase[x] + rcap[x]; }
     __device__ int rkey(int j) const { return rkeys[j]; }
@@ -190,7 +202,9 @@ struct HashView {
     }
     __device__ int rlookup(int x, int c) const { return probe(rbase, rcap, rkeys, rvals, x, c); }
     __device__ int clookup(int x, int c) const { return probe(cbase, ccap, ckeys, cvals, x, c); }
+    template <int W = WAVE>
     __device__ int sample(int t, int target, int dout_t) const {
+        static_assert(W == WAVE, "hash rows are sampled by whole wavefronts");
         const bool row = target < dout_t;
         const int* keys = row ? rkeys : ckeys;
         const int* vals = row ? rvals : cvals;
@@ -286,12 +300,13 @@ __device__ __forceinline__ float fi(int x) { return (float)x; }
 // U: slots per lane per trip of the row scans and the Hastings pass (loads in flight per lane).
 // lw >= 0: the proposal was the learned mixture w * plain + (1 - w) * net with w = lw; lqf / lqb
 // are the net's forward and reverse probabilities (negative: no learned mass in that state).
-template <class View, int U = 2>
+// W: lanes per vertex (a group of W consecutive lanes; 64 = the whole wavefront).
+template <class View, int U = 2, int W = WAVE>
 __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockState& S, const View& M, int v,
                                                 int* tab, int cap, uint64_t seed, uint32_t call, uint32_t tag,
                                                 int given_s, double given_u, bool propose_only, double lw = -1.0,
                                                 double lqf = 0.0, double lqb = 0.0) {
-    const int lane = lane_id();
+    const int lane = group_lane<W>();
     MoveResult R{};
     R.v = v;
     const int r = S.assign[v];
@@ -306,7 +321,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     int* tmbs = tab + 4 * cap;
     int* tmrb = tab + 5 * cap;
     int* tmbr = tab + 6 * cap;
-    for (int j = lane; j < cap; j += WAVE) {
+    for (int j = lane; j < cap; j += W) {
         tkey[j] = -1;
         to[j] = 0;
         ti[j] = 0;
@@ -347,7 +362,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
 
     // 1. neighbour blocks over the concatenated out and in lists
     int self_pos = INT_MAX, full = 0, cand0 = -1, cand1 = -1;
-    for (int j = lane; j < dO + dI; j += WAVE) {
+    for (int j = lane; j < dO + dI; j += W) {
         const bool is_out = j < dO;
         const int w = is_out ? G.out_idx[ob + j] : G.in_idx[ib + (j - dO)];
         const int bw = S.assign[w];
@@ -361,13 +376,13 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
             else atomicAdd(is_out ? &to[slot] : &ti[slot], 1);
         }
     }
-    self_pos = wave_min(self_pos);
-    cand0 = wave_max(cand0);
-    cand1 = wave_max(cand1);
+    self_pos = group_min<W>(self_pos);
+    cand0 = group_max<W>(cand0);
+    cand1 = group_max<W>(cand1);
     if (self && lane == 0) {
         if (tab_insert(tkey, cap, r) < 0) full = 1;
     }
-    full = wave_or(full);
+    full = group_or<W>(full);
     fence_wave();
     if (full) {
         R.status = -1;
@@ -387,7 +402,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
         const int trow = S.dout[t];
         const int total = trow + S.din[t];
         const int target = (int)mulhi32(dr.x1, (uint32_t)total);
-        s = M.sample(t, target, trow);
+        s = M.template sample<W>(t, target, trow);
     }
     R.s = s;
     R.u = u;
@@ -418,26 +433,32 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;  // column r
     const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
     long long live = 0;
-    for (int base = 0; base < T; base += U * WAVE) {
+    for (int base = 0; base < T; base += U * W) {
         int key[U], val[U], seg[U];
+        float h[U];  // the cell's current entropy, when the view keeps it
 #pragma unroll
         for (int q = 0; q < U; ++q) {
-            const int i = base + q * WAVE + lane;
+            const int i = base + q * W + lane;
             seg[q] = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : i < T ? 3 : -1;
             key[q] = -1;
             val[q] = 0;
+            h[q] = 0.0f;
             if (seg[q] == 0) {
                 key[q] = M.rkey(b0 + i);
                 val[q] = M.rval(b0 + i);
+                if constexpr (View::kHasH) h[q] = M.rh(b0 + i);
             } else if (seg[q] == 1) {
                 key[q] = M.rkey(b1 + (i - L0));
                 val[q] = M.rval(b1 + (i - L0));
+                if constexpr (View::kHasH) h[q] = M.rh(b1 + (i - L0));
             } else if (seg[q] == 2) {
                 key[q] = M.ckey(b2 + (i - T1));
                 val[q] = M.cval(b2 + (i - T1));
+                if constexpr (View::kHasH) h[q] = M.ch(b2 + (i - T1));
             } else if (seg[q] == 3) {
                 key[q] = M.ckey(b3 + (i - T2));
                 val[q] = M.cval(b3 + (i - T2));
+                if constexpr (View::kHasH) h[q] = M.ch(b3 + (i - T2));
             }
         }
         int dsel[U];
@@ -460,6 +481,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
                     if (sl >= 0) {
                         tmsb[sl] = m;
                         unchanged = to[sl] == 0;
+                        if (View::kHasH && !unchanged) acc += (double)h[q];  // a Delta key's "before" term
                     }
                 }
             } else if (seg[q] == 1) {  // row r: cell (r, c)
@@ -474,6 +496,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
                     if (sl >= 0) {
                         tmrb[sl] = m;
                         unchanged = to[sl] == 0;
+                        if (View::kHasH && !unchanged) acc += (double)h[q];
                     }
                 }
             } else {  // column s or r: cell (c, s) or (c, r), rows r and s left to the rows
@@ -482,6 +505,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
                 if (sl >= 0) {
                     (seg[q] == 2 ? tmbs : tmbr)[sl] = m;
                     unchanged = ti[sl] == 0;
+                    if (View::kHasH && !unchanged) acc += (double)h[q];
                 }
             }
             if (unchanged) dsel[q] = c;
@@ -498,29 +522,29 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
             if (seg[q] < 2) {  // unchanged cell of row x in {s, r}: din[c] may change, dout[x] does
                 const int xo = seg[q] == 0 ? dout_s : dout_r, nxo = seg[q] == 0 ? ndout_s : ndout_r;
                 const int nd = c == r ? ndin_r : (c == s ? ndin_s : d);
-                acc += (double)cell_entropy(m, fi(d), fi(xo));
+                acc += View::kHasH ? (double)h[q] : (double)cell_entropy(m, fi(d), fi(xo));
                 acc -= (double)cell_entropy(m, fi(nd), fi(nxo));
             } else {  // unchanged cell of column x: din[x] changes
                 const int xi = seg[q] == 2 ? din_s : din_r, nxi = seg[q] == 2 ? ndin_s : ndin_r;
-                acc += (double)cell_entropy(m, fi(xi), fi(d));
+                acc += View::kHasH ? (double)h[q] : (double)cell_entropy(m, fi(xi), fi(d));
                 acc -= (double)cell_entropy(m, fi(nxi), fi(d));
             }
         }
     }
-    m_ss = wave_max(m_ss);
-    m_rr = wave_max(m_rr);
-    m_sr = wave_max(m_sr);
-    m_rs = wave_max(m_rs);
+    m_ss = group_max<W>(m_ss);
+    m_rr = group_max<W>(m_rr);
+    m_sr = group_max<W>(m_sr);
+    m_rs = group_max<W>(m_rs);
     fence_wave();
 
     // 4. Delta keys and Hastings sums
     double pf = 0.0, pb = 0.0;
     int nuniq = 0;
-    for (int j0 = 0; j0 < cap; j0 += U * WAVE) {
+    for (int j0 = 0; j0 < cap; j0 += U * W) {
         int bk[U], degb[U], dinb[U], doutb[U];
 #pragma unroll
         for (int q = 0; q < U; ++q) {
-            const int j = j0 + q * WAVE + lane;
+            const int j = j0 + q * W + lane;
             bk[q] = j < cap ? tkey[j] : -1;
         }
 #pragma unroll
@@ -535,7 +559,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
         for (int q = 0; q < U; ++q) {
             const int b = bk[q];
             if (b < 0) continue;
-            const int j = j0 + q * WAVE + lane;
+            const int j = j0 + q * W + lane;
             nuniq++;
             const int ob_ = to[j], ib_ = ti[j];
             int Msb, Mbs, Mrb, Mbr, Drb, Dbr, nd;
@@ -549,16 +573,17 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
                 Msb = tmsb[j]; Mbs = tmbs[j]; Mrb = tmrb[j]; Mbr = tmbr[j]; Drb = -ob_; Dbr = -ib_;
                 nd = degb[q];
                 const int db = dinb[q], ub = doutb[q];
+                // "before" terms of the Delta keys: added by the scan when the view keeps them
                 if (ob_ > 0) {
-                    acc += (double)cell_entropy(fi(Msb), fi(db), fi(dout_s));
+                    if constexpr (!View::kHasH) acc += (double)cell_entropy(fi(Msb), fi(db), fi(dout_s));
                     acc -= (double)cell_entropy(fi(Msb + ob_), fi(db), fi(ndout_s));
-                    acc += (double)cell_entropy(fi(Mrb), fi(db), fi(dout_r));
+                    if constexpr (!View::kHasH) acc += (double)cell_entropy(fi(Mrb), fi(db), fi(dout_r));
                     acc -= (double)cell_entropy(fi(Mrb - ob_), fi(db), fi(ndout_r));
                 }
                 if (ib_ > 0) {
-                    acc += (double)cell_entropy(fi(Mbs), fi(din_s), fi(ub));
+                    if constexpr (!View::kHasH) acc += (double)cell_entropy(fi(Mbs), fi(din_s), fi(ub));
                     acc -= (double)cell_entropy(fi(Mbs + ib_), fi(ndin_s), fi(ub));
-                    acc += (double)cell_entropy(fi(Mbr), fi(din_r), fi(ub));
+                    if constexpr (!View::kHasH) acc += (double)cell_entropy(fi(Mbr), fi(din_r), fi(ub));
                     acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
                 }
             }
@@ -591,10 +616,10 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
             acc -= (double)cell_entropy(fi(m_rs + d_rs), fi(ndin_s), fi(ndout_r));
         }
     }
-    const double dS = wave_sum(acc);
-    pf = wave_sum(pf);
-    pb = wave_sum(pb);
-    R.nuniq = (int)wave_sum_ll(nuniq);
+    const double dS = group_sum<W>(acc);
+    pf = group_sum<W>(pf);
+    pb = group_sum<W>(pb);
+    R.nuniq = (int)group_sum_ll<W>(nuniq);
     double H = (k == 0) ? 1.0 : pb / pf;
     if (lw >= 0.0 && k > 0) {
         // mixture proposal: plain part by the closed form per unit of degree, learned part as given
@@ -616,7 +641,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     R.d_rr = d_rr;
     R.d_sr = d_sr;
     R.d_rs = d_rs;
-    R.nnz_rs = View::kSlots ? wave_sum_ll(live) : (long long)T;
+    R.nnz_rs = View::kSlots ? group_sum_ll<W>(live) : (long long)T;
     return R;
 }
 

@@ -86,6 +86,7 @@ struct DevBlockmodel {
     // *_cum are inclusive prefix sums of the values inside each row / column.
     int *row_ptr = nullptr, *row_col = nullptr, *row_val = nullptr, *row_cum = nullptr, *coo_row = nullptr;
     int *col_ptr = nullptr, *col_row = nullptr, *col_val = nullptr, *col_cum = nullptr;
+    float *row_h = nullptr, *col_h = nullptr;  // cell_entropy(M, din[col], dout[row]) per nonzero
     int *dout = nullptr, *din = nullptr, *deg = nullptr, *diag = nullptr, *drift = nullptr;
 };
 
@@ -244,35 +245,49 @@ struct Ctx {
     // shortens the slowest vertex when the bin holds few): while a bin's variants have fewer than
     // TUNE_TRIALS timed launches each they take turns; then the one with the lower time per
     // vertex is kept. The choice is made again after every change of the number of blocks.
-    static constexpr int NUM_VARIANTS = 2;
+    // Bin 0 (degree <= 30) has a third variant: 16 lanes per vertex, four vertices per wavefront.
+    static constexpr int NUM_VARIANTS = 3;
     static constexpr int TUNE_TRIALS = 2;
+    static int variants_of(int bin) { return bin == 0 ? 3 : (bin < NUM_LDS_BINS ? 2 : 1); }
     struct BinTuner {
-        double ms[NUM_VARIANTS] = {0, 0};
-        double vertices[NUM_VARIANTS] = {0, 0};
-        int trials[NUM_VARIANTS] = {0, 0};
+        double ms[NUM_VARIANTS] = {0, 0, 0};
+        double vertices[NUM_VARIANTS] = {0, 0, 0};
+        int trials[NUM_VARIANTS] = {0, 0, 0};
         int next = 0;
     };
     BinTuner tuner[8];
-    int tuner_B = -1;
-    int eval_variant = 0;  // 0 = tuned, 1 = first variant of every bin, 2 = the second
+    long tuner_key = -1;
+    int eval_variant = 0;  // 0 = tuned, k = variant k - 1 of every bin that has it (else the first)
+    // The choice is made again when the number of blocks changes (for a batch of graphs: when the
+    // blocks in use cross a power of two).
+    long tuning_key() const {
+        if (batch.G <= 0) return bm.B;
+        long total = 0;
+        for (int x : batch.nb) total += x;
+        int bits = 0;
+        while ((1L << (bits + 1)) <= total) ++bits;
+        return bits;
+    }
     int pick_variant(int bin) {
-        if (bin >= NUM_LDS_BINS) return 0;  // one variant only
-        if (eval_variant) return eval_variant - 1;
-        if (tuner_B != bm.B) {
+        const int nv = variants_of(bin);
+        if (nv == 1) return 0;
+        if (eval_variant) return eval_variant - 1 < nv ? eval_variant - 1 : 0;
+        const long key = tuning_key();
+        if (tuner_key != key) {
             for (auto& t : tuner) t = BinTuner();
-            tuner_B = bm.B;
+            tuner_key = key;
         }
         BinTuner& t = tuner[bin];
-        for (int v = 0; v < NUM_VARIANTS; ++v)
+        for (int v = 0; v < nv; ++v)
             if (t.trials[v] < TUNE_TRIALS) {
-                int w = (t.next + v) % NUM_VARIANTS;
+                int w = (t.next + v) % nv;
                 if (t.trials[w] < TUNE_TRIALS) {
-                    t.next = (w + 1) % NUM_VARIANTS;
+                    t.next = (w + 1) % nv;
                     return w;
                 }
             }
         int best = 0;
-        for (int v = 1; v < NUM_VARIANTS; ++v)
+        for (int v = 1; v < nv; ++v)
             if (t.ms[v] * t.vertices[best] < t.ms[best] * t.vertices[v]) best = v;
         return best;
     }

@@ -67,7 +67,7 @@ __device__ void write_result(const EvalParams& P, const MoveResult& R, int v, in
     }
 }
 
-template <int U = 2>
+template <int U = 2, int W = WAVE>
 __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
     const int v = P.list[idx];
     int gs = P.given_s ? P.given_s[idx] : -1;
@@ -79,9 +79,9 @@ __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int c
         lqf = P.lqf[v];
         lqb = P.lqb[v];
     }
-    MoveResult R = eval_move<CsrView, U>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu,
-                                         P.propose_only != 0, (double)P.lw, lqf, lqb);
-    if (lane_id() == 0) write_result(P, R, v, idx);
+    MoveResult R = eval_move<CsrView, U, W>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu,
+                                            P.propose_only != 0, (double)P.lw, lqf, lqb);
+    if (group_lane<W>() == 0) write_result(P, R, v, idx);
 }
 
 // One wavefront per vertex, its neighbour-block table in LDS (CAP slots); WAVES wavefronts per
@@ -93,6 +93,19 @@ __global__ void __launch_bounds__(WAVES * WAVE) eval_bin_kernel(EvalParams P) {
     const int idx = blockIdx.x * WAVES + wave;
     if (idx >= P.n) return;
     eval_vertex<U>(P, tab[wave], CAP, idx);
+}
+
+// Low-degree vertices packed WAVE / W to a wavefront: a group of W lanes per vertex, each with its
+// own CAP-slot table in LDS. The evaluation is a chain of dependent gathers, so more vertices in
+// flight per CU (not more lanes per vertex) is what shortens the launch.
+template <int CAP, int WAVES, int U, int W>
+__global__ void __launch_bounds__(WAVES * WAVE) eval_sub_kernel(EvalParams P) {
+    constexpr int PER = WAVES * (WAVE / W);
+    __shared__ int tab[PER][7 * CAP];
+    const int sub = threadIdx.x / W;
+    const int idx = blockIdx.x * PER + sub;
+    if (idx >= P.n) return;
+    eval_vertex<U, W>(P, tab[sub], CAP, idx);
 }
 
 // Vertices of the highest degrees: G wavefronts (the whole workgroup) share one vertex, its
@@ -122,6 +135,13 @@ __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
 
 template <int CAP, int WAVES>
 static void launch_bin(const EvalParams& P, int n, int variant, hipStream_t st) {
+    if constexpr (CAP == BIN_CAP[0]) {
+        if (variant == 2) {
+            constexpr int W = 16, SW = 4, PER = SW * (WAVE / W);
+            eval_sub_kernel<CAP, SW, 2, W><<<(n + PER - 1) / PER, SW * WAVE, 0, st>>>(P);
+            return;
+        }
+    }
     if (variant == 1)
         eval_bin_kernel<CAP, WAVES, 4><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
     else
@@ -142,7 +162,7 @@ static EvalParams make_params(Ctx& c) {
         P.S.gcall = b.d_call;
     }
     P.M = CsrView{c.bm.row_ptr, c.bm.row_col, c.bm.row_val, c.bm.row_cum,
-                  c.bm.col_ptr, c.bm.col_row, c.bm.col_val, c.bm.col_cum};
+                  c.bm.col_ptr, c.bm.col_row, c.bm.col_val, c.bm.col_cum, c.bm.row_h, c.bm.col_h};
     P.err = c.dev_err;
     P.pool = c.table_pool;
     P.lw = -1.0f;

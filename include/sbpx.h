@@ -64,9 +64,12 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            kernel: 1, 2, 4, 8 or 16, or 0 to choose by the longest block-matrix row (default).
  * option 3 = kernel variants of the batched evaluation per degree bin: 1 the first (default
  *            unroll, one wavefront per vertex), 2 the second (four row slots per lane per trip
- *            for degrees <= 126; 4 wavefronts sharing a vertex for degrees 127-510), or 0 to time
- *            both and keep the faster per bin after every change of the number of blocks
- *            (default). Results do not depend on options 2 and 3.
+ *            for degrees <= 126; 4 wavefronts sharing a vertex for degrees 127-510), 3 the third
+ *            where a bin has one (degrees <= 30: 16 lanes per vertex, four vertices per
+ *            wavefront; other bins take the first), or 0 to time them and keep the fastest per
+ *            bin after every change of the number of blocks (default). Variants that split a
+ *            vertex's sums over other lanes add them in another order, so delta MDL may differ
+ *            in its last bits between them.
  * option 4 = proposal of the batched evaluation: 0 the reference's, 1 the learned mixture
  *            (see sbpx_learned_init). */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);

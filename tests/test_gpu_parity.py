@@ -197,11 +197,12 @@ def test_apply_moves_matches_sequential_application_with_degree_drift():
 
 
 # ----------------------------------------------------------------------------- sweeps
-@pytest.mark.parametrize("batches", [1, 3, 10])
-def test_async_gibbs_sweeps_match_oracle(batches):
+@pytest.mark.parametrize("batches,variant", [(1, 0), (3, 0), (10, 0), (1, 3), (3, 3)])
+def test_async_gibbs_sweeps_match_oracle(batches, variant):
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=0)
     c = ctx_for(arrays)
+    c.set_option(3, variant)
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(4)
     for B, a in ((n, np.arange(n, dtype=np.int32)), (64, rng.integers(0, 64, n).astype(np.int32)), (8, truth)):
@@ -370,11 +371,12 @@ def heavy_arrays(seed=0):
     return O.csr_from_edges(n, sorted(edges))
 
 
-@pytest.mark.parametrize("variant", [1, 2, 0])
+@pytest.mark.parametrize("variant", [1, 2, 3, 0])
 def test_every_degree_bin_matches_oracle(variant):
     """Every degree bin and kernel variant (one wavefront per vertex with 64..1024-slot LDS
-    tables, more wavefronts per SIMD, 4 or 16 wavefronts sharing a vertex, tables in global
-    memory) against the oracle, through proposals, fixed-proposal evaluation and sweeps."""
+    tables, more wavefronts per SIMD, 16 lanes per vertex for degrees <= 30, 4 or 16 wavefronts
+    sharing a vertex, tables in global memory) against the oracle, through proposals,
+    fixed-proposal evaluation and sweeps."""
     m = sbpx()
     arrays = heavy_arrays()
     n = len(arrays[0]) - 1

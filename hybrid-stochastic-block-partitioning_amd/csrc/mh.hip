@@ -227,50 +227,96 @@ __device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab
     return true;
 }
 
-// Applies the accepted move: block matrix cells from (o_b, i_b, self), block degrees with the
-// update rule of Blockmodel::move_vertex (blockmodel.cpp:344-360), the assignment.
-__device__ void commit_move(const MhParams& P, const MoveResult& R, const int* tab, int cap, int serial) {
-    const int lane = lane_id();
-    const int r = R.r, s = R.s;
-    {  // stamp the neighbours, whose neighbour-block tables this move changes
-        const int v = R.v;
-        const int ob = P.G.out_ptr[v], oe = P.G.out_ptr[v + 1];
-        const int ib = P.G.in_ptr[v], ie = P.G.in_ptr[v + 1];
-        for (int j = ob + lane; j < oe; j += WAVE) P.touch[P.G.out_idx[j]] = serial;
-        for (int j = ib + lane; j < ie; j += WAVE) P.touch[P.G.in_idx[j]] = serial;
-    }
+// The fields of an accepted move its commit needs, shared through LDS with every wavefront.
+struct CommitInfo {
+    int v, r, s, k_out, k_in, self, d_ss, d_rr, d_sr, d_rs, m_rr, m_ss;
+};
+__device__ __forceinline__ CommitInfo commit_info(const MoveResult& R) {
+    return CommitInfo{R.v, R.r, R.s, R.k_out, R.k_in, R.self, R.d_ss, R.d_rr, R.d_sr, R.d_rs, R.m_rr, R.m_ss};
+}
+
+// Applies an accepted move with every wavefront of the workgroup: block matrix cells from
+// (o_b, i_b, self), block degrees with the update rule of Blockmodel::move_vertex
+// (blockmodel.cpp:344-360), the assignment, and the neighbours' touch stamps. Each upsert is a
+// chain of dependent global reads and atomics, so the updates are spread over lanes rather than
+// run one after another. Row s and column s are the only tables that can take several new keys
+// in one commit: wavefront 0 fills row s and wavefront 1 column s, walking the neighbour table in
+// slot order with one lane per slot and then the cells among {r, s}, so the keys land in the same
+// slots on every run and later scans see the same order. Every other update is to a key that
+// exists or is the only new key of its table, and is spread over the remaining wavefronts. The
+// committing wavefront's table `tab` is in LDS or global memory; make_room has run.
+__device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int* tab, int cap, int serial) {
+    constexpr int NT = MH_WAVES * WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const int r = C.r, s = C.s;
     const int* to = tab + cap;
     const int* ti = tab + 2 * cap;
-    for (int j = lane; j < cap; j += WAVE) {
-        int b = tab[j];
-        if (b < 0 || b == r || b == s) continue;
-        int o = to[j], i = ti[j];
-        if (o) {
-            cell_add(P, s, b, o);
-            cell_add(P, r, b, -o);
-        }
-        if (i) {
-            cell_add(P, b, s, i);
-            cell_add(P, b, r, -i);
-        }
+    {  // stamp the neighbours, whose neighbour-block tables this move changes
+        const int v = C.v;
+        const int ob = P.G.out_ptr[v], oe = P.G.out_ptr[v + 1];
+        const int ib = P.G.in_ptr[v], ie = P.G.in_ptr[v + 1];
+        for (int j = ob + tid; j < oe; j += NT) P.touch[P.G.out_idx[j]] = serial;
+        for (int j = ib + tid; j < ie; j += NT) P.touch[P.G.in_idx[j]] = serial;
     }
-    if (lane == 0) {
-        if (R.d_ss) cell_add(P, s, s, R.d_ss);
-        if (R.d_rr) cell_add(P, r, r, R.d_rr);
-        if (R.d_sr) cell_add(P, s, r, R.d_sr);
-        if (R.d_rs) cell_add(P, r, s, R.d_rs);
-        P.assign[R.v] = s;
-        const int nout_r = P.dout[r] - R.k_out, nout_s = P.dout[s] + R.k_out;
-        const int nin_r = P.din[r] - (R.k_in + R.self), nin_s = P.din[s] + (R.k_in + R.self);
-        P.dout[r] = nout_r;
-        P.dout[s] = nout_s;
-        P.din[r] = nin_r;
-        P.din[s] = nin_s;
-        // self edges read after the update, plus the Delta diagonal (blockmodel.cpp:348-359)
-        P.deg[r] = nout_r + nin_r - ((R.m_rr + R.d_rr) + R.d_rr);
-        P.deg[s] = nout_s + nin_s - ((R.m_ss + R.d_ss) + R.d_ss);
-        P.drift[r] = R.d_rr;
-        P.drift[s] = R.d_ss;
+    if (wave == 0) {  // row s: (s, b) += o_b, then (s, s) and (s, r)
+        for (int j = lane; j < cap; j += WAVE) {
+            const int b = tab[j];
+            if (b < 0 || b == r || b == s) continue;
+            const int o = to[j];
+            if (o) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, b, o);
+        }
+        if (lane == 0) {
+            if (C.d_ss) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, s, C.d_ss);
+            if (C.d_sr) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, r, C.d_sr);
+        }
+    } else if (wave == 1) {  // column s: (b, s) += i_b, then (s, s) and (r, s)
+        for (int j = lane; j < cap; j += WAVE) {
+            const int b = tab[j];
+            if (b < 0 || b == r || b == s) continue;
+            const int i = ti[j];
+            if (i) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, s, b, i);
+        }
+        if (lane == 0) {
+            if (C.d_ss) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, s, s, C.d_ss);
+            if (C.d_rs) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, s, r, C.d_rs);
+        }
+    } else if (wave == 2) {  // row r and column r at the diagonal and at s; degrees; assignment
+        if (lane == 0) {
+            if (C.d_rr) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, r, r, C.d_rr);
+            if (C.d_rs) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, r, s, C.d_rs);
+        } else if (lane == 1) {
+            if (C.d_rr) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, r, r, C.d_rr);
+            if (C.d_sr) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, r, s, C.d_sr);
+        } else if (lane == 2) {
+            P.assign[C.v] = s;
+            const int nout_r = P.dout[r] - C.k_out, nout_s = P.dout[s] + C.k_out;
+            const int nin_r = P.din[r] - (C.k_in + C.self), nin_s = P.din[s] + (C.k_in + C.self);
+            P.dout[r] = nout_r;
+            P.dout[s] = nout_s;
+            P.din[r] = nin_r;
+            P.din[s] = nin_s;
+            // self edges read after the update, plus the Delta diagonal (blockmodel.cpp:348-359)
+            P.deg[r] = nout_r + nin_r - ((C.m_rr + C.d_rr) + C.d_rr);
+            P.deg[s] = nout_s + nin_s - ((C.m_ss + C.d_ss) + C.d_ss);
+            P.drift[r] = C.d_rr;
+            P.drift[s] = C.d_ss;
+        }
+    } else {  // the other six updates per neighbour block b
+        constexpr int NW = (MH_WAVES - 3) * WAVE;
+        for (int it = tid - 3 * WAVE; it < 6 * cap; it += NW) {
+            const int k = it / cap, j = it - k * cap;
+            const int b = tab[j];
+            if (b < 0 || b == r || b == s) continue;
+            const int o = to[j], i = ti[j];
+            switch (k) {
+                case 0: if (o) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, b, s, o); break;   // (s, b)
+                case 1: if (o) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, r, b, -o); break;  // (r, b)
+                case 2: if (o) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, b, r, -o); break;  // (r, b)
+                case 3: if (i) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, b, s, i); break;   // (b, s)
+                case 4: if (i) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, b, r, -i); break;  // (b, r)
+                default: if (i) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, r, b, -i); break; // (b, r)
+            }
+        }
     }
 }
 
@@ -328,6 +374,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     __shared__ int holds[MH_WAVES]; // 1 while that evaluation still holds
     __shared__ double sdS[MH_WAVES];
     __shared__ int s_flag, s_cr, s_cs, s_ccap;
+    __shared__ CommitInfo s_commit;
     __shared__ int* s_ctab;
     __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
     __shared__ int lds_hist[MH_WAVES][256];
@@ -402,13 +449,13 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
             }
             if (wave == j) {
                 const bool ok = make_room(P, R, tab, cap);
-                if (ok) commit_move(P, R, tab, cap, serial + 1);
                 if (lane == 0) {
                     s_flag = ok ? 1 : 2;
                     s_cr = R.r;
                     s_cs = R.s;
                     s_ctab = tab;
                     s_ccap = cap;
+                    s_commit = commit_info(R);
                 }
             }
             __syncthreads();
@@ -417,6 +464,8 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
                 pos += j;
                 break;
             }
+            commit_shared(P, s_commit, s_ctab, s_ccap, serial + 1);
+            __syncthreads();
             ++serial;
             if (wave == j && lane == 0) {
                 if (P.bytes != nullptr) bytes_local += eval_bytes(P, R);
@@ -463,6 +512,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     __shared__ int holds[NG];
     __shared__ double sdS[NG];
     __shared__ int s_flag, s_cr, s_cs, s_ccap;
+    __shared__ CommitInfo s_commit;
     __shared__ int* s_ctab;
     __shared__ int lds_tab[NG][7 * LCAP];
     __shared__ int lds_hist[NG][256];
@@ -534,13 +584,13 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
             }
             if (g == j && leader) {
                 const bool ok = make_room(P, R, tab, cap);
-                if (ok) commit_move(P, R, tab, cap, serial + 1);
                 if (lane == 0) {
                     s_flag = ok ? 1 : 2;
                     s_cr = R.r;
                     s_cs = R.s;
                     s_ctab = tab;
                     s_ccap = cap;
+                    s_commit = commit_info(R);
                 }
             }
             __syncthreads();
@@ -549,6 +599,8 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
                 pos += j;
                 break;
             }
+            commit_shared(P, s_commit, s_ctab, s_ccap, serial + 1);
+            __syncthreads();
             ++serial;
             if (g == j && leader && lane == 0) {
                 if (P.bytes != nullptr) bytes_local += eval_bytes(P, R);

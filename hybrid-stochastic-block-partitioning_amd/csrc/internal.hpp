@@ -124,6 +124,7 @@ struct MhState {
     int max_cap = 0;        // slots of the longest row or column when the tables were built
     double mean_scan = -1.0;  // mean row/column slots scanned per evaluation in the last chunk
     int last_B = -1;          // number of blocks when mean_scan was measured
+    int g_yield = 1;  // wavefronts per vertex the last chunk's window yield calls for
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
     int max_deg_high = 0;

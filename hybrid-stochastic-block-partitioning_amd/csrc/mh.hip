@@ -106,7 +106,8 @@ struct MhParams {
     long long* counters;  // attempts, evals, moves
     long long* bytes;     // algorithmic bytes of the evaluations that stand, or null
     int* touch;           // [N] serial of the last commit that moved a neighbour of the vertex
-    double* scan;         // [2] slots scanned in rows/columns r and s, and evaluations, that stand
+    double* scan;         // [3] slots scanned in rows/columns r and s, and evaluations, that stand;
+                          // windows evaluated
     int stop_at;          // queue position where this launch ends
     double* dS_sum;
     int* err;
@@ -388,8 +389,10 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     int pos = P.start;
     int stopped = 0;
     int serial = P.status[4];
+    double windows = 0.0;
     while (pos < P.stop_at && stopped == 0) {
         // a window of up to 16 queue entries, each evaluated against the current state
+        windows += 1.0;
         const int idx = pos + wave;
         MoveResult R{};
         const bool active = idx < P.n;
@@ -493,6 +496,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
         atomicAdd(&P.scan[0], scan_local);
         atomicAdd(&P.scan[1], scan_count);
     }
+    if (threadIdx.x == 0) atomicAdd(&P.scan[2], windows);
     if (threadIdx.x == 0) {
         P.status[0] = pos;
         P.status[1] = stopped;
@@ -530,7 +534,9 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     int pos = P.start;
     int stopped = 0;
     int serial = P.status[4];
+    double windows = 0.0;
     while (pos < P.stop_at && stopped == 0) {
+        windows += 1.0;
         const int idx = pos + g;
         const bool active = idx < P.n;
         const int v = active ? P.list[idx] : -1;
@@ -628,6 +634,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
         atomicAdd(&P.scan[0], scan_local);
         atomicAdd(&P.scan[1], scan_count);
     }
+    if (threadIdx.x == 0) atomicAdd(&P.scan[2], windows);
     if (threadIdx.x == 0) {
         P.status[0] = pos;
         P.status[1] = stopped;
@@ -1000,7 +1007,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     }
     int pos = 0;
     int rounds = 0;
-    double* scan = c.red + 2052;  // [2] scanned slots and evaluations of the current chunk
+    double* scan = c.red + 2052;  // [3] scanned slots, evaluations and windows of the current chunk
     while (pos < n) {
         // each restart doubles the pool headroom, so the number of restarts is logarithmic
         if (rounds++ > 24) throw StateError("Metropolis-Hastings pass keeps running out of table room");
@@ -1040,20 +1047,25 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.err = c.dev_err;
         bool out_of_room = false;
         // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
-        // mean number of row/column slots the previous chunk's evaluations scanned (long scans
-        // gain from more lanes per vertex, short ones from more vertices in flight). The first
-        // chunk at a new number of blocks goes by the longest row instead.
+        // previous chunk: the mean number of row/column slots its evaluations scanned (long scans
+        // gain from more lanes per vertex, short ones from more vertices in flight) and the
+        // queue entries a window of speculative evaluations advanced before a commit
+        // invalidated the rest (when few stand, fewer vertices per window at more wavefronts
+        // each finish a window sooner). The first chunk at a new number of blocks goes by the
+        // longest row instead.
         while (pos < n) {
             int G = c.mh_group;
             if (G == 0) {
-                if (h.last_B != B || h.mean_scan < 0.0)
+                if (h.last_B != B || h.mean_scan < 0.0) {
                     G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : 1);
-                else
+                } else {
                     G = h.mean_scan >= 4000.0 ? 16 : (h.mean_scan >= 1500.0 ? 4 : 1);
+                    G = std::max(G, h.g_yield);
+                }
             }
             P.start = pos;
             P.stop_at = std::min(n, pos + MH_CHUNK);
-            SBPX_HIP(hipMemsetAsync(scan, 0, 2 * sizeof(double), c.stream));
+            SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));
             if (timed) c.timer_begin();
             switch (G) {
                 case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
@@ -1065,15 +1077,24 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             SBPX_HIP(hipGetLastError());
             if (timed) c.timer_end(KERNEL_MH);
             int stat[5];
-            double sc[2];
+            double sc[3];
             SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipMemcpyAsync(sc, scan, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipStreamSynchronize(c.stream));
             check_device_error(c);
+            const int advanced = stat[0] - pos;
             pos = stat[0];
             if (sc[1] > 0.0) {
                 h.mean_scan = sc[0] / sc[1];
                 h.last_B = B;
+            }
+            // Entries a window advanced, at 16 / G vertices per window. Near that cap the windows
+            // could be longer: halve G; below it the yield is the measured one: about 3 or fewer
+            // standing entries take 4 wavefronts per vertex, up to 6 two, more one.
+            if (sc[2] > 0.0 && advanced > 0) {
+                const double y = (double)advanced / sc[2];
+                if (y >= 0.75 * (double)(MH_WAVES / G)) h.g_yield = std::max(1, G / 2);
+                else h.g_yield = y < 3.0 ? 4 : (y < 6.0 ? 2 : 1);
             }
             if (stat[4] > (1 << 30)) {  // restart the commit serials well before they wrap
                 SBPX_HIP(hipMemsetAsync(h.touch, 0, sizeof(int) * c.g.N, c.stream));

@@ -283,7 +283,8 @@ void require_batch(Ctx& c) {
 
 void free_batch(Ctx& c) {
     DevBatch& b = c.batch;
-    for (int** p : {&b.d_vgraph, &b.d_bgraph, &b.d_vo, &b.d_bo, &b.d_nb, &b.d_req, &b.d_plan, &b.d_plan_huge})
+    for (int** p : {&b.d_vgraph, &b.d_bgraph, &b.d_vo, &b.d_bo, &b.d_nb, &b.d_req, &b.d_plan, &b.d_plan_huge,
+                    &b.d_dbo, &b.d_dense, &b.d_inv})
         dfree(p);
     dfree(&b.d_seed);
     dfree(&b.d_call);
@@ -1266,9 +1267,20 @@ void batch_select(Ctx& c, int n, const int32_t* graphs, const uint64_t* seeds, c
     }
 }
 
+// Uploads the blocks in use per graph and renumbers the blocks in use densely (the rebuild's
+// sort keys).
 void batch_upload_nb(Ctx& c) {
     DevBatch& b = c.batch;
+    b.h_dbo.resize(b.G);
+    int used = 0;
+    for (int g = 0; g < b.G; ++g) {
+        b.h_dbo[g] = used;
+        used += b.nb[g];
+    }
+    b.used = used;
     SBPX_HIP(hipMemcpyAsync(b.d_nb, b.nb.data(), sizeof(int) * b.G, hipMemcpyHostToDevice, c.stream));
+    SBPX_HIP(hipMemcpyAsync(b.d_dbo, b.h_dbo.data(), sizeof(int) * b.G, hipMemcpyHostToDevice, c.stream));
+    launch_batch_dense(c);
 }
 
 // The reference's batches of every graph (finetune.cpp:43-47, 352-366, as in build_plan), split
@@ -1409,6 +1421,9 @@ sbpx_status sbpx_batch_define(sbpx_ctx* ctx, int32_t num_graphs, const int32_t* 
         dalloc(&b.d_call, G);
         dalloc(&b.d_active, G);
         dalloc(&b.d_req, 2 * (size_t)G + 2);
+        dalloc(&b.d_dbo, G);
+        dalloc(&b.d_dense, b.range);
+        dalloc(&b.d_inv, b.range);
         std::vector<int> bgraph(b.range), assign(N);
         for (int g = 0; g < G; ++g) {
             for (int x = bo[g]; x < bo[g + 1]; ++x) bgraph[x] = g;
@@ -1418,7 +1433,7 @@ sbpx_status sbpx_batch_define(sbpx_ctx* ctx, int32_t num_graphs, const int32_t* 
         SBPX_HIP(hipMemcpy(b.d_bgraph, bgraph.data(), sizeof(int) * b.range, hipMemcpyHostToDevice));
         SBPX_HIP(hipMemcpy(b.d_vo, b.vo.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
         SBPX_HIP(hipMemcpy(b.d_bo, b.bo.data(), sizeof(int) * (G + 1), hipMemcpyHostToDevice));
-        SBPX_HIP(hipMemcpy(b.d_nb, b.nb.data(), sizeof(int) * G, hipMemcpyHostToDevice));
+        batch_upload_nb(c);
         SBPX_HIP(hipMemset(b.d_seed, 0, sizeof(uint64_t) * G));
         SBPX_HIP(hipMemset(b.d_call, 0, sizeof(uint32_t) * G));
         SBPX_HIP(hipMemset(b.d_active, 0, G));

@@ -20,10 +20,13 @@ static int bits_for(int B) {
     return bits;
 }
 
+// Sort keys use block ids, or, for a batch of graphs, their dense numbering among the blocks in
+// use (dense[x], increasing with x, -1 for ids not in use; inv maps it back), so that the keys
+// are as narrow as the blocks in use allow.
 template <typename K>
 __global__ void edge_keys_kernel(const int* __restrict__ edge_src, const int* __restrict__ out_idx,
                                  const int* __restrict__ assign, int E, int B, int shift, K* __restrict__ keys,
-                                 int* __restrict__ err) {
+                                 int* __restrict__ err, const int* __restrict__ dense) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= E) return;
     int bu = assign[edge_src[i]];
@@ -32,6 +35,15 @@ __global__ void edge_keys_kernel(const int* __restrict__ edge_src, const int* __
         atomicOr(err, DEV_ERR_BLOCK_RANGE);
         bu = 0;
         bw = 0;
+    }
+    if (dense != nullptr) {
+        bu = dense[bu];
+        bw = dense[bw];
+        if (bu < 0 || bw < 0) {
+            atomicOr(err, DEV_ERR_BLOCK_RANGE);
+            bu = 0;
+            bw = 0;
+        }
     }
     keys[i] = ((K)bu << shift) | (K)bw;
 }
@@ -61,12 +73,17 @@ __global__ void compact_runs_kernel(const K* __restrict__ keys, const int* __res
 template <typename K>
 __global__ void decode_csr_kernel(const K* __restrict__ ukeys, const int* __restrict__ start, int nnz,
                                   int shift, K mask, int* __restrict__ coo_row, int* __restrict__ row_col,
-                                  int* __restrict__ row_val) {
+                                  int* __restrict__ row_val, const int* __restrict__ inv) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nnz) return;
     K key = ukeys[j];
-    coo_row[j] = (int)(key >> shift);
-    row_col[j] = (int)(key & mask);
+    int row = (int)(key >> shift), col = (int)(key & mask);
+    if (inv != nullptr) {
+        row = inv[row];
+        col = inv[col];
+    }
+    coo_row[j] = row;
+    row_col[j] = col;
     row_val[j] = start[j + 1] - start[j];
 }
 
@@ -101,21 +118,31 @@ __global__ void diag_kernel(const int* __restrict__ coo_row, const int* __restri
 
 template <typename K>
 __global__ void csc_keys_kernel(const int* __restrict__ coo_row, const int* __restrict__ row_col, int nnz,
-                                int shift, K* __restrict__ keys) {
+                                int shift, K* __restrict__ keys, const int* __restrict__ dense) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nnz) return;
-    keys[j] = ((K)row_col[j] << shift) | (K)coo_row[j];
+    int row = coo_row[j], col = row_col[j];
+    if (dense != nullptr) {
+        row = dense[row];
+        col = dense[col];
+    }
+    keys[j] = ((K)col << shift) | (K)row;
 }
 
 template <typename K>
 __global__ void decode_csc_kernel(const K* __restrict__ keys, const int* __restrict__ vals, int nnz, int shift,
                                   K mask, int* __restrict__ col_of, int* __restrict__ col_row,
-                                  int* __restrict__ col_val) {
+                                  int* __restrict__ col_val, const int* __restrict__ inv) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nnz) return;
     K key = keys[j];
-    col_of[j] = (int)(key >> shift);
-    col_row[j] = (int)(key & mask);
+    int col = (int)(key >> shift), row = (int)(key & mask);
+    if (inv != nullptr) {
+        col = inv[col];
+        row = inv[row];
+    }
+    col_of[j] = col;
+    col_row[j] = row;
     col_val[j] = vals[j];
 }
 
@@ -146,10 +173,14 @@ static void build_impl(Ctx& c) {
     DevBlockmodel& bm = c.bm;
     hipStream_t st = c.stream;
     const int B = bm.B;
-    const int bits = bits_for(B);
-    const bool packed = sizeof(K) == 4;
-    const int shift = packed ? bits : 32;
-    const K mask = packed ? (K)((1u << bits) - 1u) : (K)0xFFFFFFFFu;
+    const bool dn = c.batch.G > 0;  // a batch: keys over the blocks in use
+    const int* dense = dn ? c.batch.d_dense : nullptr;
+    const int* inv = dn ? c.batch.d_inv : nullptr;
+    const int bits = bits_for(dn ? c.batch.used : B);
+    // key = row << bits | column: 2 * bits significant bits in 32- or 64-bit keys, sorted by
+    // 8-bit digits from the lowest
+    const int shift = bits;
+    const K mask = (K)((((K)1) << bits) - 1);
     K* keys0 = (K*)c.keys0;
     K* keys1 = (K*)c.keys1;
     const int E = g.E;
@@ -158,16 +189,11 @@ static void build_impl(Ctx& c) {
     int shifts[8];
     int np = 0;
     int nbytes = (bits + 7) / 8;
-    if (packed) {
-        for (int b = 0; b < 2 * bits; b += 8) shifts[np++] = b;
-    } else {
-        for (int i = 0; i < nbytes; ++i) shifts[np++] = 8 * i;
-        for (int i = 0; i < nbytes; ++i) shifts[np++] = 32 + 8 * i;
-    }
+    for (int b = 0; b < 2 * bits; b += 8) shifts[np++] = b;
     int nnz = 0;
     if (E > 0) {
         edge_keys_kernel<K><<<blocks_for(E), TB, 0, st>>>(g.edge_src, g.out_idx, bm.assign, E, B, shift, keys0,
-                                                          c.dev_err);
+                                                          c.dev_err, dense);
         int which = radix_sort<K>(keys0, keys1, nullptr, nullptr, E, shifts, np, c.hist, c.scan_tmp, st);
         K* sorted = which ? keys1 : keys0;
         K* ukeys = which ? keys0 : keys1;
@@ -178,7 +204,7 @@ static void build_impl(Ctx& c) {
         compact_runs_kernel<K><<<blocks_for(E), TB, 0, st>>>(sorted, c.flags, c.pos, E, ukeys, c.vals0);
         SBPX_HIP(hipStreamSynchronize(st));
         decode_csr_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(ukeys, c.vals0, nnz, shift, mask, bm.coo_row,
-                                                             bm.row_col, bm.row_val);
+                                                             bm.row_col, bm.row_val, inv);
     }
     bm.nnz = nnz;
     // 3. row pointers, row prefix sums, dout, diagonal
@@ -190,7 +216,7 @@ static void build_impl(Ctx& c) {
     if (nnz > 0) diag_kernel<<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, bm.row_val, nnz, bm.diag);
     // 4. CSC: stable sort of the unique cells by column (input already ordered by row)
     if (nnz > 0) {
-        csc_keys_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, nnz, shift, keys0);
+        csc_keys_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, nnz, shift, keys0, dense);
         SBPX_HIP(hipMemcpyAsync(c.vals0, bm.row_val, sizeof(int) * nnz, hipMemcpyDeviceToDevice, st));
         int cshifts[4];
         int cnp = 0;
@@ -200,7 +226,7 @@ static void build_impl(Ctx& c) {
         int* cv = which ? c.vals1 : c.vals0;
         int* col_of = c.flags;  // column index per CSC entry (scratch)
         decode_csc_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(ck, cv, nnz, shift, mask, col_of, bm.col_row,
-                                                             bm.col_val);
+                                                             bm.col_val, inv);
         segment_ptr_kernel<<<blocks_for(B + 1), TB, 0, st>>>(col_of, nnz, B, bm.col_ptr);
         scan_exclusive(bm.col_val, c.pos, nnz, c.scan_tmp, st);
         segment_cum_kernel<<<blocks_for(nnz), TB, 0, st>>>(col_of, bm.col_ptr, c.pos, nnz, bm.col_cum);
@@ -216,7 +242,7 @@ static void build_impl(Ctx& c) {
 }
 
 void rebuild_keep_drift(Ctx& c) {
-    const int bits = bits_for(c.bm.B);
+    const int bits = bits_for(c.batch.G > 0 ? c.batch.used : c.bm.B);
     if (2 * bits <= 32) build_impl<uint32_t>(c);
     else build_impl<uint64_t>(c);
 }

@@ -148,9 +148,14 @@ struct DevBatch {
     uint32_t* d_call = nullptr;
     uint8_t* d_active = nullptr;  // [G] graphs taking part in the current operation
     int* d_req = nullptr;         // [2 G + 2] request staging
+    int* d_dbo = nullptr;         // [G] first dense id of each graph (prefix of nb)
+    int* d_dense = nullptr;       // [range] dense number of each block id in use, -1 otherwise
+    int* d_inv = nullptr;         // [range] block id of each dense number
+    int used = 0;                 // blocks in use over all graphs (sum of nb)
     std::vector<uint8_t> h_active;
     std::vector<uint64_t> h_seed;
     std::vector<uint32_t> h_call;
+    std::vector<int> h_dbo;
     // the reference's batches of every graph, split by degree bin: lists[g][k][bin] (union ids)
     std::vector<std::vector<std::array<std::vector<int>, NUM_BINS>>> lists;
     int lists_batches = 0;
@@ -368,6 +373,7 @@ void launch_merge_proposals(Ctx& c, uint64_t seed, uint32_t call);  // union: pe
 void launch_batch_relabel(Ctx& c, int lo, int hi);  // assign[v] = map[assign[v] - lo] where in [lo, hi)
 void launch_zero_drift_active(Ctx& c);              // drift = 0 on the blocks of active graphs
 void launch_batch_scatter(Ctx& c, int n, int total);  // assignments of n requests into the union
+void launch_batch_dense(Ctx& c);                       // dense/inv from nb (d_nb, d_dbo uploaded)
 void launch_block_counts(Ctx& c);
 void launch_relabel(Ctx& c);
 

@@ -193,6 +193,27 @@ __global__ void batch_scatter_kernel(const int* __restrict__ vals, int total, co
     assign[vo[g] + (p - req[i])] = bo[g] + vals[p];
 }
 
+__global__ void batch_dense_kernel(int range, const int* __restrict__ bgraph, const int* __restrict__ bo,
+                                   const int* __restrict__ nb, const int* __restrict__ dbo, int* __restrict__ dense,
+                                   int* __restrict__ inv) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= range) return;
+    const int g = bgraph[x], l = x - bo[g];
+    if (l < nb[g]) {
+        dense[x] = dbo[g] + l;
+        inv[dbo[g] + l] = x;
+    } else {
+        dense[x] = -1;
+    }
+}
+
+void launch_batch_dense(Ctx& c) {
+    const DevBatch& b = c.batch;
+    batch_dense_kernel<<<(b.range + 255) / 256, 256, 0, c.stream>>>(b.range, b.d_bgraph, b.d_bo, b.d_nb, b.d_dbo,
+                                                                    b.d_dense, b.d_inv);
+    SBPX_HIP(hipGetLastError());
+}
+
 void launch_batch_relabel(Ctx& c, int lo, int hi) {
     batch_relabel_kernel<<<(c.g.N + 255) / 256, 256, 0, c.stream>>>(c.bm.assign, c.g.N, c.m_map, lo, hi);
     SBPX_HIP(hipGetLastError());

@@ -1516,11 +1516,10 @@ sbpx_status sbpx_batch_trace(sbpx_ctx* ctx, int64_t* trace) {
         require_batch(c);
         if (!trace) throw ArgError("null output");
         const DevBatch& b = c.batch;
-        std::vector<int> diag(b.range);
-        SBPX_HIP(hipMemcpyAsync(diag.data(), c.bm.diag, sizeof(int) * b.range, hipMemcpyDeviceToHost, c.stream));
+        long long* d_trace = (long long*)c.m_dS;  // merge scratch, free between merge phases
+        launch_batch_trace(c, d_trace);
+        SBPX_HIP(hipMemcpyAsync(trace, d_trace, sizeof(int64_t) * b.G, hipMemcpyDeviceToHost, c.stream));
         c.sync();
-        for (int g = 0; g < b.G; ++g)
-            trace[g] = std::accumulate(diag.begin() + b.bo[g], diag.begin() + b.bo[g] + b.nb[g], (int64_t)0);
     });
 }
 

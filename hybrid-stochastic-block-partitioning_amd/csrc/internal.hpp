@@ -374,6 +374,7 @@ void launch_batch_relabel(Ctx& c, int lo, int hi);  // assign[v] = map[assign[v]
 void launch_zero_drift_active(Ctx& c);              // drift = 0 on the blocks of active graphs
 void launch_batch_scatter(Ctx& c, int n, int total);  // assignments of n requests into the union
 void launch_batch_dense(Ctx& c);                       // dense/inv from nb (d_nb, d_dbo uploaded)
+void launch_batch_trace(Ctx& c, long long* d_trace);   // per-graph sum of the block-matrix diagonal
 void launch_block_counts(Ctx& c);
 void launch_relabel(Ctx& c);
 

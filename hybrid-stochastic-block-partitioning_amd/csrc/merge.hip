@@ -207,6 +207,23 @@ __global__ void batch_dense_kernel(int range, const int* __restrict__ bgraph, co
     }
 }
 
+// trace[g] = sum of diag over graph g's blocks in use; one wavefront per graph.
+__global__ void batch_trace_kernel(const int* __restrict__ diag, int G, const int* __restrict__ bo,
+                                   const int* __restrict__ nb, long long* __restrict__ trace) {
+    const int g = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (g >= G) return;
+    long long acc = 0;
+    for (int x = bo[g] + lane_id(); x < bo[g] + nb[g]; x += WAVE) acc += diag[x];
+    acc = wave_sum_ll(acc);
+    if (lane_id() == 0) trace[g] = acc;
+}
+
+void launch_batch_trace(Ctx& c, long long* d_trace) {
+    const DevBatch& b = c.batch;
+    batch_trace_kernel<<<(b.G + 3) / 4, 4 * WAVE, 0, c.stream>>>(c.bm.diag, b.G, b.d_bo, b.d_nb, d_trace);
+    SBPX_HIP(hipGetLastError());
+}
+
 void launch_batch_dense(Ctx& c) {
     const DevBatch& b = c.batch;
     batch_dense_kernel<<<(b.range + 255) / 256, 256, 0, c.stream>>>(b.range, b.d_bgraph, b.d_bo, b.d_nb, b.d_dbo,

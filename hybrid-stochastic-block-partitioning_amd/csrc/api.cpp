@@ -490,6 +490,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
         } else if (option == 3) {
             if (value < 0 || value > 3) throw ArgError("evaluation kernel variant must be 0 (tuned), 1, 2 or 3");
             ctx->c.eval_variant = (int)value;
+        } else if (option == 5) {
+            if (value != 0 && value != 1) throw ArgError("incremental update must be 0 (off) or 1 (on)");
+            ctx->c.incremental = (int)value;
         } else if (option == 2) {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");

@@ -167,45 +167,39 @@ __global__ void degrees_kernel(const int* __restrict__ dout, const int* __restri
     deg[x] = dout[x] + din[x] - diag[x] - drift[x];
 }
 
+// Sort-key layout of one build: key = row << bits | column, over block ids or, for a batch of
+// graphs, their dense numbering among the blocks in use.
 template <typename K>
-static void build_impl(Ctx& c) {
-    DevGraph& g = c.g;
+struct KeySpace {
+    int bits, shift;
+    K mask;
+    const int* dense;
+    const int* inv;
+};
+
+template <typename K>
+static KeySpace<K> key_space(const Ctx& c) {
+    const bool dn = c.batch.G > 0;  // a batch: keys over the blocks in use
+    KeySpace<K> ks;
+    ks.bits = bits_for(dn ? c.batch.used : c.bm.B);
+    ks.shift = ks.bits;
+    ks.mask = (K)((((K)1) << ks.bits) - 1);
+    ks.dense = dn ? c.batch.d_dense : nullptr;
+    ks.inv = dn ? c.batch.d_inv : nullptr;
+    return ks;
+}
+
+// Steps 3-4 of a build from the cells (coo_row, row_col, row_val; nnz of them, sorted by row
+// then column): row pointers, prefix sums and totals, the diagonal, the CSC orientation, cell
+// entropies and degrees.
+template <typename K>
+static void finish_build(Ctx& c, const KeySpace<K>& ks, int nnz) {
     DevBlockmodel& bm = c.bm;
     hipStream_t st = c.stream;
     const int B = bm.B;
-    const bool dn = c.batch.G > 0;  // a batch: keys over the blocks in use
-    const int* dense = dn ? c.batch.d_dense : nullptr;
-    const int* inv = dn ? c.batch.d_inv : nullptr;
-    const int bits = bits_for(dn ? c.batch.used : B);
-    // key = row << bits | column: 2 * bits significant bits in 32- or 64-bit keys, sorted by
-    // 8-bit digits from the lowest
-    const int shift = bits;
-    const K mask = (K)((((K)1) << bits) - 1);
     K* keys0 = (K*)c.keys0;
     K* keys1 = (K*)c.keys1;
-    const int E = g.E;
-
-    // 1. edge keys (block(u), block(v)) in CSR edge order, sorted by (row, col)
-    int shifts[8];
-    int np = 0;
-    int nbytes = (bits + 7) / 8;
-    for (int b = 0; b < 2 * bits; b += 8) shifts[np++] = b;
-    int nnz = 0;
-    if (E > 0) {
-        edge_keys_kernel<K><<<blocks_for(E), TB, 0, st>>>(g.edge_src, g.out_idx, bm.assign, E, B, shift, keys0,
-                                                          c.dev_err, dense);
-        int which = radix_sort<K>(keys0, keys1, nullptr, nullptr, E, shifts, np, c.hist, c.scan_tmp, st);
-        K* sorted = which ? keys1 : keys0;
-        K* ukeys = which ? keys0 : keys1;
-        // 2. run-length encode into unique cells with counts
-        head_flags_kernel<K><<<blocks_for(E), TB, 0, st>>>(sorted, E, c.flags);
-        scan_exclusive(c.flags, c.pos, E, c.scan_tmp, st);
-        SBPX_HIP(hipMemcpyAsync(&nnz, c.pos + E, sizeof(int), hipMemcpyDeviceToHost, st));
-        compact_runs_kernel<K><<<blocks_for(E), TB, 0, st>>>(sorted, c.flags, c.pos, E, ukeys, c.vals0);
-        SBPX_HIP(hipStreamSynchronize(st));
-        decode_csr_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(ukeys, c.vals0, nnz, shift, mask, bm.coo_row,
-                                                             bm.row_col, bm.row_val, inv);
-    }
+    const int nbytes = (ks.bits + 7) / 8;
     bm.nnz = nnz;
     // 3. row pointers, row prefix sums, dout, diagonal
     segment_ptr_kernel<<<blocks_for(B + 1), TB, 0, st>>>(bm.coo_row, nnz, B, bm.row_ptr);
@@ -216,17 +210,17 @@ static void build_impl(Ctx& c) {
     if (nnz > 0) diag_kernel<<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, bm.row_val, nnz, bm.diag);
     // 4. CSC: stable sort of the unique cells by column (input already ordered by row)
     if (nnz > 0) {
-        csc_keys_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, nnz, shift, keys0, dense);
+        csc_keys_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(bm.coo_row, bm.row_col, nnz, ks.shift, keys0, ks.dense);
         SBPX_HIP(hipMemcpyAsync(c.vals0, bm.row_val, sizeof(int) * nnz, hipMemcpyDeviceToDevice, st));
         int cshifts[4];
         int cnp = 0;
-        for (int i = 0; i < nbytes; ++i) cshifts[cnp++] = shift + 8 * i;
+        for (int i = 0; i < nbytes; ++i) cshifts[cnp++] = ks.shift + 8 * i;
         int which = radix_sort<K>(keys0, keys1, c.vals0, c.vals1, nnz, cshifts, cnp, c.hist, c.scan_tmp, st);
         K* ck = which ? keys1 : keys0;
         int* cv = which ? c.vals1 : c.vals0;
         int* col_of = c.flags;  // column index per CSC entry (scratch)
-        decode_csc_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(ck, cv, nnz, shift, mask, col_of, bm.col_row,
-                                                             bm.col_val, inv);
+        decode_csc_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(ck, cv, nnz, ks.shift, ks.mask, col_of, bm.col_row,
+                                                             bm.col_val, ks.inv);
         segment_ptr_kernel<<<blocks_for(B + 1), TB, 0, st>>>(col_of, nnz, B, bm.col_ptr);
         scan_exclusive(bm.col_val, c.pos, nnz, c.scan_tmp, st);
         segment_cum_kernel<<<blocks_for(nnz), TB, 0, st>>>(col_of, bm.col_ptr, c.pos, nnz, bm.col_cum);
@@ -241,10 +235,364 @@ static void build_impl(Ctx& c) {
     bm.valid = true;
 }
 
+template <typename K>
+static void build_impl(Ctx& c) {
+    DevGraph& g = c.g;
+    DevBlockmodel& bm = c.bm;
+    hipStream_t st = c.stream;
+    const KeySpace<K> ks = key_space<K>(c);
+    K* keys0 = (K*)c.keys0;
+    K* keys1 = (K*)c.keys1;
+    const int E = g.E;
+    // 1. edge keys (block(u), block(v)) in CSR edge order, sorted by (row, col), 8-bit digits
+    int shifts[8];
+    int np = 0;
+    for (int b = 0; b < 2 * ks.bits; b += 8) shifts[np++] = b;
+    int nnz = 0;
+    if (E > 0) {
+        edge_keys_kernel<K><<<blocks_for(E), TB, 0, st>>>(g.edge_src, g.out_idx, bm.assign, E, bm.B, ks.shift, keys0,
+                                                          c.dev_err, ks.dense);
+        int which = radix_sort<K>(keys0, keys1, nullptr, nullptr, E, shifts, np, c.hist, c.scan_tmp, st);
+        K* sorted = which ? keys1 : keys0;
+        K* ukeys = which ? keys0 : keys1;
+        // 2. run-length encode into unique cells with counts
+        head_flags_kernel<K><<<blocks_for(E), TB, 0, st>>>(sorted, E, c.flags);
+        scan_exclusive(c.flags, c.pos, E, c.scan_tmp, st);
+        SBPX_HIP(hipMemcpyAsync(&nnz, c.pos + E, sizeof(int), hipMemcpyDeviceToHost, st));
+        compact_runs_kernel<K><<<blocks_for(E), TB, 0, st>>>(sorted, c.flags, c.pos, E, ukeys, c.vals0);
+        SBPX_HIP(hipStreamSynchronize(st));
+        decode_csr_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(ukeys, c.vals0, nnz, ks.shift, ks.mask, bm.coo_row,
+                                                             bm.row_col, bm.row_val, ks.inv);
+    }
+    finish_build<K>(c, ks, nnz);
+}
+
+// ------------------------------------------------------------------------------------------
+// Incremental update after a batch of moves. Only the edges of moved vertices change cells:
+// each contributes (old key, -1) and (new key, +1), an edge between two movers once (as its
+// source's out-edge). The changes are sorted and summed per key, then merged into the previous
+// cells (sorted keys with counts) by ranks: a change to an existing key adds to its count, a new
+// key is placed among the old ones, and cells whose count drops to zero are removed. The cells
+// then pass through the same finishing steps as a full build, so the result is the same arrays.
+
+template <typename K>
+__device__ __forceinline__ K cell_key(int row, int col, int shift, const int* dense) {
+    if (dense != nullptr) {
+        row = dense[row];
+        col = dense[col];
+    }
+    return ((K)row << shift) | (K)col;
+}
+
+// Entries per listed vertex: 2 per incident edge when it moves in this batch, else 0.
+__global__ void delta_count_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ stat,
+                                   const int* __restrict__ out_ptr, const int* __restrict__ in_ptr,
+                                   int* __restrict__ cnt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = list[i];
+    cnt[i] = stat[v] == 2 ? 2 * ((out_ptr[v + 1] - out_ptr[v]) + (in_ptr[v + 1] - in_ptr[v])) : 0;
+}
+
+// One wavefront per listed vertex; reads the assignment before the batch is committed. Unused
+// slots (in-edges from movers and the self-loop's in-list entry) get the largest key, value 0.
+template <typename K>
+__global__ void delta_emit_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ stat,
+                                  const int* __restrict__ off, const int* __restrict__ out_ptr,
+                                  const int* __restrict__ out_idx, const int* __restrict__ in_ptr,
+                                  const int* __restrict__ in_idx, const int* __restrict__ assign,
+                                  const int* __restrict__ new_block, const int* __restrict__ moved_epoch, int epoch,
+                                  int shift, const int* __restrict__ dense, K* __restrict__ dkey,
+                                  int* __restrict__ dval) {
+    const int i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
+    if (i >= n) return;
+    const int v = list[i];
+    if (stat[v] != 2) return;
+    const int lane = lane_id();
+    const int r = assign[v], s = new_block[v];
+    const int base = off[i];
+    const int ob = out_ptr[v], dO = out_ptr[v + 1] - ob;
+    const int ib = in_ptr[v], dI = in_ptr[v + 1] - ib;
+    for (int j = lane; j < dO; j += WAVE) {
+        const int w = out_idx[ob + j];
+        const int bo = assign[w];
+        const int bn = moved_epoch[w] == epoch ? new_block[w] : bo;
+        dkey[base + 2 * j] = cell_key<K>(r, bo, shift, dense);
+        dval[base + 2 * j] = -1;
+        dkey[base + 2 * j + 1] = cell_key<K>(s, bn, shift, dense);
+        dval[base + 2 * j + 1] = 1;
+    }
+    for (int j = lane; j < dI; j += WAVE) {
+        const int u = in_idx[ib + j];
+        const int q = base + 2 * dO + 2 * j;
+        if (u == v || moved_epoch[u] == epoch) {
+            dkey[q] = ~(K)0;
+            dval[q] = 0;
+            dkey[q + 1] = ~(K)0;
+            dval[q + 1] = 0;
+        } else {
+            const int bu = assign[u];
+            dkey[q] = cell_key<K>(bu, r, shift, dense);
+            dval[q] = -1;
+            dkey[q + 1] = cell_key<K>(bu, s, shift, dense);
+            dval[q + 1] = 1;
+        }
+    }
+}
+
+// Sum of the values of each run of equal sorted keys (run j: [start[j], start[j + 1])), kept
+// when nonzero and not the padding key.
+template <typename K>
+__global__ void run_sums_kernel(const K* __restrict__ ukeys, const int* __restrict__ start, const int* __restrict__ vex,
+                                int nruns, int* __restrict__ sum, int* __restrict__ keep) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nruns) return;
+    const int t = vex[start[j + 1]] - vex[start[j]];
+    sum[j] = t;
+    keep[j] = (t != 0 && ukeys[j] != ~(K)0) ? 1 : 0;
+}
+
+template <typename K>
+__global__ void compact_pairs_kernel(const K* __restrict__ k, const int* __restrict__ v, const int* __restrict__ keep,
+                                     const int* __restrict__ pos, int n, K* __restrict__ ko, int* __restrict__ vo) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !keep[j]) return;
+    ko[pos[j]] = k[j];
+    vo[pos[j]] = v[j];
+}
+
+template <typename K>
+__device__ __forceinline__ int lower_bound_key(const K* __restrict__ a, int n, K x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename K>
+__global__ void old_keys_kernel(const int* __restrict__ coo_row, const int* __restrict__ row_col,
+                                const int* __restrict__ row_val, int nnz, int shift, const int* __restrict__ dense,
+                                K* __restrict__ okey, int* __restrict__ ocount) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    okey[i] = cell_key<K>(coo_row[i], row_col[i], shift, dense);
+    ocount[i] = row_val[i];
+}
+
+// A change to an existing cell adds to its count; others are flagged new, with their rank
+// among the old keys.
+template <typename K>
+__global__ void merge_match_kernel(const K* __restrict__ dkey, const int* __restrict__ dsum, int nd,
+                                   const K* __restrict__ okey, int nold, int* __restrict__ ocount,
+                                   int* __restrict__ isnew, int* __restrict__ lb) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nd) return;
+    const int p = lower_bound_key<K>(okey, nold, dkey[j]);
+    lb[j] = p;
+    if (p < nold && okey[p] == dkey[j]) {
+        ocount[p] += dsum[j];  // keys are unique on both sides
+        isnew[j] = 0;
+    } else {
+        isnew[j] = 1;
+    }
+}
+
+template <typename K>
+__global__ void merge_place_old_kernel(const K* __restrict__ okey, const int* __restrict__ ocount, int nold,
+                                       const K* __restrict__ dkey, int nd, const int* __restrict__ rank,
+                                       K* __restrict__ mkey, int* __restrict__ mcount) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nold) return;
+    const int p = i + rank[lower_bound_key<K>(dkey, nd, okey[i])];
+    mkey[p] = okey[i];
+    mcount[p] = ocount[i];
+}
+
+template <typename K>
+__global__ void merge_place_new_kernel(const K* __restrict__ dkey, const int* __restrict__ dsum, int nd,
+                                       const int* __restrict__ isnew, const int* __restrict__ lb,
+                                       const int* __restrict__ rank, K* __restrict__ mkey, int* __restrict__ mcount,
+                                       int* __restrict__ err) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nd || !isnew[j]) return;
+    const int p = rank[j] + lb[j];
+    mkey[p] = dkey[j];
+    mcount[p] = dsum[j];
+    if (dsum[j] < 0) atomicOr(err, DEV_ERR_BLOCK_RANGE);  // removes from a cell that is not there
+}
+
+__global__ void nonzero_flags_kernel(const int* __restrict__ count, int n, int* __restrict__ flag) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = count[i] != 0 ? 1 : 0;
+}
+
+template <typename K>
+__global__ void decode_cells_kernel(const K* __restrict__ keys, const int* __restrict__ count, int nnz, int shift,
+                                    K mask, int* __restrict__ coo_row, int* __restrict__ row_col,
+                                    int* __restrict__ row_val, const int* __restrict__ inv) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nnz) return;
+    K key = keys[j];
+    int row = (int)(key >> shift), col = (int)(key & mask);
+    if (inv != nullptr) {
+        row = inv[row];
+        col = inv[col];
+    }
+    coo_row[j] = row;
+    row_col[j] = col;
+    row_val[j] = count[j];
+}
+
+static void ensure_inc(Ctx& c) {
+    const size_t E = (size_t)c.g.E + 2;
+    const size_t need = std::max(E, (size_t)c.g.N + 2);
+    if (c.inc.cap >= need) return;
+    IncBuffers& b = c.inc;
+    for (void** p : {&b.okey, &b.mkey, &b.dkey}) {
+        if (*p) SBPX_HIP(hipFree(*p));
+        *p = nullptr;
+    }
+    for (int** p : {&b.ocount, &b.mcount, &b.dsum, &b.lb, &b.rank, &b.flag, &b.tmp}) {
+        if (*p) SBPX_HIP(hipFree(*p));
+        *p = nullptr;
+    }
+    // merged list: old cells (<= E) plus new keys (<= a quarter of E, see update_blockmodel)
+    SBPX_HIP(hipMalloc(&b.okey, sizeof(uint64_t) * need));
+    SBPX_HIP(hipMalloc(&b.mkey, sizeof(uint64_t) * 2 * need));
+    SBPX_HIP(hipMalloc(&b.dkey, sizeof(uint64_t) * need));
+    SBPX_HIP(hipMalloc((void**)&b.ocount, sizeof(int) * need));
+    SBPX_HIP(hipMalloc((void**)&b.mcount, sizeof(int) * 2 * need));
+    SBPX_HIP(hipMalloc((void**)&b.dsum, sizeof(int) * need));
+    SBPX_HIP(hipMalloc((void**)&b.lb, sizeof(int) * need));
+    SBPX_HIP(hipMalloc((void**)&b.rank, sizeof(int) * (2 * need + 1)));
+    SBPX_HIP(hipMalloc((void**)&b.flag, sizeof(int) * 2 * need));
+    b.tmp_ints = scan_tmp_ints(2 * (long)need + 1);
+    SBPX_HIP(hipMalloc((void**)&b.tmp, sizeof(int) * b.tmp_ints));
+    b.cap = need;
+}
+
+template <typename K>
+static bool update_impl(Ctx& c, const int* list, int n, int total) {
+    DevBlockmodel& bm = c.bm;
+    hipStream_t st = c.stream;
+    const KeySpace<K> ks = key_space<K>(c);
+    IncBuffers& ib = c.inc;
+    K* keys0 = (K*)c.keys0;
+    K* keys1 = (K*)c.keys1;
+    // sort the changes by key (values follow), all 2 * bits of the key
+    int shifts[8];
+    int np = 0;
+    for (int b = 0; b < 2 * ks.bits; b += 8) shifts[np++] = b;
+    const int which = radix_sort<K>(keys0, keys1, c.vals0, c.vals1, total, shifts, np, c.hist, c.scan_tmp, st);
+    K* sk = which ? keys1 : keys0;
+    int* sv = which ? c.vals1 : c.vals0;
+    K* uk = which ? keys0 : keys1;  // run keys
+    // sum per key: runs of the sorted keys, value prefix sums
+    head_flags_kernel<K><<<blocks_for(total), TB, 0, st>>>(sk, total, c.flags);
+    scan_exclusive(c.flags, c.pos, total, c.scan_tmp, st);
+    int nruns = 0;
+    SBPX_HIP(hipMemcpyAsync(&nruns, c.pos + total, sizeof(int), hipMemcpyDeviceToHost, st));
+    int* start = ib.lb;  // run starts (nruns + 1), scratch until the merge
+    compact_runs_kernel<K><<<blocks_for(total), TB, 0, st>>>(sk, c.flags, c.pos, total, uk, start);
+    scan_exclusive(sv, ib.rank, total, ib.tmp, st);  // value prefix (rank as scratch)
+    SBPX_HIP(hipStreamSynchronize(st));
+    run_sums_kernel<K><<<blocks_for(nruns), TB, 0, st>>>(uk, start, ib.rank, nruns, ib.mcount, ib.flag);
+    scan_exclusive(ib.flag, c.pos, nruns, c.scan_tmp, st);
+    int nd = 0;
+    SBPX_HIP(hipMemcpyAsync(&nd, c.pos + nruns, sizeof(int), hipMemcpyDeviceToHost, st));
+    K* dkey = (K*)ib.dkey;
+    compact_pairs_kernel<K><<<blocks_for(nruns), TB, 0, st>>>(uk, ib.mcount, ib.flag, c.pos, nruns, dkey, ib.dsum);
+    // the previous cells as sorted keys with counts
+    const int nold = bm.nnz;
+    K* okey = (K*)ib.okey;
+    if (nold > 0)
+        old_keys_kernel<K><<<blocks_for(nold), TB, 0, st>>>(bm.coo_row, bm.row_col, bm.row_val, nold, ks.shift,
+                                                            ks.dense, okey, ib.ocount);
+    SBPX_HIP(hipStreamSynchronize(st));
+    // merge by ranks
+    int nnew = 0;
+    if (nd > 0) {
+        merge_match_kernel<K><<<blocks_for(nd), TB, 0, st>>>(dkey, ib.dsum, nd, okey, nold, ib.ocount, ib.flag, ib.lb);
+        scan_exclusive(ib.flag, ib.rank, nd, ib.tmp, st);
+        SBPX_HIP(hipMemcpyAsync(&nnew, ib.rank + nd, sizeof(int), hipMemcpyDeviceToHost, st));
+    } else {
+        SBPX_HIP(hipMemsetAsync(ib.rank, 0, sizeof(int), st));
+    }
+    K* mkey = (K*)ib.mkey;
+    if (nold > 0)
+        merge_place_old_kernel<K><<<blocks_for(nold), TB, 0, st>>>(okey, ib.ocount, nold, dkey, nd, ib.rank, mkey,
+                                                                   ib.mcount);
+    SBPX_HIP(hipStreamSynchronize(st));
+    if (nnew > 0)
+        merge_place_new_kernel<K><<<blocks_for(nd), TB, 0, st>>>(dkey, ib.dsum, nd, ib.flag, ib.lb, ib.rank, mkey,
+                                                                 ib.mcount, c.dev_err);
+    // drop the cells that reached zero
+    const int nm = nold + nnew;
+    int nnz = 0;
+    if (nm > 0) {
+        nonzero_flags_kernel<<<blocks_for(nm), TB, 0, st>>>(ib.mcount, nm, ib.flag);
+        scan_exclusive(ib.flag, ib.rank, nm, ib.tmp, st);
+        SBPX_HIP(hipMemcpyAsync(&nnz, ib.rank + nm, sizeof(int), hipMemcpyDeviceToHost, st));
+        compact_pairs_kernel<K><<<blocks_for(nm), TB, 0, st>>>(mkey, ib.mcount, ib.flag, ib.rank, nm, okey, ib.ocount);
+        SBPX_HIP(hipStreamSynchronize(st));
+        decode_cells_kernel<K><<<blocks_for(nnz), TB, 0, st>>>(okey, ib.ocount, nnz, ks.shift, ks.mask, bm.coo_row,
+                                                               bm.row_col, bm.row_val, ks.inv);
+    }
+    finish_build<K>(c, ks, nnz);
+    return true;
+}
+
 void rebuild_keep_drift(Ctx& c) {
     const int bits = bits_for(c.batch.G > 0 ? c.batch.used : c.bm.B);
     if (2 * bits <= 32) build_impl<uint32_t>(c);
     else build_impl<uint64_t>(c);
+}
+
+// Called before the batch's moves are committed to the assignment (moved_epoch / new_block of
+// this epoch mark them). Emits the changes of the moved vertices' edges when they are at most a
+// quarter of the edges' worth (incremental update after the commit), else returns -1 (full
+// rebuild). Returns the number of change entries.
+template <typename K>
+static int emit_delta(Ctx& c, const int* list, int n) {
+    ensure_inc(c);
+    hipStream_t st = c.stream;
+    const KeySpace<K> ks = key_space<K>(c);
+    int* cnt = c.inc.flag;
+    delta_count_kernel<<<blocks_for(n), TB, 0, st>>>(list, n, c.ev_acc, c.g.out_ptr, c.g.in_ptr, cnt);
+    scan_exclusive(cnt, c.inc.rank, n, c.inc.tmp, st);
+    int total = 0;
+    SBPX_HIP(hipMemcpyAsync(&total, c.inc.rank + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    SBPX_HIP(hipStreamSynchronize(st));
+    if (total > c.g.E / 2) return -1;  // more than a quarter of the edges changed
+    if (total == 0) return 0;
+    const int wpb = TB / WAVE;
+    delta_emit_kernel<K><<<(n + wpb - 1) / wpb, TB, 0, st>>>(list, n, c.ev_acc, c.inc.rank, c.g.out_ptr, c.g.out_idx,
+                                                            c.g.in_ptr, c.g.in_idx, c.bm.assign, c.new_block,
+                                                            c.moved_epoch, c.epoch, ks.shift, ks.dense,
+                                                            (K*)c.keys0, c.vals0);
+    SBPX_HIP(hipGetLastError());
+    return total;
+}
+
+int prepare_update(Ctx& c, const int* list, int n) {
+    if (!c.bm.valid || c.g.E <= 0 || c.incremental == 0) return -1;
+    const int bits = bits_for(c.batch.G > 0 ? c.batch.used : c.bm.B);
+    return 2 * bits <= 32 ? emit_delta<uint32_t>(c, list, n) : emit_delta<uint64_t>(c, list, n);
+}
+
+void finish_update(Ctx& c, const int* list, int n, int total) {
+    if (total == 0) {  // nothing moved: the cells stand; degrees take the new drift
+        degrees_kernel<<<blocks_for(c.bm.B), TB, 0, c.stream>>>(c.bm.dout, c.bm.din, c.bm.diag, c.bm.drift, c.bm.B,
+                                                                c.bm.deg);
+        SBPX_HIP(hipGetLastError());
+        return;
+    }
+    const int bits = bits_for(c.batch.G > 0 ? c.batch.used : c.bm.B);
+    if (2 * bits <= 32) update_impl<uint32_t>(c, list, n, total);
+    else update_impl<uint64_t>(c, list, n, total);
 }
 
 void build_blockmodel(Ctx& c) {

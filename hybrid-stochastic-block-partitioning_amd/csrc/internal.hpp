@@ -168,6 +168,14 @@ struct DevBatch {
     size_t plan_slots = 0, plan_huge_slots = 0;
 };
 
+// Buffers of the incremental block-matrix update (allocated on first use).
+struct IncBuffers {
+    void *okey = nullptr, *mkey = nullptr, *dkey = nullptr;
+    int *ocount = nullptr, *mcount = nullptr, *dsum = nullptr, *lb = nullptr, *rank = nullptr, *flag = nullptr,
+        *tmp = nullptr;
+    size_t cap = 0, tmp_ints = 0;
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -178,6 +186,8 @@ struct Ctx {
     MhState mh;
     MlpState mlp;
     DevBatch batch;
+    IncBuffers inc;
+    int incremental = 1;  // 1: update the block matrix from the moved vertices' edges when few changed
     int proposal = 0;  // 0 plain, 1 learned mixture for the batched low-degree vertices
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
 
@@ -319,6 +329,10 @@ int radix_sort(K* keys, K* keys_alt, int* vals, int* vals_alt, long n, const int
 // blockmodel.hip
 void build_blockmodel(Ctx& c);               // from c.bm.assign, resets drift
 void rebuild_keep_drift(Ctx& c);             // from c.bm.assign, keeps drift
+// Incremental update after a batch: prepare before the moves are committed (returns the change
+// count, or -1 for a full rebuild), finish after.
+int prepare_update(Ctx& c, const int* list, int n);
+void finish_update(Ctx& c, const int* list, int n, int total);
 double log_posterior(Ctx& c);                // sum of M * logf(M / (dout din)) in FP64
 // Per-graph log posteriors of a union (out[G]); a one-graph context gives out[0]. Each graph's
 // cells are summed in fixed chunks from its first cell, so the value does not depend on the

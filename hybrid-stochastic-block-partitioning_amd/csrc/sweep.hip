@@ -356,9 +356,11 @@ void apply_accepted(Ctx& c, const int* list, int n) {
     drift_kernel<<<(n + wpb - 1) / wpb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.g.out_ptr, c.g.out_idx, c.g.in_ptr,
                                                                 c.g.in_idx, c.bm.assign, c.epoch, c.moved_epoch,
                                                                 c.new_block, c.last_touch, c.bm.drift);
+    const int total = prepare_update(c, list, n);  // reads the assignment before the commit
     commit_moves_kernel<<<nb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.new_block, c.bm.assign);
     SBPX_HIP(hipGetLastError());
-    rebuild_keep_drift(c);
+    if (total >= 0) finish_update(c, list, n, total);
+    else rebuild_keep_drift(c);
 }
 
 }  // namespace sbpx

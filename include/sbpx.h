@@ -71,7 +71,10 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            vertex's sums over other lanes add them in another order, so delta MDL may differ
  *            in its last bits between them.
  * option 4 = proposal of the batched evaluation: 0 the reference's, 1 the learned mixture
- *            (see sbpx_learned_init). */
+ *            (see sbpx_learned_init).
+ * option 5 = block matrix after a batch of moves: 1 (default) updated from the moved vertices'
+ *            edges when they are at most a quarter of the edges, else rebuilt by a full sort;
+ *            0 always rebuilt. Both give the same arrays. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

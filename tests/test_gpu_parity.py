@@ -216,6 +216,37 @@ def test_async_gibbs_sweeps_match_oracle(batches, variant):
             assert c.mdl() == pytest.approx(bm.mdl(), rel=MDL_RTOL)
 
 
+@pytest.mark.parametrize("batches", [1, 4])
+def test_incremental_update_equals_full_rebuild(batches):
+    """The block matrix updated from the moved vertices' edges after each batch equals the one a
+    full sort rebuilds from the assignment: every CSR/CSC array, the degrees and the MDL, through
+    sweeps from many moves (full rebuilds) down to few (updates), and across a merge."""
+    m = sbpx()
+    n = 3000
+    arrays, truth = gen_arrays(n, 12, p_in=0.7, alpha=1.0, seed=6)
+    cs = []
+    for inc in (0, 1):
+        c = ctx_for(arrays)
+        c.set_option(m.OPT_INCREMENTAL, inc)
+        c.build(n, np.arange(n, dtype=np.int32))
+        c.merge_blocks(n // 2, 4, 0)
+        cs.append(c)
+    for call in range(12):
+        if call == 6:
+            for c in cs:
+                c.merge_blocks(c.info()[0] // 2, 4, 1)
+        st = [c.sweep(m.ALGO_ASYNC_GIBBS, batches, 17, call) for c in cs]
+        assert (st[0].moves, st[0].evals) == (st[1].moves, st[1].evals)
+        s0, s1 = cs[0].state(), cs[1].state()
+        for a, b in zip(s0, s1):
+            assert (np.asarray(a) == np.asarray(b)).all()
+        for by_column in (0, 1):
+            for a, b in zip(cs[0].coo(by_column), cs[1].coo(by_column)):
+                assert (a == b).all()
+        assert cs[0].info() == cs[1].info()
+        assert cs[0].mdl() == cs[1].mdl()
+
+
 # ----------------------------------------------------------------------------- merges
 def test_merge_proposals_match_oracle():
     rng = np.random.default_rng(5)

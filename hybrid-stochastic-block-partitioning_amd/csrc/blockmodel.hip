@@ -295,14 +295,16 @@ __global__ void delta_count_kernel(const int* __restrict__ list, int n, const ui
 }
 
 // One wavefront per listed vertex; reads the assignment before the batch is committed. Unused
-// slots (in-edges from movers and the self-loop's in-list entry) get the largest key, value 0.
+// slots (in-edges from movers and the self-loop's in-list entry) get value 0 and the largest key
+// of the sorted bit range, `pad` (the key of cell (2^bits - 1, 2^bits - 1)): the sort orders only
+// those bits, so a key outside them would interleave with the cell whose low bits it shares.
 template <typename K>
 __global__ void delta_emit_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ stat,
                                   const int* __restrict__ off, const int* __restrict__ out_ptr,
                                   const int* __restrict__ out_idx, const int* __restrict__ in_ptr,
                                   const int* __restrict__ in_idx, const int* __restrict__ assign,
                                   const int* __restrict__ new_block, const int* __restrict__ moved_epoch, int epoch,
-                                  int shift, const int* __restrict__ dense, K* __restrict__ dkey,
+                                  int shift, const int* __restrict__ dense, K pad, K* __restrict__ dkey,
                                   int* __restrict__ dval) {
     const int i = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
     if (i >= n) return;
@@ -326,9 +328,9 @@ __global__ void delta_emit_kernel(const int* __restrict__ list, int n, const uin
         const int u = in_idx[ib + j];
         const int q = base + 2 * dO + 2 * j;
         if (u == v || moved_epoch[u] == epoch) {
-            dkey[q] = ~(K)0;
+            dkey[q] = pad;
             dval[q] = 0;
-            dkey[q + 1] = ~(K)0;
+            dkey[q + 1] = pad;
             dval[q + 1] = 0;
         } else {
             const int bu = assign[u];
@@ -341,15 +343,15 @@ __global__ void delta_emit_kernel(const int* __restrict__ list, int n, const uin
 }
 
 // Sum of the values of each run of equal sorted keys (run j: [start[j], start[j + 1])), kept
-// when nonzero and not the padding key.
-template <typename K>
-__global__ void run_sums_kernel(const K* __restrict__ ukeys, const int* __restrict__ start, const int* __restrict__ vex,
-                                int nruns, int* __restrict__ sum, int* __restrict__ keep) {
+// when nonzero. Padding entries carry 0, so a run of padding alone is dropped, and one that
+// shares its key with real changes (a full key space) sums those alone.
+__global__ void run_sums_kernel(const int* __restrict__ start, const int* __restrict__ vex, int nruns,
+                                int* __restrict__ sum, int* __restrict__ keep) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nruns) return;
     const int t = vex[start[j + 1]] - vex[start[j]];
     sum[j] = t;
-    keep[j] = (t != 0 && ukeys[j] != ~(K)0) ? 1 : 0;
+    keep[j] = t != 0 ? 1 : 0;
 }
 
 template <typename K>
@@ -499,7 +501,7 @@ static bool update_impl(Ctx& c, const int* list, int n, int total) {
     compact_runs_kernel<K><<<blocks_for(total), TB, 0, st>>>(sk, c.flags, c.pos, total, uk, start);
     scan_exclusive(sv, ib.rank, total, ib.tmp, st);  // value prefix (rank as scratch)
     SBPX_HIP(hipStreamSynchronize(st));
-    run_sums_kernel<K><<<blocks_for(nruns), TB, 0, st>>>(uk, start, ib.rank, nruns, ib.mcount, ib.flag);
+    run_sums_kernel<<<blocks_for(nruns), TB, 0, st>>>(start, ib.rank, nruns, ib.mcount, ib.flag);
     scan_exclusive(ib.flag, c.pos, nruns, c.scan_tmp, st);
     int nd = 0;
     SBPX_HIP(hipMemcpyAsync(&nd, c.pos + nruns, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -572,7 +574,7 @@ static int emit_delta(Ctx& c, const int* list, int n) {
     delta_emit_kernel<K><<<(n + wpb - 1) / wpb, TB, 0, st>>>(list, n, c.ev_acc, c.inc.rank, c.g.out_ptr, c.g.out_idx,
                                                             c.g.in_ptr, c.g.in_idx, c.bm.assign, c.new_block,
                                                             c.moved_epoch, c.epoch, ks.shift, ks.dense,
-                                                            (K*)c.keys0, c.vals0);
+                                                            (K)((((K)1) << (2 * ks.bits)) - 1), (K*)c.keys0, c.vals0);
     SBPX_HIP(hipGetLastError());
     return total;
 }

@@ -220,7 +220,9 @@ def test_async_gibbs_sweeps_match_oracle(batches, variant):
 def test_incremental_update_equals_full_rebuild(batches):
     """The block matrix updated from the moved vertices' edges after each batch equals the one a
     full sort rebuilds from the assignment: every CSR/CSC array, the degrees and the MDL, through
-    sweeps from many moves (full rebuilds) down to few (updates), and across a merge."""
+    sweeps from many moves (full rebuilds) down to few (updates), across merges, at block counts
+    that are powers of two (the largest key then fills the sorted bit range: the padding of the
+    change list shares it) and at one that is not."""
     m = sbpx()
     n = 3000
     arrays, truth = gen_arrays(n, 12, p_in=0.7, alpha=1.0, seed=6)
@@ -229,12 +231,13 @@ def test_incremental_update_equals_full_rebuild(batches):
         c = ctx_for(arrays)
         c.set_option(m.OPT_INCREMENTAL, inc)
         c.build(n, np.arange(n, dtype=np.int32))
-        c.merge_blocks(n // 2, 4, 0)
+        c.merge_blocks(n - 1024, 4, 0)
         cs.append(c)
-    for call in range(12):
-        if call == 6:
+    for call in range(18):
+        if call in (6, 12):
             for c in cs:
-                c.merge_blocks(c.info()[0] // 2, 4, 1)
+                B = c.info()[0]
+                c.merge_blocks(B - (16 if call == 6 else 13), 4, call)
         st = [c.sweep(m.ALGO_ASYNC_GIBBS, batches, 17, call) for c in cs]
         assert (st[0].moves, st[0].evals) == (st[1].moves, st[1].evals)
         s0, s1 = cs[0].state(), cs[1].state()

@@ -112,8 +112,10 @@ void free_graph_buffers(Ctx& c) {
     c.table_pool_slots = 0;
     MhState& h = c.mh;
     for (int** p : {&h.rbase, &h.cbase, &h.rkeys, &h.rvals, &h.ckeys, &h.cvals, &h.rocc, &h.cocc, &h.rcap,
-                    &h.ccap, &h.tables, &h.status, &h.touch, &h.d_all, &h.d_high})
+                    &h.ccap, &h.tables, &h.status, &h.touch, &h.d_all, &h.d_high, &h.ctrl})
         dfree(p);
+    dfree(&h.wstamp);
+    if (h.wide) (void)hipFree(h.wide);
     h = MhState();
     c.plan = SweepPlan();
     c.bm = DevBlockmodel();
@@ -497,6 +499,10 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
                 throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");
             ctx->c.mh_group = (int)value;
+        } else if (option == 6) {
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
+                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield), 1, 2, 4, 8 or 16");
+            ctx->c.mh_wide = (int)value;
         } else {
             throw ArgError("unknown option");
         }

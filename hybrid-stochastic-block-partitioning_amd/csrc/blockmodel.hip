@@ -750,6 +750,7 @@ void check_device_error(Ctx& c) {
         if (e & DEV_ERR_NAN) throw NumericError("NaN or Inf in a delta MDL");
         if (e & DEV_ERR_TABLE) throw StateError("neighbour-block table overflow");
         if (e & DEV_ERR_NO_PROPOSAL) throw StateError("block without a merge proposal");
+        if (e & DEV_ERR_BARRIER) throw StateError("a workgroup of the Metropolis-Hastings pass missed a barrier");
         throw StateError("device error");
     }
 }

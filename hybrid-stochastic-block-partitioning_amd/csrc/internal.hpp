@@ -41,6 +41,7 @@ enum : int {
     DEV_ERR_TABLE = 4,         // neighbour-block table overflow
     DEV_ERR_BLOCK_RANGE = 8,   // block id out of range
     DEV_ERR_NO_PROPOSAL = 16,  // a merge candidate without a proposed block
+    DEV_ERR_BARRIER = 32,      // a grid barrier of a multi-workgroup kernel was not reached in time
 };
 
 // Degree bins for the vertex-move kernels: neighbour-block table capacity per bin.
@@ -125,6 +126,10 @@ struct MhState {
     double mean_scan = -1.0;  // mean row/column slots scanned per evaluation in the last chunk
     int last_B = -1;          // number of blocks when mean_scan was measured
     int g_yield = 1;  // wavefronts per vertex the last chunk's window yield calls for
+    int k_yield = 8;  // workgroups of the multi-workgroup pass the last chunk's yield calls for
+    unsigned long long* wstamp = nullptr;  // [N] accepted-entry stamps of the multi-workgroup pass
+    void* wide = nullptr;                  // its per-entry records
+    int* ctrl = nullptr;                   // its barrier counter and window control words
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
     int max_deg_high = 0;
@@ -190,6 +195,7 @@ struct Ctx {
     int incremental = 1;  // 1: update the block matrix from the moved vertices' edges when few changed
     int proposal = 0;  // 0 plain, 1 learned mixture for the batched low-degree vertices
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
+    int mh_wide = 0;   // workgroups of the Metropolis-Hastings pass; 0 = by window yield
 
     // scratch
     void* keys0 = nullptr;

@@ -12,7 +12,8 @@
 // room: a row that is too full moves to a larger segment taken from the free end of the pool
 // (the other wavefronts wait at the barrier meanwhile). Only when the pool itself runs out does
 // the kernel stop; the host then rebuilds the tables with a larger pool and the pass resumes at
-// the same vertex.
+// the same vertex. When windows keep most of their entries, mh_wide_kernel spreads a window of
+// 16 K entries over K workgroups with the same outcome.
 #include <algorithm>
 #include <numeric>
 #include <cstdint>
@@ -73,6 +74,8 @@ __global__ void hash_fill_kernel(const int* __restrict__ ptr, const int* __restr
 }
 
 // ------------------------------------------------------------------------------------------
+struct WideSlot;
+
 struct MhParams {
     GraphView G;
     // mutable state, read and written through the same (non-restrict) pointers
@@ -111,6 +114,11 @@ struct MhParams {
     int stop_at;          // queue position where this launch ends
     double* dS_sum;
     int* err;
+    // the pass over several workgroups (mh_wide_kernel)
+    WideSlot* wide;                 // [16 K] one record per window entry
+    unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
+    int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
+                                    // [2] next queue position, [3] stop reason
 };
 
 __device__ __forceinline__ int room(const int* capa, const int* occ, int x) { return (capa[x] * 3) / 4 - occ[x]; }
@@ -349,24 +357,29 @@ __device__ __forceinline__ long long eval_bytes(const MhParams& P, const MoveRes
 // the blocks keyed in rows and columns r and s. It holds when none of these was touched; rows
 // r, s and t were then neither changed nor moved, so a new evaluation would read the same
 // values in the same order.
+// Check q (1-13) of that test, the neighbour check (0) aside: whether the evaluation of (r, s, t)
+// with neighbour-block table tab/cap read something the commit cr -> cs (table ctab/ccap) changes.
+__device__ __forceinline__ bool reads_changed(const HashView& M, int q, int r, int s, int t, const int* tab, int cap,
+                                              const int* ctab, int ccap, int cr, int cs) {
+    if (q <= 3) {  // r, s or t is one of the blocks whose cells changed
+        const int y = q == 1 ? r : (q == 2 ? s : t);
+        return y >= 0 && (y == cr || y == cs || tab_find(ctab, ccap, y) >= 0);
+    }
+    if (q <= 11) {  // cr or cs keyed in rows / columns r and s
+        const int k = q - 4;
+        const int x = (k & 1) ? cs : cr;
+        const int row = (k & 2) ? s : r;
+        return ((k & 4) ? M.clookup(row, x) : M.rlookup(row, x)) != 0;
+    }
+    return tab_find(tab, cap, q == 12 ? cr : cs) >= 0;  // cr or cs a neighbour block
+}
+
 __device__ bool still_holds(const MhParams& P, const HashView& M, const MoveResult& R, const int* tab, int cap,
                             const int* ctab, int ccap, int cr, int cs, int serial) {
     const int lane = lane_id();
     int bad = 0;
-    if (lane == 0) {
-        bad = P.touch[R.v] == serial;  // a neighbour moved
-    } else if (lane <= 3) {            // r, s or t is one of the blocks whose cells changed
-        const int y = lane == 1 ? R.r : (lane == 2 ? R.s : R.t);
-        if (y >= 0) bad = y == cr || y == cs || tab_find(ctab, ccap, y) >= 0;
-    } else if (lane <= 11) {  // cr or cs keyed in rows / columns r and s
-        const int q = lane - 4;
-        const int x = (q & 1) ? cs : cr;
-        const int row = (q & 2) ? R.s : R.r;
-        bad = ((q & 4) ? M.clookup(row, x) : M.rlookup(row, x)) != 0;
-    } else if (lane <= 13) {  // cr or cs a neighbour block
-        const int x = lane == 12 ? cr : cs;
-        bad = tab_find(tab, cap, x) >= 0;
-    }
+    if (lane == 0) bad = P.touch[R.v] == serial;  // a neighbour moved
+    else if (lane <= 13) bad = reads_changed(M, lane, R.r, R.s, R.t, tab, cap, ctab, ccap, cr, cs);
     return wave_or(bad) == 0;
 }
 
@@ -639,6 +652,287 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
         P.status[0] = pos;
         P.status[1] = stopped;
         P.status[4] = serial;
+        atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
+        atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
+        atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)mvs);
+        P.dS_sum[0] += dsum;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// The same pass over several compute units. K workgroups of 16 wavefronts evaluate a window of
+// 16 K queue entries against the current state, one entry per wavefront, and the window ends as
+// the one-workgroup walk ends it: an entry's evaluation stands unless an accepted entry before it
+// commits a change to something it read. Every accepted entry before the first entry that fails
+// this check holds and is committed, so that entry ends the window whatever the later entries'
+// outcomes. Each wavefront checks its entry against every accepted entry before it (the checks of
+// still_holds, on the state before the window: a row a commit extends is one the check already
+// names), the first failing entry is an atomic minimum, and workgroup 0 commits the accepted
+// entries before it in queue order with commit_shared, exactly as the one-workgroup kernels do.
+// Three grid barriers per window: a counter in device memory with device-scope release/acquire
+// fences (the L2 of each XCD is written back and invalidated) and a bounded wait; a workgroup
+// whose wait runs out reports DEV_ERR_BARRIER and leaves, and so does every other one at its
+// next barrier.
+constexpr int MH_WIDE_MAX = 16;  // workgroups
+constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WAVES;
+constexpr int BARRIER_SPINS = 1 << 22;
+
+struct WideSlot {
+    int st;  // the evaluation's status, 3 = idle
+    int r, s, nuniq, cap;
+    int bad;  // its evaluation read something an accepted entry before it changes
+    CommitInfo ci;
+    double dS;
+    long long bytes, nnz_rs;
+};
+
+// The n-th barrier of the launch (n >= 1) completes when ctrl[0] reaches n * gridDim.x.
+__device__ bool grid_barrier(int* ctrl, int n, int* s_ok) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(&ctrl[0], 1);
+        const int target = n * (int)gridDim.x;
+        int ok = 1;
+        for (int spin = 0; __hip_atomic_load(&ctrl[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++spin) {
+            if (spin >= BARRIER_SPINS) {
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(8);
+        }
+        __threadfence();
+        *s_ok = ok;
+    }
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+__global__ void wide_init_kernel(int* ctrl) {
+    ctrl[0] = 0;
+    ctrl[1] = INT32_MAX;
+    ctrl[2] = 0;
+    ctrl[3] = 0;
+}
+
+__global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
+    __shared__ int s_ok, s_flag, s_n, s_end, s_stop;
+    __shared__ CommitInfo s_commit;
+    __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
+    __shared__ int lds_hist[MH_WAVES][256];
+    __shared__ int s_acc[MH_WIDE_SLOTS], s_acc_r[MH_WIDE_SLOTS], s_acc_s[MH_WIDE_SLOTS], s_acc_cap[MH_WIDE_SLOTS];
+    __shared__ double s_dS[MH_WIDE_SLOTS];
+    __shared__ long long s_red[MH_WAVES][4];
+    __shared__ double s_redd[MH_WAVES][2];
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    const int slot = (int)blockIdx.x * MH_WAVES + wave;
+    const int WS = (int)gridDim.x * MH_WAVES;  // entries per window
+    int* const gtab = P.tables + (size_t)slot * 7 * P.tcap;
+    const GraphView G = P.G;
+    const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
+    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[wave]};
+    const bool lead = blockIdx.x == 0;
+    long long att = 0, evs = 0, mvs = 0, bytes_sum = 0;
+    double dsum = 0.0, scan_sum = 0.0, scan_count = 0.0, windows = 0.0;
+    int pos = P.start;
+    int stopped = 0;
+    int serial = P.status[4];
+    unsigned wserial = (unsigned)P.status[9];
+    int nbar = 0;
+    if (threadIdx.x == 0) s_ok = 1;
+    __syncthreads();
+    while (pos < P.stop_at && stopped == 0) {
+        ++wserial;
+        windows += 1.0;
+        const int Wn = min(WS, P.n - pos);
+        // 1. evaluate this wavefront's entry
+        const bool active = slot < Wn;
+        MoveResult R{};
+        int cap = 16;
+        int* tab = gtab;
+        if (active) {
+            const int v = P.list[pos + slot];
+            cap = vertex_table_cap(G, v);
+            if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
+            R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+            if (R.status == 2 && tab != gtab) {  // the commit and the other checks read it from memory
+                for (int j = lane; j < 7 * cap; j += WAVE) gtab[j] = tab[j];
+            }
+        }
+        if (lane == 0) {
+            WideSlot& o = P.wide[slot];
+            o.st = active ? R.status : 3;
+            o.r = R.r;
+            o.s = R.s;
+            o.nuniq = R.nuniq;
+            o.cap = cap;
+            o.bad = 0;
+            o.ci = commit_info(R);
+            o.dS = R.dS;
+            o.bytes = (active && P.bytes != nullptr) ? eval_bytes(P, R) : 0;
+            o.nnz_rs = R.nnz_rs;
+            if (active && R.status == 2) P.wstamp[R.v] = ((unsigned long long)wserial << 32) | (unsigned)slot;
+        }
+        if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        // 2. the accepted entries of the window, in queue order
+        if (wave == 0) {
+            int cnt = 0;
+            for (int b0 = 0; b0 < Wn; b0 += WAVE) {
+                const int j = b0 + lane;
+                const bool acc = j < Wn && P.wide[j].st == 2;
+                const unsigned long long m = __ballot(acc);
+                if (acc) {
+                    const int k = cnt + __popcll(m & ((1ull << lane) - 1));
+                    s_acc[k] = j;
+                    s_acc_r[k] = P.wide[j].r;
+                    s_acc_s[k] = P.wide[j].s;
+                    s_acc_cap[k] = P.wide[j].cap;
+                    s_dS[k] = P.wide[j].dS;
+                }
+                cnt += __popcll(m);
+            }
+            if (lane == 0) s_n = cnt;
+        }
+        __syncthreads();
+        // 3. whether this entry's evaluation holds after the accepted entries before it
+        if (active) {
+            int nb = 0, hi = s_n;  // accepted entries before this one
+            while (nb < hi) {
+                const int mid = (nb + hi) >> 1;
+                if (s_acc[mid] < slot) nb = mid + 1;
+                else hi = mid;
+            }
+            int bad = 0;
+            if (nb > 0) {
+                const int v = R.v;
+                for (int j = G.out_ptr[v] + lane; j < G.out_ptr[v + 1] && !bad; j += WAVE) {
+                    const unsigned long long w = P.wstamp[G.out_idx[j]];
+                    bad = (unsigned)(w >> 32) == wserial && (int)(unsigned)w < slot;
+                }
+                for (int j = G.in_ptr[v] + lane; j < G.in_ptr[v + 1] && !bad; j += WAVE) {
+                    const unsigned long long w = P.wstamp[G.in_idx[j]];
+                    bad = (unsigned)(w >> 32) == wserial && (int)(unsigned)w < slot;
+                }
+                bad = wave_or(bad);
+                // four accepted entries per trip, checks 1-13 on lanes 1-13 of each quarter
+                for (int k0 = 0; k0 < nb && !bad; k0 += 4) {
+                    const int k = k0 + lane / 16, q = lane & 15;
+                    int b = 0;
+                    if (k < nb && q >= 1 && q <= 13) {
+                        const int* ctab = P.tables + (size_t)s_acc[k] * 7 * P.tcap;
+                        b = reads_changed(M, q, R.r, R.s, R.t, tab, cap, ctab, s_acc_cap[k], s_acc_r[k], s_acc_s[k]);
+                    }
+                    bad = wave_or(b);
+                }
+            }
+            if (lane == 0 && (bad || R.status < 0)) {
+                P.wide[slot].bad = bad;
+                atomicMin(&P.ctrl[1], slot);
+            }
+        }
+        if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        // 4. workgroup 0 commits the accepted entries before the first that does not hold
+        if (lead) {
+            if (threadIdx.x == 0) {
+                int end = min(__hip_atomic_load(&P.ctrl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), Wn);
+                int stop = 0;
+                if (end < Wn && P.wide[end].st < 0 && P.wide[end].bad == 0) {
+                    atomicOr(P.err, DEV_ERR_TABLE);
+                    stop = 2;
+                }
+                s_end = end;
+                s_stop = stop;
+            }
+            __syncthreads();
+            for (int k = 0; k < s_n && s_acc[k] < s_end; ++k) {
+                const int j = s_acc[k];
+                int* const ctab = P.tables + (size_t)j * 7 * P.tcap;
+                if (wave == 0) {
+                    MoveResult Rj{};
+                    Rj.r = s_acc_r[k];
+                    Rj.s = s_acc_s[k];
+                    Rj.nuniq = P.wide[j].nuniq;
+                    const bool ok = make_room(P, Rj, ctab, s_acc_cap[k]);
+                    if (lane == 0) {
+                        s_flag = ok ? 1 : 2;
+                        s_commit = P.wide[j].ci;
+                    }
+                }
+                __syncthreads();
+                if (s_flag == 2) {  // the pool is full: the host rebuilds it and resumes here
+                    if (threadIdx.x == 0) {
+                        s_end = j;
+                        s_stop = 1;
+                    }
+                    __syncthreads();
+                    break;
+                }
+                commit_shared(P, s_commit, ctab, s_acc_cap[k], serial + 1);
+                __syncthreads();
+                ++serial;
+                if (threadIdx.x == 0) {
+                    mvs++;
+                    dsum += s_dS[k];
+                }
+            }
+            // counts over the entries that stand
+            const int end = s_end;
+            long long a = 0, e = 0, by = 0;
+            double sc = 0.0, scn = 0.0;
+            for (int i = threadIdx.x; i < end; i += MH_WAVES * WAVE) {
+                const WideSlot& o = P.wide[i];
+                if (o.st == 3) continue;
+                a += 1;
+                e += o.st >= 1;
+                by += o.bytes;
+                if (o.st >= 1) {
+                    sc += (double)o.nnz_rs;
+                    scn += 1.0;
+                }
+            }
+            a = wave_sum_ll(a);
+            e = wave_sum_ll(e);
+            by = wave_sum_ll(by);
+            sc = wave_sum(sc);
+            scn = wave_sum(scn);
+            if (lane == 0) {
+                s_red[wave][0] = a;
+                s_red[wave][1] = e;
+                s_red[wave][2] = by;
+                s_redd[wave][0] = sc;
+                s_redd[wave][1] = scn;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                for (int w = 0; w < MH_WAVES; ++w) {
+                    att += s_red[w][0];
+                    evs += s_red[w][1];
+                    bytes_sum += s_red[w][2];
+                    scan_sum += s_redd[w][0];
+                    scan_count += s_redd[w][1];
+                }
+                P.ctrl[1] = INT32_MAX;
+                P.ctrl[2] = pos + end;
+                P.ctrl[3] = s_stop;
+            }
+        }
+        if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        pos = __hip_atomic_load(&P.ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        stopped = __hip_atomic_load(&P.ctrl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!s_ok) {
+        if (threadIdx.x == 0) atomicOr(P.err, DEV_ERR_BARRIER);
+        return;
+    }
+    if (lead && threadIdx.x == 0) {
+        if (P.bytes != nullptr && bytes_sum > 0) atomicAdd((unsigned long long*)P.bytes, (unsigned long long)bytes_sum);
+        atomicAdd(&P.scan[0], scan_sum);
+        atomicAdd(&P.scan[1], scan_count);
+        atomicAdd(&P.scan[2], windows);
+        P.status[0] = pos;
+        P.status[1] = stopped;
+        P.status[4] = serial;
+        P.status[9] = (int)wserial;
         atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
         atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
         atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)mvs);
@@ -999,7 +1293,15 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     // max_deg counts a self-loop once; vertex_table_cap counts it in both lists
     int tcap = 16;
     while (tcap < 2 * (max_deg + 3)) tcap <<= 1;
-    const size_t tslots = (size_t)MH_WAVES * 7 * tcap;
+    const bool wide = c.mh_wide != 1;  // the multi-workgroup pass may run
+    const size_t tslots = (size_t)(wide ? MH_WIDE_SLOTS : MH_WAVES) * 7 * tcap;
+    if (wide && !h.wide) {
+        SBPX_HIP(hipMalloc(&h.wide, sizeof(WideSlot) * MH_WIDE_SLOTS));
+        SBPX_HIP(hipMalloc((void**)&h.ctrl, sizeof(int) * 4));
+        SBPX_HIP(hipMalloc((void**)&h.wstamp, sizeof(unsigned long long) * std::max(c.g.N, 1)));
+        SBPX_HIP(hipMemsetAsync(h.wstamp, 0, sizeof(unsigned long long) * std::max(c.g.N, 1), c.stream));
+        SBPX_HIP(hipMemsetAsync(h.status + 9, 0, sizeof(int), c.stream));
+    }
     if (tslots > h.table_slots) {
         if (h.tables) SBPX_HIP(hipFree(h.tables));
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
@@ -1045,6 +1347,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.scan = scan;
         P.dS_sum = c.red + 2048;
         P.err = c.dev_err;
+        P.wide = (WideSlot*)h.wide;
+        P.wstamp = h.wstamp;
+        P.ctrl = h.ctrl;
         bool out_of_room = false;
         // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
         // previous chunk: the mean number of row/column slots its evaluations scanned (long scans
@@ -1063,20 +1368,30 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                     G = std::max(G, h.g_yield);
                 }
             }
+            // workgroups: with one wavefront per vertex, as many as the window yield calls for
+            int K = 1;
+            if (G == 1 && wide) K = c.mh_wide ? c.mh_wide : h.k_yield;
             P.start = pos;
             P.stop_at = std::min(n, pos + MH_CHUNK);
             SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));
+            if (K > 1) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
             if (timed) c.timer_begin();
-            switch (G) {
-                case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                case 2: mh_group_kernel<2><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                case 4: mh_group_kernel<4><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                case 8: mh_group_kernel<8><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                default: mh_group_kernel<16><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+            if (K > 1) {
+                void* args[] = {&P};
+                SBPX_HIP(hipLaunchCooperativeKernel((const void*)mh_wide_kernel, dim3(K), dim3(MH_WAVES * WAVE), args, 0,
+                                                    c.stream));
+            } else {
+                switch (G) {
+                    case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                    case 2: mh_group_kernel<2><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                    case 4: mh_group_kernel<4><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                    case 8: mh_group_kernel<8><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                    default: mh_group_kernel<16><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                }
             }
             SBPX_HIP(hipGetLastError());
             if (timed) c.timer_end(KERNEL_MH);
-            int stat[5];
+            int stat[10];
             double sc[3];
             SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipMemcpyAsync(sc, scan, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
@@ -1091,10 +1406,24 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             // Entries a window advanced, at 16 / G vertices per window. Near that cap the windows
             // could be longer: halve G; below it the yield is the measured one: about 3 or fewer
             // standing entries take 4 wavefronts per vertex, up to 6 two, more one.
+            // With K workgroups a window holds 16 K entries: near that, K doubles (up to 16);
+            // below 3/8 of it, K halves, down to the one-workgroup kernels.
             if (sc[2] > 0.0 && advanced > 0) {
                 const double y = (double)advanced / sc[2];
-                if (y >= 0.75 * (double)(MH_WAVES / G)) h.g_yield = std::max(1, G / 2);
-                else h.g_yield = y < 3.0 ? 4 : (y < 6.0 ? 2 : 1);
+                if (K > 1) {
+                    const double full = (double)(MH_WAVES * K);
+                    if (y >= 0.75 * full) h.k_yield = std::min(2 * K, MH_WIDE_MAX);
+                    else if (y < 0.375 * full) h.k_yield = K / 2;
+                    h.g_yield = 1;
+                } else {
+                    if (y >= 0.75 * (double)(MH_WAVES / G)) h.g_yield = std::max(1, G / 2);
+                    else h.g_yield = y < 3.0 ? 4 : (y < 6.0 ? 2 : 1);
+                    if (G == 1 && y >= 0.75 * (double)MH_WAVES) h.k_yield = 2;
+                }
+            }
+            if (stat[9] > (1 << 30)) {  // restart the window serials well before they wrap
+                SBPX_HIP(hipMemsetAsync(h.wstamp, 0, sizeof(unsigned long long) * c.g.N, c.stream));
+                SBPX_HIP(hipMemsetAsync(h.status + 9, 0, sizeof(int), c.stream));
             }
             if (stat[4] > (1 << 30)) {  // restart the commit serials well before they wrap
                 SBPX_HIP(hipMemsetAsync(h.touch, 0, sizeof(int) * c.g.N, c.stream));

@@ -328,16 +328,19 @@ def test_sbp_async_gibbs_end_to_end_matches_oracle():
 
 
 # ----------------------------------------------------------------------------- sequential MH
-@pytest.mark.parametrize("algo,group", [("hybrid", 0), ("mh", 0), ("mh", 1), ("mh", 2), ("hybrid", 4), ("mh", 8),
-                                        ("mh", 16)])
-def test_sequential_sweeps_match_oracle(algo, group):
+@pytest.mark.parametrize("algo,group,wide", [("hybrid", 0, 0), ("mh", 0, 0), ("mh", 1, 1), ("mh", 2, 0), ("hybrid", 4, 0),
+                                             ("mh", 8, 0), ("mh", 16, 0), ("mh", 1, 2), ("mh", 1, 4), ("mh", 1, 16),
+                                             ("hybrid", 1, 8)])
+def test_sequential_sweeps_match_oracle(algo, group, wide):
     """Sequential Metropolis-Hastings (alone, or on the high-degree vertices of the hybrid) equals
-    the oracle's one-vertex-at-a-time pass for every number of wavefronts per vertex."""
+    the oracle's one-vertex-at-a-time pass for every number of wavefronts per vertex and every
+    number of workgroups sharing a window (wide > 1: 16 * wide speculative evaluations)."""
     m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=6)
     c = ctx_for(arrays)
-    c.set_option(2, group)
+    c.set_option(m.OPT_MH_GROUP, group)
+    c.set_option(m.OPT_MH_WIDE, wide)
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(7)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH

@@ -500,8 +500,8 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
                 throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");
             ctx->c.mh_group = (int)value;
         } else if (option == 6) {
-            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16)
-                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield), 1, 2, 4, 8 or 16");
+            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield), 1, 2, 4, 8, 16 or 32");
             ctx->c.mh_wide = (int)value;
         } else {
             throw ArgError("unknown option");

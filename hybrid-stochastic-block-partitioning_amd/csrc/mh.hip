@@ -202,7 +202,8 @@ __device__ bool relocate(int* base, int* capa, int* keys, int* vals, int* occ, i
 // per neighbour block plus the four cells among {r, s}; rows/columns r four; every other
 // neighbour block one cell in its row and one in its column (two kept, for slack). Returns false
 // when the pool ran out.
-__device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab, int cap) {
+// check_blocks = false skips the neighbour blocks (blocks_need_room found none short of room).
+__device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab, int cap, bool check_blocks) {
     const int lane = lane_id();
     const int r = R.r, s = R.s;
     const int big = R.nuniq + 4;
@@ -216,6 +217,7 @@ __device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab
         return false;
     if (room(P.ccap, P.cocc, r) < 4 && !relocate(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, ctop, P.pool, r, 4))
         return false;
+    if (!check_blocks) return true;
     for (int j0 = 0; j0 < cap; j0 += WAVE) {
         const int j = j0 + lane;
         const int b = j < cap ? tab[j] : -1;
@@ -236,6 +238,20 @@ __device__ bool make_room(const MhParams& P, const MoveResult& R, const int* tab
         }
     }
     return true;
+}
+
+// Whether any neighbour block b (other than r and s) of a move's table lacks the room make_room
+// keeps in its row or column. The whole workgroup looks, one slot per lane, so that the committing
+// wavefront walks the table (relocating, in slot order) only when one does.
+__device__ bool blocks_need_room(const MhParams& P, const int* tab, int cap, int r, int s, int* s_need) {
+    if (threadIdx.x == 0) *s_need = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < cap; j += blockDim.x) {
+        const int b = tab[j];
+        if (b >= 0 && b != r && b != s && (room(P.rcap, P.rocc, b) < 2 || room(P.ccap, P.cocc, b) < 2)) *s_need = 1;
+    }
+    __syncthreads();
+    return *s_need != 0;
 }
 
 // The fields of an accepted move its commit needs, shared through LDS with every wavefront.
@@ -389,7 +405,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     __shared__ int st[MH_WAVES];    // status of each wavefront's evaluation, 3 = idle
     __shared__ int holds[MH_WAVES]; // 1 while that evaluation still holds
     __shared__ double sdS[MH_WAVES];
-    __shared__ int s_flag, s_cr, s_cs, s_ccap;
+    __shared__ int s_flag, s_cr, s_cs, s_ccap, s_need;
     __shared__ CommitInfo s_commit;
     __shared__ int* s_ctab;
     __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
@@ -465,15 +481,19 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
                 pos += j;
                 break;
             }
-            if (wave == j) {
-                const bool ok = make_room(P, R, tab, cap);
-                if (lane == 0) {
-                    s_flag = ok ? 1 : 2;
-                    s_cr = R.r;
-                    s_cs = R.s;
-                    s_ctab = tab;
-                    s_ccap = cap;
-                    s_commit = commit_info(R);
+            if (wave == j && lane == 0) {
+                s_cr = R.r;
+                s_cs = R.s;
+                s_ctab = tab;
+                s_ccap = cap;
+                s_commit = commit_info(R);
+            }
+            __syncthreads();
+            {
+                const bool need = blocks_need_room(P, s_ctab, s_ccap, s_cr, s_cs, &s_need);
+                if (wave == j) {
+                    const bool ok = make_room(P, R, tab, cap, need);
+                    if (lane == 0) s_flag = ok ? 1 : 2;
                 }
             }
             __syncthreads();
@@ -530,7 +550,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     __shared__ int st[NG];
     __shared__ int holds[NG];
     __shared__ double sdS[NG];
-    __shared__ int s_flag, s_cr, s_cs, s_ccap;
+    __shared__ int s_flag, s_cr, s_cs, s_ccap, s_need;
     __shared__ CommitInfo s_commit;
     __shared__ int* s_ctab;
     __shared__ int lds_tab[NG][7 * LCAP];
@@ -603,15 +623,19 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
                 pos += j;
                 break;
             }
-            if (g == j && leader) {
-                const bool ok = make_room(P, R, tab, cap);
-                if (lane == 0) {
-                    s_flag = ok ? 1 : 2;
-                    s_cr = R.r;
-                    s_cs = R.s;
-                    s_ctab = tab;
-                    s_ccap = cap;
-                    s_commit = commit_info(R);
+            if (g == j && leader && lane == 0) {
+                s_cr = R.r;
+                s_cs = R.s;
+                s_ctab = tab;
+                s_ccap = cap;
+                s_commit = commit_info(R);
+            }
+            __syncthreads();
+            {
+                const bool need = blocks_need_room(P, s_ctab, s_ccap, s_cr, s_cs, &s_need);
+                if (g == j && leader) {
+                    const bool ok = make_room(P, R, tab, cap, need);
+                    if (lane == 0) s_flag = ok ? 1 : 2;
                 }
             }
             __syncthreads();
@@ -675,7 +699,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 // fences (the L2 of each XCD is written back and invalidated) and a bounded wait; a workgroup
 // whose wait runs out reports DEV_ERR_BARRIER and leaves, and so does every other one at its
 // next barrier.
-constexpr int MH_WIDE_MAX = 16;  // workgroups
+constexpr int MH_WIDE_MAX = 32;  // workgroups
 constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WAVES;
 constexpr int BARRIER_SPINS = 1 << 22;
 
@@ -718,7 +742,7 @@ __global__ void wide_init_kernel(int* ctrl) {
 }
 
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
-    __shared__ int s_ok, s_flag, s_n, s_end, s_stop;
+    __shared__ int s_ok, s_flag, s_n, s_end, s_stop, s_need;
     __shared__ CommitInfo s_commit;
     __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
     __shared__ int lds_hist[MH_WAVES][256];
@@ -849,12 +873,13 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             for (int k = 0; k < s_n && s_acc[k] < s_end; ++k) {
                 const int j = s_acc[k];
                 int* const ctab = P.tables + (size_t)j * 7 * P.tcap;
+                const bool need = blocks_need_room(P, ctab, s_acc_cap[k], s_acc_r[k], s_acc_s[k], &s_need);
                 if (wave == 0) {
                     MoveResult Rj{};
                     Rj.r = s_acc_r[k];
                     Rj.s = s_acc_s[k];
                     Rj.nuniq = P.wide[j].nuniq;
-                    const bool ok = make_room(P, Rj, ctab, s_acc_cap[k]);
+                    const bool ok = make_room(P, Rj, ctab, s_acc_cap[k], need);
                     if (lane == 0) {
                         s_flag = ok ? 1 : 2;
                         s_commit = P.wide[j].ci;
@@ -1408,7 +1433,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             // Entries a window advanced, at 16 / G vertices per window. Near that cap the windows
             // could be longer: halve G; below it the yield is the measured one: about 3 or fewer
             // standing entries take 4 wavefronts per vertex, up to 6 two, more one.
-            // With K workgroups a window holds 16 K entries: near that, K doubles (up to 16);
+            // With K workgroups a window holds 16 K entries: near that, K doubles (up to 32);
             // below 3/8 of it, K halves, down to the one-workgroup kernels.
             if (sc[2] > 0.0 && advanced > 0) {
                 const double y = (double)advanced / sc[2];

@@ -76,8 +76,8 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            edges when they are at most a quarter of the edges, else rebuilt by a full sort;
  *            0 always rebuilt. Both give the same arrays.
  * option 6 = workgroups of the sequential Metropolis-Hastings pass when one wavefront evaluates
- *            each vertex: 1 (one workgroup, 16 speculative evaluations per window), 2, 4, 8 or 16
- *            (16 per workgroup in one cooperative launch), or 0 to follow the window yield
+ *            each vertex: 1 (one workgroup, 16 speculative evaluations per window), 2, 4, 8, 16 or
+ *            32 (16 per workgroup in one cooperative launch), or 0 to follow the window yield
  *            (default). Every choice applies the same moves in the same order. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 

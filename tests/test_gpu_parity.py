@@ -330,7 +330,7 @@ def test_sbp_async_gibbs_end_to_end_matches_oracle():
 # ----------------------------------------------------------------------------- sequential MH
 @pytest.mark.parametrize("algo,group,wide", [("hybrid", 0, 0), ("mh", 0, 0), ("mh", 1, 1), ("mh", 2, 0), ("hybrid", 4, 0),
                                              ("mh", 8, 0), ("mh", 16, 0), ("mh", 1, 2), ("mh", 1, 4), ("mh", 1, 16),
-                                             ("hybrid", 1, 8)])
+                                             ("mh", 1, 32), ("hybrid", 1, 8)])
 def test_sequential_sweeps_match_oracle(algo, group, wide):
     """Sequential Metropolis-Hastings (alone, or on the high-degree vertices of the hybrid) equals
     the oracle's one-vertex-at-a-time pass for every number of wavefronts per vertex and every

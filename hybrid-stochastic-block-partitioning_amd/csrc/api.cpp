@@ -503,6 +503,10 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
             if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
                 throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield), 1, 2, 4, 8, 16 or 32");
             ctx->c.mh_wide = (int)value;
+        } else if (option == 7) {
+            if (value < 0 || value > (int64_t)1 << 30) throw ArgError("table headroom must be in [0, 2^30] slots");
+            ctx->c.mh_headroom = value;
+            ctx->c.mh.headroom = (size_t)value;
         } else {
             throw ArgError("unknown option");
         }

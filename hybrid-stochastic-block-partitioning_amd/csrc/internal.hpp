@@ -116,6 +116,7 @@ struct MhState {
     int *rkeys = nullptr, *rvals = nullptr, *ckeys = nullptr, *cvals = nullptr;
     size_t slots = 0;      // pool size of each of the four arrays
     size_t headroom = 0;   // slots beyond the initial layout, for rows that outgrow their segment
+    size_t pool_limit = 0;  // slots of each pool array the kernels may use
     int *rocc = nullptr, *cocc = nullptr;    // [N] keys held by each row / column
     int* tables = nullptr;  // neighbour-block tables, one per wavefront
     size_t table_slots = 0;
@@ -196,6 +197,7 @@ struct Ctx {
     int proposal = 0;  // 0 plain, 1 learned mixture for the batched low-degree vertices
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
     int mh_wide = 0;   // workgroups of the Metropolis-Hastings pass; 0 = by window yield
+    int64_t mh_headroom = 0;  // initial table pool headroom in slots; 0 = the initial layout (>= 1M)
 
     // scratch
     void* keys0 = nullptr;

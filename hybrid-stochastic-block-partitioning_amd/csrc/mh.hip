@@ -985,10 +985,12 @@ static void build_tables(Ctx& c) {
     SBPX_HIP(hipMemcpyAsync(&h.max_cap, h.status + 5, sizeof(int), hipMemcpyDeviceToHost, st));
     SBPX_HIP(hipStreamSynchronize(st));
     const size_t used = (size_t)std::max(tot[0], tot[1]);
-    h.headroom = std::max(h.headroom, std::max(used, (size_t)1 << 20));
+    if (c.mh_headroom > 0) h.headroom = std::max(h.headroom, (size_t)c.mh_headroom);
+    else h.headroom = std::max(h.headroom, std::max(used, (size_t)1 << 20));
     // pool tops stay below 2^31 even after a failed claim
     const size_t need = std::min(used + h.headroom, (size_t)(INT32_MAX / 2));
-    if (need < used + 64) throw StateError("block matrix too large for the Metropolis-Hastings tables");
+    if (need < used + std::min(h.headroom, (size_t)64))
+        throw StateError("block matrix too large for the Metropolis-Hastings tables");
     if (need > h.slots) {
         for (int** p : {&h.rkeys, &h.rvals, &h.ckeys, &h.cvals}) {
             if (*p) SBPX_HIP(hipFree(*p));
@@ -996,6 +998,8 @@ static void build_tables(Ctx& c) {
         }
         h.slots = need;
     }
+    // the pool the kernels may fill: the whole allocation, or exactly the headroom asked for
+    h.pool_limit = c.mh_headroom > 0 ? need : h.slots;
     const int wpb = tb / WAVE;
     hash_fill_kernel<<<(B + wpb - 1) / wpb, tb, 0, st>>>(c.bm.row_ptr, c.bm.row_col, c.bm.row_val, B, h.rbase,
                                                           h.rkeys, h.rvals, h.rocc);
@@ -1268,7 +1272,7 @@ void merge_heap_select(Ctx& c, int num_to_merge, const double* dS, const int32_t
             P.ckeys = h.ckeys;
             P.cvals = h.cvals;
             P.cocc = h.cocc;
-            P.pool = (int)h.slots;
+            P.pool = (int)h.pool_limit;
             P.status = h.status;
             P.err = c.dev_err;
             merge_heap_kernel<<<1, WAVE, 0, c.stream>>>(P, d_heap, len, d_parent, merged, num_to_merge, pending_r,
@@ -1359,7 +1363,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.ckeys = h.ckeys;
         P.cvals = h.cvals;
         P.cocc = h.cocc;
-        P.pool = (int)h.slots;
+        P.pool = (int)h.pool_limit;
         P.list = d_list;
         P.n = n;
         P.seed = seed;

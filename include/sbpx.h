@@ -78,7 +78,11 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  * option 6 = workgroups of the sequential Metropolis-Hastings pass when one wavefront evaluates
  *            each vertex: 1 (one workgroup, 16 speculative evaluations per window), 2, 4, 8, 16 or
  *            32 (16 per workgroup in one cooperative launch), or 0 to follow the window yield
- *            (default). Every choice applies the same moves in the same order. */
+ *            (default). Every choice applies the same moves in the same order.
+ * option 7 = spare slots of the Metropolis-Hastings / heap-merge table pool when it is built:
+ *            0 (default) as many as the initial layout holds (at least 2^20); a positive value
+ *            limits the pool to exactly that many (and restarts the doubling that follows each
+ *            time the pool runs out), so that the rebuild-and-resume path runs; for tests. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

@@ -267,14 +267,17 @@ def test_merge_proposals_match_oracle():
         assert np.allclose(dS, odS, rtol=DS_RTOL, atol=1e-9)
 
 
-@pytest.mark.parametrize("approximate", [True, False])
-def test_merge_blocks_matches_oracle(approximate):
+@pytest.mark.parametrize("approximate,headroom", [(True, 0), (False, 0), (False, 48)])
+def test_merge_blocks_matches_oracle(approximate, headroom):
     """Merge phases from the identity partition: proposals, then the sorted greedy selection
-    (--approximate) or the heap-ordered selection with re-scoring on the current matrix."""
+    (--approximate) or the heap-ordered selection with re-scoring on the current matrix. With a
+    table pool of `headroom` spare slots the heap selection runs out of room repeatedly and
+    resumes with the merge it had accepted."""
     m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=2)
     c = ctx_for(arrays)
+    c.set_option(m.OPT_MH_HEADROOM, headroom)
     g = O.Graph(n, *arrays)
     a = np.arange(n, dtype=np.int32)
     c.build(n, a)
@@ -335,12 +338,26 @@ def test_sequential_sweeps_match_oracle(algo, group, wide):
     """Sequential Metropolis-Hastings (alone, or on the high-degree vertices of the hybrid) equals
     the oracle's one-vertex-at-a-time pass for every number of wavefronts per vertex and every
     number of workgroups sharing a window (wide > 1: 16 * wide speculative evaluations)."""
+    sequential_case(algo, group, wide)
+
+
+@pytest.mark.parametrize("algo,group,wide", [("mh", 1, 1), ("mh", 1, 8), ("mh", 16, 1), ("hybrid", 0, 0)])
+def test_sequential_sweeps_resume_after_table_pool_runs_out(algo, group, wide):
+    """With a table pool of 48 spare slots the pass runs out of room over and over; each time the
+    host rebuilds the tables from the moves so far with twice the headroom and resumes at the
+    same queue entry. The pass still equals the oracle's."""
+    launches = sequential_case(algo, group, wide, headroom=48)
+    assert max(launches) > 2, "the pool never ran out"
+
+
+def sequential_case(algo, group, wide, headroom=0):
     m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=6)
     c = ctx_for(arrays)
     c.set_option(m.OPT_MH_GROUP, group)
     c.set_option(m.OPT_MH_WIDE, wide)
+    launches = []
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(7)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
@@ -353,7 +370,9 @@ def test_sequential_sweeps_match_oracle(algo, group, wide):
         c.build(B, a)
         bm = O.Blockmodel(g, B, a)
         for call in range(3):
+            c.set_option(m.OPT_MH_HEADROOM, headroom)  # also restarts the headroom doubling
             st = c.sweep(code, 2, 4321, call)
+            launches.append(st.eval_launches[3])
             if algo == "hybrid":
                 o1 = bm.mh_pass(4321, call)
                 o2 = bm.async_sweep(2, 4321, call, hybrid_low=True)
@@ -364,6 +383,7 @@ def test_sequential_sweeps_match_oracle(algo, group, wide):
                 assert st.dS_accepted == pytest.approx(o1.dS_sum, rel=1e-9, abs=1e-9)
             assert (st.attempts, st.evals, st.moves) == ost
             assert_same_state(c, bm)
+    return launches
 
 
 @pytest.mark.parametrize("approximate", [True, False])

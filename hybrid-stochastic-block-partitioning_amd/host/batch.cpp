@@ -36,7 +36,11 @@ struct Request {
     double value = 0.0;
     int blocks_left = 0;
     std::string error;
+    // set when the round that ran the request is over; each driver waits on its own request, so a
+    // round wakes its drivers one by one instead of all of them on one condition variable
     bool done = false;
+    std::mutex wake_mu;
+    std::condition_variable wake;
 };
 
 }  // namespace
@@ -45,7 +49,6 @@ struct Batch {
     sbpx_ctx* ctx = nullptr;
     std::vector<int> vertex_offsets;
     std::mutex mu;
-    std::condition_variable cv;
     int live = 0;  // drivers still running
     std::vector<Request*> pending;
     long rounds = 0;
@@ -200,10 +203,11 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
 }
 
 // Called with the lock held when every running driver waits: runs the round without the lock
-// (no driver can act meanwhile) and wakes them.
+// (no driver can act meanwhile) and wakes them. Returns with the lock released.
 void run_round(Batch& b, std::unique_lock<std::mutex>& lk) {
     std::vector<Request*> reqs;
     reqs.swap(b.pending);
+    b.rounds++;
     lk.unlock();
     try {
         execute(b, reqs);
@@ -211,10 +215,13 @@ void run_round(Batch& b, std::unique_lock<std::mutex>& lk) {
         for (Request* r : reqs)
             if (r->error.empty()) r->error = e.what();
     }
-    lk.lock();
-    b.rounds++;
-    for (Request* r : reqs) r->done = true;
-    b.cv.notify_all();
+    // notified under the request's own lock: its driver may return (and free the request) as
+    // soon as it sees `done`, which it cannot before this lock is released
+    for (Request* r : reqs) {
+        std::lock_guard<std::mutex> g(r->wake_mu);
+        r->done = true;
+        r->wake.notify_one();
+    }
 }
 
 void submit(Batch& b, Request& r) {
@@ -228,8 +235,13 @@ void submit(Batch& b, Request& r) {
             r.slot < (int)b.lp_valid.size())
             b.lp_valid[r.slot] = 0;
         b.pending.push_back(&r);
-        if ((int)b.pending.size() == b.live) run_round(b, lk);
-        else b.cv.wait(lk, [&r] { return r.done; });
+        if ((int)b.pending.size() == b.live) {
+            run_round(b, lk);
+        } else {
+            lk.unlock();
+            std::unique_lock<std::mutex> w(r.wake_mu);
+            r.wake.wait(w, [&r] { return r.done; });
+        }
     }
     if (!r.error.empty()) throw std::runtime_error(r.error);
 }

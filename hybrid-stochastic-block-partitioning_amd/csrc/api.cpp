@@ -1,10 +1,14 @@
 // C ABI (include/sbpx.h) over the device kernels.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
+#include <exception>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <numeric>
+#include <thread>
 
 #include "internal.hpp"
 
@@ -291,6 +295,9 @@ void free_batch(Ctx& c) {
     dfree(&b.d_seed);
     dfree(&b.d_call);
     dfree(&b.d_active);
+    if (b.h_dS) (void)hipHostFree(b.h_dS);
+    for (int* p : {b.h_best, b.h_count, b.h_map})
+        if (p) (void)hipHostFree(p);
     b = DevBatch();
 }
 
@@ -969,6 +976,8 @@ static int uf_root(std::vector<int>& parent, int x) {
 // comparator, so ties fall in the same order as the reference's). block_map always names the
 // block a block has been merged into; a union-find with the merge target as root keeps the same
 // mapping in near-linear time.
+constexpr int MERGE_SELECT_THREADS = 8;  // host threads sharing a batched merge round's selections
+
 static void select_merges_sorted(int B, int num_to_merge, const double* dS, const int32_t* best,
                                  std::vector<int>& parent) {
     std::vector<int> order(B);
@@ -1593,23 +1602,51 @@ sbpx_status sbpx_batch_merge(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, co
         launch_merge_proposals(c, 0, 0);
         launch_block_counts(c);
         const int span = hi - lo;
-        std::vector<double> dS(span);
-        std::vector<int> best(span), count(span), map(span);
-        SBPX_HIP(hipMemcpyAsync(dS.data(), c.m_dS + lo, sizeof(double) * span, hipMemcpyDeviceToHost, c.stream));
-        SBPX_HIP(hipMemcpyAsync(best.data(), c.m_best + lo, sizeof(int) * span, hipMemcpyDeviceToHost, c.stream));
-        SBPX_HIP(hipMemcpyAsync(count.data(), c.m_count + lo, sizeof(int) * span, hipMemcpyDeviceToHost, c.stream));
+        if (!b.h_dS) {
+            SBPX_HIP(hipHostMalloc((void**)&b.h_dS, sizeof(double) * b.range));
+            for (int** p : {&b.h_best, &b.h_count, &b.h_map}) SBPX_HIP(hipHostMalloc((void**)p, sizeof(int) * b.range));
+        }
+        double* dS = b.h_dS;
+        int *best = b.h_best, *count = b.h_count, *map = b.h_map;
+        SBPX_HIP(hipMemcpyAsync(dS, c.m_dS + lo, sizeof(double) * span, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(best, c.m_best + lo, sizeof(int) * span, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(count, c.m_count + lo, sizeof(int) * span, hipMemcpyDeviceToHost, c.stream));
+        std::iota(map, map + span, lo);  // blocks of graphs not listed keep their ids
         c.sync();
         check_device_error(c);
-        std::iota(map.begin(), map.end(), lo);
-        for (int i = 0; i < n; ++i) {
+        // the graphs' selections are independent (each reads and writes its own slice): a few
+        // host threads share them
+        auto select = [&](int i) {
             const int g = graphs[i], B = b.nb[g], o = b.bo[g] - lo;
             std::vector<int> best_local(B), parent(B);
             for (int x = 0; x < B; ++x) best_local[x] = best[o + x] >= 0 ? best[o + x] - b.bo[g] : -1;
             std::iota(parent.begin(), parent.end(), 0);
-            select_merges_sorted(B, num_to_merge[i], dS.data() + o, best_local.data(), parent);
-            merge_mapping(B, num_to_merge[i], count.data() + o, parent, b.bo[g], map.data() + o);
+            select_merges_sorted(B, num_to_merge[i], dS + o, best_local.data(), parent);
+            merge_mapping(B, num_to_merge[i], count + o, parent, b.bo[g], map + o);
+        };
+        const int nt = std::min(n, MERGE_SELECT_THREADS);
+        if (nt <= 1) {
+            for (int i = 0; i < n; ++i) select(i);
+        } else {
+            std::atomic<int> next(0);
+            std::mutex err_mu;
+            std::exception_ptr err;
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nt; ++t)
+                pool.emplace_back([&] {
+                    for (int i = next++; i < n; i = next++) {
+                        try {
+                            select(i);
+                        } catch (...) {
+                            std::lock_guard<std::mutex> g(err_mu);
+                            if (!err) err = std::current_exception();
+                        }
+                    }
+                });
+            for (std::thread& t : pool) t.join();
+            if (err) std::rethrow_exception(err);
         }
-        SBPX_HIP(hipMemcpyAsync(c.m_map, map.data(), sizeof(int) * span, hipMemcpyHostToDevice, c.stream));
+        SBPX_HIP(hipMemcpyAsync(c.m_map, map, sizeof(int) * span, hipMemcpyHostToDevice, c.stream));
         launch_batch_relabel(c, lo, hi);
         for (int i = 0; i < n; ++i) b.nb[graphs[i]] -= num_to_merge[i];
         batch_upload_nb(c);

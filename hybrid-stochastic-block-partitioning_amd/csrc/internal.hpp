@@ -172,6 +172,10 @@ struct DevBatch {
     int* d_plan = nullptr;
     int* d_plan_huge = nullptr;
     size_t plan_slots = 0, plan_huge_slots = 0;
+    // pinned host staging of a merge round over block ids [lo, hi): scores, proposals, block
+    // sizes and the new labels (each `range` long, allocated on the first merge)
+    double* h_dS = nullptr;
+    int *h_best = nullptr, *h_count = nullptr, *h_map = nullptr;
 };
 
 // Buffers of the incremental block-matrix update (allocated on first use).

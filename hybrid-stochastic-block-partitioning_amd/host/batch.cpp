@@ -8,6 +8,8 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -60,6 +62,11 @@ struct Batch {
     std::vector<double> lp;
     std::vector<char> lp_valid;
     std::vector<int32_t> assign;
+    // wall time of the rounds by request kind (the log posterior refresh under LOGP) and between
+    // rounds, reported on stderr when SBP_TRACE is set
+    double kind_s[6] = {0, 0, 0, 0, 0, 0};
+    double between_s = 0.0;
+    std::chrono::steady_clock::time_point last_end{};
 };
 
 namespace {
@@ -79,6 +86,12 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
     // steps that change states first, then the log posteriors of all graphs, then the reads
     for (int kind : {Request::BUILD, Request::MERGE, Request::SWEEP, Request::LOGP, Request::DOWNLOAD,
                      Request::TRACE}) {
+        const auto t_kind = std::chrono::steady_clock::now();
+        struct KindTimer {
+            double& acc;
+            std::chrono::steady_clock::time_point t;
+            ~KindTimer() { acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t).count(); }
+        } kind_timer{b.kind_s[kind], t_kind};
         if (kind == Request::LOGP) {
             const bool changed = by_kind.count(Request::BUILD) || by_kind.count(Request::MERGE) ||
                                  by_kind.count(Request::SWEEP);
@@ -207,6 +220,8 @@ void execute(Batch& b, std::vector<Request*>& reqs) {
 void run_round(Batch& b, std::unique_lock<std::mutex>& lk) {
     std::vector<Request*> reqs;
     reqs.swap(b.pending);
+    const auto t_start = std::chrono::steady_clock::now();
+    if (b.rounds > 0) b.between_s += std::chrono::duration<double>(t_start - b.last_end).count();
     b.rounds++;
     lk.unlock();
     try {
@@ -215,6 +230,7 @@ void run_round(Batch& b, std::unique_lock<std::mutex>& lk) {
         for (Request* r : reqs)
             if (r->error.empty()) r->error = e.what();
     }
+    b.last_end = std::chrono::steady_clock::now();
     // notified under the request's own lock: its driver may return (and free the request) as
     // soon as it sees `done`, which it cannot before this lock is released
     for (Request* r : reqs) {
@@ -385,6 +401,13 @@ BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::
     run.rounds = batch.rounds;
     run.sweeps = batch.sweeps;
     run.totals = batch.totals;
+    if (std::getenv("SBP_TRACE"))
+        std::fprintf(stderr,
+                     "batch rounds %ld: build %.3f s, merge %.3f s, sweep %.3f s, log posterior %.3f s, "
+                     "download %.3f s, trace %.3f s; between rounds %.3f s\n",
+                     batch.rounds, batch.kind_s[sbpx_host::Request::BUILD], batch.kind_s[sbpx_host::Request::MERGE],
+                     batch.kind_s[sbpx_host::Request::SWEEP], batch.kind_s[sbpx_host::Request::LOGP], batch.kind_s[sbpx_host::Request::DOWNLOAD],
+                     batch.kind_s[sbpx_host::Request::TRACE], batch.between_s);
     return run;
 }
 

@@ -267,12 +267,22 @@ __device__ __forceinline__ CommitInfo commit_info(const MoveResult& R) {
 // (blockmodel.cpp:344-360), the assignment, and the neighbours' touch stamps. Each upsert is a
 // chain of dependent global reads and atomics, so the updates are spread over lanes rather than
 // run one after another. Row s and column s are the only tables that can take several new keys
-// in one commit: wavefront 0 fills row s and wavefront 1 column s, walking the neighbour table in
-// slot order with one lane per slot and then the cells among {r, s}, so the keys land in the same
-// slots on every run and later scans see the same order. Every other update is to a key that
+// in one commit: wavefront 0 fills row s and wavefront 1 column s, taking the neighbour blocks in
+// slot order with one lane per block and then the cells among {r, s}, so the keys land in the
+// same slots on every run and later scans see the same order. Every other update is to a key that
 // exists or is the only new key of its table, and is spread over the remaining wavefronts. The
 // committing wavefront's table `tab` is in LDS or global memory; make_room has run.
-__device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int* tab, int cap, int serial) {
+constexpr int COMMIT_LIST_MAX = 2048;  // neighbour-block table slots a commit compacts in LDS
+
+// LDS a commit works in: the occupied slots of the committing table (other than r and s) in slot
+// order, and per-wavefront counts for the compaction.
+struct CommitLds {
+    int list[COMMIT_LIST_MAX];
+    int wcount[MH_WAVES];
+};
+
+__device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int* tab, int cap, int serial,
+                              CommitLds& L) {
     constexpr int NT = MH_WAVES * WAVE;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     const int r = C.r, s = C.s;
@@ -285,23 +295,53 @@ __device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int*
         for (int j = ob + tid; j < oe; j += NT) P.touch[P.G.out_idx[j]] = serial;
         for (int j = ib + tid; j < ie; j += NT) P.touch[P.G.in_idx[j]] = serial;
     }
+    // The neighbour blocks b (other than r and s), in slot order: compacted into LDS when the
+    // table fits, so that the loops below run over the blocks rather than over every slot.
+    const bool compact = cap <= COMMIT_LIST_MAX;
+    int n = cap;
+    if (compact) {
+        int base = 0;
+        for (int j0 = 0; j0 < cap; j0 += NT) {
+            const int j = j0 + tid;
+            const int b = j < cap ? tab[j] : -1;
+            const bool keep = b >= 0 && b != r && b != s;
+            const unsigned long long m = __ballot(keep);
+            if (lane == 0) L.wcount[wave] = __popcll(m);
+            __syncthreads();
+            int off = base, total = base;
+            for (int w = 0; w < MH_WAVES; ++w) {
+                if (w < wave) off += L.wcount[w];
+                total += L.wcount[w];
+            }
+            if (keep) L.list[off + __popcll(m & ((1ull << lane) - 1))] = j;
+            __syncthreads();
+            base = total;
+        }
+        n = base;
+    }
+    // slot of the k-th neighbour block (or -1 for a slot to skip when not compacted)
+    auto slot_of = [&](int k) -> int {
+        if (compact) return L.list[k];
+        const int b = tab[k];
+        return (b < 0 || b == r || b == s) ? -1 : k;
+    };
     if (wave == 0) {  // row s: (s, b) += o_b, then (s, s) and (s, r)
-        for (int j = lane; j < cap; j += WAVE) {
-            const int b = tab[j];
-            if (b < 0 || b == r || b == s) continue;
+        for (int k = lane; k < n; k += WAVE) {
+            const int j = slot_of(k);
+            if (j < 0) continue;
             const int o = to[j];
-            if (o) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, b, o);
+            if (o) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, tab[j], o);
         }
         if (lane == 0) {
             if (C.d_ss) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, s, C.d_ss);
             if (C.d_sr) upsert(P.rbase, P.rcap, P.rkeys, P.rvals, P.rocc, s, r, C.d_sr);
         }
     } else if (wave == 1) {  // column s: (b, s) += i_b, then (s, s) and (r, s)
-        for (int j = lane; j < cap; j += WAVE) {
-            const int b = tab[j];
-            if (b < 0 || b == r || b == s) continue;
+        for (int k = lane; k < n; k += WAVE) {
+            const int j = slot_of(k);
+            if (j < 0) continue;
             const int i = ti[j];
-            if (i) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, s, b, i);
+            if (i) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, s, tab[j], i);
         }
         if (lane == 0) {
             if (C.d_ss) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, s, s, C.d_ss);
@@ -330,10 +370,11 @@ __device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int*
         }
     } else {  // the other six updates per neighbour block b
         constexpr int NW = (MH_WAVES - 3) * WAVE;
-        for (int it = tid - 3 * WAVE; it < 6 * cap; it += NW) {
-            const int k = it / cap, j = it - k * cap;
+        for (int it = tid - 3 * WAVE; it < 6 * n; it += NW) {
+            const int k = it / n, q = it - k * n;
+            const int j = slot_of(q);
+            if (j < 0) continue;
             const int b = tab[j];
-            if (b < 0 || b == r || b == s) continue;
             const int o = to[j], i = ti[j];
             switch (k) {
                 case 0: if (o) upsert(P.cbase, P.ccap, P.ckeys, P.cvals, P.cocc, b, s, o); break;   // (s, b)
@@ -407,6 +448,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     __shared__ double sdS[MH_WAVES];
     __shared__ int s_flag, s_cr, s_cs, s_ccap, s_need;
     __shared__ CommitInfo s_commit;
+    __shared__ CommitLds s_clds;
     __shared__ int* s_ctab;
     __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
     __shared__ int lds_hist[MH_WAVES][256];
@@ -502,7 +544,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
                 pos += j;
                 break;
             }
-            commit_shared(P, s_commit, s_ctab, s_ccap, serial + 1);
+            commit_shared(P, s_commit, s_ctab, s_ccap, serial + 1, s_clds);
             __syncthreads();
             ++serial;
             if (wave == j && lane == 0) {
@@ -552,6 +594,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     __shared__ double sdS[NG];
     __shared__ int s_flag, s_cr, s_cs, s_ccap, s_need;
     __shared__ CommitInfo s_commit;
+    __shared__ CommitLds s_clds;
     __shared__ int* s_ctab;
     __shared__ int lds_tab[NG][7 * LCAP];
     __shared__ int lds_hist[NG][256];
@@ -644,7 +687,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
                 pos += j;
                 break;
             }
-            commit_shared(P, s_commit, s_ctab, s_ccap, serial + 1);
+            commit_shared(P, s_commit, s_ctab, s_ccap, serial + 1, s_clds);
             __syncthreads();
             ++serial;
             if (g == j && leader && lane == 0) {
@@ -744,6 +787,7 @@ __global__ void wide_init_kernel(int* ctrl) {
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     __shared__ int s_ok, s_flag, s_n, s_end, s_stop, s_need;
     __shared__ CommitInfo s_commit;
+    __shared__ CommitLds s_clds;
     __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
     __shared__ int lds_hist[MH_WAVES][256];
     __shared__ int s_acc[MH_WIDE_SLOTS], s_acc_r[MH_WIDE_SLOTS], s_acc_s[MH_WIDE_SLOTS], s_acc_cap[MH_WIDE_SLOTS];
@@ -894,7 +938,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
                     __syncthreads();
                     break;
                 }
-                commit_shared(P, s_commit, ctab, s_acc_cap[k], serial + 1);
+                commit_shared(P, s_commit, ctab, s_acc_cap[k], serial + 1, s_clds);
                 __syncthreads();
                 ++serial;
                 if (threadIdx.x == 0) {

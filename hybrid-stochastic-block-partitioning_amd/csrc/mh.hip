@@ -757,6 +757,10 @@ struct WideSlot {
 
 // The n-th barrier of the launch (n >= 1) completes when ctrl[0] reaches n * gridDim.x.
 __device__ bool grid_barrier(int* ctrl, int n, int* s_ok) {
+    if (gridDim.x == 1) {  // one workgroup: a workgroup barrier is enough
+        __syncthreads();
+        return true;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -1446,15 +1450,19 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             // workgroups: with one wavefront per vertex, as many as the window yield calls for
             int K = 1;
             if (G == 1 && wide) K = c.mh_wide ? c.mh_wide : h.k_yield;
+            // one workgroup, one wavefront per vertex: the window checks of the wide kernel
+            const bool wide1 = K == 1 && G == 1 && wide && c.mh_wide == 0;
             P.start = pos;
             P.stop_at = std::min(n, pos + MH_CHUNK);
             SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));
-            if (K > 1) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
+            if (K > 1 || wide1) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
             if (timed) c.timer_begin();
             if (K > 1) {
                 void* args[] = {&P};
                 SBPX_HIP(hipLaunchCooperativeKernel((const void*)mh_wide_kernel, dim3(K), dim3(MH_WAVES * WAVE), args, 0,
                                                     c.stream));
+            } else if (wide1) {
+                mh_wide_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
             } else {
                 switch (G) {
                     case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;

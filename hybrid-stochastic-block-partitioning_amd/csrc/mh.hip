@@ -3,17 +3,19 @@
 // the block matrix before the next vertex is proposed.
 //
 // The block matrix is held, for the duration of the pass, as a per-row open-addressing table in
-// both orientations (HashView), built from the CSR snapshot. One workgroup of 16 wavefronts runs
-// the pass: in each round the wavefronts evaluate the next 16 vertices of the queue against the
-// current state; the first accepted move in queue order is committed and the round restarts
-// after it. Every vertex before it was rejected and so left the state unchanged, which makes the
-// outcome identical to evaluating one vertex at a time (proposals and uniforms are keyed by
-// vertex, not by evaluation order). Before a commit, every row and column it may extend is given
-// room: a row that is too full moves to a larger segment taken from the free end of the pool
-// (the other wavefronts wait at the barrier meanwhile). Only when the pool itself runs out does
-// the kernel stop; the host then rebuilds the tables with a larger pool and the pass resumes at
-// the same vertex. When windows keep most of their entries, mh_wide_kernel spreads a window of
-// 16 K entries over K workgroups with the same outcome.
+// both orientations (HashView), built from the CSR snapshot. A window of queue entries is
+// evaluated speculatively against the current state, one wavefront (or a group of them) per
+// entry, and walked in queue order: a rejected entry is final, an accepted one is committed, and
+// a later entry is kept only if no commit before it changed anything its evaluation read. The
+// first entry that is not kept ends the window; the next window starts at it. The outcome is
+// that of evaluating one vertex at a time, because proposals and uniforms are keyed by vertex,
+// not by evaluation order. mh_kernel and mh_group_kernel re-check the later entries after every
+// commit (one workgroup, 16 entries per window); mh_wide_kernel checks every entry against the
+// accepted entries before it at once, over K workgroups (16 K entries per window, K = 1 too).
+// Before a commit, every row and column it may extend is given room: a row that is too full moves
+// to a larger segment taken from the free end of the pool (the other wavefronts wait at the
+// barrier meanwhile). Only when the pool itself runs out does the kernel stop; the host then
+// rebuilds the tables with a larger pool and the pass resumes at the same vertex.
 #include <algorithm>
 #include <numeric>
 #include <cstdint>
@@ -741,7 +743,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 // Three grid barriers per window: a counter in device memory with device-scope release/acquire
 // fences (the L2 of each XCD is written back and invalidated) and a bounded wait; a workgroup
 // whose wait runs out reports DEV_ERR_BARRIER and leaves, and so does every other one at its
-// next barrier.
+// next barrier. With one workgroup they are workgroup barriers.
 constexpr int MH_WIDE_MAX = 32;  // workgroups
 constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WAVES;
 constexpr int BARRIER_SPINS = 1 << 22;

@@ -116,7 +116,7 @@ void free_graph_buffers(Ctx& c) {
     c.table_pool_slots = 0;
     MhState& h = c.mh;
     for (int** p : {&h.rbase, &h.cbase, &h.rkeys, &h.rvals, &h.ckeys, &h.cvals, &h.rocc, &h.cocc, &h.rcap,
-                    &h.ccap, &h.tables, &h.status, &h.touch, &h.d_all, &h.d_high, &h.ctrl})
+                    &h.ccap, &h.tables, &h.status, &h.touch, &h.d_all, &h.d_high, &h.ctrl, &h.assign0})
         dfree(p);
     dfree(&h.wstamp);
     if (h.wide) (void)hipFree(h.wide);

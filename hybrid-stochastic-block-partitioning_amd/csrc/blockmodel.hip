@@ -558,7 +558,7 @@ void rebuild_keep_drift(Ctx& c) {
 // quarter of the edges' worth (incremental update after the commit), else returns -1 (full
 // rebuild). Returns the number of change entries.
 template <typename K>
-static int emit_delta(Ctx& c, const int* list, int n) {
+static int emit_delta(Ctx& c, const int* list, int n, const int* old_assign) {
     ensure_inc(c);
     hipStream_t st = c.stream;
     const KeySpace<K> ks = key_space<K>(c);
@@ -572,17 +572,18 @@ static int emit_delta(Ctx& c, const int* list, int n) {
     if (total == 0) return 0;
     const int wpb = TB / WAVE;
     delta_emit_kernel<K><<<(n + wpb - 1) / wpb, TB, 0, st>>>(list, n, c.ev_acc, c.inc.rank, c.g.out_ptr, c.g.out_idx,
-                                                            c.g.in_ptr, c.g.in_idx, c.bm.assign, c.new_block,
+                                                            c.g.in_ptr, c.g.in_idx, old_assign, c.new_block,
                                                             c.moved_epoch, c.epoch, ks.shift, ks.dense,
                                                             (K)((((K)1) << (2 * ks.bits)) - 1), (K*)c.keys0, c.vals0);
     SBPX_HIP(hipGetLastError());
     return total;
 }
 
-int prepare_update(Ctx& c, const int* list, int n) {
+int prepare_update(Ctx& c, const int* list, int n, const int* old_assign) {
     if (!c.bm.valid || c.g.E <= 0 || c.incremental == 0) return -1;
+    if (old_assign == nullptr) old_assign = c.bm.assign;
     const int bits = bits_for(c.batch.G > 0 ? c.batch.used : c.bm.B);
-    return 2 * bits <= 32 ? emit_delta<uint32_t>(c, list, n) : emit_delta<uint64_t>(c, list, n);
+    return 2 * bits <= 32 ? emit_delta<uint32_t>(c, list, n, old_assign) : emit_delta<uint64_t>(c, list, n, old_assign);
 }
 
 void finish_update(Ctx& c, const int* list, int n, int total) {

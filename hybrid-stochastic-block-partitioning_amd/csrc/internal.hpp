@@ -131,6 +131,7 @@ struct MhState {
     unsigned long long* wstamp = nullptr;  // [N] accepted-entry stamps of the multi-workgroup pass
     void* wide = nullptr;                  // its per-entry records
     int* ctrl = nullptr;                   // its barrier counter and window control words
+    int* assign0 = nullptr;  // [N] the assignment the block-matrix snapshot was built from
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
     int max_deg_high = 0;
@@ -342,8 +343,10 @@ int radix_sort(K* keys, K* keys_alt, int* vals, int* vals_alt, long n, const int
 void build_blockmodel(Ctx& c);               // from c.bm.assign, resets drift
 void rebuild_keep_drift(Ctx& c);             // from c.bm.assign, keeps drift
 // Incremental update after a batch: prepare before the moves are committed (returns the change
-// count, or -1 for a full rebuild), finish after.
-int prepare_update(Ctx& c, const int* list, int n);
+// count, or -1 for a full rebuild), finish after. The moved vertices of `list` are those with
+// ev_acc == 2 (new block in new_block, moved_epoch == epoch); old_assign is the assignment the
+// block matrix was built from (c.bm.assign when null).
+int prepare_update(Ctx& c, const int* list, int n, const int* old_assign = nullptr);
 void finish_update(Ctx& c, const int* list, int n, int total);
 double log_posterior(Ctx& c);                // sum of M * logf(M / (dout din)) in FP64
 // Per-graph log posteriors of a union (out[G]); a one-graph context gives out[0]. Each graph's

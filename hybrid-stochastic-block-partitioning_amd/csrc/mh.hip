@@ -1018,6 +1018,41 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Marks the queue vertices the pass moved (their block differs from the snapshot's) as a batch of
+// moves for the incremental block-matrix update.
+__global__ void mark_pass_moves_kernel(const int* __restrict__ list, int n, const int* __restrict__ assign0,
+                                       const int* __restrict__ assign, int epoch, uint8_t* __restrict__ stat,
+                                       int* __restrict__ new_block, int* __restrict__ moved_epoch) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = list[i];
+    const int a = assign[v];
+    if (a != assign0[v]) {
+        stat[v] = 2;
+        new_block[v] = a;
+        moved_epoch[v] = epoch;
+    } else {
+        stat[v] = 0;
+    }
+}
+
+// Brings the CSR snapshot up to date with the pass's moves so far (dout/din exact, deg keeps the
+// drift the commits left): from the moved vertices' edges when they are few, else by a full
+// rebuild; then takes a new snapshot of the assignment.
+static void update_after_pass(Ctx& c, const int* d_list, int n) {
+    MhState& h = c.mh;
+    const int N = c.g.N;
+    c.epoch++;
+    mark_pass_moves_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(d_list, n, h.assign0, c.bm.assign, c.epoch,
+                                                                 c.ev_acc, c.new_block, c.moved_epoch);
+    SBPX_HIP(hipGetLastError());
+    const int total = prepare_update(c, d_list, n, h.assign0);
+    if (total >= 0) finish_update(c, d_list, n, total);
+    else rebuild_keep_drift(c);
+    SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * N, hipMemcpyDeviceToDevice, c.stream));
+}
+
+// ------------------------------------------------------------------------------------------
 static void build_tables(Ctx& c) {
     const int B = c.bm.B;
     hipStream_t st = c.stream;
@@ -1388,6 +1423,8 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
         h.table_slots = tslots;
     }
+    if (!h.assign0) SBPX_HIP(hipMalloc((void**)&h.assign0, sizeof(int) * std::max(c.g.N, 1)));
+    SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToDevice, c.stream));
     int pos = 0;
     int rounds = 0;
     double* scan = c.red + 2052;  // [3] scanned slots, evaluations and windows of the current chunk
@@ -1521,7 +1558,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (pos < P.stop_at) throw StateError("Metropolis-Hastings pass stopped early");
         }
         // the matrix changed: bring the CSR snapshot up to date (dout/din exact, deg keeps drift)
-        rebuild_keep_drift(c);
+        update_after_pass(c, d_list, n);
         if (out_of_room) h.headroom *= 2;
     }
 }

@@ -138,7 +138,7 @@ void free_mlp(Ctx& c) {
     for (float** p : {&m.z, &m.loss, &m.sz, &m.sz2, &m.qf, &m.qb}) dfree(p);
     dfree(&m.ds_mask);
     dfree(&m.tmask);
-    for (int** p : {&m.ds_y, &m.ds_count, &m.cand, &m.cnt, &m.ps, &m.ty, &m.tidx}) dfree(p);
+    for (int** p : {&m.ds_y, &m.ds_count, &m.cand, &m.cnt, &m.ps, &m.ty, &m.tidx, &m.log_flag, &m.log_rank}) dfree(p);
     dfree(&m.pu);
     m = MlpState();
 }

@@ -244,14 +244,13 @@ __global__ void reverse_kernel(const int* __restrict__ list, int n, const int* _
 }
 
 // Appends the accepted moves whose target is a candidate to the training set.
-__global__ void log_moves_kernel(const int* __restrict__ list, int n, const int* __restrict__ assign,
-                                 const uint8_t* __restrict__ stat, const int* __restrict__ out_s,
-                                 const __bf16* __restrict__ x, const int* __restrict__ cand, __bf16* ds_x,
-                                 uint32_t* ds_mask, int* ds_y, int* count, int cap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// Whether the move of list entry i is a training row: accepted, with its target among the
+// candidates (mask: the candidates other than the own block; label: the target's index).
+__device__ __forceinline__ bool log_row(const int* __restrict__ list, int i, const int* __restrict__ assign,
+                                        const uint8_t* __restrict__ stat, const int* __restrict__ out_s,
+                                        const int* __restrict__ cand, uint32_t* mask_out, int* label_out) {
     const int v = list[i];
-    if (stat[v] != 2) return;
+    if (stat[v] != 2) return false;
     const int s = out_s[v], r = assign[v];
     const int* c = cand + (long)i * MLP_CAND;
     int label = -1;
@@ -260,13 +259,41 @@ __global__ void log_moves_kernel(const int* __restrict__ list, int n, const int*
         if (c[j] >= 0 && c[j] != r) mask |= 1u << j;
         if (c[j] == s) label = j;
     }
-    if (label < 0 || !(mask >> label & 1u)) return;
-    const int slot = atomicAdd(count, 1);
+    *mask_out = mask;
+    *label_out = label;
+    return label >= 0 && (mask >> label & 1u);
+}
+
+__global__ void log_flags_kernel(const int* __restrict__ list, int n, const int* __restrict__ assign,
+                                 const uint8_t* __restrict__ stat, const int* __restrict__ out_s,
+                                 const int* __restrict__ cand, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t mask;
+    int label;
+    flag[i] = log_row(list, i, assign, stat, out_s, cand, &mask, &label) ? 1 : 0;
+}
+
+// Rows go to count + (their rank in the list), so the log holds the same rows in the same order
+// on every run; rows past the capacity are dropped.
+__global__ void log_moves_kernel(const int* __restrict__ list, int n, const int* __restrict__ assign,
+                                 const uint8_t* __restrict__ stat, const int* __restrict__ out_s,
+                                 const __bf16* __restrict__ x, const int* __restrict__ cand,
+                                 const int* __restrict__ rank, __bf16* ds_x, uint32_t* ds_mask, int* ds_y,
+                                 const int* __restrict__ count, int cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t mask;
+    int label;
+    if (!log_row(list, i, assign, stat, out_s, cand, &mask, &label)) return;
+    const int slot = count[0] + rank[i];
     if (slot >= cap) return;
     for (int j = 0; j < MLP_IN; ++j) ds_x[(long)slot * MLP_IN + j] = x[(long)i * MLP_IN + j];
     ds_mask[slot] = mask;
     ds_y[slot] = label;
 }
+
+__global__ void log_advance_kernel(int* count, const int* __restrict__ rank, int n) { count[0] += rank[n]; }
 
 // ------------------------------------------------------------------------------------------
 static FeatParams feat_params(Ctx& c, const int* list, int n) {
@@ -293,8 +320,21 @@ void learned_features(Ctx& c, const int* list, int n) {
 void learned_log(Ctx& c, const int* list, int n) {
     MlpState& m = c.mlp;
     if (n <= 0 || !m.logging) return;
-    log_moves_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(list, n, c.bm.assign, c.ev_acc, c.ev_s, m.x, m.cand,
-                                                           m.ds_x, m.ds_mask, m.ds_y, m.ds_count, m.ds_cap);
+    if ((size_t)n + 1 > m.log_slots) {
+        if (m.log_flag) SBPX_HIP(hipFree(m.log_flag));
+        if (m.log_rank) SBPX_HIP(hipFree(m.log_rank));
+        m.log_flag = m.log_rank = nullptr;
+        m.log_slots = 0;
+        SBPX_HIP(hipMalloc((void**)&m.log_flag, sizeof(int) * ((size_t)n + 1)));
+        SBPX_HIP(hipMalloc((void**)&m.log_rank, sizeof(int) * ((size_t)n + 1)));
+        m.log_slots = (size_t)n + 1;
+    }
+    const int nb = (n + 255) / 256;
+    log_flags_kernel<<<nb, 256, 0, c.stream>>>(list, n, c.bm.assign, c.ev_acc, c.ev_s, m.cand, m.log_flag);
+    scan_exclusive(m.log_flag, m.log_rank, n, c.scan_tmp, c.stream);
+    log_moves_kernel<<<nb, 256, 0, c.stream>>>(list, n, c.bm.assign, c.ev_acc, c.ev_s, m.x, m.cand, m.log_rank,
+                                               m.ds_x, m.ds_mask, m.ds_y, m.ds_count, m.ds_cap);
+    log_advance_kernel<<<1, 1, 0, c.stream>>>(m.ds_count, m.log_rank, n);
     SBPX_HIP(hipGetLastError());
 }
 

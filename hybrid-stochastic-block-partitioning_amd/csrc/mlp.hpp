@@ -48,6 +48,8 @@ struct MlpState {
     uint32_t* ds_mask = nullptr;
     int* ds_y = nullptr;
     int* ds_count = nullptr;  // device counter
+    int *log_flag = nullptr, *log_rank = nullptr;  // [log_slots] rows of a batch and their ranks
+    size_t log_slots = 0;
     int ds_cap = 0;
     bool logging = false;
     // sweep-time buffers, [N] rows

@@ -160,3 +160,31 @@ def test_trained_network_gains_over_plain_proposal():
             final[proposal] = c.mdl()
         print("B", B0, "plain", final[0], "learned", final[1])
         assert final[1] < final[0]
+
+
+def test_training_is_reproducible():
+    """Logged rows keep the order of the vertices they come from, and minibatches are drawn from a
+    seeded stream, so two contexts that log and train the same way end with the same network."""
+    m = sbpx()
+    n = 3000
+    params = []
+    for _ in range(2):
+        c = m.Context(0)
+        c.learned_init(2)
+        c.learned_log(200_000)
+        c.upload_graph(*_graph(n, 3))
+        c.split()
+        c.build(n, np.arange(n, dtype=np.int32))
+        B = n
+        for call in range(3):
+            c.merge_blocks(B // 2, 3, call)
+            B -= B // 2
+            for i in range(4):
+                c.sweep(m.ALGO_ASYNC_GIBBS, 1, 3, 10 * call + i)
+        c.learned_log(0)
+        count = c.learned_log_count()
+        c.learned_train(30, 1024, 1e-3, 9)
+        params.append((count, c.learned_params()))
+        c.close()
+    assert params[0][0] == params[1][0] > 200
+    assert np.array_equal(params[0][1], params[1][1])

@@ -216,8 +216,10 @@ void init_mlp(Ctx& c, uint64_t seed) {
     c.sync();
 }
 
+}  // namespace
+
 // Per-vertex buffers of the learned sweeps ([N] rows).
-void ensure_mlp_rows(Ctx& c) {
+void sbpx::ensure_mlp_rows(Ctx& c) {
     MlpState& m = c.mlp;
     const size_t N = (size_t)std::max(c.g.N, 1);
     if (m.rows >= (int)N) return;
@@ -235,6 +237,8 @@ void ensure_mlp_rows(Ctx& c) {
     dalloc(&m.pu, N);
     m.rows = (int)N;
 }
+
+namespace {
 
 int bin_of(int degree) {
     for (int b = 0; b < NUM_LDS_BINS; ++b)
@@ -494,7 +498,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
         if (option == 1) {
             ctx->count_bytes = value != 0;
         } else if (option == 4) {
-            if (value != 0 && value != 1) throw ArgError("proposal must be 0 (plain) or 1 (learned mixture)");
+            if (value < 0 || value > 2)
+                throw ArgError("proposal must be 0 (plain), 1 (learned mixture in the batched sweeps) or 2 (also in "
+                               "the sequential pass)");
             ctx->c.proposal = (int)value;
         } else if (option == 3) {
             if (value < 0 || value > 3) throw ArgError("evaluation kernel variant must be 0 (tuned), 1, 2 or 3");
@@ -934,7 +940,7 @@ sbpx_status sbpx_sweep(sbpx_ctx* ctx, int algo, int32_t batches, uint64_t seed, 
         else if (algo == SBPX_ALGO_HYBRID)
             mh_sequential(c, c.mh.d_high, (int)c.g.high.size(), c.mh.max_deg_high, seed, call, TAG_MH, mh_bytes,
                           true);
-        const bool learned = c.proposal == 1;
+        const bool learned = c.proposal >= 1;
         if (learned && !c.mlp.ready) throw StateError("learned proposal selected but no network initialised");
         if ((learned || c.mlp.logging) && !c.mlp.rows) ensure_mlp_rows(c);
         for (const BinnedList& bl : c.plan.batches_async) {

@@ -23,6 +23,7 @@
 #include <climits>
 
 #include "device_common.hpp"
+#include "mlp.hpp"
 
 namespace sbpx {
 
@@ -289,6 +290,27 @@ __device__ __forceinline__ int tab_find(const int* key, int cap, int b) {
 
 __device__ __forceinline__ float fi(int x) { return (float)x; }
 
+// Softmax probability of candidate `pick` among the candidates other than `own` (-1 when no
+// candidate qualifies; 0 when pick is not one of them).
+__device__ inline float cand_prob(const float* z, const int* cand, int own, int pick) {
+    float mx = -1e30f;
+    int any = 0;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            mx = fmaxf(mx, z[j]);
+            any = 1;
+        }
+    if (!any) return -1.0f;
+    float den = 0.0f, num = 0.0f;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            const float e = expf(z[j] - mx);
+            den += e;
+            if (cand[j] == pick) num = e;
+        }
+    return num / den;
+}
+
 // Table layout: 7 arrays of cap ints: key, o, i, M[s,b], M[b,s], M[r,b], M[b,r].
 // given_s >= 0 fixes the proposal (with uniform given_u); propose_only stops after step 2.
 //
@@ -300,12 +322,16 @@ __device__ __forceinline__ float fi(int x) { return (float)x; }
 // U: slots per lane per trip of the row scans and the Hastings pass (loads in flight per lane).
 // lw >= 0: the proposal was the learned mixture w * plain + (1 - w) * net with w = lw; lqf / lqb
 // are the net's forward and reverse probabilities (negative: no learned mass in that state).
+// With lz / lcand (the vertex's candidate scores and blocks) they are computed here once the
+// proposal is known instead: the net's part is then the softmax of those fixed scores over the
+// candidates other than the vertex's current block, in both directions.
 // W: lanes per vertex (a group of W consecutive lanes; 64 = the whole wavefront).
 template <class View, int U = 2, int W = WAVE>
 __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockState& S, const View& M, int v,
                                                 int* tab, int cap, uint64_t seed, uint32_t call, uint32_t tag,
                                                 int given_s, double given_u, bool propose_only, double lw = -1.0,
-                                                double lqf = 0.0, double lqb = 0.0) {
+                                                double lqf = 0.0, double lqb = 0.0, const float* lz = nullptr,
+                                                const int* lcand = nullptr) {
     const int lane = group_lane<W>();
     MoveResult R{};
     R.v = v;
@@ -406,6 +432,10 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     }
     R.s = s;
     R.u = u;
+    if (lw >= 0.0 && lz != nullptr) {
+        lqf = cand_prob(lz, lcand, r, s);
+        lqb = cand_prob(lz, lcand, s, r);
+    }
     R.H = 1.0;
     R.dS = 0.0;
     if (propose_only || s == r) {

@@ -383,6 +383,9 @@ void finish_degrees(Ctx& c);
 void learned_features(Ctx& c, const int* list, int n);
 void learned_log(Ctx& c, const int* list, int n);
 void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call);
+// Scores the queue of a sequential pass once and draws each vertex's learned branch.
+void learned_mh_draws(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call, uint32_t tag);
+void ensure_mlp_rows(Ctx& c);
 void mlp_forward(Ctx& c, const __bf16* X, int V, __bf16* H1, __bf16* H2, float* Z);
 void mlp_backward(Ctx& c, const __bf16* X, const uint32_t* cmask, const int* y, int V, float* loss,
                   float* grads[3], float* gbias[3]);

@@ -134,27 +134,6 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
         f2bf(feature(lane, cand, cnt, k_out, k_in, r, din_of, dout_of, P.inv_logE, P.log_ratio));
 }
 
-// Softmax probability of candidate `pick` among the candidates other than `own` (-1 when no
-// candidate qualifies; 0 when pick is not one of them).
-__device__ float cand_prob(const float* z, const int* cand, int own, int pick) {
-    float mx = -1e30f;
-    int any = 0;
-    for (int j = 0; j < MLP_CAND; ++j)
-        if (cand[j] >= 0 && cand[j] != own) {
-            mx = fmaxf(mx, z[j]);
-            any = 1;
-        }
-    if (!any) return -1.0f;
-    float den = 0.0f, num = 0.0f;
-    for (int j = 0; j < MLP_CAND; ++j)
-        if (cand[j] >= 0 && cand[j] != own) {
-            const float e = expf(z[j] - mx);
-            den += e;
-            if (cand[j] == pick) num = e;
-        }
-    return num / den;
-}
-
 struct MixParams {
     GraphView G;
     BlockState S;
@@ -231,6 +210,61 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) mixture_kernel(MixParams P)
     };
     P.x2[(long)i * MLP_IN + lane] =
         f2bf(feature(lane, cand, cnt, k_out, k_in, s, din_of, dout_of, P.inv_logE, P.log_ratio));
+}
+
+// The learned part for a sequential pass, scores fixed for the pass: per queue vertex the
+// branch and, on the learned branch, the candidate drawn and the accept uniform of the plain
+// branch's stream. ps[v] = -2: no learned part (degree above LEARN_MAXDEG); -1: the plain branch,
+// drawn during the pass; else the target.
+__global__ void mh_mixture_kernel(const int* __restrict__ list, int n, GraphView G, const int* __restrict__ assign,
+                                  const float* __restrict__ z, const int* __restrict__ cand, uint64_t seed,
+                                  uint32_t call, uint32_t tag, float w_plain, int* __restrict__ ps,
+                                  double* __restrict__ pu) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = list[i];
+    const int deg = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
+    if (deg > LEARN_MAXDEG) {
+        ps[v] = -2;
+        return;
+    }
+    const float* zi = z + (long)i * MLP_OUT;
+    const int* ci = cand + (long)i * MLP_CAND;
+    const int r = assign[v];
+    const Draw4 dl = philox4x32_10(seed, (uint32_t)v, call, 0u, TAG_LEARN);
+    if (cand_prob(zi, ci, r, -1) < 0.0f || unit53(dl.x0, dl.x1) < (double)w_plain) {
+        ps[v] = -1;
+        return;
+    }
+    float mx = -1e30f;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (ci[j] >= 0 && ci[j] != r) mx = fmaxf(mx, zi[j]);
+    float den = 0.0f;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (ci[j] >= 0 && ci[j] != r) den += expf(zi[j] - mx);
+    const float target = (float)unit53(dl.x2, dl.x3) * den;
+    float run = 0.0f;
+    int pick = -1;
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (ci[j] >= 0 && ci[j] != r) {
+            run += expf(zi[j] - mx);
+            pick = ci[j];
+            if (run > target) break;
+        }
+    const Draw4 dr = philox4x32_10(seed, (uint32_t)v, call, 0u, tag);
+    ps[v] = pick;
+    pu[v] = unit53(dr.x2, dr.x3);
+}
+
+void learned_mh_draws(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call, uint32_t tag) {
+    if (n <= 0) return;
+    MlpState& m = c.mlp;
+    learned_features(c, list, n);
+    mlp_forward(c, m.x, n, m.sh1, m.sh2, m.sz);
+    const GraphView G{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
+    mh_mixture_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(list, n, G, c.bm.assign, m.sz, m.cand, seed, call, tag,
+                                                            LEARN_W_PLAIN, m.ps, m.pu);
+    SBPX_HIP(hipGetLastError());
 }
 
 // Learned reverse probability q_net'(r | v in s) from the scores after the move.

@@ -122,7 +122,13 @@ struct MhParams {
     WideSlot* wide;                 // [16 K] one record per window entry
     unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
     int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
-                                    // [2] next queue position, [3] stop reason
+                                    // [2] next queue position, [3] stop reason    // the learned mixture (null: plain proposals): per queue entry the candidate scores and blocks,
+    // per vertex the learned draw (-2: plain only, -1: the plain branch, else the target) and its
+    // accept uniform
+    const float* lz;
+    const int* lcand;
+    const int* lps;
+    const double* lpu;
 };
 
 __device__ __forceinline__ int room(const int* capa, const int* occ, int x) { return (capa[x] * 3) / 4 - occ[x]; }
@@ -830,7 +836,16 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             const int v = P.list[pos + slot];
             cap = vertex_table_cap(G, v);
             if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
-            R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+            if (P.lps != nullptr) {  // the learned mixture: scores fixed for the pass
+                const int lp = P.lps[v];
+                const double lw = lp >= -1 ? (double)LEARN_W_PLAIN : -1.0;
+                const long row = pos + slot;
+                R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, lp >= 0 ? lp : -1,
+                                                   lp >= 0 ? P.lpu[v] : 0.0, false, lw, 0.0, 0.0,
+                                                   P.lz + row * MLP_OUT, P.lcand + row * MLP_CAND);
+            } else {
+                R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+            }
             if (R.status == 2 && tab != gtab) {  // the commit and the other checks read it from memory
                 for (int j = lane; j < 7 * cap; j += WAVE) gtab[j] = tab[j];
             }
@@ -1423,6 +1438,14 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
         h.table_slots = tslots;
     }
+    // the learned mixture: the queue's candidate scores, fixed for the pass, and each vertex's branch
+    const bool learned = c.proposal == 2;
+    if (learned) {
+        if (!c.mlp.ready) throw StateError("learned proposal selected but no network initialised");
+        if (c.mh_wide == 1) throw StateError("the learned sequential pass runs in the window-check kernel (option 6 != 1)");
+        ensure_mlp_rows(c);
+        learned_mh_draws(c, d_list, n, seed, call, tag);
+    }
     if (!h.assign0) SBPX_HIP(hipMalloc((void**)&h.assign0, sizeof(int) * std::max(c.g.N, 1)));
     SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToDevice, c.stream));
     int pos = 0;
@@ -1468,6 +1491,10 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.wide = (WideSlot*)h.wide;
         P.wstamp = h.wstamp;
         P.ctrl = h.ctrl;
+        P.lz = learned ? c.mlp.sz : nullptr;
+        P.lcand = learned ? c.mlp.cand : nullptr;
+        P.lps = learned ? c.mlp.ps : nullptr;
+        P.lpu = learned ? c.mlp.pu : nullptr;
         bool out_of_room = false;
         // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
         // previous chunk: the mean number of row/column slots its evaluations scanned (long scans
@@ -1477,7 +1504,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // each finish a window sooner). The first chunk at a new number of blocks goes by the
         // longest row instead.
         while (pos < n) {
-            int G = c.mh_group;
+            int G = learned ? 1 : c.mh_group;  // the learned mixture: one wavefront per vertex
             if (G == 0) {
                 if (h.last_B != B || h.mean_scan < 0.0) {
                     G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : 1);

@@ -188,3 +188,34 @@ def test_training_is_reproducible():
         c.close()
     assert params[0][0] == params[1][0] > 200
     assert np.array_equal(params[0][1], params[1][1])
+
+
+@pytest.mark.parametrize("algo", ["hybrid", "mh"])
+def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo):
+    """Proposal 2: the sequential pass proposes from the mixture too (the network's scores of each
+    queue vertex's candidates fixed for the pass). Every move is still judged by delta MDL and
+    applied as in the plain pass; the state matches a rebuild from the assignment."""
+    m = sbpx()
+    n = 3000
+    arrays = _graph(n, 12)
+    c = m.Context(0)
+    c.upload_graph(*arrays)
+    c.split()
+    c.learned_init(6)
+    g = O.Graph(n, *arrays)
+    _merge_to(c, n, 300, 4)
+    c.set_option(m.OPT_PROPOSAL, 2)
+    code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
+    moves = 0
+    from test_gpu_parity import assert_same_state
+    for call in range(3):
+        st = c.sweep(code, 1, 31, call)
+        assert st.attempts == n
+        moves += st.moves
+        bm = O.Blockmodel(g, c.info()[0], c.state()[0])
+        assert_same_state(c, bm, check_deg=False)
+        assert np.isfinite(c.mdl())
+    assert moves > 0
+    with pytest.raises(m.SbpxError):  # the learned sequential pass needs the window-check kernel
+        c.set_option(m.OPT_MH_WIDE, 1)
+        c.sweep(code, 1, 31, 9)

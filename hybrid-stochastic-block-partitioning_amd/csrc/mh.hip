@@ -1504,7 +1504,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // each finish a window sooner). The first chunk at a new number of blocks goes by the
         // longest row instead.
         while (pos < n) {
-            int G = learned ? 1 : c.mh_group;  // the learned mixture: one wavefront per vertex
+            int G = c.mh_group;
             if (G == 0) {
                 if (h.last_B != B || h.mean_scan < 0.0) {
                     G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : 1);
@@ -1518,6 +1518,13 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (G == 1 && wide) K = c.mh_wide ? c.mh_wide : h.k_yield;
             // one workgroup, one wavefront per vertex: the window checks of the wide kernel
             const bool wide1 = K == 1 && G == 1 && wide && c.mh_wide == 0;
+            // the learned mixture runs where one wavefront evaluates each vertex; chunks whose rows
+            // call for wavefront groups use the plain proposal (each step keeps its own valid kernel)
+            const bool lchunk = learned && G == 1;
+            P.lz = lchunk ? c.mlp.sz : nullptr;
+            P.lcand = lchunk ? c.mlp.cand : nullptr;
+            P.lps = lchunk ? c.mlp.ps : nullptr;
+            P.lpu = lchunk ? c.mlp.pu : nullptr;
             P.start = pos;
             P.stop_at = std::min(n, pos + MH_CHUNK);
             SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));

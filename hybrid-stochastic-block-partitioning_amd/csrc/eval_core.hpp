@@ -295,6 +295,7 @@ __device__ __forceinline__ float fi(int x) { return (float)x; }
 __device__ inline float cand_prob(const float* z, const int* cand, int own, int pick) {
     float mx = -1e30f;
     int any = 0;
+#pragma unroll 1
     for (int j = 0; j < MLP_CAND; ++j)
         if (cand[j] >= 0 && cand[j] != own) {
             mx = fmaxf(mx, z[j]);
@@ -302,6 +303,7 @@ __device__ inline float cand_prob(const float* z, const int* cand, int own, int 
         }
     if (!any) return -1.0f;
     float den = 0.0f, num = 0.0f;
+#pragma unroll 1
     for (int j = 0; j < MLP_CAND; ++j)
         if (cand[j] >= 0 && cand[j] != own) {
             const float e = expf(z[j] - mx);

@@ -796,6 +796,10 @@ __global__ void wide_init_kernel(int* ctrl) {
     ctrl[3] = 0;
 }
 
+// LEARNED: the queue's proposals are the learned mixture (P.lz, P.lcand, P.lps, P.lpu set). A
+// separate instantiation keeps the plain kernel free of the second inlined evaluator, whose
+// registers would otherwise spill in the plain pass too.
+template <bool LEARNED>
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     __shared__ int s_ok, s_flag, s_n, s_end, s_stop, s_need;
     __shared__ CommitInfo s_commit;
@@ -836,7 +840,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             const int v = P.list[pos + slot];
             cap = vertex_table_cap(G, v);
             if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
-            if (P.lps != nullptr) {  // the learned mixture: scores fixed for the pass
+            if constexpr (LEARNED) {  // the learned mixture: scores fixed for the pass
                 const int lp = P.lps[v];
                 const double lw = lp >= -1 ? (double)LEARN_W_PLAIN : -1.0;
                 const long row = pos + slot;
@@ -1532,10 +1536,13 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (timed) c.timer_begin();
             if (K > 1) {
                 void* args[] = {&P};
-                SBPX_HIP(hipLaunchCooperativeKernel((const void*)mh_wide_kernel, dim3(K), dim3(MH_WAVES * WAVE), args, 0,
-                                                    c.stream));
+                const void* kern = lchunk ? (const void*)mh_wide_kernel<true> : (const void*)mh_wide_kernel<false>;
+                SBPX_HIP(hipLaunchCooperativeKernel(kern, dim3(K), dim3(MH_WAVES * WAVE), args, 0, c.stream));
             } else if (wide1) {
-                mh_wide_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
+                if (lchunk)
+                    mh_wide_kernel<true><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
+                else
+                    mh_wide_kernel<false><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
             } else {
                 switch (G) {
                     case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;

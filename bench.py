@@ -266,6 +266,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true", help="skip the untimed byte-counting run")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal comparison")
+    ap.add_argument("--proposal", type=int, default=2, choices=[1, 2, 3],
+                    help="learned mixture in: 1 the batched sweeps, 2 also the sequential pass's one-wavefront "
+                         "chunks, 3 also its wavefront-group chunks")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batched workload (--config C3)")
     ap.add_argument("--sample", type=int, default=12, help="C3: graphs also run alone (GPU) and by the oracle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -393,7 +396,7 @@ def main():
         params, info = train_learned(m, cfg, seeds=(args.seed + 1, args.seed + 2))
         g.load_learned(params)
         t0 = time.perf_counter()
-        la, lst = g.partition(learned=2, **kw)
+        la, lst = g.partition(learned=args.proposal, **kw)
         tl = time.perf_counter() - t0
         plain_t = float(np.mean(times))
         # time at equal MDL: when the learned run's outer loop first reached the plain run's final MDL
@@ -402,8 +405,11 @@ def main():
         t_equal = float(lt[hit[0]]) if hit.size else None
         out["learned"] = {
             "proposal": "0.2 * plain + 0.8 * network (Hastings-corrected) for vertices of degree <= 126: in the "
-                        "batched sweeps, and in the sequential Metropolis-Hastings pass with candidate scores "
-                        "fixed per pass (chunks evaluated by wavefront groups keep the plain proposal)",
+                        "batched sweeps" + {1: "", 2: ", and in the sequential Metropolis-Hastings pass with "
+                        "candidate scores fixed per pass (chunks evaluated by wavefront groups keep the plain "
+                        "proposal)", 3: ", and in the whole sequential Metropolis-Hastings pass with candidate "
+                        "scores fixed per pass"}[args.proposal],
+            "proposal_option": args.proposal,
             "time_to_partition_s": tl, "speedup_vs_plain": plain_t / tl,
             "time_to_plain_final_mdl_s": t_equal,
             "speedup_at_equal_mdl": (plain_t / t_equal) if t_equal else None,

@@ -524,7 +524,9 @@ class ResidentGraph:
                   learned=False, checkpoint=None, resume=False, max_iterations=0):
         """sbp::stochastic_block_partition; with `checkpoint`, the driver state is written after every
         outer iteration, `resume` continues from it and `max_iterations` > 0 stops early."""
-        self.set_option(OPT_PROPOSAL, int(learned))  # True / 1: batched sweeps; 2: also the sequential pass
+        # True / 1: batched sweeps; 2: also the sequential pass (one wavefront per vertex); 3: also
+        # its wavefront groups
+        self.set_option(OPT_PROPOSAL, int(learned))
         a = np.zeros(self.N, np.int32)
         st = RunStats()
         err = C.create_string_buffer(512)

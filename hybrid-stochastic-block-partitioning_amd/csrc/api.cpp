@@ -498,9 +498,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
         if (option == 1) {
             ctx->count_bytes = value != 0;
         } else if (option == 4) {
-            if (value < 0 || value > 2)
-                throw ArgError("proposal must be 0 (plain), 1 (learned mixture in the batched sweeps) or 2 (also in "
-                               "the sequential pass)");
+            if (value < 0 || value > 3)
+                throw ArgError("proposal must be 0 (plain), 1 (learned mixture in the batched sweeps), 2 (also in "
+                               "the sequential pass, one wavefront per vertex) or 3 (also its wavefront groups)");
             ctx->c.proposal = (int)value;
         } else if (option == 3) {
             if (value < 0 || value > 3) throw ArgError("evaluation kernel variant must be 0 (tuned), 1, 2 or 3");

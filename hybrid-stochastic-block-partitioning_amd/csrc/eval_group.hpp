@@ -11,14 +11,16 @@ namespace sbpx {
 // included), so the barriers never depend on the data. The neighbour-block table is shared by
 // the group (LDS or global memory) and partial results meet in LDS. Results equal those of
 // eval_move: the same cells are visited and the same terms formed; only the order of the FP64
-// sums differs.
+// sums differs. LRN with lw >= 0 and lz / lcand: the proposal was the learned mixture, as in
+// eval_move (given_s / given_u carry the learned branch's draw).
 constexpr int RED_D = 4;   // per-wavefront double partials: dS, pf, pb
 constexpr int RED_I = 12;  // per-wavefront int partials
 
-template <int G, class View>
+template <int G, class View, bool LRN = false>
 __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const View& M, int v, int* tab, int cap,
                                  uint64_t seed, uint32_t call, uint32_t tag, int given_s, double given_u,
-                                 bool propose_only, int* hist, double (*redd)[RED_D], int (*redi)[RED_I]) {
+                                 bool propose_only, int* hist, double (*redd)[RED_D], int (*redi)[RED_I],
+                                 double lw = -1.0, const float* lz = nullptr, const int* lcand = nullptr) {
     constexpr int GS = G * WAVE;
     const int lane = lane_id();
     const int wave = threadIdx.x / WAVE;
@@ -205,6 +207,11 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
     }
     R.s = s;
     R.u = u;
+    double lqf = 0.0, lqb = 0.0;
+    if (LRN && act && lw >= 0.0 && lz != nullptr) {
+        lqf = cand_prob(lz, lcand, r, s);
+        lqb = cand_prob(lz, lcand, s, r);
+    }
     R.H = 1.0;
     R.dS = 0.0;
     if (propose_only) {  // uniform over the launch
@@ -431,7 +438,15 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         R.status = 0;
         return R;
     }
-    const double H = (k == 0) ? 1.0 : pb / pf;
+    double H = (k == 0) ? 1.0 : pb / pf;
+    if (LRN && lw >= 0.0 && k > 0) {
+        // mixture proposal: plain part by the closed form per unit of degree, learned part from the
+        // fixed candidate scores
+        const double plain_f = pf / (double)k, plain_b = pb / (double)k;
+        const double fwd = lw * plain_f + (1.0 - lw) * (lqf >= 0.0 ? lqf : plain_f);
+        const double bwd = lw * plain_b + (1.0 - lw) * (lqb >= 0.0 ? lqb : plain_b);
+        H = bwd / fwd;
+    }
     double a = exp(-3.0 * dS) * H;
     a = (a >= 1.0) ? 1.0 : a;
     R.dS = dS;

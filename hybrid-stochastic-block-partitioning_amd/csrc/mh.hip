@@ -593,7 +593,8 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     }
 }
 
-template <int G>
+// LEARNED: the learned mixture of the pass (as in mh_wide_kernel).
+template <int G, bool LEARNED>
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     constexpr int NG = MH_WAVES / G;
     constexpr int LCAP = MH_WAVES * MH_LDS_CAP / NG;  // LDS table slots per group
@@ -628,8 +629,18 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
         const int v = active ? P.list[idx] : -1;
         const int cap = active ? vertex_table_cap(Gr, v) : 16;
         int* tab = cap <= LCAP ? lds_tab[g] : gtab;
-        MoveResult R = eval_group<G, HashView>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false,
-                                                lds_hist[g], redd, redi);
+        MoveResult R;
+        if constexpr (LEARNED) {  // scores fixed for the pass; the draw made with them
+            const int lp = active ? P.lps[v] : -2;
+            const double lw = lp >= -1 ? (double)LEARN_W_PLAIN : -1.0;
+            R = eval_group<G, HashView, true>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, lp >= 0 ? lp : -1,
+                                        lp >= 0 ? P.lpu[v] : 0.0, false, lds_hist[g], redd, redi, lw,
+                                        active ? P.lz + (long)idx * MLP_OUT : nullptr,
+                                        active ? P.lcand + (long)idx * MLP_CAND : nullptr);
+        } else {
+            R = eval_group<G, HashView>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g], redd,
+                                        redi);
+        }
         if (leader && lane == 0) {
             st[g] = active ? R.status : 3;
             holds[g] = 1;
@@ -1420,6 +1431,15 @@ void merge_heap_select(Ctx& c, int num_to_merge, const double* dS, const int32_t
     SBPX_HIP(hipFree(d_parent));
 }
 
+// One chunk of the sequential pass on one workgroup with G wavefronts per vertex.
+template <int G>
+static void launch_group(const MhParams& P, bool learned, hipStream_t st) {
+    if (learned)
+        mh_group_kernel<G, true><<<1, MH_WAVES * WAVE, 0, st>>>(P);
+    else
+        mh_group_kernel<G, false><<<1, MH_WAVES * WAVE, 0, st>>>(P);
+}
+
 void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,
                    long long* bytes, bool timed) {
     if (n <= 0) return;
@@ -1443,7 +1463,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         h.table_slots = tslots;
     }
     // the learned mixture: the queue's candidate scores, fixed for the pass, and each vertex's branch
-    const bool learned = c.proposal == 2;
+    const bool learned = c.proposal >= 2;
     if (learned) {
         if (!c.mlp.ready) throw StateError("learned proposal selected but no network initialised");
         if (c.mh_wide == 1) throw StateError("the learned sequential pass runs in the window-check kernel (option 6 != 1)");
@@ -1522,9 +1542,11 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (G == 1 && wide) K = c.mh_wide ? c.mh_wide : h.k_yield;
             // one workgroup, one wavefront per vertex: the window checks of the wide kernel
             const bool wide1 = K == 1 && G == 1 && wide && c.mh_wide == 0;
-            // the learned mixture runs where one wavefront evaluates each vertex; chunks whose rows
-            // call for wavefront groups use the plain proposal (each step keeps its own valid kernel)
-            const bool lchunk = learned && G == 1;
+            // the learned mixture: the window-check kernel for one wavefront per vertex (proposal 2
+            // and 3; mh_kernel does not run, the learned pass needs option 6 != 1), the group kernels
+            // for wavefront groups (proposal 3; with 2 those chunks keep the plain proposal, each
+            // step of the chain a valid MH step either way)
+            const bool lchunk = learned && (G == 1 || c.proposal == 3);
             P.lz = lchunk ? c.mlp.sz : nullptr;
             P.lcand = lchunk ? c.mlp.cand : nullptr;
             P.lps = lchunk ? c.mlp.ps : nullptr;
@@ -1544,12 +1566,13 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                 else
                     mh_wide_kernel<false><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
             } else {
+                if (lchunk && G == 1) throw StateError("learned sequential pass without the window-check kernel");
                 switch (G) {
                     case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                    case 2: mh_group_kernel<2><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                    case 4: mh_group_kernel<4><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                    case 8: mh_group_kernel<8><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                    default: mh_group_kernel<16><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
+                    case 2: launch_group<2>(P, lchunk, c.stream); break;
+                    case 4: launch_group<4>(P, lchunk, c.stream); break;
+                    case 8: launch_group<8>(P, lchunk, c.stream); break;
+                    default: launch_group<16>(P, lchunk, c.stream); break;
                 }
             }
             SBPX_HIP(hipGetLastError());

@@ -72,7 +72,9 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            in its last bits between them.
  * option 4 = proposal: 0 the reference's, 1 the learned mixture in the batched evaluation
  *            (see sbpx_learned_init), 2 also in the sequential Metropolis-Hastings pass (there the
- *            network scores each queue vertex's candidates once per pass; needs option 6 != 1).
+ *            network scores each queue vertex's candidates once per pass; needs option 6 != 1)
+ *            where one wavefront evaluates each vertex, 3 also in the pass's chunks that
+ *            wavefront groups evaluate (option 2 > 1 or chosen by row length).
  * option 5 = block matrix after a batch of moves: 1 (default) updated from the moved vertices'
  *            edges when they are at most a quarter of the edges, else rebuilt by a full sort;
  *            0 always rebuilt. Both give the same arrays.

@@ -190,11 +190,13 @@ def test_training_is_reproducible():
     assert np.array_equal(params[0][1], params[1][1])
 
 
-@pytest.mark.parametrize("algo", ["hybrid", "mh"])
-def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo):
-    """Proposal 2: the sequential pass proposes from the mixture too (the network's scores of each
-    queue vertex's candidates fixed for the pass). Every move is still judged by delta MDL and
-    applied as in the plain pass; the state matches a rebuild from the assignment."""
+@pytest.mark.parametrize("algo,group,proposal", [("hybrid", 0, 2), ("mh", 0, 2), ("mh", 4, 2), ("mh", 4, 3),
+                                                 ("hybrid", 16, 3), ("hybrid", 0, 3)])
+def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, proposal):
+    """Proposals 2 and 3: the sequential pass proposes from the mixture too (the network's scores
+    of each queue vertex's candidates fixed for the pass), where one wavefront evaluates each
+    vertex (2) or also in wavefront groups (3). Every move is still judged by delta MDL and applied
+    as in the plain pass; the state matches a rebuild from the assignment."""
     m = sbpx()
     n = 3000
     arrays = _graph(n, 12)
@@ -204,7 +206,8 @@ def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo):
     c.learned_init(6)
     g = O.Graph(n, *arrays)
     _merge_to(c, n, 300, 4)
-    c.set_option(m.OPT_PROPOSAL, 2)
+    c.set_option(m.OPT_PROPOSAL, proposal)
+    c.set_option(m.OPT_MH_GROUP, group)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
     moves = 0
     from test_gpu_parity import assert_same_state
@@ -219,3 +222,31 @@ def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo):
     with pytest.raises(m.SbpxError):  # the learned sequential pass needs the window-check kernel
         c.set_option(m.OPT_MH_WIDE, 1)
         c.sweep(code, 1, 31, 9)
+
+
+def test_learned_sequential_pass_same_moves_with_wavefront_groups():
+    """The learned mixture (proposal 3) gives the same chain whether one wavefront or a group of 4
+    or 16 evaluates each vertex: the same proposals (the draws are keyed by vertex and pass), the same
+    Hastings terms from the same candidate scores, and delta MDL equal up to the order of its FP64
+    sums, so every accept decision and the final state agree."""
+    m = sbpx()
+    n = 3000
+    arrays = _graph(n, 12)
+    states = []
+    for group in (1, 4, 16):
+        c = m.Context(0)
+        c.upload_graph(*arrays)
+        c.split()
+        c.learned_init(6)
+        _merge_to(c, n, 300, 4)
+        c.set_option(m.OPT_PROPOSAL, 3)
+        c.set_option(m.OPT_MH_GROUP, group)
+        moves = [c.sweep(m.ALGO_MH, 1, 31, call).moves for call in range(2)]
+        a, dout, din, _ = c.state()
+        states.append((moves, a.copy(), dout.copy(), din.copy(), c.mdl()))
+        c.close()
+    assert states[0][0][0] > 0
+    for other in states[1:]:
+        assert other[0] == states[0][0]
+        assert all(np.array_equal(x, y) for x, y in zip(other[1:4], states[0][1:4]))
+        assert other[4] == pytest.approx(states[0][4], rel=1e-9)

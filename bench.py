@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--proposal", type=int, default=2, choices=[1, 2, 3],
                     help="learned mixture in: 1 the batched sweeps, 2 also the sequential pass's one-wavefront "
                          "chunks, 3 also its wavefront-group chunks")
+    ap.add_argument("--learn-maxdeg", type=int, default=126,
+                    help="largest vertex degree the learned mixture covers in the sequential pass (<= 510)")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batched workload (--config C3)")
     ap.add_argument("--sample", type=int, default=12, help="C3: graphs also run alone (GPU) and by the oracle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -395,6 +397,7 @@ def main():
     if not args.no_learned and args.algorithm != "metropolis_hastings":
         params, info = train_learned(m, cfg, seeds=(args.seed + 1, args.seed + 2))
         g.load_learned(params)
+        g.set_option(m.OPT_LEARN_MH_MAXDEG, args.learn_maxdeg)
         t0 = time.perf_counter()
         la, lst = g.partition(learned=args.proposal, **kw)
         tl = time.perf_counter() - t0
@@ -409,7 +412,7 @@ def main():
                         "candidate scores fixed per pass (chunks evaluated by wavefront groups keep the plain "
                         "proposal)", 3: ", and in the whole sequential Metropolis-Hastings pass with candidate "
                         "scores fixed per pass"}[args.proposal],
-            "proposal_option": args.proposal,
+            "proposal_option": args.proposal, "learn_mh_maxdeg": args.learn_maxdeg,
             "time_to_partition_s": tl, "speedup_vs_plain": plain_t / tl,
             "time_to_plain_final_mdl_s": t_equal,
             "speedup_at_equal_mdl": (plain_t / t_equal) if t_equal else None,

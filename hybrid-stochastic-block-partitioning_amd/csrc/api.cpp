@@ -520,6 +520,10 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
             if (value < 0 || value > (int64_t)1 << 30) throw ArgError("table headroom must be in [0, 2^30] slots");
             ctx->c.mh_headroom = value;
             ctx->c.mh.headroom = (size_t)value;
+        } else if (option == 8) {
+            if (value < 0 || value > LEARN_MH_MAXDEG)
+                throw ArgError("learned degree bound of the sequential pass must be in [0, 510]");
+            ctx->c.learn_mh_maxdeg = (int)value;
         } else {
             throw ArgError("unknown option");
         }

@@ -54,9 +54,11 @@ constexpr int NUM_LDS_BINS = 5;
 constexpr int BIN_MAXDEG[NUM_LDS_BINS] = {30, 62, 126, 254, 510};
 constexpr int BIN_CAP[NUM_LDS_BINS] = {64, 128, 256, 512, 1024};
 constexpr int GROUP_MAXDEG = 2047;
-// The learned proposal covers the vertices of the first three bins (degree <= 126); the mixture
-// keeps the plain proposal with this weight.
+// The learned proposal covers the vertices of the first three bins (degree <= 126) in the
+// batched sweeps and, in the sequential pass, those up to a bound chosen per context (126 by
+// default, at most LEARN_MH_MAXDEG); the mixture keeps the plain proposal with this weight.
 constexpr int LEARN_MAXDEG = 126;
+constexpr int LEARN_MH_MAXDEG = 510;
 constexpr float LEARN_W_PLAIN = 0.2f;
 constexpr int NUM_BINS = NUM_LDS_BINS + 2;
 // timing / byte classes reported per sweep: 0 = LDS bins with tables up to 256 slots,
@@ -203,6 +205,7 @@ struct Ctx {
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
     int mh_wide = 0;   // workgroups of the Metropolis-Hastings pass; 0 = by window yield
     int64_t mh_headroom = 0;  // initial table pool headroom in slots; 0 = the initial layout (>= 1M)
+    int learn_mh_maxdeg = LEARN_MAXDEG;  // largest degree the learned mixture covers in the sequential pass
 
     // scratch
     void* keys0 = nullptr;
@@ -380,11 +383,12 @@ void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
 void finish_degrees(Ctx& c);
 // learned.hip / mlp.hip
-void learned_features(Ctx& c, const int* list, int n);
+// max_deg: the largest degree in the list (a larger neighbour-block table above LEARN_MAXDEG).
+void learned_features(Ctx& c, const int* list, int n, int max_deg = LEARN_MAXDEG);
 void learned_log(Ctx& c, const int* list, int n);
 void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call);
 // Scores the queue of a sequential pass once and draws each vertex's learned branch.
-void learned_mh_draws(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call, uint32_t tag);
+void learned_mh_draws(Ctx& c, const int* list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag);
 void ensure_mlp_rows(Ctx& c);
 void mlp_forward(Ctx& c, const __bf16* X, int V, __bf16* H1, __bf16* H2, float* Z);
 void mlp_backward(Ctx& c, const __bf16* X, const uint32_t* cmask, const int* y, int V, float* loss,

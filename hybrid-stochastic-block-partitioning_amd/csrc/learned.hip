@@ -24,7 +24,10 @@
 namespace sbpx {
 
 constexpr int FEAT_WAVES = 4;
-constexpr int FEAT_CAP = 256;  // neighbour-block table per wavefront (degree <= LEARN_MAXDEG)
+// neighbour-block table per wavefront: 256 slots for degree <= LEARN_MAXDEG, 1024 for degree
+// <= LEARN_MH_MAXDEG (at most half full either way)
+constexpr int FEAT_CAP = 256;
+constexpr int FEAT_CAP_MH = 1024;
 
 __device__ __forceinline__ __bf16 f2bf(float f) {
     uint32_t u = __float_as_uint(f);
@@ -72,15 +75,16 @@ __device__ __forceinline__ float feature(int j, const int* cand, const int* cnt,
     }
 }
 
+template <int CAP>
 __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams P) {
-    __shared__ int tkey[FEAT_WAVES][FEAT_CAP];
-    __shared__ int tcnt[FEAT_WAVES][FEAT_CAP];
+    __shared__ int tkey[FEAT_WAVES][CAP];
+    __shared__ int tcnt[FEAT_WAVES][CAP];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
     const int i = blockIdx.x * FEAT_WAVES + wave;
     if (i >= P.n) return;
     int* key = tkey[wave];
     int* cntt = tcnt[wave];
-    for (int j = lane; j < FEAT_CAP; j += WAVE) {
+    for (int j = lane; j < CAP; j += WAVE) {
         key[j] = -1;
         cntt[j] = 0;
     }
@@ -93,7 +97,7 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
     for (int j = lane; j < dO + dI; j += WAVE) {
         const int w = j < dO ? P.G.out_idx[ob + j] : P.G.in_idx[ib + (j - dO)];
         if (w == v) continue;
-        const int slot = tab_insert(key, FEAT_CAP, P.S.assign[w]);
+        const int slot = tab_insert(key, CAP, P.S.assign[w]);
         if (slot >= 0) atomicAdd(&cntt[slot], 1);
     }
     fence_wave();
@@ -102,7 +106,7 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
     int* cnt = P.cnt + (long)i * MLP_CAND;
     for (int t = 0; t < MLP_CAND; ++t) {
         long long best = -1;
-        for (int j = lane; j < FEAT_CAP; j += WAVE) {
+        for (int j = lane; j < CAP; j += WAVE) {
             const int kk = key[j], c = cntt[j];
             if (kk >= 0 && c > 0) {
                 const long long score = ((long long)c << 32) | (long long)(0x7FFFFFFF - kk);
@@ -118,7 +122,7 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
         if (best >= 0) {
             b = 0x7FFFFFFF - (int)(best & 0xFFFFFFFF);
             c = (int)(best >> 32);
-            for (int j = lane; j < FEAT_CAP; j += WAVE)
+            for (int j = lane; j < CAP; j += WAVE)
                 if (key[j] == b) cntt[j] = 0;  // taken
         }
         if (lane == 0) {
@@ -214,17 +218,17 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) mixture_kernel(MixParams P)
 
 // The learned part for a sequential pass, scores fixed for the pass: per queue vertex the
 // branch and, on the learned branch, the candidate drawn and the accept uniform of the plain
-// branch's stream. ps[v] = -2: no learned part (degree above LEARN_MAXDEG); -1: the plain branch,
+// branch's stream. ps[v] = -2: no learned part (degree above maxdeg); -1: the plain branch,
 // drawn during the pass; else the target.
 __global__ void mh_mixture_kernel(const int* __restrict__ list, int n, GraphView G, const int* __restrict__ assign,
                                   const float* __restrict__ z, const int* __restrict__ cand, uint64_t seed,
-                                  uint32_t call, uint32_t tag, float w_plain, int* __restrict__ ps,
+                                  uint32_t call, uint32_t tag, float w_plain, int maxdeg, int* __restrict__ ps,
                                   double* __restrict__ pu) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int v = list[i];
     const int deg = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
-    if (deg > LEARN_MAXDEG) {
+    if (deg > maxdeg) {
         ps[v] = -2;
         return;
     }
@@ -256,14 +260,15 @@ __global__ void mh_mixture_kernel(const int* __restrict__ list, int n, GraphView
     pu[v] = unit53(dr.x2, dr.x3);
 }
 
-void learned_mh_draws(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call, uint32_t tag) {
+void learned_mh_draws(Ctx& c, const int* list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag) {
     if (n <= 0) return;
     MlpState& m = c.mlp;
-    learned_features(c, list, n);
+    const int maxdeg = std::min(c.learn_mh_maxdeg, LEARN_MH_MAXDEG);
+    learned_features(c, list, n, std::min(max_deg, maxdeg));
     mlp_forward(c, m.x, n, m.sh1, m.sh2, m.sz);
     const GraphView G{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
     mh_mixture_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(list, n, G, c.bm.assign, m.sz, m.cand, seed, call, tag,
-                                                            LEARN_W_PLAIN, m.ps, m.pu);
+                                                            LEARN_W_PLAIN, maxdeg, m.ps, m.pu);
     SBPX_HIP(hipGetLastError());
 }
 
@@ -344,10 +349,15 @@ static FeatParams feat_params(Ctx& c, const int* list, int n) {
     return P;
 }
 
-void learned_features(Ctx& c, const int* list, int n) {
+void learned_features(Ctx& c, const int* list, int n, int max_deg) {
     if (n <= 0) return;
     FeatParams P = feat_params(c, list, n);
-    features_kernel<<<(n + FEAT_WAVES - 1) / FEAT_WAVES, FEAT_WAVES * WAVE, 0, c.stream>>>(P);
+    // vertices above the table's degree bound get features from a partial table; their scores
+    // are not used (the mixture gives them no learned part)
+    if (max_deg > LEARN_MAXDEG)
+        features_kernel<FEAT_CAP_MH><<<(n + FEAT_WAVES - 1) / FEAT_WAVES, FEAT_WAVES * WAVE, 0, c.stream>>>(P);
+    else
+        features_kernel<FEAT_CAP><<<(n + FEAT_WAVES - 1) / FEAT_WAVES, FEAT_WAVES * WAVE, 0, c.stream>>>(P);
     SBPX_HIP(hipGetLastError());
 }
 

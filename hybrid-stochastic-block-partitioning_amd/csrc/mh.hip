@@ -1468,7 +1468,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         if (!c.mlp.ready) throw StateError("learned proposal selected but no network initialised");
         if (c.mh_wide == 1) throw StateError("the learned sequential pass runs in the window-check kernel (option 6 != 1)");
         ensure_mlp_rows(c);
-        learned_mh_draws(c, d_list, n, seed, call, tag);
+        learned_mh_draws(c, d_list, n, max_deg, seed, call, tag);
     }
     if (!h.assign0) SBPX_HIP(hipMalloc((void**)&h.assign0, sizeof(int) * std::max(c.g.N, 1)));
     SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToDevice, c.stream));

@@ -190,13 +190,15 @@ def test_training_is_reproducible():
     assert np.array_equal(params[0][1], params[1][1])
 
 
-@pytest.mark.parametrize("algo,group,proposal", [("hybrid", 0, 2), ("mh", 0, 2), ("mh", 4, 2), ("mh", 4, 3),
-                                                 ("hybrid", 16, 3), ("hybrid", 0, 3)])
-def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, proposal):
+@pytest.mark.parametrize("algo,group,proposal,maxdeg", [("hybrid", 0, 2, 126), ("mh", 0, 2, 126), ("mh", 4, 2, 126),
+                                                        ("mh", 4, 3, 126), ("hybrid", 16, 3, 126),
+                                                        ("hybrid", 0, 3, 510), ("mh", 0, 2, 510)])
+def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, proposal, maxdeg):
     """Proposals 2 and 3: the sequential pass proposes from the mixture too (the network's scores
     of each queue vertex's candidates fixed for the pass), where one wavefront evaluates each
-    vertex (2) or also in wavefront groups (3). Every move is still judged by delta MDL and applied
-    as in the plain pass; the state matches a rebuild from the assignment."""
+    vertex (2) or also in wavefront groups (3), for vertices up to degree 126 or 510 (the larger
+    bound builds the features in a 1024-slot table). Every move is still judged by delta MDL and
+    applied as in the plain pass; the state matches a rebuild from the assignment."""
     m = sbpx()
     n = 3000
     arrays = _graph(n, 12)
@@ -208,6 +210,7 @@ def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, pr
     _merge_to(c, n, 300, 4)
     c.set_option(m.OPT_PROPOSAL, proposal)
     c.set_option(m.OPT_MH_GROUP, group)
+    c.set_option(m.OPT_LEARN_MH_MAXDEG, maxdeg)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
     moves = 0
     from test_gpu_parity import assert_same_state

@@ -46,7 +46,7 @@ KERNELS = ["eval_bin_kernel<64..256> (batched vertex-move evaluation, degree <= 
            "mh_kernel and mh_group_kernel<G> on one)"]
 
 
-def train_learned(m, cfg, seeds, log_sweeps=8):
+def train_learned(m, cfg, seeds, log_sweeps=8, steps=600):
     """Trains the proposal network (untimed) on accepted moves logged from plain asynchronous Gibbs
     sweeps on other graphs of the same family (one merge phase at a time, `log_sweeps` sweeps
     each, down to about twice the planted block count)."""
@@ -71,11 +71,11 @@ def train_learned(m, cfg, seeds, log_sweeps=8):
     rows = ctx.learned_log_count()
     t1 = time.perf_counter()
     first = ctx.learned_train(50, 4096, 1e-3, 7)
-    last = ctx.learned_train(550, 4096, 1e-3, 8)
+    last = ctx.learned_train(steps - 50, 4096, 1e-3, 8)
     t2 = time.perf_counter()
     params = ctx.learned_params()
     ctx.close()
-    return params, {"graphs": [int(s) for s in seeds], "rows": rows, "steps": 600, "batch": 4096,
+    return params, {"graphs": [int(s) for s in seeds], "rows": rows, "steps": steps, "batch": 4096,
                     "loss_first": first, "loss_last": last, "log_s": t1 - t0, "train_s": t2 - t1}
 
 
@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--proposal", type=int, default=2, choices=[1, 2, 3],
                     help="learned mixture in: 1 the batched sweeps, 2 also the sequential pass's one-wavefront "
                          "chunks, 3 also its wavefront-group chunks")
+    ap.add_argument("--train-graphs", type=int, default=2, help="graphs the proposal network is trained on")
+    ap.add_argument("--train-steps", type=int, default=600, help="Adam steps of the proposal network (>= 100)")
     ap.add_argument("--learn-maxdeg", type=int, default=126,
                     help="largest vertex degree the learned mixture covers in the sequential pass (<= 510)")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batched workload (--config C3)")
@@ -395,7 +397,8 @@ def main():
             for i in range(4) if launches_per_run[i] > 0],
     }
     if not args.no_learned and args.algorithm != "metropolis_hastings":
-        params, info = train_learned(m, cfg, seeds=(args.seed + 1, args.seed + 2))
+        params, info = train_learned(m, cfg, seeds=tuple(args.seed + 1 + i for i in range(args.train_graphs)),
+                                     steps=args.train_steps)
         g.load_learned(params)
         g.set_option(m.OPT_LEARN_MH_MAXDEG, args.learn_maxdeg)
         t0 = time.perf_counter()

@@ -272,7 +272,7 @@ def main():
     ap.add_argument("--train-graphs", type=int, default=2, help="graphs the proposal network is trained on")
     ap.add_argument("--train-steps", type=int, default=600, help="Adam steps of the proposal network (>= 100)")
     ap.add_argument("--learn-maxdeg", type=int, default=126,
-                    help="largest vertex degree the learned mixture covers in the sequential pass (<= 510)")
+                    help="degree bound of the learned mixture (<= 510; batched bins 3-4 from 254 and 510)")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batched workload (--config C3)")
     ap.add_argument("--sample", type=int, default=12, help="C3: graphs also run alone (GPU) and by the oracle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -400,7 +400,7 @@ def main():
         params, info = train_learned(m, cfg, seeds=tuple(args.seed + 1 + i for i in range(args.train_graphs)),
                                      steps=args.train_steps)
         g.load_learned(params)
-        g.set_option(m.OPT_LEARN_MH_MAXDEG, args.learn_maxdeg)
+        g.set_option(m.OPT_LEARN_MAXDEG, args.learn_maxdeg)
         t0 = time.perf_counter()
         la, lst = g.partition(learned=args.proposal, **kw)
         tl = time.perf_counter() - t0
@@ -415,7 +415,7 @@ def main():
                         "candidate scores fixed per pass (chunks evaluated by wavefront groups keep the plain "
                         "proposal)", 3: ", and in the whole sequential Metropolis-Hastings pass with candidate "
                         "scores fixed per pass"}[args.proposal],
-            "proposal_option": args.proposal, "learn_mh_maxdeg": args.learn_maxdeg,
+            "proposal_option": args.proposal, "learn_maxdeg": args.learn_maxdeg,
             "time_to_partition_s": tl, "speedup_vs_plain": plain_t / tl,
             "time_to_plain_final_mdl_s": t_equal,
             "speedup_at_equal_mdl": (plain_t / t_equal) if t_equal else None,

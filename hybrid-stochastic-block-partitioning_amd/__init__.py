@@ -15,7 +15,7 @@ MERGE_APPROXIMATE, MERGE_ADVANCED = 0, 1
 LEARNED_PARAMS = 90656  # W1[256][64], b1[256], W2[256][256], b2[256], W3[32][256], b3[32]
 OPT_COUNT_BYTES, OPT_MH_GROUP, OPT_EVAL_VARIANT, OPT_PROPOSAL, OPT_INCREMENTAL, OPT_MH_WIDE = 1, 2, 3, 4, 5, 6
 OPT_MH_HEADROOM = 7
-OPT_LEARN_MH_MAXDEG = 8
+OPT_LEARN_MAXDEG = 8
 ALGORITHMS = {"metropolis_hastings": ALGO_MH, "async_gibbs": ALGO_ASYNC_GIBBS, "hybrid_mcmc": ALGO_HYBRID}
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

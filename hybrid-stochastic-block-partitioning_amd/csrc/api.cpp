@@ -317,12 +317,20 @@ double model_term(long N, long E, int B) {
 void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, const int* d_huge_off,
                       uint64_t seed, uint32_t call, uint32_t tag, long long* bytes, bool timed, bool learned = false) {
     Ctx& c = ctx->c;
-    // the vertices the learned proposal covers: bins 0-2, contiguous in the list
+    // the vertices the learned proposal covers: bins 0-2, and the bins 3-4 whose degrees stay
+    // within the context's learned degree bound; contiguous in the list
+    int nb = 3;
+    while (nb < NUM_LDS_BINS && BIN_MAXDEG[nb] <= c.learn_maxdeg) ++nb;
     const int* low = d_list + bl.offset[0];
-    const int nlow = bl.count[0] + bl.count[1] + bl.count[2];
+    int nlow = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (bl.count[b] > 0 && bl.offset[b] != bl.offset[0] + nlow) throw StateError("degree bins not contiguous");
+        nlow += bl.count[b];
+    }
+    const int nlog = bl.count[0] + bl.count[1] + bl.count[2];  // training rows: degree <= 126
     if (learned && nlow > 0) {
         // plain proposals and accept uniforms (vertex-indexed), then the mixture's draws
-        for (int b = 0; b < 3; ++b) {
+        for (int b = 0; b < nb; ++b) {
             if (bl.count[b] == 0) continue;
             EvalLaunch L{};
             L.list = d_list + bl.offset[b];
@@ -336,10 +344,10 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
             L.out_u = c.ev_u;
             launch_eval(c, L);
         }
-        learned_features(c, low, nlow);
+        learned_features(c, low, nlow, BIN_MAXDEG[nb - 1]);
         learned_mixture(c, low, nlow, seed, call);
-    } else if (c.mlp.logging && nlow > 0) {
-        learned_features(c, low, nlow);  // same snapshot as the evaluation
+    } else if (c.mlp.logging && nlog > 0) {
+        learned_features(c, low, nlog);  // same snapshot as the evaluation
     }
     for (int b = 0; b < NUM_BINS; ++b) {
         if (bl.count[b] == 0) continue;
@@ -349,7 +357,7 @@ void eval_vertex_list(sbpx_ctx* ctx, const int* d_list, const BinnedList& bl, co
         L.n = bl.count[b];
         L.bin = b;
         L.variant = c.pick_variant(b);
-        L.learned = learned && b < 3;
+        L.learned = learned && b < nb;
         L.seed = seed;
         L.call = call;
         L.tag = tag;
@@ -522,8 +530,8 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
             ctx->c.mh.headroom = (size_t)value;
         } else if (option == 8) {
             if (value < 0 || value > LEARN_MH_MAXDEG)
-                throw ArgError("learned degree bound of the sequential pass must be in [0, 510]");
-            ctx->c.learn_mh_maxdeg = (int)value;
+                throw ArgError("learned degree bound must be in [0, 510]");
+            ctx->c.learn_maxdeg = (int)value;
         } else {
             throw ArgError("unknown option");
         }

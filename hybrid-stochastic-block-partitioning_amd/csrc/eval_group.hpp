@@ -11,8 +11,9 @@ namespace sbpx {
 // included), so the barriers never depend on the data. The neighbour-block table is shared by
 // the group (LDS or global memory) and partial results meet in LDS. Results equal those of
 // eval_move: the same cells are visited and the same terms formed; only the order of the FP64
-// sums differs. LRN with lw >= 0 and lz / lcand: the proposal was the learned mixture, as in
-// eval_move (given_s / given_u carry the learned branch's draw).
+// sums differs. LRN with lw >= 0: the proposal was the learned mixture, as in eval_move (given_s /
+// given_u carry the learned branch's draw); the net's forward and reverse probabilities come from
+// the fixed scores lz / lcand, or with lz null as lqf_in / lqb_in.
 constexpr int RED_D = 4;   // per-wavefront double partials: dS, pf, pb
 constexpr int RED_I = 12;  // per-wavefront int partials
 
@@ -20,7 +21,8 @@ template <int G, class View, bool LRN = false>
 __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const View& M, int v, int* tab, int cap,
                                  uint64_t seed, uint32_t call, uint32_t tag, int given_s, double given_u,
                                  bool propose_only, int* hist, double (*redd)[RED_D], int (*redi)[RED_I],
-                                 double lw = -1.0, const float* lz = nullptr, const int* lcand = nullptr) {
+                                 double lw = -1.0, const float* lz = nullptr, const int* lcand = nullptr,
+                                 double lqf_in = 0.0, double lqb_in = 0.0) {
     constexpr int GS = G * WAVE;
     const int lane = lane_id();
     const int wave = threadIdx.x / WAVE;
@@ -207,7 +209,7 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
     }
     R.s = s;
     R.u = u;
-    double lqf = 0.0, lqb = 0.0;
+    double lqf = lqf_in, lqb = lqb_in;
     if (LRN && act && lw >= 0.0 && lz != nullptr) {
         lqf = cand_prob(lz, lcand, r, s);
         lqb = cand_prob(lz, lcand, s, r);

@@ -54,9 +54,11 @@ constexpr int NUM_LDS_BINS = 5;
 constexpr int BIN_MAXDEG[NUM_LDS_BINS] = {30, 62, 126, 254, 510};
 constexpr int BIN_CAP[NUM_LDS_BINS] = {64, 128, 256, 512, 1024};
 constexpr int GROUP_MAXDEG = 2047;
-// The learned proposal covers the vertices of the first three bins (degree <= 126) in the
-// batched sweeps and, in the sequential pass, those up to a bound chosen per context (126 by
-// default, at most LEARN_MH_MAXDEG); the mixture keeps the plain proposal with this weight.
+// The learned proposal covers the vertices up to a degree bound chosen per context (126 by
+// default, at most LEARN_MH_MAXDEG): in the sequential pass every vertex within it, in the
+// batched sweeps the first three bins (degree <= 126) and the bins 3-4 whose largest degree is
+// within it. Training rows come from the first three bins. The mixture keeps the plain proposal
+// with this weight.
 constexpr int LEARN_MAXDEG = 126;
 constexpr int LEARN_MH_MAXDEG = 510;
 constexpr float LEARN_W_PLAIN = 0.2f;
@@ -205,7 +207,7 @@ struct Ctx {
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
     int mh_wide = 0;   // workgroups of the Metropolis-Hastings pass; 0 = by window yield
     int64_t mh_headroom = 0;  // initial table pool headroom in slots; 0 = the initial layout (>= 1M)
-    int learn_mh_maxdeg = LEARN_MAXDEG;  // largest degree the learned mixture covers in the sequential pass
+    int learn_maxdeg = LEARN_MAXDEG;  // learned mixture's degree bound (sequential pass; whole batched bins)
 
     // scratch
     void* keys0 = nullptr;

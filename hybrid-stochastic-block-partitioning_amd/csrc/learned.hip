@@ -263,7 +263,7 @@ __global__ void mh_mixture_kernel(const int* __restrict__ list, int n, GraphView
 void learned_mh_draws(Ctx& c, const int* list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag) {
     if (n <= 0) return;
     MlpState& m = c.mlp;
-    const int maxdeg = std::min(c.learn_mh_maxdeg, LEARN_MH_MAXDEG);
+    const int maxdeg = std::min(c.learn_maxdeg, LEARN_MH_MAXDEG);
     learned_features(c, list, n, std::min(max_deg, maxdeg));
     mlp_forward(c, m.x, n, m.sh1, m.sh2, m.sz);
     const GraphView G{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};

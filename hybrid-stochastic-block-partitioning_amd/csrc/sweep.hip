@@ -109,8 +109,9 @@ __global__ void __launch_bounds__(WAVES * WAVE) eval_sub_kernel(EvalParams P) {
 }
 
 // Vertices of the highest degrees: G wavefronts (the whole workgroup) share one vertex, its
-// table in LDS (CAP slots) or, with CAP = 0, in global memory at a per-vertex offset.
-template <int G, int CAP>
+// table in LDS (CAP slots) or, with CAP = 0, in global memory at a per-vertex offset. LRN: the
+// learned mixture's draws and probabilities (P.lw >= 0) in a separate instantiation.
+template <int G, int CAP, bool LRN = false>
 __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
     __shared__ int tab_lds[CAP > 0 ? 7 * CAP : 1];
     __shared__ int hist[256];
@@ -126,10 +127,17 @@ __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
         cap = (P.pool_off[idx + 1] - base) / 7;
         tab = P.pool + base;
     }
-    const int gs = P.given_s ? P.given_s[idx] : -1;
-    const double gu = P.given_u ? P.given_u[idx] : 0.0;
-    MoveResult R = eval_group<G, CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu,
-                                          P.propose_only != 0, hist, redd, redi);
+    MoveResult R;
+    if constexpr (LRN) {
+        R = eval_group<G, CsrView, true>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, P.lps[v], P.lpu[v],
+                                         P.propose_only != 0, hist, redd, redi, (double)P.lw, nullptr, nullptr,
+                                         (double)P.lqf[v], (double)P.lqb[v]);
+    } else {
+        const int gs = P.given_s ? P.given_s[idx] : -1;
+        const double gu = P.given_u ? P.given_u[idx] : 0.0;
+        R = eval_group<G, CsrView>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, P.propose_only != 0,
+                                   hist, redd, redi);
+    }
     if (threadIdx.x == 0) write_result(P, R, v, idx);
 }
 
@@ -190,7 +198,7 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     P.bytes = L.bytes;
     P.pool_off = L.huge_offsets;
     if (L.learned) {
-        if (L.bin >= 3) throw StateError("learned proposals are evaluated by the low-degree bins only");
+        if (L.bin >= NUM_LDS_BINS) throw StateError("learned proposals are evaluated by the LDS-table bins only");
         P.lps = c.mlp.ps;
         P.lpu = c.ev_u;
         P.lqf = c.mlp.qf;
@@ -205,11 +213,13 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
         case 1: launch_bin<128, 4>(P, L.n, L.variant, c.stream); break;
         case 2: launch_bin<256, 4>(P, L.n, L.variant, c.stream); break;
         case 3:
-            if (L.variant == 1) eval_group_kernel<4, 512><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
+            if (L.variant == 1 && L.learned) eval_group_kernel<4, 512, true><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
+            else if (L.variant == 1) eval_group_kernel<4, 512><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
             else launch_bin<512, 2>(P, L.n, 0, c.stream);
             break;
         case 4:
-            if (L.variant == 1) eval_group_kernel<4, 1024><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
+            if (L.variant == 1 && L.learned) eval_group_kernel<4, 1024, true><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
+            else if (L.variant == 1) eval_group_kernel<4, 1024><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
             else launch_bin<1024, 1>(P, L.n, 0, c.stream);
             break;
         case 5: eval_group_kernel<4, 4096><<<L.n, 4 * WAVE, 0, c.stream>>>(P); break;

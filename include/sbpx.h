@@ -86,9 +86,10 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            0 (default) as many as the initial layout holds (at least 2^20); a positive value
  *            limits the pool to exactly that many (and restarts the doubling that follows each
  *            time the pool runs out), so that the rebuild-and-resume path runs; for tests.
- * option 8 = largest vertex degree (out + in) that the learned mixture covers in the sequential
- *            pass (proposals 2 and 3): 0 to 510, default 126 (the batched sweeps' bound); higher
- *            vertices keep the plain proposal. */
+ * option 8 = degree bound (out + in) of the learned mixture: 0 to 510, default 126. The
+ *            sequential pass (proposals 2 and 3) covers every vertex within it; the batched
+ *            sweeps cover degree <= 126 and, when the bound reaches 254 or 510, the vertices of
+ *            degree 127-254 or 127-510 too. Other vertices keep the plain proposal. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

@@ -210,7 +210,7 @@ def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, pr
     _merge_to(c, n, 300, 4)
     c.set_option(m.OPT_PROPOSAL, proposal)
     c.set_option(m.OPT_MH_GROUP, group)
-    c.set_option(m.OPT_LEARN_MH_MAXDEG, maxdeg)
+    c.set_option(m.OPT_LEARN_MAXDEG, maxdeg)
     code = m.ALGO_HYBRID if algo == "hybrid" else m.ALGO_MH
     moves = 0
     from test_gpu_parity import assert_same_state
@@ -253,3 +253,46 @@ def test_learned_sequential_pass_same_moves_with_wavefront_groups():
         assert other[0] == states[0][0]
         assert all(np.array_equal(x, y) for x, y in zip(other[1:4], states[0][1:4]))
         assert other[4] == pytest.approx(states[0][4], rel=1e-9)
+
+
+def _high_degree_graph(n, seed):
+    """A DC-SBM graph whose vertices mostly fall in the degree bins 127-254 and 255-510."""
+    m = sbpx()
+    op, oi, ip, ii, _ = m.generate_dcsbm(n, 12, 0.6, 1.0, 60, 300, 1.5, seed=seed)
+    return (op, oi, ip, ii)
+
+
+@pytest.mark.parametrize("maxdeg", [254, 510])
+def test_learned_batched_bins_above_126_keep_the_same_chain_in_both_kernels(maxdeg):
+    """With the degree bound at 254 or 510 the batched sweeps give the vertices of degree 127-254
+    (and 255-510) the learned mixture too. Both kernel variants of those bins (one wavefront per
+    vertex; four wavefronts sharing a vertex) take the same draws and the same net probabilities,
+    so they accept the same moves; the state matches a rebuild from the assignment; and the
+    learned run differs from the bound-126 run, which keeps those bins plain."""
+    m = sbpx()
+    n = 3000
+    arrays = _high_degree_graph(n, 13)
+    deg = np.diff(arrays[0]) + np.diff(arrays[2])
+    assert ((deg > 126) & (deg <= 254)).sum() > 500 and ((deg > 254) & (deg <= 510)).sum() > 100
+    g = O.Graph(n, *arrays)
+    from test_gpu_parity import assert_same_state
+    runs = []
+    for bound, variant in ((maxdeg, 1), (maxdeg, 2), (126, 1)):
+        c = m.Context(0)
+        c.upload_graph(*arrays)
+        c.split()
+        c.learned_init(5)
+        c.set_option(m.OPT_PROPOSAL, 1)
+        c.set_option(m.OPT_LEARN_MAXDEG, bound)
+        c.set_option(m.OPT_EVAL_VARIANT, variant)
+        _merge_to(c, n, 200, 3)
+        moves = [c.sweep(m.ALGO_ASYNC_GIBBS, 1, 21, call).moves for call in range(3)]
+        a = c.state()[0].copy()
+        bm = O.Blockmodel(g, c.info()[0], a)
+        assert_same_state(c, bm, check_deg=False)
+        runs.append((moves, a, c.mdl()))
+        c.close()
+    assert sum(runs[0][0]) > 0
+    assert runs[1][0] == runs[0][0] and np.array_equal(runs[1][1], runs[0][1])
+    assert runs[1][2] == pytest.approx(runs[0][2], rel=1e-9)
+    assert not np.array_equal(runs[2][1], runs[0][1])

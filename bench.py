@@ -67,6 +67,7 @@ def train_learned(m, cfg, seeds, log_sweeps=8, steps=600):
             for i in range(log_sweeps):
                 ctx.sweep(m.ALGO_ASYNC_GIBBS, 1, seed, 1000 * call + i)
             call += 1
+            print(f"# training graph {seed}: B={B}, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     ctx.learned_log(0)
     rows = ctx.learned_log_count()
     t1 = time.perf_counter()

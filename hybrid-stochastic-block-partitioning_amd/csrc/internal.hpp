@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <climits>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -385,8 +386,9 @@ void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
 void finish_degrees(Ctx& c);
 // learned.hip / mlp.hip
-// max_deg: the largest degree in the list (a larger neighbour-block table above LEARN_MAXDEG).
-void learned_features(Ctx& c, const int* list, int n, int max_deg = LEARN_MAXDEG);
+// max_deg: the largest degree in the list (a larger neighbour-block table above LEARN_MAXDEG);
+// vertices with more than skip_deg out + in list entries get an empty row (no learned part).
+void learned_features(Ctx& c, const int* list, int n, int max_deg = LEARN_MAXDEG, int skip_deg = INT_MAX);
 void learned_log(Ctx& c, const int* list, int n);
 void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call);
 // Scores the queue of a sequential pass once and draws each vertex's learned branch.

@@ -46,6 +46,7 @@ struct FeatParams {
     __bf16* x;        // [n][64]
     int* cand;        // [n][16]
     int* cnt;         // [n][16]
+    int skip_deg;     // vertices with more list entries (out + in) get an empty row
 };
 
 // One lane writes feature j (< 64) of a vertex whose own block is `own`; din_of / dout_of give
@@ -94,6 +95,14 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
     const int ob = P.G.out_ptr[v], dO = P.G.out_ptr[v + 1] - ob;
     const int ib = P.G.in_ptr[v], dI = P.G.in_ptr[v + 1] - ib;
     const int self = P.G.self[v];
+    if (dO + dI > P.skip_deg) {  // no learned part for this vertex: its row is never read
+        if (lane < MLP_CAND) {
+            P.cand[(long)i * MLP_CAND + lane] = -1;
+            P.cnt[(long)i * MLP_CAND + lane] = 0;
+        }
+        P.x[(long)i * MLP_IN + lane] = f2bf(0.0f);
+        return;
+    }
     for (int j = lane; j < dO + dI; j += WAVE) {
         const int w = j < dO ? P.G.out_idx[ob + j] : P.G.in_idx[ib + (j - dO)];
         if (w == v) continue;
@@ -264,7 +273,7 @@ void learned_mh_draws(Ctx& c, const int* list, int n, int max_deg, uint64_t seed
     if (n <= 0) return;
     MlpState& m = c.mlp;
     const int maxdeg = std::min(c.learn_maxdeg, LEARN_MH_MAXDEG);
-    learned_features(c, list, n, std::min(max_deg, maxdeg));
+    learned_features(c, list, n, std::min(max_deg, maxdeg), maxdeg);
     mlp_forward(c, m.x, n, m.sh1, m.sh2, m.sz);
     const GraphView G{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
     mh_mixture_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(list, n, G, c.bm.assign, m.sz, m.cand, seed, call, tag,
@@ -349,9 +358,10 @@ static FeatParams feat_params(Ctx& c, const int* list, int n) {
     return P;
 }
 
-void learned_features(Ctx& c, const int* list, int n, int max_deg) {
+void learned_features(Ctx& c, const int* list, int n, int max_deg, int skip_deg) {
     if (n <= 0) return;
     FeatParams P = feat_params(c, list, n);
+    P.skip_deg = skip_deg;
     // vertices above the table's degree bound get features from a partial table; their scores
     // are not used (the mixture gives them no learned part)
     if (max_deg > LEARN_MAXDEG)

@@ -82,10 +82,72 @@ __global__ void __launch_bounds__(GEMM_WAVES * WAVE) gemm_bf16_kernel(GemmArgs g
     }
 }
 
+// The forward products (A and B both contiguous in k, K a multiple of 32, 16-byte aligned rows):
+// each wavefront computes 16 rows x 64 columns, four 16 x 16 tiles that share one A fragment per
+// k step, and each fragment is one 16-byte load per lane. Every tile runs the k steps and the
+// MFMA of gemm_bf16_kernel in the same order, so the results are the same bits.
+constexpr int GEMM_NT = 4;  // 16-column tiles per wavefront
+
+__global__ void __launch_bounds__(GEMM_WAVES * WAVE) gemm_bf16_kcontig_kernel(GemmArgs g) {
+    const int lane = lane_id();
+    const int wave = threadIdx.x / WAVE;
+    const int m0 = (blockIdx.y * GEMM_WAVES + wave) * 16;
+    const int n0 = blockIdx.x * 16 * GEMM_NT;
+    if (m0 >= g.M) return;
+    const int am = m0 + (lane & 15);
+    const int kq = 8 * (lane >> 4);
+    const bool arow = am < g.M;
+    const __bf16* Ap = g.A + (long)(arow ? am : 0) * g.sam + kq;
+    const __bf16* Bp[GEMM_NT];
+    bool bcol[GEMM_NT];
+#pragma unroll
+    for (int j = 0; j < GEMM_NT; ++j) {
+        const int bn = n0 + 16 * j + (lane & 15);
+        bcol[j] = bn < g.N;
+        Bp[j] = g.B + (long)(bcol[j] ? bn : 0) * g.sbn + kq;
+    }
+    const bf16x8 zero8 = {};
+    f32x4 acc[GEMM_NT];
+#pragma unroll
+    for (int j = 0; j < GEMM_NT; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+        const bf16x8 a = arow ? *(const bf16x8*)(Ap + k0) : zero8;
+#pragma unroll
+        for (int j = 0; j < GEMM_NT; ++j) {
+            if (n0 + 16 * j >= g.N) break;  // uniform over the wavefront
+            const bf16x8 b = bcol[j] ? *(const bf16x8*)(Bp[j] + k0) : zero8;
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < GEMM_NT; ++j) {
+        const int cn = n0 + 16 * j + (lane & 15);
+        if (cn >= g.N) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int cm = m0 + 4 * (lane >> 4) + i;
+            if (cm >= g.M) continue;
+            float x = acc[j][i];
+            if (g.bias) x += g.bias[cn];
+            if (g.relu) x = x > 0.0f ? x : 0.0f;
+            if (g.mask && !(from_bf16(g.mask[(long)cm * g.ldc + cn]) > 0.0f)) x = 0.0f;
+            if (g.C32) g.C32[(long)cm * g.ldc + cn] = x;
+            if (g.C16) g.C16[(long)cm * g.ldc + cn] = to_bf16(x);
+        }
+    }
+}
+
 static void gemm(hipStream_t st, const GemmArgs& g) {
     if (g.M <= 0 || g.N <= 0) return;
-    dim3 grid((g.N + 16 * GEMM_WAVES - 1) / (16 * GEMM_WAVES), (g.M + 15) / 16);
-    gemm_bf16_kernel<<<grid, GEMM_WAVES * WAVE, 0, st>>>(g);
+    const bool kcontig = g.sak == 1 && g.sbk == 1 && g.K % 32 == 0 && g.sam % 8 == 0 && g.sbn % 8 == 0 &&
+                         (uintptr_t)g.A % 16 == 0 && (uintptr_t)g.B % 16 == 0;
+    if (kcontig) {
+        dim3 grid((g.N + 16 * GEMM_NT - 1) / (16 * GEMM_NT), (g.M + 16 * GEMM_WAVES - 1) / (16 * GEMM_WAVES));
+        gemm_bf16_kcontig_kernel<<<grid, GEMM_WAVES * WAVE, 0, st>>>(g);
+    } else {
+        dim3 grid((g.N + 16 * GEMM_WAVES - 1) / (16 * GEMM_WAVES), (g.M + 15) / 16);
+        gemm_bf16_kernel<<<grid, GEMM_WAVES * WAVE, 0, st>>>(g);
+    }
     SBPX_HIP(hipGetLastError());
 }
 

@@ -110,18 +110,21 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
         if (slot >= 0) atomicAdd(&cntt[slot], 1);
     }
     fence_wave();
-    // top 16 by (count desc, block asc)
+    // top 16 by (count desc, block asc): each lane holds the scores of its CAP / 64 slots
+    // (count << 32 | ~block, unique per block) and the wavefront takes the largest 16 times
     int* cand = P.cand + (long)i * MLP_CAND;
     int* cnt = P.cnt + (long)i * MLP_CAND;
+    constexpr int PER = CAP / WAVE;
+    long long sc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int kk = key[lane + q * WAVE], c = cntt[lane + q * WAVE];
+        sc[q] = (kk >= 0 && c > 0) ? (((long long)c << 32) | (long long)(0x7FFFFFFF - kk)) : -1;
+    }
     for (int t = 0; t < MLP_CAND; ++t) {
         long long best = -1;
-        for (int j = lane; j < CAP; j += WAVE) {
-            const int kk = key[j], c = cntt[j];
-            if (kk >= 0 && c > 0) {
-                const long long score = ((long long)c << 32) | (long long)(0x7FFFFFFF - kk);
-                best = score > best ? score : best;
-            }
-        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) best = sc[q] > best ? sc[q] : best;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             const long long o = __shfl_xor(best, off, WAVE);
@@ -131,15 +134,16 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
         if (best >= 0) {
             b = 0x7FFFFFFF - (int)(best & 0xFFFFFFFF);
             c = (int)(best >> 32);
-            for (int j = lane; j < CAP; j += WAVE)
-                if (key[j] == b) cntt[j] = 0;  // taken
+#pragma unroll
+            for (int q = 0; q < PER; ++q)
+                if (sc[q] == best) sc[q] = -1;  // taken
         }
         if (lane == 0) {
             cand[t] = b;
             cnt[t] = c;
         }
-        fence_wave();
     }
+    fence_wave();  // lane 0's candidate rows before every lane reads them
     const int k_out = dO, k_in = dI - self;
     auto din_of = [&](int b) { return P.S.din[b]; };
     auto dout_of = [&](int b) { return P.S.dout[b]; };
@@ -179,28 +183,47 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) mixture_kernel(MixParams P)
     const float* z = P.z + (long)i * MLP_OUT;
     const Draw4 dr = philox4x32_10(P.seed, (uint32_t)v, P.call, 0u, TAG_LEARN);
     int s = P.s_plain[v];
-    float qf = cand_prob(z, cand, r, -1);  // -1: no learned mass
-    if (qf >= 0.0f) {
+    // softmax weights of the candidates other than r, formed once (the values and summation
+    // order of cand_prob): the categorical draw and the forward probability both use them
+    float qf = -1.0f;  // no learned mass
+    float mx = -1e30f;
+    int any = 0;
+#pragma unroll
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != r) {
+            mx = fmaxf(mx, z[j]);
+            any = 1;
+        }
+    if (any) {
+        float e[MLP_CAND];
+        float den = 0.0f;
+#pragma unroll
+        for (int j = 0; j < MLP_CAND; ++j) {
+            e[j] = 0.0f;
+            if (cand[j] >= 0 && cand[j] != r) {
+                e[j] = expf(z[j] - mx);
+                den += e[j];
+            }
+        }
         if (unit53(dr.x0, dr.x1) >= (double)P.w_plain) {
             // categorical draw over the softmax, candidates in slot order
-            float mx = -1e30f;
-            for (int j = 0; j < MLP_CAND; ++j)
-                if (cand[j] >= 0 && cand[j] != r) mx = fmaxf(mx, z[j]);
-            float den = 0.0f;
-            for (int j = 0; j < MLP_CAND; ++j)
-                if (cand[j] >= 0 && cand[j] != r) den += expf(z[j] - mx);
             const float target = (float)unit53(dr.x2, dr.x3) * den;
             float run = 0.0f;
             int pick = -1;
+#pragma unroll
             for (int j = 0; j < MLP_CAND; ++j)
                 if (cand[j] >= 0 && cand[j] != r) {
-                    run += expf(z[j] - mx);
+                    run += e[j];
                     pick = cand[j];
                     if (run > target) break;
                 }
             s = pick;
         }
-        qf = cand_prob(z, cand, r, s);
+        float num = 0.0f;
+#pragma unroll
+        for (int j = 0; j < MLP_CAND; ++j)
+            if (cand[j] >= 0 && cand[j] != r && cand[j] == s) num = e[j];
+        qf = num / den;
     }
     if (lane == 0) {
         P.ps[v] = s;

@@ -333,4 +333,37 @@ int oracle_sbp(void* gp, int algo, int batches, int approximate, uint64_t seed, 
     GUARD_END
 }
 
+// finetune::early_stop (finetune.cpp:144-162) on a list of per-sweep MDL changes.
+int oracle_early_stop(int iteration, int bracket, double initial, const double* deltas, int n) {
+    if (n <= 0) return -1;
+    std::vector<double> d(deltas, deltas + n);
+    return early_stop(iteration, bracket != 0, initial, d) ? 1 : 0;
+}
+
+// The golden-section search of the driver (sbp.cpp:61-112, blockmodel_triplet.cpp:3-100) with
+// the merge phase and the sweeps replaced by a table: the blockmodel starts at n blocks with
+// num_to_merge = int(n * 0.5); each outer iteration ends at B - num_to_merge blocks with overall
+// entropy (float)score[B] and hands it to Triplet::next. Writes the block count of every outer
+// iteration and then the block count the search returns; returns how many were written (-1 when
+// cap is too small).
+int oracle_golden_trace(int n, const double* score, int* out, int cap) {
+    Blockmodel bm;
+    bm.empty = false;
+    bm.B = n;
+    bm.num_to_merge = int(n * 0.5f);
+    Triplet t;
+    int k = 0;
+    while (!t.optimal) {
+        bm.B -= bm.num_to_merge;
+        bm.num_to_merge = 0;
+        bm.overall_entropy = (float)score[bm.B];
+        if (k >= cap) return -1;
+        out[k++] = bm.B;
+        bm = t.next(bm);
+    }
+    if (k >= cap) return -1;
+    out[k++] = bm.B;
+    return k;
+}
+
 }  // extern "C"

@@ -42,6 +42,8 @@ def lib():
         subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
     L = C.CDLL(ORACLE_SO)
     L.oracle_last_error.restype = C.c_char_p
+    L.oracle_early_stop.argtypes = [C.c_int, C.c_int, C.c_double, f64p, C.c_int]
+    L.oracle_golden_trace.argtypes = [C.c_int, f64p, i32p, C.c_int]
     L.oracle_philox.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, u32p]
     L.oracle_logf.argtypes = [C.c_float]
     L.oracle_logf.restype = C.c_float

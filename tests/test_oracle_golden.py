@@ -471,3 +471,51 @@ def test_sbp_end_to_end_small(toy):
         assert 1 <= run.B <= n
         initial = O.Blockmodel(g, n).mdl()
         assert run.mdl < initial
+
+
+# ----------------------------------------------------------------------------- early stop, golden section
+def _early_stop(iteration, bracket, initial, deltas):
+    L = O.lib()
+    return L.oracle_early_stop(iteration, int(bracket), initial, np.asarray(deltas, np.float64), len(deltas))
+
+
+def test_early_stop_hand_worked_cases():
+    """finetune::early_stop (finetune.cpp:144-162), cases worked by hand from the reference:
+    stop at once when the last change is exactly 0; never before the third iteration; otherwise
+    stop when minus the mean of the LAST THREE changes is strictly below 5e-4 * S0 before the
+    bracket forms and 1e-4 * S0 after."""
+    assert _early_stop(1, False, 1000.0, [-5.0, 0.0]) == 1          # last change 0
+    assert _early_stop(2, False, 1000.0, [-1.0, -1.0, -1e-9]) == 0  # iteration < 3
+    # -mean(-1, -0.4, -0.05) = 0.48333 < 0.5 (no bracket) but not < 0.1 (bracket)
+    assert _early_stop(3, False, 1000.0, [-1.0, -0.4, -0.05]) == 1
+    assert _early_stop(3, True, 1000.0, [-1.0, -0.4, -0.05]) == 0
+    # exactly at the threshold: 0.5 < 0.5 is false
+    assert _early_stop(3, False, 1000.0, [-0.75, -0.5, -0.25]) == 0
+    # only the last three count: mean of all four would be 25.075
+    assert _early_stop(4, False, 1000.0, [-100.0, -0.1, -0.1, -0.1]) == 1
+    # a rising MDL (positive changes) has a negative mean decrease, below any threshold
+    assert _early_stop(3, True, 1000.0, [0.3, 0.3, 0.3]) == 1
+
+
+def _golden_trace(n, score):
+    L = O.lib()
+    out = np.zeros(256, np.int32)
+    k = L.oracle_golden_trace(n, np.asarray(score, np.float64), out, len(out))
+    assert k > 0
+    return list(out[:k - 1]), int(out[k - 1])
+
+
+def test_golden_section_hand_worked_traces():
+    """The driver's search over the number of blocks (sbp.cpp:61-112 with
+    BlockmodelTriplet::get_next_blockmodel / update / is_done, blockmodel_triplet.cpp:3-100),
+    scored by a table instead of merges and sweeps. Traces worked by hand from the reference."""
+    # (B - 10)^2 + 100: halving 32, 16, 8, 4 brackets at 8; then 0.618 steps with int truncation:
+    # 16 - 12 = 4 -> 12 (ties with 8 at 104 and takes the middle), 14, 10, 11, 9; done when
+    # the outer pair is 2 apart (11 and 9), returning the middle, 10
+    score = [(b - 10) ** 2 + 100.0 for b in range(65)]
+    assert _golden_trace(64, score) == ([32, 16, 8, 4, 12, 14, 10, 11, 9], 10)
+    # MDL increasing in B: halving until num_to_merge = int(1 * 0.5) = 0 ends the search at 1
+    assert _golden_trace(8, [float(b) for b in range(9)]) == ([4, 2, 1], 1)
+    # best at the first iteration (upper slot empty): steps from the lower side, 20 - 10 = 10 ->
+    # 10 + int(6.18) = 16, then 18, 19; done when middle and lower are 1 apart
+    assert _golden_trace(40, [abs(b - 20.0) for b in range(41)]) == ([20, 10, 16, 18, 19], 20)

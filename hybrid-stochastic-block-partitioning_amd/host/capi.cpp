@@ -242,4 +242,39 @@ int sbp_run(int32_t N, const int32_t* out_ptr, const int32_t* out_idx, const int
     return rc;
 }
 
+// finetune::early_stop (finetune.cpp:144-162) of this library on a list of per-sweep MDL changes;
+// bracket != 0: the golden-section bracket has formed. No device work.
+int sbp_early_stop(int iteration, int bracket, double initial, const double* deltas, int n) {
+    if (n <= 0 || !deltas) return -1;
+    auto dev = std::make_shared<sbpx_host::Device>();
+    BlockmodelTriplet t;
+    if (bracket) t.get(2) = Blockmodel::detached(1, 0.0f, dev);
+    std::vector<double> d(deltas, deltas + n);
+    return finetune::early_stop(iteration, t, initial, d) ? 1 : 0;
+}
+
+// The driver's search over the number of blocks (sbp.cpp:61-112, blockmodel_triplet.cpp:3-100)
+// as this library runs it, with the merge phase and the sweeps replaced by a table: the
+// blockmodel starts at n blocks; each outer iteration ends at B - num_blocks_to_merge blocks
+// with overall entropy (float)score[B] and goes to BlockmodelTriplet::get_next_blockmodel.
+// Writes the block count of every outer iteration, then the one the search returns; returns
+// how many were written (-1 when cap is too small). No device work.
+int sbp_golden_trace(int n, const double* score, int* out, int cap) {
+    if (n <= 0 || !score || !out) return -1;
+    auto dev = std::make_shared<sbpx_host::Device>();
+    Blockmodel bm = Blockmodel::detached(n, 0.0f, dev);
+    BlockmodelTriplet t;
+    int k = 0;
+    while (!sbp::done_blockmodeling(bm, t)) {
+        const int B = bm.getNum_blocks() - bm.getNum_blocks_to_merge();
+        Blockmodel next = Blockmodel::detached(B, (float)score[B], dev);
+        if (k >= cap) return -1;
+        out[k++] = B;
+        bm = t.get_next_blockmodel(next);
+    }
+    if (k >= cap) return -1;
+    out[k++] = bm.getNum_blocks();
+    return k;
+}
+
 }  // extern "C"

@@ -128,6 +128,17 @@ void Blockmodel::make_live() const {
 }
 
 // Blockmodel::copy (blockmodel.cpp:95-105): a deep copy with num_blocks_to_merge = 0.
+Blockmodel Blockmodel::detached(int num_blocks, float overall_entropy, std::shared_ptr<sbpx_host::Device> dev) {
+    Blockmodel b;
+    b.empty = false;
+    b.dev_ = std::move(dev);
+    b.id_ = b.dev_->next_id++;
+    b.num_blocks = num_blocks;
+    b.num_blocks_to_merge = int(num_blocks * BLOCK_REDUCTION_RATE);
+    b.overall_entropy = overall_entropy;
+    return b;
+}
+
 Blockmodel Blockmodel::copy() {
     Blockmodel c;
     c.empty = empty;

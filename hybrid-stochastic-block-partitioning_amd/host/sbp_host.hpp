@@ -104,6 +104,10 @@ public:
     // checkpoint I/O of the fields and the assignment (the device state is rebuilt when used)
     void write(std::FILE* f) const;
     static Blockmodel read(std::FILE* f, const Graph& graph);
+    /// A blockmodel that holds only its block count and overall entropy, with no device state:
+    /// for running the search over the number of blocks alone (it supports copy() and the
+    /// fields, not the device steps).
+    static Blockmodel detached(int num_blocks, float overall_entropy, std::shared_ptr<sbpx_host::Device> dev);
 
 private:
     std::shared_ptr<sbpx_host::Device> dev_;

@@ -3,6 +3,7 @@ include/sbpx.h) and to the C++ driver (libsbp_host.so). The compute path is the 
 this module only moves host arrays in and out. It raises if the libraries are not built."""
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -221,7 +222,10 @@ class Context:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # no device calls while the interpreter shuts down: a profiler or the HIP runtime may
+        # already have finalised; close() contexts explicitly instead
+        if not sys.is_finalizing():
+            self.close()
 
     def _chk(self, st):
         if st != 0:
@@ -561,4 +565,5 @@ class ResidentGraph:
             self.h = None
 
     def __del__(self):
-        self.close()
+        if not sys.is_finalizing():
+            self.close()

@@ -6,6 +6,8 @@
 // the nonzeros with the FP32 log the reference uses.
 #include <algorithm>
 
+#include <limits>
+
 #include "internal.hpp"
 
 namespace sbpx {
@@ -176,6 +178,13 @@ struct KeySpace {
     const int* dense;
     const int* inv;
 };
+
+// The largest key of 2 * bits bits (all ones): padding that sorts after every real key. Shifting
+// the maximum down avoids a shift by the key's full width when 2 * bits equals it.
+template <typename K>
+static inline K pad_key(int bits) {
+    return std::numeric_limits<K>::max() >> (8 * (int)sizeof(K) - 2 * bits);
+}
 
 template <typename K>
 static KeySpace<K> key_space(const Ctx& c) {
@@ -574,7 +583,7 @@ static int emit_delta(Ctx& c, const int* list, int n, const int* old_assign) {
     delta_emit_kernel<K><<<(n + wpb - 1) / wpb, TB, 0, st>>>(list, n, c.ev_acc, c.inc.rank, c.g.out_ptr, c.g.out_idx,
                                                             c.g.in_ptr, c.g.in_idx, old_assign, c.new_block,
                                                             c.moved_epoch, c.epoch, ks.shift, ks.dense,
-                                                            (K)((((K)1) << (2 * ks.bits)) - 1), (K*)c.keys0, c.vals0);
+                                                            pad_key<K>(ks.bits), (K*)c.keys0, c.vals0);
     SBPX_HIP(hipGetLastError());
     return total;
 }

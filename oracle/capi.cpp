@@ -200,6 +200,15 @@ int oracle_merge_delta_mdl(void* bmp, int r, int s, double* out) {
     GUARD_END
 }
 
+// One merge proposal of block r (draw i of merge phase `call`), before the repeat check:
+// the target block the sampler of propose_merge_sparse draws.
+int oracle_propose_merge_draw(void* bmp, int r, int i, uint64_t seed, uint32_t call, int32_t* s) {
+    GUARD_BEGIN
+    std::unordered_map<int, bool> past;
+    *s = propose_merge(*(Blockmodel*)bmp, r, i, past, seed, call).s;
+    GUARD_END
+}
+
 // Evaluate fixed (v, s, u) proposals against the current state without applying them.
 int oracle_eval_moves(void* gp, void* bmp, int n, const int32_t* v, const int32_t* s, const double* u,
                       double* dS, double* H, uint8_t* acc) {
@@ -314,12 +323,18 @@ struct OracleRun {
     double seconds, mcmc_seconds, merge_seconds, mdl;
 };
 
-int oracle_sbp(void* gp, int algo, int batches, int approximate, uint64_t seed, int32_t* assign,
-               OracleRun* out) {
+// trace_B / trace_mdl (may be NULL) receive up to trace_cap entries of the per-outer-iteration
+// trace (blocks after the merge phase, float MDL after the sweeps).
+int oracle_sbp_traced(void* gp, int algo, int batches, int approximate, uint64_t seed, int32_t* assign,
+                      OracleRun* out, int32_t* trace_B, float* trace_mdl, int trace_cap) {
     GUARD_BEGIN
     Graph* g = (Graph*)gp;
     RunResult r = stochastic_block_partition(*g, algo, batches, approximate != 0, seed);
     if (assign) std::memcpy(assign, r.bm.assign.data(), g->N * 4);
+    for (int i = 0; i < (int)r.trace_B.size() && i < trace_cap; ++i) {
+        if (trace_B) trace_B[i] = r.trace_B[i];
+        if (trace_mdl) trace_mdl[i] = r.trace_mdl[i];
+    }
     out->B = r.bm.B;
     out->outer_iterations = r.outer_iterations;
     out->sweeps = r.sweeps;
@@ -331,6 +346,11 @@ int oracle_sbp(void* gp, int algo, int batches, int approximate, uint64_t seed, 
     out->merge_seconds = r.merge_seconds;
     out->mdl = r.bm.overall_entropy;
     GUARD_END
+}
+
+int oracle_sbp(void* gp, int algo, int batches, int approximate, uint64_t seed, int32_t* assign,
+               OracleRun* out) {
+    return oracle_sbp_traced(gp, algo, batches, approximate, seed, assign, out, nullptr, nullptr, 0);
 }
 
 // finetune::early_stop (finetune.cpp:144-162) on a list of per-sweep MDL changes.

@@ -857,6 +857,8 @@ RunResult stochastic_block_partition(const Graph& g, int algo, int batches, bool
         res.evals += f.stats.evals;
         res.moves += f.stats.moves;
         res.outer_iterations++;
+        res.trace_B.push_back(bm.B);
+        res.trace_mdl.push_back(bm.overall_entropy);
         if (getenv("ORACLE_TRACE"))
             fprintf(stderr, "oracle iter %d: B=%d mdl=%.3f sweeps=%d moves=%ld\n", res.outer_iterations, bm.B,
                     (double)bm.overall_entropy, f.sweeps, f.stats.moves);

@@ -141,6 +141,9 @@ struct RunResult {
     int outer_iterations = 0;
     long sweeps = 0, attempts = 0, evals = 0, moves = 0;
     double seconds = 0.0, mcmc_seconds = 0.0, merge_seconds = 0.0;
+    // per outer iteration: blocks after the merge phase and the float MDL after the sweeps
+    std::vector<int> trace_B;
+    std::vector<float> trace_mdl;
 };
 RunResult stochastic_block_partition(const Graph& g, int algo, int batches, bool approximate,
                                      uint64_t seed);

@@ -95,6 +95,10 @@ def lib():
     L.oracle_carry_out.argtypes = [C.c_void_p, C.c_void_p, f64p, i32p, C.c_int]
     L.oracle_merge_blocks.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_uint32]
     L.oracle_sbp.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, i32p, C.POINTER(OracleRun)]
+    L.oracle_propose_merge_draw.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint32,
+                                            C.POINTER(C.c_int32)]
+    L.oracle_sbp_traced.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, i32p, C.POINTER(OracleRun),
+                                    i32p, f32p, C.c_int]
     _lib = L
     return L
 
@@ -303,6 +307,11 @@ class Blockmodel:
         _check(lib().oracle_merge_proposals(self.h, seed, call, dS, best))
         return dS, best
 
+    def propose_merge_draw(self, r, i, seed, call):
+        out = C.c_int32()
+        _check(lib().oracle_propose_merge_draw(self.h, r, i, seed, call, C.byref(out)))
+        return out.value
+
     def carry_out(self, dS, best, advanced=False):
         _check(lib().oracle_carry_out(self.g.h, self.h, np.ascontiguousarray(dS, np.float64),
                                       np.ascontiguousarray(best, np.int32), int(advanced)))
@@ -316,6 +325,18 @@ def sbp(graph, algo, batches, approximate, seed):
     run = OracleRun()
     _check(lib().oracle_sbp(graph.h, algo, batches, int(approximate), seed, a, C.byref(run)))
     return a, run
+
+
+def sbp_traced(graph, algo, batches, approximate, seed, cap=256):
+    """As sbp(), plus the per-outer-iteration trace: blocks after each merge phase and the
+    float MDL after its sweeps."""
+    a = np.zeros(graph.n, np.int32)
+    run = OracleRun()
+    tb = np.zeros(cap, np.int32)
+    tm = np.zeros(cap, np.float32)
+    _check(lib().oracle_sbp_traced(graph.h, algo, batches, int(approximate), seed, a, C.byref(run), tb, tm, cap))
+    k = min(run.outer_iterations, cap)
+    return a, run, tb[:k].copy(), tm[:k].copy()
 
 
 def null_mdl_v1(E):

@@ -438,6 +438,103 @@ def test_vertex_proposal_distribution(toy):
         assert chi2 < 30.0, (v, counts, expected)
 
 
+def _merge_law(M, r, t_from_row_only=False, s_from_row_only=False, keep_r=False):
+    """P(s | r) of propose_merge_sparse (block_merge.cpp:335-368 with common.cpp:131-182,
+    block_merge=true): t in proportion to row r + column r (a diagonal cell counts twice), then s
+    in proportion to row t + column t with index r zeroed; a uniform other block when row r and
+    column r are empty or when t has no weight left after zeroing r. The keyword arguments give
+    the laws of plausible mistakes, which the test must be able to tell apart."""
+    B = M.shape[0]
+    uniform = np.full(B, 1.0 / (B - 1))
+    uniform[r] = 0.0
+    w = M[r, :] + (0 if t_from_row_only else M[:, r])
+    if w.sum() == 0:
+        return uniform
+    p = np.zeros(B)
+    for t in range(B):
+        if w[t] == 0:
+            continue
+        e = M[t, :] + (0 if s_from_row_only else M[:, t])
+        e = e.astype(float)
+        if not keep_r:
+            e[r] = 0.0
+        p += w[t] / w.sum() * (e / e.sum() if e.sum() > 0 else uniform)
+    return p
+
+
+def _chi2(counts, p, n):
+    expected = p * n
+    mask = expected > 0
+    if counts[~mask].sum() > 0:
+        return float("inf")
+    return float(((counts[mask] - expected[mask]) ** 2 / expected[mask]).sum())
+
+
+def test_merge_proposal_distribution():
+    """The oracle's merge sampler against the closed-form law, by chi-squared over many draws,
+    on a blockmodel built to reach every branch: diagonal cells (t = r), a block whose only
+    neighbour is r (t has nothing left once r is zeroed: uniform fallback), an empty block (k = 0:
+    uniform), and asymmetric rows/columns, so that drawing t from row r alone, s from row t
+    alone, or s without zeroing r all fail."""
+    rng = np.random.default_rng(21)
+    B = 7
+    sizes = [6, 5, 5, 4, 4, 3, 0]  # block 6 is empty
+    a = np.concatenate([np.full(k, b) for b, k in enumerate(sizes)]).astype(np.int32)
+    n = len(a)
+    members = [np.flatnonzero(a == b) for b in range(B)]
+    edges = set()
+    # blocks 0-4: random, directed-asymmetric block pattern with in-block edges (diagonal cells)
+    pattern = rng.integers(0, 4, (5, 5)) * (rng.random((5, 5)) < 0.7)
+    for x in range(5):
+        for y in range(5):
+            for _ in range(int(pattern[x, y])):
+                edges.add((int(rng.choice(members[x])), int(rng.choice(members[y]))))
+    # block 5 talks only to block 0, in one direction
+    for v in members[5]:
+        edges.add((int(v), int(rng.choice(members[0]))))
+    g = O.Graph.from_edges(n, sorted(edges))
+    bm = O.Blockmodel(g, B, a)
+    M = bm.dense().astype(float)
+    assert M[6].sum() == 0 and M[:, 6].sum() == 0
+    assert M[5, 0] > 0 and M[5].sum() == M[5, 0] and M[:, 5].sum() == 0
+    assert np.diag(M)[:5].sum() > 0
+    draws = 6000
+    mistakes = ({"t_from_row_only": True}, {"s_from_row_only": True}, {"keep_r": True})
+    alt = {i: [] for i in range(len(mistakes))}
+    for r in range(B):
+        s = np.array([bm.propose_merge_draw(r, c % 10, 99, c // 10) for c in range(draws)])
+        counts = np.bincount(s, minlength=B).astype(float)
+        assert counts[r] == 0
+        assert _chi2(counts, _merge_law(M, r), draws) < 35.0, (r, counts, _merge_law(M, r) * draws)
+        for i, kw in enumerate(mistakes):
+            alt[i].append(_chi2(counts, _merge_law(M, r, **kw), draws))
+    # each plausible mistake's law is rejected on at least one block
+    for i, kw in enumerate(mistakes):
+        assert max(alt[i]) > 200.0, (kw, alt[i])
+
+
+def test_async_batches_skip_the_truncated_tail():
+    """finetune.cpp:44-45: batch_size = ceil(N / batches) and the loop runs N / batch_size
+    batches in integer division. Worked by hand: N = 1001, batches = 10 gives batch_size 101 and
+    9 batches, so vertices 909-1000 are never proposed (909 attempts); N = 1000, batches = 3 gives
+    334 and 2 batches, 668 attempts; N = 1000, batches = 10 divides evenly, 1000 attempts."""
+    rng = np.random.default_rng(4)
+    for n, batches, want in ((1001, 10, 909), (1000, 3, 668), (1000, 10, 1000)):
+        edges = set()
+        for u in range(n):
+            for w in rng.integers(0, n, 6):
+                if int(w) != u:
+                    edges.add((u, int(w)))
+        g = O.Graph.from_edges(n, sorted(edges))
+        bm = O.Blockmodel(g, n // 4, (np.arange(n) % (n // 4)).astype(np.int32))
+        before = bm.state()[0].copy()
+        st = bm.async_sweep(batches, 5, 0)
+        assert st.attempts == want, (n, batches, st.attempts)
+        after = bm.state()[0]
+        moved = np.flatnonzero(after != before)
+        assert len(moved) > 0 and moved.max() < want, (n, batches, moved.max())
+
+
 def test_merge_proposals_are_in_support(toy):
     g = toy_graph(toy)
     bm = O.Blockmodel(g, 6, toy["complex"]["assignment"])

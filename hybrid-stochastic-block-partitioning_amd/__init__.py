@@ -108,6 +108,8 @@ def load():
     L.sbpx_learned_log_count.argtypes = [vp, C.POINTER(C.c_int32)]
     L.sbpx_learned_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_float, C.c_uint64, C.POINTER(C.c_float)]
     L.sbpx_learned_forward.argtypes = [vp, C.c_int32, f32p, f32p]
+    L.sbpx_learned_eval_moves.argtypes = [vp, C.c_int32, i32p, i32p, f64p, f64p, f64p, u8p, f32p, f32p]
+    L.sbpx_learned_scores.argtypes = [vp, C.c_int32, i32p, C.c_int, i32p, f32p]
     L.sbpx_learned_gradients.argtypes = [vp, C.c_int32, f32p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                          i32p, f32p, f32p]
     L.sbpx_batch_define.argtypes = [vp, C.c_int32, i32p]
@@ -175,7 +177,8 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_learned_params", "sbpx_learned_log", "sbpx_learned_log_count", "sbpx_learned_train",
             "sbpx_learned_forward", "sbpx_learned_gradients", "sbpx_batch_define", "sbpx_batch_info",
             "sbpx_batch_build", "sbpx_batch_download", "sbpx_batch_log_posterior", "sbpx_batch_sweep",
-            "sbpx_batch_merge", "sbpx_blockmodel_trace", "sbpx_batch_trace"]
+            "sbpx_batch_merge", "sbpx_blockmodel_trace", "sbpx_batch_trace", "sbpx_learned_eval_moves",
+            "sbpx_learned_scores"]
 
 
 class SbpxError(RuntimeError):
@@ -404,6 +407,27 @@ class Context:
         z = np.zeros((x.shape[0], 32), np.float32)
         self._chk(self.L.sbpx_learned_forward(self.h, x.shape[0], x, z))
         return z
+
+    def learned_eval_moves(self, v, s, u):
+        """Fixed proposals evaluated as the learned mixture's: (dS, H, accept, q_net(s), q_net(r))."""
+        v = np.ascontiguousarray(v, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        u = np.ascontiguousarray(u, np.float64)
+        n = len(v)
+        dS, H = np.zeros(n), np.zeros(n)
+        acc = np.zeros(n, np.uint8)
+        qf, qb = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        self._chk(self.L.sbpx_learned_eval_moves(self.h, n, v, s, u, dS, H, acc, qf, qb))
+        return dS, H, acc, qf, qb
+
+    def learned_scores(self, v, scorer=0):
+        """Candidates [n][32] and scores [n][32] of the listed vertices: scorer 0 the batched path,
+        1 the sequential pass's in-wavefront scorer."""
+        v = np.ascontiguousarray(v, np.int32)
+        cand = np.zeros((len(v), 32), np.int32)
+        z = np.zeros((len(v), 32), np.float32)
+        self._chk(self.L.sbpx_learned_scores(self.h, len(v), v, int(scorer), cand, z))
+        return cand, z
 
     def learned_gradients(self, x, mask, y):
         x = np.ascontiguousarray(x, np.float32)

@@ -132,10 +132,11 @@ void free_mlp(Ctx& c) {
     for (int l = 0; l < 3; ++l) {
         for (float** p : {&m.w[l], &m.b[l], &m.gw[l], &m.gb[l], &m.m1w[l], &m.m2w[l], &m.m1b[l], &m.m2b[l]}) dfree(p);
         dfree(&m.w16[l]);
+        dfree(&m.wt[l]);
     }
-    for (__bf16** p : {&m.h1, &m.h2, &m.dz, &m.dh1, &m.dh2, &m.ones, &m.ds_x, &m.x, &m.x2, &m.sh1, &m.sh2, &m.tx})
+    for (__bf16** p : {&m.h1, &m.h2, &m.dz, &m.dh1, &m.dh2, &m.ones, &m.ds_x, &m.x, &m.sh1, &m.sh2, &m.tx})
         dfree(p);
-    for (float** p : {&m.z, &m.loss, &m.sz, &m.sz2, &m.qf, &m.qb}) dfree(p);
+    for (float** p : {&m.z, &m.loss, &m.sz, &m.qf, &m.qb}) dfree(p);
     dfree(&m.ds_mask);
     dfree(&m.tmask);
     for (int** p : {&m.ds_y, &m.ds_count, &m.cand, &m.cnt, &m.ps, &m.ty, &m.tidx, &m.log_flag, &m.log_rank}) dfree(p);
@@ -153,6 +154,7 @@ void init_mlp(Ctx& c, uint64_t seed) {
             for (float** p : {&m.w[l], &m.gw[l], &m.m1w[l], &m.m2w[l]}) dalloc(p, n);
             for (float** p : {&m.b[l], &m.gb[l], &m.m1b[l], &m.m2b[l]}) dalloc(p, nb);
             dalloc(&m.w16[l], n);
+            dalloc(&m.wt[l], n);
         }
         for (__bf16** p : {&m.h1, &m.h2, &m.dh1, &m.dh2}) dalloc(p, (size_t)MLP_BATCH * MLP_H);
         dalloc(&m.dz, (size_t)MLP_BATCH * MLP_OUT);
@@ -224,11 +226,9 @@ void sbpx::ensure_mlp_rows(Ctx& c) {
     const size_t N = (size_t)std::max(c.g.N, 1);
     if (m.rows >= (int)N) return;
     dalloc(&m.x, N * MLP_IN);
-    dalloc(&m.x2, N * MLP_IN);
     dalloc(&m.sh1, N * MLP_H);
     dalloc(&m.sh2, N * MLP_H);
     dalloc(&m.sz, N * MLP_OUT);
-    dalloc(&m.sz2, N * MLP_OUT);
     dalloc(&m.cand, N * MLP_CAND);
     dalloc(&m.cnt, N * MLP_CAND);
     dalloc(&m.qf, N);
@@ -506,9 +506,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
         if (option == 1) {
             ctx->count_bytes = value != 0;
         } else if (option == 4) {
-            if (value < 0 || value > 3)
-                throw ArgError("proposal must be 0 (plain), 1 (learned mixture in the batched sweeps), 2 (also in "
-                               "the sequential pass, one wavefront per vertex) or 3 (also its wavefront groups)");
+            if (value < 0 || value > 2)
+                throw ArgError("proposal must be 0 (plain), 1 (learned mixture in the batched sweeps) or 2 (also in "
+                               "the sequential Metropolis-Hastings pass)");
             ctx->c.proposal = (int)value;
         } else if (option == 3) {
             if (value < 0 || value > 3) throw ArgError("evaluation kernel variant must be 0 (tuned), 1, 2 or 3");
@@ -787,14 +787,23 @@ sbpx_status sbpx_mdl(sbpx_ctx* ctx, double* out) {
     });
 }
 
+// Accept uniforms of a fixed evaluation, by vertex, for the learned bins.
+__global__ void scatter_u_kernel(const int* __restrict__ list, int n, const double* __restrict__ u,
+                                 double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[list[i]] = u[i];
+}
+
 static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
                              double* dS, double* H, uint8_t* acc, uint64_t seed, uint32_t call, int tag,
-                             int32_t* s_out, double* u_out);
+                             int32_t* s_out, double* u_out, bool learned, float* qf, float* qb);
 
-// Requests longer than the per-vertex buffers (N entries) run in chunks.
+// Requests longer than the per-vertex buffers (N entries) run in chunks. learned: the proposals
+// are the learned mixture's (Hastings ratio of the mixture; qf / qb receive the network's forward
+// and reverse probabilities, -1 where the vertex has no learned part).
 static void eval_fixed(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u, double* dS,
                        double* H, uint8_t* acc, uint64_t seed, uint32_t call, int tag, int32_t* s_out,
-                       double* u_out) {
+                       double* u_out, bool learned = false, float* qf = nullptr, float* qb = nullptr) {
     Ctx& c = ctx->c;
     require_bm(c);
     if (n < 0) throw ArgError("negative count");
@@ -803,13 +812,14 @@ static void eval_fixed(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t
         int m = std::min(chunk, n - off);
         eval_fixed_chunk(ctx, m, v + off, s ? s + off : nullptr, u ? u + off : nullptr, dS ? dS + off : nullptr,
                          H ? H + off : nullptr, acc ? acc + off : nullptr, seed, call, tag,
-                         s_out ? s_out + off : nullptr, u_out ? u_out + off : nullptr);
+                         s_out ? s_out + off : nullptr, u_out ? u_out + off : nullptr, learned,
+                         qf ? qf + off : nullptr, qb ? qb + off : nullptr);
     }
 }
 
 static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
                              double* dS, double* H, uint8_t* acc, uint64_t seed, uint32_t call, int tag,
-                             int32_t* s_out, double* u_out) {
+                             int32_t* s_out, double* u_out, bool learned, float* qf, float* qb) {
     Ctx& c = ctx->c;
     if (n == 0) return;
     for (int i = 0; i < n; ++i)
@@ -839,6 +849,19 @@ static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const i
     ensure_pool(c, (size_t)hoff.back());
     int* d_hoff = c.aux_buf + (c.g.N + 1);
     SBPX_HIP(hipMemcpyAsync(d_hoff, hoff.data(), sizeof(int) * hoff.size(), hipMemcpyHostToDevice, c.stream));
+    // the learned bins (as a sweep covers them): contiguous at the start of the ordered list
+    int nb = 0, nlow = 0;
+    if (learned) {
+        nb = 3;
+        while (nb < NUM_LDS_BINS && BIN_MAXDEG[nb] <= c.learn_maxdeg) ++nb;
+        for (int b = 0; b < nb; ++b) nlow += counts[b];
+        if (nlow > 0) {
+            learned_features(c, d_list, nlow, BIN_MAXDEG[nb - 1]);
+            learned_mixture(c, d_list, nlow, 0, 0, d_s);
+            scatter_u_kernel<<<(nlow + 255) / 256, 256, 0, c.stream>>>(d_list, nlow, d_u, c.mlp.pu);
+            SBPX_HIP(hipGetLastError());
+        }
+    }
     int off = 0;
     for (int b = 0; b < NUM_BINS; ++b) {
         if (counts[b]) {
@@ -846,6 +869,8 @@ static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const i
             L.list = d_list + off;
             L.n = counts[b];
             L.bin = b;
+            L.learned = learned && b < nb;
+            L.learned_u = c.mlp.pu;
             L.variant = c.eval_variant ? c.pick_variant(b) : 0;  // a fixed variant applies here too
             L.given_s = s ? d_s + off : nullptr;
             L.given_u = s ? d_u + off : nullptr;
@@ -864,6 +889,13 @@ static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const i
     std::vector<int> rs(n);
     std::vector<uint8_t> racc(n);
     std::vector<double> rdS(n), rH(n), ru(n);
+    std::vector<float> rqf, rqb;
+    if (learned && (qf || qb)) {
+        rqf.assign(c.g.N, -1.0f);
+        rqb.assign(c.g.N, -1.0f);
+        SBPX_HIP(hipMemcpyAsync(rqf.data(), c.mlp.qf, sizeof(float) * c.g.N, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(rqb.data(), c.mlp.qb, sizeof(float) * c.g.N, hipMemcpyDeviceToHost, c.stream));
+    }
     SBPX_HIP(hipMemcpyAsync(rs.data(), c.ev_s, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
     SBPX_HIP(hipMemcpyAsync(racc.data(), c.ev_acc, n, hipMemcpyDeviceToHost, c.stream));
     SBPX_HIP(hipMemcpyAsync(rdS.data(), c.ev_dS, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
@@ -878,6 +910,8 @@ static void eval_fixed_chunk(sbpx_ctx* ctx, int32_t n, const int32_t* v, const i
         if (acc) acc[o] = racc[i] == 2 ? 1 : 0;
         if (s_out) s_out[o] = rs[i];
         if (u_out) u_out[o] = ru[i];
+        if (qf) qf[o] = i < nlow ? rqf[ordered[i]] : -1.0f;
+        if (qb) qb[o] = i < nlow ? rqb[ordered[i]] : -1.0f;
     }
 }
 
@@ -886,6 +920,57 @@ sbpx_status sbpx_eval_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const in
     return guarded(ctx, [&] {
         if (n > 0 && (!v || !s || !u)) throw ArgError("null proposal array");
         eval_fixed(ctx, n, v, s, u, dS, H, accept, 0, 0, 0, nullptr, nullptr);
+    });
+}
+
+sbpx_status sbpx_learned_eval_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
+                                    double* dS, double* H, uint8_t* accept, float* qf, float* qb) {
+    return guarded(ctx, [&] {
+        if (n > 0 && (!v || !s || !u)) throw ArgError("null proposal array");
+        Ctx& c = ctx->c;
+        if (!c.mlp.ready) throw StateError("no proposal network initialised (sbpx_learned_init)");
+        require_bm(c);
+        std::vector<char> seen(c.g.N, 0);
+        for (int i = 0; i < n; ++i) {
+            if (v[i] < 0 || v[i] >= c.g.N) throw ArgError("vertex out of range");
+            if (seen[v[i]]) throw ArgError("vertex listed twice");
+            seen[v[i]] = 1;
+        }
+        ensure_mlp_rows(c);
+        eval_fixed(ctx, n, v, s, u, dS, H, accept, 0, 0, 0, nullptr, nullptr, true, qf, qb);
+    });
+}
+
+sbpx_status sbpx_learned_scores(sbpx_ctx* ctx, int32_t n, const int32_t* v, int scorer, int32_t* cand, float* z) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        if (!c.mlp.ready) throw StateError("no proposal network initialised (sbpx_learned_init)");
+        require_bm(c);
+        if (n < 0 || n > c.g.N || n > MLP_BATCH) throw ArgError("count must be in [0, min(N, 8192)]");
+        if (scorer != 0 && scorer != 1) throw ArgError("scorer must be 0 (batched) or 1 (sequential pass)");
+        if (n > 0 && (!v || !cand || !z)) throw ArgError("null array");
+        int max_deg = 0;
+        for (int i = 0; i < n; ++i) {
+            if (v[i] < 0 || v[i] >= c.g.N) throw ArgError("vertex out of range");
+            max_deg = std::max(max_deg, c.g.degree[v[i]]);
+        }
+        if (max_deg > LEARN_MH_MAXDEG) throw ArgError("vertex degree above 510");
+        if (n == 0) return;
+        ensure_mlp_rows(c);
+        int* d_list = c.list_buf;
+        SBPX_HIP(hipMemcpyAsync(d_list, v, sizeof(int) * n, hipMemcpyHostToDevice, c.stream));
+        int* d_cand = c.mlp.cand;
+        float* d_z = c.mlp.sz;
+        if (scorer == 0) {
+            learned_features(c, d_list, n, max_deg);
+            mlp_forward(c, c.mlp.x, n, c.mlp.sh1, c.mlp.sh2, d_z);
+        } else {
+            learned_scores_probe(c, d_list, n, max_deg, d_cand, d_z);
+        }
+        SBPX_HIP(hipMemcpyAsync(cand, d_cand, sizeof(int) * n * MLP_CAND, hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipMemcpyAsync(z, d_z, sizeof(float) * n * MLP_OUT, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        check_device_error(c);
     });
 }
 

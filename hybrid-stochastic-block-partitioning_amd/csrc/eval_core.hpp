@@ -290,6 +290,40 @@ __device__ __forceinline__ int tab_find(const int* key, int cap, int b) {
 
 __device__ __forceinline__ float fi(int x) { return (float)x; }
 
+// Feature j (< 64) of the learned proposal for a vertex in block `own` with candidates cand[32]
+// (blocks, -1 past the last) and their edge counts cnt[32], nuniq distinct neighbour blocks. Block
+// degrees are taken without the vertex's own contribution, so the features, and the scores, are
+// the same before and after the vertex moves.
+__device__ inline float learned_feature(int j, const int* cand, const int* cnt, int k_out, int k_in, int self, int own,
+                                        int nuniq, const int* din, const int* dout, float inv_logE,
+                                        float log_ratio) {
+    const int k = k_out + k_in;
+    const float invk = k > 0 ? 1.0f / (float)k : 0.0f;
+    if (j < 16) return cand[j] >= 0 ? (float)cnt[j] * invk : 0.0f;
+    if (j < 32) {
+        const int b = cand[j - 16];
+        return b >= 0 ? log1pf((float)(din[b] - (b == own ? k_in + self : 0))) * inv_logE : 0.0f;
+    }
+    if (j < 48) {
+        const int b = cand[j - 32];
+        return b >= 0 ? log1pf((float)(dout[b] - (b == own ? k_out : 0))) * inv_logE : 0.0f;
+    }
+    const float inv_logD = 1.0f / logf(4097.0f);
+    switch (j - 48) {
+        case 0: return log1pf((float)k_out) * inv_logD;
+        case 1: return log1pf((float)k_in) * inv_logD;
+        case 2: {
+            int rest = 0;
+            for (int i = 16; i < MLP_CAND; ++i) rest += cand[i] >= 0 ? cnt[i] : 0;
+            return (float)rest * invk;
+        }
+        case 3: return (float)nuniq * invk;
+        case 4: return log_ratio;
+        case 5: return 1.0f;
+        default: return 0.0f;
+    }
+}
+
 // Softmax probability of candidate `pick` among the candidates other than `own` (-1 when no
 // candidate qualifies; 0 when pick is not one of them).
 __device__ inline float cand_prob(const float* z, const int* cand, int own, int pick) {
@@ -311,6 +345,30 @@ __device__ inline float cand_prob(const float* z, const int* cand, int own, int 
             if (cand[j] == pick) num = e;
         }
     return num / den;
+}
+
+// A draw from the softmax of the scores over the candidates other than `own`, walked in slot
+// order with the weights cand_prob forms (u01 uniform in [0, 1)); -1 when no candidate qualifies.
+__device__ inline int learned_pick(const float* z, const int* cand, int own, double u01) {
+    float mx = -1e30f;
+#pragma unroll 1
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) mx = fmaxf(mx, z[j]);
+    float den = 0.0f;
+#pragma unroll 1
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) den += expf(z[j] - mx);
+    const float target = (float)u01 * den;
+    float run = 0.0f;
+    int pick = -1;
+#pragma unroll 1
+    for (int j = 0; j < MLP_CAND; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            run += expf(z[j] - mx);
+            pick = cand[j];
+            if (run > target) break;
+        }
+    return pick;
 }
 
 // Table layout: 7 arrays of cap ints: key, o, i, M[s,b], M[b,s], M[r,b], M[b,r].

@@ -381,6 +381,7 @@ struct EvalLaunch {
     double* out_u;          // optional: the accept uniform used
     const int* huge_offsets;
     long long* bytes;       // optional algorithmic byte counter
+    const double* learned_u;  // learned bins: accept uniforms by vertex (null: the propose pass's)
 };
 void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
@@ -390,9 +391,9 @@ void finish_degrees(Ctx& c);
 // vertices with more than skip_deg out + in list entries get an empty row (no learned part).
 void learned_features(Ctx& c, const int* list, int n, int max_deg = LEARN_MAXDEG, int skip_deg = INT_MAX);
 void learned_log(Ctx& c, const int* list, int n);
-void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call);
-// Scores the queue of a sequential pass once and draws each vertex's learned branch.
-void learned_mh_draws(Ctx& c, const int* list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag);
+// The mixture's proposals (drawn, or s_given[i] for list entry i) and the net's forward and
+// reverse probabilities, for the listed vertices whose features learned_features wrote.
+void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t call, const int* s_given = nullptr);
 void ensure_mlp_rows(Ctx& c);
 void mlp_forward(Ctx& c, const __bf16* X, int V, __bf16* H1, __bf16* H2, float* Z);
 void mlp_backward(Ctx& c, const __bf16* X, const uint32_t* cmask, const int* y, int V, float* loss,
@@ -400,6 +401,10 @@ void mlp_backward(Ctx& c, const __bf16* X, const uint32_t* cmask, const int* y, 
 void mlp_adam_step(Ctx& c, float lr);
 void mlp_gather(Ctx& c, const int* idx, int n);
 void mlp_sync_bf16(Ctx& c);
+void mlp_transpose_bf16(Ctx& c);  // input-major BF16 weights for the sequential pass
+// The sequential pass's in-wavefront scorer on the listed vertices (degree <= max_deg): candidates
+// and scores [n][32].
+void learned_scores_probe(Ctx& c, const int* d_list, int n, int max_deg, int* d_cand, float* d_z);
 void mlp_convert_to_bf16(Ctx& c, const float* x, long n, __bf16* y);
 void mlp_convert_to_f32(Ctx& c, const __bf16* x, long n, float* y);
 

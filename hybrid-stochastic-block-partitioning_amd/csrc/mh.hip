@@ -80,6 +80,19 @@ __global__ void hash_fill_kernel(const int* __restrict__ ptr, const int* __restr
 // ------------------------------------------------------------------------------------------
 struct WideSlot;
 
+// The proposal network as the sequential pass reads it (see learned_scores_wave).
+struct LearnedNet {
+    const __bf16* w1t;  // [64][256], input-major
+    const __bf16* w2t;  // [256][256]
+    const __bf16* w3t;  // [256][32]
+    const float* b1;
+    const float* b2;
+    const float* b3;
+    float inv_logE;     // 1 / log1p(E)
+    float log_ratio;    // log(B) / log(N)
+    int maxdeg;         // vertices of larger degree (out + in) keep the plain proposal
+};
+
 struct MhParams {
     GraphView G;
     // mutable state, read and written through the same (non-restrict) pointers
@@ -122,13 +135,9 @@ struct MhParams {
     WideSlot* wide;                 // [16 K] one record per window entry
     unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
     int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
-                                    // [2] next queue position, [3] stop reason    // the learned mixture (null: plain proposals): per queue entry the candidate scores and blocks,
-    // per vertex the learned draw (-2: plain only, -1: the plain branch, else the target) and its
-    // accept uniform
-    const float* lz;
-    const int* lcand;
-    const int* lps;
-    const double* lpu;
+                                    // [2] next queue position, [3] stop reason
+    // the learned mixture (LEARNED kernels only): the network and the degree bound
+    LearnedNet net;
 };
 
 __device__ __forceinline__ int room(const int* capa, const int* occ, int x) { return (capa[x] * 3) / 4 - occ[x]; }
@@ -406,6 +415,185 @@ __device__ __forceinline__ int vertex_table_cap(const GraphView& G, int v) {
     return cap;
 }
 
+// ------------------------------------------------------------------------------------------
+// The learned mixture in the sequential pass. Every step's proposal must be a function of the
+// state the step starts from, so the network scores each vertex's candidates inside its
+// evaluation, from the current state: one wavefront gathers the neighbour blocks, takes the top
+// 32, forms the 64 features and runs the 64 -> 256 -> 256 -> 32 network on its own lanes (weights
+// read input-major in BF16 from the L2, activations rounded to BF16 as the batched products round
+// them, FP32 sums). The features do not change when the vertex itself moves, so these scores give
+// both the forward and the reverse learned probability. Everything they read (the neighbours'
+// blocks, the degrees of the vertex's neighbour blocks and of its own block) is covered by the
+// window checks, as the evaluation's own reads are.
+
+__device__ __forceinline__ float bf16_round(float f) {  // nearest BF16, ties to even, as a float
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+__device__ __forceinline__ float lane_value(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// Scores of vertex v (block r) in one wavefront. tab (cap slots, 2 * cap ints) is scratch for the
+// neighbour-block counts; cand[32], cnt[32] and z[32] receive the candidates, their edge counts
+// and their scores (wavefront-private, LDS). Returns whether some candidate is not r.
+__device__ bool learned_scores_wave(const GraphView& G, const BlockState& S, int v, int r, int* tab, int cap,
+                                    const LearnedNet& N, int* cand, int* cnt, float* z) {
+    const int lane = lane_id();
+    int* key = tab;
+    int* count = tab + cap;
+    for (int j = lane; j < cap; j += WAVE) {
+        key[j] = -1;
+        count[j] = 0;
+    }
+    fence_wave();
+    const int ob = G.out_ptr[v], dO = G.out_ptr[v + 1] - ob;
+    const int ib = G.in_ptr[v], dI = G.in_ptr[v + 1] - ib;
+    const int self = G.self[v];
+    for (int j = lane; j < dO + dI; j += WAVE) {
+        const int w = j < dO ? G.out_idx[ob + j] : G.in_idx[ib + (j - dO)];
+        if (w == v) continue;
+        const int slot = tab_insert(key, cap, S.assign[w]);
+        if (slot >= 0) atomicAdd(&count[slot], 1);
+    }
+    fence_wave();
+    int nuniq = 0;
+    for (int j = lane; j < cap; j += WAVE) nuniq += key[j] >= 0 ? 1 : 0;
+    nuniq = (int)wave_sum_ll(nuniq);
+    // top 32 by (count desc, block asc); a taken slot's count is negated
+    for (int t = 0; t < MLP_CAND; ++t) {
+        long long best = -1;
+        int bslot = -1;
+        for (int j = lane; j < cap; j += WAVE) {
+            const int kk = key[j], c = count[j];
+            const long long sc = (kk >= 0 && c > 0) ? (((long long)c << 32) | (long long)(0x7FFFFFFF - kk)) : -1;
+            if (sc > best) {
+                best = sc;
+                bslot = j;
+            }
+        }
+        long long wb = best;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const long long o = __shfl_xor(wb, off, WAVE);
+            wb = o > wb ? o : wb;
+        }
+        int b = -1, c = 0;
+        if (wb >= 0) {
+            b = 0x7FFFFFFF - (int)(wb & 0xFFFFFFFF);
+            c = (int)(wb >> 32);
+            if (best == wb) count[bslot] = -c;  // one slot per block: one lane owns it
+        }
+        if (lane == 0) {
+            cand[t] = b;
+            cnt[t] = c;
+        }
+        fence_wave();
+    }
+    const int k_out = dO, k_in = dI - self;
+    const float x = bf16_round(
+        learned_feature(lane, cand, cnt, k_out, k_in, self, r, nuniq, S.din, S.dout, N.inv_logE, N.log_ratio));
+    // layer 1: lane l computes outputs 4l .. 4l + 3
+    const int o4 = 4 * lane;
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    {
+        const uint2* w = (const uint2*)N.w1t + lane;
+#pragma unroll 4
+        for (int j = 0; j < MLP_IN; ++j) {
+            const float xj = lane_value(x, j);
+            const uint2 q = w[j * (MLP_H / 4)];
+            a0 = fmaf(xj, bf16_lo(q.x), a0);
+            a1 = fmaf(xj, bf16_hi(q.x), a1);
+            a2 = fmaf(xj, bf16_lo(q.y), a2);
+            a3 = fmaf(xj, bf16_hi(q.y), a3);
+        }
+    }
+    const float h0 = bf16_round(fmaxf(a0 + N.b1[o4], 0.0f)), h1 = bf16_round(fmaxf(a1 + N.b1[o4 + 1], 0.0f));
+    const float h2 = bf16_round(fmaxf(a2 + N.b1[o4 + 2], 0.0f)), h3 = bf16_round(fmaxf(a3 + N.b1[o4 + 3], 0.0f));
+    // layer 2: input j = 4 m + q sits in lane m, component q
+    a0 = a1 = a2 = a3 = 0.0f;
+    {
+        const uint2* w = (const uint2*)N.w2t + lane;
+#pragma unroll 2
+        for (int m = 0; m < MLP_H / 4; ++m) {
+            const float hj[4] = {lane_value(h0, m), lane_value(h1, m), lane_value(h2, m), lane_value(h3, m)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint2 wq = w[(4 * m + q) * (MLP_H / 4)];
+                a0 = fmaf(hj[q], bf16_lo(wq.x), a0);
+                a1 = fmaf(hj[q], bf16_hi(wq.x), a1);
+                a2 = fmaf(hj[q], bf16_lo(wq.y), a2);
+                a3 = fmaf(hj[q], bf16_hi(wq.y), a3);
+            }
+        }
+    }
+    const float g0 = bf16_round(fmaxf(a0 + N.b2[o4], 0.0f)), g1 = bf16_round(fmaxf(a1 + N.b2[o4 + 1], 0.0f));
+    const float g2 = bf16_round(fmaxf(a2 + N.b2[o4 + 2], 0.0f)), g3 = bf16_round(fmaxf(a3 + N.b2[o4 + 3], 0.0f));
+    // layer 3: lane l takes output l & 31 over inputs [128 (l >> 5), 128 (l >> 5) + 128); the two
+    // halves are added with one exchange
+    const int o = lane & 31, half = lane >> 5;
+    float e = 0.0f;
+    {
+        const unsigned short* w = (const unsigned short*)N.w3t + (half * 128) * MLP_OUT + o;
+#pragma unroll 2
+        for (int m = 0; m < 32; ++m) {
+            const float gl[4] = {lane_value(g0, m), lane_value(g1, m), lane_value(g2, m), lane_value(g3, m)};
+            const float gh[4] = {lane_value(g0, 32 + m), lane_value(g1, 32 + m), lane_value(g2, 32 + m),
+                                 lane_value(g3, 32 + m)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float wv = __uint_as_float((uint32_t)w[(4 * m + q) * MLP_OUT] << 16);
+                e = fmaf(half ? gh[q] : gl[q], wv, e);
+            }
+        }
+    }
+    e += __shfl_xor(e, 32, WAVE);
+    if (lane < MLP_OUT) z[o] = e + N.b3[o];
+    fence_wave();
+    int any = 0;
+    for (int j = 0; j < MLP_CAND; ++j) any |= cand[j] >= 0 && cand[j] != r;
+    return any != 0;
+}
+
+// The mixture's choice for vertex v (block r) with scores in z / cand: on the learned branch the
+// target and the accept uniform of the plain stream; otherwise gs = -1 (the plain draw follows).
+__device__ __forceinline__ void learned_branch(uint64_t seed, uint32_t call, uint32_t tag, int v, int r,
+                                               const float* z, const int* cand, bool has, int* gs, double* gu) {
+    *gs = -1;
+    *gu = 0.0;
+    if (!has) return;
+    const Draw4 dl = philox4x32_10(seed, (uint32_t)v, call, 0u, TAG_LEARN);
+    if (unit53(dl.x0, dl.x1) < (double)LEARN_W_PLAIN) return;
+    *gs = learned_pick(z, cand, r, unit53(dl.x2, dl.x3));
+    const Draw4 dr = philox4x32_10(seed, (uint32_t)v, call, 0u, tag);
+    *gu = unit53(dr.x2, dr.x3);
+}
+
+// The sequential pass's scorer on its own, one wavefront per listed vertex (tables: 2 * tcap ints
+// of scratch per vertex), for comparing its candidates and scores with the batched path's.
+constexpr int PROBE_WAVES = 4;
+__global__ void __launch_bounds__(PROBE_WAVES * WAVE) scorer_probe_kernel(GraphView G, BlockState S,
+                                                                          const int* __restrict__ list, int n,
+                                                                          int* tables, int tcap, LearnedNet N,
+                                                                          int* __restrict__ cand_out,
+                                                                          float* __restrict__ z_out) {
+    __shared__ int lc[PROBE_WAVES][MLP_CAND], lcnt[PROBE_WAVES][MLP_CAND];
+    __shared__ float lz[PROBE_WAVES][MLP_OUT];
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    const int i = blockIdx.x * PROBE_WAVES + wave;
+    if (i >= n) return;
+    const int v = list[i];
+    learned_scores_wave(G, S, v, S.assign[v], tables + (size_t)i * 2 * tcap, vertex_table_cap(G, v), N, lc[wave],
+                        lcnt[wave], lz[wave]);
+    if (lane < MLP_CAND) {
+        cand_out[(long)i * MLP_CAND + lane] = lc[wave][lane];
+        z_out[(long)i * MLP_OUT + lane] = lz[wave][lane];
+    }
+}
+
 // Algorithmic bytes of an evaluation that stands: the formula of the batched kernels, with rows
 // and columns of the hash tables counted by the keys they hold.
 __device__ __forceinline__ long long eval_bytes(const MhParams& P, const MoveResult& R) {
@@ -593,7 +781,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     }
 }
 
-// LEARNED: the learned mixture of the pass (as in mh_wide_kernel).
+// LEARNED: the learned mixture, scored per evaluation on the current state (learned_scores_wave).
 template <int G, bool LEARNED>
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     constexpr int NG = MH_WAVES / G;
@@ -609,6 +797,11 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     __shared__ int lds_hist[NG][256];
     __shared__ double redd[MH_WAVES][RED_D];
     __shared__ int redi[MH_WAVES][RED_I];
+    // the learned mixture: per group the candidates, counts, scores and the branch drawn
+    __shared__ int s_lc[LEARNED ? NG : 1][MLP_CAND], s_lcnt[LEARNED ? NG : 1][MLP_CAND];
+    __shared__ float s_lz[LEARNED ? NG : 1][MLP_OUT];
+    __shared__ int s_lon[NG], s_lgs[NG];
+    __shared__ double s_lgu[NG];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
     const int g = wave / G, wig = wave % G;
     const bool leader = wig == 0;  // the group's first wavefront commits and checks
@@ -630,13 +823,28 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
         const int cap = active ? vertex_table_cap(Gr, v) : 16;
         int* tab = cap <= LCAP ? lds_tab[g] : gtab;
         MoveResult R;
-        if constexpr (LEARNED) {  // scores fixed for the pass; the draw made with them
-            const int lp = active ? P.lps[v] : -2;
-            const double lw = lp >= -1 ? (double)LEARN_W_PLAIN : -1.0;
-            R = eval_group<G, HashView, true>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, lp >= 0 ? lp : -1,
-                                        lp >= 0 ? P.lpu[v] : 0.0, false, lds_hist[g], redd, redi, lw,
-                                        active ? P.lz + (long)idx * MLP_OUT : nullptr,
-                                        active ? P.lcand + (long)idx * MLP_CAND : nullptr);
+        if constexpr (LEARNED) {  // the group's first wavefront scores the vertex on the current state
+            if (leader) {
+                int on = 0, gs = -1;
+                double gu = 0.0;
+                if (active && (Gr.out_ptr[v + 1] - Gr.out_ptr[v]) + (Gr.in_ptr[v + 1] - Gr.in_ptr[v]) <= P.net.maxdeg) {
+                    on = 1;
+                    const int r = S.assign[v];
+                    const bool has = learned_scores_wave(Gr, S, v, r, tab, cap, P.net, s_lc[g], s_lcnt[g], s_lz[g]);
+                    learned_branch(P.seed, P.call, P.tag, v, r, s_lz[g], s_lc[g], has, &gs, &gu);
+                }
+                if (lane == 0) {
+                    s_lon[g] = on;
+                    s_lgs[g] = gs;
+                    s_lgu[g] = gu;
+                }
+            }
+            __syncthreads();
+            const bool on = s_lon[g] != 0;
+            R = eval_group<G, HashView, true>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
+                                              on ? s_lgu[g] : 0.0, false, lds_hist[g], redd, redi,
+                                              on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
+                                              on ? s_lc[g] : nullptr);
         } else {
             R = eval_group<G, HashView>(Gr, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g], redd,
                                         redi);
@@ -807,9 +1015,8 @@ __global__ void wide_init_kernel(int* ctrl) {
     ctrl[3] = 0;
 }
 
-// LEARNED: the queue's proposals are the learned mixture (P.lz, P.lcand, P.lps, P.lpu set). A
-// separate instantiation keeps the plain kernel free of the second inlined evaluator, whose
-// registers would otherwise spill in the plain pass too.
+// LEARNED: the proposals are the learned mixture, scored per evaluation on the current state
+// (learned_scores_wave). A separate instantiation keeps the plain kernel free of the scorer.
 template <bool LEARNED>
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     __shared__ int s_ok, s_flag, s_n, s_end, s_stop, s_need;
@@ -821,6 +1028,9 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     __shared__ double s_dS[MH_WIDE_SLOTS];
     __shared__ long long s_red[MH_WAVES][4];
     __shared__ double s_redd[MH_WAVES][2];
+    // the learned mixture's candidates, their counts and scores, per wavefront
+    __shared__ int s_lc[LEARNED ? MH_WAVES : 1][MLP_CAND], s_lcnt[LEARNED ? MH_WAVES : 1][MLP_CAND];
+    __shared__ float s_lz[LEARNED ? MH_WAVES : 1][MLP_OUT];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
     const int slot = (int)blockIdx.x * MH_WAVES + wave;
     const int WS = (int)gridDim.x * MH_WAVES;  // entries per window
@@ -851,13 +1061,20 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             const int v = P.list[pos + slot];
             cap = vertex_table_cap(G, v);
             if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
-            if constexpr (LEARNED) {  // the learned mixture: scores fixed for the pass
-                const int lp = P.lps[v];
-                const double lw = lp >= -1 ? (double)LEARN_W_PLAIN : -1.0;
-                const long row = pos + slot;
-                R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, lp >= 0 ? lp : -1,
-                                                   lp >= 0 ? P.lpu[v] : 0.0, false, lw, 0.0, 0.0,
-                                                   P.lz + row * MLP_OUT, P.lcand + row * MLP_CAND);
+            if constexpr (LEARNED) {  // the learned mixture, scored on the current state
+                const int deg = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
+                const bool on = deg <= P.net.maxdeg;
+                int gs = -1;
+                double gu = 0.0;
+                if (on) {
+                    const int r = S.assign[v];
+                    const bool has =
+                        learned_scores_wave(G, S, v, r, tab, cap, P.net, s_lc[wave], s_lcnt[wave], s_lz[wave]);
+                    learned_branch(P.seed, P.call, P.tag, v, r, s_lz[wave], s_lc[wave], has, &gs, &gu);
+                }
+                R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, false,
+                                                   on ? (double)LEARN_W_PLAIN : -1.0, 0.0, 0.0,
+                                                   on ? s_lz[wave] : nullptr, on ? s_lc[wave] : nullptr);
             } else {
                 R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
             }
@@ -1440,6 +1657,31 @@ static void launch_group(const MhParams& P, bool learned, hipStream_t st) {
         mh_group_kernel<G, false><<<1, MH_WAVES * WAVE, 0, st>>>(P);
 }
 
+static LearnedNet learned_net(Ctx& c) {
+    mlp_transpose_bf16(c);
+    return LearnedNet{c.mlp.wt[0], c.mlp.wt[1], c.mlp.wt[2], c.mlp.b[0], c.mlp.b[1], c.mlp.b[2],
+                      1.0f / log1pf((float)c.g.E),
+                      c.g.N > 1 ? (float)(log((double)c.bm.B) / log((double)c.g.N)) : 0.0f,
+                      std::min(c.learn_maxdeg, LEARN_MH_MAXDEG)};
+}
+
+void learned_scores_probe(Ctx& c, const int* d_list, int n, int max_deg, int* d_cand, float* d_z) {
+    if (n <= 0) return;
+    int tcap = 16;  // max_deg counts a self-loop once; vertex_table_cap counts it in both lists
+    while (tcap < 2 * (max_deg + 3)) tcap <<= 1;
+    int* tables = nullptr;
+    SBPX_HIP(hipMalloc((void**)&tables, sizeof(int) * 2 * (size_t)tcap * n));
+    const LearnedNet net = learned_net(c);
+    const GraphView G{c.g.out_ptr, c.g.out_idx, c.g.in_ptr, c.g.in_idx, c.g.self_flag};
+    const BlockState S{c.bm.assign, c.bm.dout, c.bm.din, c.bm.deg, c.bm.B};
+    scorer_probe_kernel<<<(n + PROBE_WAVES - 1) / PROBE_WAVES, PROBE_WAVES * WAVE, 0, c.stream>>>(
+        G, S, d_list, n, tables, tcap, net, d_cand, d_z);
+    const hipError_t e = hipGetLastError();
+    SBPX_HIP(hipStreamSynchronize(c.stream));
+    SBPX_HIP(hipFree(tables));
+    SBPX_HIP(e);
+}
+
 void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed, uint32_t call, uint32_t tag,
                    long long* bytes, bool timed) {
     if (n <= 0) return;
@@ -1462,13 +1704,13 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
         h.table_slots = tslots;
     }
-    // the learned mixture: the queue's candidate scores, fixed for the pass, and each vertex's branch
+    // the learned mixture: scored per evaluation on the current state, from input-major BF16 weights
     const bool learned = c.proposal >= 2;
+    LearnedNet net{};
     if (learned) {
         if (!c.mlp.ready) throw StateError("learned proposal selected but no network initialised");
         if (c.mh_wide == 1) throw StateError("the learned sequential pass runs in the window-check kernel (option 6 != 1)");
-        ensure_mlp_rows(c);
-        learned_mh_draws(c, d_list, n, max_deg, seed, call, tag);
+        net = learned_net(c);
     }
     if (!h.assign0) SBPX_HIP(hipMalloc((void**)&h.assign0, sizeof(int) * std::max(c.g.N, 1)));
     SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToDevice, c.stream));
@@ -1515,10 +1757,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.wide = (WideSlot*)h.wide;
         P.wstamp = h.wstamp;
         P.ctrl = h.ctrl;
-        P.lz = learned ? c.mlp.sz : nullptr;
-        P.lcand = learned ? c.mlp.cand : nullptr;
-        P.lps = learned ? c.mlp.ps : nullptr;
-        P.lpu = learned ? c.mlp.pu : nullptr;
+        P.net = net;
         bool out_of_room = false;
         // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
         // previous chunk: the mean number of row/column slots its evaluations scanned (long scans
@@ -1542,15 +1781,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (G == 1 && wide) K = c.mh_wide ? c.mh_wide : h.k_yield;
             // one workgroup, one wavefront per vertex: the window checks of the wide kernel
             const bool wide1 = K == 1 && G == 1 && wide && c.mh_wide == 0;
-            // the learned mixture: the window-check kernel for one wavefront per vertex (proposal 2
-            // and 3; mh_kernel does not run, the learned pass needs option 6 != 1), the group kernels
-            // for wavefront groups (proposal 3; with 2 those chunks keep the plain proposal, each
-            // step of the chain a valid MH step either way)
-            const bool lchunk = learned && (G == 1 || c.proposal == 3);
-            P.lz = lchunk ? c.mlp.sz : nullptr;
-            P.lcand = lchunk ? c.mlp.cand : nullptr;
-            P.lps = lchunk ? c.mlp.ps : nullptr;
-            P.lpu = lchunk ? c.mlp.pu : nullptr;
+            // the learned mixture runs in every chunk: the window-check kernel for one wavefront
+            // per vertex, the group kernels for wavefront groups
+            const bool lchunk = learned;
             P.start = pos;
             P.stop_at = std::min(n, pos + MH_CHUNK);
             SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));

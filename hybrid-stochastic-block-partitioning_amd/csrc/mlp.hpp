@@ -9,7 +9,7 @@ namespace sbpx {
 constexpr int MLP_IN = 64;     // features per vertex
 constexpr int MLP_H = 256;     // hidden width (two layers)
 constexpr int MLP_OUT = 32;    // output scores
-constexpr int MLP_CAND = 16;   // candidate blocks scored (the first 16 outputs)
+constexpr int MLP_CAND = 32;   // candidate blocks scored (one output each)
 constexpr int MLP_BATCH = 8192;  // training rows per step (upper bound)
 
 // Weights are stored output-major (W_l[out][in]), so a GEMM reads eight consecutive inputs of one
@@ -27,6 +27,7 @@ struct MlpState {
     bool ready = false;       // weights allocated and initialised
     float* w[3] = {};         // FP32 master weights
     __bf16* w16[3] = {};      // BF16 copies read by the GEMMs
+    __bf16* wt[3] = {};       // BF16 input-major copies read by the sequential pass's scorer
     float* b[3] = {};         // FP32 biases
     float* gw[3] = {};        // gradients
     float* gb[3] = {};
@@ -54,8 +55,8 @@ struct MlpState {
     bool logging = false;
     // sweep-time buffers, [N] rows
     int rows = 0;
-    __bf16 *x = nullptr, *x2 = nullptr, *sh1 = nullptr, *sh2 = nullptr;
-    float *sz = nullptr, *sz2 = nullptr;
+    __bf16 *x = nullptr, *sh1 = nullptr, *sh2 = nullptr;
+    float* sz = nullptr;
     int* cand = nullptr;      // [N][MLP_CAND] candidate blocks (-1: none)
     int* cnt = nullptr;       // [N][MLP_CAND] edges of the vertex to each candidate
     float* qf = nullptr;      // [N] learned forward proposal probability (< 0: no learned mass)

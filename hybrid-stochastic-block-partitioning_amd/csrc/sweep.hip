@@ -200,7 +200,7 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     if (L.learned) {
         if (L.bin >= NUM_LDS_BINS) throw StateError("learned proposals are evaluated by the LDS-table bins only");
         P.lps = c.mlp.ps;
-        P.lpu = c.ev_u;
+        P.lpu = L.learned_u ? L.learned_u : c.ev_u;
         P.lqf = c.mlp.qf;
         P.lqb = c.mlp.qb;
         P.lw = LEARN_W_PLAIN;

@@ -71,10 +71,9 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            vertex's sums over other lanes add them in another order, so delta MDL may differ
  *            in its last bits between them.
  * option 4 = proposal: 0 the reference's, 1 the learned mixture in the batched evaluation
- *            (see sbpx_learned_init), 2 also in the sequential Metropolis-Hastings pass (there the
- *            network scores each queue vertex's candidates once per pass; needs option 6 != 1)
- *            where one wavefront evaluates each vertex, 3 also in the pass's chunks that
- *            wavefront groups evaluate (option 2 > 1 or chosen by row length).
+ *            (see sbpx_learned_init), 2 also in the sequential Metropolis-Hastings pass, where
+ *            each evaluation scores the vertex's candidates on the current state (needs option
+ *            6 != 1).
  * option 5 = block matrix after a batch of moves: 1 (default) updated from the moved vertices'
  *            edges when they are at most a quarter of the edges, else rebuilt by a full sort;
  *            0 always rebuilt. Both give the same arrays.
@@ -87,7 +86,7 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            limits the pool to exactly that many (and restarts the doubling that follows each
  *            time the pool runs out), so that the rebuild-and-resume path runs; for tests.
  * option 8 = degree bound (out + in) of the learned mixture: 0 to 510, default 126. The
- *            sequential pass (proposals 2 and 3) covers every vertex within it; the batched
+ *            sequential pass (proposal 2) covers every vertex within it; the batched
  *            sweeps cover degree <= 126 and, when the bound reaches 254 or 510, the vertices of
  *            degree 127-254 or 127-510 too. Other vertices keep the plain proposal. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
@@ -152,12 +151,15 @@ sbpx_status sbpx_merge_blocks(sbpx_ctx* ctx, int32_t num_to_merge, int mode, uin
                               uint32_t call);
 
 /* ---------------------------------------------------------------------------------------
- * Learned proposal. No counterpart in the reference: a proposal network for the batched
- * (asynchronous Gibbs) evaluation, mixed with the reference's proposal and corrected in the
- * Hastings ratio, so the chain's target and MDL stay the reference's. Network: 64 features ->
- * 256 -> 256 -> 32 scores (ReLU), BF16 weights on the matrix cores, FP32 accumulation; it scores
- * the 16 blocks a vertex's neighbours fall in most often. Selected per context with option 4
- * (0 plain proposal, 1 mixture 0.2 * plain + 0.8 * network) for vertices of degree <= 126.
+ * Learned proposal. No counterpart in the reference: a proposal network mixed with the
+ * reference's proposal and corrected in the Hastings ratio, so the chain's target and MDL stay
+ * the reference's. Network: 64 features -> 256 -> 256 -> 32 scores (ReLU), BF16 weights on the
+ * matrix cores, FP32 accumulation; it scores the 32 blocks a vertex's neighbours fall in most
+ * often. The features leave out the vertex's own contribution to every block degree, so they are
+ * the same before and after the vertex moves, and one set of scores gives the network's forward
+ * and reverse probabilities. Selected per context with option 4 (0 plain proposal, 1 mixture
+ * 0.2 * plain + 0.8 * network in the batched sweeps, 2 also in the sequential pass) for vertices
+ * within the degree bound of option 8.
  * Parameters, flat, in this order: W1[256][64], b1[256], W2[256][256], b2[256], W3[32][256],
  * b3[32] (SBPX_LEARNED_PARAMS floats; weights output-major). */
 #define SBPX_LEARNED_PARAMS 90656
@@ -175,6 +177,21 @@ sbpx_status sbpx_learned_log_count(sbpx_ctx* ctx, int32_t* count);
  * mean loss of the last minibatch. */
 sbpx_status sbpx_learned_train(sbpx_ctx* ctx, int32_t steps, int32_t batch, float lr, uint64_t seed,
                                float* mean_loss);
+/* Evaluate fixed proposals (v[i] -> s[i], uniform u[i], each vertex at most once) as the learned
+ * mixture proposal would have drawn them, on the current state, without applying them: delta MDL
+ * (entropy.cpp:296-401), the mixture's Hastings ratio H = [w q_plain(r) + (1 - w) q_net(r)] /
+ * [w q_plain(s) + (1 - w) q_net(s)] with q_plain the reference's closed form
+ * (entropy.cpp:495-557) divided by the vertex degree, and the accept decision
+ * (finetune.cpp:19-29). qf / qb (may be NULL) receive q_net(s) and q_net(r), -1 where the vertex
+ * has no learned part (degree beyond option 8's bound, or no candidate outside its block); there H
+ * is the plain ratio. The network must be initialised (sbpx_learned_init). */
+sbpx_status sbpx_learned_eval_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, const int32_t* s, const double* u,
+                                    double* dS, double* H, uint8_t* accept, float* qf, float* qb);
+/* The candidates (cand[n][32], -1 past the last) and scores (z[n][32]) of the listed vertices
+ * (degree <= 510) on the current state: scorer 0 = the batched sweeps' path (features kernel and
+ * matrix-core products), 1 = the sequential pass's scorer that one wavefront runs inside an
+ * evaluation. The two sum in other orders, so scores agree to BF16 rounding, not bit for bit. */
+sbpx_status sbpx_learned_scores(sbpx_ctx* ctx, int32_t n, const int32_t* v, int scorer, int32_t* cand, float* z);
 /* The network on n (<= 8192) rows of 64 features (rounded to BF16): z[n][32] scores. */
 sbpx_status sbpx_learned_forward(sbpx_ctx* ctx, int32_t n, const float* x, float* z);
 /* Mean softmax cross-entropy of n rows (candidate bit masks, labels in their masks): per-row

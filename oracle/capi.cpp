@@ -209,6 +209,42 @@ int oracle_propose_merge_draw(void* bmp, int r, int i, uint64_t seed, uint32_t c
     GUARD_END
 }
 
+// Learned proposal: candidates, counts and features of vertex v on the current state.
+int oracle_learned_features(void* gp, void* bmp, int v, int32_t* cand, int32_t* cnt, float* x) {
+    GUARD_BEGIN
+    int c[32], k[32];
+    learned_features(*(Graph*)gp, *(Blockmodel*)bmp, v, c, k, x);
+    std::memcpy(cand, c, sizeof(c));
+    std::memcpy(cnt, k, sizeof(k));
+    GUARD_END
+}
+
+// Hastings ratio of the learned mixture (weight w on the plain proposal) for v -> s with the given
+// scores z[32] of the candidates cand[32]; qf / qb receive the net's forward / reverse probability.
+int oracle_mixture_hastings(void* gp, void* bmp, int v, int s, const float* z, const int32_t* cand, double w,
+                            double* H, double* qf, double* qb) {
+    GUARD_BEGIN
+    int c[32];
+    float zz[32];
+    for (int j = 0; j < 32; ++j) {
+        c[j] = cand[j];
+        zz[j] = z[j];
+    }
+    *H = mixture_hastings(*(Graph*)gp, *(Blockmodel*)bmp, v, s, zz, c, w, qf, qb);
+    GUARD_END
+}
+
+// The two sums of the plain Hastings correction for v -> s.
+int oracle_hastings_terms(void* gp, void* bmp, int v, int s, double* fwd, double* bwd) {
+    GUARD_BEGIN
+    Graph* g = (Graph*)gp;
+    Blockmodel* bm = (Blockmodel*)bmp;
+    const int r = bm->assign[v];
+    Delta d = vertex_delta(*g, *bm, v, r, s);
+    hastings_terms(*g, *bm, v, d, r, vertex_counts(*g, v, s), fwd, bwd);
+    GUARD_END
+}
+
 // Evaluate fixed (v, s, u) proposals against the current state without applying them.
 int oracle_eval_moves(void* gp, void* bmp, int n, const int32_t* v, const int32_t* s, const double* u,
                       double* dS, double* H, uint8_t* acc) {

@@ -275,6 +275,15 @@ double delta_mdl(const Blockmodel& bm, const Delta& d, const Proposal& p) {
 // neighbour-block counts from out-neighbours (with a self-loop) and in-neighbours (without).
 double hastings(const Graph& g, const Blockmodel& bm, int v, const Delta& d, int r, const Proposal& p) {
     if (p.k == 0) return 1.0;
+    double p_fwd, p_bwd;
+    hastings_terms(g, bm, v, d, r, p, &p_fwd, &p_bwd);
+    return p_bwd / p_fwd;
+}
+
+// The two sums of hastings_correction: p_fwd over the current matrix, p_bwd over the matrix
+// after the move.
+void hastings_terms(const Graph& g, const Blockmodel& bm, int v, const Delta& d, int r, const Proposal& p,
+                    double* fwd, double* bwd) {
     Row block_counts;
     for (int w : g.out[v]) block_counts[bm.assign[w]] += 1;
     for (int w : g.in[v]) {
@@ -304,7 +313,93 @@ double hastings(const Graph& g, const Blockmodel& bm, int v, const Delta& d, int
         p_fwd += count * proposal_weight / block_degree;
         p_bwd += count * block_weight / proposal_degree;
     }
-    return p_bwd / p_fwd;
+    *fwd = p_fwd;
+    *bwd = p_bwd;
+}
+
+// ------------------------------------------------------------------------------------------
+// The learned mixture proposal (no counterpart in the reference). Candidates: the 32 blocks v's
+// neighbours fall in most often (self-loop excluded; ties: smaller block id). Features: for
+// candidate i < 16 its share of v's edges and log1p of its in/out degree without v over
+// log1p(E); then log1p(k_out), log1p(k_in) over log(4097), the share of v's edges into
+// candidates 16-31, distinct neighbour blocks / k, log(B) / log(N), 1, zeros. The scores z come
+// from the caller (the network runs elsewhere).
+void learned_features(const Graph& g, const Blockmodel& bm, int v, int cand[32], int cnt[32], float x[64]) {
+    std::map<int, int> counts;
+    for (int w : g.out[v])
+        if (w != v) counts[bm.assign[w]] += 1;
+    for (int w : g.in[v])
+        if (w != v) counts[bm.assign[w]] += 1;
+    std::vector<std::pair<int, int>> order(counts.begin(), counts.end());  // (block, count)
+    std::stable_sort(order.begin(), order.end(),
+                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.second > b.second; });
+    for (int i = 0; i < 32; ++i) {
+        cand[i] = i < (int)order.size() ? order[i].first : -1;
+        cnt[i] = i < (int)order.size() ? order[i].second : 0;
+    }
+    const int own = bm.assign[v];
+    const int k_out = (int)g.out[v].size(), self = g.self[v];
+    const int k_in = (int)g.in[v].size() - self;
+    const float k = (float)(k_out + k_in);
+    const float invk = k > 0 ? 1.0f / k : 0.0f;
+    const float inv_logE = 1.0f / log1pf((float)g.E);
+    for (int i = 0; i < 64; ++i) x[i] = 0.0f;
+    for (int i = 0; i < 16; ++i) {
+        const int b = cand[i];
+        if (b < 0) continue;
+        x[i] = (float)cnt[i] * invk;
+        x[16 + i] = log1pf((float)(bm.din[b] - (b == own ? k_in + self : 0))) * inv_logE;
+        x[32 + i] = log1pf((float)(bm.dout[b] - (b == own ? k_out : 0))) * inv_logE;
+    }
+    const float inv_logD = 1.0f / logf(4097.0f);
+    x[48] = log1pf((float)k_out) * inv_logD;
+    x[49] = log1pf((float)k_in) * inv_logD;
+    int rest = 0;
+    for (int i = 16; i < 32; ++i) rest += cand[i] >= 0 ? cnt[i] : 0;
+    x[50] = (float)rest * invk;
+    x[51] = (float)counts.size() * invk;
+    x[52] = g.N > 1 ? (float)(log((double)bm.B) / log((double)g.N)) : 0.0f;
+    x[53] = 1.0f;
+}
+
+// Softmax of z over the candidates other than `own`, at `pick` (-1: no candidate qualifies).
+static double net_prob(const float z[32], const int cand[32], int own, int pick) {
+    double mx = -1e300;
+    int any = 0;
+    for (int j = 0; j < 32; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            mx = std::max(mx, (double)z[j]);
+            any = 1;
+        }
+    if (!any) return -1.0;
+    double den = 0.0, num = 0.0;
+    for (int j = 0; j < 32; ++j)
+        if (cand[j] >= 0 && cand[j] != own) {
+            const double e = exp((double)z[j] - mx);
+            den += e;
+            if (cand[j] == pick) num = e;
+        }
+    return num / den;
+}
+
+// Hastings ratio of the mixture w q_plain + (1 - w) q_net for the move v: r -> s, with q_plain the
+// reference's closed form per unit of degree and q_net the softmax of the scores over the
+// candidates other than the vertex's block in each direction; without learned mass (no candidate
+// outside the block) the plain ratio.
+double mixture_hastings(const Graph& g, const Blockmodel& bm, int v, int s, const float z[32], const int cand[32],
+                        double w, double* qf, double* qb) {
+    const int r = bm.assign[v];
+    Proposal p = vertex_counts(g, v, s);
+    *qf = net_prob(z, cand, r, s);
+    *qb = net_prob(z, cand, s, r);
+    if (p.k == 0) return 1.0;
+    Delta d = vertex_delta(g, bm, v, r, s);
+    double pf, pb;
+    hastings_terms(g, bm, v, d, r, p, &pf, &pb);
+    const double plain_f = pf / p.k, plain_b = pb / p.k;
+    const double fwd = w * plain_f + (1.0 - w) * (*qf >= 0.0 ? *qf : plain_f);
+    const double bwd = w * plain_b + (1.0 - w) * (*qb >= 0.0 ? *qb : plain_b);
+    return bwd / fwd;
 }
 
 // finetune::accept (finetune.cpp:19-29), with the uniform injected.

@@ -21,6 +21,8 @@
 //   merge_blocks              src/block_merge.cpp:278-368
 //   merge selection           src/blockmodel/blockmodel.cpp:29-78, src/block_merge.cpp:123-217
 //   triplet / driver          src/blockmodel/blockmodel_triplet.cpp:3-100, src/sbp.cpp:61-112
+//   learned mixture Hastings  no reference counterpart: the plain terms of src/entropy.cpp:495-557
+//                             mixed with a softmax over given scores
 //   degree split              src/graph.cpp:15-22, 215-228
 //
 // Randomness: the reference draws from unseeded std engines (include/common.hpp:25-26), so its
@@ -92,6 +94,12 @@ struct Proposal { int s, k_out, k_in, k; };
 Delta vertex_delta(const Graph& g, const Blockmodel& bm, int v, int r, int s);
 double delta_mdl(const Blockmodel& bm, const Delta& d, const Proposal& p);
 double hastings(const Graph& g, const Blockmodel& bm, int v, const Delta& d, int r, const Proposal& p);
+void hastings_terms(const Graph& g, const Blockmodel& bm, int v, const Delta& d, int r, const Proposal& p,
+                    double* fwd, double* bwd);
+// learned mixture proposal (this project's own; see cpu_ref.cpp)
+void learned_features(const Graph& g, const Blockmodel& bm, int v, int cand[32], int cnt[32], float x[64]);
+double mixture_hastings(const Graph& g, const Blockmodel& bm, int v, int s, const float z[32], const int cand[32],
+                        double w, double* qf, double* qb);
 bool accept(double dS, double H, double u);
 void move_vertex(Blockmodel& bm, int v, const Delta& d, const Proposal& p);
 Proposal vertex_counts(const Graph& g, int v, int s);

@@ -97,6 +97,11 @@ def lib():
     L.oracle_sbp.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, i32p, C.POINTER(OracleRun)]
     L.oracle_propose_merge_draw.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_uint64, C.c_uint32,
                                             C.POINTER(C.c_int32)]
+    L.oracle_learned_features.argtypes = [C.c_void_p, C.c_void_p, C.c_int, i32p, i32p, f32p]
+    L.oracle_mixture_hastings.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, f32p, i32p, C.c_double,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.oracle_hastings_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double)]
     L.oracle_sbp_traced.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, i32p, C.POINTER(OracleRun),
                                     i32p, f32p, C.c_int]
     _lib = L
@@ -311,6 +316,27 @@ class Blockmodel:
         out = C.c_int32()
         _check(lib().oracle_propose_merge_draw(self.h, r, i, seed, call, C.byref(out)))
         return out.value
+
+    def learned_features(self, v):
+        """Learned proposal inputs of vertex v: candidates[32], counts[32], features[64]."""
+        cand = np.zeros(32, np.int32)
+        cnt = np.zeros(32, np.int32)
+        x = np.zeros(64, np.float32)
+        _check(lib().oracle_learned_features(self.g.h, self.h, int(v), cand, cnt, x))
+        return cand, cnt, x
+
+    def mixture_hastings(self, v, s, z, cand, w=0.2):
+        """(H, q_net(s), q_net(r)) of the learned mixture for v -> s with scores z of candidates cand."""
+        H, qf, qb = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().oracle_mixture_hastings(self.g.h, self.h, int(v), int(s), np.ascontiguousarray(z, np.float32),
+                                             np.ascontiguousarray(cand, np.int32), float(w), C.byref(H),
+                                             C.byref(qf), C.byref(qb)))
+        return H.value, qf.value, qb.value
+
+    def hastings_terms(self, v, s):
+        f, b = C.c_double(), C.c_double()
+        _check(lib().oracle_hastings_terms(self.g.h, self.h, int(v), int(s), C.byref(f), C.byref(b)))
+        return f.value, b.value
 
     def carry_out(self, dS, best, advanced=False):
         _check(lib().oracle_carry_out(self.g.h, self.h, np.ascontiguousarray(dS, np.float64),

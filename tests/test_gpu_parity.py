@@ -113,13 +113,13 @@ def test_build_rejects_bad_block_ids(toy):
 def test_options_reject_values_outside_their_range(toy):
     c = ctx_for(toy_arrays(toy))
     m = sbpx()
-    for option, bad in ((m.OPT_MH_GROUP, 3), (m.OPT_EVAL_VARIANT, 4), (m.OPT_PROPOSAL, 4), (m.OPT_INCREMENTAL, 2),
+    for option, bad in ((m.OPT_MH_GROUP, 3), (m.OPT_EVAL_VARIANT, 4), (m.OPT_PROPOSAL, 3), (m.OPT_INCREMENTAL, 2),
                         (m.OPT_MH_WIDE, 3), (m.OPT_MH_WIDE, 64), (m.OPT_MH_HEADROOM, -1),
                         (m.OPT_LEARN_MAXDEG, -1), (m.OPT_LEARN_MAXDEG, 511), (99, 0)):
         with pytest.raises(m.SbpxError):
             c.set_option(option, bad)
     for option, good in ((m.OPT_MH_WIDE, 32), (m.OPT_MH_WIDE, 0), (m.OPT_MH_HEADROOM, 0), (m.OPT_PROPOSAL, 2),
-                         (m.OPT_PROPOSAL, 3), (m.OPT_PROPOSAL, 0), (m.OPT_LEARN_MAXDEG, 510),
+                         (m.OPT_PROPOSAL, 1), (m.OPT_PROPOSAL, 0), (m.OPT_LEARN_MAXDEG, 510),
                          (m.OPT_LEARN_MAXDEG, 126)):
         c.set_option(option, good)
 

@@ -616,3 +616,84 @@ def test_golden_section_hand_worked_traces():
     # best at the first iteration (upper slot empty): steps from the lower side, 20 - 10 = 10 ->
     # 10 + int(6.18) = 16, then 18, 19; done when middle and lower are 1 apart
     assert _golden_trace(40, [abs(b - 20.0) for b in range(41)]) == ([20, 10, 16, 18, 19], 20)
+
+
+# ----------------------------------------------------------------------------- learned mixture
+def _scores(x, seed=3):
+    """A fixed deterministic function of the features standing in for the network."""
+    A = np.random.default_rng(seed).normal(size=(32, 64)).astype(np.float32)
+    return np.tanh(A @ np.asarray(x, np.float32)).astype(np.float32) * 3.0
+
+
+def _mixture_graph():
+    rng = np.random.default_rng(17)
+    n, B = 60, 5
+    edges = set()
+    for u in range(n):
+        for w in rng.integers(0, n, 5):
+            if int(w) != u:
+                edges.add((u, int(w)))
+    a = rng.integers(0, B, n).astype(np.int32)
+    return n, B, sorted(edges), a
+
+
+def test_learned_features_do_not_change_when_the_vertex_moves():
+    """The learned proposal's candidates and features of vertex v are the same in the state after v
+    moves r -> s: its neighbours' blocks do not change, and block degrees are taken without v."""
+    n, B, edges, a = _mixture_graph()
+    g = O.Graph.from_edges(n, edges)
+    bm = O.Blockmodel(g, B, a)
+    checked = 0
+    for v in range(n):
+        cand, cnt, x = bm.learned_features(v)
+        for s in range(B):
+            if s == a[v]:
+                continue
+            a2 = a.copy()
+            a2[v] = s
+            bm2 = O.Blockmodel(g, B, a2)
+            c2, k2, x2 = bm2.learned_features(v)
+            assert (c2 == cand).all() and (k2 == cnt).all() and (x2 == x).all(), (v, s)
+            checked += 1
+    assert checked > 100
+
+
+def test_mixture_hastings_reduces_and_reverses(toy):
+    """The oracle's mixture Hastings ratio: with all weight on the plain proposal it is the
+    reference's Hastings correction; with uniform scores and all weight on the network it is
+    (candidates other than r) / (candidates other than s); and for every self-loop-free move whose
+    target is a candidate, H(x -> x') H(x' -> x) = 1 with the reverse evaluated from scratch in the
+    state after the move (detailed balance of the mixture proposal)."""
+    n, B, edges, a = _mixture_graph()
+    g = O.Graph.from_edges(n, edges)
+    bm = O.Blockmodel(g, B, a)
+    self_loop = {u for (u, w) in edges if u == w}
+    rng = np.random.default_rng(2)
+    products = 0
+    for v in range(n):
+        r = int(a[v])
+        cand, cnt, x = bm.learned_features(v)
+        z = _scores(x)
+        for s in range(B):
+            if s == r:
+                continue
+            _, H_plain, _ = bm.eval_moves([v], [s], [0.5])
+            H1, _, _ = bm.mixture_hastings(v, s, z, cand, w=1.0)
+            assert H1 == pytest.approx(float(H_plain[0]), rel=1e-12)
+            valid = cand[cand >= 0]
+            if s in valid and r in valid:
+                Hu, qf, qb = bm.mixture_hastings(v, s, np.zeros(32, np.float32), cand, w=0.0)
+                assert qf == pytest.approx(1.0 / (len(valid) - 1)) and qb == pytest.approx(1.0 / (len(valid) - 1))
+                assert Hu == pytest.approx(1.0, rel=1e-12)
+            if s not in valid or v in self_loop:
+                continue
+            w = float(rng.choice([0.2, 0.5, 0.9]))
+            H, _, _ = bm.mixture_hastings(v, s, z, cand, w=w)
+            a2 = a.copy()
+            a2[v] = s
+            bm2 = O.Blockmodel(g, B, a2)
+            c2, _, x2 = bm2.learned_features(v)
+            Hr, _, _ = bm2.mixture_hastings(v, r, _scores(x2), c2, w=w)
+            assert H * Hr == pytest.approx(1.0, rel=1e-12), (v, r, s)
+            products += 1
+    assert products > 50

@@ -62,7 +62,7 @@ class RunStats(C.Structure):
     _fields_ = [("num_blocks", C.c_int32), ("outer_iterations", C.c_int32), ("sweeps", C.c_int64),
                 ("attempts", C.c_int64), ("evals", C.c_int64), ("moves", C.c_int64), ("seconds", C.c_double),
                 ("mcmc_seconds", C.c_double), ("eval_ms", C.c_double * 4), ("eval_bytes", C.c_double * 4),
-                ("eval_launches", C.c_int64 * 4), ("mdl", C.c_double)]
+                ("eval_launches", C.c_int64 * 4), ("mdl", C.c_double), ("stopped_at_limit", C.c_int32)]
 
 
 _lib = None
@@ -139,6 +139,7 @@ def load_host():
     H.sbp_run.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_int32, C.c_int, C.c_uint64, C.c_int,
                           C.c_int, i32p, C.POINTER(RunStats), C.c_char_p, C.c_int]
     H.sbp_graph_new.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, C.c_int, C.c_int64, C.c_char_p, C.c_int]
+    H.sbp_set_time_limit.argtypes = [C.c_double]
     H.sbp_file_load.argtypes = [C.c_char_p, C.c_int, C.c_char_p, C.c_int]
     H.sbp_file_load.restype = C.c_void_p
     H.sbp_file_info.argtypes = [C.c_void_p] + [C.c_void_p] * 5
@@ -178,7 +179,8 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_learned_forward", "sbpx_learned_gradients", "sbpx_batch_define", "sbpx_batch_info",
             "sbpx_batch_build", "sbpx_batch_download", "sbpx_batch_log_posterior", "sbpx_batch_sweep",
             "sbpx_batch_merge", "sbpx_blockmodel_trace", "sbpx_batch_trace", "sbpx_learned_eval_moves",
-            "sbpx_learned_scores"]
+            "sbpx_learned_scores", "sbpx_delta_mdl", "sbpx_merge_eval",
+            "sbpx_blockmodel_cells", "sbpx_blockmodel_line"]
 
 
 class SbpxError(RuntimeError):
@@ -550,9 +552,12 @@ class ResidentGraph:
             raise RuntimeError("sbp_graph_learned: " + err.value.decode())
 
     def partition(self, algorithm="async_gibbs", batches=1, approximate=True, seed=0, count_bytes=False,
-                  learned=False, checkpoint=None, resume=False, max_iterations=0):
+                  learned=False, checkpoint=None, resume=False, max_iterations=0, max_seconds=0.0):
         """sbp::stochastic_block_partition; with `checkpoint`, the driver state is written after every
-        outer iteration, `resume` continues from it and `max_iterations` > 0 stops early."""
+        outer iteration, `resume` continues from it and `max_iterations` > 0 stops early;
+        `max_seconds` > 0 stops the run after the sweep in progress once it has run that long
+        (stats.stopped_at_limit is then 1)."""
+        self.H.sbp_set_time_limit(float(max_seconds))
         # True / 1: batched sweeps; 2: also the sequential pass (one wavefront per vertex); 3: also
         # its wavefront groups
         self.set_option(OPT_PROPOSAL, int(learned))

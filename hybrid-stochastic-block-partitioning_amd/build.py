@@ -14,7 +14,7 @@ HOST_LIB = os.path.join(HERE, "libsbp_host.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SBPX_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "merge.hip", "mlp.hip", "learned.hip", "api.cpp",
+DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "merge.hip", "mlp.hip", "learned.hip", "cells.hip", "api.cpp",
                   "generator.cpp"]
 HOST_SOURCES = ["sbp_host.cpp", "graph_io.cpp", "batch.cpp", "capi.cpp"]
 
@@ -47,9 +47,24 @@ def build_host():
     os.replace(tmp, HOST_LIB)
 
 
+CALLER_SRC = os.path.join(ROOT, "tests", "cpp", "toy_caller.cpp")
+CALLER_BIN = os.path.join(ROOT, "tests", "cpp", "toy_caller")
+
+
+def build_caller():
+    """A reference-style C++ caller of the host layer, for the tests."""
+    if not os.path.exists(CALLER_SRC):
+        return
+    tmp = CALLER_BIN + ".tmp"
+    run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST}", CALLER_SRC, "-o", tmp,
+         f"-L{HERE}", "-lsbp_host", "-lsbpx", "-Wl,-rpath,$ORIGIN/../../hybrid-stochastic-block-partitioning_amd"])
+    os.replace(tmp, CALLER_BIN)
+
+
 def build():
     build_device()
     build_host()
+    build_caller()
 
 
 if __name__ == "__main__":

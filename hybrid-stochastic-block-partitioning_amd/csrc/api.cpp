@@ -1,5 +1,6 @@
 // C ABI (include/sbpx.h) over the device kernels.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cfloat>
 #include <exception>
@@ -120,6 +121,7 @@ void free_graph_buffers(Ctx& c) {
         dfree(p);
     dfree(&h.wstamp);
     if (h.wide) (void)hipFree(h.wide);
+    if (h.wst) (void)hipFree(h.wst);
     h = MhState();
     c.plan = SweepPlan();
     c.bm = DevBlockmodel();
@@ -455,6 +457,19 @@ void build_plan(Ctx& c, int algo, int batches) {
 
 }  // namespace
 
+// Device copies of small host arrays for the one-move entry points below, freed on scope exit.
+namespace {
+template <typename T>
+struct DevArray {
+    T* p = nullptr;
+    explicit DevArray(size_t n) { dalloc(&p, std::max<size_t>(n, 1)); }
+    DevArray(const T* src, size_t n, hipStream_t st) : DevArray(n) {
+        if (n) SBPX_HIP(hipMemcpyAsync(p, src, sizeof(T) * n, hipMemcpyHostToDevice, st));
+    }
+    ~DevArray() { dfree(&p); }
+};
+}  // namespace
+
 extern "C" {
 
 sbpx_status sbpx_create(sbpx_ctx** out, int device) {
@@ -521,8 +536,8 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
                 throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");
             ctx->c.mh_group = (int)value;
         } else if (option == 6) {
-            if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
-                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield), 1, 2, 4, 8, 16 or 32");
+            if (value < 0 || value > 256 || (value & (value - 1)) != 0)
+                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield) or a power of two up to 256");
             ctx->c.mh_wide = (int)value;
         } else if (option == 7) {
             if (value < 0 || value > (int64_t)1 << 30) throw ArgError("table headroom must be in [0, 2^30] slots");
@@ -775,6 +790,96 @@ sbpx_status sbpx_log_posterior(sbpx_ctx* ctx, double* out) {
     return guarded(ctx, [&] {
         require_bm(ctx->c);
         *out = log_posterior(ctx->c);
+    });
+}
+
+sbpx_status sbpx_delta_mdl(sbpx_ctx* ctx, int32_t r, int32_t s, int32_t n, const int32_t* rows, const int32_t* cols,
+                           const int32_t* deltas, int32_t k_out, int32_t k_in, int32_t self_weight, double* dS) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        const int B = c.bm.B;
+        if (r < 0 || r >= B || s < 0 || s >= B || r == s) throw ArgError("blocks must be distinct and in [0, B)");
+        if (n < 0 || (n > 0 && (!rows || !cols || !deltas)) || !dS) throw ArgError("bad Delta arrays");
+        std::vector<std::array<int, 3>> e(n);
+        for (int i = 0; i < n; ++i) {
+            if (rows[i] < 0 || rows[i] >= B || cols[i] < 0 || cols[i] >= B) throw ArgError("Delta cell out of range");
+            if (rows[i] != r && rows[i] != s && cols[i] != r && cols[i] != s)
+                throw ArgError("Delta cell in neither row/column r nor s");
+            e[i] = {rows[i], cols[i], deltas[i]};
+        }
+        std::sort(e.begin(), e.end());
+        for (int i = 1; i < n; ++i)
+            if (e[i][0] == e[i - 1][0] && e[i][1] == e[i - 1][1]) throw ArgError("Delta cell listed twice");
+        std::vector<int> er(n), ec(n), ed(n);
+        for (int i = 0; i < n; ++i) {
+            er[i] = e[i][0];
+            ec[i] = e[i][1];
+            ed[i] = e[i][2];
+        }
+        DevArray<int> der(er.data(), n, c.stream), dec(ec.data(), n, c.stream), ded(ed.data(), n, c.stream);
+        DevArray<double> dout(1);
+        launch_delta_mdl(c, r, s, n, der.p, dec.p, ded.p, k_out, k_in, self_weight, dout.p);
+        SBPX_HIP(hipMemcpyAsync(dS, dout.p, sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        if (std::isnan(*dS) || std::isinf(*dS)) throw NumericError("NaN or Inf in a delta MDL");
+    });
+}
+
+sbpx_status sbpx_merge_eval(sbpx_ctx* ctx, int32_t n, const int32_t* r, const int32_t* s, double* dS) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        if (n < 0 || (n > 0 && (!r || !s || !dS))) throw ArgError("bad merge arrays");
+        for (int i = 0; i < n; ++i)
+            if (r[i] < 0 || r[i] >= c.bm.B || s[i] < 0 || s[i] >= c.bm.B || r[i] == s[i])
+                throw ArgError("merge blocks must be distinct and in [0, B)");
+        if (n == 0) return;
+        DevArray<int> dr(r, n, c.stream), ds(s, n, c.stream);
+        DevArray<double> out(n);
+        launch_merge_eval(c, n, dr.p, ds.p, out.p);
+        SBPX_HIP(hipMemcpyAsync(dS, out.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+sbpx_status sbpx_blockmodel_cells(sbpx_ctx* ctx, int32_t n, const int32_t* rows, const int32_t* cols, int32_t* vals) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        if (n < 0 || (n > 0 && (!rows || !cols || !vals))) throw ArgError("bad cell arrays");
+        for (int i = 0; i < n; ++i)
+            if (rows[i] < 0 || rows[i] >= c.bm.B || cols[i] < 0 || cols[i] >= c.bm.B)
+                throw ArgError("block id out of range [0, B)");
+        if (n == 0) return;
+        DevArray<int> dr(rows, n, c.stream), dc(cols, n, c.stream), out(n);
+        launch_cells(c, n, dr.p, dc.p, out.p);
+        SBPX_HIP(hipMemcpyAsync(vals, out.p, sizeof(int) * n, hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+    });
+}
+
+sbpx_status sbpx_blockmodel_line(sbpx_ctx* ctx, int32_t x, int by_column, int32_t cap, int32_t* idx, int32_t* vals,
+                                 int32_t* count) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_bm(c);
+        if (x < 0 || x >= c.bm.B) throw ArgError("block id out of range [0, B)");
+        if (!count || cap < 0 || (cap > 0 && (!idx || !vals))) throw ArgError("bad output arrays");
+        const int* ptr = by_column ? c.bm.col_ptr : c.bm.row_ptr;
+        int lohi[2];
+        SBPX_HIP(hipMemcpyAsync(lohi, ptr + x, sizeof(lohi), hipMemcpyDeviceToHost, c.stream));
+        c.sync();
+        const int len = lohi[1] - lohi[0];
+        *count = len;
+        const int k = std::min(len, (int)cap);
+        if (k > 0) {
+            SBPX_HIP(hipMemcpyAsync(idx, (by_column ? c.bm.col_row : c.bm.row_col) + lohi[0], sizeof(int) * k,
+                                    hipMemcpyDeviceToHost, c.stream));
+            SBPX_HIP(hipMemcpyAsync(vals, (by_column ? c.bm.col_val : c.bm.row_val) + lohi[0], sizeof(int) * k,
+                                    hipMemcpyDeviceToHost, c.stream));
+            c.sync();
+        }
     });
 }
 

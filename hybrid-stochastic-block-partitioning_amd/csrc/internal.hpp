@@ -133,8 +133,10 @@ struct MhState {
     int last_B = -1;          // number of blocks when mean_scan was measured
     int g_yield = 1;  // wavefronts per vertex the last chunk's window yield calls for
     int k_yield = 8;  // workgroups of the multi-workgroup pass the last chunk's yield calls for
+    int k_max = 1;    // the most workgroups that fit on the device at once (a power of two)
     unsigned long long* wstamp = nullptr;  // [N] accepted-entry stamps of the multi-workgroup pass
     void* wide = nullptr;                  // its per-entry records
+    int* wst = nullptr;                    // their evaluation status, dense
     int* ctrl = nullptr;                   // its barrier counter and window control words
     int* assign0 = nullptr;  // [N] the assignment the block-matrix snapshot was built from
     int* d_all = nullptr;   // 0..N-1
@@ -402,6 +404,11 @@ void mlp_adam_step(Ctx& c, float lr);
 void mlp_gather(Ctx& c, const int* idx, int n);
 void mlp_sync_bf16(Ctx& c);
 void mlp_transpose_bf16(Ctx& c);  // input-major BF16 weights for the sequential pass
+// cells.hip: one Delta's delta MDL, fixed merges' delta MDL, cell lookups
+void launch_delta_mdl(Ctx& c, int r, int s, int n, const int* er, const int* ec, const int* ed, int k_out, int k_in,
+                      int self_w, double* d_out);
+void launch_merge_eval(Ctx& c, int n, const int* r, const int* s, double* d_out);
+void launch_cells(Ctx& c, int n, const int* rows, const int* cols, int* d_out);
 // The sequential pass's in-wavefront scorer on the listed vertices (degree <= max_deg): candidates
 // and scores [n][32].
 void learned_scores_probe(Ctx& c, const int* d_list, int n, int max_deg, int* d_cand, float* d_z);

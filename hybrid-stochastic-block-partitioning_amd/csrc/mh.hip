@@ -17,6 +17,8 @@
 // barrier meanwhile). Only when the pool itself runs out does the kernel stop; the host then
 // rebuilds the tables with a larger pool and the pass resumes at the same vertex.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <cstdint>
 
@@ -133,11 +135,15 @@ struct MhParams {
     int* err;
     // the pass over several workgroups (mh_wide_kernel)
     WideSlot* wide;                 // [16 K] one record per window entry
+    int* wst;                       // [16 K] the entries' evaluation status, dense
     unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
     int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
                                     // [2] next queue position, [3] stop reason
     // the learned mixture (LEARNED kernels only): the network and the degree bound
     LearnedNet net;
+    // optional phase clock of mh_wide_kernel (workgroup 0, 100 MHz ticks): evaluation and first
+    // barrier, checks and second barrier, commits, third barrier; then the number of commits
+    double* phase;
 };
 
 __device__ __forceinline__ int room(const int* capa, const int* occ, int x) { return (capa[x] * 3) / 4 - occ[x]; }
@@ -969,8 +975,11 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 // fences (the L2 of each XCD is written back and invalidated) and a bounded wait; a workgroup
 // whose wait runs out reports DEV_ERR_BARRIER and leaves, and so does every other one at its
 // next barrier. With one workgroup they are workgroup barriers.
-constexpr int MH_WIDE_MAX = 32;  // workgroups
+constexpr int MH_WIDE_MAX = 256;  // workgroups (at most one per compute unit: co-resident)
 constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WAVES;
+// Accepted entries a window tracks; a window with more ends before the next one (its commits
+// and checks see only these).
+constexpr int MH_ACC_MAX = 512;
 constexpr int BARRIER_SPINS = 1 << 22;
 
 struct WideSlot {
@@ -1024,8 +1033,9 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     __shared__ CommitLds s_clds;
     __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
     __shared__ int lds_hist[MH_WAVES][256];
-    __shared__ int s_acc[MH_WIDE_SLOTS], s_acc_r[MH_WIDE_SLOTS], s_acc_s[MH_WIDE_SLOTS], s_acc_cap[MH_WIDE_SLOTS];
-    __shared__ double s_dS[MH_WIDE_SLOTS];
+    __shared__ int s_acc[MH_ACC_MAX], s_acc_r[MH_ACC_MAX], s_acc_s[MH_ACC_MAX], s_acc_cap[MH_ACC_MAX];
+    __shared__ double s_dS[MH_ACC_MAX];
+    __shared__ int s_wcnt[MH_WAVES], s_cut;
     __shared__ long long s_red[MH_WAVES][4];
     __shared__ double s_redd[MH_WAVES][2];
     // the learned mixture's candidates, their counts and scores, per wavefront
@@ -1048,6 +1058,16 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     int nbar = 0;
     if (threadIdx.x == 0) s_ok = 1;
     __syncthreads();
+    const bool clocked = P.phase != nullptr && blockIdx.x == 0 && threadIdx.x == 0;
+    double ph[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    unsigned long long tclk = clocked ? wall_clock64() : 0ull;
+    auto tick = [&](int k) {
+        if (clocked) {
+            const unsigned long long t = wall_clock64();
+            ph[k] += (double)(t - tclk);
+            tclk = t;
+        }
+    };
     while (pos < P.stop_at && stopped == 0) {
         ++wserial;
         windows += 1.0;
@@ -1084,6 +1104,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
         }
         if (lane == 0) {
             WideSlot& o = P.wide[slot];
+            P.wst[slot] = active ? R.status : 3;
             o.st = active ? R.status : 3;
             o.r = R.r;
             o.s = R.s;
@@ -1097,26 +1118,46 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             if (active && R.status == 2) P.wstamp[R.v] = ((unsigned long long)wserial << 32) | (unsigned)slot;
         }
         if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
-        // 2. the accepted entries of the window, in queue order
-        if (wave == 0) {
+        tick(0);
+        // 2. the accepted entries of the window, in queue order: each wavefront counts a slice of
+        // the dense status array, then fills its part of the list (the first MH_ACC_MAX; the
+        // window ends before the next accepted entry)
+        {
+            const int per = ((Wn + MH_WAVES * WAVE - 1) / (MH_WAVES * WAVE)) * WAVE;
+            const int lo = wave * per, hi = min(Wn, lo + per);
             int cnt = 0;
-            for (int b0 = 0; b0 < Wn; b0 += WAVE) {
+            for (int j = lo + lane; j < hi; j += WAVE) cnt += P.wst[j] == 2 ? 1 : 0;
+            cnt = (int)wave_sum_ll(cnt);
+            if (lane == 0) s_wcnt[wave] = cnt;
+            if (threadIdx.x == 0) s_cut = INT32_MAX;
+            __syncthreads();
+            int base = 0, total = 0;
+            for (int w = 0; w < MH_WAVES; ++w) {
+                base += w < wave ? s_wcnt[w] : 0;
+                total += s_wcnt[w];
+            }
+            for (int b0 = lo; b0 < hi && base <= MH_ACC_MAX; b0 += WAVE) {
                 const int j = b0 + lane;
-                const bool acc = j < Wn && P.wide[j].st == 2;
+                const bool acc = j < hi && P.wst[j] == 2;
                 const unsigned long long m = __ballot(acc);
                 if (acc) {
-                    const int k = cnt + __popcll(m & ((1ull << lane) - 1));
-                    s_acc[k] = j;
-                    s_acc_r[k] = P.wide[j].r;
-                    s_acc_s[k] = P.wide[j].s;
-                    s_acc_cap[k] = P.wide[j].cap;
-                    s_dS[k] = P.wide[j].dS;
+                    const int k = base + __popcll(m & ((1ull << lane) - 1));
+                    if (k < MH_ACC_MAX) {
+                        s_acc[k] = j;
+                        s_acc_r[k] = P.wide[j].r;
+                        s_acc_s[k] = P.wide[j].s;
+                        s_acc_cap[k] = P.wide[j].cap;
+                        s_dS[k] = P.wide[j].dS;
+                    } else if (k == MH_ACC_MAX) {
+                        s_cut = j;
+                    }
                 }
-                cnt += __popcll(m);
+                base += __popcll(m);
             }
-            if (lane == 0) s_n = cnt;
+            if (threadIdx.x == 0) s_n = min(total, MH_ACC_MAX);
         }
         __syncthreads();
+        if (threadIdx.x == 0 && s_cut < Wn) atomicMin(&P.ctrl[1], s_cut);
         // 3. whether this entry's evaluation holds after the accepted entries before it
         if (active) {
             int nb = 0, hi = s_n;  // accepted entries before this one
@@ -1154,6 +1195,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             }
         }
         if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        tick(1);
         // 4. workgroup 0 commits the accepted entries before the first that does not hold
         if (lead) {
             if (threadIdx.x == 0) {
@@ -1240,7 +1282,9 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
                 P.ctrl[3] = s_stop;
             }
         }
+        tick(2);
         if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        tick(3);
         pos = __hip_atomic_load(&P.ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         stopped = __hip_atomic_load(&P.ctrl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -1257,6 +1301,10 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
         P.status[1] = stopped;
         P.status[4] = serial;
         P.status[9] = (int)wserial;
+        if (clocked) {
+            for (int k = 0; k < 4; ++k) P.phase[k] += ph[k];
+            P.phase[4] += (double)mvs;
+        }
         atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
         atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
         atomicAdd((unsigned long long*)&P.counters[2], (unsigned long long)mvs);
@@ -1691,14 +1739,24 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     int tcap = 16;
     while (tcap < 2 * (max_deg + 3)) tcap <<= 1;
     const bool wide = c.mh_wide != 1;  // the multi-workgroup pass may run
-    const size_t tslots = (size_t)(wide ? MH_WIDE_SLOTS : MH_WAVES) * 7 * tcap;
     if (wide && !h.wide) {
         SBPX_HIP(hipMalloc(&h.wide, sizeof(WideSlot) * MH_WIDE_SLOTS));
+        SBPX_HIP(hipMalloc((void**)&h.wst, sizeof(int) * MH_WIDE_SLOTS));
+        // the most workgroups of the window kernels that fit on the device at once (a
+        // cooperative launch needs them all resident)
+        int per_cu = 1, per_cu_l = 1, cus = 1;
+        SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mh_wide_kernel<false>, MH_WAVES * WAVE, 0));
+        SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_l, mh_wide_kernel<true>, MH_WAVES * WAVE, 0));
+        SBPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
+        h.k_max = 1;
+        while (2 * h.k_max <= std::min(MH_WIDE_MAX, std::min(per_cu, per_cu_l) * cus)) h.k_max *= 2;
         SBPX_HIP(hipMalloc((void**)&h.ctrl, sizeof(int) * 4));
         SBPX_HIP(hipMalloc((void**)&h.wstamp, sizeof(unsigned long long) * std::max(c.g.N, 1)));
         SBPX_HIP(hipMemsetAsync(h.wstamp, 0, sizeof(unsigned long long) * std::max(c.g.N, 1), c.stream));
         SBPX_HIP(hipMemsetAsync(h.status + 9, 0, sizeof(int), c.stream));
     }
+    // one table per wavefront of the widest window (or of one workgroup)
+    const size_t tslots = (size_t)(wide ? h.k_max * MH_WAVES : MH_WAVES) * 7 * tcap;
     if (tslots > h.table_slots) {
         if (h.tables) SBPX_HIP(hipFree(h.tables));
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
@@ -1717,6 +1775,17 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     int pos = 0;
     int rounds = 0;
     double* scan = c.red + 2052;  // [3] scanned slots, evaluations and windows of the current chunk
+    // SBPX_TRACE_MH=1: one line per pass on stderr (windows, entries per window, workgroups and
+    // wavefronts per vertex chosen, kernel time)
+    static const bool trace = std::getenv("SBPX_TRACE_MH") != nullptr;
+    double tr_windows = 0.0, tr_scan = 0.0, tr_evals = 0.0, tr_k = 0.0, tr_ms = 0.0;
+    int tr_chunks = 0, tr_g1 = 0;
+    hipEvent_t tr_e0 = nullptr, tr_e1 = nullptr;
+    if (trace) {
+        SBPX_HIP(hipEventCreate(&tr_e0));
+        SBPX_HIP(hipEventCreate(&tr_e1));
+        SBPX_HIP(hipMemsetAsync(c.red + 3000, 0, 5 * sizeof(double), c.stream));
+    }
     while (pos < n) {
         // each restart doubles the pool headroom, so the number of restarts is logarithmic
         if (rounds++ > 24) throw StateError("Metropolis-Hastings pass keeps running out of table room");
@@ -1755,9 +1824,11 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.dS_sum = c.red + 2048;
         P.err = c.dev_err;
         P.wide = (WideSlot*)h.wide;
+        P.wst = h.wst;
         P.wstamp = h.wstamp;
         P.ctrl = h.ctrl;
         P.net = net;
+        P.phase = trace ? c.red + 3000 : nullptr;
         bool out_of_room = false;
         // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
         // previous chunk: the mean number of row/column slots its evaluations scanned (long scans
@@ -1778,17 +1849,19 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             }
             // workgroups: with one wavefront per vertex, as many as the window yield calls for
             int K = 1;
-            if (G == 1 && wide) K = c.mh_wide ? c.mh_wide : h.k_yield;
+            if (G == 1 && wide) K = std::min(c.mh_wide ? c.mh_wide : h.k_yield, h.k_max);
             // one workgroup, one wavefront per vertex: the window checks of the wide kernel
             const bool wide1 = K == 1 && G == 1 && wide && c.mh_wide == 0;
             // the learned mixture runs in every chunk: the window-check kernel for one wavefront
             // per vertex, the group kernels for wavefront groups
             const bool lchunk = learned;
             P.start = pos;
-            P.stop_at = std::min(n, pos + MH_CHUNK);
+            // a launch covers at least MH_CHUNK entries and four full windows
+            P.stop_at = std::min(n, pos + std::max(MH_CHUNK, 4 * MH_WAVES * K));
             SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));
             if (K > 1 || wide1) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
             if (timed) c.timer_begin();
+            if (trace) SBPX_HIP(hipEventRecord(tr_e0, c.stream));
             if (K > 1) {
                 void* args[] = {&P};
                 const void* kern = lchunk ? (const void*)mh_wide_kernel<true> : (const void*)mh_wide_kernel<false>;
@@ -1810,12 +1883,24 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             }
             SBPX_HIP(hipGetLastError());
             if (timed) c.timer_end(KERNEL_MH);
+            if (trace) SBPX_HIP(hipEventRecord(tr_e1, c.stream));
             int stat[10];
             double sc[3];
             SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipMemcpyAsync(sc, scan, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipStreamSynchronize(c.stream));
             check_device_error(c);
+            if (trace) {
+                float ms = 0.0f;
+                SBPX_HIP(hipEventElapsedTime(&ms, tr_e0, tr_e1));
+                tr_ms += ms;
+                tr_windows += sc[2];
+                tr_scan += sc[0];
+                tr_evals += sc[1];
+                tr_k += K * sc[2];
+                tr_chunks++;
+                tr_g1 += G == 1;
+            }
             const int advanced = stat[0] - pos;
             pos = stat[0];
             if (sc[1] > 0.0) {
@@ -1831,7 +1916,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                 const double y = (double)advanced / sc[2];
                 if (K > 1) {
                     const double full = (double)(MH_WAVES * K);
-                    if (y >= 0.75 * full) h.k_yield = std::min(2 * K, MH_WIDE_MAX);
+                    if (y >= 0.75 * full) h.k_yield = std::min(2 * K, h.k_max);
                     else if (y < 0.375 * full) h.k_yield = K / 2;
                     h.g_yield = 1;
                 } else {
@@ -1857,6 +1942,21 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // the matrix changed: bring the CSR snapshot up to date (dout/din exact, deg keeps drift)
         update_after_pass(c, d_list, n);
         if (out_of_room) h.headroom *= 2;
+    }
+    if (trace) {
+        double ph[5];
+        SBPX_HIP(hipMemcpyAsync(ph, c.red + 3000, sizeof(ph), hipMemcpyDeviceToHost, c.stream));
+        SBPX_HIP(hipStreamSynchronize(c.stream));
+        std::fprintf(stderr, "mh phases (wide kernel, ms): evaluate %.2f checks %.2f commits %.2f (%.0f) barrier %.2f\n",
+                     ph[0] * 1e-5, ph[1] * 1e-5, ph[2] * 1e-5, ph[4], ph[3] * 1e-5);
+        std::fprintf(stderr,
+                     "mh pass: B=%d n=%d chunks=%d (G=1 in %d) windows=%.0f entries/window=%.1f "
+                     "workgroups/window=%.2f evals=%.0f slots/eval=%.0f kernel=%.2f ms\n",
+                     B, n, tr_chunks, tr_g1, tr_windows, tr_windows > 0 ? n / tr_windows : 0.0,
+                     tr_windows > 0 ? tr_k / tr_windows : 0.0, tr_evals, tr_evals > 0 ? tr_scan / tr_evals : 0.0,
+                     tr_ms);
+        (void)hipEventDestroy(tr_e0);
+        (void)hipEventDestroy(tr_e1);
     }
 }
 

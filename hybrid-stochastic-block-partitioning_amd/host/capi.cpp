@@ -19,7 +19,12 @@ typedef struct sbp_run_stats {
     double eval_bytes[4]; // their algorithmic bytes (when counting is on)
     int64_t eval_launches[4];
     double mdl;           // final MDL (float, as Blockmodel caches it)
+    int32_t stopped_at_limit;  // 1 when the run stopped at the time limit (sbp_set_time_limit)
 } sbp_run_stats;
+
+// Time limit in seconds of the following partitions on this thread (Args::max_seconds; 0: none).
+static thread_local double g_time_limit = 0.0;
+extern "C" void sbp_set_time_limit(double seconds) { g_time_limit = seconds > 0.0 ? seconds : 0.0; }
 
 static void set_err(char* err, int errlen, const char* msg) {
     if (err && errlen > 0) {
@@ -151,6 +156,7 @@ int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int app
         if (checkpoint) a.checkpoint = checkpoint;
         a.resume = resume != 0;
         a.max_iterations = max_iterations;
+        a.max_seconds = g_time_limit;
         auto t0 = std::chrono::steady_clock::now();
         Blockmodel bm = sbp::stochastic_block_partition(graph, a);
         double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -171,6 +177,7 @@ int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int app
                 stats->eval_launches[b] = finetune::totals.eval_launches[b];
             }
             stats->mdl = bm.getOverall_entropy();
+            stats->stopped_at_limit = sbp::stopped_at_time_limit() ? 1 : 0;
         }
         return 0;
     } catch (const std::exception& e) {

@@ -15,6 +15,29 @@
 
 thread_local Args args;
 
+// The time limit of the partition in progress (Args::max_seconds), measured from its start.
+static thread_local std::chrono::steady_clock::time_point limit_start;
+static thread_local bool limit_reached = false;
+// With SBP_TRACE set: a line on stderr at most every 20 s from inside the sweeps of a phase.
+static thread_local std::chrono::steady_clock::time_point last_trace;
+static void trace_progress(int iteration, int num_blocks, double entropy) {
+    static const bool on = std::getenv("SBP_TRACE") != nullptr;
+    if (!on) return;
+    const auto now = std::chrono::steady_clock::now();
+    if (std::chrono::duration<double>(now - last_trace).count() < 20.0) return;
+    last_trace = now;
+    std::fprintf(stderr, "sbp sweeps: B=%d sweep %d of this phase, mdl=%.3f, %.1f s into the partition\n", num_blocks,
+                 iteration + 1, entropy, std::chrono::duration<double>(now - limit_start).count());
+}
+
+static bool past_time_limit() {
+    if (args.max_seconds <= 0.0) return false;
+    if (std::chrono::duration<double>(std::chrono::steady_clock::now() - limit_start).count() < args.max_seconds)
+        return false;
+    limit_reached = true;
+    return true;
+}
+
 namespace sbpx_host {
 
 static void check(sbpx_ctx* ctx, sbpx_status st, const char* what) {
@@ -95,7 +118,84 @@ Graph::Graph(int num_vertices, std::vector<int32_t> out_ptr, std::vector<int32_t
     high_.resize(nh);
     low_.resize(num_vertices - nh);
     check(dev_->ctx, sbpx_graph_split(dev_->ctx, high_.data(), &nh, low_.data()), "sbpx_graph_split");
+    out_ptr_ = std::make_shared<const std::vector<int32_t>>(std::move(out_ptr));
+    out_idx_ = std::make_shared<const std::vector<int32_t>>(std::move(out_idx));
+    in_ptr_ = std::make_shared<const std::vector<int32_t>>(std::move(in_ptr));
+    in_idx_ = std::make_shared<const std::vector<int32_t>>(std::move(in_idx));
 }
+
+static std::shared_ptr<NeighborList> neighbor_lists(const std::vector<int32_t>& ptr, const std::vector<int32_t>& idx,
+                                                    int n) {
+    auto lists = std::make_shared<NeighborList>(n);
+    for (int v = 0; v < n; ++v) (*lists)[v].assign(idx.begin() + ptr[v], idx.begin() + ptr[v + 1]);
+    return lists;
+}
+
+const NeighborList& Graph::out_neighbors() const {
+    if (!out_ptr_) throw std::runtime_error("Graph::out_neighbors: no host adjacency for a graph of a batch");
+    if (!out_lists_) out_lists_ = neighbor_lists(*out_ptr_, *out_idx_, num_vertices_);
+    return *out_lists_;
+}
+
+const NeighborList& Graph::in_neighbors() const {
+    if (!in_ptr_) throw std::runtime_error("Graph::in_neighbors: no host adjacency for a graph of a batch");
+    if (!in_lists_) in_lists_ = neighbor_lists(*in_ptr_, *in_idx_, num_vertices_);
+    return *in_lists_;
+}
+
+// ---------------------------------------------------------------------------- Delta
+std::map<int, int>& Delta::route(int row, int col, int& key) {
+    if (row == current_) { key = col; return current_row_; }
+    if (row == proposed_) { key = col; return proposed_row_; }
+    if (col == current_) { key = row; return current_col_; }
+    if (col == proposed_) { key = row; return proposed_col_; }
+    throw std::logic_error("Neither the row nor column are current_block or proposed_block.");
+}
+
+int Delta::get(int row, int col) const {
+    const std::map<int, int>* m;
+    int key;
+    if (row == current_) { m = &current_row_; key = col; }
+    else if (row == proposed_) { m = &proposed_row_; key = col; }
+    else if (col == current_) { m = &current_col_; key = row; }
+    else if (col == proposed_) { m = &proposed_col_; key = row; }
+    else throw std::logic_error("Neither the row nor column are current_block or proposed_block.");
+    auto it = m->find(key);
+    return it == m->end() ? 0 : it->second;
+}
+
+std::vector<std::tuple<int, int, int>> Delta::entries() const {
+    std::vector<std::tuple<int, int, int>> out;
+    for (const auto& e : current_row_) out.emplace_back(current_, e.first, e.second);
+    for (const auto& e : proposed_row_) out.emplace_back(proposed_, e.first, e.second);
+    for (const auto& e : current_col_) out.emplace_back(e.first, current_, e.second);
+    for (const auto& e : proposed_col_) out.emplace_back(e.first, proposed_, e.second);
+    return out;
+}
+
+// ---------------------------------------------------------------------------- BlockMatrix
+int BlockMatrix::get(int row, int col) const {
+    owner_.make_live();
+    sbpx_ctx* c = owner_.ctx();
+    int32_t r = row, k = col, v = 0;
+    check(c, sbpx_blockmodel_cells(c, 1, &r, &k, &v), "blockmatrix()->get");
+    return v;
+}
+
+static std::map<int, int> matrix_line(const Blockmodel& owner, int x, int by_column) {
+    owner.make_live();
+    sbpx_ctx* c = owner.ctx();
+    int32_t n = 0;
+    check(c, sbpx_blockmodel_line(c, x, by_column, 0, nullptr, nullptr, &n), "blockmatrix() line");
+    std::vector<int32_t> idx(n), val(n);
+    if (n > 0) check(c, sbpx_blockmodel_line(c, x, by_column, n, idx.data(), val.data(), &n), "blockmatrix() line");
+    std::map<int, int> out;
+    for (int i = 0; i < n; ++i) out[idx[i]] = val[i];
+    return out;
+}
+
+std::map<int, int> BlockMatrix::getrow_sparse(int row) const { return matrix_line(owner_, row, 0); }
+std::map<int, int> BlockMatrix::getcol_sparse(int col) const { return matrix_line(owner_, col, 1); }
 
 // ---------------------------------------------------------------------------- Blockmodel
 Blockmodel::Blockmodel(int num_blocks_, const Graph& graph, float rate) : empty(false) {
@@ -159,6 +259,22 @@ std::vector<int> Blockmodel::block_assignment() const {
     dev_->download(a.data());
     return a;
 }
+int Blockmodel::block_assignment(int vertex) const {
+    if (vertex < 0 || vertex >= dev_->N) throw std::out_of_range("vertex out of range");
+    return block_assignment()[vertex];
+}
+
+void Blockmodel::move_vertex(int vertex, const Delta& delta, utils::ProposalAndEdgeCounts& proposal) {
+    make_live();
+    sbpx_ctx* c = ctx();
+    if (delta.current_block() != block_assignment(vertex) || delta.proposed_block() != proposal.proposal)
+        throw std::invalid_argument("move_vertex: the Delta is not this vertex's move to the proposal");
+    int32_t v = vertex, s = proposal.proposal;
+    check(c, sbpx_apply_moves(c, 1, &v, &s), "move_vertex");
+    snapshot_.clear();
+    mark_live();
+}
+
 std::vector<int> Blockmodel::degrees() const {
     make_live();
     std::vector<int> d(num_blocks);
@@ -391,7 +507,9 @@ static Blockmodel& batched_mcmc(int algo, Blockmodel& blockmodel, const Graph& g
         double entropy = entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges());
         delta_entropies.push_back(entropy - last_iteration_entropy);
         MCMC_iterations++;
+        trace_progress(iteration, blockmodel.getNum_blocks(), entropy);
         if (early_stop(iteration, blockmodels, initial_entropy, delta_entropies)) break;
+        if (past_time_limit()) break;
     }
     blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
     return blockmodel;
@@ -420,14 +538,119 @@ Blockmodel& metropolis_hastings(Blockmodel& blockmodel, const Graph& graph, Bloc
         delta_entropies.push_back(st.dS_accepted);
         MCMC_iterations++;
         if (early_stop(iteration, blockmodels, blockmodel.getOverall_entropy(), delta_entropies)) break;
+        if (past_time_limit()) break;
     }
     blockmodel.setOverall_entropy(entropy::mdl(blockmodel, graph.num_vertices(), graph.num_edges()));
     return blockmodel;
 }
+
+EdgeWeights edge_weights(const NeighborList& neighbors, int vertex, bool ignore_self) {
+    EdgeWeights w;
+    for (int neighbor : neighbors[vertex]) {
+        if (ignore_self && neighbor == vertex) continue;
+        w.indices.push_back(neighbor);
+        w.values.push_back(1);
+    }
+    return w;
+}
+
+// The out loop adds each edge to the proposed block's row and takes it from the current block's
+// row; the in loop does the same for the columns; a self edge goes to (proposed, proposed).
+Delta blockmodel_delta(int vertex, int current_block, int proposed_block, const EdgeWeights& out_edges,
+                       const EdgeWeights& in_edges, const Blockmodel& blockmodel) {
+    const std::vector<int> assignment = blockmodel.block_assignment();
+    Delta delta(current_block, proposed_block);
+    for (size_t i = 0; i < out_edges.indices.size(); ++i) {
+        const int w = out_edges.indices[i], b = assignment.at(w), weight = out_edges.values[i];
+        if (w == vertex) {
+            delta.add(proposed_block, proposed_block, weight);
+            delta.self_edge_weight(1);
+        } else {
+            delta.add(proposed_block, b, weight);
+        }
+        delta.sub(current_block, b, weight);
+    }
+    for (size_t i = 0; i < in_edges.indices.size(); ++i) {
+        const int w = in_edges.indices[i], b = assignment.at(w), weight = in_edges.values[i];
+        if (w == vertex) {
+            delta.add(proposed_block, proposed_block, weight);
+            delta.self_edge_weight(1);
+        } else {
+            delta.add(b, proposed_block, weight);
+        }
+        delta.sub(b, current_block, weight);
+    }
+    return delta;
+}
 }  // namespace finetune
+
+namespace entropy {
+double delta_mdl(const Blockmodel& blockmodel, const Delta& delta, const utils::ProposalAndEdgeCounts& proposal) {
+    blockmodel.make_live();
+    sbpx_ctx* c = blockmodel.ctx();
+    std::vector<int32_t> rows, cols, vals;
+    for (const auto& e : delta.entries()) {
+        rows.push_back(std::get<0>(e));
+        cols.push_back(std::get<1>(e));
+        vals.push_back(std::get<2>(e));
+    }
+    double dS = 0.0;
+    sbpx_host::check(c,
+                     sbpx_delta_mdl(c, delta.current_block(), delta.proposed_block(), (int32_t)rows.size(), rows.data(),
+                                    cols.data(), vals.data(), proposal.num_out_neighbor_edges,
+                                    proposal.num_in_neighbor_edges, delta.self_edge_weight(), &dS),
+                     "entropy::delta_mdl");
+    return dS;
+}
+
+double hastings_correction(int vertex, const Graph& graph, const Blockmodel& blockmodel, const Delta& delta,
+                           int current_block, const utils::ProposalAndEdgeCounts& proposal) {
+    (void)graph;
+    blockmodel.make_live();
+    sbpx_ctx* c = blockmodel.ctx();
+    if (current_block != blockmodel.block_assignment(vertex) || delta.proposed_block() != proposal.proposal)
+        throw std::invalid_argument("hastings_correction: the Delta is not this vertex's move to the proposal");
+    int32_t v = vertex, s = proposal.proposal;
+    double u = 0.5, dS = 0.0, H = 1.0;
+    uint8_t acc = 0;
+    sbpx_host::check(c, sbpx_eval_moves(c, 1, &v, &s, &u, &dS, &H, &acc), "entropy::hastings_correction");
+    return H;
+}
+
+double block_merge_delta_mdl(int current_block, utils::ProposalAndEdgeCounts proposal, const Blockmodel& blockmodel,
+                             const Delta& delta) {
+    blockmodel.make_live();
+    sbpx_ctx* c = blockmodel.ctx();
+    if (delta.current_block() != current_block || delta.proposed_block() != proposal.proposal)
+        throw std::invalid_argument("block_merge_delta_mdl: the Delta is not this merge's");
+    int32_t r = current_block, s = proposal.proposal;
+    double dS = 0.0;
+    sbpx_host::check(c, sbpx_merge_eval(c, 1, &r, &s, &dS), "entropy::block_merge_delta_mdl");
+    return dS;
+}
+}  // namespace entropy
 
 // ---------------------------------------------------------------------------- block merge
 namespace block_merge {
+Delta blockmodel_delta(int current_block, int proposed_block, const Blockmodel& blockmodel) {
+    Delta delta(current_block, proposed_block);
+    const std::shared_ptr<BlockMatrix> m = blockmodel.blockmatrix();
+    for (const auto& e : m->getrow_sparse(current_block)) {
+        const int col = e.first, value = e.second;
+        if (col == current_block || col == proposed_block) delta.add(proposed_block, proposed_block, value);
+        else delta.add(proposed_block, col, value);
+        delta.sub(current_block, col, value);
+    }
+    for (const auto& e : m->getcol_sparse(current_block)) {
+        const int row = e.first, value = e.second;
+        if (row == current_block) continue;
+        if (row == proposed_block) delta.add(proposed_block, proposed_block, value);
+        else delta.add(row, proposed_block, value);
+        delta.sub(row, current_block, value);
+    }
+    return delta;
+}
+
 Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edges) {
     (void)num_edges;
     blockmodel.make_live();
@@ -474,9 +697,13 @@ bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_tr
 }
 
 // sbp::stochastic_block_partition (sbp.cpp:61-98).
+bool stopped_at_time_limit() { return limit_reached; }
+
 Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
     args = a;
     partition_start = std::chrono::steady_clock::now();
+    limit_start = partition_start;
+    limit_reached = false;
     intermediate_results.clear();
     finetune::MCMC_iterations = 0;
     finetune::MCMC_time = 0.0;
@@ -513,6 +740,7 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
     int iterations_this_call = 0;
     while (!done_blockmodeling(blockmodel, blockmodel_triplet)) {
         if (args.max_iterations > 0 && iterations_this_call++ >= args.max_iterations) break;
+        if (past_time_limit()) break;
         blockmodel = block_merge::merge_blocks(blockmodel, graph, graph.num_edges());
         auto start = std::chrono::steady_clock::now();
         if (args.algorithm == "async_gibbs" && iteration < float(args.asynciterations))
@@ -527,6 +755,7 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& a) {
         if (std::getenv("SBP_TRACE"))
             std::fprintf(stderr, "sbp iter %.0f: B=%d mdl=%.3f mcmc_iterations=%d\n", iteration,
                          blockmodel.getNum_blocks(), (double)blockmodel.getOverall_entropy(), finetune::MCMC_iterations);
+        if (limit_reached) break;  // keep the state the sweeps reached
         blockmodel = blockmodel_triplet.get_next_blockmodel(blockmodel);
         if (!args.checkpoint.empty()) {  // written to a temporary file, then renamed over the old one
             const std::string tmp = args.checkpoint + ".tmp";

@@ -8,8 +8,10 @@
 #include <cstdio>
 #include <cstdint>
 #include <limits>
+#include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "sbpx.h"
@@ -30,6 +32,9 @@ struct Args {
     std::string checkpoint;
     bool resume = false;
     int max_iterations = 0;
+    // time limit (no counterpart in the reference): > 0 stops the partition, after the sweep in
+    // progress, once it has run this many seconds; sbp::stopped_at_time_limit() then reports it
+    double max_seconds = 0.0;
 };
 
 namespace sbpx_host {
@@ -49,6 +54,44 @@ struct LoadedGraph {
 LoadedGraph load(const std::string& base_path, bool undirected);
 }  // namespace graph_io
 
+/// typedefs.hpp:20, utils.hpp:21-28 and the EdgeWeights of the reference.
+using NeighborList = std::vector<std::vector<int>>;
+struct EdgeWeights {
+    std::vector<int> indices;
+    std::vector<int> values;
+};
+namespace utils {
+typedef struct proposal_and_edge_counts_t {
+    int proposal;
+    int num_out_neighbor_edges;
+    int num_in_neighbor_edges;
+    int num_neighbor_edges;
+} ProposalAndEdgeCounts;
+}  // namespace utils
+
+/// Delta (delta.hpp:22-149): the changes a move makes to the cells of rows and columns r
+/// (current block) and s (proposed block), routed row first. A cell keeps its key when its change
+/// returns to zero, as the reference's hash maps do; entries() lists them in ascending order.
+class Delta {
+public:
+    Delta() = default;
+    Delta(int current_block, int proposed_block, int buckets = 10)
+        : current_(current_block), proposed_(proposed_block) { (void)buckets; }
+    void add(int row, int col, int value) { int key; route(row, col, key)[key] += value; }
+    void sub(int row, int col, int value) { int key; route(row, col, key)[key] -= value; }
+    int get(int row, int col) const;
+    std::vector<std::tuple<int, int, int>> entries() const;
+    int self_edge_weight() const { return self_weight_; }
+    void self_edge_weight(int weight) { self_weight_ = weight; }
+    int current_block() const { return current_; }
+    int proposed_block() const { return proposed_; }
+
+private:
+    std::map<int, int>& route(int row, int col, int& key);
+    int current_ = -1, proposed_ = -1, self_weight_ = 0;
+    std::map<int, int> current_row_, proposed_row_, current_col_, proposed_col_;
+};
+
 class Graph {
 public:
     Graph() = default;
@@ -66,11 +109,32 @@ public:
     const std::vector<int>& high_degree_vertices() const { return high_; }
     const std::vector<int>& low_degree_vertices() const { return low_; }
     std::shared_ptr<sbpx_host::Device> device() const { return dev_; }
+    /// Graph::out_neighbors / in_neighbors (graph.hpp): adjacency lists in stored order, built on
+    /// first use from the host copy of the CSR / CSC (not available for a graph of a batch).
+    const NeighborList& out_neighbors() const;
+    const NeighborList& in_neighbors() const;
 
 private:
     int num_vertices_ = 0, num_edges_ = 0;
     std::vector<int> high_, low_;
     std::shared_ptr<sbpx_host::Device> dev_;
+    std::shared_ptr<const std::vector<int32_t>> out_ptr_, out_idx_, in_ptr_, in_idx_;
+    mutable std::shared_ptr<NeighborList> out_lists_, in_lists_;
+};
+
+class Blockmodel;
+/// The block matrix as Blockmodel::blockmatrix() returns it (ISparseMatrix, csparse_matrix.hpp:
+/// 62-139), read from the device holding the blockmodel.
+class BlockMatrix {
+public:
+    explicit BlockMatrix(const Blockmodel& owner) : owner_(owner) {}
+    int get(int row, int col) const;
+    /// Nonzeros of a row / column as (index, value) in ascending index order.
+    std::map<int, int> getrow_sparse(int row) const;
+    std::map<int, int> getcol_sparse(int col) const;
+
+private:
+    const Blockmodel& owner_;
 };
 
 static const double BLOCK_REDUCTION_RATE = 0.5;
@@ -94,6 +158,14 @@ public:
     float& getBlock_reduction_rate() { return block_reduction_rate; }
     double block_size_variation() const;
     double interblock_edges() const;
+    /// block_assignment(vertex) (blockmodel.hpp:124).
+    int block_assignment(int vertex) const;
+    /// blockmatrix() (blockmodel.hpp:120): cell reads go to the device.
+    std::shared_ptr<BlockMatrix> blockmatrix() const { return std::make_shared<BlockMatrix>(*this); }
+    /// move_vertex(vertex, delta, proposal) (blockmodel.cpp:344-360) on the device: the vertex takes
+    /// proposal.proposal, the matrix and dout/din follow, and the total degrees of the two blocks
+    /// keep the reference's drift. `delta` must be the move's own (finetune::blockmodel_delta).
+    void move_vertex(int vertex, const Delta& delta, utils::ProposalAndEdgeCounts& proposal);
     bool empty;
 
     // device binding
@@ -140,6 +212,18 @@ protected:
 
 namespace entropy {
 double mdl(const Blockmodel& blockmodel, int num_vertices, int num_edges);
+/// delta_mdl(blockmodel, delta, proposal) (entropy.hpp:43, entropy.cpp:296-401), on the device.
+double delta_mdl(const Blockmodel& blockmodel, const Delta& delta, const utils::ProposalAndEdgeCounts& proposal);
+/// hastings_correction(vertex, graph, blockmodel, delta, current_block, proposal) (entropy.hpp:56,
+/// entropy.cpp:495-557): the device evaluates the move of `vertex` to proposal.proposal; `delta`
+/// must be that move's Delta, as in every call the reference makes.
+double hastings_correction(int vertex, const Graph& graph, const Blockmodel& blockmodel, const Delta& delta,
+                           int current_block, const utils::ProposalAndEdgeCounts& proposal);
+/// block_merge_delta_mdl(current_block, proposal, blockmodel, delta) (entropy.hpp:27,
+/// entropy.cpp:127-180): the device scores merging current_block into proposal.proposal; `delta`
+/// must be that merge's (block_merge::blockmodel_delta).
+double block_merge_delta_mdl(int current_block, utils::ProposalAndEdgeCounts proposal, const Blockmodel& blockmodel,
+                             const Delta& delta);
 double null_mdl_v1(long num_edges);
 double null_mdl_v2(long num_vertices, long num_edges);
 double normalize_mdl_v1(double mdl, long num_edges);
@@ -161,10 +245,19 @@ Blockmodel& hybrid_mcmc(Blockmodel& blockmodel, const Graph& graph, BlockmodelTr
 Blockmodel& metropolis_hastings(Blockmodel& blockmodel, const Graph& graph, BlockmodelTriplet& blockmodels);
 bool early_stop(int iteration, BlockmodelTriplet& blockmodels, double initial_entropy,
                 std::vector<double>& delta_entropies);
+/// edge_weights (finetune.cpp:275-290): the neighbours of `vertex`, weight 1 each.
+EdgeWeights edge_weights(const NeighborList& neighbors, int vertex, bool ignore_self = false);
+/// blockmodel_delta (finetune.hpp:67-68, finetune.cpp:108-142): the Delta of moving `vertex` from
+/// current_block to proposed_block, from the blocks its neighbours hold on the device.
+Delta blockmodel_delta(int vertex, int current_block, int proposed_block, const EdgeWeights& out_edges,
+                       const EdgeWeights& in_edges, const Blockmodel& blockmodel);
 }  // namespace finetune
 
 namespace block_merge {
 static const int NUM_AGG_PROPOSALS_PER_BLOCK = 10;
+/// blockmodel_delta (block_merge.hpp:37, block_merge.cpp:13-37): the Delta of merging
+/// current_block into proposed_block, from row and column current_block on the device.
+Delta blockmodel_delta(int current_block, int proposed_block, const Blockmodel& blockmodel);
 Blockmodel& merge_blocks(Blockmodel& blockmodel, const Graph& graph, int num_edges);
 }  // namespace block_merge
 
@@ -212,6 +305,8 @@ BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::
                          const std::vector<Args>& graph_args, int device = 0, int eval_variant = 0,
                          bool count_bytes = false);
 bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks = 0);
+/// Whether the last stochastic_block_partition on this thread stopped at Args::max_seconds.
+bool stopped_at_time_limit();
 }  // namespace sbp
 
 // The driver's globals are per thread, so that the graphs of a batch run their drivers side by side.

@@ -78,9 +78,10 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            edges when they are at most a quarter of the edges, else rebuilt by a full sort;
  *            0 always rebuilt. Both give the same arrays.
  * option 6 = workgroups of the sequential Metropolis-Hastings pass when one wavefront evaluates
- *            each vertex: 1 (one workgroup, 16 speculative evaluations per window), 2, 4, 8, 16 or
- *            32 (16 per workgroup in one cooperative launch), or 0 to follow the window yield
- *            (default). Every choice applies the same moves in the same order.
+ *            each vertex: 1 (one workgroup, 16 speculative evaluations per window), a power of two
+ *            up to 256 (16 per workgroup in one cooperative launch; capped at the workgroups the
+ *            device holds at once), or 0 to follow the window yield (default). Every choice
+ *            applies the same moves in the same order.
  * option 7 = spare slots of the Metropolis-Hastings / heap-merge table pool when it is built:
  *            0 (default) as many as the initial layout holds (at least 2^20); a positive value
  *            limits the pool to exactly that many (and restarts the doubling that follows each
@@ -120,6 +121,24 @@ sbpx_status sbpx_blockmodel_trace(sbpx_ctx* ctx, int64_t* trace, int64_t* edges)
  * (blockmodel.cpp:267-285). */
 sbpx_status sbpx_mdl(sbpx_ctx* ctx, double* mdl);
 sbpx_status sbpx_log_posterior(sbpx_ctx* ctx, double* log_posterior);
+
+/* entropy::delta_mdl(blockmodel, delta, proposal) (entropy.cpp:296-401) of a given Delta of a
+ * vertex move r -> s: its n stored cells (rows[i], cols[i]) with changes deltas[i] (a cell stored
+ * with change 0 counts as the reference's Delta counts it), the proposal's k_out and k_in and the
+ * Delta's self-edge weight. Each cell must lie in row or column r or s and appear once. */
+sbpx_status sbpx_delta_mdl(sbpx_ctx* ctx, int32_t r, int32_t s, int32_t n, const int32_t* rows, const int32_t* cols,
+                           const int32_t* deltas, int32_t k_out, int32_t k_in, int32_t self_weight, double* dS);
+/* entropy::block_merge_delta_mdl (entropy.cpp:127-180) of merging block r[i] into s[i], each on the
+ * current state (block_merge::blockmodel_delta, block_merge.cpp:13-37, gives the Delta). */
+sbpx_status sbpx_merge_eval(sbpx_ctx* ctx, int32_t n, const int32_t* r, const int32_t* s, double* dS);
+/* Block-matrix cells M[rows[i], cols[i]] (0 where absent): blockmatrix()->get(row, col)
+ * (dict_transpose_matrix.cpp:74-84). */
+sbpx_status sbpx_blockmodel_cells(sbpx_ctx* ctx, int32_t n, const int32_t* rows, const int32_t* cols, int32_t* vals);
+/* The nonzeros of row x (by_column = 0) or column x (1) in ascending index order: getrow_sparse /
+ * getcol_sparse (dict_transpose_matrix.cpp:86-122). count receives their number; at most cap are
+ * written. */
+sbpx_status sbpx_blockmodel_line(sbpx_ctx* ctx, int32_t x, int by_column, int32_t cap, int32_t* idx, int32_t* vals,
+                                 int32_t* count);
 
 /* Evaluate fixed proposals (v[i] -> s[i], uniform u[i]) against the current state without
  * applying them: entropy::delta_mdl (entropy.cpp:296-401), entropy::hastings_correction

@@ -110,6 +110,7 @@ def load():
     L.sbpx_learned_forward.argtypes = [vp, C.c_int32, f32p, f32p]
     L.sbpx_learned_eval_moves.argtypes = [vp, C.c_int32, i32p, i32p, f64p, f64p, f64p, u8p, f32p, f32p]
     L.sbpx_learned_scores.argtypes = [vp, C.c_int32, i32p, C.c_int, i32p, f32p]
+    L.sbpx_log_posterior_noalloc.argtypes = [vp, C.c_int64, vp, vp, vp, vp, vp, vp, vp]
     L.sbpx_learned_gradients.argtypes = [vp, C.c_int32, f32p, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                          i32p, f32p, f32p]
     L.sbpx_batch_define.argtypes = [vp, C.c_int32, i32p]
@@ -179,7 +180,7 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_learned_forward", "sbpx_learned_gradients", "sbpx_batch_define", "sbpx_batch_info",
             "sbpx_batch_build", "sbpx_batch_download", "sbpx_batch_log_posterior", "sbpx_batch_sweep",
             "sbpx_batch_merge", "sbpx_blockmodel_trace", "sbpx_batch_trace", "sbpx_learned_eval_moves",
-            "sbpx_learned_scores", "sbpx_delta_mdl", "sbpx_merge_eval",
+            "sbpx_learned_scores", "sbpx_log_posterior_noalloc", "sbpx_delta_mdl", "sbpx_merge_eval",
             "sbpx_blockmodel_cells", "sbpx_blockmodel_line"]
 
 

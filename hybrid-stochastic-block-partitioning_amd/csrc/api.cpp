@@ -483,6 +483,7 @@ sbpx_status sbpx_create(sbpx_ctx** out, int device) {
         ctx->c.device = device;
         SBPX_HIP(hipSetDevice(device));
         SBPX_HIP(hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking));
+        ctx->c.noalloc_scratch = noalloc_scratch_address();  // with the context's device current
         dalloc(&ctx->c.dev_err, 4);
         SBPX_HIP(hipMemset(ctx->c.dev_err, 0, 4 * sizeof(int)));
         dalloc(&ctx->c.counters, 8);
@@ -880,6 +881,20 @@ sbpx_status sbpx_blockmodel_line(sbpx_ctx* ctx, int32_t x, int by_column, int32_
                                     hipMemcpyDeviceToHost, c.stream));
             c.sync();
         }
+    });
+}
+
+sbpx_status sbpx_log_posterior_noalloc(sbpx_ctx* ctx, int64_t nnz, const int32_t* coo_row, const int32_t* col,
+                                       const int32_t* val, const int32_t* dout, const int32_t* din, double* out,
+                                       void* stream) {
+    return guarded(ctx, [&] {
+        if (nnz < 0 || nnz > SBPX_NOALLOC_MAX_CELLS) throw ArgError("cell count must be in [0, SBPX_NOALLOC_MAX_CELLS]");
+        if (nnz > 0 && (!coo_row || !col || !val || !dout || !din)) throw ArgError("null cell or degree array");
+        if (!out) throw ArgError("null output");
+        if (!ctx->c.noalloc_scratch) throw StateError("no scratch array on this device");
+        SBPX_HIP(hipSetDevice(ctx->c.device));
+        log_posterior_noalloc(ctx->c.noalloc_scratch, nnz, coo_row, col, val, dout, din, out,
+                              stream ? (hipStream_t)stream : ctx->c.stream);
     });
 }
 

@@ -732,6 +732,38 @@ void log_posterior_segments(Ctx& c, double* out) {
     check_device_error(c);
 }
 
+// ------------------------------------------------------------------------------------------
+// The log posterior without allocations (sbpx_log_posterior_noalloc): the kernels above, with the
+// segment words, the error word and the chunk partial sums in a fixed device array. Each device
+// holds its own copy of the array until the process ends; every call on a device shares it.
+constexpr long long NOALLOC_MAX_CELLS = 1LL << 28;
+constexpr size_t NOALLOC_PARTS = (size_t)(NOALLOC_MAX_CELLS / LP_CHUNK) + 8;
+constexpr size_t NOALLOC_HEAD = 256;  // bytes before the partial sums: segment words, error word
+static __device__ __align__(256) unsigned char scratch[NOALLOC_HEAD + sizeof(double) * NOALLOC_PARTS];
+
+// The current device's address of the array (host code must not use the symbol itself).
+void* noalloc_scratch_address() {
+    void* p = nullptr;
+    SBPX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    return p;
+}
+
+__global__ void zero_word_kernel(int* w) { *w = 0; }
+
+void log_posterior_noalloc(void* base, long long nnz, const int* coo_row, const int* col, const int* val,
+                           const int* dout, const int* din, double* out, hipStream_t st) {
+    if (nnz < 0 || nnz > NOALLOC_MAX_CELLS) throw ArgError("cell count must be in [0, 2^28]");
+    int* seg = (int*)base;  // lo, hi and the chunk offsets of one graph: 4 words
+    int* err = seg + 8;
+    double* part = (double*)((unsigned char*)base + NOALLOC_HEAD);
+    const int chunks = (int)((nnz + LP_CHUNK - 1) / LP_CHUNK);
+    zero_word_kernel<<<1, 1, 0, st>>>(err);
+    lp_segments_kernel<<<1, TB, 0, st>>>(nullptr, 1, nullptr, nullptr, (int)nnz, seg);
+    if (chunks > 0) lp_chunk_kernel<<<chunks, TB, 0, st>>>(coo_row, col, val, dout, din, seg, 1, part, err);
+    lp_graph_sum_kernel<<<1, TB, 0, st>>>(part, seg, 1, out);
+    SBPX_HIP(hipGetLastError());
+}
+
 double log_posterior(Ctx& c) {
     if (c.batch.G > 0) throw StateError("the context holds a batch of graphs");
     double out = 0.0;

@@ -197,6 +197,7 @@ struct IncBuffers {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void* noalloc_scratch = nullptr;  // this device's address of the allocation-free path's array
     std::string err;
     DevGraph g;
     DevBlockmodel bm;
@@ -409,6 +410,10 @@ void launch_delta_mdl(Ctx& c, int r, int s, int n, const int* er, const int* ec,
                       int self_w, double* d_out);
 void launch_merge_eval(Ctx& c, int n, const int* r, const int* s, double* d_out);
 void launch_cells(Ctx& c, int n, const int* rows, const int* cols, int* d_out);
+// blockmodel.hip: the fixed scratch array of the allocation-free log posterior
+void* noalloc_scratch_address();
+void log_posterior_noalloc(void* base, long long nnz, const int* coo_row, const int* col, const int* val,
+                           const int* dout, const int* din, double* out, hipStream_t st);
 // The sequential pass's in-wavefront scorer on the listed vertices (degree <= max_deg): candidates
 // and scores [n][32].
 void learned_scores_probe(Ctx& c, const int* d_list, int n, int max_deg, int* d_cand, float* d_z);

@@ -140,6 +140,21 @@ sbpx_status sbpx_blockmodel_cells(sbpx_ctx* ctx, int32_t n, const int32_t* rows,
 sbpx_status sbpx_blockmodel_line(sbpx_ctx* ctx, int32_t x, int by_column, int32_t cap, int32_t* idx, int32_t* vals,
                                  int32_t* count);
 
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid allocation). The log posterior of
+ * sbpx_log_posterior, of a block matrix the caller holds in device memory: nnz nonzero cells in
+ * row-major order (coo_row[i], col[i], val[i]), block degrees dout/din; the result, one double, is
+ * written to the device address `out` on `stream` (a hipStream_t; NULL: the context's stream) and
+ * the call returns without waiting. All pointers are device addresses the caller owns. No device
+ * memory is allocated: partial sums live in a fixed array of the library on the context's device,
+ * sized for nnz <= SBPX_NOALLOC_MAX_CELLS (larger: SBPX_ERR_ARG), held until the process ends.
+ * Calls that overlap on one device race on that array. A nonzero cell whose row's dout or column's
+ * din is zero makes the result infinite (no status: the call does not read back). */
+#define SBPX_NOALLOC_MAX_CELLS (1LL << 28)
+sbpx_status sbpx_log_posterior_noalloc(sbpx_ctx* ctx, int64_t nnz, const int32_t* coo_row, const int32_t* col,
+                                       const int32_t* val, const int32_t* dout, const int32_t* din, double* out,
+                                       void* stream);
+
 /* Evaluate fixed proposals (v[i] -> s[i], uniform u[i]) against the current state without
  * applying them: entropy::delta_mdl (entropy.cpp:296-401), entropy::hastings_correction
  * (entropy.cpp:495-557), finetune::accept (finetune.cpp:19-29). */

@@ -257,6 +257,7 @@ def step_sweeps(m, arrays, args):
         mdl0 = c.mdl()
         for w in range(args.warmup):
             c.sweep(algo, args.batches, args.seed, w)
+        c.mdl()  # untimed; its log-posterior kernels also mark where the timed sweeps begin in a trace
         evals = attempts = moves = 0
         ms = np.zeros(4)
         launches = np.zeros(4)

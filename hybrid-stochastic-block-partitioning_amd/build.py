@@ -19,7 +19,8 @@ DEVICE_SOURCES = ["sort_scan.hip", "blockmodel.hip", "sweep.hip", "mh.hip", "mer
 HOST_SOURCES = ["sbp_host.cpp", "graph_io.cpp", "batch.cpp", "capi.cpp"]
 
 # No FMA contraction (the objective's FP32 cell chain and FP64 sums must match the reference's
-# uncontracted arithmetic); packed FP32 instructions off.
+# uncontracted arithmetic); packed FP32 instructions off (the flag acts in the device pass; the
+# host pass reports it as unknown and ignores it). check_no_packed_fp32 verifies the result.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result"]
@@ -35,6 +36,46 @@ def build_device():
     tmp = LIB + ".tmp"
     run([HIPCC] + HIP_FLAGS + ["-shared", "-x", "hip"] + srcs + ["-o", tmp])
     os.replace(tmp, LIB)
+
+
+LLVM = os.environ.get("SBPX_LLVM_BIN", "/opt/rocm/llvm/bin")
+PACKED_FP32 = ("v_pk_add_f32", "v_pk_mul_f32", "v_pk_fma_f32")
+
+
+def check_no_packed_fp32(lib=LIB):
+    """Fails the build if the library's gfx950 code contains packed FP32 instructions: the device
+    code objects (one bundle per source in the .hip_fatbin section) are unbundled and
+    disassembled, and the listing searched."""
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        fat = os.path.join(tmp, "fatbin.bin")
+        subprocess.check_call([os.path.join(LLVM, "llvm-objcopy"), "--dump-section", f".hip_fatbin={fat}", lib])
+        data = open(fat, "rb").read()
+        magic = b"__CLANG_OFFLOAD_BUNDLE__"
+        starts = []
+        i = data.find(magic)
+        while i != -1:
+            starts.append(i)
+            i = data.find(magic, i + 1)
+        found = []
+        for k, a in enumerate(starts):
+            b = starts[k + 1] if k + 1 < len(starts) else len(data)
+            bundle = os.path.join(tmp, f"b{k}.bin")
+            code = os.path.join(tmp, f"b{k}.o")
+            with open(bundle, "wb") as f:
+                f.write(data[a:b])
+            subprocess.check_call([os.path.join(LLVM, "clang-offload-bundler"), "--type=o",
+                                   f"--targets=hipv4-amdgcn-amd-amdhsa--{ARCH}", f"--input={bundle}",
+                                   f"--output={code}", "--unbundle"])
+            if os.path.getsize(code) == 0:
+                continue
+            listing = subprocess.run([os.path.join(LLVM, "llvm-objdump"), "-d", code], capture_output=True,
+                                     text=True, check=True).stdout
+            found += [line.strip() for line in listing.splitlines() if any(op in line for op in PACKED_FP32)]
+        if found:
+            raise SystemExit("packed FP32 instructions in the device code:\n" + "\n".join(found[:20]))
+        print(f"device code: {len(starts)} code-object bundles, no packed FP32 instructions", flush=True)
 
 
 def build_host():
@@ -63,6 +104,7 @@ def build_caller():
 
 def build():
     build_device()
+    check_no_packed_fp32()
     build_host()
     build_caller()
 

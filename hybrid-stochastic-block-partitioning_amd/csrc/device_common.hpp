@@ -51,7 +51,9 @@ __device__ __forceinline__ int random_block(int r, int B, uint32_t x) {
 
 // ------------------------------------------------------------------------------------------
 // logf as the host C library computes it (glibc >= 2.28): the table-driven method of
-// Szabolcs Nagy's optimized-routines logf. x = 2^k z with z in [OFF, 2 OFF); z is split into
+// Szabolcs Nagy's optimized-routines logf. The 16-entry table and the polynomial coefficients
+// below are that routine's (Arm optimized-routines, Copyright (c) 2017-2018 Arm Limited,
+// SPDX-License-Identifier: MIT OR Apache-2.0 WITH LLVM-exception; glibc ships the same code). x = 2^k z with z in [OFF, 2 OFF); z is split into
 // 16 subintervals, each with a reciprocal c^-1 and log(c); log(x) = k ln2 + log(c) +
 // log1p(z/c - 1), with log1p approximated by a degree-4 polynomial evaluated in double and
 // rounded once to float. The objective depends on this function bit for bit, so the device

@@ -2,6 +2,12 @@
 // Control flow follows src/sbp.cpp:61-112, src/finetune.cpp:31-89, 342-444, 144-162,
 // src/block_merge.cpp:278-311 and src/blockmodel/blockmodel_triplet.cpp:3-100; each sweep,
 // merge phase and MDL runs on the device.
+//
+// Attribution: the golden-section search (BlockmodelTriplet), finetune::early_stop, the null
+// models, Blockmodel::block_size_variation and the driver loop restate, step for step, logic of
+// vtsynergy/Hybrid-Stochastic-Block-Partitioning (GNU LGPL 2.1), because the search over the
+// number of blocks must take the same steps as the reference's; that logic remains under the
+// reference's licence.
 #include "sbp_host.hpp"
 
 #include <chrono>

@@ -22,7 +22,10 @@ struct Args {
     bool approximate = false;
     int asynciterations = INT_MAX;
     int batches = 10;
-    bool transpose = true;  // the device keeps both orientations; kept for flag parity
+    // The reference defaults to false (include/args.hpp:98; the published runs pass --transpose).
+    // The device always keeps both orientations, so the flag has no effect here; it defaults to
+    // true to say so.
+    bool transpose = true;
     int threads = 0;        // unused on the device path
     uint64_t seed = 0;
     int device = 0;

@@ -389,6 +389,25 @@ int oracle_sbp(void* gp, int algo, int batches, int approximate, uint64_t seed, 
     return oracle_sbp_traced(gp, algo, batches, approximate, seed, assign, out, nullptr, nullptr, 0);
 }
 
+// entropy::mdl (entropy.cpp:584-589) of a block matrix given as nonzero cells (any order) with its
+// block degrees, B blocks, for a graph of N vertices and E edges: the log posterior summed in
+// cell order with the reference's FP32 logf rule (blockmodel.cpp:267-285).
+int oracle_mdl_cells(long nnz, const int32_t* rows, const int32_t* cols, const int32_t* vals,
+                     const int32_t* dout, const int32_t* din, int B, long N, long E, double* out) {
+    GUARD_BEGIN
+    double lp = 0.0;
+    #pragma omp parallel for reduction(+ : lp) schedule(static)
+    for (long i = 0; i < nnz; ++i) {
+        double value = vals[i];
+        double d_out = dout[rows[i]], d_in = din[cols[i]];
+        lp += value * (double)logf((float)(value / (d_out * d_in)));
+    }
+    double x = pow(B, 2) / E;
+    double h = ((1 + x) * log(1 + x)) - (x * log(x));
+    *out = (E * h) + (N * log(B)) - lp;
+    GUARD_END
+}
+
 // finetune::early_stop (finetune.cpp:144-162) on a list of per-sweep MDL changes.
 int oracle_early_stop(int iteration, int bracket, double initial, const double* deltas, int n) {
     if (n <= 0) return -1;

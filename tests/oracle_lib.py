@@ -102,6 +102,8 @@ def lib():
                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.oracle_hastings_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]
+    L.oracle_mdl_cells.argtypes = [C.c_long, i32p, i32p, i32p, i32p, i32p, C.c_int, C.c_long, C.c_long,
+                                   C.POINTER(C.c_double)]
     L.oracle_sbp_traced.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, i32p, C.POINTER(OracleRun),
                                     i32p, f32p, C.c_int]
     _lib = L
@@ -363,6 +365,14 @@ def sbp_traced(graph, algo, batches, approximate, seed, cap=256):
     _check(lib().oracle_sbp_traced(graph.h, algo, batches, int(approximate), seed, a, C.byref(run), tb, tm, cap))
     k = min(run.outer_iterations, cap)
     return a, run, tb[:k].copy(), tm[:k].copy()
+
+
+def mdl_cells(rows, cols, vals, dout, din, B, N, E):
+    """entropy::mdl of a block matrix given as nonzero cells and block degrees (FP32 logf rule)."""
+    out = C.c_double()
+    a = [np.ascontiguousarray(x, np.int32) for x in (rows, cols, vals, dout, din)]
+    _check(lib().oracle_mdl_cells(len(a[0]), *a, int(B), int(N), int(E), C.byref(out)))
+    return out.value
 
 
 def null_mdl_v1(E):

@@ -697,3 +697,25 @@ def test_mixture_hastings_reduces_and_reverses(toy):
             assert H * Hr == pytest.approx(1.0, rel=1e-12), (v, r, s)
             products += 1
     assert products > 50
+
+
+def test_mdl_from_cells_equals_blockmodel_mdl(toy):
+    """The cell-list MDL used by the full-size checks equals the blockmodel's entropy::mdl (KAT on
+    the toy graph; random assignments of a random graph)."""
+    g = O.Graph.from_edges(toy["num_vertices"], toy["edges"])
+    bm = O.Blockmodel(g, 3, toy["assignment"])
+    r, c, v = bm.coo()
+    _, dout, din, _ = bm.state()
+    got = O.mdl_cells(r, c, v, dout, din, 3, toy["num_vertices"], g.E)
+    assert got == pytest.approx(toy["entropy"], rel=1e-7)
+    assert got == pytest.approx(bm.mdl(), rel=1e-12)
+    rng = np.random.default_rng(8)
+    n = 300
+    edges = sorted({(int(u), int(w)) for u, w in rng.integers(0, n, (2000, 2))})
+    g = O.Graph.from_edges(n, edges)
+    for B in (n, 40, 3):
+        a = np.arange(n, dtype=np.int32) if B == n else rng.integers(0, B, n).astype(np.int32)
+        bm = O.Blockmodel(g, B, a)
+        r, c, v = bm.coo()
+        _, dout, din, _ = bm.state()
+        assert O.mdl_cells(r, c, v, dout, din, B, n, g.E) == pytest.approx(bm.mdl(), rel=1e-12)

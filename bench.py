@@ -50,7 +50,7 @@ def load_pkg():
 KERNELS = ["eval_bin_kernel<64..256> (batched vertex-move evaluation, degree <= 126, LDS tables)",
            "eval_bin_kernel<512..1024> (batched vertex-move evaluation, degree 127-510, LDS tables)",
            "eval_group_kernel (batched vertex-move evaluation, degree > 510, 4-16 wavefronts per vertex)",
-           "mh_kernels (sequential Metropolis-Hastings evaluation and commit: mh_wide_kernel on 2-32 workgroups, "
+           "mh_kernels (sequential Metropolis-Hastings evaluation and commit: mh_wide_kernel on up to 512 workgroups, "
            "mh_kernel and mh_group_kernel<G> on one)"]
 
 

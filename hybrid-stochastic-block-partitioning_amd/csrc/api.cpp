@@ -537,8 +537,8 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
                 throw ArgError("wavefronts per vertex must be 0 (by row length), 1, 2, 4, 8 or 16");
             ctx->c.mh_group = (int)value;
         } else if (option == 6) {
-            if (value < 0 || value > 256 || (value & (value - 1)) != 0)
-                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield) or a power of two up to 256");
+            if (value < 0 || value > 512 || (value & (value - 1)) != 0)
+                throw ArgError("Metropolis-Hastings workgroups must be 0 (by window yield) or a power of two up to 512");
             ctx->c.mh_wide = (int)value;
         } else if (option == 7) {
             if (value < 0 || value > (int64_t)1 << 30) throw ArgError("table headroom must be in [0, 2^30] slots");

@@ -17,7 +17,9 @@ namespace sbpx {
 constexpr int RED_D = 4;   // per-wavefront double partials: dS, pf, pb
 constexpr int RED_I = 12;  // per-wavefront int partials
 
-template <int G, class View, bool LRN = false>
+// GU: row slots per lane per trip of the row/column pass (the proposal's row scans take 8, or 2
+// with the learned mixture).
+template <int G, class View, bool LRN = false, int GU = 2>
 __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const View& M, int v, int* tab, int cap,
                                  uint64_t seed, uint32_t call, uint32_t tag, int given_s, double given_u,
                                  bool propose_only, int* hist, double (*redd)[RED_D], int (*redi)[RED_I],
@@ -155,13 +157,22 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
             __syncthreads();
             if (sample) {
                 const int up = shift + 8;
-                for (int j = slo + gl; j < shi; j += GS) {
-                    const int kk = skeys[j];
-                    if (kk < 0) continue;
-                    const int w = svals[j];
-                    if (w == 0) continue;
-                    const int hp = up >= 31 ? 0 : (kk >> up);
-                    if (hp == prefix) atomicAdd(&hist[(kk >> shift) & 255], w);
+                // SU slots per lane per trip, their keys and counts loaded before any is used
+                constexpr int SU = LRN ? 2 : 8;  // the learned scorer leaves fewer registers
+                for (int j0 = slo + gl; j0 < shi; j0 += SU * GS) {
+                    int kk[SU], w[SU];
+#pragma unroll
+                    for (int q = 0; q < SU; ++q) {
+                        const int j = j0 + q * GS;
+                        kk[q] = j < shi ? skeys[j] : -1;
+                        w[q] = j < shi ? svals[j] : 0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < SU; ++q) {
+                        if (kk[q] < 0 || w[q] == 0) continue;
+                        const int hp = up >= 31 ? 0 : (kk[q] >> up);
+                        if (hp == prefix) atomicAdd(&hist[(kk[q] >> shift) & 255], w[q]);
+                    }
                 }
             }
             __syncthreads();
@@ -248,74 +259,93 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         const int b2 = M.cbeg(s), L2 = M.cend(s) - b2;
         const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;
         const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
-        for (int i = gl; i < T; i += GS) {
-            const int seg = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : 3;
-            int c, m;
-            if (seg == 0) {
-                c = M.rkey(b0 + i);
-                m = M.rval(b0 + i);
-            } else if (seg == 1) {
-                c = M.rkey(b1 + (i - L0));
-                m = M.rval(b1 + (i - L0));
-            } else if (seg == 2) {
-                c = M.ckey(b2 + (i - T1));
-                m = M.cval(b2 + (i - T1));
-            } else {
-                c = M.ckey(b3 + (i - T2));
-                m = M.cval(b3 + (i - T2));
-            }
-            if (View::kSlots && (c < 0 || m == 0)) continue;
-            ++live;
-            bool unchanged = true;
-            if (seg == 0) {
-                if (c == s) {
-                    m_ss = m;
-                    unchanged = !t_ss;
-                } else if (c == r) {
-                    m_sr = m;
-                    unchanged = !t_sr || d_sr == 0;
-                } else {
-                    const int sl = tab_find(tkey, cap, c);
-                    if (sl >= 0) {
-                        tmsb[sl] = m;
-                        unchanged = to[sl] == 0;
-                    }
-                }
-            } else if (seg == 1) {
-                if (c == r) {
-                    m_rr = m;
-                    unchanged = !t_rr;
-                } else if (c == s) {
-                    m_rs = m;
-                    unchanged = !t_rs || d_rs == 0;
-                } else {
-                    const int sl = tab_find(tkey, cap, c);
-                    if (sl >= 0) {
-                        tmrb[sl] = m;
-                        unchanged = to[sl] == 0;
-                    }
-                }
-            } else {
-                if (c == r || c == s) continue;
-                const int sl = tab_find(tkey, cap, c);
-                if (sl >= 0) {
-                    (seg == 2 ? tmbs : tmbr)[sl] = m;
-                    unchanged = ti[sl] == 0;
+        // GU slots per lane per trip: keys and counts, then the neighbour-block records, then the
+        // degrees of the unchanged cells' other blocks, each stage's loads issued together
+        for (int i0 = gl; i0 < T; i0 += GU * GS) {
+            int cc[GU], mm[GU], sg[GU], dsel[GU];
+#pragma unroll
+            for (int q = 0; q < GU; ++q) {
+                const int i = i0 + q * GS;
+                sg[q] = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : i < T ? 3 : -1;
+                cc[q] = -1;
+                mm[q] = 0;
+                if (sg[q] == 0) {
+                    cc[q] = M.rkey(b0 + i);
+                    mm[q] = M.rval(b0 + i);
+                } else if (sg[q] == 1) {
+                    cc[q] = M.rkey(b1 + (i - L0));
+                    mm[q] = M.rval(b1 + (i - L0));
+                } else if (sg[q] == 2) {
+                    cc[q] = M.ckey(b2 + (i - T1));
+                    mm[q] = M.cval(b2 + (i - T1));
+                } else if (sg[q] == 3) {
+                    cc[q] = M.ckey(b3 + (i - T2));
+                    mm[q] = M.cval(b3 + (i - T2));
                 }
             }
-            if (!unchanged) continue;
-            const float mf = fi(m);
-            if (seg < 2) {
-                const int d = S.din[c];
-                const int xo = seg == 0 ? dout_s : dout_r, nxo = seg == 0 ? ndout_s : ndout_r;
-                const int nd = c == r ? ndin_r : (c == s ? ndin_s : d);
-                acc += (double)cell_entropy(mf, fi(d), fi(xo));
-                acc -= (double)cell_entropy(mf, fi(nd), fi(nxo));
-            } else {
-                const int d = S.dout[c];
-                const int xi = seg == 2 ? din_s : din_r, nxi = seg == 2 ? ndin_s : ndin_r;
-                acc += (double)cell_entropy(mf, fi(xi), fi(d));
-                acc -= (double)cell_entropy(mf, fi(nxi), fi(d));
+#pragma unroll
+            for (int q = 0; q < GU; ++q) {
+                dsel[q] = -1;
+                const int seg = sg[q], c = cc[q], m = mm[q];
+                if (seg < 0 || (View::kSlots && (c < 0 || m == 0))) continue;
+                ++live;
+                bool unchanged = true;
+                if (seg == 0) {
+                    if (c == s) {
+                        m_ss = m;
+                        unchanged = !t_ss;
+                    } else if (c == r) {
+                        m_sr = m;
+                        unchanged = !t_sr || d_sr == 0;
+                    } else {
+                        const int sl = tab_find(tkey, cap, c);
+                        if (sl >= 0) {
+                            tmsb[sl] = m;
+                            unchanged = to[sl] == 0;
+                        }
+                    }
+                } else if (seg == 1) {
+                    if (c == r) {
+                        m_rr = m;
+                        unchanged = !t_rr;
+                    } else if (c == s) {
+                        m_rs = m;
+                        unchanged = !t_rs || d_rs == 0;
+                    } else {
+                        const int sl = tab_find(tkey, cap, c);
+                        if (sl >= 0) {
+                            tmrb[sl] = m;
+                            unchanged = to[sl] == 0;
+                        }
+                    }
+                } else {
+                    if (c == r || c == s) continue;
+                    const int sl = tab_find(tkey, cap, c);
+                    if (sl >= 0) {
+                        (seg == 2 ? tmbs : tmbr)[sl] = m;
+                        unchanged = ti[sl] == 0;
+                    }
+                }
+                if (unchanged) dsel[q] = c;
+            }
+            int dv[GU];
+#pragma unroll
+            for (int q = 0; q < GU; ++q) dv[q] = dsel[q] >= 0 ? (sg[q] < 2 ? S.din[dsel[q]] : S.dout[dsel[q]]) : 0;
+#pragma unroll
+            for (int q = 0; q < GU; ++q) {
+                if (dsel[q] < 0) continue;
+                const int seg = sg[q], c = dsel[q], d = dv[q];
+                const float mf = fi(mm[q]);
+                if (seg < 2) {
+                    const int xo = seg == 0 ? dout_s : dout_r, nxo = seg == 0 ? ndout_s : ndout_r;
+                    const int nd = c == r ? ndin_r : (c == s ? ndin_s : d);
+                    acc += (double)cell_entropy(mf, fi(d), fi(xo));
+                    acc -= (double)cell_entropy(mf, fi(nd), fi(nxo));
+                } else {
+                    const int xi = seg == 2 ? din_s : din_r, nxi = seg == 2 ? ndin_s : ndin_r;
+                    acc += (double)cell_entropy(mf, fi(xi), fi(d));
+                    acc -= (double)cell_entropy(mf, fi(nxi), fi(d));
+                }
             }
         }
     }

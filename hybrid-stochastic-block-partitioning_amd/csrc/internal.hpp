@@ -132,8 +132,9 @@ struct MhState {
     double mean_scan = -1.0;  // mean row/column slots scanned per evaluation in the last chunk
     int last_B = -1;          // number of blocks when mean_scan was measured
     int g_yield = 1;  // wavefronts per vertex the last chunk's window yield calls for
-    int k_yield = 8;  // workgroups of the multi-workgroup pass the last chunk's yield calls for
+    double y_est = 16.0;  // queue entries a window advanced in the last chunk
     int k_max = 1;    // the most workgroups that fit on the device at once (a power of two)
+    int k_max_l = 1;  // the same for the learned-mixture instantiation
     unsigned long long* wstamp = nullptr;  // [N] accepted-entry stamps of the multi-workgroup pass
     void* wide = nullptr;                  // its per-entry records
     int* wst = nullptr;                    // their evaluation status, dense

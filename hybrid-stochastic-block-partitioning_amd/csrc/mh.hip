@@ -11,7 +11,7 @@
 // that of evaluating one vertex at a time, because proposals and uniforms are keyed by vertex,
 // not by evaluation order. mh_kernel and mh_group_kernel re-check the later entries after every
 // commit (one workgroup, 16 entries per window); mh_wide_kernel checks every entry against the
-// accepted entries before it at once, over K workgroups (16 K entries per window, K = 1 too).
+// accepted entries before it at once, over K workgroups (8 K entries per window, K = 1 too).
 // Before a commit, every row and column it may extend is given room: a row that is too full moves
 // to a larger segment taken from the free end of the pool (the other wavefronts wait at the
 // barrier meanwhile). Only when the pool itself runs out does the kernel stop; the host then
@@ -134,8 +134,8 @@ struct MhParams {
     double* dS_sum;
     int* err;
     // the pass over several workgroups (mh_wide_kernel)
-    WideSlot* wide;                 // [16 K] one record per window entry
-    int* wst;                       // [16 K] the entries' evaluation status, dense
+    WideSlot* wide;                 // [8 K] one record per window entry
+    int* wst;                       // [8 K] the entries' evaluation status, dense
     unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
     int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
                                     // [2] next queue position, [3] stop reason
@@ -304,9 +304,12 @@ struct CommitLds {
     int wcount[MH_WAVES];
 };
 
+// NWV: wavefronts of the workgroup (at least 4).
+template <int NWV = MH_WAVES>
 __device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int* tab, int cap, int serial,
                               CommitLds& L) {
-    constexpr int NT = MH_WAVES * WAVE;
+    static_assert(NWV >= 4 && NWV <= MH_WAVES, "commit_shared needs 4 to 16 wavefronts");
+    constexpr int NT = NWV * WAVE;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     const int r = C.r, s = C.s;
     const int* to = tab + cap;
@@ -332,7 +335,7 @@ __device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int*
             if (lane == 0) L.wcount[wave] = __popcll(m);
             __syncthreads();
             int off = base, total = base;
-            for (int w = 0; w < MH_WAVES; ++w) {
+            for (int w = 0; w < NWV; ++w) {
                 if (w < wave) off += L.wcount[w];
                 total += L.wcount[w];
             }
@@ -392,7 +395,7 @@ __device__ void commit_shared(const MhParams& P, const CommitInfo& C, const int*
             P.drift[s] = C.d_ss;
         }
     } else {  // the other six updates per neighbour block b
-        constexpr int NW = (MH_WAVES - 3) * WAVE;
+        constexpr int NW = (NWV - 3) * WAVE;
         for (int it = tid - 3 * WAVE; it < 6 * n; it += NW) {
             const int k = it / n, q = it - k * n;
             const int j = slot_of(q);
@@ -962,8 +965,8 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 }
 
 // ------------------------------------------------------------------------------------------
-// The same pass over several compute units. K workgroups of 16 wavefronts evaluate a window of
-// 16 K queue entries against the current state, one entry per wavefront, and the window ends as
+// The same pass over several compute units. K workgroups of 8 wavefronts evaluate a window of
+// 8 K queue entries against the current state, one entry per wavefront, and the window ends as
 // the one-workgroup walk ends it: an entry's evaluation stands unless an accepted entry before it
 // commits a change to something it read. Every accepted entry before the first entry that fails
 // this check holds and is committed, so that entry ends the window whatever the later entries'
@@ -975,8 +978,11 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 // fences (the L2 of each XCD is written back and invalidated) and a bounded wait; a workgroup
 // whose wait runs out reports DEV_ERR_BARRIER and leaves, and so does every other one at its
 // next barrier. With one workgroup they are workgroup barriers.
-constexpr int MH_WIDE_MAX = 256;  // workgroups (at most one per compute unit: co-resident)
-constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WAVES;
+// Wavefronts per workgroup: 8, so that each has 256 registers (the evaluation, the window
+// bookkeeping and the commit stay out of scratch memory) and two workgroups share a compute unit.
+constexpr int MH_WIDE_WAVES = 8;
+constexpr int MH_WIDE_MAX = 512;  // workgroups (co-resident: at most two per compute unit)
+constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WIDE_WAVES;
 // Accepted entries a window tracks; a window with more ends before the next one (its commits
 // and checks see only these).
 constexpr int MH_ACC_MAX = 512;
@@ -1024,30 +1030,48 @@ __global__ void wide_init_kernel(int* ctrl) {
     ctrl[3] = 0;
 }
 
+// GW: wavefronts per entry (1, 2, 4 or 8; a workgroup holds 8 / GW entries, evaluated by
+// eval_move for GW = 1 and by eval_group otherwise; each group's first wavefront checks its entry).
 // LEARNED: the proposals are the learned mixture, scored per evaluation on the current state
 // (learned_scores_wave). A separate instantiation keeps the plain kernel free of the scorer.
-template <bool LEARNED>
-__global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
+template <int GW, bool LEARNED>
+__global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams P) {
+    static_assert(GW == 1 || GW == 2 || GW == 4 || GW == 8, "1, 2, 4 or 8 wavefronts per entry");
+    constexpr int NG = MH_WIDE_WAVES / GW;  // entries per workgroup
+    constexpr int LCAP = GW * MH_LDS_CAP;   // LDS table slots per entry
     __shared__ int s_ok, s_flag, s_n, s_end, s_stop, s_need;
     __shared__ CommitInfo s_commit;
-    __shared__ CommitLds s_clds;
-    __shared__ int lds_tab[MH_WAVES][7 * MH_LDS_CAP];
-    __shared__ int lds_hist[MH_WAVES][256];
+    // the wavefronts' neighbour-block tables (evaluations and checks) share their LDS with the
+    // commits' lists (workgroup 0, after the checks)
+    __shared__ union {
+        int tab[MH_WIDE_WAVES][7 * MH_LDS_CAP];
+        CommitLds clds;
+    } s_u;
+    CommitLds& s_clds = s_u.clds;
+    __shared__ int lds_hist[MH_WIDE_WAVES][256];
     __shared__ int s_acc[MH_ACC_MAX], s_acc_r[MH_ACC_MAX], s_acc_s[MH_ACC_MAX], s_acc_cap[MH_ACC_MAX];
     __shared__ double s_dS[MH_ACC_MAX];
-    __shared__ int s_wcnt[MH_WAVES], s_cut;
-    __shared__ long long s_red[MH_WAVES][4];
-    __shared__ double s_redd[MH_WAVES][2];
-    // the learned mixture's candidates, their counts and scores, per wavefront
-    __shared__ int s_lc[LEARNED ? MH_WAVES : 1][MLP_CAND], s_lcnt[LEARNED ? MH_WAVES : 1][MLP_CAND];
-    __shared__ float s_lz[LEARNED ? MH_WAVES : 1][MLP_OUT];
+    __shared__ int s_wcnt[MH_WIDE_WAVES], s_cut;
+    __shared__ long long s_red[MH_WIDE_WAVES][4];
+    __shared__ double s_redd[MH_WIDE_WAVES][2];
+    // the learned mixture's candidates, their counts and scores, per entry; with GW > 1 also the
+    // branch the group's first wavefront drew
+    __shared__ int s_lc[LEARNED ? NG : 1][MLP_CAND], s_lcnt[LEARNED ? NG : 1][MLP_CAND];
+    __shared__ float s_lz[LEARNED ? NG : 1][MLP_OUT];
+    __shared__ int s_lon[LEARNED ? NG : 1], s_lgs[LEARNED ? NG : 1];
+    __shared__ double s_lgu[LEARNED ? NG : 1];
+    // eval_group's per-wavefront partial sums
+    __shared__ double redd[GW > 1 ? MH_WIDE_WAVES : 1][RED_D];
+    __shared__ int redi[GW > 1 ? MH_WIDE_WAVES : 1][RED_I];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
-    const int slot = (int)blockIdx.x * MH_WAVES + wave;
-    const int WS = (int)gridDim.x * MH_WAVES;  // entries per window
+    const int g = wave / GW, wig = wave % GW;
+    const bool leader = wig == 0;  // the entry's first wavefront records and checks it
+    const int slot = (int)blockIdx.x * NG + g;
+    const int WS = (int)gridDim.x * NG;  // entries per window
     int* const gtab = P.tables + (size_t)slot * 7 * P.tcap;
     const GraphView G = P.G;
     const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
-    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[wave]};
+    const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[g]};
     const bool lead = blockIdx.x == 0;
     long long att = 0, evs = 0, mvs = 0, bytes_sum = 0;
     double dsum = 0.0, scan_sum = 0.0, scan_count = 0.0, windows = 0.0;
@@ -1072,37 +1096,70 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
         ++wserial;
         windows += 1.0;
         const int Wn = min(WS, P.n - pos);
-        // 1. evaluate this wavefront's entry
+        // 1. evaluate this group's entry
         const bool active = slot < Wn;
         MoveResult R{};
         int cap = 16;
         int* tab = gtab;
+        const int v = active ? P.list[pos + slot] : -1;
         if (active) {
-            const int v = P.list[pos + slot];
             cap = vertex_table_cap(G, v);
-            if (cap <= MH_LDS_CAP) tab = lds_tab[wave];
-            if constexpr (LEARNED) {  // the learned mixture, scored on the current state
-                const int deg = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
-                const bool on = deg <= P.net.maxdeg;
-                int gs = -1;
-                double gu = 0.0;
-                if (on) {
-                    const int r = S.assign[v];
-                    const bool has =
-                        learned_scores_wave(G, S, v, r, tab, cap, P.net, s_lc[wave], s_lcnt[wave], s_lz[wave]);
-                    learned_branch(P.seed, P.call, P.tag, v, r, s_lz[wave], s_lc[wave], has, &gs, &gu);
+            if (cap <= LCAP) tab = s_u.tab[g * GW];
+        }
+        if constexpr (GW == 1) {
+            if (active) {
+                if constexpr (LEARNED) {  // the learned mixture, scored on the current state
+                    const int deg = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
+                    const bool on = deg <= P.net.maxdeg;
+                    int gs = -1;
+                    double gu = 0.0;
+                    if (on) {
+                        const int r = S.assign[v];
+                        const bool has =
+                            learned_scores_wave(G, S, v, r, tab, cap, P.net, s_lc[g], s_lcnt[g], s_lz[g]);
+                        learned_branch(P.seed, P.call, P.tag, v, r, s_lz[g], s_lc[g], has, &gs, &gu);
+                    }
+                    R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, false,
+                                                       on ? (double)LEARN_W_PLAIN : -1.0, 0.0, 0.0,
+                                                       on ? s_lz[g] : nullptr, on ? s_lc[g] : nullptr);
+                } else {
+                    R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
                 }
-                R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, false,
-                                                   on ? (double)LEARN_W_PLAIN : -1.0, 0.0, 0.0,
-                                                   on ? s_lz[wave] : nullptr, on ? s_lc[wave] : nullptr);
-            } else {
-                R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
             }
-            if (R.status == 2 && tab != gtab) {  // the commit and the other checks read it from memory
-                for (int j = lane; j < 7 * cap; j += WAVE) gtab[j] = tab[j];
+        } else {
+            // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
+            if constexpr (LEARNED) {  // the group's first wavefront scores the vertex on the current state
+                if (leader) {
+                    int on = 0, gs = -1;
+                    double gu = 0.0;
+                    if (active && (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]) <= P.net.maxdeg) {
+                        on = 1;
+                        const int r = S.assign[v];
+                        const bool has =
+                            learned_scores_wave(G, S, v, r, tab, cap, P.net, s_lc[g], s_lcnt[g], s_lz[g]);
+                        learned_branch(P.seed, P.call, P.tag, v, r, s_lz[g], s_lc[g], has, &gs, &gu);
+                    }
+                    if (lane == 0) {
+                        s_lon[g] = on;
+                        s_lgs[g] = gs;
+                        s_lgu[g] = gu;
+                    }
+                }
+                __syncthreads();
+                const bool on = s_lon[g] != 0;
+                R = eval_group<GW, HashView, true>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
+                                                   on ? s_lgu[g] : 0.0, false, lds_hist[g], redd, redi,
+                                                   on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
+                                                   on ? s_lc[g] : nullptr);
+            } else {
+                R = eval_group<GW, HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g],
+                                             redd, redi);
             }
         }
-        if (lane == 0) {
+        if (active && R.status == 2 && tab != gtab) {  // the commit and the other checks read it from memory
+            for (int j = wig * WAVE + lane; j < 7 * cap; j += GW * WAVE) gtab[j] = tab[j];
+        }
+        if (leader && lane == 0) {
             WideSlot& o = P.wide[slot];
             P.wst[slot] = active ? R.status : 3;
             o.st = active ? R.status : 3;
@@ -1123,7 +1180,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
         // the dense status array, then fills its part of the list (the first MH_ACC_MAX; the
         // window ends before the next accepted entry)
         {
-            const int per = ((Wn + MH_WAVES * WAVE - 1) / (MH_WAVES * WAVE)) * WAVE;
+            const int per = ((Wn + MH_WIDE_WAVES * WAVE - 1) / (MH_WIDE_WAVES * WAVE)) * WAVE;
             const int lo = wave * per, hi = min(Wn, lo + per);
             int cnt = 0;
             for (int j = lo + lane; j < hi; j += WAVE) cnt += P.wst[j] == 2 ? 1 : 0;
@@ -1132,7 +1189,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             if (threadIdx.x == 0) s_cut = INT32_MAX;
             __syncthreads();
             int base = 0, total = 0;
-            for (int w = 0; w < MH_WAVES; ++w) {
+            for (int w = 0; w < MH_WIDE_WAVES; ++w) {
                 base += w < wave ? s_wcnt[w] : 0;
                 total += s_wcnt[w];
             }
@@ -1159,7 +1216,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
         __syncthreads();
         if (threadIdx.x == 0 && s_cut < Wn) atomicMin(&P.ctrl[1], s_cut);
         // 3. whether this entry's evaluation holds after the accepted entries before it
-        if (active) {
+        if (active && leader) {
             int nb = 0, hi = s_n;  // accepted entries before this one
             while (nb < hi) {
                 const int mid = (nb + hi) >> 1;
@@ -1233,7 +1290,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
                     __syncthreads();
                     break;
                 }
-                commit_shared(P, s_commit, ctab, s_acc_cap[k], serial + 1, s_clds);
+                commit_shared<MH_WIDE_WAVES>(P, s_commit, ctab, s_acc_cap[k], serial + 1, s_clds);
                 __syncthreads();
                 ++serial;
                 if (threadIdx.x == 0) {
@@ -1245,7 +1302,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             const int end = s_end;
             long long a = 0, e = 0, by = 0;
             double sc = 0.0, scn = 0.0;
-            for (int i = threadIdx.x; i < end; i += MH_WAVES * WAVE) {
+            for (int i = threadIdx.x; i < end; i += MH_WIDE_WAVES * WAVE) {
                 const WideSlot& o = P.wide[i];
                 if (o.st == 3) continue;
                 a += 1;
@@ -1270,7 +1327,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_wide_kernel(MhParams P) {
             }
             __syncthreads();
             if (threadIdx.x == 0) {
-                for (int w = 0; w < MH_WAVES; ++w) {
+                for (int w = 0; w < MH_WIDE_WAVES; ++w) {
                     att += s_red[w][0];
                     evs += s_red[w][1];
                     bytes_sum += s_red[w][2];
@@ -1696,6 +1753,20 @@ void merge_heap_select(Ctx& c, int num_to_merge, const double* dS, const int32_t
     SBPX_HIP(hipFree(d_parent));
 }
 
+// The window-check kernel for GW wavefronts per entry, plain or learned.
+template <int GW>
+static const void* wide_kernel_gw(bool learned) {
+    return learned ? (const void*)mh_wide_kernel<GW, true> : (const void*)mh_wide_kernel<GW, false>;
+}
+static const void* wide_kernel(int gw, bool learned) {
+    switch (gw) {
+        case 1: return wide_kernel_gw<1>(learned);
+        case 2: return wide_kernel_gw<2>(learned);
+        case 4: return wide_kernel_gw<4>(learned);
+        default: return wide_kernel_gw<8>(learned);
+    }
+}
+
 // One chunk of the sequential pass on one workgroup with G wavefronts per vertex.
 template <int G>
 static void launch_group(const MhParams& P, bool learned, hipStream_t st) {
@@ -1744,19 +1815,26 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         SBPX_HIP(hipMalloc((void**)&h.wst, sizeof(int) * MH_WIDE_SLOTS));
         // the most workgroups of the window kernels that fit on the device at once (a
         // cooperative launch needs them all resident)
-        int per_cu = 1, per_cu_l = 1, cus = 1;
-        SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mh_wide_kernel<false>, MH_WAVES * WAVE, 0));
-        SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_l, mh_wide_kernel<true>, MH_WAVES * WAVE, 0));
+        int per_cu = 1 << 20, per_cu_l = 1 << 20, cus = 1;
+        for (int gw = 1; gw <= 8; gw *= 2) {
+            int p = 1, pl = 1;
+            SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, wide_kernel(gw, false), MH_WIDE_WAVES * WAVE, 0));
+            SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pl, wide_kernel(gw, true), MH_WIDE_WAVES * WAVE, 0));
+            per_cu = std::min(per_cu, std::max(p, 1));
+            per_cu_l = std::min(per_cu_l, std::max(pl, 1));
+        }
         SBPX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device));
         h.k_max = 1;
-        while (2 * h.k_max <= std::min(MH_WIDE_MAX, std::min(per_cu, per_cu_l) * cus)) h.k_max *= 2;
+        while (2 * h.k_max <= std::min(MH_WIDE_MAX, per_cu * cus)) h.k_max *= 2;
+        h.k_max_l = 1;
+        while (2 * h.k_max_l <= std::min(MH_WIDE_MAX, per_cu_l * cus)) h.k_max_l *= 2;
         SBPX_HIP(hipMalloc((void**)&h.ctrl, sizeof(int) * 4));
         SBPX_HIP(hipMalloc((void**)&h.wstamp, sizeof(unsigned long long) * std::max(c.g.N, 1)));
         SBPX_HIP(hipMemsetAsync(h.wstamp, 0, sizeof(unsigned long long) * std::max(c.g.N, 1), c.stream));
         SBPX_HIP(hipMemsetAsync(h.status + 9, 0, sizeof(int), c.stream));
     }
     // one table per wavefront of the widest window (or of one workgroup)
-    const size_t tslots = (size_t)(wide ? h.k_max * MH_WAVES : MH_WAVES) * 7 * tcap;
+    const size_t tslots = (size_t)(wide ? std::max(h.k_max * MH_WIDE_WAVES, MH_WAVES) : MH_WAVES) * 7 * tcap;
     if (tslots > h.table_slots) {
         if (h.tables) SBPX_HIP(hipFree(h.tables));
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
@@ -1830,55 +1908,64 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.net = net;
         P.phase = trace ? c.red + 3000 : nullptr;
         bool out_of_room = false;
-        // The queue runs in chunks; before each, the wavefronts per vertex are chosen from the
-        // previous chunk: the mean number of row/column slots its evaluations scanned (long scans
-        // gain from more lanes per vertex, short ones from more vertices in flight) and the
-        // queue entries a window of speculative evaluations advanced before a commit
-        // invalidated the rest (when few stand, fewer vertices per window at more wavefronts
-        // each finish a window sooner). The first chunk at a new number of blocks goes by the
-        // longest row instead.
+        // The queue runs in chunks; before each, the wavefronts per entry (G) and the workgroups
+        // (K) are chosen from the previous chunk: G by the mean number of row/column slots its
+        // evaluations scanned (long scans gain from more lanes per vertex, short ones from more
+        // vertices in flight), K so that a window holds about 1.5 times the queue entries the
+        // previous chunk's windows advanced before a commit invalidated the rest. Windows of about
+        // one entry with long scans take one workgroup of 16 wavefronts. The first chunk at a new
+        // number of blocks goes by the longest row. With option 6 = 1 the one-workgroup kernels
+        // run instead (16 / G entries per window).
         while (pos < n) {
             int G = c.mh_group;
+            const int k_cap = learned ? h.k_max_l : h.k_max;
+            const bool fresh = h.last_B != B || h.mean_scan < 0.0;
+            if (fresh) h.y_est = std::min(h.y_est, 16.0);
             if (G == 0) {
-                if (h.last_B != B || h.mean_scan < 0.0) {
+                if (fresh) {
                     G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : 1);
+                } else if (wide) {
+                    G = h.mean_scan >= 2400.0 ? 8 : (h.mean_scan >= 1200.0 ? 4 : (h.mean_scan >= 600.0 ? 2 : 1));
+                    if (G == 8 && h.y_est < 1.5) G = 16;
                 } else {
                     G = h.mean_scan >= 4000.0 ? 16 : (h.mean_scan >= 1500.0 ? 4 : 1);
                     G = std::max(G, h.g_yield);
                 }
             }
-            // workgroups: with one wavefront per vertex, as many as the window yield calls for
+            const bool wk = wide && G <= 8;  // the window-check kernel
             int K = 1;
-            if (G == 1 && wide) K = std::min(c.mh_wide ? c.mh_wide : h.k_yield, h.k_max);
-            // one workgroup, one wavefront per vertex: the window checks of the wide kernel
-            const bool wide1 = K == 1 && G == 1 && wide && c.mh_wide == 0;
-            // the learned mixture runs in every chunk: the window-check kernel for one wavefront
-            // per vertex, the group kernels for wavefront groups
-            const bool lchunk = learned;
+            if (wk) {
+                if (c.mh_wide) {
+                    K = std::min(c.mh_wide, k_cap);
+                } else {
+                    const double want = 1.5 * h.y_est * G / MH_WIDE_WAVES;
+                    while (K < k_cap && K < want) K *= 2;
+                }
+            }
             P.start = pos;
             // a launch covers at least MH_CHUNK entries and four full windows
-            P.stop_at = std::min(n, pos + std::max(MH_CHUNK, 4 * MH_WAVES * K));
+            P.stop_at = std::min(n, pos + std::max(MH_CHUNK, 4 * (MH_WIDE_WAVES / std::min(G, 8)) * K));
             SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));
-            if (K > 1 || wide1) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
+            if (wk) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
             if (timed) c.timer_begin();
             if (trace) SBPX_HIP(hipEventRecord(tr_e0, c.stream));
-            if (K > 1) {
+            if (wk) {
                 void* args[] = {&P};
-                const void* kern = lchunk ? (const void*)mh_wide_kernel<true> : (const void*)mh_wide_kernel<false>;
-                SBPX_HIP(hipLaunchCooperativeKernel(kern, dim3(K), dim3(MH_WAVES * WAVE), args, 0, c.stream));
-            } else if (wide1) {
-                if (lchunk)
-                    mh_wide_kernel<true><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
+                const void* kern = wide_kernel(G, learned);
+                if (K > 1)  // every workgroup resident at once: the grid barriers need them all
+                    SBPX_HIP(hipLaunchCooperativeKernel(kern, dim3(K), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream));
                 else
-                    mh_wide_kernel<false><<<1, MH_WAVES * WAVE, 0, c.stream>>>(P);
+                    SBPX_HIP(hipLaunchKernel(kern, dim3(1), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream));
             } else {
-                if (lchunk && G == 1) throw StateError("learned sequential pass without the window-check kernel");
+                // the learned mixture runs in every chunk: the window-check kernel, or the group
+                // kernels for wavefront groups
+                if (learned && G == 1) throw StateError("learned sequential pass without the window-check kernel");
                 switch (G) {
                     case 1: mh_kernel<<<1, MH_WAVES * WAVE, 0, c.stream>>>(P); break;
-                    case 2: launch_group<2>(P, lchunk, c.stream); break;
-                    case 4: launch_group<4>(P, lchunk, c.stream); break;
-                    case 8: launch_group<8>(P, lchunk, c.stream); break;
-                    default: launch_group<16>(P, lchunk, c.stream); break;
+                    case 2: launch_group<2>(P, learned, c.stream); break;
+                    case 4: launch_group<4>(P, learned, c.stream); break;
+                    case 8: launch_group<8>(P, learned, c.stream); break;
+                    default: launch_group<16>(P, learned, c.stream); break;
                 }
             }
             SBPX_HIP(hipGetLastError());
@@ -1907,22 +1994,19 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                 h.mean_scan = sc[0] / sc[1];
                 h.last_B = B;
             }
-            // Entries a window advanced, at 16 / G vertices per window. Near that cap the windows
-            // could be longer: halve G; below it the yield is the measured one: about 3 or fewer
-            // standing entries take 4 wavefronts per vertex, up to 6 two, more one.
-            // With K workgroups a window holds 16 K entries: near that, K doubles (up to 32);
-            // below 3/8 of it, K halves, down to the one-workgroup kernels.
+            // Entries a window advanced. For the one-workgroup kernels (16 / G vertices per
+            // window): near that cap the windows could be longer, so G halves; below it about 3 or
+            // fewer standing entries take 4 wavefronts per vertex, up to 6 two, more one.
             if (sc[2] > 0.0 && advanced > 0) {
                 const double y = (double)advanced / sc[2];
-                if (K > 1) {
-                    const double full = (double)(MH_WAVES * K);
-                    if (y >= 0.75 * full) h.k_yield = std::min(2 * K, h.k_max);
-                    else if (y < 0.375 * full) h.k_yield = K / 2;
-                    h.g_yield = 1;
-                } else {
+                // windows that ran full say only that longer ones may stand: ask for twice as many
+                const double full = wk ? (double)(K * (MH_WIDE_WAVES / G)) : (double)(MH_WAVES / G);
+                h.y_est = y >= 0.75 * full ? 2.0 * full : y;
+                if (!wk) {
                     if (y >= 0.75 * (double)(MH_WAVES / G)) h.g_yield = std::max(1, G / 2);
                     else h.g_yield = y < 3.0 ? 4 : (y < 6.0 ? 2 : 1);
-                    if (G == 1 && y >= 0.75 * (double)MH_WAVES) h.k_yield = 2;
+                } else {
+                    h.g_yield = 1;
                 }
             }
             if (stat[9] > (1 << 30)) {  // restart the window serials well before they wrap

@@ -79,7 +79,7 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            0 always rebuilt. Both give the same arrays.
  * option 6 = workgroups of the sequential Metropolis-Hastings pass when one wavefront evaluates
  *            each vertex: 1 (one workgroup, 16 speculative evaluations per window), a power of two
- *            up to 256 (16 per workgroup in one cooperative launch; capped at the workgroups the
+ *            up to 512 (8 per workgroup in one cooperative launch; capped at the workgroups the
  *            device holds at once), or 0 to follow the window yield (default). Every choice
  *            applies the same moves in the same order.
  * option 7 = spare slots of the Metropolis-Hastings / heap-merge table pool when it is built:

@@ -190,7 +190,7 @@ def test_training_is_reproducible():
     assert np.array_equal(params[0][1], params[1][1])
 
 
-@pytest.mark.parametrize("algo,group,maxdeg", [("hybrid", 0, 126), ("mh", 0, 126), ("mh", 4, 126),
+@pytest.mark.parametrize("algo,group,maxdeg", [("hybrid", 0, 126), ("mh", 0, 126), ("mh", 4, 126), ("mh", 8, 126),
                                               ("hybrid", 16, 126), ("hybrid", 0, 510), ("mh", 0, 510)])
 def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, maxdeg):
     """Proposal 2: the sequential pass proposes from the mixture too, each evaluation scoring its
@@ -225,9 +225,9 @@ def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, ma
         c.sweep(code, 1, 31, 9)
 
 
-@pytest.mark.parametrize("option,values", [("group", (1, 4, 16)), ("wide", (0, 2, 8))])
+@pytest.mark.parametrize("option,values", [("group", (1, 2, 4, 8, 16)), ("wide", (0, 2, 8, 512))])
 def test_learned_sequential_pass_same_moves_however_evaluated(option, values):
-    """The learned sequential pass gives the same chain whether one wavefront or a group of 4 or 16
+    """The learned sequential pass gives the same chain whether one wavefront or a group of 2 to 16
     evaluates each vertex, and whatever number of workgroups shares a window of speculative
     evaluations: the proposals are keyed by vertex and pass, the scores come from the state the
     vertex's step starts from (the window checks cover what the scorer reads), and delta MDL is equal

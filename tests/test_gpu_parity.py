@@ -114,7 +114,7 @@ def test_options_reject_values_outside_their_range(toy):
     c = ctx_for(toy_arrays(toy))
     m = sbpx()
     for option, bad in ((m.OPT_MH_GROUP, 3), (m.OPT_EVAL_VARIANT, 4), (m.OPT_PROPOSAL, 3), (m.OPT_INCREMENTAL, 2),
-                        (m.OPT_MH_WIDE, 3), (m.OPT_MH_WIDE, 512), (m.OPT_MH_HEADROOM, -1),
+                        (m.OPT_MH_WIDE, 3), (m.OPT_MH_WIDE, 1024), (m.OPT_MH_HEADROOM, -1),
                         (m.OPT_LEARN_MAXDEG, -1), (m.OPT_LEARN_MAXDEG, 511), (99, 0)):
         with pytest.raises(m.SbpxError):
             c.set_option(option, bad)
@@ -345,15 +345,18 @@ def test_sbp_async_gibbs_end_to_end_matches_oracle():
 
 
 # ----------------------------------------------------------------------------- sequential MH
-@pytest.mark.parametrize("algo,group,wide", [("hybrid", 0, 0), ("mh", 0, 0), ("mh", 1, 1), ("mh", 2, 0), ("hybrid", 4, 0),
-                                             ("mh", 8, 0), ("mh", 16, 0), ("mh", 1, 2), ("mh", 1, 4), ("mh", 1, 16),
-                                             ("mh", 1, 32), ("hybrid", 1, 8), ("mh", 1, 64), ("mh", 1, 256),
-                                             ("hybrid", 1, 128)])
+@pytest.mark.parametrize("algo,group,wide", [("hybrid", 0, 0), ("mh", 0, 0), ("mh", 1, 1), ("mh", 2, 1), ("hybrid", 4, 1),
+                                             ("mh", 8, 1), ("mh", 16, 0), ("mh", 2, 0), ("hybrid", 4, 0), ("mh", 8, 0),
+                                             ("mh", 1, 2), ("mh", 1, 4), ("mh", 1, 16), ("mh", 2, 64), ("mh", 4, 32),
+                                             ("hybrid", 8, 128), ("mh", 8, 512), ("mh", 1, 32), ("hybrid", 1, 8),
+                                             ("mh", 1, 64), ("mh", 1, 256), ("hybrid", 1, 128), ("mh", 1, 512)])
 def test_sequential_sweeps_match_oracle(algo, group, wide):
     """Sequential Metropolis-Hastings (alone, or on the high-degree vertices of the hybrid) equals
     the oracle's one-vertex-at-a-time pass for every number of wavefronts per vertex and every
-    number of workgroups sharing a window (wide > 1: 16 * wide speculative evaluations, up to a
-    window of 4096 that covers the whole queue here)."""
+    number of workgroups sharing a window: wide = 1 runs the one-workgroup kernels (16 / group
+    entries per window); otherwise group <= 8 wavefronts per entry run the window-check kernel
+    on wide workgroups of 8 / group entries each (wide = 0: as many as the window yield calls
+    for; up to 4096 entries, which covers the whole queue here), and group = 16 one workgroup."""
     sequential_case(algo, group, wide)
 
 

@@ -997,15 +997,19 @@ struct WideSlot {
     long long bytes, nnz_rs;
 };
 
-// The n-th barrier of the launch (n >= 1) completes when ctrl[0] reaches n * gridDim.x.
-__device__ bool grid_barrier(int* ctrl, int n, int* s_ok) {
+// The n-th barrier of the launch (n >= 1) completes when ctrl[0] reaches n * gridDim.x. rel: this
+// workgroup's writes since its last barrier are read by other workgroups after this one (a
+// device-scope release: the XCD's L2 written back); acq: it reads what other workgroups wrote
+// before this barrier (a device-scope acquire: the L2 invalidated). A workgroup skips what it does
+// not need, since every invalidation makes the next window fetch its rows again.
+__device__ bool grid_barrier(int* ctrl, int n, int* s_ok, bool rel, bool acq) {
     if (gridDim.x == 1) {  // one workgroup: a workgroup barrier is enough
         __syncthreads();
         return true;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
+        if (rel) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         atomicAdd(&ctrl[0], 1);
         const int target = n * (int)gridDim.x;
         int ok = 1;
@@ -1016,7 +1020,7 @@ __device__ bool grid_barrier(int* ctrl, int n, int* s_ok) {
             }
             __builtin_amdgcn_s_sleep(8);
         }
-        __threadfence();
+        if (acq) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_ok = ok;
     }
     __syncthreads();
@@ -1174,7 +1178,8 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             o.nnz_rs = R.nnz_rs;
             if (active && R.status == 2) P.wstamp[R.v] = ((unsigned long long)wserial << 32) | (unsigned)slot;
         }
-        if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        // every entry's record and table, read by every workgroup's list and checks
+        if (!grid_barrier(P.ctrl, ++nbar, &s_ok, true, true)) break;
         tick(0);
         // 2. the accepted entries of the window, in queue order: each wavefront counts a slice of
         // the dense status array, then fills its part of the list (the first MH_ACC_MAX; the
@@ -1251,7 +1256,8 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                 atomicMin(&P.ctrl[1], slot);
             }
         }
-        if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        // the checks' outcomes, read by workgroup 0 only
+        if (!grid_barrier(P.ctrl, ++nbar, &s_ok, !lead, lead)) break;
         tick(1);
         // 4. workgroup 0 commits the accepted entries before the first that does not hold
         if (lead) {
@@ -1340,7 +1346,8 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             }
         }
         tick(2);
-        if (!grid_barrier(P.ctrl, ++nbar, &s_ok)) break;
+        // workgroup 0's commits, read by every workgroup's next evaluations
+        if (!grid_barrier(P.ctrl, ++nbar, &s_ok, lead, !lead)) break;
         tick(3);
         pos = __hip_atomic_load(&P.ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         stopped = __hip_atomic_load(&P.ctrl[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

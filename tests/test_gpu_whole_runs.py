@@ -45,3 +45,38 @@ def test_five_seeds_within_half_percent_of_oracle(algorithm, code):
     r = np.array([row[3:] for row in rows])
     assert r[:, 0].mean() >= r[:, 1].mean() - 0.01, r
     assert r[:, 2].mean() >= r[:, 3].mean() - 0.01, r
+
+
+def test_c2_whole_runs_against_oracle_fixtures():
+    """Whole hybrid runs (batches 1, --approximate) on the 50,000-vertex, 44-block DC-SBM shape
+    (Dirichlet(1) block sizes, p_in 0.6, out-degrees on [10, 100]) for the seeds the oracle ran in
+    the build container (tests/golden/whole_runs_C2.json, written by make_whole_run_fixtures.py):
+    each device run's final MDL within 0.5% of the oracle's, and the mean NMI and modularity
+    against the planted partition over the seeds no worse than the oracle's (up to 0.01)."""
+    import hashlib
+    import json
+
+    m = sbpx()
+    ev = evaluation()
+    with open(os.path.join(ROOT, "tests", "golden", "whole_runs_C2.json")) as f:
+        doc = json.load(f)
+    assert doc["algorithm"] == "hybrid_mcmc" and doc["batches"] == 1 and doc["approximate"]
+    rows, identical = [], 0
+    for seed_s, want in sorted(doc["runs"].items(), key=lambda kv: int(kv[0])):
+        seed = int(seed_s)
+        op, oi, ip, ii, truth = m.generate_dcsbm(**m.CONFIGS["C2"], seed=seed)
+        assert int(op[-1]) == want["num_edges"]
+        a, st = m.run_sbp(op, oi, ip, ii, algorithm="hybrid_mcmc", batches=1, approximate=True, seed=seed)
+        same = hashlib.sha256(np.ascontiguousarray(a, np.int32).tobytes()).hexdigest() == want["assignment_sha256"]
+        identical += same
+        rows.append((st.mdl, want["mdl"], ev.nmi(truth, a), want["nmi"], ev.modularity(op, oi, ip, a),
+                     want["modularity"]))
+        print("C2 seed", seed, "device B", st.num_blocks, "mdl", st.mdl, "oracle B", want["num_blocks"], "mdl",
+              want["mdl"], "identical assignment:", same, flush=True)
+    assert len(rows) >= 5
+    for mdl, omdl, *_ in rows:
+        assert abs(mdl / omdl - 1.0) < 5e-3, (mdl, omdl)
+    r = np.array([row[2:] for row in rows])
+    assert r[:, 0].mean() >= r[:, 1].mean() - 0.01, r
+    assert r[:, 2].mean() >= r[:, 3].mean() - 0.01, r
+    print("identical to the oracle's assignment on", identical, "of", len(rows), "seeds")

@@ -135,6 +135,7 @@ struct MhState {
     double y_est = 16.0;  // queue entries a window advanced in the last chunk
     int k_max = 1;    // the most workgroups that fit on the device at once (a power of two)
     int k_max_l = 1;  // the same for the learned-mixture instantiation
+    int coop_refused = 0;  // cooperative launches the runtime refused (then k_max = 1)
     unsigned long long* wstamp = nullptr;  // [N] accepted-entry stamps of the multi-workgroup pass
     void* wide = nullptr;                  // its per-entry records
     int* wst = nullptr;                    // their evaluation status, dense

@@ -1930,9 +1930,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (fresh) h.y_est = std::min(h.y_est, 16.0);
             if (G == 0) {
                 if (fresh) {
-                    G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : 1);
+                    G = h.max_cap >= 16384 ? 16 : (h.max_cap >= 2048 ? 4 : (wide ? 2 : 1));
                 } else if (wide) {
-                    G = h.mean_scan >= 2400.0 ? 8 : (h.mean_scan >= 1200.0 ? 4 : (h.mean_scan >= 600.0 ? 2 : 1));
+                    G = h.mean_scan >= 2400.0 ? 8 : (h.mean_scan >= 1200.0 ? 4 : 2);
                     if (G == 8 && h.y_est < 1.5) G = 16;
                 } else {
                     G = h.mean_scan >= 4000.0 ? 16 : (h.mean_scan >= 1500.0 ? 4 : 1);
@@ -1959,10 +1959,20 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (wk) {
                 void* args[] = {&P};
                 const void* kern = wide_kernel(G, learned);
-                if (K > 1)  // every workgroup resident at once: the grid barriers need them all
-                    SBPX_HIP(hipLaunchCooperativeKernel(kern, dim3(K), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream));
-                else
-                    SBPX_HIP(hipLaunchKernel(kern, dim3(1), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream));
+                if (K > 1) {  // every workgroup resident at once: the grid barriers need them all
+                    const hipError_t e =
+                        hipLaunchCooperativeKernel(kern, dim3(K), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream);
+                    if (e != hipSuccess) {
+                        // refused at launch (nothing ran), e.g. while another process on the device
+                        // holds its cooperative-launch resources: this context goes on with one
+                        // workgroup per window, which applies the same moves
+                        (void)hipGetLastError();
+                        h.k_max = h.k_max_l = 1;
+                        K = 1;
+                        h.coop_refused++;
+                    }
+                }
+                if (K == 1) SBPX_HIP(hipLaunchKernel(kern, dim3(1), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream));
             } else {
                 // the learned mixture runs in every chunk: the window-check kernel, or the group
                 // kernels for wavefront groups

@@ -50,8 +50,8 @@ def load_pkg():
 KERNELS = ["eval_bin_kernel<64..256> (batched vertex-move evaluation, degree <= 126, LDS tables)",
            "eval_bin_kernel<512..1024> (batched vertex-move evaluation, degree 127-510, LDS tables)",
            "eval_group_kernel (batched vertex-move evaluation, degree > 510, 4-16 wavefronts per vertex)",
-           "mh_kernels (sequential Metropolis-Hastings evaluation and commit: mh_wide_kernel on up to 512 workgroups, "
-           "mh_kernel and mh_group_kernel<G> on one)"]
+           "mh_kernels (sequential Metropolis-Hastings evaluation and commit: mh_wide_kernel<G> with G = 1-8 "
+           "wavefronts per entry on up to 512 workgroups, mh_kernel and mh_group_kernel<G> on one)"]
 
 
 def train_learned(m, cfg, seeds, log_sweeps=8, steps=600):

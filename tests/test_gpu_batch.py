@@ -1,7 +1,8 @@
 """GPU tests of many graphs partitioned side by side in one context (the batched form): every
 batched device step leaves each listed graph exactly as the same step leaves it in a context of
 its own (assignment, block count and log posterior bit for bit) and the graphs it does not list
-untouched; whole batched runs end exactly as the runs of the graphs alone with the same seeds."""
+untouched; whole batched runs end exactly as the runs of the graphs alone with the same seeds, and
+a sample of graphs of the 512-graph workload's shape ends as the oracle's whole runs do."""
 import numpy as np
 import pytest
 
@@ -130,3 +131,27 @@ def test_batched_runs_end_as_runs_alone():
         assert (assignment == a).all(), g
         assert mdl == st.mdl, g
         rg.close()
+
+
+def test_batched_c3_sample_against_oracle_whole_runs():
+    """Graphs of the 512-graph workload's shape (the first eight of its draw with N <= 2500)
+    partitioned side by side in one context, against the oracle's whole runs of the same graphs
+    with the same seeds (async_gibbs, batches 1, approximate merges; sbp.cpp:61-98): each graph's
+    final MDL within 0.5% of the oracle's, and most assignments identical to the oracle's."""
+    import oracle_lib as O
+
+    m = sbpx()
+    params = [p for p in m.c3_graph_params(96, 0) if p["num_vertices"] <= 2500][:8]
+    assert len(params) == 8
+    graphs = [m.generate_dcsbm(**p)[:4] for p in params]
+    seeds = [7919 + g for g in range(len(graphs))]
+    runs, tot = m.partition_batch(graphs, batches=1, seeds=seeds)
+    identical = 0
+    for (op, oi, ip, ii), seed, (a, B, mdl, _) in zip(graphs, seeds, runs):
+        n = len(op) - 1
+        oa, ost = O.sbp(O.Graph(n, op, oi, ip, ii), 1, 1, True, seed)
+        print("N", n, "batched B", B, "mdl", mdl, "oracle B", ost.B, "mdl", ost.mdl, flush=True)
+        assert abs(mdl / ost.mdl - 1.0) < 5e-3, (n, mdl, ost.mdl)
+        identical += int(B == ost.B and (a == oa).all())
+    print("identical to the oracle on", identical, "of", len(graphs), "graphs")
+    assert identical >= len(graphs) // 2

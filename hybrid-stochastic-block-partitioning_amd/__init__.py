@@ -17,6 +17,7 @@ LEARNED_PARAMS = 90656  # W1[256][64], b1[256], W2[256][256], b2[256], W3[32][25
 OPT_COUNT_BYTES, OPT_MH_GROUP, OPT_EVAL_VARIANT, OPT_PROPOSAL, OPT_INCREMENTAL, OPT_MH_WIDE = 1, 2, 3, 4, 5, 6
 OPT_MH_HEADROOM = 7
 OPT_LEARN_MAXDEG = 8
+OPT_BATCH_GRAPH = 9
 ALGORITHMS = {"metropolis_hastings": ALGO_MH, "async_gibbs": ALGO_ASYNC_GIBBS, "hybrid_mcmc": ALGO_HYBRID}
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -121,6 +122,7 @@ def load():
     L.sbpx_batch_sweep.argtypes = [vp, C.c_int, C.c_int32, C.c_int32, i32p,
                                    np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
                                    np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"), C.POINTER(SweepStats)]
+    L.sbpx_batch_replays.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.sbpx_batch_merge.argtypes = [vp, C.c_int32, i32p, i32p, C.c_int,
                                    np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"),
                                    np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")]
@@ -163,7 +165,7 @@ def load_host():
                                 C.POINTER(RunStats), C.c_char_p, C.c_int]
     H.sbp_partition_batch.argtypes = [C.c_int32, i32p, i32p, i32p, i32p, i32p, C.c_void_p, C.c_int32,
                                       np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS"), C.c_int, C.c_int,
-                                      C.c_int, i32p, i32p, f64p, i32p, f64p, C.c_char_p, C.c_int]
+                                      C.c_int, C.c_int, i32p, i32p, f64p, i32p, f64p, C.c_char_p, C.c_int]
     _host = H
     return H
 
@@ -181,7 +183,7 @@ EXPORTED = ["sbpx_create", "sbpx_destroy", "sbpx_last_error", "sbpx_set_option",
             "sbpx_batch_build", "sbpx_batch_download", "sbpx_batch_log_posterior", "sbpx_batch_sweep",
             "sbpx_batch_merge", "sbpx_blockmodel_trace", "sbpx_batch_trace", "sbpx_learned_eval_moves",
             "sbpx_learned_scores", "sbpx_log_posterior_noalloc", "sbpx_delta_mdl", "sbpx_merge_eval",
-            "sbpx_blockmodel_cells", "sbpx_blockmodel_line"]
+            "sbpx_blockmodel_cells", "sbpx_blockmodel_line", "sbpx_batch_replays"]
 
 
 class SbpxError(RuntimeError):
@@ -374,6 +376,12 @@ class Context:
                                           np.ascontiguousarray(calls, np.uint32), C.byref(st)))
         return st
 
+    def batch_replays(self):
+        """(rounds replayed from the captured graph, graph captures) since batch_define."""
+        r, k = C.c_int64(), C.c_int64()
+        self._chk(self.L.sbpx_batch_replays(self.h, C.byref(r), C.byref(k)))
+        return r.value, k.value
+
     def batch_merge(self, graphs, num_to_merge, seeds, calls, mode=MERGE_APPROXIMATE):
         g = np.ascontiguousarray(graphs, np.int32)
         self._chk(self.L.sbpx_batch_merge(self.h, len(g), g, np.ascontiguousarray(num_to_merge, np.int32), mode,
@@ -499,12 +507,15 @@ def union_graphs(graphs):
     return np.array(offs, np.int32), cat(ops), cat(ois), cat(ips), cat(iis)
 
 
-def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eval_variant=0, count_bytes=False):
+def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eval_variant=0, count_bytes=False,
+                    graph_replay=False):
     """sbp::partition_batch: each graph of the list (out_ptr, out_idx, in_ptr, in_idx) partitioned by
     async_gibbs with approximate merges, all on one GPU at once; graph g runs as it would alone with
     seed seeds[g] (eval_variant as OPT_EVAL_VARIANT; count_bytes counts the evaluation kernels'
-    algorithmic bytes). Returns (list of (assignment, num_blocks, mdl, mcmc_iterations), totals) with
-    totals = dict(seconds, rounds, sweeps, evals, eval_ms[4], eval_bytes[4], eval_launches[4])."""
+    algorithmic bytes; graph_replay replays sweep rounds from a captured HIP graph once the kernel
+    variants are timed). Returns (list of (assignment, num_blocks, mdl, mcmc_iterations), totals) with
+    totals = dict(seconds, rounds, sweeps, evals, eval_ms[4], eval_bytes[4], eval_launches[4], replayed,
+    captures)."""
     H = load_host()
     G = len(graphs)
     offs, op, oi, ip, ii = union_graphs(graphs)
@@ -514,15 +525,16 @@ def partition_batch(graphs, batches=1, seeds=None, num_edges=None, device=0, eva
     B = np.zeros(G, np.int32)
     mdl = np.zeros(G)
     its = np.zeros(G, np.int32)
-    tot = np.zeros(16)
+    tot = np.zeros(18)
     err = C.create_string_buffer(512)
     rc = H.sbp_partition_batch(G, offs, op, oi, ip, ii, None if ne is None else ne.ctypes.data, batches, seeds, device,
-                               int(eval_variant), int(count_bytes), a, B, mdl, its, tot, err, 512)
+                               int(eval_variant), int(count_bytes), int(graph_replay), a, B, mdl, its, tot, err, 512)
     if rc != 0:
         raise RuntimeError("sbp_partition_batch: " + err.value.decode())
     out = [(a[offs[g]:offs[g + 1]].copy(), int(B[g]), float(mdl[g]), int(its[g])) for g in range(G)]
     return out, dict(seconds=tot[0], rounds=int(tot[1]), sweeps=int(tot[2]), evals=int(tot[3]),
-                     eval_ms=tot[4:8].copy(), eval_bytes=tot[8:12].copy(), eval_launches=tot[12:16].copy())
+                     eval_ms=tot[4:8].copy(), eval_bytes=tot[8:12].copy(), eval_launches=tot[12:16].copy(),
+                     replayed=int(tot[16]), captures=int(tot[17]))
 
 
 class ResidentGraph:

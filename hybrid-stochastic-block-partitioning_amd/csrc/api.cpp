@@ -295,6 +295,11 @@ void require_batch(Ctx& c) {
 
 void free_batch(Ctx& c) {
     DevBatch& b = c.batch;
+    if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+    b.gexec = nullptr;
+    dfree(&b.d_round);
+    dfree(&b.d_queue);
+    if (b.h_round) (void)hipHostFree(b.h_round);
     for (int** p : {&b.d_vgraph, &b.d_bgraph, &b.d_vo, &b.d_bo, &b.d_nb, &b.d_req, &b.d_plan, &b.d_plan_huge,
                     &b.d_dbo, &b.d_dense, &b.d_inv})
         dfree(p);
@@ -488,6 +493,8 @@ sbpx_status sbpx_create(sbpx_ctx** out, int device) {
         SBPX_HIP(hipMemset(ctx->c.dev_err, 0, 4 * sizeof(int)));
         dalloc(&ctx->c.counters, 8);
         dalloc(&ctx->c.red, 4096);
+        dalloc(&ctx->c.words, NUM_WORDS);
+        SBPX_HIP(hipMemset(ctx->c.words, 0, NUM_WORDS * sizeof(int)));
     });
     if (st != SBPX_OK) {
         static thread_local std::string last;
@@ -508,6 +515,7 @@ void sbpx_destroy(sbpx_ctx* ctx) {
     dfree(&ctx->c.dev_err);
     dfree(&ctx->c.counters);
     dfree(&ctx->c.red);
+    dfree(&ctx->c.words);
     dfree(&ctx->c.lp_part);
     dfree(&ctx->c.lp_seg);
     for (hipEvent_t e : ctx->c.events) (void)hipEventDestroy(e);
@@ -548,6 +556,9 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
             if (value < 0 || value > LEARN_MH_MAXDEG)
                 throw ArgError("learned degree bound must be in [0, 510]");
             ctx->c.learn_maxdeg = (int)value;
+        } else if (option == 9) {
+            if (value != 0 && value != 1) throw ArgError("graph replay of batched sweeps must be 0 (off) or 1 (on)");
+            ctx->c.batch_graph = (int)value;
         } else {
             throw ArgError("unknown option");
         }
@@ -1550,6 +1561,36 @@ void batch_lists(Ctx& c, int batches) {
     }
     b.lists_batches = batches;
     b.plan_active.clear();
+    // bounds over all graphs, and plan buffers that hold any subset of them
+    size_t K = 0;
+    for (int g = 0; g < b.G; ++g) K = std::max(K, b.lists[g].size());
+    b.bound_bin.assign(K * NUM_BINS, 0);
+    b.bound_total.assign(K, 0);
+    b.bound_pool = 0;
+    size_t slots = 0, huge_slots = 0;
+    for (size_t k = 0; k < K; ++k) {
+        size_t pool = 0;
+        for (int g = 0; g < b.G; ++g) {
+            if (k >= b.lists[g].size()) continue;
+            for (int bin = 0; bin < NUM_BINS; ++bin) {
+                b.bound_bin[k * NUM_BINS + bin] += (int)b.lists[g][k][bin].size();
+                b.bound_total[k] += (int)b.lists[g][k][bin].size();
+            }
+            for (int u : b.lists[g][k][NUM_BINS - 1]) pool += 7 * (size_t)table_cap(c.g.degree[u] + 1);
+        }
+        b.bound_pool = std::max(b.bound_pool, pool);
+        slots += b.bound_total[k];
+        huge_slots += b.bound_bin[k * NUM_BINS + NUM_BINS - 1] + 1;
+    }
+    if (slots > b.plan_slots) {
+        dalloc(&b.d_plan, slots);
+        b.plan_slots = slots;
+    }
+    if (huge_slots > b.plan_huge_slots) {
+        dalloc(&b.d_plan_huge, huge_slots);
+        b.plan_huge_slots = huge_slots;
+    }
+    ensure_pool(c, b.bound_pool);
 }
 
 // Union batch k: batch k of every active graph that has one, binned; kept while the same graphs
@@ -1599,6 +1640,109 @@ void batch_plan(Ctx& c, const std::vector<char>& active, int batches) {
     ensure_pool(c, pool_need);
     b.plan_active = active;
     b.plan_batches = batches;
+}
+
+// The shape a captured round is valid for: plan batches, number of batches, key width, kernel
+// variants, the addresses of the buffers its launches name, and the update rule.
+std::vector<long long> round_key(const Ctx& c, int K, int batches, const int* variants) {
+    const DevBatch& b = c.batch;
+    std::vector<long long> key = {K, batches, sort_key_bits(c),
+                                  (long long)(intptr_t)c.table_pool, (long long)(intptr_t)b.d_plan,
+                                  (long long)(intptr_t)b.d_plan_huge, (long long)(intptr_t)b.d_round,
+                                  c.incremental, c.g.E, c.bm.B};
+    for (int bin = 0; bin < NUM_BINS; ++bin) key.push_back(variants[bin]);
+    return key;
+}
+
+// Captures one round of the plan's K batches: per batch, the evaluation of every bin with vertices
+// in some graph (persistent launches over the bin's list, counts in the round words), then the
+// application and the block-matrix update with device-held sizes.
+void capture_round(Ctx& c, int K, const int* variants) {
+    DevBatch& b = c.batch;
+    if (b.gexec) (void)hipGraphExecDestroy(b.gexec);
+    b.gexec = nullptr;
+    b.gkey.clear();
+    ensure_update_buffers(c);
+    const size_t need = (size_t)K * ROUND_WORDS;
+    if (b.round_slots < need) {
+        dfree(&b.d_round);
+        dfree(&b.d_queue);
+        if (b.h_round) (void)hipHostFree(b.h_round);
+        b.h_round = nullptr;
+        b.round_slots = 0;
+        dalloc(&b.d_round, need);
+        dalloc(&b.d_queue, (size_t)K * NUM_BINS);
+        SBPX_HIP(hipHostMalloc((void**)&b.h_round, sizeof(int) * need));
+        b.round_slots = need;
+    }
+    hipGraph_t graph = nullptr;
+    SBPX_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try {
+        launch_round_zero(c, b.d_queue, K * NUM_BINS);
+        for (int k = 0; k < K; ++k) {
+            const int* w = b.d_round + (size_t)k * ROUND_WORDS;
+            for (int bin = 0; bin < NUM_BINS; ++bin) {
+                const int bound = b.bound_bin[(size_t)k * NUM_BINS + bin];
+                if (bound == 0) continue;
+                EvalLaunch L{};
+                L.list = b.d_plan;
+                L.list_off = w + ROUND_OFF + bin;
+                L.n = bound;
+                L.n_at = w + ROUND_CNT + bin;
+                L.bin = bin;
+                L.variant = variants[bin];
+                L.tag = TAG_ASYNC;
+                L.out_by_vertex = 1;
+                L.huge_offsets = b.d_plan_huge;
+                L.huge_off_at = w + ROUND_HUGE;
+                L.queue = b.d_queue + (size_t)k * NUM_BINS + bin;
+                launch_eval(c, L);
+            }
+            apply_accepted_dyn(c, VList(b.d_plan, w + ROUND_FIRST, Dyn(b.bound_total[k], w + ROUND_TOTAL)),
+                               Dyn(0, w + ROUND_EPOCH));
+        }
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    SBPX_HIP(hipStreamEndCapture(c.stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&b.gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    SBPX_HIP(e);
+}
+
+// One round replayed from the captured graph: the round words of the plan's batches (list
+// offsets and counts per bin, huge-table bases, epochs) staged and copied in, the graph launched,
+// the cell count read back with the counters.
+void replay_round(sbpx_ctx* ctx, sbpx_sweep_stats* stats) {
+    Ctx& c = ctx->c;
+    DevBatch& b = c.batch;
+    const int K = (int)b.plan.size();
+    for (int k = 0; k < K; ++k) {
+        const BinnedList& bl = b.plan[k];
+        int* w = b.h_round + (size_t)k * ROUND_WORDS;
+        for (int bin = 0; bin < NUM_BINS; ++bin) {
+            if (bl.count[bin] > b.bound_bin[(size_t)k * NUM_BINS + bin]) throw StateError("plan exceeds its bound");
+            w[ROUND_OFF + bin] = bl.offset[bin];
+            w[ROUND_CNT + bin] = bl.count[bin];
+        }
+        w[ROUND_HUGE] = bl.huge_base;
+        w[ROUND_FIRST] = bl.offset[0];
+        w[ROUND_TOTAL] = bl.total;
+        w[ROUND_EPOCH] = c.epoch + 1 + k;
+    }
+    SBPX_HIP(hipMemcpyAsync(b.d_round, b.h_round, sizeof(int) * (size_t)K * ROUND_WORDS, hipMemcpyHostToDevice,
+                            c.stream));
+    SBPX_HIP(hipGraphLaunch(b.gexec, c.stream));
+    c.epoch += K;
+    SBPX_HIP(hipMemcpyAsync(&b.h_nnz, c.words + W_NNZ, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    c.timed.clear();
+    c.timed_bins.clear();
+    sweep_stats(ctx, stats);  // waits for the stream
+    c.bm.nnz = b.h_nnz;
+    b.graph_rounds++;
 }
 
 }  // namespace
@@ -1790,6 +1934,22 @@ sbpx_status sbpx_batch_sweep(sbpx_ctx* ctx, int algo, int32_t batches, int32_t n
         batch_select(c, n, graphs, seeds, calls, active);
         batch_lists(c, batches);
         batch_plan(c, active, batches);
+        DevBatch& bw = c.batch;
+        const int K = (int)bw.plan.size();
+        int present[NUM_BINS] = {};
+        for (const BinnedList& bl : bw.plan)
+            for (int bin = 0; bin < NUM_BINS; ++bin) present[bin] += bl.count[bin];
+        if (c.batch_graph && !ctx->count_bytes && K > 0 && c.g.E > 0 && c.bm.valid && c.variants_settled(present)) {
+            int variants[NUM_BINS];
+            for (int bin = 0; bin < NUM_BINS; ++bin) variants[bin] = c.settled_variant(bin);
+            if (!bw.gexec || bw.gkey != round_key(c, K, batches, variants) || bw.round_slots < (size_t)K * ROUND_WORDS) {
+                capture_round(c, K, variants);
+                bw.gkey = round_key(c, K, batches, variants);
+                bw.graph_captures++;
+            }
+            replay_round(ctx, stats);
+            return;
+        }
         SBPX_HIP(hipMemsetAsync(c.counters, 0, sizeof(long long) * 8, c.stream));
         SBPX_HIP(hipMemsetAsync(c.red + 2048, 0, sizeof(double), c.stream));
         c.timed.clear();
@@ -1801,6 +1961,15 @@ sbpx_status sbpx_batch_sweep(sbpx_ctx* ctx, int algo, int32_t batches, int32_t n
             apply_accepted(c, b.d_plan + bl.offset[0], bl.total);
         }
         sweep_stats(ctx, stats);
+    });
+}
+
+sbpx_status sbpx_batch_replays(sbpx_ctx* ctx, int64_t* rounds, int64_t* captures) {
+    return guarded(ctx, [&] {
+        Ctx& c = ctx->c;
+        require_batch(c);
+        if (rounds) *rounds = c.batch.graph_rounds;
+        if (captures) *captures = c.batch.graph_captures;
     });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <climits>
 #include <cstdint>
@@ -33,6 +34,53 @@ struct StateError : std::runtime_error {
 };
 struct NumericError : std::runtime_error {
     using std::runtime_error::runtime_error;
+};
+
+// An int kernel argument that the host knows (p null: the value v) or that a kernel wrote to
+// device memory earlier on the stream (*p; v is then an upper bound the host sizes launches by).
+// Launches over device-held counts can be captured in a graph once and replayed with whatever
+// sizes the device computes in each replay.
+struct Dyn {
+    int v = 0;
+    const int* p = nullptr;
+    Dyn() = default;
+    Dyn(int value) : v(value) {}
+    Dyn(int bound, const int* at) : v(bound), p(at) {}
+    __device__ __forceinline__ int get() const { return p ? *p : v; }
+};
+
+// A vertex list in device memory: base + *off (off null: base) and its length.
+struct VList {
+    const int* base = nullptr;
+    const int* off = nullptr;
+    Dyn n;
+    VList() = default;
+    VList(const int* list, int count) : base(list), n(count) {}
+    VList(const int* b, const int* o, Dyn count) : base(b), off(o), n(count) {}
+    __device__ __forceinline__ const int* ptr() const { return off ? base + *off : base; }
+};
+
+// Grid-stride loop over [0, count): one trip per thread when the launch covers the count, more
+// when a fixed grid covers a device-held count.
+#define GRID_LOOP(i, count)                                                                       \
+    for (int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), i##_end = (count); i < i##_end; \
+         i += (int)(gridDim.x * blockDim.x))
+
+// Workgroups of a launch over n items of `per` each: all of them for a host-known count, at most
+// DYN_GRID (the kernels loop) for a device-held one.
+inline int grid_for(const Dyn& n, int per);
+
+// Device words of a batched sweep round replayed from a graph (Ctx::words): the sizes and
+// decisions of the block-matrix update, which kernels write and later kernels read.
+enum DynWord : int {
+    W_NNZ = 0,     // cells of the block matrix (kept equal to bm.nnz)
+    W_TOTAL,       // change entries of the batch's movers
+    W_INC_TOTAL,   // the same when the incremental update was chosen, else 0
+    W_EMIT_N,      // list length when incremental, else 0
+    W_FULL_E,      // edge count when the full rebuild was chosen, else 0
+    W_NOLD,        // previous cells when incremental, else 0
+    W_NRUNS, W_ND, W_NNEW, W_NM, W_NNZ_INC, W_NNZ_FULL,
+    NUM_WORDS = 16
 };
 
 // Device error word bits raised by kernels.
@@ -186,7 +234,27 @@ struct DevBatch {
     // sizes and the new labels (each `range` long, allocated on the first merge)
     double* h_dS = nullptr;
     int *h_best = nullptr, *h_count = nullptr, *h_map = nullptr;
+    // Graph mode (option 9): upper bounds over all graphs of batch k's bin counts (bound_bin[k *
+    // NUM_BINS + bin]), list lengths and huge-table slots, fixed per number of batches; the plan
+    // buffers sized for them, so that their addresses do not change between rounds.
+    std::vector<int> bound_bin, bound_total;
+    size_t bound_pool = 0;
+    // the captured round: its shape (number of plan batches, key width, kernel variants, buffer
+    // addresses), the executable graph, the per-round words it reads (ROUND_WORDS per plan
+    // batch: list offset and count per bin, huge-table base, first list entry, list length,
+    // epoch), their pinned staging, and the work counters of its persistent launches
+    std::vector<long long> gkey;
+    hipGraphExec_t gexec = nullptr;
+    int* d_round = nullptr;
+    int* h_round = nullptr;
+    int* d_queue = nullptr;
+    size_t round_slots = 0;
+    int h_nnz = 0;
+    long long graph_rounds = 0;  // rounds replayed from the graph
+    long long graph_captures = 0;
 };
+constexpr int ROUND_OFF = 0, ROUND_CNT = NUM_BINS, ROUND_HUGE = 2 * NUM_BINS, ROUND_FIRST = 2 * NUM_BINS + 1,
+              ROUND_TOTAL = 2 * NUM_BINS + 2, ROUND_EPOCH = 2 * NUM_BINS + 3, ROUND_WORDS = 2 * NUM_BINS + 4;
 
 // Buffers of the incremental block-matrix update (allocated on first use).
 struct IncBuffers {
@@ -214,6 +282,7 @@ struct Ctx {
     int mh_wide = 0;   // workgroups of the Metropolis-Hastings pass; 0 = by window yield
     int64_t mh_headroom = 0;  // initial table pool headroom in slots; 0 = the initial layout (>= 1M)
     int learn_maxdeg = LEARN_MAXDEG;  // learned mixture's degree bound (sequential pass; whole batched bins)
+    int batch_graph = 0;  // batched sweeps replayed from a captured graph once the variant tuner settled
 
     // scratch
     void* keys0 = nullptr;
@@ -232,6 +301,7 @@ struct Ctx {
     int lp_seg_graphs = 0;
     int* dev_err = nullptr;
     long long* counters = nullptr;  // [8]: attempts, evals, moves
+    int* words = nullptr;           // [NUM_WORDS] device words of graph-mode rounds (DynWord)
 
     // per-vertex move buffers
     int* ev_s = nullptr;
@@ -331,6 +401,30 @@ struct Ctx {
             if (t.ms[v] * t.vertices[best] < t.ms[best] * t.vertices[v]) best = v;
         return best;
     }
+    // The variant pick_variant keeps for a bin once its timing is done (without starting a trial);
+    // whether every bin with vertices in `counts` is done.
+    int settled_variant(int bin) const {
+        const int nv = variants_of(bin);
+        if (nv == 1) return 0;
+        if (eval_variant) return eval_variant - 1 < nv ? eval_variant - 1 : 0;
+        const BinTuner& t = tuner[bin];
+        for (int v = 0; v < nv; ++v)
+            if (tuner_key != tuning_key() || t.trials[v] < TUNE_TRIALS) return 0;
+        int best = 0;
+        for (int v = 1; v < nv; ++v)
+            if (t.ms[v] * t.vertices[best] < t.ms[best] * t.vertices[v]) best = v;
+        return best;
+    }
+    bool variants_settled(const int* counts) const {
+        if (eval_variant) return true;
+        for (int b = 0; b < NUM_LDS_BINS; ++b) {
+            if (counts[b] == 0 || variants_of(b) == 1) continue;
+            if (tuner_key != tuning_key()) return false;
+            for (int v = 0; v < variants_of(b); ++v)
+                if (tuner[b].trials[v] < TUNE_TRIALS) return false;
+        }
+        return true;
+    }
     void record_variant(int bin, int variant, int n, double ms) {
         if (bin < 0 || bin >= NUM_LDS_BINS || n <= 0) return;
         BinTuner& t = tuner[bin];
@@ -349,6 +443,21 @@ void scan_exclusive(const int* in, int* out, int n, int* tmp, hipStream_t st);
 template <typename K>
 int radix_sort(K* keys, K* keys_alt, int* vals, int* vals_alt, long n, const int* shifts, int npass,
                int* hist, int* scan_tmp, hipStream_t st);
+// The same over a count held in device memory (Dyn with p set; v is its upper bound), in launches
+// of a fixed grid, so that a captured graph replays them with whatever count the device wrote.
+// The scan writes the total to out[n] and, when total_at is set, to *total_at.
+void scan_exclusive(const int* in, int* out, Dyn n, int* tmp, hipStream_t st, int* total_at = nullptr);
+template <typename K>
+int radix_sort(K* keys, K* keys_alt, int* vals, int* vals_alt, Dyn n, const int* shifts, int npass,
+               int* hist, int* scan_tmp, hipStream_t st);
+constexpr int DYN_SCAN_BLOCKS = 1024;   // workgroups of a scan over a device-held count
+constexpr int DYN_SORT_BLOCKS = 2048;   // workgroups of a radix pass over a device-held count
+constexpr int DYN_GRID = 2048;          // workgroups of an elementwise launch over a device-held count
+inline size_t dyn_hist_ints() { return 256 * (size_t)DYN_SORT_BLOCKS + 1; }
+inline int grid_for(const Dyn& n, int per) {
+    const long g = ((long)n.v + per - 1) / per;
+    return (int)std::max<long>(1, n.p ? std::min<long>(g, DYN_GRID) : g);
+}
 
 // blockmodel.hip
 void build_blockmodel(Ctx& c);               // from c.bm.assign, resets drift
@@ -359,6 +468,11 @@ void rebuild_keep_drift(Ctx& c);             // from c.bm.assign, keeps drift
 // block matrix was built from (c.bm.assign when null).
 int prepare_update(Ctx& c, const int* list, int n, const int* old_assign = nullptr);
 void finish_update(Ctx& c, const int* list, int n, int total);
+// The same with device-held sizes (graph mode): prepare before the commit, finish after.
+void prepare_update_dyn(Ctx& c, const VList& L, Dyn epoch);
+void finish_update_dyn(Ctx& c);
+void ensure_update_buffers(Ctx& c);
+int sort_key_bits(const Ctx& c);  // bits per block id in the block matrix's sort keys
 double log_posterior(Ctx& c);                // sum of M * logf(M / (dout din)) in FP64
 // Per-graph log posteriors of a union (out[G]); a one-graph context gives out[0]. Each graph's
 // cells are summed in fixed chunks from its first cell, so the value does not depend on the
@@ -387,9 +501,17 @@ struct EvalLaunch {
     const int* huge_offsets;
     long long* bytes;       // optional algorithmic byte counter
     const double* learned_u;  // learned bins: accept uniforms by vertex (null: the propose pass's)
+    // graph mode: the count in device memory (n its upper bound), the list at list + *list_off,
+    // the huge-table offsets at huge_offsets + *huge_off_at, and the launch's work counter
+    const int* n_at;
+    const int* list_off;
+    const int* huge_off_at;
+    int* queue;
 };
 void launch_eval(Ctx& c, const EvalLaunch& L);
 void apply_accepted(Ctx& c, const int* list, int n);
+void apply_accepted_dyn(Ctx& c, const VList& L, Dyn epoch);
+void launch_round_zero(Ctx& c, int* queue, int nq);
 void finish_degrees(Ctx& c);
 // learned.hip / mlp.hip
 // max_deg: the largest degree in the list (a larger neighbour-block table above LEARN_MAXDEG);

@@ -15,7 +15,13 @@ struct EvalParams {
     BlockState S;
     CsrView M;
     const int* __restrict__ list;
-    int n;
+    Dyn n;
+    // graph mode: the list starts at list + *list_off and the huge-table offsets at pool_off +
+    // *pool_off_at (device words the host fills per round); persistent workgroups take list
+    // entries from the work counter *queue until it passes n
+    const int* __restrict__ list_off;
+    const int* __restrict__ pool_off_at;
+    int* __restrict__ queue;
     const int* __restrict__ given_s;
     const double* __restrict__ given_u;
     uint64_t seed;
@@ -68,8 +74,8 @@ __device__ void write_result(const EvalParams& P, const MoveResult& R, int v, in
 }
 
 template <int U = 2, int W = WAVE>
-__device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int cap, int idx) {
-    const int v = P.list[idx];
+__device__ __forceinline__ void eval_vertex(const EvalParams& P, const int* list, int* tab, int cap, int idx) {
+    const int v = list[idx];
     int gs = P.given_s ? P.given_s[idx] : -1;
     double gu = P.given_u ? P.given_u[idx] : 0.0;
     double lqf = 0.0, lqb = 0.0;
@@ -84,15 +90,36 @@ __device__ __forceinline__ void eval_vertex(const EvalParams& P, int* tab, int c
     if (group_lane<W>() == 0) write_result(P, R, v, idx);
 }
 
+__device__ __forceinline__ const int* eval_list(const EvalParams& P) {
+    return P.list_off ? P.list + *P.list_off : P.list;
+}
+
+// The next `take` list entries of a persistent wavefront (graph mode): one atomic per wavefront
+// on the launch's work counter; the first of them, the same in every lane.
+__device__ __forceinline__ int next_entries(int* queue, int take) {
+    int idx = 0;
+    if (lane_id() == 0) idx = atomicAdd(queue, take);
+    return __shfl(idx, 0, WAVE);
+}
+
 // One wavefront per vertex, its neighbour-block table in LDS (CAP slots); WAVES wavefronts per
-// workgroup share nothing but the launch. U: row slots per lane per trip of the scans.
+// workgroup share nothing but the launch. U: row slots per lane per trip of the scans. In graph
+// mode the workgroups stay resident and each wavefront takes vertices from the work counter
+// until the list is done.
 template <int CAP, int WAVES, int U>
 __global__ void __launch_bounds__(WAVES * WAVE) eval_bin_kernel(EvalParams P) {
     __shared__ int tab[WAVES][7 * CAP];
     const int wave = threadIdx.x / WAVE;
+    const int n = P.n.get();
+    const int* list = eval_list(P);
+    if (P.queue) {
+        for (int idx = next_entries(P.queue, 1); idx < n; idx = next_entries(P.queue, 1))
+            eval_vertex<U>(P, list, tab[wave], CAP, idx);
+        return;
+    }
     const int idx = blockIdx.x * WAVES + wave;
-    if (idx >= P.n) return;
-    eval_vertex<U>(P, tab[wave], CAP, idx);
+    if (idx >= n) return;
+    eval_vertex<U>(P, list, tab[wave], CAP, idx);
 }
 
 // Low-degree vertices packed WAVE / W to a wavefront: a group of W lanes per vertex, each with its
@@ -101,30 +128,38 @@ __global__ void __launch_bounds__(WAVES * WAVE) eval_bin_kernel(EvalParams P) {
 template <int CAP, int WAVES, int U, int W>
 __global__ void __launch_bounds__(WAVES * WAVE) eval_sub_kernel(EvalParams P) {
     constexpr int PER = WAVES * (WAVE / W);
+    constexpr int SUBS = WAVE / W;  // vertices per wavefront
     __shared__ int tab[PER][7 * CAP];
     const int sub = threadIdx.x / W;
+    const int n = P.n.get();
+    const int* list = eval_list(P);
+    if (P.queue) {
+        // the wavefront takes SUBS entries at a time, one per lane group; the groups run the same
+        // trip count (the evaluation's wave-level steps need every lane)
+        for (int base = next_entries(P.queue, SUBS); base < n; base = next_entries(P.queue, SUBS)) {
+            const int idx = base + (lane_id() / W);
+            if (idx < n) eval_vertex<U, W>(P, list, tab[sub], CAP, idx);
+        }
+        return;
+    }
     const int idx = blockIdx.x * PER + sub;
-    if (idx >= P.n) return;
-    eval_vertex<U, W>(P, tab[sub], CAP, idx);
+    if (idx >= n) return;
+    eval_vertex<U, W>(P, list, tab[sub], CAP, idx);
 }
 
 // Vertices of the highest degrees: G wavefronts (the whole workgroup) share one vertex, its
 // table in LDS (CAP slots) or, with CAP = 0, in global memory at a per-vertex offset. LRN: the
 // learned mixture's draws and probabilities (P.lw >= 0) in a separate instantiation.
-template <int G, int CAP, bool LRN = false>
-__global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
-    __shared__ int tab_lds[CAP > 0 ? 7 * CAP : 1];
-    __shared__ int hist[256];
-    __shared__ double redd[G][RED_D];
-    __shared__ int redi[G][RED_I];
-    const int idx = blockIdx.x;
-    if (idx >= P.n) return;
-    const int v = P.list[idx];
+template <int G, int CAP, bool LRN>
+__device__ void eval_group_entry(const EvalParams& P, const int* list, int idx, int* tab_lds, int* hist,
+                                 double (*redd)[RED_D], int (*redi)[RED_I]) {
+    const int v = list[idx];
     int* tab = tab_lds;
     int cap = CAP;
     if constexpr (CAP == 0) {
-        const int base = P.pool_off[idx];
-        cap = (P.pool_off[idx + 1] - base) / 7;
+        const int* pool_off = P.pool_off_at ? P.pool_off + *P.pool_off_at : P.pool_off;
+        const int base = pool_off[idx];
+        cap = (pool_off[idx + 1] - base) / 7;
         tab = P.pool + base;
     }
     MoveResult R;
@@ -141,19 +176,54 @@ __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
     if (threadIdx.x == 0) write_result(P, R, v, idx);
 }
 
+// In graph mode the workgroup stays resident and takes vertices from the work counter; the two
+// barriers around the shared index keep the previous vertex's LDS in use until every wavefront
+// has finished it.
+template <int G, int CAP, bool LRN = false>
+__global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
+    __shared__ int tab_lds[CAP > 0 ? 7 * CAP : 1];
+    __shared__ int hist[256];
+    __shared__ double redd[G][RED_D];
+    __shared__ int redi[G][RED_I];
+    __shared__ int s_idx;
+    const int n = P.n.get();
+    const int* list = eval_list(P);
+    if (P.queue) {
+        for (;;) {
+            if (threadIdx.x == 0) s_idx = atomicAdd(P.queue, 1);
+            __syncthreads();
+            const int idx = s_idx;
+            __syncthreads();
+            if (idx >= n) return;
+            eval_group_entry<G, CAP, LRN>(P, list, idx, tab_lds, hist, redd, redi);
+        }
+    }
+    const int idx = blockIdx.x;
+    if (idx >= n) return;
+    eval_group_entry<G, CAP, LRN>(P, list, idx, tab_lds, hist, redd, redi);
+}
+
+// Workgroups of an evaluation launch that covers `per` vertices each: every vertex its own
+// workgroup slot for a host-known count; in graph mode at most PERSIST_GRID persistent workgroups.
+constexpr int PERSIST_GRID = 1024;
+static int eval_grid(const Dyn& n, int per) {
+    const long g = ((long)n.v + per - 1) / per;
+    return (int)std::max<long>(1, n.p ? std::min<long>(g, PERSIST_GRID) : g);
+}
+
 template <int CAP, int WAVES>
-static void launch_bin(const EvalParams& P, int n, int variant, hipStream_t st) {
+static void launch_bin(const EvalParams& P, int variant, hipStream_t st) {
     if constexpr (CAP == BIN_CAP[0]) {
         if (variant == 2) {
             constexpr int W = 16, SW = 4, PER = SW * (WAVE / W);
-            eval_sub_kernel<CAP, SW, 2, W><<<(n + PER - 1) / PER, SW * WAVE, 0, st>>>(P);
+            eval_sub_kernel<CAP, SW, 2, W><<<eval_grid(P.n, PER), SW * WAVE, 0, st>>>(P);
             return;
         }
     }
     if (variant == 1)
-        eval_bin_kernel<CAP, WAVES, 4><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
+        eval_bin_kernel<CAP, WAVES, 4><<<eval_grid(P.n, WAVES), WAVES * WAVE, 0, st>>>(P);
     else
-        eval_bin_kernel<CAP, WAVES, 2><<<(n + WAVES - 1) / WAVES, WAVES * WAVE, 0, st>>>(P);
+        eval_bin_kernel<CAP, WAVES, 2><<<eval_grid(P.n, WAVES), WAVES * WAVE, 0, st>>>(P);
 }
 
 static EvalParams make_params(Ctx& c) {
@@ -181,7 +251,11 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     if (L.n <= 0) return;
     EvalParams P = make_params(c);
     P.list = L.list;
-    P.n = L.n;
+    P.n = L.n_at ? Dyn(L.n, L.n_at) : Dyn(L.n);
+    P.list_off = L.list_off;
+    P.pool_off_at = L.huge_off_at;
+    P.queue = L.queue;
+    if ((L.n_at != nullptr) != (L.queue != nullptr)) throw StateError("a device-held count needs a work counter");
     P.given_s = L.given_s;
     P.given_u = L.given_u;
     P.seed = L.seed;
@@ -208,22 +282,23 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     static_assert(BIN_CAP[0] == 64 && BIN_CAP[1] == 128 && BIN_CAP[2] == 256 && BIN_CAP[3] == 512 &&
                       BIN_CAP[4] == 1024,
                   "bin table sizes");
+    const int g1 = eval_grid(P.n, 1);
     switch (L.bin) {
-        case 0: launch_bin<64, 8>(P, L.n, L.variant, c.stream); break;
-        case 1: launch_bin<128, 4>(P, L.n, L.variant, c.stream); break;
-        case 2: launch_bin<256, 4>(P, L.n, L.variant, c.stream); break;
+        case 0: launch_bin<64, 8>(P, L.variant, c.stream); break;
+        case 1: launch_bin<128, 4>(P, L.variant, c.stream); break;
+        case 2: launch_bin<256, 4>(P, L.variant, c.stream); break;
         case 3:
-            if (L.variant == 1 && L.learned) eval_group_kernel<4, 512, true><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
-            else if (L.variant == 1) eval_group_kernel<4, 512><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
-            else launch_bin<512, 2>(P, L.n, 0, c.stream);
+            if (L.variant == 1 && L.learned) eval_group_kernel<4, 512, true><<<g1, 4 * WAVE, 0, c.stream>>>(P);
+            else if (L.variant == 1) eval_group_kernel<4, 512><<<g1, 4 * WAVE, 0, c.stream>>>(P);
+            else launch_bin<512, 2>(P, 0, c.stream);
             break;
         case 4:
-            if (L.variant == 1 && L.learned) eval_group_kernel<4, 1024, true><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
-            else if (L.variant == 1) eval_group_kernel<4, 1024><<<L.n, 4 * WAVE, 0, c.stream>>>(P);
-            else launch_bin<1024, 1>(P, L.n, 0, c.stream);
+            if (L.variant == 1 && L.learned) eval_group_kernel<4, 1024, true><<<g1, 4 * WAVE, 0, c.stream>>>(P);
+            else if (L.variant == 1) eval_group_kernel<4, 1024><<<g1, 4 * WAVE, 0, c.stream>>>(P);
+            else launch_bin<1024, 1>(P, 0, c.stream);
             break;
-        case 5: eval_group_kernel<4, 4096><<<L.n, 4 * WAVE, 0, c.stream>>>(P); break;
-        default: eval_group_kernel<16, 0><<<L.n, 16 * WAVE, 0, c.stream>>>(P); break;
+        case 5: eval_group_kernel<4, 4096><<<g1, 4 * WAVE, 0, c.stream>>>(P); break;
+        default: eval_group_kernel<16, 0><<<g1, 16 * WAVE, 0, c.stream>>>(P); break;
     }
     SBPX_HIP(hipGetLastError());
 }
@@ -237,21 +312,22 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
 // that point (moved earlier in the batch => new block).
 constexpr int APPLY_TB = 256;
 
-__global__ void mark_moves_kernel(const int* __restrict__ list, int n, const int* __restrict__ out_s,
-                                  const uint8_t* __restrict__ out_stat, const int* __restrict__ assign, int epoch,
-                                  int* __restrict__ moved_epoch, int* __restrict__ new_block,
-                                  unsigned long long* __restrict__ last_touch, long long* __restrict__ counters) {
+__global__ void mark_moves_kernel(VList L, const int* __restrict__ out_s, const uint8_t* __restrict__ out_stat,
+                                  const int* __restrict__ assign, Dyn ep, int* __restrict__ moved_epoch,
+                                  int* __restrict__ new_block, unsigned long long* __restrict__ last_touch,
+                                  long long* __restrict__ counters) {
     __shared__ long long cnt[3];
     if (threadIdx.x < 3) cnt[threadIdx.x] = 0;
     __syncthreads();
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int* list = L.ptr();
+    const int epoch = ep.get();
     long long a = 0, e = 0, m = 0;
-    if (i < n) {
+    GRID_LOOP(i, L.n.get()) {
         int v = list[i];
         int st = out_stat[v];
-        a = 1;
-        e = st >= 1;
-        m = st == 2;
+        a += 1;
+        e += st >= 1;
+        m += st == 2;
         if (st == 2) {
             int s = out_s[v], r = assign[v];
             moved_epoch[v] = epoch;
@@ -274,21 +350,23 @@ __global__ void mark_moves_kernel(const int* __restrict__ list, int n, const int
 }
 
 // One wavefront per batch vertex; only the last mover of a block computes its drift.
-__global__ void drift_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
+__global__ void drift_kernel(VList L, const uint8_t* __restrict__ out_stat,
                              const int* __restrict__ out_ptr, const int* __restrict__ out_idx,
                              const int* __restrict__ in_ptr, const int* __restrict__ in_idx,
-                             const int* __restrict__ assign, int epoch, const int* __restrict__ moved_epoch,
+                             const int* __restrict__ assign, Dyn ep, const int* __restrict__ moved_epoch,
                              const int* __restrict__ new_block, const unsigned long long* __restrict__ last_touch,
                              int* __restrict__ drift) {
-    const int idx = blockIdx.x * (blockDim.x / WAVE) + threadIdx.x / WAVE;
-    if (idx >= n) return;
+    const int* list = L.ptr();
+    const int n = L.n.get(), epoch = ep.get();
+    const int wpb = blockDim.x / WAVE;
+    for (int idx = blockIdx.x * wpb + threadIdx.x / WAVE; idx < n; idx += gridDim.x * wpb) {
     const int v = list[idx];
-    if (out_stat[v] != 2) return;
+    if (out_stat[v] != 2) continue;
     const int lane = lane_id();
     const int r = assign[v], s = new_block[v];
     const unsigned long long key = ((unsigned long long)(unsigned)epoch << 32) | (unsigned)v;
     const bool do_r = last_touch[r] == key, do_s = last_touch[s] == key;
-    if (!do_r && !do_s) return;
+    if (!do_r && !do_s) continue;
     int cr = 0, cs = 0, self = 0;
     for (int j = out_ptr[v] + lane; j < out_ptr[v + 1]; j += WAVE) {
         int w = out_idx[j];
@@ -314,24 +392,27 @@ __global__ void drift_kernel(const int* __restrict__ list, int n, const uint8_t*
         if (do_r) drift[r] = -(cr + self);
         if (do_s) drift[s] = cs + self;
     }
+    }
 }
 
-__global__ void commit_moves_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
-                                    const int* __restrict__ new_block, int* __restrict__ assign) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int v = list[i];
-    if (out_stat[v] == 2) assign[v] = new_block[v];
+__global__ void commit_moves_kernel(VList L, const uint8_t* __restrict__ out_stat, const int* __restrict__ new_block,
+                                    int* __restrict__ assign) {
+    const int* list = L.ptr();
+    GRID_LOOP(i, L.n.get()) {
+        int v = list[i];
+        if (out_stat[v] == 2) assign[v] = new_block[v];
+    }
 }
 
 // Sums dS over the accepted moves of the batch (vertex-indexed outputs): SUM_BLOCKS partial
 // sums over a grid-stride split, then one workgroup adds them in order (deterministic).
 constexpr int SUM_BLOCKS = 256;
-__global__ void sum_accepted_kernel(const int* __restrict__ list, int n, const uint8_t* __restrict__ out_stat,
-                                    const double* __restrict__ out_dS, double* __restrict__ partial) {
+__global__ void sum_accepted_kernel(VList L, const uint8_t* __restrict__ out_stat, const double* __restrict__ out_dS,
+                                    double* __restrict__ partial) {
     __shared__ double lds[APPLY_TB / WAVE];
     double acc = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int* list = L.ptr();
+    GRID_LOOP(i, L.n.get()) {
         int v = list[i];
         if (out_stat[v] == 2) acc += out_dS[v];
     }
@@ -352,25 +433,52 @@ __global__ void add_partials_kernel(const double* __restrict__ partial, int n, d
     if (threadIdx.x == 0) out[0] += acc;
 }
 
+// Start of a round replayed from a graph: the sweep counters, the accepted-dS sum and the work
+// counters of the persistent evaluation launches start at zero (kernel stores: no memset nodes).
+__global__ void round_zero_kernel(long long* __restrict__ counters, double* __restrict__ dsum, int* __restrict__ queue,
+                                  int nq) {
+    for (int i = threadIdx.x; i < 8; i += blockDim.x) counters[i] = 0;
+    if (threadIdx.x == 0) *dsum = 0.0;
+    for (int i = threadIdx.x; i < nq; i += blockDim.x) queue[i] = 0;
+}
+
+void launch_round_zero(Ctx& c, int* queue, int nq) {
+    round_zero_kernel<<<1, APPLY_TB, 0, c.stream>>>(c.counters, c.red + 2048, queue, nq);
+}
+
+// Marks the accepted moves, sums their dS and computes the degree drift (before the commit).
+static void apply_marks(Ctx& c, const VList& L, Dyn epoch) {
+    mark_moves_kernel<<<grid_for(L.n, APPLY_TB), APPLY_TB, 0, c.stream>>>(L, c.ev_s, c.ev_acc, c.bm.assign, epoch,
+                                                                         c.moved_epoch, c.new_block, c.last_touch,
+                                                                         c.counters);
+    sum_accepted_kernel<<<SUM_BLOCKS, APPLY_TB, 0, c.stream>>>(L, c.ev_acc, c.ev_dS, c.red + 2560);
+    add_partials_kernel<<<1, WAVE, 0, c.stream>>>(c.red + 2560, SUM_BLOCKS, c.red + 2048);
+    const int wpb = APPLY_TB / WAVE;
+    drift_kernel<<<grid_for(L.n, wpb), APPLY_TB, 0, c.stream>>>(L, c.ev_acc, c.g.out_ptr, c.g.out_idx, c.g.in_ptr,
+                                                               c.g.in_idx, c.bm.assign, epoch, c.moved_epoch,
+                                                               c.new_block, c.last_touch, c.bm.drift);
+}
+
 // list: the batch's vertices (any order); outputs are vertex-indexed.
 void apply_accepted(Ctx& c, const int* list, int n) {
     if (n <= 0) return;
     c.epoch++;
-    const int nb = (n + APPLY_TB - 1) / APPLY_TB;
-    mark_moves_kernel<<<nb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_s, c.ev_acc, c.bm.assign, c.epoch, c.moved_epoch,
-                                                     c.new_block, c.last_touch, c.counters);
-    const int sb = std::min(SUM_BLOCKS, nb);
-    sum_accepted_kernel<<<sb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.ev_dS, c.red + 2560);
-    add_partials_kernel<<<1, WAVE, 0, c.stream>>>(c.red + 2560, sb, c.red + 2048);
-    const int wpb = APPLY_TB / WAVE;
-    drift_kernel<<<(n + wpb - 1) / wpb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.g.out_ptr, c.g.out_idx, c.g.in_ptr,
-                                                                c.g.in_idx, c.bm.assign, c.epoch, c.moved_epoch,
-                                                                c.new_block, c.last_touch, c.bm.drift);
+    const VList L(list, n);
+    apply_marks(c, L, Dyn(c.epoch));
     const int total = prepare_update(c, list, n);  // reads the assignment before the commit
-    commit_moves_kernel<<<nb, APPLY_TB, 0, c.stream>>>(list, n, c.ev_acc, c.new_block, c.bm.assign);
+    commit_moves_kernel<<<grid_for(L.n, APPLY_TB), APPLY_TB, 0, c.stream>>>(L, c.ev_acc, c.new_block, c.bm.assign);
     SBPX_HIP(hipGetLastError());
     if (total >= 0) finish_update(c, list, n, total);
     else rebuild_keep_drift(c);
+}
+
+// The same with the list's offset, length and epoch in device memory (graph mode): fixed grids,
+// and the block-matrix update sized on the device.
+void apply_accepted_dyn(Ctx& c, const VList& L, Dyn epoch) {
+    apply_marks(c, L, epoch);
+    prepare_update_dyn(c, L, epoch);
+    commit_moves_kernel<<<grid_for(L.n, APPLY_TB), APPLY_TB, 0, c.stream>>>(L, c.ev_acc, c.new_block, c.bm.assign);
+    finish_update_dyn(c);
 }
 
 }  // namespace sbpx

@@ -328,7 +328,8 @@ namespace sbp {
 BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::vector<int32_t>& out_ptr,
                          const std::vector<int32_t>& out_idx, const std::vector<int32_t>& in_ptr,
                          const std::vector<int32_t>& in_idx, const std::vector<long>& num_edges,
-                         const std::vector<Args>& graph_args, int device, int eval_variant, bool count_bytes) {
+                         const std::vector<Args>& graph_args, int device, int eval_variant, bool count_bytes,
+                         bool graph_replay) {
     using sbpx_host::Batch;
     using sbpx_host::Device;
     const int G = (int)vertex_offsets.size() - 1;
@@ -360,6 +361,7 @@ BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::
     check(sbpx_batch_define(batch.ctx, G, vertex_offsets.data()), "sbpx_batch_define");
     check(sbpx_set_option(batch.ctx, 3, eval_variant), "sbpx_set_option");
     check(sbpx_set_option(batch.ctx, 1, count_bytes ? 1 : 0), "sbpx_set_option");
+    check(sbpx_set_option(batch.ctx, 9, graph_replay ? 1 : 0), "sbpx_set_option");
 
     BatchRun run;
     run.graphs.resize(G);
@@ -401,6 +403,10 @@ BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::
     run.rounds = batch.rounds;
     run.sweeps = batch.sweeps;
     run.totals = batch.totals;
+    int64_t replayed = 0, captures = 0;
+    check(sbpx_batch_replays(batch.ctx, &replayed, &captures), "sbpx_batch_replays");
+    run.replayed = replayed;
+    run.captures = captures;
     if (std::getenv("SBP_TRACE"))
         std::fprintf(stderr,
                      "batch rounds %ld: build %.3f s, merge %.3f s, sweep %.3f s, log posterior %.3f s, "

@@ -188,13 +188,15 @@ int sbp_partition_checkpointed(void* gp, int algorithm, int32_t batches, int app
 
 // sbp::partition_batch: G graphs given as one union (vertex_offsets[G + 1]), each with its seed
 // and edge count (num_edges may be null: stored edges), async_gibbs with approximate merges and
-// `batches` batches; eval_variant and count_bytes as options 3 and 1 of sbpx_set_option. Writes
-// graph-local assignments (N entries), blocks and MDL per graph, and totals[16] = {wall seconds,
-// rounds, batched sweeps, vertex-move evaluations, eval_ms[4], eval_bytes[4], eval_launches[4]}
-// (per kernel class, as in sbpx_sweep_stats).
+// `batches` batches; eval_variant, count_bytes and graph_replay as options 3, 1 and 9 of
+// sbpx_set_option. Writes graph-local assignments (N entries), blocks and MDL per graph, and
+// totals[18] = {wall seconds, rounds, batched sweeps, vertex-move evaluations, eval_ms[4],
+// eval_bytes[4], eval_launches[4] (per kernel class, as in sbpx_sweep_stats), sweep rounds replayed
+// from the captured graph, graph captures}.
 int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t* out_ptr, const int32_t* out_idx,
                         const int32_t* in_ptr, const int32_t* in_idx, const int64_t* num_edges, int32_t batches,
-                        const uint64_t* seeds, int device, int eval_variant, int count_bytes, int32_t* assignment,
+                        const uint64_t* seeds, int device, int eval_variant, int count_bytes, int graph_replay,
+                        int32_t* assignment,
                         int32_t* num_blocks,
                         double* mdl, int32_t* mcmc_iterations, double* totals, char* err, int errlen) {
     try {
@@ -213,7 +215,8 @@ int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t*
         sbp::BatchRun run = sbp::partition_batch(
             std::vector<int32_t>(vertex_offsets, vertex_offsets + G + 1), std::vector<int32_t>(out_ptr, out_ptr + N + 1),
             std::vector<int32_t>(out_idx, out_idx + E), std::vector<int32_t>(in_ptr, in_ptr + N + 1),
-            std::vector<int32_t>(in_idx, in_idx + E), ne, args, device, eval_variant, count_bytes != 0);
+            std::vector<int32_t>(in_idx, in_idx + E), ne, args, device, eval_variant, count_bytes != 0,
+            graph_replay != 0);
         for (int g = 0; g < G; ++g) {
             const sbp::BatchRun::Graph& r = run.graphs[g];
             if (assignment) std::copy(r.assignment.begin(), r.assignment.end(), assignment + vertex_offsets[g]);
@@ -231,6 +234,8 @@ int sbp_partition_batch(int32_t G, const int32_t* vertex_offsets, const int32_t*
                 totals[8 + k] = run.totals.eval_bytes[k];
                 totals[12 + k] = (double)run.totals.eval_launches[k];
             }
+            totals[16] = (double)run.replayed;
+            totals[17] = (double)run.captures;
         }
         return 0;
     } catch (const std::exception& e) {

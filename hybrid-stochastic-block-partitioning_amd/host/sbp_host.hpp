@@ -287,7 +287,8 @@ Blockmodel stochastic_block_partition(Graph& graph, Args& args);
 /// joins two graphs. Each runs stochastic_block_partition with its own Args (async_gibbs with
 /// approximate merges) and ends as it would alone; num_edges[g] < 0 means its stored edges.
 /// eval_variant: the evaluation kernel variant (option 3 of sbpx_set_option; 0 = tuned);
-/// count_bytes: count the evaluation kernels' algorithmic bytes (option 1).
+/// count_bytes: count the evaluation kernels' algorithmic bytes (option 1); graph_replay: rounds
+/// replayed from a captured HIP graph once the variant timing settled (option 9).
 struct BatchRun {
     struct Graph {
         std::vector<int> assignment;  // graph-local block ids
@@ -301,12 +302,13 @@ struct BatchRun {
     long rounds = 0;       // rounds of batched device steps
     long long sweeps = 0;  // batched sweeps
     sbpx_sweep_stats totals{};
+    long long replayed = 0, captures = 0;  // sweep rounds replayed from a captured graph; captures
 };
 BatchRun partition_batch(const std::vector<int32_t>& vertex_offsets, const std::vector<int32_t>& out_ptr,
                          const std::vector<int32_t>& out_idx, const std::vector<int32_t>& in_ptr,
                          const std::vector<int32_t>& in_idx, const std::vector<long>& num_edges,
                          const std::vector<Args>& graph_args, int device = 0, int eval_variant = 0,
-                         bool count_bytes = false);
+                         bool count_bytes = false, bool graph_replay = false);
 bool done_blockmodeling(Blockmodel& blockmodel, BlockmodelTriplet& blockmodel_triplet, int min_num_blocks = 0);
 /// Whether the last stochastic_block_partition on this thread stopped at Args::max_seconds.
 bool stopped_at_time_limit();

@@ -89,7 +89,14 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  * option 8 = degree bound (out + in) of the learned mixture: 0 to 510, default 126. The
  *            sequential pass (proposal 2) covers every vertex within it; the batched
  *            sweeps cover degree <= 126 and, when the bound reaches 254 or 510, the vertices of
- *            degree 127-254 or 127-510 too. Other vertices keep the plain proposal. */
+ *            degree 127-254 or 127-510 too. Other vertices keep the plain proposal.
+ * option 9 = batched sweeps (sbpx_batch_sweep) replayed from a captured HIP graph: 1 once the
+ *            kernel-variant timing of option 3 has settled, 0 never (default). A round's
+ *            evaluation, application and block-matrix update are captured once per plan shape
+ *            (number of batches, key width, kernel variants) with every size in device memory,
+ *            and replayed for any subset of the graphs; the evaluation launches are persistent
+ *            workgroups that take vertices from per-bin work counters. The states are the same
+ *            either way. Rounds replayed from the graph report no per-kernel times. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as
@@ -269,6 +276,11 @@ sbpx_status sbpx_batch_log_posterior(sbpx_ctx* ctx, double* out);
  * its own seed and call; stats are totals over the graphs. */
 sbpx_status sbpx_batch_sweep(sbpx_ctx* ctx, int algo, int32_t batches, int32_t n, const int32_t* graphs,
                              const uint64_t* seeds, const uint32_t* calls, sbpx_sweep_stats* stats);
+
+/* Rounds of sbpx_batch_sweep replayed from the captured graph (option 9) since the batch was
+ * defined, and the graph captures made for them. No counterpart in the reference (which runs one
+ * graph per process); for callers that check how a batch ran. */
+sbpx_status sbpx_batch_replays(sbpx_ctx* ctx, int64_t* rounds, int64_t* captures);
 
 /* One merge phase (block_merge::merge_blocks, block_merge.cpp:278-311) of each listed graph,
  * merging num_to_merge[i] blocks of graphs[i]; SBPX_MERGE_APPROXIMATE only. */

@@ -2,7 +2,9 @@
 batched device step leaves each listed graph exactly as the same step leaves it in a context of
 its own (assignment, block count and log posterior bit for bit) and the graphs it does not list
 untouched; whole batched runs end exactly as the runs of the graphs alone with the same seeds, and
-a sample of graphs of the 512-graph workload's shape ends as the oracle's whole runs do."""
+a sample of graphs of the 512-graph workload's shape ends as the oracle's whole runs do. Sweep
+rounds replayed from a captured HIP graph (option 9, on by default) leave the same states as rounds
+launched one kernel at a time."""
 import numpy as np
 import pytest
 
@@ -92,6 +94,89 @@ def test_batched_steps_match_separate_contexts():
     singles[1].build(B, perm[a])
     sweep([1, 4], 9)
     _check(cb, offs, singles)
+
+
+def _steps(m, cb, G, seeds):
+    """A fixed sequence of batched merges and sweeps over changing subsets of the graphs, with one
+    and three batches; returns the states after each step."""
+    out = []
+
+    def snap():
+        out.append((cb.batch_download(), cb.batch_info().copy(), cb.batch_log_posterior()))
+
+    def merge(gs, frac, call):
+        k = [int(cb.batch_info()[g] * frac) for g in gs]
+        cb.batch_merge(gs, k, seeds[gs], np.full(len(gs), call, np.uint32))
+
+    def sweep(gs, call, batches=1):
+        cb.batch_sweep(gs, seeds[gs], np.full(len(gs), call, np.uint32), batches=batches)
+
+    for call in range(3):  # the identity partition: many moves, full rebuilds
+        sweep(list(range(G)), call)
+    snap()
+    merge(list(range(G)), 0.5, 0)
+    for call in range(3, 6):
+        sweep([g for g in range(G) if (g + call) % 3], call)
+        snap()
+    sweep([0, 2, 3], 6, batches=3)
+    sweep(list(range(G)), 7, batches=3)
+    snap()
+    merge([1, 2, 3], 0.6, 1)
+    for call in range(8, 14):
+        sweep(list(range(G)) if call % 2 else [1, 3], call)
+    snap()
+    return out
+
+
+GRAPH_REPLAY_OFF = ("graph replay of batched rounds is switched off: its first GPU run went silent for 3 minutes "
+                    "and the cause is not found yet")
+
+
+@pytest.mark.skip(reason=GRAPH_REPLAY_OFF)
+@pytest.mark.parametrize("variant", [1, 2, 3])
+def test_graph_replayed_rounds_match_launched_rounds(variant):
+    m = sbpx()
+    graphs = _graphs([900, 300, 1700, 600], seed0=40) + [heavy_arrays(5)]
+    offs, op, oi, ip, ii = m.union_graphs(graphs)
+    G = len(graphs)
+    seeds = np.arange(G, dtype=np.uint64) * 13 + 5
+    states, replays = [], []
+    for graph_on in (0, 1):
+        cb = m.Context(0)
+        cb.upload_graph(op, oi, ip, ii)
+        cb.batch_define(offs)
+        cb.set_option(m.OPT_EVAL_VARIANT, variant)
+        cb.set_option(m.OPT_BATCH_GRAPH, graph_on)
+        states.append(_steps(m, cb, G, seeds))
+        replays.append(cb.batch_replays())
+        cb.close()
+    assert replays[0] == (0, 0)
+    rounds, captures = replays[1]
+    print("replayed rounds", rounds, "captures", captures)
+    assert rounds == 14 and 1 <= captures < rounds  # every sweep round of _steps
+    for i, (x, y) in enumerate(zip(*states)):
+        assert (x[0] == y[0]).all(), i
+        assert (x[1] == y[1]).all(), i
+        assert (x[2] == y[2]).all(), i
+
+
+@pytest.mark.skip(reason=GRAPH_REPLAY_OFF)
+def test_graph_replay_with_tuned_variants_ends_as_launched():
+    """The tuner times the kernel variants in launched rounds; then the rounds are replayed. The
+    whole runs keep valid states (the final MDL of each graph equals that of its assignment
+    rebuilt alone) and replay most sweep rounds."""
+    m = sbpx()
+    graphs = _graphs([1200, 400, 2500, 800, 1600], seed0=60)
+    seeds = [3, 4, 5, 6, 7]
+    runs, tot = m.partition_batch(graphs, batches=1, seeds=seeds, graph_replay=True)
+    print("rounds", tot["rounds"], "sweeps", tot["sweeps"], "replayed", tot["replayed"], "captures", tot["captures"])
+    assert tot["replayed"] > 0 and tot["captures"] >= 1
+    for (op, oi, ip, ii), (a, B, mdl, _) in zip(graphs, runs):
+        c = m.Context(0)
+        c.upload_graph(op, oi, ip, ii)
+        c.build(B, a)
+        assert abs(c.mdl() - mdl) <= 1e-6 * abs(mdl)  # the driver keeps the MDL as a float
+        c.close()
 
 
 def test_batch_rejects_what_it_does_not_run():

@@ -134,7 +134,6 @@ void free_mlp(Ctx& c) {
     for (int l = 0; l < 3; ++l) {
         for (float** p : {&m.w[l], &m.b[l], &m.gw[l], &m.gb[l], &m.m1w[l], &m.m2w[l], &m.m1b[l], &m.m2b[l]}) dfree(p);
         dfree(&m.w16[l]);
-        dfree(&m.wt[l]);
     }
     for (__bf16** p : {&m.h1, &m.h2, &m.dz, &m.dh1, &m.dh2, &m.ones, &m.ds_x, &m.x, &m.sh1, &m.sh2, &m.tx})
         dfree(p);
@@ -156,7 +155,6 @@ void init_mlp(Ctx& c, uint64_t seed) {
             for (float** p : {&m.w[l], &m.gw[l], &m.m1w[l], &m.m2w[l]}) dalloc(p, n);
             for (float** p : {&m.b[l], &m.gb[l], &m.m1b[l], &m.m2b[l]}) dalloc(p, nb);
             dalloc(&m.w16[l], n);
-            dalloc(&m.wt[l], n);
         }
         for (__bf16** p : {&m.h1, &m.h2, &m.dh1, &m.dh2}) dalloc(p, (size_t)MLP_BATCH * MLP_H);
         dalloc(&m.dz, (size_t)MLP_BATCH * MLP_OUT);

@@ -528,7 +528,6 @@ void mlp_backward(Ctx& c, const __bf16* X, const uint32_t* cmask, const int* y, 
 void mlp_adam_step(Ctx& c, float lr);
 void mlp_gather(Ctx& c, const int* idx, int n);
 void mlp_sync_bf16(Ctx& c);
-void mlp_transpose_bf16(Ctx& c);  // input-major BF16 weights for the sequential pass
 // cells.hip: one Delta's delta MDL, fixed merges' delta MDL, cell lookups
 void launch_delta_mdl(Ctx& c, int r, int s, int n, const int* er, const int* ec, const int* ed, int k_out, int k_in,
                       int self_w, double* d_out);
@@ -538,7 +537,7 @@ void launch_cells(Ctx& c, int n, const int* rows, const int* cols, int* d_out);
 void* noalloc_scratch_address();
 void log_posterior_noalloc(void* base, long long nnz, const int* coo_row, const int* col, const int* val,
                            const int* dout, const int* din, double* out, hipStream_t st);
-// The sequential pass's in-wavefront scorer on the listed vertices (degree <= max_deg): candidates
+// The sequential pass's workgroup scorer on the listed vertices (degree <= max_deg): candidates
 // and scores [n][32].
 void learned_scores_probe(Ctx& c, const int* d_list, int n, int max_deg, int* d_cand, float* d_z);
 void mlp_convert_to_bf16(Ctx& c, const float* x, long n, __bf16* y);

@@ -82,11 +82,11 @@ __global__ void hash_fill_kernel(const int* __restrict__ ptr, const int* __restr
 // ------------------------------------------------------------------------------------------
 struct WideSlot;
 
-// The proposal network as the sequential pass reads it (see learned_scores_wave).
+// The proposal network as the sequential pass reads it (see learned_mlp_group).
 struct LearnedNet {
-    const __bf16* w1t;  // [64][256], input-major
-    const __bf16* w2t;  // [256][256]
-    const __bf16* w3t;  // [256][32]
+    const __bf16* w1;   // [256][64], output-major (the batched products' layout)
+    const __bf16* w2;   // [256][256]
+    const __bf16* w3;   // [32][256]
     const float* b1;
     const float* b2;
     const float* b3;
@@ -426,31 +426,33 @@ __device__ __forceinline__ int vertex_table_cap(const GraphView& G, int v) {
 
 // ------------------------------------------------------------------------------------------
 // The learned mixture in the sequential pass. Every step's proposal must be a function of the
-// state the step starts from, so the network scores each vertex's candidates inside its
-// evaluation, from the current state: one wavefront gathers the neighbour blocks, takes the top
-// 32, forms the 64 features and runs the 64 -> 256 -> 256 -> 32 network on its own lanes (weights
-// read input-major in BF16 from the L2, activations rounded to BF16 as the batched products round
-// them, FP32 sums). The features do not change when the vertex itself moves, so these scores give
-// both the forward and the reverse learned probability. Everything they read (the neighbours'
-// blocks, the degrees of the vertex's neighbour blocks and of its own block) is covered by the
-// window checks, as the evaluation's own reads are.
+// state the step starts from, so each window scores its entries' candidates on the current state:
+// each entry's first wavefront gathers the neighbour blocks, takes the top 32 and forms the 64
+// features (learned_features_wave); then the whole workgroup runs the 64 -> 256 -> 256 -> 32
+// network on its entries' rows with the matrix-core products of the batched forward pass
+// (learned_mlp_group), so that the weights are read once per workgroup and window and the scores
+// are the bits the batched path gives the same features. The features do not change when the
+// vertex itself moves, so these scores give both the forward and the reverse learned
+// probability. Everything they read (the neighbours' blocks, the degrees of the vertex's
+// neighbour blocks and of its own block) is covered by the window checks, as the evaluation's own
+// reads are.
 
-__device__ __forceinline__ float bf16_round(float f) {  // nearest BF16, ties to even, as a float
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+constexpr int MLP_ROWS = 16;  // feature rows a workgroup scores at once (one matrix-core tile)
+
+__device__ __forceinline__ unsigned short bf16_bits(float f) {  // nearest BF16, ties to even
     uint32_t u = __float_as_uint(f);
     u += 0x7FFFu + ((u >> 16) & 1u);
-    return __uint_as_float(u & 0xFFFF0000u);
-}
-__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
-__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
-__device__ __forceinline__ float lane_value(float x, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+    return (unsigned short)(u >> 16);
 }
 
-// Scores of vertex v (block r) in one wavefront. tab (cap slots, 2 * cap ints) is scratch for the
-// neighbour-block counts; cand[32], cnt[32] and z[32] receive the candidates, their edge counts
-// and their scores (wavefront-private, LDS). Returns whether some candidate is not r.
-__device__ bool learned_scores_wave(const GraphView& G, const BlockState& S, int v, int r, int* tab, int cap,
-                                    const LearnedNet& N, int* cand, int* cnt, float* z) {
+// Candidates and features of vertex v (block r) in one wavefront. tab (cap slots, 2 * cap ints) is
+// scratch for the neighbour-block counts; cand[32] and cnt[32] receive the candidates and their
+// edge counts (LDS) and xrow[64] the features in BF16 (LDS). Returns whether some candidate is
+// not r.
+__device__ bool learned_features_wave(const GraphView& G, const BlockState& S, int v, int r, int* tab, int cap,
+                                      const LearnedNet& N, int* cand, int* cnt, unsigned short* xrow) {
     const int lane = lane_id();
     int* key = tab;
     int* count = tab + cap;
@@ -503,68 +505,69 @@ __device__ bool learned_scores_wave(const GraphView& G, const BlockState& S, int
         fence_wave();
     }
     const int k_out = dO, k_in = dI - self;
-    const float x = bf16_round(
+    static_assert(MLP_IN == WAVE, "one feature per lane");
+    xrow[lane] = bf16_bits(
         learned_feature(lane, cand, cnt, k_out, k_in, self, r, nuniq, S.din, S.dout, N.inv_logE, N.log_ratio));
-    // layer 1: lane l computes outputs 4l .. 4l + 3
-    const int o4 = 4 * lane;
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    {
-        const uint2* w = (const uint2*)N.w1t + lane;
-#pragma unroll 4
-        for (int j = 0; j < MLP_IN; ++j) {
-            const float xj = lane_value(x, j);
-            const uint2 q = w[j * (MLP_H / 4)];
-            a0 = fmaf(xj, bf16_lo(q.x), a0);
-            a1 = fmaf(xj, bf16_hi(q.x), a1);
-            a2 = fmaf(xj, bf16_lo(q.y), a2);
-            a3 = fmaf(xj, bf16_hi(q.y), a3);
-        }
-    }
-    const float h0 = bf16_round(fmaxf(a0 + N.b1[o4], 0.0f)), h1 = bf16_round(fmaxf(a1 + N.b1[o4 + 1], 0.0f));
-    const float h2 = bf16_round(fmaxf(a2 + N.b1[o4 + 2], 0.0f)), h3 = bf16_round(fmaxf(a3 + N.b1[o4 + 3], 0.0f));
-    // layer 2: input j = 4 m + q sits in lane m, component q
-    a0 = a1 = a2 = a3 = 0.0f;
-    {
-        const uint2* w = (const uint2*)N.w2t + lane;
-#pragma unroll 2
-        for (int m = 0; m < MLP_H / 4; ++m) {
-            const float hj[4] = {lane_value(h0, m), lane_value(h1, m), lane_value(h2, m), lane_value(h3, m)};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint2 wq = w[(4 * m + q) * (MLP_H / 4)];
-                a0 = fmaf(hj[q], bf16_lo(wq.x), a0);
-                a1 = fmaf(hj[q], bf16_hi(wq.x), a1);
-                a2 = fmaf(hj[q], bf16_lo(wq.y), a2);
-                a3 = fmaf(hj[q], bf16_hi(wq.y), a3);
-            }
-        }
-    }
-    const float g0 = bf16_round(fmaxf(a0 + N.b2[o4], 0.0f)), g1 = bf16_round(fmaxf(a1 + N.b2[o4 + 1], 0.0f));
-    const float g2 = bf16_round(fmaxf(a2 + N.b2[o4 + 2], 0.0f)), g3 = bf16_round(fmaxf(a3 + N.b2[o4 + 3], 0.0f));
-    // layer 3: lane l takes output l & 31 over inputs [128 (l >> 5), 128 (l >> 5) + 128); the two
-    // halves are added with one exchange
-    const int o = lane & 31, half = lane >> 5;
-    float e = 0.0f;
-    {
-        const unsigned short* w = (const unsigned short*)N.w3t + (half * 128) * MLP_OUT + o;
-#pragma unroll 2
-        for (int m = 0; m < 32; ++m) {
-            const float gl[4] = {lane_value(g0, m), lane_value(g1, m), lane_value(g2, m), lane_value(g3, m)};
-            const float gh[4] = {lane_value(g0, 32 + m), lane_value(g1, 32 + m), lane_value(g2, 32 + m),
-                                 lane_value(g3, 32 + m)};
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float wv = __uint_as_float((uint32_t)w[(4 * m + q) * MLP_OUT] << 16);
-                e = fmaf(half ? gh[q] : gl[q], wv, e);
-            }
-        }
-    }
-    e += __shfl_xor(e, 32, WAVE);
-    if (lane < MLP_OUT) z[o] = e + N.b3[o];
     fence_wave();
     int any = 0;
     for (int j = 0; j < MLP_CAND; ++j) any |= cand[j] >= 0 && cand[j] != r;
     return any != 0;
+}
+
+// One 16 x 16 tile of A W^T (columns n0 .. n0 + 15): A is MLP_ROWS rows of K BF16 in LDS, of which
+// the first nrows are used (the others read as zero), W is output-major in global memory. The
+// fragments, the k steps and the matrix-core instruction are those of the batched forward product
+// (gemm_bf16_kcontig_kernel), so each row's sums are the same bits.
+template <int K>
+__device__ __forceinline__ f32x4_t mlp_tile(const unsigned short* A, int nrows, const __bf16* W, int n0) {
+    const int lane = lane_id();
+    const int row = lane & 15, kq = 8 * (lane >> 4);
+    const bf16x8_t zero8 = {};
+    const __bf16* wp = W + (long)(n0 + row) * K + kq;  // B fragment: column n0 + (lane & 15)
+    f32x4_t acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        const bf16x8_t a = row < nrows ? *(const bf16x8_t*)(A + row * K + k0 + kq) : zero8;
+        const bf16x8_t b = *(const bf16x8_t*)(wp + k0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// The network on feature rows [0, nrows) of x (MLP_ROWS x 64 BF16 in LDS): h1 and h2 (MLP_ROWS x 256
+// BF16 in LDS; h2 may share x's space) receive the hidden layers, z[row][32] the scores of the rows
+// below nrows. Every wavefront of the workgroup (NW of them) calls it; it ends with the scores
+// visible to all of them.
+template <int NW>
+__device__ void learned_mlp_group(const LearnedNet& N, const unsigned short* x, unsigned short* h1, unsigned short* h2,
+                                  float (*z)[MLP_OUT], int nrows) {
+    const int wave = threadIdx.x / WAVE, lane = lane_id();
+    const int col = lane & 15, r4 = 4 * (lane >> 4);
+    for (int n0 = 16 * wave; n0 < MLP_H; n0 += 16 * NW) {  // relu(x W1^T + b1)
+        const f32x4_t acc = mlp_tile<MLP_IN>(x, nrows, N.w1, n0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float t = acc[i] + N.b1[n0 + col];
+            h1[(r4 + i) * MLP_H + n0 + col] = bf16_bits(t > 0.0f ? t : 0.0f);
+        }
+    }
+    __syncthreads();
+    for (int n0 = 16 * wave; n0 < MLP_H; n0 += 16 * NW) {  // relu(h1 W2^T + b2)
+        const f32x4_t acc = mlp_tile<MLP_H>(h1, nrows, N.w2, n0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float t = acc[i] + N.b2[n0 + col];
+            h2[(r4 + i) * MLP_H + n0 + col] = bf16_bits(t > 0.0f ? t : 0.0f);
+        }
+    }
+    __syncthreads();
+    for (int n0 = 16 * wave; n0 < MLP_OUT; n0 += 16 * NW) {  // h2 W3^T + b3
+        const f32x4_t acc = mlp_tile<MLP_H>(h2, nrows, N.w3, n0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (r4 + i < nrows) z[r4 + i][n0 + col] = acc[i] + N.b3[n0 + col];
+    }
+    __syncthreads();
 }
 
 // The mixture's choice for vertex v (block r) with scores in z / cand: on the learned branch the
@@ -581,9 +584,10 @@ __device__ __forceinline__ void learned_branch(uint64_t seed, uint32_t call, uin
     *gu = unit53(dr.x2, dr.x3);
 }
 
-// The sequential pass's scorer on its own, one wavefront per listed vertex (tables: 2 * tcap ints
-// of scratch per vertex), for comparing its candidates and scores with the batched path's.
-constexpr int PROBE_WAVES = 4;
+// The sequential pass's scorer on its own, one wavefront per listed vertex and PROBE_WAVES vertices
+// per workgroup scored together (tables: 2 * tcap ints of scratch per vertex), for comparing its
+// candidates and scores with the batched path's.
+constexpr int PROBE_WAVES = 8;
 __global__ void __launch_bounds__(PROBE_WAVES * WAVE) scorer_probe_kernel(GraphView G, BlockState S,
                                                                           const int* __restrict__ list, int n,
                                                                           int* tables, int tcap, LearnedNet N,
@@ -591,13 +595,18 @@ __global__ void __launch_bounds__(PROBE_WAVES * WAVE) scorer_probe_kernel(GraphV
                                                                           float* __restrict__ z_out) {
     __shared__ int lc[PROBE_WAVES][MLP_CAND], lcnt[PROBE_WAVES][MLP_CAND];
     __shared__ float lz[PROBE_WAVES][MLP_OUT];
+    __shared__ __align__(16) unsigned short mlp[2][MLP_ROWS * MLP_H];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
     const int i = blockIdx.x * PROBE_WAVES + wave;
-    if (i >= n) return;
-    const int v = list[i];
-    learned_scores_wave(G, S, v, S.assign[v], tables + (size_t)i * 2 * tcap, vertex_table_cap(G, v), N, lc[wave],
-                        lcnt[wave], lz[wave]);
-    if (lane < MLP_CAND) {
+    unsigned short* x = mlp[1];
+    if (i < n) {
+        const int v = list[i];
+        learned_features_wave(G, S, v, S.assign[v], tables + (size_t)i * 2 * tcap, vertex_table_cap(G, v), N,
+                              lc[wave], lcnt[wave], x + wave * MLP_IN);
+    }
+    __syncthreads();
+    learned_mlp_group<PROBE_WAVES>(N, x, mlp[0], mlp[1], lz, min(PROBE_WAVES, n - (int)blockIdx.x * PROBE_WAVES));
+    if (i < n && lane < MLP_CAND) {
         cand_out[(long)i * MLP_CAND + lane] = lc[wave][lane];
         z_out[(long)i * MLP_OUT + lane] = lz[wave][lane];
     }
@@ -790,7 +799,8 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_kernel(MhParams P) {
     }
 }
 
-// LEARNED: the learned mixture, scored per evaluation on the current state (learned_scores_wave).
+// LEARNED: the learned mixture, its entries scored together on the current state in each window
+// (learned_features_wave, learned_mlp_group).
 template <int G, bool LEARNED>
 __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     constexpr int NG = MH_WAVES / G;
@@ -809,8 +819,10 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
     // the learned mixture: per group the candidates, counts, scores and the branch drawn
     __shared__ int s_lc[LEARNED ? NG : 1][MLP_CAND], s_lcnt[LEARNED ? NG : 1][MLP_CAND];
     __shared__ float s_lz[LEARNED ? NG : 1][MLP_OUT];
-    __shared__ int s_lon[NG], s_lgs[NG];
+    __shared__ int s_lon[NG], s_lgs[NG], s_lhas[NG];
     __shared__ double s_lgu[NG];
+    __shared__ __align__(16) unsigned short s_mlp[LEARNED ? 2 : 1][LEARNED ? MLP_ROWS * MLP_H : 1];
+    static_assert(NG <= MLP_ROWS, "one feature row per group");
     const int wave = threadIdx.x / WAVE, lane = lane_id();
     const int g = wave / G, wig = wave % G;
     const bool leader = wig == 0;  // the group's first wavefront commits and checks
@@ -832,18 +844,28 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
         const int cap = active ? vertex_table_cap(Gr, v) : 16;
         int* tab = cap <= LCAP ? lds_tab[g] : gtab;
         MoveResult R;
-        if constexpr (LEARNED) {  // the group's first wavefront scores the vertex on the current state
+        if constexpr (LEARNED) {
+            // each group's first wavefront forms its vertex's features on the current state, the
+            // workgroup scores them together, then the first wavefront draws the branch
             if (leader) {
-                int on = 0, gs = -1;
-                double gu = 0.0;
+                int on = 0, has = 0;
                 if (active && (Gr.out_ptr[v + 1] - Gr.out_ptr[v]) + (Gr.in_ptr[v + 1] - Gr.in_ptr[v]) <= P.net.maxdeg) {
                     on = 1;
-                    const int r = S.assign[v];
-                    const bool has = learned_scores_wave(Gr, S, v, r, tab, cap, P.net, s_lc[g], s_lcnt[g], s_lz[g]);
-                    learned_branch(P.seed, P.call, P.tag, v, r, s_lz[g], s_lc[g], has, &gs, &gu);
+                    has = learned_features_wave(Gr, S, v, S.assign[v], tab, cap, P.net, s_lc[g], s_lcnt[g],
+                                                s_mlp[1] + g * MLP_IN);
                 }
                 if (lane == 0) {
                     s_lon[g] = on;
+                    s_lhas[g] = has;
+                }
+            }
+            __syncthreads();
+            learned_mlp_group<MH_WAVES>(P.net, s_mlp[1], s_mlp[0], s_mlp[1], s_lz, NG);
+            if (leader) {
+                int gs = -1;
+                double gu = 0.0;
+                if (s_lon[g]) learned_branch(P.seed, P.call, P.tag, v, S.assign[v], s_lz[g], s_lc[g], s_lhas[g] != 0, &gs, &gu);
+                if (lane == 0) {
                     s_lgs[g] = gs;
                     s_lgu[g] = gu;
                 }
@@ -1036,8 +1058,9 @@ __global__ void wide_init_kernel(int* ctrl) {
 
 // GW: wavefronts per entry (1, 2, 4 or 8; a workgroup holds 8 / GW entries, evaluated by
 // eval_move for GW = 1 and by eval_group otherwise; each group's first wavefront checks its entry).
-// LEARNED: the proposals are the learned mixture, scored per evaluation on the current state
-// (learned_scores_wave). A separate instantiation keeps the plain kernel free of the scorer.
+// LEARNED: the proposals are the learned mixture, each window's entries scored together on the
+// current state (learned_features_wave, learned_mlp_group). A separate instantiation keeps the
+// plain kernel free of the scorer.
 template <int GW, bool LEARNED>
 __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     static_assert(GW == 1 || GW == 2 || GW == 4 || GW == 8, "1, 2, 4 or 8 wavefronts per entry");
@@ -1052,9 +1075,22 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
         CommitLds clds;
     } s_u;
     CommitLds& s_clds = s_u.clds;
-    __shared__ int lds_hist[MH_WIDE_WAVES][256];
-    __shared__ int s_acc[MH_ACC_MAX], s_acc_r[MH_ACC_MAX], s_acc_s[MH_ACC_MAX], s_acc_cap[MH_ACC_MAX];
-    __shared__ double s_dS[MH_ACC_MAX];
+    // the evaluations' histograms and the window's accepted list share their LDS with the learned
+    // network's feature and hidden rows, which are done with before the evaluations start
+    __shared__ __align__(16) union {
+        struct {
+            int hist[MH_WIDE_WAVES][256];
+            int acc[MH_ACC_MAX], acc_r[MH_ACC_MAX], acc_s[MH_ACC_MAX], acc_cap[MH_ACC_MAX];
+            double dS[MH_ACC_MAX];
+        } w;
+        unsigned short mlp[2][MLP_ROWS * MLP_H];
+    } s_w;
+    auto& lds_hist = s_w.w.hist;
+    auto& s_acc = s_w.w.acc;
+    auto& s_acc_r = s_w.w.acc_r;
+    auto& s_acc_s = s_w.w.acc_s;
+    auto& s_acc_cap = s_w.w.acc_cap;
+    auto& s_dS = s_w.w.dS;
     __shared__ int s_wcnt[MH_WIDE_WAVES], s_cut;
     __shared__ long long s_red[MH_WIDE_WAVES][4];
     __shared__ double s_redd[MH_WIDE_WAVES][2];
@@ -1062,7 +1098,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
     // branch the group's first wavefront drew
     __shared__ int s_lc[LEARNED ? NG : 1][MLP_CAND], s_lcnt[LEARNED ? NG : 1][MLP_CAND];
     __shared__ float s_lz[LEARNED ? NG : 1][MLP_OUT];
-    __shared__ int s_lon[LEARNED ? NG : 1], s_lgs[LEARNED ? NG : 1];
+    __shared__ int s_lon[LEARNED ? NG : 1], s_lgs[LEARNED ? NG : 1], s_lhas[LEARNED ? NG : 1];
     __shared__ double s_lgu[LEARNED ? NG : 1];
     // eval_group's per-wavefront partial sums
     __shared__ double redd[GW > 1 ? MH_WIDE_WAVES : 1][RED_D];
@@ -1110,55 +1146,50 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             cap = vertex_table_cap(G, v);
             if (cap <= LCAP) tab = s_u.tab[g * GW];
         }
-        if constexpr (GW == 1) {
-            if (active) {
-                if constexpr (LEARNED) {  // the learned mixture, scored on the current state
-                    const int deg = (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]);
-                    const bool on = deg <= P.net.maxdeg;
-                    int gs = -1;
-                    double gu = 0.0;
-                    if (on) {
-                        const int r = S.assign[v];
-                        const bool has =
-                            learned_scores_wave(G, S, v, r, tab, cap, P.net, s_lc[g], s_lcnt[g], s_lz[g]);
-                        learned_branch(P.seed, P.call, P.tag, v, r, s_lz[g], s_lc[g], has, &gs, &gu);
-                    }
-                    R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, gs, gu, false,
-                                                       on ? (double)LEARN_W_PLAIN : -1.0, 0.0, 0.0,
-                                                       on ? s_lz[g] : nullptr, on ? s_lc[g] : nullptr);
-                } else {
-                    R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+        if constexpr (LEARNED) {
+            // each entry's first wavefront forms its features on the current state, the workgroup
+            // scores its entries together, then the first wavefront draws the branch
+            if (leader) {
+                int on = 0, has = 0;
+                if (active && (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]) <= P.net.maxdeg) {
+                    on = 1;
+                    has = learned_features_wave(G, S, v, S.assign[v], tab, cap, P.net, s_lc[g], s_lcnt[g],
+                                                s_w.mlp[1] + g * MLP_IN);
+                }
+                if (lane == 0) {
+                    s_lon[g] = on;
+                    s_lhas[g] = has;
                 }
             }
-        } else {
-            // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
-            if constexpr (LEARNED) {  // the group's first wavefront scores the vertex on the current state
-                if (leader) {
-                    int on = 0, gs = -1;
-                    double gu = 0.0;
-                    if (active && (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]) <= P.net.maxdeg) {
-                        on = 1;
-                        const int r = S.assign[v];
-                        const bool has =
-                            learned_scores_wave(G, S, v, r, tab, cap, P.net, s_lc[g], s_lcnt[g], s_lz[g]);
-                        learned_branch(P.seed, P.call, P.tag, v, r, s_lz[g], s_lc[g], has, &gs, &gu);
-                    }
-                    if (lane == 0) {
-                        s_lon[g] = on;
-                        s_lgs[g] = gs;
-                        s_lgu[g] = gu;
-                    }
+            __syncthreads();
+            learned_mlp_group<MH_WIDE_WAVES>(P.net, s_w.mlp[1], s_w.mlp[0], s_w.mlp[1], s_lz, NG);
+            if (leader) {
+                int gs = -1;
+                double gu = 0.0;
+                if (s_lon[g]) learned_branch(P.seed, P.call, P.tag, v, S.assign[v], s_lz[g], s_lc[g], s_lhas[g] != 0, &gs, &gu);
+                if (lane == 0) {
+                    s_lgs[g] = gs;
+                    s_lgu[g] = gu;
                 }
-                __syncthreads();
-                const bool on = s_lon[g] != 0;
+            }
+            __syncthreads();
+            const bool on = s_lon[g] != 0;
+            if constexpr (GW == 1) {
+                if (active)
+                    R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
+                                                       on ? s_lgu[g] : 0.0, false, on ? (double)LEARN_W_PLAIN : -1.0,
+                                                       0.0, 0.0, on ? s_lz[g] : nullptr, on ? s_lc[g] : nullptr);
+            } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
                 R = eval_group<GW, HashView, true>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
                                                    on ? s_lgu[g] : 0.0, false, lds_hist[g], redd, redi,
                                                    on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
                                                    on ? s_lc[g] : nullptr);
-            } else {
-                R = eval_group<GW, HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g],
-                                             redd, redi);
             }
+        } else if constexpr (GW == 1) {
+            if (active) R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+        } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
+            R = eval_group<GW, HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g], redd,
+                                         redi);
         }
         if (active && R.status == 2 && tab != gtab) {  // the commit and the other checks read it from memory
             for (int j = wig * WAVE + lane; j < 7 * cap; j += GW * WAVE) gtab[j] = tab[j];
@@ -1784,8 +1815,7 @@ static void launch_group(const MhParams& P, bool learned, hipStream_t st) {
 }
 
 static LearnedNet learned_net(Ctx& c) {
-    mlp_transpose_bf16(c);
-    return LearnedNet{c.mlp.wt[0], c.mlp.wt[1], c.mlp.wt[2], c.mlp.b[0], c.mlp.b[1], c.mlp.b[2],
+    return LearnedNet{c.mlp.w16[0], c.mlp.w16[1], c.mlp.w16[2], c.mlp.b[0], c.mlp.b[1], c.mlp.b[2],
                       1.0f / log1pf((float)c.g.E),
                       c.g.N > 1 ? (float)(log((double)c.bm.B) / log((double)c.g.N)) : 0.0f,
                       std::min(c.learn_maxdeg, LEARN_MH_MAXDEG)};

@@ -300,23 +300,6 @@ void mlp_sync_bf16(Ctx& c) {
     for (int l = 0; l < 3; ++l) to_bf16(c.stream, c.mlp.w[l], mlp_weight_count(l), c.mlp.w16[l]);
 }
 
-// W[out][in] (FP32) -> Wt[in][out] (BF16).
-__global__ void transpose_bf16_kernel(const float* __restrict__ w, int nout, int nin, __bf16* __restrict__ wt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nout * nin) return;
-    const int o = i / nin, k = i % nin;
-    wt[(long)k * nout + o] = to_bf16(w[i]);
-}
-
-void mlp_transpose_bf16(Ctx& c) {
-    const int nout[3] = {MLP_H, MLP_H, MLP_OUT}, nin[3] = {MLP_IN, MLP_H, MLP_H};
-    for (int l = 0; l < 3; ++l) {
-        const int n = nout[l] * nin[l];
-        transpose_bf16_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(c.mlp.w[l], nout[l], nin[l], c.mlp.wt[l]);
-    }
-    SBPX_HIP(hipGetLastError());
-}
-
 void mlp_convert_to_bf16(Ctx& c, const float* x, long n, __bf16* y) { to_bf16(c.stream, x, n, y); }
 
 void mlp_convert_to_f32(Ctx& c, const __bf16* x, long n, float* y) {

@@ -27,7 +27,6 @@ struct MlpState {
     bool ready = false;       // weights allocated and initialised
     float* w[3] = {};         // FP32 master weights
     __bf16* w16[3] = {};      // BF16 copies read by the GEMMs
-    __bf16* wt[3] = {};       // BF16 input-major copies read by the sequential pass's scorer
     float* b[3] = {};         // FP32 biases
     float* gw[3] = {};        // gradients
     float* gb[3] = {};

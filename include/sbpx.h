@@ -230,8 +230,9 @@ sbpx_status sbpx_learned_eval_moves(sbpx_ctx* ctx, int32_t n, const int32_t* v, 
                                     double* dS, double* H, uint8_t* accept, float* qf, float* qb);
 /* The candidates (cand[n][32], -1 past the last) and scores (z[n][32]) of the listed vertices
  * (degree <= 510) on the current state: scorer 0 = the batched sweeps' path (features kernel and
- * matrix-core products), 1 = the sequential pass's scorer that one wavefront runs inside an
- * evaluation. The two sum in other orders, so scores agree to BF16 rounding, not bit for bit. */
+ * matrix-core products), 1 = the sequential pass's scorer (each vertex's features formed by one
+ * wavefront, a workgroup's vertices scored together with the same matrix-core products). Both
+ * give the same bits. */
 sbpx_status sbpx_learned_scores(sbpx_ctx* ctx, int32_t n, const int32_t* v, int scorer, int32_t* cand, float* z);
 /* The network on n (<= 8192) rows of 64 features (rounded to BF16): z[n][32] scores. */
 sbpx_status sbpx_learned_forward(sbpx_ctx* ctx, int32_t n, const float* x, float* z);

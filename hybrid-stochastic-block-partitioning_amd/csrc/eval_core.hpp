@@ -627,8 +627,12 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
     m_rs = group_max<W>(m_rs);
     fence_wave();
 
-    // 4. Delta keys and Hastings sums
-    double pf = 0.0, pb = 0.0;
+    // 4. Delta keys and Hastings sums. With the learned mixture also pf0 and pb0: the plain
+    // sampler's own probabilities of s now and of r after the move, times k (the reference's
+    // closed form smooths its weights by 1 and B for a random-block branch the sampler never
+    // takes, so it cannot be mixed with the network's probabilities)
+    const bool mix = lw >= 0.0;
+    double pf = 0.0, pb = 0.0, pf0 = 0.0, pb0 = 0.0;
     int nuniq = 0;
     for (int j0 = 0; j0 < cap; j0 += U * W) {
         int bk[U], degb[U], dinb[U], doutb[U];
@@ -677,6 +681,12 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
                     acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
                 }
             }
+            if (mix && ob_ + ib_ > 0) {  // v's neighbour slots in b (the self-loop's slot: below)
+                const int tf = b == r ? dout_r + din_r : (b == s ? dout_s + din_s : doutb[q] + dinb[q]);
+                const int tb = b == r ? ndout_r + ndin_r : (b == s ? ndout_s + ndin_s : doutb[q] + dinb[q]);
+                if (tf > 0) pf0 += (double)(ob_ + ib_) * (double)(Msb + Mbs) / (double)tf;
+                if (tb > 0) pb0 += (double)(ob_ + ib_) * (double)(Mrb + Drb + Mbr + Dbr) / (double)tb;
+            }
             const int nb = ob_ + ib_ + (b == r ? self : 0);
             if (nb == 0) continue;
             const double count = (double)nb;
@@ -706,14 +716,21 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
             acc -= (double)cell_entropy(fi(m_rs + d_rs), fi(ndin_s), fi(ndout_r));
         }
     }
+    if (mix && self && lane == 0) {  // the self-loop's slot: block r now, block s after the move
+        const int tf = dout_r + din_r, tb = ndout_s + ndin_s;
+        if (tf > 0) pf0 += (double)(m_sr + m_rs) / (double)tf;
+        if (tb > 0) pb0 += (double)((m_rs + d_rs) + (m_sr + d_sr)) / (double)tb;
+    }
     const double dS = group_sum<W>(acc);
     pf = group_sum<W>(pf);
     pb = group_sum<W>(pb);
     R.nuniq = (int)group_sum_ll<W>(nuniq);
     double H = (k == 0) ? 1.0 : pb / pf;
-    if (lw >= 0.0 && k > 0) {
-        // mixture proposal: plain part by the closed form per unit of degree, learned part as given
-        const double plain_f = pf / (double)k, plain_b = pb / (double)k;
+    if (mix && k > 0) {
+        // mixture proposal w q_plain + (1 - w) q_net, both the samplers' own probabilities
+        pf0 = group_sum<W>(pf0);
+        pb0 = group_sum<W>(pb0);
+        const double plain_f = pf0 / (double)k, plain_b = pb0 / (double)k;
         const double fwd = lw * plain_f + (1.0 - lw) * (lqf >= 0.0 ? lqf : plain_f);
         const double bwd = lw * plain_b + (1.0 - lw) * (lqb >= 0.0 ? lqb : plain_b);
         H = bwd / fwd;

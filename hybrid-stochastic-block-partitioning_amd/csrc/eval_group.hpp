@@ -14,7 +14,7 @@ namespace sbpx {
 // sums differs. LRN with lw >= 0: the proposal was the learned mixture, as in eval_move (given_s /
 // given_u carry the learned branch's draw); the net's forward and reverse probabilities come from
 // the fixed scores lz / lcand, or with lz null as lqf_in / lqb_in.
-constexpr int RED_D = 4;   // per-wavefront double partials: dS, pf, pb
+constexpr int RED_D = 6;   // per-wavefront double partials: dS, pf, pb, and the mixture's pf0, pb0
 constexpr int RED_I = 12;  // per-wavefront int partials
 
 // GU: row slots per lane per trip of the row/column pass (the proposal's row scans take 8, or 2
@@ -371,8 +371,10 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         live += redi[w0 + i][8];
     }
 
-    // 4. Delta keys and Hastings sums
-    double pf = 0.0, pb = 0.0;
+    // 4. Delta keys and Hastings sums; with the learned mixture also the plain sampler's own
+    // probabilities (times k) of s now and of r after the move, as in eval_move
+    const bool mix = LRN && lw >= 0.0;
+    double pf = 0.0, pb = 0.0, pf0 = 0.0, pb0 = 0.0;
     int nuniq = 0;
     if (ev) {
         for (int j = gl; j < cap; j += GS) {
@@ -405,11 +407,22 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
                     acc -= (double)cell_entropy(fi(Mbr - ib_), fi(ndin_r), fi(ub));
                 }
             }
+            if (mix && ob_ + ib_ > 0) {  // v's neighbour slots in b (the self-loop's slot: below)
+                const int tf = b == r ? dout_r + din_r : (b == s ? dout_s + din_s : S.dout[b] + S.din[b]);
+                const int tb = b == r ? ndout_r + ndin_r : (b == s ? ndout_s + ndin_s : S.dout[b] + S.din[b]);
+                if (tf > 0) pf0 += (double)(ob_ + ib_) * (double)(Msb + Mbs) / (double)tf;
+                if (tb > 0) pb0 += (double)(ob_ + ib_) * (double)(Mrb + Drb + Mbr + Dbr) / (double)tb;
+            }
             const int nb = ob_ + ib_ + (b == r ? self : 0);
             if (nb == 0) continue;
             const double count = (double)nb;
             pf += count * ((double)(Msb + Mbs) + 1.0) / (double)(degb + B);
             pb += count * ((double)(Mrb + Drb + Mbr + Dbr) + 1.0) / (double)(nd + B);
+        }
+        if (gl == 0 && mix && self) {  // the self-loop's slot: block r now, block s after the move
+            const int tf = dout_r + din_r, tb = ndout_s + ndin_s;
+            if (tf > 0) pf0 += (double)(m_sr + m_rs) / (double)tf;
+            if (tb > 0) pb0 += (double)((m_rs + d_rs) + (m_sr + d_sr)) / (double)tb;
         }
         if (gl == 0) {
             if (t_ss) {
@@ -433,22 +446,32 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
     acc = wave_sum(acc);
     pf = wave_sum(pf);
     pb = wave_sum(pb);
+    if (mix) {
+        pf0 = wave_sum(pf0);
+        pb0 = wave_sum(pb0);
+    }
     nuniq = (int)wave_sum_ll(nuniq);
     if (lane == 0) {
         redd[wave][0] = acc;
         redd[wave][1] = pf;
         redd[wave][2] = pb;
+        redd[wave][3] = pf0;
+        redd[wave][4] = pb0;
         redi[wave][9] = nuniq;
     }
     __syncthreads();
     double dS = 0.0;
     pf = 0.0;
     pb = 0.0;
+    pf0 = 0.0;
+    pb0 = 0.0;
     nuniq = 0;
     for (int i = 0; i < G; ++i) {
         dS += redd[w0 + i][0];
         pf += redd[w0 + i][1];
         pb += redd[w0 + i][2];
+        pf0 += redd[w0 + i][3];
+        pb0 += redd[w0 + i][4];
         nuniq += redi[w0 + i][9];
     }
     R.nuniq = nuniq;
@@ -471,10 +494,10 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         return R;
     }
     double H = (k == 0) ? 1.0 : pb / pf;
-    if (LRN && lw >= 0.0 && k > 0) {
-        // mixture proposal: plain part by the closed form per unit of degree, learned part from the
-        // fixed candidate scores
-        const double plain_f = pf / (double)k, plain_b = pb / (double)k;
+    if (mix && k > 0) {
+        // mixture proposal w q_plain + (1 - w) q_net, both the samplers' own probabilities, the
+        // learned part from the fixed candidate scores
+        const double plain_f = pf0 / (double)k, plain_b = pb0 / (double)k;
         const double fwd = lw * plain_f + (1.0 - lw) * (lqf >= 0.0 ? lqf : plain_f);
         const double bwd = lw * plain_b + (1.0 - lw) * (lqb >= 0.0 ? lqb : plain_b);
         H = bwd / fwd;

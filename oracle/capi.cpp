@@ -235,6 +235,13 @@ int oracle_mixture_hastings(void* gp, void* bmp, int v, int s, const float* z, c
 }
 
 // The two sums of the plain Hastings correction for v -> s.
+// P(s | current state) and P(r | state after the move) of the plain sampler for v -> s.
+int oracle_plain_probabilities(void* gp, void* bmp, int v, int s, double* fwd, double* bwd) {
+    GUARD_BEGIN
+    plain_probabilities(*(Graph*)gp, *(Blockmodel*)bmp, v, s, fwd, bwd);
+    GUARD_END
+}
+
 int oracle_hastings_terms(void* gp, void* bmp, int v, int s, double* fwd, double* bwd) {
     GUARD_BEGIN
     Graph* g = (Graph*)gp;

@@ -382,10 +382,43 @@ static double net_prob(const float z[32], const int cand[32], int own, int pick)
     return num / den;
 }
 
+// The plain sampler's law (common.cpp:131-192 as this implementation draws it: the random-block
+// branch never taken, Q3): a uniform neighbour slot of v (out-neighbours with the self-loop,
+// in-neighbours without it), t its block, then s in proportion to row t + column t. fwd =
+// P(s | current state), bwd = P(r | the state after v moves to s), the self-loop's slot then in s.
+void plain_probabilities(const Graph& g, const Blockmodel& bm, int v, int s, double* fwd, double* bwd) {
+    const int r = bm.assign[v];
+    const Delta d = vertex_delta(g, bm, v, r, s);
+    const int k_out = (int)g.out[v].size(), k_in_all = (int)g.in[v].size();
+    const int self = g.self[v];
+    const int k = k_out + k_in_all - self;
+    *fwd = 0.0;
+    *bwd = 0.0;
+    if (k == 0) return;
+    auto slots = [&](auto&& f) {
+        for (int w : g.out[v]) f(w);
+        for (int w : g.in[v])
+            if (w != v) f(w);
+    };
+    auto dout_after = [&](int t) { return bm.dout[t] - (t == r ? k_out : 0) + (t == s ? k_out : 0); };
+    auto din_after = [&](int t) { return bm.din[t] - (t == r ? k_in_all : 0) + (t == s ? k_in_all : 0); };
+    double f = 0.0, b = 0.0;
+    slots([&](int w) {
+        const int t = bm.assign[w];  // the self-loop's slot: r now
+        const double tf = (double)bm.dout[t] + bm.din[t];
+        if (tf > 0) f += (bm.get(t, s) + bm.get(s, t)) / tf;
+        const int t2 = w == v ? s : t;  // after the move
+        const double tb = (double)dout_after(t2) + din_after(t2);
+        if (tb > 0) b += (bm.get(t2, r) + d.get(t2, r) + bm.get(r, t2) + d.get(r, t2)) / tb;
+    });
+    *fwd = f / k;
+    *bwd = b / k;
+}
+
 // Hastings ratio of the mixture w q_plain + (1 - w) q_net for the move v: r -> s, with q_plain the
-// reference's closed form per unit of degree and q_net the softmax of the scores over the
+// plain sampler's law (plain_probabilities) and q_net the softmax of the scores over the
 // candidates other than the vertex's block in each direction; without learned mass (no candidate
-// outside the block) the plain ratio.
+// outside the block) the plain part alone.
 double mixture_hastings(const Graph& g, const Blockmodel& bm, int v, int s, const float z[32], const int cand[32],
                         double w, double* qf, double* qb) {
     const int r = bm.assign[v];
@@ -393,10 +426,8 @@ double mixture_hastings(const Graph& g, const Blockmodel& bm, int v, int s, cons
     *qf = net_prob(z, cand, r, s);
     *qb = net_prob(z, cand, s, r);
     if (p.k == 0) return 1.0;
-    Delta d = vertex_delta(g, bm, v, r, s);
-    double pf, pb;
-    hastings_terms(g, bm, v, d, r, p, &pf, &pb);
-    const double plain_f = pf / p.k, plain_b = pb / p.k;
+    double plain_f, plain_b;
+    plain_probabilities(g, bm, v, s, &plain_f, &plain_b);
     const double fwd = w * plain_f + (1.0 - w) * (*qf >= 0.0 ? *qf : plain_f);
     const double bwd = w * plain_b + (1.0 - w) * (*qb >= 0.0 ? *qb : plain_b);
     return bwd / fwd;

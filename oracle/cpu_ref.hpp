@@ -98,6 +98,7 @@ void hastings_terms(const Graph& g, const Blockmodel& bm, int v, const Delta& d,
                     double* fwd, double* bwd);
 // learned mixture proposal (this project's own; see cpu_ref.cpp)
 void learned_features(const Graph& g, const Blockmodel& bm, int v, int cand[32], int cnt[32], float x[64]);
+void plain_probabilities(const Graph& g, const Blockmodel& bm, int v, int s, double* fwd, double* bwd);
 double mixture_hastings(const Graph& g, const Blockmodel& bm, int v, int s, const float z[32], const int cand[32],
                         double w, double* qf, double* qb);
 bool accept(double dS, double H, double u);

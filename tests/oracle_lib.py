@@ -102,6 +102,8 @@ def lib():
                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.oracle_hastings_terms.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]
+    L.oracle_plain_probabilities.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double)]
     L.oracle_mdl_cells.argtypes = [C.c_long, i32p, i32p, i32p, i32p, i32p, C.c_int, C.c_long, C.c_long,
                                    C.POINTER(C.c_double)]
     L.oracle_sbp_traced.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_uint64, i32p, C.POINTER(OracleRun),
@@ -338,6 +340,12 @@ class Blockmodel:
     def hastings_terms(self, v, s):
         f, b = C.c_double(), C.c_double()
         _check(lib().oracle_hastings_terms(self.g.h, self.h, int(v), int(s), C.byref(f), C.byref(b)))
+        return f.value, b.value
+
+    def plain_probabilities(self, v, s):
+        """(P(s | now), P(r | after v moves to s)) of the plain proposal sampler."""
+        f, b = C.c_double(), C.c_double()
+        _check(lib().oracle_plain_probabilities(self.g.h, self.h, int(v), int(s), C.byref(f), C.byref(b)))
         return f.value, b.value
 
     def carry_out(self, dS, best, advanced=False):

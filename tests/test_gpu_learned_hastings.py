@@ -5,8 +5,8 @@
   path, bit for bit.
 * For fixed moves, some to a candidate (as the learned branch draws them) and some drawn by the
   plain proposal: the device's net probabilities q_net(s), q_net(r), its Hastings ratio and delta
-  MDL against the oracle's mixture ratio with the same scores (plain terms as entropy.cpp:495-557)
-  and the oracle's delta MDL; accept decisions against u <= min(1, exp(-3 dS) H).
+  MDL against the oracle's mixture ratio with the same scores (plain terms: the plain sampler's
+  own probabilities, pinned on the CPU) and the oracle's delta MDL; accept decisions against u <= min(1, exp(-3 dS) H).
 * Detailed balance: H(x -> x') H(x' -> x) = 1 on the device, the reverse evaluated from scratch in
   the state after the move, for self-loop-free vertices.
 """
@@ -67,20 +67,25 @@ def test_learned_hastings_matches_oracle_mixture():
     rng = np.random.default_rng(1)
     v = rng.choice(np.flatnonzero(deg <= 126), 600, replace=False).astype(np.int32)
     cand, z = c.learned_scores(v, scorer=0)
-    s = np.zeros(len(v), np.int32)
-    plain_s, _ = c.propose(v, 77, 0)
+    s = np.full(len(v), -1, np.int32)
+    plain_s = [c.propose(v, 77, call)[0] for call in range(4)]
     for i, x in enumerate(v):
         others = [b for b in cand[i] if b >= 0 and b != a[x]]
+        drawn = [int(p[i]) for p in plain_s if p[i] != a[x]]
         if i % 2 == 0 and others:  # a move the learned branch can draw
             s[i] = rng.choice(others)
-        else:  # the plain proposal's draw, or another block when it drew the own one
-            s[i] = plain_s[i] if plain_s[i] != a[x] else (a[x] + 1) % B
+        elif drawn:  # a move the plain proposal drew
+            s[i] = drawn[0]
+        elif others:
+            s[i] = rng.choice(others)
+    keep = s >= 0
+    v, s, cand, z = v[keep], s[keep], cand[keep], z[keep]
     u = rng.random(len(v))
     dS, H, acc, qf, qb = c.learned_eval_moves(v, s, u)
     odS, _, _ = bm.eval_moves(v, s, u)
     assert np.allclose(dS, odS, rtol=1e-6, atol=1e-9)
     learned_moves = 0
-    caught = 0
+    caught = caught_smoothed = 0
     for i, x in enumerate(v):
         Ho, qfo, qbo = bm.mixture_hastings(int(x), int(s[i]), z[i], cand[i], w=W_PLAIN)
         assert qf[i] == pytest.approx(qfo, rel=1e-5, abs=1e-7), i
@@ -88,16 +93,21 @@ def test_learned_hastings_matches_oracle_mixture():
         assert H[i] == pytest.approx(Ho, rel=1e-5), i
         if qfo > 0:
             learned_moves += 1
-            # a plausible mistake, the reverse net term taken from the forward softmax, would
-            # move H by more than the tolerance above on many of these moves
+            # plausible mistakes, the reverse net term taken from the forward softmax or the plain
+            # part taken from the reference's smoothed closed form, would move H by more than the
+            # tolerance above on many of these moves
+            pf0, pb0 = bm.plain_probabilities(int(x), int(s[i]))
+            wrong = (W_PLAIN * pb0 + (1 - W_PLAIN) * qfo) / (W_PLAIN * pf0 + (1 - W_PLAIN) * qfo)
             k = int(deg[x] - self_loop[x])
             pf, pb = bm.hastings_terms(int(x), int(s[i]))
-            wrong = (W_PLAIN * pb / k + (1 - W_PLAIN) * qfo) / (W_PLAIN * pf / k + (1 - W_PLAIN) * qfo)
-            caught += abs(wrong / Ho - 1.0) > 1e-3
+            smoothed = (W_PLAIN * pb / k + (1 - W_PLAIN) * qbo) / (W_PLAIN * pf / k + (1 - W_PLAIN) * qfo)
+            if Ho > 0:  # H = 0: the reverse proposal cannot draw r, the move is never accepted
+                caught += abs(wrong / Ho - 1.0) > 1e-3
+                caught_smoothed += abs(smoothed / Ho - 1.0) > 1e-3
         a_i = min(1.0, np.exp(-3.0 * dS[i]) * H[i])
         assert bool(acc[i]) == (u[i] <= a_i)
     assert learned_moves > 100
-    assert caught > 10
+    assert caught > 10 and caught_smoothed > 10
 
 
 def test_learned_hastings_reverses_on_the_device():
@@ -106,14 +116,21 @@ def test_learned_hastings_reverses_on_the_device():
     pool = np.flatnonzero((deg <= 126) & ~self_loop)
     v = rng.choice(pool, 60, replace=False).astype(np.int32)
     cand, _ = c.learned_scores(v, scorer=0)
-    checked = 0
+    checked = impossible = 0
     for i, x in enumerate(v):
+        x = int(x)
         r = int(a[x])
         others = [b for b in cand[i] if b >= 0 and b != r]
         if not others:
             continue
         s = int(rng.choice(others))
         _, H1, _, _, _ = c.learned_eval_moves([x], [s], [0.5])
+        if H1[0] == 0:
+            # neither part of the mixture can propose r from the moved state: the oracle agrees
+            # that r is no candidate and the plain sampler's reverse probability is zero
+            assert r not in cand[i] and bm.plain_probabilities(x, s)[1] == 0.0
+            impossible += 1
+            continue
         a2 = a.copy()
         a2[x] = s
         c.build(B, a2)
@@ -121,4 +138,4 @@ def test_learned_hastings_reverses_on_the_device():
         c.build(B, a)
         assert H1[0] * H2[0] == pytest.approx(1.0, rel=1e-4), (x, r, s, H1, H2)
         checked += 1
-    assert checked > 40
+    assert checked > 30

@@ -438,6 +438,52 @@ def test_vertex_proposal_distribution(toy):
         assert chi2 < 30.0, (v, counts, expected)
 
 
+def _plain_law(g_edges, a, M, dout, din, v):
+    """The sampler's law of test_vertex_proposal_distribution, for every target block."""
+    nbrs = [w for (x, w) in g_edges if x == v] + [x for (x, w) in g_edges if w == v and x != v]
+    p = np.zeros(M.shape[0])
+    for w in nbrs:
+        t = a[w]
+        p += (M[t, :] + M[:, t]) / (dout[t] + din[t]) / len(nbrs)
+    return p
+
+
+def test_plain_probabilities_are_the_samplers_law_both_ways(toy):
+    """The learned mixture's plain part: P(s | now) is the sampler's law pinned above (closed form
+    and chi-squared), and P(r | after the move) equals P(r) of a blockmodel rebuilt from the moved
+    assignment, so the reverse term, the degrees after the move and the self-loop's slot are pinned
+    by an independent build. The reference's closed form (smoothed by 1 and B) is not that law."""
+    g = toy_graph(toy)
+    edges = sorted(set(map(tuple, toy["edges"])))
+    B = 6
+    a0 = np.array(toy["complex"]["assignment"], np.int32)
+    bm = O.Blockmodel(g, B, a0)
+    M = bm.dense().astype(float)
+    a, dout, din, _ = bm.state()
+    checked = differs = 0
+    for v in range(len(a)):
+        law = _plain_law(edges, a, M, dout, din, v)
+        r = int(a[v])
+        for s in range(B):
+            if s == r:
+                continue
+            fwd, bwd = bm.plain_probabilities(v, s)
+            assert fwd == pytest.approx(law[s], rel=1e-12, abs=1e-15), (v, s)
+            a2 = a.copy()
+            a2[v] = s
+            bm2 = O.Blockmodel(g, B, a2)
+            M2 = bm2.dense().astype(float)
+            _, dout2, din2, _ = bm2.state()
+            back = _plain_law(edges, a2, M2, dout2, din2, v)[r]
+            assert bwd == pytest.approx(back, rel=1e-12, abs=1e-15), (v, s)
+            assert bm2.plain_probabilities(v, r)[0] == pytest.approx(bwd, rel=1e-12, abs=1e-15)
+            pf, _ = bm.hastings_terms(v, s)
+            k = len([w for (x, w) in edges if x == v]) + len([x for (x, w) in edges if w == v and x != v])
+            differs += abs(pf / k - fwd) > 1e-3 * max(fwd, 1e-9)
+            checked += 1
+    assert checked > 40 and differs > checked // 2
+
+
 def _merge_law(M, r, t_from_row_only=False, s_from_row_only=False, keep_r=False):
     """P(s | r) of propose_merge_sparse (block_merge.cpp:335-368 with common.cpp:131-182,
     block_merge=true): t in proportion to row r + column r (a diagonal cell counts twice), then s
@@ -633,6 +679,8 @@ def _mixture_graph():
         for w in rng.integers(0, n, 5):
             if int(w) != u:
                 edges.add((u, int(w)))
+        if u % 7 == 0:  # a few self-loops
+            edges.add((u, u))
     a = rng.integers(0, B, n).astype(np.int32)
     return n, B, sorted(edges), a
 
@@ -659,17 +707,18 @@ def test_learned_features_do_not_change_when_the_vertex_moves():
 
 
 def test_mixture_hastings_reduces_and_reverses(toy):
-    """The oracle's mixture Hastings ratio: with all weight on the plain proposal it is the
-    reference's Hastings correction; with uniform scores and all weight on the network it is
-    (candidates other than r) / (candidates other than s); and for every self-loop-free move whose
-    target is a candidate, H(x -> x') H(x' -> x) = 1 with the reverse evaluated from scratch in the
-    state after the move (detailed balance of the mixture proposal)."""
+    """The oracle's mixture Hastings ratio: with all weight on the plain proposal it is the ratio of
+    the plain sampler's own probabilities (not the reference's smoothed correction, which differs
+    on most moves); with uniform scores and all weight on the network it is (candidates other than
+    r) / (candidates other than s); and for every move whose target is a candidate, self-loops
+    included, H(x -> x') H(x' -> x) = 1 with the reverse evaluated from scratch in the state after
+    the move (detailed balance of the mixture proposal)."""
     n, B, edges, a = _mixture_graph()
     g = O.Graph.from_edges(n, edges)
     bm = O.Blockmodel(g, B, a)
     self_loop = {u for (u, w) in edges if u == w}
     rng = np.random.default_rng(2)
-    products = 0
+    products = looped = differs = moves = 0
     for v in range(n):
         r = int(a[v])
         cand, cnt, x = bm.learned_features(v)
@@ -677,15 +726,19 @@ def test_mixture_hastings_reduces_and_reverses(toy):
         for s in range(B):
             if s == r:
                 continue
-            _, H_plain, _ = bm.eval_moves([v], [s], [0.5])
+            _, H_ref, _ = bm.eval_moves([v], [s], [0.5])
             H1, _, _ = bm.mixture_hastings(v, s, z, cand, w=1.0)
-            assert H1 == pytest.approx(float(H_plain[0]), rel=1e-12)
+            pf0, pb0 = bm.plain_probabilities(v, s)
+            if pf0 > 0:
+                assert H1 == pytest.approx(pb0 / pf0, rel=1e-12)
+                moves += 1
+                differs += abs(H1 / float(H_ref[0]) - 1.0) > 1e-6
             valid = cand[cand >= 0]
             if s in valid and r in valid:
                 Hu, qf, qb = bm.mixture_hastings(v, s, np.zeros(32, np.float32), cand, w=0.0)
                 assert qf == pytest.approx(1.0 / (len(valid) - 1)) and qb == pytest.approx(1.0 / (len(valid) - 1))
                 assert Hu == pytest.approx(1.0, rel=1e-12)
-            if s not in valid or v in self_loop:
+            if s not in valid:
                 continue
             w = float(rng.choice([0.2, 0.5, 0.9]))
             H, _, _ = bm.mixture_hastings(v, s, z, cand, w=w)
@@ -696,7 +749,9 @@ def test_mixture_hastings_reduces_and_reverses(toy):
             Hr, _, _ = bm2.mixture_hastings(v, r, _scores(x2), c2, w=w)
             assert H * Hr == pytest.approx(1.0, rel=1e-12), (v, r, s)
             products += 1
-    assert products > 50
+            looped += v in self_loop
+    assert products > 50 and looped > 0
+    assert differs > moves // 2
 
 
 def test_mdl_from_cells_equals_blockmodel_mdl(toy):

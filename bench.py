@@ -324,7 +324,7 @@ def main():
                                                                    "metropolis_hastings"])
     ap.add_argument("--batches", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--budget", type=float, default=540.0,
+    ap.add_argument("--budget", type=float, default=510.0,
                     help="seconds for the whole run; later phases that would overrun it are skipped")
     ap.add_argument("--cpu-n", type=int, default=3000, help="vertices of the CPU oracle's whole-partition graph")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="C3 only: CPU oracle sample length")
@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--no-count", action="store_true", help="skip the untimed byte-counting replay")
     ap.add_argument("--no-partition", action="store_true", help="skip the whole plain and learned partitions")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal partition")
-    ap.add_argument("--proposal", type=int, default=1, choices=[1, 2],
+    ap.add_argument("--proposal", type=int, default=2, choices=[1, 2],
                     help="learned mixture in: 1 the batched sweeps, 2 also the sequential pass")
     ap.add_argument("--train-graphs", type=int, default=10, help="training graphs (seeds 0 .. n-1)")
     ap.add_argument("--train-n", type=int, default=200000, help="vertices per training graph")
@@ -455,7 +455,7 @@ def main():
                 out["learned"] = {
                     "proposal": ("0.2 * plain + 0.8 * network (Hastings-corrected), vertices of degree <= "
                                  f"{args.learn_maxdeg}: in the batched sweeps"
-                                 + (" and in the sequential Metropolis-Hastings pass (scored per evaluation on the "
+                                 + (" and in the sequential Metropolis-Hastings pass (each window's entries scored on the "
                                     "current state)" if args.proposal == 2 else "")),
                     "proposal_option": args.proposal, "learn_maxdeg": args.learn_maxdeg,
                     "time_to_partition_s": tl, "completed": done, "time_limit_s": lcap,

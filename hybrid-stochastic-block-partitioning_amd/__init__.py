@@ -18,6 +18,7 @@ OPT_COUNT_BYTES, OPT_MH_GROUP, OPT_EVAL_VARIANT, OPT_PROPOSAL, OPT_INCREMENTAL, 
 OPT_MH_HEADROOM = 7
 OPT_LEARN_MAXDEG = 8
 OPT_BATCH_GRAPH = 9
+OPT_MH_TEAM = 10
 ALGORITHMS = {"metropolis_hastings": ALGO_MH, "async_gibbs": ALGO_ASYNC_GIBBS, "hybrid_mcmc": ALGO_HYBRID}
 
 i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")

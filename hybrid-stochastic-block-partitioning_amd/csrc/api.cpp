@@ -120,6 +120,8 @@ void free_graph_buffers(Ctx& c) {
                     &h.ccap, &h.tables, &h.status, &h.touch, &h.d_all, &h.d_high, &h.ctrl, &h.assign0})
         dfree(p);
     dfree(&h.wstamp);
+    dfree(&h.team_hist);
+    dfree(&h.team_part);
     if (h.wide) (void)hipFree(h.wide);
     if (h.wst) (void)hipFree(h.wst);
     h = MhState();
@@ -557,6 +559,10 @@ sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value) {
         } else if (option == 9) {
             if (value != 0 && value != 1) throw ArgError("graph replay of batched sweeps must be 0 (off) or 1 (on)");
             ctx->c.batch_graph = (int)value;
+        } else if (option == 10) {
+            if (value < 0 || value > 32 || (value & (value - 1)) != 0)
+                throw ArgError("workgroups per entry must be 0 (by row length) or a power of two up to 32");
+            ctx->c.mh_team = (int)value;
         } else {
             throw ArgError("unknown option");
         }

@@ -17,14 +17,33 @@ namespace sbpx {
 constexpr int RED_D = 6;   // per-wavefront double partials: dS, pf, pb, and the mixture's pf0, pb0
 constexpr int RED_I = 12;  // per-wavefront int partials
 
+// An entry evaluated by one workgroup: no team.
+struct NoTeam {
+    static constexpr bool on = false;
+    int rank = 0, size = 1;
+    int* hist = nullptr;     // unused
+    double* part = nullptr;  // unused
+    __device__ bool sync() const { return true; }
+};
+// A team of `size` workgroups evaluates one entry (the Metropolis-Hastings window kernel, for
+// rows of tens of thousands of slots): every member forms the same neighbour table and draws the
+// same proposal; the row scans of the proposal's radix selection and of the delta-MDL rows are
+// split into `size` contiguous slices. The members' 256-bin histograms meet in `hist` ([2][size]
+// [256], one half per selection pass in turn), their delta-MDL partial sums and scanned-slot
+// counts in `part` ([size][2]), each exchange behind `sync()` (a barrier over the launch). Member
+// 0 adds the terms of the changed cells and the Hastings sums, reading the neighbour blocks'
+// cells by hash lookups (the scans no longer see every slot), and forms the result; the other
+// members return status 3.
+
 // GU: row slots per lane per trip of the row/column pass (the proposal's row scans take 8, or 2
 // with the learned mixture).
-template <int G, class View, bool LRN = false, int GU = 2>
+template <int G, class View, bool LRN = false, int GU = 2, class Team = NoTeam>
 __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const View& M, int v, int* tab, int cap,
                                  uint64_t seed, uint32_t call, uint32_t tag, int given_s, double given_u,
                                  bool propose_only, int* hist, double (*redd)[RED_D], int (*redi)[RED_I],
                                  double lw = -1.0, const float* lz = nullptr, const int* lcand = nullptr,
-                                 double lqf_in = 0.0, double lqb_in = 0.0) {
+                                 double lqf_in = 0.0, double lqb_in = 0.0, const Team& team = Team{}) {
+    static_assert(!Team::on || (View::kSlots && G == 8), "teams: hash rows, one 8-wavefront group per workgroup");
     constexpr int GS = G * WAVE;
     const int lane = lane_id();
     const int wave = threadIdx.x / WAVE;
@@ -144,8 +163,15 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         const bool row = samp_target < samp_trow;
         const int* skeys = row ? M.rkeys : M.ckeys;
         const int* svals = row ? M.rvals : M.cvals;
-        const int slo = sample ? (row ? M.rbase[samp_t] : M.cbase[samp_t]) : 0;
-        const int shi = sample ? slo + (row ? M.rcap[samp_t] : M.ccap[samp_t]) : 0;
+        int slo = sample ? (row ? M.rbase[samp_t] : M.cbase[samp_t]) : 0;
+        int shi = sample ? slo + (row ? M.rcap[samp_t] : M.ccap[samp_t]) : 0;
+        if constexpr (Team::on) {  // this member's slice of the row
+            const int per = (shi - slo + team.size - 1) / team.size;
+            const int lo = slo + team.rank * per;
+            shi = min(shi, lo + per);
+            slo = min(lo, shi);
+        }
+        [[maybe_unused]] int pass = 0;
         const int starget = row ? samp_target : samp_target - samp_trow;
         int nbits = 1;
         while (nbits < 31 && (1 << nbits) < Bids) ++nbits;
@@ -176,6 +202,20 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
                 }
             }
             __syncthreads();
+            if constexpr (Team::on) {
+                // publish this slice's counts, wait for the team, then every member sums the
+                // team's counts in member order
+                int* th = team.hist + (size_t)(pass & 1) * team.size * 256;
+                for (int j = gl; j < 256; j += GS) th[team.rank * 256 + j] = hist[j];
+                team.sync();
+                for (int j = gl; j < 256; j += GS) {
+                    int h = 0;
+                    for (int q = 0; q < team.size; ++q) h += th[q * 256 + j];
+                    hist[j] = h;
+                }
+                __syncthreads();
+                ++pass;
+            }
             if (sample) {
                 long long h[4];
                 long long local = 0;
@@ -259,14 +299,20 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         const int b2 = M.cbeg(s), L2 = M.cend(s) - b2;
         const int b3 = M.cbeg(r), L3 = M.cend(r) - b3;
         const int T1 = L0 + L1, T2 = T1 + L2, T = T2 + L3;
+        int lo = 0, hi = T;
+        if constexpr (Team::on) {  // this member's slice of the four rows, taken end to end
+            const int per = (T + team.size - 1) / team.size;
+            lo = min(T, team.rank * per);
+            hi = min(T, lo + per);
+        }
         // GU slots per lane per trip: keys and counts, then the neighbour-block records, then the
         // degrees of the unchanged cells' other blocks, each stage's loads issued together
-        for (int i0 = gl; i0 < T; i0 += GU * GS) {
+        for (int i0 = lo + gl; i0 < hi; i0 += GU * GS) {
             int cc[GU], mm[GU], sg[GU], dsel[GU];
 #pragma unroll
             for (int q = 0; q < GU; ++q) {
                 const int i = i0 + q * GS;
-                sg[q] = i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : i < T ? 3 : -1;
+                sg[q] = i >= hi ? -1 : i < L0 ? 0 : i < T1 ? 1 : i < T2 ? 2 : 3;
                 cc[q] = -1;
                 mm[q] = 0;
                 if (sg[q] == 0) {
@@ -370,13 +416,33 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         m_rs = max(m_rs, redi[w0 + i][7]);
         live += redi[w0 + i][8];
     }
+    // a member saw only its slice of the rows: member 0 reads the cells of the neighbour blocks
+    // and of r and s by lookups instead
+    const bool lead4 = ev && (!Team::on || team.rank == 0);
+    if constexpr (Team::on) {
+        if (lead4) {
+            m_ss = M.rlookup(s, s);
+            m_rr = M.rlookup(r, r);
+            m_sr = M.rlookup(s, r);
+            m_rs = M.rlookup(r, s);
+            for (int j = gl; j < cap; j += GS) {
+                const int b = tkey[j];
+                if (b < 0 || b == r || b == s) continue;
+                tmsb[j] = M.rlookup(s, b);
+                tmrb[j] = M.rlookup(r, b);
+                tmbs[j] = M.clookup(s, b);
+                tmbr[j] = M.clookup(r, b);
+            }
+        }
+        __syncthreads();
+    }
 
     // 4. Delta keys and Hastings sums; with the learned mixture also the plain sampler's own
     // probabilities (times k) of s now and of r after the move, as in eval_move
     const bool mix = LRN && lw >= 0.0;
     double pf = 0.0, pb = 0.0, pf0 = 0.0, pb0 = 0.0;
     int nuniq = 0;
-    if (ev) {
+    if (lead4) {
         for (int j = gl; j < cap; j += GS) {
             const int b = tkey[j];
             if (b < 0) continue;
@@ -473,6 +539,25 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         pf0 += redd[w0 + i][3];
         pb0 += redd[w0 + i][4];
         nuniq += redi[w0 + i][9];
+    }
+    if constexpr (Team::on) {
+        // the members' partial sums, added in member order by member 0
+        if (gl == 0) {
+            team.part[2 * team.rank] = dS;
+            team.part[2 * team.rank + 1] = (double)live;
+        }
+        team.sync();
+        if (team.rank != 0) {
+            R.status = 3;
+            return R;
+        }
+        dS = 0.0;
+        double lv = 0.0;
+        for (int q = 0; q < team.size; ++q) {
+            dS += team.part[2 * q];
+            lv += team.part[2 * q + 1];
+        }
+        live = (long long)lv;
     }
     R.nuniq = nuniq;
     R.m_ss = m_ss;

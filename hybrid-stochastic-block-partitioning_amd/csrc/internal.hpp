@@ -188,6 +188,9 @@ struct MhState {
     void* wide = nullptr;                  // its per-entry records
     int* wst = nullptr;                    // their evaluation status, dense
     int* ctrl = nullptr;                   // its barrier counter and window control words
+    int* team_hist = nullptr;     // [k_max * 512] radix histograms of workgroup teams ([2][team][256] per entry)
+    double* team_part = nullptr;  // [k_max * 2] their partial delta-MDL sums and scanned slots
+    int last_team = 1;            // workgroups per entry in the last chunk
     int* assign0 = nullptr;  // [N] the assignment the block-matrix snapshot was built from
     int* d_all = nullptr;   // 0..N-1
     int* d_high = nullptr;  // high-degree vertices in degree order
@@ -280,6 +283,7 @@ struct Ctx {
     int proposal = 0;  // 0 plain, 1 learned mixture for the batched low-degree vertices
     int mh_group = 0;  // wavefronts per vertex in the Metropolis-Hastings kernel; 0 = by row length
     int mh_wide = 0;   // workgroups of the Metropolis-Hastings pass; 0 = by window yield
+    int mh_team = 0;   // workgroups per window entry; 0 = by scanned row length, 1 = one
     int64_t mh_headroom = 0;  // initial table pool headroom in slots; 0 = the initial layout (>= 1M)
     int learn_maxdeg = LEARN_MAXDEG;  // learned mixture's degree bound (sequential pass; whole batched bins)
     int batch_graph = 0;  // batched sweeps replayed from a captured graph once the variant tuner settled

@@ -139,6 +139,10 @@ struct MhParams {
     unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
     int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
                                     // [2] next queue position, [3] stop reason
+    // workgroups per window entry (mh_wide_kernel<8, *, true>), their histograms and partial sums
+    int team;
+    int* team_hist;
+    double* team_part;
     // the learned mixture (LEARNED kernels only): the network and the degree bound
     LearnedNet net;
     // optional phase clock of mh_wide_kernel (workgroup 0, 100 MHz ticks): evaluation and first
@@ -1004,6 +1008,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 // bookkeeping and the commit stay out of scratch memory) and two workgroups share a compute unit.
 constexpr int MH_WIDE_WAVES = 8;
 constexpr int MH_WIDE_MAX = 512;  // workgroups (co-resident: at most two per compute unit)
+constexpr int MH_TEAM_SCAN = 16384;  // mean slots scanned per evaluation from which entries take teams
 constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WIDE_WAVES;
 // Accepted entries a window tracks; a window with more ends before the next one (its commits
 // and checks see only these).
@@ -1049,6 +1054,23 @@ __device__ bool grid_barrier(int* ctrl, int n, int* s_ok, bool rel, bool acq) {
     return *s_ok != 0;
 }
 
+// The team of workgroups that evaluates one window entry (eval_group's Team): its exchanges are
+// barriers over the whole launch, which every workgroup takes the same number of times per window.
+// After a barrier that timed out the others are skipped (the window loop then stops).
+struct WideTeam {
+    static constexpr bool on = true;
+    int rank, size;
+    int* hist;
+    double* part;
+    int* ctrl;
+    int* nbar;
+    int* s_ok;
+    __device__ bool sync() const {
+        if (*s_ok == 0) return false;
+        return grid_barrier(ctrl, ++*nbar, s_ok, true, true);
+    }
+};
+
 __global__ void wide_init_kernel(int* ctrl) {
     ctrl[0] = 0;
     ctrl[1] = INT32_MAX;
@@ -1061,9 +1083,12 @@ __global__ void wide_init_kernel(int* ctrl) {
 // LEARNED: the proposals are the learned mixture, each window's entries scored together on the
 // current state (learned_features_wave, learned_mlp_group). A separate instantiation keeps the
 // plain kernel free of the scorer.
-template <int GW, bool LEARNED>
+// TEAM (GW = 8): P.team consecutive workgroups evaluate each entry together (WideTeam), for rows
+// of tens of thousands of slots; the team's first workgroup records and checks the entry.
+template <int GW, bool LEARNED, bool TEAM = false>
 __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams P) {
     static_assert(GW == 1 || GW == 2 || GW == 4 || GW == 8, "1, 2, 4 or 8 wavefronts per entry");
+    static_assert(!TEAM || GW == 8, "a team evaluates one entry per workgroup");
     constexpr int NG = MH_WIDE_WAVES / GW;  // entries per workgroup
     constexpr int LCAP = GW * MH_LDS_CAP;   // LDS table slots per entry
     __shared__ int s_ok, s_flag, s_n, s_end, s_stop, s_need;
@@ -1105,10 +1130,15 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
     __shared__ int redi[GW > 1 ? MH_WIDE_WAVES : 1][RED_I];
     const int wave = threadIdx.x / WAVE, lane = lane_id();
     const int g = wave / GW, wig = wave % GW;
-    const bool leader = wig == 0;  // the entry's first wavefront records and checks it
-    const int slot = (int)blockIdx.x * NG + g;
-    const int WS = (int)gridDim.x * NG;  // entries per window
-    int* const gtab = P.tables + (size_t)slot * 7 * P.tcap;
+    const int TS = TEAM ? P.team : 1;
+    const int rank = TEAM ? (int)blockIdx.x % TS : 0;  // place in the entry's team
+    const bool first = wig == 0;               // the group's first wavefront (features, learned branch)
+    const bool leader = first && rank == 0;    // the entry's first wavefront records and checks it
+    const int slot = TEAM ? (int)blockIdx.x / TS : (int)blockIdx.x * NG + g;
+    const int WS = TEAM ? (int)gridDim.x / TS : (int)gridDim.x * NG;  // entries per window
+    // the entry's table (read by the checks and the commit) is its first workgroup's; the other
+    // members keep theirs after the entries' tables
+    int* const gtab = P.tables + (size_t)(rank == 0 ? slot : WS + slot * (TS - 1) + rank - 1) * 7 * P.tcap;
     const GraphView G = P.G;
     const BlockState S{P.assign, P.dout, P.din, P.deg, P.B};
     const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[g]};
@@ -1149,7 +1179,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
         if constexpr (LEARNED) {
             // each entry's first wavefront forms its features on the current state, the workgroup
             // scores its entries together, then the first wavefront draws the branch
-            if (leader) {
+            if (first) {
                 int on = 0, has = 0;
                 if (active && (G.out_ptr[v + 1] - G.out_ptr[v]) + (G.in_ptr[v + 1] - G.in_ptr[v]) <= P.net.maxdeg) {
                     on = 1;
@@ -1163,7 +1193,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             }
             __syncthreads();
             learned_mlp_group<MH_WIDE_WAVES>(P.net, s_w.mlp[1], s_w.mlp[0], s_w.mlp[1], s_lz, NG);
-            if (leader) {
+            if (first) {
                 int gs = -1;
                 double gu = 0.0;
                 if (s_lon[g]) learned_branch(P.seed, P.call, P.tag, v, S.assign[v], s_lz[g], s_lc[g], s_lhas[g] != 0, &gs, &gu);
@@ -1179,6 +1209,13 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                     R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
                                                        on ? s_lgu[g] : 0.0, false, on ? (double)LEARN_W_PLAIN : -1.0,
                                                        0.0, 0.0, on ? s_lz[g] : nullptr, on ? s_lc[g] : nullptr);
+            } else if constexpr (TEAM) {
+                const WideTeam tm{rank, TS, P.team_hist + (size_t)slot * 2 * TS * 256, P.team_part + (size_t)slot * 2 * TS,
+                                  P.ctrl, &nbar, &s_ok};
+                R = eval_group<GW, HashView, true, 2, WideTeam>(
+                    G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1, on ? s_lgu[g] : 0.0, false,
+                    lds_hist[g], redd, redi, on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
+                    on ? s_lc[g] : nullptr, 0.0, 0.0, tm);
             } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
                 R = eval_group<GW, HashView, true>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
                                                    on ? s_lgu[g] : 0.0, false, lds_hist[g], redd, redi,
@@ -1187,9 +1224,18 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             }
         } else if constexpr (GW == 1) {
             if (active) R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
+        } else if constexpr (TEAM) {
+            const WideTeam tm{rank, TS, P.team_hist + (size_t)slot * 2 * TS * 256, P.team_part + (size_t)slot * 2 * TS,
+                              P.ctrl, &nbar, &s_ok};
+            R = eval_group<GW, HashView, false, 2, WideTeam>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false,
+                                                             lds_hist[g], redd, redi, -1.0, nullptr, nullptr, 0.0, 0.0,
+                                                             tm);
         } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
             R = eval_group<GW, HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g], redd,
                                          redi);
+        }
+        if constexpr (TEAM) {
+            if (!s_ok) break;  // a team barrier timed out
         }
         if (active && R.status == 2 && tab != gtab) {  // the commit and the other checks read it from memory
             for (int j = wig * WAVE + lane; j < 7 * cap; j += GW * WAVE) gtab[j] = tab[j];
@@ -1796,7 +1842,8 @@ template <int GW>
 static const void* wide_kernel_gw(bool learned) {
     return learned ? (const void*)mh_wide_kernel<GW, true> : (const void*)mh_wide_kernel<GW, false>;
 }
-static const void* wide_kernel(int gw, bool learned) {
+static const void* wide_kernel(int gw, bool learned, bool team = false) {
+    if (team) return learned ? (const void*)mh_wide_kernel<8, true, true> : (const void*)mh_wide_kernel<8, false, true>;
     switch (gw) {
         case 1: return wide_kernel_gw<1>(learned);
         case 2: return wide_kernel_gw<2>(learned);
@@ -1853,10 +1900,12 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // the most workgroups of the window kernels that fit on the device at once (a
         // cooperative launch needs them all resident)
         int per_cu = 1 << 20, per_cu_l = 1 << 20, cus = 1;
-        for (int gw = 1; gw <= 8; gw *= 2) {
+        for (int gw = 1; gw <= 16; gw *= 2) {  // 16: the team kernels
             int p = 1, pl = 1;
-            SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, wide_kernel(gw, false), MH_WIDE_WAVES * WAVE, 0));
-            SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pl, wide_kernel(gw, true), MH_WIDE_WAVES * WAVE, 0));
+            SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&p, wide_kernel(std::min(gw, 8), false, gw == 16),
+                                                                  MH_WIDE_WAVES * WAVE, 0));
+            SBPX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&pl, wide_kernel(std::min(gw, 8), true, gw == 16),
+                                                                  MH_WIDE_WAVES * WAVE, 0));
             per_cu = std::min(per_cu, std::max(p, 1));
             per_cu_l = std::min(per_cu_l, std::max(pl, 1));
         }
@@ -1866,12 +1915,15 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         h.k_max_l = 1;
         while (2 * h.k_max_l <= std::min(MH_WIDE_MAX, per_cu_l * cus)) h.k_max_l *= 2;
         SBPX_HIP(hipMalloc((void**)&h.ctrl, sizeof(int) * 4));
+        SBPX_HIP(hipMalloc((void**)&h.team_hist, sizeof(int) * 512 * (size_t)std::max(h.k_max, h.k_max_l)));
+        SBPX_HIP(hipMalloc((void**)&h.team_part, sizeof(double) * 2 * (size_t)std::max(h.k_max, h.k_max_l)));
         SBPX_HIP(hipMalloc((void**)&h.wstamp, sizeof(unsigned long long) * std::max(c.g.N, 1)));
         SBPX_HIP(hipMemsetAsync(h.wstamp, 0, sizeof(unsigned long long) * std::max(c.g.N, 1), c.stream));
         SBPX_HIP(hipMemsetAsync(h.status + 9, 0, sizeof(int), c.stream));
     }
     // one table per wavefront of the widest window (or of one workgroup)
-    const size_t tslots = (size_t)(wide ? std::max(h.k_max * MH_WIDE_WAVES, MH_WAVES) : MH_WAVES) * 7 * tcap;
+    const size_t tslots =
+        (size_t)(wide ? std::max(std::max(h.k_max, h.k_max_l) * MH_WIDE_WAVES, MH_WAVES) : MH_WAVES) * 7 * tcap;
     if (tslots > h.table_slots) {
         if (h.tables) SBPX_HIP(hipFree(h.tables));
         SBPX_HIP(hipMalloc((void**)&h.tables, sizeof(int) * tslots));
@@ -1894,7 +1946,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     // wavefronts per vertex chosen, kernel time)
     static const bool trace = std::getenv("SBPX_TRACE_MH") != nullptr;
     double tr_windows = 0.0, tr_scan = 0.0, tr_evals = 0.0, tr_k = 0.0, tr_ms = 0.0;
-    int tr_chunks = 0, tr_g1 = 0;
+    int tr_chunks = 0, tr_g1 = 0, tr_team = 0;
     hipEvent_t tr_e0 = nullptr, tr_e1 = nullptr;
     if (trace) {
         SBPX_HIP(hipEventCreate(&tr_e0));
@@ -1942,6 +1994,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.wst = h.wst;
         P.wstamp = h.wstamp;
         P.ctrl = h.ctrl;
+        P.team = 1;
+        P.team_hist = h.team_hist;
+        P.team_part = h.team_part;
         P.net = net;
         P.phase = trace ? c.red + 3000 : nullptr;
         bool out_of_room = false;
@@ -1953,6 +2008,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // one entry with long scans take one workgroup of 16 wavefronts. The first chunk at a new
         // number of blocks goes by the longest row. With option 6 = 1 the one-workgroup kernels
         // run instead (16 / G entries per window).
+        // Scans of tens of thousands of slots (rows of a block that holds a large share of the
+        // graph, as a merge can leave behind) take teams: TS workgroups of 8 wavefronts share each
+        // entry, about 8192 slots per member, up to 32 (option 10 fixes TS).
         while (pos < n) {
             int G = c.mh_group;
             const int k_cap = learned ? h.k_max_l : h.k_max;
@@ -1969,9 +2027,30 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                     G = std::max(G, h.g_yield);
                 }
             }
+            int TS = 1;
+            if (wide && c.mh_team >= 2) {
+                TS = c.mh_team;
+            } else if (wide && c.mh_team == 0 && c.mh_group == 0 && G >= 8) {
+                const double sc = fresh ? (double)h.max_cap : h.mean_scan;
+                if (sc >= MH_TEAM_SCAN) {
+                    TS = 2;
+                    while (TS < 32 && TS * (MH_TEAM_SCAN / 2) < sc) TS *= 2;
+                }
+            }
+            TS = std::min(TS, k_cap);
+            if (TS > 1) G = 8;
             const bool wk = wide && G <= 8;  // the window-check kernel
             int K = 1;
-            if (wk) {
+            if (wk && TS > 1) {
+                if (c.mh_wide) {
+                    K = std::min(c.mh_wide, k_cap);
+                    TS = std::min(TS, K);
+                } else {
+                    int E = 1;  // entries per window
+                    while (2 * E * TS <= k_cap && E < 1.5 * h.y_est) E *= 2;
+                    K = E * TS;
+                }
+            } else if (wk) {
                 if (c.mh_wide) {
                     K = std::min(c.mh_wide, k_cap);
                 } else {
@@ -1979,6 +2058,8 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                     while (K < k_cap && K < want) K *= 2;
                 }
             }
+            P.team = TS;
+            h.last_team = TS;
             P.start = pos;
             // a launch covers at least MH_CHUNK entries and four full windows
             P.stop_at = std::min(n, pos + std::max(MH_CHUNK, 4 * (MH_WIDE_WAVES / std::min(G, 8)) * K));
@@ -1988,7 +2069,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (trace) SBPX_HIP(hipEventRecord(tr_e0, c.stream));
             if (wk) {
                 void* args[] = {&P};
-                const void* kern = wide_kernel(G, learned);
+                const void* kern = wide_kernel(G, learned, TS > 1);
                 if (K > 1) {  // every workgroup resident at once: the grid barriers need them all
                     const hipError_t e =
                         hipLaunchCooperativeKernel(kern, dim3(K), dim3(MH_WIDE_WAVES * WAVE), args, 0, c.stream);
@@ -1999,6 +2080,10 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                         (void)hipGetLastError();
                         h.k_max = h.k_max_l = 1;
                         K = 1;
+                        TS = 1;
+                        P.team = 1;
+                        h.last_team = 1;
+                        kern = wide_kernel(G, learned, false);
                         h.coop_refused++;
                     }
                 }
@@ -2034,6 +2119,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                 tr_k += K * sc[2];
                 tr_chunks++;
                 tr_g1 += G == 1;
+                tr_team += TS > 1;
             }
             const int advanced = stat[0] - pos;
             pos = stat[0];
@@ -2047,7 +2133,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (sc[2] > 0.0 && advanced > 0) {
                 const double y = (double)advanced / sc[2];
                 // windows that ran full say only that longer ones may stand: ask for twice as many
-                const double full = wk ? (double)(K * (MH_WIDE_WAVES / G)) : (double)(MH_WAVES / G);
+                const double full = wk ? (double)(K / TS * (MH_WIDE_WAVES / G)) : (double)(MH_WAVES / G);
                 h.y_est = y >= 0.75 * full ? 2.0 * full : y;
                 if (!wk) {
                     if (y >= 0.75 * (double)(MH_WAVES / G)) h.g_yield = std::max(1, G / 2);
@@ -2081,9 +2167,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         std::fprintf(stderr, "mh phases (wide kernel, ms): evaluate %.2f checks %.2f commits %.2f (%.0f) barrier %.2f\n",
                      ph[0] * 1e-5, ph[1] * 1e-5, ph[2] * 1e-5, ph[4], ph[3] * 1e-5);
         std::fprintf(stderr,
-                     "mh pass: B=%d n=%d chunks=%d (G=1 in %d) windows=%.0f entries/window=%.1f "
+                     "mh pass: B=%d n=%d chunks=%d (G=1 in %d, teams in %d) windows=%.0f entries/window=%.1f "
                      "workgroups/window=%.2f evals=%.0f slots/eval=%.0f kernel=%.2f ms\n",
-                     B, n, tr_chunks, tr_g1, tr_windows, tr_windows > 0 ? n / tr_windows : 0.0,
+                     B, n, tr_chunks, tr_g1, tr_team, tr_windows, tr_windows > 0 ? n / tr_windows : 0.0,
                      tr_windows > 0 ? tr_k / tr_windows : 0.0, tr_evals, tr_evals > 0 ? tr_scan / tr_evals : 0.0,
                      tr_ms);
         (void)hipEventDestroy(tr_e0);

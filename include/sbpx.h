@@ -96,7 +96,12 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            (number of batches, key width, kernel variants) with every size in device memory,
  *            and replayed for any subset of the graphs; the evaluation launches are persistent
  *            workgroups that take vertices from per-bin work counters. The states are the same
- *            either way. Rounds replayed from the graph report no per-kernel times. */
+ *            either way. Rounds replayed from the graph report no per-kernel times.
+ * option 10 = workgroups sharing one entry of the sequential Metropolis-Hastings pass's windows
+ *            (a team of 8-wavefront workgroups, each scanning a slice of the rows): 0 to follow
+ *            the mean row/column slots an evaluation scans (teams from 16384 slots, about 8192
+ *            per member, up to 32; default), 1 never, or a power of two up to 32 (then 8
+ *            wavefronts per member). Every choice applies the same moves in the same order. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

@@ -225,11 +225,12 @@ def test_learned_sequential_pass_keeps_the_blockmodel_consistent(algo, group, ma
         c.sweep(code, 1, 31, 9)
 
 
-@pytest.mark.parametrize("option,values", [("group", (1, 2, 4, 8, 16)), ("wide", (0, 2, 8, 512))])
+@pytest.mark.parametrize("option,values", [("group", (1, 2, 4, 8, 16)), ("wide", (0, 2, 8, 512)),
+                                           ("team", (1, 2, 8, 32))])
 def test_learned_sequential_pass_same_moves_however_evaluated(option, values):
     """The learned sequential pass gives the same chain whether one wavefront or a group of 2 to 16
-    evaluates each vertex, and whatever number of workgroups shares a window of speculative
-    evaluations: the proposals are keyed by vertex and pass, the scores come from the state the
+    evaluates each vertex, whatever number of workgroups shares a window of speculative
+    evaluations, and whether teams of 2 to 32 workgroups share each entry: the proposals are keyed by vertex and pass, the scores come from the state the
     vertex's step starts from (the window checks cover what the scorer reads), and delta MDL is equal
     up to the order of its FP64 sums, so every accept decision and the final state agree."""
     m = sbpx()
@@ -245,6 +246,9 @@ def test_learned_sequential_pass_same_moves_however_evaluated(option, values):
         c.set_option(m.OPT_PROPOSAL, 2)
         if option == "group":
             c.set_option(m.OPT_MH_GROUP, value)
+        elif option == "team":
+            c.set_option(m.OPT_MH_GROUP, 8)
+            c.set_option(m.OPT_MH_TEAM, value)
         else:
             c.set_option(m.OPT_MH_GROUP, 1)
             c.set_option(m.OPT_MH_WIDE, value)

@@ -369,13 +369,29 @@ def test_sequential_sweeps_resume_after_table_pool_runs_out(algo, group, wide):
     assert max(launches) > 2, "the pool never ran out"
 
 
-def sequential_case(algo, group, wide, headroom=0):
+@pytest.mark.parametrize("algo,wide,team", [("mh", 0, 2), ("mh", 4, 4), ("hybrid", 64, 8), ("mh", 0, 32),
+                                             ("hybrid", 0, 16), ("mh", 512, 2)])
+def test_sequential_sweeps_with_workgroup_teams_match_oracle(algo, wide, team):
+    """Teams of 2 to 32 workgroups sharing each window entry (each scans a slice of the proposal's
+    row and of the delta-MDL rows, member 0 reads the neighbour blocks' cells by lookups and adds
+    the slices' sums) give the oracle's pass: the same moves and state as one vertex at a time."""
+    sequential_case(algo, 8, wide, team=team)
+
+
+def test_sequential_sweeps_with_teams_resume_after_table_pool_runs_out():
+    """A team pass that runs out of table room resumes at the same queue entry, as the others do."""
+    launches = sequential_case("mh", 8, 16, headroom=48, team=4)
+    assert max(launches) > 2, "the pool never ran out"
+
+
+def sequential_case(algo, group, wide, headroom=0, team=0):
     m = sbpx()
     n = 1000
     arrays, truth = gen_arrays(n, 8, seed=6)
     c = ctx_for(arrays)
     c.set_option(m.OPT_MH_GROUP, group)
     c.set_option(m.OPT_MH_WIDE, wide)
+    c.set_option(m.OPT_MH_TEAM, team)
     launches = []
     g = O.Graph(n, *arrays)
     rng = np.random.default_rng(7)

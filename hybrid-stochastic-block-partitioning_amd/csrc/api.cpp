@@ -122,6 +122,7 @@ void free_graph_buffers(Ctx& c) {
     dfree(&h.wstamp);
     dfree(&h.team_hist);
     dfree(&h.team_part);
+    dfree(&h.team_arrivals);
     if (h.wide) (void)hipFree(h.wide);
     if (h.wst) (void)hipFree(h.wst);
     h = MhState();

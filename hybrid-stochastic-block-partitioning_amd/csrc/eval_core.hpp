@@ -265,6 +265,7 @@ struct MoveResult {
     int d_ss, d_rr, d_sr, d_rs;
     int nuniq;
     long long nnz_rs;  // nonzeros read in rows r, s and columns r, s
+    int tslots;        // slots of the row or column of t the proposal's selection scans (hash rows)
 };
 
 // Insert (or find) block b; returns its slot, or -1 when the table is full.
@@ -489,6 +490,7 @@ __device__ __forceinline__ MoveResult eval_move(const GraphView& G, const BlockS
         const int total = trow + S.din[t];
         const int target = (int)mulhi32(dr.x1, (uint32_t)total);
         s = M.template sample<W>(t, target, trow);
+        if constexpr (View::kSlots) R.tslots = target < trow ? M.nnz_row(t) : M.nnz_col(t);
     }
     R.s = s;
     R.u = u;

@@ -17,6 +17,18 @@ namespace sbpx {
 constexpr int RED_D = 6;   // per-wavefront double partials: dS, pf, pb, and the mixture's pf0, pb0
 constexpr int RED_I = 12;  // per-wavefront int partials
 
+// One entry's rows and columns r and s for the workgroup-shared scan (eval_group with CG > 1): hash
+// segment bases and lengths (row s, row r, column s, column r), r, s, its neighbour-block table,
+// what the unchanged-cell test needs, the degrees before and after the move, and the cells among
+// {r, s} the scan finds.
+struct ScanShare {
+    int b[4], L[4];
+    int r, s, cap, flags, d_sr, d_rs;
+    int dout_s, dout_r, ndout_s, ndout_r, din_s, din_r, ndin_s, ndin_r;
+    int* tab;
+    int m[4];  // m_ss, m_rr, m_sr, m_rs
+};
+
 // An entry evaluated by one workgroup: no team.
 struct NoTeam {
     static constexpr bool on = false;
@@ -37,13 +49,19 @@ struct NoTeam {
 
 // GU: row slots per lane per trip of the row/column pass (the proposal's row scans take 8, or 2
 // with the learned mixture).
-template <int G, class View, bool LRN = false, int GU = 2, class Team = NoTeam>
+// CG > 1: the workgroup holds CG groups (every wavefront runs eval_group), and their rows and
+// columns r and s are scanned together: the union of the CG entries' slots is spread over all of
+// the workgroup's lanes, so an entry with long rows does not hold the others at the barriers that
+// follow. Each lane keeps one partial sum per entry; the partials are added in wavefront order, so
+// the result does not depend on timing.
+template <int G, class View, bool LRN = false, int GU = 2, class Team = NoTeam, int CG = 0>
 __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const View& M, int v, int* tab, int cap,
                                  uint64_t seed, uint32_t call, uint32_t tag, int given_s, double given_u,
                                  bool propose_only, int* hist, double (*redd)[RED_D], int (*redi)[RED_I],
                                  double lw = -1.0, const float* lz = nullptr, const int* lcand = nullptr,
                                  double lqf_in = 0.0, double lqb_in = 0.0, const Team& team = Team{}) {
     static_assert(!Team::on || (View::kSlots && G == 8), "teams: hash rows, one 8-wavefront group per workgroup");
+    static_assert(CG <= 1 || (View::kSlots && !Team::on && CG <= 4), "shared scans: hash rows, up to 4 groups");
     constexpr int GS = G * WAVE;
     const int lane = lane_id();
     const int wave = threadIdx.x / WAVE;
@@ -165,6 +183,7 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
         const int* svals = row ? M.rvals : M.cvals;
         int slo = sample ? (row ? M.rbase[samp_t] : M.cbase[samp_t]) : 0;
         int shi = sample ? slo + (row ? M.rcap[samp_t] : M.ccap[samp_t]) : 0;
+        R.tslots = shi - slo;
         if constexpr (Team::on) {  // this member's slice of the row
             const int per = (shi - slo + team.size - 1) / team.size;
             const int lo = slo + team.rank * per;
@@ -293,7 +312,189 @@ __device__ MoveResult eval_group(const GraphView& Gr, const BlockState& S, const
     int m_ss = 0, m_rr = 0, m_sr = 0, m_rs = 0;
     double acc = 0.0;
     long long live = 0;
-    if (ev) {
+    if constexpr (CG > 1) {
+        __shared__ ScanShare sh[CG > 1 ? CG : 1];
+        __shared__ double sh_acc[CG > 1 ? CG * G : 1][CG > 1 ? CG : 1];
+        __shared__ int sh_live[CG > 1 ? CG * G : 1][CG > 1 ? CG : 1];
+        const int grp = w0 / G;
+        if (gl == 0) {
+            ScanShare& E = sh[grp];
+            if (ev) {
+                E.b[0] = M.rbeg(s);
+                E.L[0] = M.rend(s) - E.b[0];
+                E.b[1] = M.rbeg(r);
+                E.L[1] = M.rend(r) - E.b[1];
+                E.b[2] = M.cbeg(s);
+                E.L[2] = M.cend(s) - E.b[2];
+                E.b[3] = M.cbeg(r);
+                E.L[3] = M.cend(r) - E.b[3];
+            } else {
+                for (int q = 0; q < 4; ++q) E.b[q] = E.L[q] = 0;
+            }
+            E.r = r;
+            E.s = s;
+            E.cap = cap;
+            E.tab = tab;
+            E.flags = (t_ss ? 1 : 0) | (t_rr ? 2 : 0) | (t_sr ? 4 : 0) | (t_rs ? 8 : 0);
+            E.d_sr = d_sr;
+            E.d_rs = d_rs;
+            E.dout_s = dout_s;
+            E.dout_r = dout_r;
+            E.ndout_s = ndout_s;
+            E.ndout_r = ndout_r;
+            E.din_s = din_s;
+            E.din_r = din_r;
+            E.ndin_s = ndin_s;
+            E.ndin_r = ndin_r;
+            for (int q = 0; q < 4; ++q) E.m[q] = 0;
+        }
+        __syncthreads();  // every entry's scan described
+        int off[CG + 1];
+        off[0] = 0;
+#pragma unroll
+        for (int e = 0; e < CG; ++e) off[e + 1] = off[e] + sh[e].L[0] + sh[e].L[1] + sh[e].L[2] + sh[e].L[3];
+        const int TT = off[CG];
+        constexpr int NT = CG * G * WAVE;
+        double ae[CG];
+        int le[CG];
+#pragma unroll
+        for (int e = 0; e < CG; ++e) {
+            ae[e] = 0.0;
+            le[e] = 0;
+        }
+        for (int i0 = (int)threadIdx.x; i0 < TT; i0 += GU * NT) {
+            int cc[GU], mm[GU], sg[GU], en[GU], dsel[GU];
+#pragma unroll
+            for (int q = 0; q < GU; ++q) {
+                const int i = i0 + q * NT;
+                cc[q] = -1;
+                mm[q] = 0;
+                sg[q] = -1;
+                en[q] = 0;
+                if (i < TT) {
+                    int e = 0;
+#pragma unroll
+                    for (int k = 1; k < CG; ++k) e += i >= off[k] ? 1 : 0;
+                    en[q] = e;
+                    const ScanShare& E = sh[e];
+                    int j = i - off[e];
+                    if (j < E.L[0]) {
+                        sg[q] = 0;
+                        cc[q] = M.rkey(E.b[0] + j);
+                        mm[q] = M.rval(E.b[0] + j);
+                    } else if ((j -= E.L[0]) < E.L[1]) {
+                        sg[q] = 1;
+                        cc[q] = M.rkey(E.b[1] + j);
+                        mm[q] = M.rval(E.b[1] + j);
+                    } else if ((j -= E.L[1]) < E.L[2]) {
+                        sg[q] = 2;
+                        cc[q] = M.ckey(E.b[2] + j);
+                        mm[q] = M.cval(E.b[2] + j);
+                    } else {
+                        j -= E.L[2];
+                        sg[q] = 3;
+                        cc[q] = M.ckey(E.b[3] + j);
+                        mm[q] = M.cval(E.b[3] + j);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < GU; ++q) {
+                dsel[q] = -1;
+                const int seg = sg[q], c = cc[q], m = mm[q];
+                if (seg < 0 || c < 0 || m == 0) continue;
+                ScanShare& E = sh[en[q]];
+#pragma unroll
+                for (int k = 0; k < CG; ++k) le[k] += k == en[q] ? 1 : 0;
+                const int er = E.r, es = E.s, ecap = E.cap, fl = E.flags;
+                int* const et = E.tab;
+                bool unchanged = true;
+                if (seg == 0) {
+                    if (c == es) {
+                        E.m[0] = m;
+                        unchanged = (fl & 1) == 0;
+                    } else if (c == er) {
+                        E.m[2] = m;
+                        unchanged = (fl & 4) == 0 || E.d_sr == 0;
+                    } else {
+                        const int sl = tab_find(et, ecap, c);
+                        if (sl >= 0) {
+                            et[3 * ecap + sl] = m;
+                            unchanged = et[ecap + sl] == 0;
+                        }
+                    }
+                } else if (seg == 1) {
+                    if (c == er) {
+                        E.m[1] = m;
+                        unchanged = (fl & 2) == 0;
+                    } else if (c == es) {
+                        E.m[3] = m;
+                        unchanged = (fl & 8) == 0 || E.d_rs == 0;
+                    } else {
+                        const int sl = tab_find(et, ecap, c);
+                        if (sl >= 0) {
+                            et[5 * ecap + sl] = m;
+                            unchanged = et[ecap + sl] == 0;
+                        }
+                    }
+                } else {
+                    if (c == er || c == es) continue;
+                    const int sl = tab_find(et, ecap, c);
+                    if (sl >= 0) {
+                        et[(seg == 2 ? 4 : 6) * ecap + sl] = m;
+                        unchanged = et[2 * ecap + sl] == 0;
+                    }
+                }
+                if (unchanged) dsel[q] = c;
+            }
+            int dv[GU];
+#pragma unroll
+            for (int q = 0; q < GU; ++q) dv[q] = dsel[q] >= 0 ? (sg[q] < 2 ? S.din[dsel[q]] : S.dout[dsel[q]]) : 0;
+#pragma unroll
+            for (int q = 0; q < GU; ++q) {
+                if (dsel[q] < 0) continue;
+                const ScanShare& E = sh[en[q]];
+                const int seg = sg[q], c = dsel[q], d = dv[q];
+                const float mf = fi(mm[q]);
+                double term;
+                if (seg < 2) {
+                    const int xo = seg == 0 ? E.dout_s : E.dout_r, nxo = seg == 0 ? E.ndout_s : E.ndout_r;
+                    const int nd = c == E.r ? E.ndin_r : (c == E.s ? E.ndin_s : d);
+                    const double t0 = (double)cell_entropy(mf, fi(d), fi(xo));
+                    term = t0 - (double)cell_entropy(mf, fi(nd), fi(nxo));
+                } else {
+                    const int xi = seg == 2 ? E.din_s : E.din_r, nxi = seg == 2 ? E.ndin_s : E.ndin_r;
+                    const double t0 = (double)cell_entropy(mf, fi(xi), fi(d));
+                    term = t0 - (double)cell_entropy(mf, fi(nxi), fi(d));
+                }
+#pragma unroll
+                for (int k = 0; k < CG; ++k)
+                    if (k == en[q]) ae[k] += term;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < CG; ++e) {
+            const double a = wave_sum(ae[e]);
+            const long long l = wave_sum_ll(le[e]);
+            if (lane == 0) {
+                sh_acc[wave][e] = a;
+                sh_live[wave][e] = (int)l;
+            }
+        }
+        __syncthreads();  // the entries' partials and cells among {r, s} visible
+        double tot = 0.0;
+        long long lv = 0;
+        for (int w = 0; w < CG * G; ++w) {
+            tot += sh_acc[w][grp];
+            lv += sh_live[w][grp];
+        }
+        acc = gl == 0 ? tot : 0.0;
+        live = gl == 0 ? lv : 0;
+        m_ss = sh[grp].m[0];
+        m_rr = sh[grp].m[1];
+        m_sr = sh[grp].m[2];
+        m_rs = sh[grp].m[3];
+    } else if (ev) {
         const int b0 = M.rbeg(s), L0 = M.rend(s) - b0;
         const int b1 = M.rbeg(r), L1 = M.rend(r) - b1;
         const int b2 = M.cbeg(s), L2 = M.cend(s) - b2;

@@ -178,6 +178,7 @@ struct MhState {
     int* touch = nullptr;   // [N] serial of the last commit that moved a neighbour of each vertex
     int max_cap = 0;        // slots of the longest row or column when the tables were built
     double mean_scan = -1.0;  // mean row/column slots scanned per evaluation in the last chunk
+    double mean_prop = 0.0;   // mean slots of the proposal's row per evaluation (window kernel)
     int last_B = -1;          // number of blocks when mean_scan was measured
     int g_yield = 1;  // wavefronts per vertex the last chunk's window yield calls for
     double y_est = 16.0;  // queue entries a window advanced in the last chunk
@@ -190,6 +191,7 @@ struct MhState {
     int* ctrl = nullptr;                   // its barrier counter and window control words
     int* team_hist = nullptr;     // [k_max * 512] radix histograms of workgroup teams ([2][team][256] per entry)
     double* team_part = nullptr;  // [k_max * 2] their partial delta-MDL sums and scanned slots
+    int* team_arrivals = nullptr;  // [k_max] their barrier arrival counters
     int last_team = 1;            // workgroups per entry in the last chunk
     int* assign0 = nullptr;  // [N] the assignment the block-matrix snapshot was built from
     int* d_all = nullptr;   // 0..N-1

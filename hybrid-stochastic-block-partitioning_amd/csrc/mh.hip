@@ -139,10 +139,12 @@ struct MhParams {
     unsigned long long* wstamp;     // [N] (window serial << 32 | entry) of the accepted entries
     int* ctrl;                      // [0] barrier arrivals, [1] first entry that does not hold,
                                     // [2] next queue position, [3] stop reason
-    // workgroups per window entry (mh_wide_kernel<8, *, true>), their histograms and partial sums
+    // workgroups per window entry (mh_wide_kernel<8, *, true>), their histograms, partial sums and
+    // arrival counters
     int team;
     int* team_hist;
     double* team_part;
+    int* team_arrivals;
     // the learned mixture (LEARNED kernels only): the network and the degree bound
     LearnedNet net;
     // optional phase clock of mh_wide_kernel (workgroup 0, 100 MHz ticks): evaluation and first
@@ -1008,7 +1010,7 @@ __global__ void __launch_bounds__(MH_WAVES * WAVE) mh_group_kernel(MhParams P) {
 // bookkeeping and the commit stay out of scratch memory) and two workgroups share a compute unit.
 constexpr int MH_WIDE_WAVES = 8;
 constexpr int MH_WIDE_MAX = 512;  // workgroups (co-resident: at most two per compute unit)
-constexpr int MH_TEAM_SCAN = 16384;  // mean slots scanned per evaluation from which entries take teams
+constexpr int MH_TEAM_SCAN = 8192;  // mean slots scanned per evaluation from which entries take teams
 constexpr int MH_WIDE_SLOTS = MH_WIDE_MAX * MH_WIDE_WAVES;
 // Accepted entries a window tracks; a window with more ends before the next one (its commits
 // and checks see only these).
@@ -1022,6 +1024,9 @@ struct WideSlot {
     CommitInfo ci;
     double dS;
     long long bytes, nnz_rs;
+    int tslots;  // slots of the proposal's row or column of t
+    int ticks;   // with the phase clock: the evaluation's duration (100 MHz ticks)
+    int ticks0;  // of which the learned scoring before it
 };
 
 // The n-th barrier of the launch (n >= 1) completes when ctrl[0] reaches n * gridDim.x. rel: this
@@ -1054,28 +1059,52 @@ __device__ bool grid_barrier(int* ctrl, int n, int* s_ok, bool rel, bool acq) {
     return *s_ok != 0;
 }
 
-// The team of workgroups that evaluates one window entry (eval_group's Team): its exchanges are
-// barriers over the whole launch, which every workgroup takes the same number of times per window.
-// After a barrier that timed out the others are skipped (the window loop then stops).
+// The team of workgroups that evaluates one window entry (eval_group's Team). Its exchanges are
+// barriers among the team's members only: an arrival counter per entry (zeroed before each launch),
+// the n-th exchange waiting for n * size arrivals, with the same release/acquire fences and bounded
+// wait as grid_barrier. After a barrier that timed out the others are skipped (the window loop
+// then stops).
 struct WideTeam {
     static constexpr bool on = true;
     int rank, size;
     int* hist;
     double* part;
-    int* ctrl;
-    int* nbar;
+    int* arrivals;  // this entry's counter
+    int* nsync;     // exchanges so far in this launch (the same in every member)
     int* s_ok;
     __device__ bool sync() const {
         if (*s_ok == 0) return false;
-        return grid_barrier(ctrl, ++*nbar, s_ok, true, true);
+        const int target = ++*nsync * size;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            atomicAdd(arrivals, 1);
+            int ok = 1;
+            for (int spin = 0; __hip_atomic_load(arrivals, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target;
+                 ++spin) {
+                if (spin >= BARRIER_SPINS) {
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            *s_ok = ok;
+        }
+        __syncthreads();
+        return *s_ok != 0;
     }
 };
 
-__global__ void wide_init_kernel(int* ctrl) {
-    ctrl[0] = 0;
-    ctrl[1] = INT32_MAX;
-    ctrl[2] = 0;
-    ctrl[3] = 0;
+// Resets the window kernel's control words and the first `teams` team arrival counters.
+__global__ void wide_init_kernel(int* ctrl, int* team_arrivals, int teams) {
+    if (threadIdx.x == 0) {
+        ctrl[0] = 0;
+        ctrl[1] = INT32_MAX;
+        ctrl[2] = 0;
+        ctrl[3] = 0;
+    }
+    for (int i = threadIdx.x; i < teams; i += blockDim.x) team_arrivals[i] = 0;
 }
 
 // GW: wavefronts per entry (1, 2, 4 or 8; a workgroup holds 8 / GW entries, evaluated by
@@ -1118,7 +1147,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
     auto& s_dS = s_w.w.dS;
     __shared__ int s_wcnt[MH_WIDE_WAVES], s_cut;
     __shared__ long long s_red[MH_WIDE_WAVES][4];
-    __shared__ double s_redd[MH_WIDE_WAVES][2];
+    __shared__ double s_redd[MH_WIDE_WAVES][3];
     // the learned mixture's candidates, their counts and scores, per entry; with GW > 1 also the
     // branch the group's first wavefront drew
     __shared__ int s_lc[LEARNED ? NG : 1][MLP_CAND], s_lcnt[LEARNED ? NG : 1][MLP_CAND];
@@ -1144,16 +1173,20 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
     const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[g]};
     const bool lead = blockIdx.x == 0;
     long long att = 0, evs = 0, mvs = 0, bytes_sum = 0;
-    double dsum = 0.0, scan_sum = 0.0, scan_count = 0.0, windows = 0.0;
+    double dsum = 0.0, scan_sum = 0.0, scan_count = 0.0, scan_prop = 0.0, windows = 0.0;
     int pos = P.start;
     int stopped = 0;
     int serial = P.status[4];
     unsigned wserial = (unsigned)P.status[9];
     int nbar = 0;
+    [[maybe_unused]] int nsync = 0;  // team exchanges (TEAM)
     if (threadIdx.x == 0) s_ok = 1;
     __syncthreads();
     const bool clocked = P.phase != nullptr && blockIdx.x == 0 && threadIdx.x == 0;
-    double ph[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // phases (4), commits, then per window the slowest evaluation: its ticks, delta-MDL slots,
+    // proposal row slots and vertex degree, and the windows counted
+    double ph[11] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    __shared__ long long s_tmax[MH_WIDE_WAVES];
     unsigned long long tclk = clocked ? wall_clock64() : 0ull;
     auto tick = [&](int k) {
         if (clocked) {
@@ -1172,6 +1205,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
         int cap = 16;
         int* tab = gtab;
         const int v = active ? P.list[pos + slot] : -1;
+        const unsigned long long te0 = P.phase != nullptr ? wall_clock64() : 0ull;
         if (active) {
             cap = vertex_table_cap(G, v);
             if (cap <= LCAP) tab = s_u.tab[g * GW];
@@ -1204,6 +1238,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             }
             __syncthreads();
             const bool on = s_lon[g] != 0;
+            if (P.phase != nullptr && leader && lane == 0) P.wide[slot].ticks0 = (int)(wall_clock64() - te0);
             if constexpr (GW == 1) {
                 if (active)
                     R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
@@ -1211,28 +1246,29 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                                                        0.0, 0.0, on ? s_lz[g] : nullptr, on ? s_lc[g] : nullptr);
             } else if constexpr (TEAM) {
                 const WideTeam tm{rank, TS, P.team_hist + (size_t)slot * 2 * TS * 256, P.team_part + (size_t)slot * 2 * TS,
-                                  P.ctrl, &nbar, &s_ok};
+                                  P.team_arrivals + slot, &nsync, &s_ok};
                 R = eval_group<GW, HashView, true, 2, WideTeam>(
                     G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1, on ? s_lgu[g] : 0.0, false,
                     lds_hist[g], redd, redi, on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
                     on ? s_lc[g] : nullptr, 0.0, 0.0, tm);
             } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
-                R = eval_group<GW, HashView, true>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1,
-                                                   on ? s_lgu[g] : 0.0, false, lds_hist[g], redd, redi,
-                                                   on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
-                                                   on ? s_lc[g] : nullptr);
+                R = eval_group<GW, HashView, true, 2, NoTeam, (NG > 1 ? NG : 0)>(
+                    G, S, M, v, tab, cap, P.seed, P.call, P.tag, on ? s_lgs[g] : -1, on ? s_lgu[g] : 0.0, false,
+                    lds_hist[g], redd, redi, on ? (double)LEARN_W_PLAIN : -1.0, on ? s_lz[g] : nullptr,
+                    on ? s_lc[g] : nullptr);
             }
         } else if constexpr (GW == 1) {
             if (active) R = eval_move<HashView, MH_UNROLL>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false);
         } else if constexpr (TEAM) {
             const WideTeam tm{rank, TS, P.team_hist + (size_t)slot * 2 * TS * 256, P.team_part + (size_t)slot * 2 * TS,
-                              P.ctrl, &nbar, &s_ok};
+                              P.team_arrivals + slot, &nsync, &s_ok};
             R = eval_group<GW, HashView, false, 2, WideTeam>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false,
                                                              lds_hist[g], redd, redi, -1.0, nullptr, nullptr, 0.0, 0.0,
                                                              tm);
-        } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers)
-            R = eval_group<GW, HashView>(G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g], redd,
-                                         redi);
+        } else {  // every wavefront of the workgroup takes part (eval_group's workgroup barriers); the
+                  // workgroup's entries scan their rows together
+            R = eval_group<GW, HashView, false, 2, NoTeam, (NG > 1 ? NG : 0)>(
+                G, S, M, v, tab, cap, P.seed, P.call, P.tag, -1, 0.0, false, lds_hist[g], redd, redi);
         }
         if constexpr (TEAM) {
             if (!s_ok) break;  // a team barrier timed out
@@ -1253,6 +1289,8 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             o.dS = R.dS;
             o.bytes = (active && P.bytes != nullptr) ? eval_bytes(P, R) : 0;
             o.nnz_rs = R.nnz_rs;
+            o.tslots = R.tslots;
+            o.ticks = P.phase != nullptr ? (int)(wall_clock64() - te0) : 0;
             if (active && R.status == 2) P.wstamp[R.v] = ((unsigned long long)wserial << 32) | (unsigned)slot;
         }
         // every entry's record and table, read by every workgroup's list and checks
@@ -1384,7 +1422,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             // counts over the entries that stand
             const int end = s_end;
             long long a = 0, e = 0, by = 0;
-            double sc = 0.0, scn = 0.0;
+            double sc = 0.0, scn = 0.0, sp = 0.0;
             for (int i = threadIdx.x; i < end; i += MH_WIDE_WAVES * WAVE) {
                 const WideSlot& o = P.wide[i];
                 if (o.st == 3) continue;
@@ -1393,6 +1431,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                 by += o.bytes;
                 if (o.st >= 1) {
                     sc += (double)o.nnz_rs;
+                    sp += (double)o.tslots;
                     scn += 1.0;
                 }
             }
@@ -1400,6 +1439,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             e = wave_sum_ll(e);
             by = wave_sum_ll(by);
             sc = wave_sum(sc);
+            sp = wave_sum(sp);
             scn = wave_sum(scn);
             if (lane == 0) {
                 s_red[wave][0] = a;
@@ -1407,15 +1447,44 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                 s_red[wave][2] = by;
                 s_redd[wave][0] = sc;
                 s_redd[wave][1] = scn;
+                s_redd[wave][2] = sp;
+            }
+            __syncthreads();
+            if (P.phase != nullptr) {  // the window's slowest evaluation (every entry evaluated)
+                long long best = -1;
+                for (int i = threadIdx.x; i < Wn; i += MH_WIDE_WAVES * WAVE) {
+                    const long long key = ((long long)P.wide[i].ticks << 20) | i;
+                    best = key > best ? key : best;
+                }
+                for (int off = WAVE / 2; off > 0; off >>= 1) {
+                    const long long y = __shfl_xor(best, off, WAVE);
+                    best = y > best ? y : best;
+                }
+                if (lane == 0) s_tmax[wave] = best;
             }
             __syncthreads();
             if (threadIdx.x == 0) {
+                if (P.phase != nullptr) {
+                    long long best = -1;
+                    for (int w = 0; w < MH_WIDE_WAVES; ++w) best = s_tmax[w] > best ? s_tmax[w] : best;
+                    if (best >= 0) {
+                        const int i = (int)(best & ((1 << 20) - 1));
+                        const int u = P.list[pos + i];
+                        ph[5] += (double)(best >> 20);
+                        ph[6] += (double)P.wide[i].nnz_rs;
+                        ph[7] += (double)P.wide[i].tslots;
+                        ph[8] += (double)((G.out_ptr[u + 1] - G.out_ptr[u]) + (G.in_ptr[u + 1] - G.in_ptr[u]));
+                        ph[9] += 1.0;
+                        ph[10] += LEARNED ? (double)P.wide[i].ticks0 : 0.0;
+                    }
+                }
                 for (int w = 0; w < MH_WIDE_WAVES; ++w) {
                     att += s_red[w][0];
                     evs += s_red[w][1];
                     bytes_sum += s_red[w][2];
                     scan_sum += s_redd[w][0];
                     scan_count += s_redd[w][1];
+                    scan_prop += s_redd[w][2];
                 }
                 P.ctrl[1] = INT32_MAX;
                 P.ctrl[2] = pos + end;
@@ -1437,6 +1506,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
         if (P.bytes != nullptr && bytes_sum > 0) atomicAdd((unsigned long long*)P.bytes, (unsigned long long)bytes_sum);
         atomicAdd(&P.scan[0], scan_sum);
         atomicAdd(&P.scan[1], scan_count);
+        atomicAdd(&P.scan[3], scan_prop);
         atomicAdd(&P.scan[2], windows);
         P.status[0] = pos;
         P.status[1] = stopped;
@@ -1445,6 +1515,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
         if (clocked) {
             for (int k = 0; k < 4; ++k) P.phase[k] += ph[k];
             P.phase[4] += (double)mvs;
+            for (int k = 5; k < 11; ++k) P.phase[k] += ph[k];
         }
         atomicAdd((unsigned long long*)&P.counters[0], (unsigned long long)att);
         atomicAdd((unsigned long long*)&P.counters[1], (unsigned long long)evs);
@@ -1917,6 +1988,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         SBPX_HIP(hipMalloc((void**)&h.ctrl, sizeof(int) * 4));
         SBPX_HIP(hipMalloc((void**)&h.team_hist, sizeof(int) * 512 * (size_t)std::max(h.k_max, h.k_max_l)));
         SBPX_HIP(hipMalloc((void**)&h.team_part, sizeof(double) * 2 * (size_t)std::max(h.k_max, h.k_max_l)));
+        SBPX_HIP(hipMalloc((void**)&h.team_arrivals, sizeof(int) * (size_t)std::max(h.k_max, h.k_max_l)));
         SBPX_HIP(hipMalloc((void**)&h.wstamp, sizeof(unsigned long long) * std::max(c.g.N, 1)));
         SBPX_HIP(hipMemsetAsync(h.wstamp, 0, sizeof(unsigned long long) * std::max(c.g.N, 1), c.stream));
         SBPX_HIP(hipMemsetAsync(h.status + 9, 0, sizeof(int), c.stream));
@@ -1941,17 +2013,19 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToDevice, c.stream));
     int pos = 0;
     int rounds = 0;
-    double* scan = c.red + 2052;  // [3] scanned slots, evaluations and windows of the current chunk
+    // [4] delta-MDL row/column slots scanned, evaluations, windows, and proposal row slots of the
+    // current chunk
+    double* scan = c.red + 2052;
     // SBPX_TRACE_MH=1: one line per pass on stderr (windows, entries per window, workgroups and
     // wavefronts per vertex chosen, kernel time)
     static const bool trace = std::getenv("SBPX_TRACE_MH") != nullptr;
-    double tr_windows = 0.0, tr_scan = 0.0, tr_evals = 0.0, tr_k = 0.0, tr_ms = 0.0;
+    double tr_windows = 0.0, tr_scan = 0.0, tr_prop = 0.0, tr_evals = 0.0, tr_k = 0.0, tr_ms = 0.0;
     int tr_chunks = 0, tr_g1 = 0, tr_team = 0;
     hipEvent_t tr_e0 = nullptr, tr_e1 = nullptr;
     if (trace) {
         SBPX_HIP(hipEventCreate(&tr_e0));
         SBPX_HIP(hipEventCreate(&tr_e1));
-        SBPX_HIP(hipMemsetAsync(c.red + 3000, 0, 5 * sizeof(double), c.stream));
+        SBPX_HIP(hipMemsetAsync(c.red + 3000, 0, 11 * sizeof(double), c.stream));
     }
     while (pos < n) {
         // each restart doubles the pool headroom, so the number of restarts is logarithmic
@@ -1997,6 +2071,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         P.team = 1;
         P.team_hist = h.team_hist;
         P.team_part = h.team_part;
+        P.team_arrivals = h.team_arrivals;
         P.net = net;
         P.phase = trace ? c.red + 3000 : nullptr;
         bool out_of_room = false;
@@ -2010,10 +2085,15 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // run instead (16 / G entries per window).
         // Scans of tens of thousands of slots (rows of a block that holds a large share of the
         // graph, as a merge can leave behind) take teams: TS workgroups of 8 wavefronts share each
-        // entry, about 8192 slots per member, up to 32 (option 10 fixes TS).
+        // entry, about 4096 slots per member, up to 32 (option 10 fixes TS).
         while (pos < n) {
             int G = c.mh_group;
             const int k_cap = learned ? h.k_max_l : h.k_max;
+            // slots an evaluation visits in the window kernel: the delta-MDL rows once, the
+            // proposal's row once per pass of its radix selection
+            int nbits = 1;
+            while (nbits < 31 && (1 << nbits) < B) ++nbits;
+            const double visits = h.mean_scan + ((nbits + 7) / 8) * h.mean_prop;
             const bool fresh = h.last_B != B || h.mean_scan < 0.0;
             if (fresh) h.y_est = std::min(h.y_est, 16.0);
             if (G == 0) {
@@ -2030,8 +2110,8 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             int TS = 1;
             if (wide && c.mh_team >= 2) {
                 TS = c.mh_team;
-            } else if (wide && c.mh_team == 0 && c.mh_group == 0 && G >= 8) {
-                const double sc = fresh ? (double)h.max_cap : h.mean_scan;
+            } else if (wide && c.mh_team == 0 && c.mh_group == 0) {
+                const double sc = fresh ? (double)h.max_cap : visits;
                 if (sc >= MH_TEAM_SCAN) {
                     TS = 2;
                     while (TS < 32 && TS * (MH_TEAM_SCAN / 2) < sc) TS *= 2;
@@ -2063,8 +2143,8 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             P.start = pos;
             // a launch covers at least MH_CHUNK entries and four full windows
             P.stop_at = std::min(n, pos + std::max(MH_CHUNK, 4 * (MH_WIDE_WAVES / std::min(G, 8)) * K));
-            SBPX_HIP(hipMemsetAsync(scan, 0, 3 * sizeof(double), c.stream));
-            if (wk) wide_init_kernel<<<1, 1, 0, c.stream>>>(h.ctrl);
+            SBPX_HIP(hipMemsetAsync(scan, 0, 4 * sizeof(double), c.stream));
+            if (wk) wide_init_kernel<<<1, 256, 0, c.stream>>>(h.ctrl, h.team_arrivals, TS > 1 ? K / TS : 0);
             if (timed) c.timer_begin();
             if (trace) SBPX_HIP(hipEventRecord(tr_e0, c.stream));
             if (wk) {
@@ -2104,7 +2184,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (timed) c.timer_end(KERNEL_MH);
             if (trace) SBPX_HIP(hipEventRecord(tr_e1, c.stream));
             int stat[10];
-            double sc[3];
+            double sc[4];
             SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipMemcpyAsync(sc, scan, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipStreamSynchronize(c.stream));
@@ -2115,6 +2195,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
                 tr_ms += ms;
                 tr_windows += sc[2];
                 tr_scan += sc[0];
+                tr_prop += sc[3];
                 tr_evals += sc[1];
                 tr_k += K * sc[2];
                 tr_chunks++;
@@ -2125,6 +2206,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             pos = stat[0];
             if (sc[1] > 0.0) {
                 h.mean_scan = sc[0] / sc[1];
+                h.mean_prop = sc[3] / sc[1];
                 h.last_B = B;
             }
             // Entries a window advanced. For the one-workgroup kernels (16 / G vertices per
@@ -2161,16 +2243,23 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         if (out_of_room) h.headroom *= 2;
     }
     if (trace) {
-        double ph[5];
+        double ph[11];
         SBPX_HIP(hipMemcpyAsync(ph, c.red + 3000, sizeof(ph), hipMemcpyDeviceToHost, c.stream));
         SBPX_HIP(hipStreamSynchronize(c.stream));
         std::fprintf(stderr, "mh phases (wide kernel, ms): evaluate %.2f checks %.2f commits %.2f (%.0f) barrier %.2f\n",
                      ph[0] * 1e-5, ph[1] * 1e-5, ph[2] * 1e-5, ph[4], ph[3] * 1e-5);
+        if (ph[9] > 0.0)
+            std::fprintf(stderr,
+                         "mh slowest evaluation per window: %.1f us (learned scoring %.1f us), delta-MDL slots %.0f, "
+                         "proposal slots %.0f, degree %.0f (%.0f windows)\n",
+                         ph[5] / ph[9] * 1e-2, ph[10] / ph[9] * 1e-2, ph[6] / ph[9], ph[7] / ph[9], ph[8] / ph[9],
+                         ph[9]);
         std::fprintf(stderr,
                      "mh pass: B=%d n=%d chunks=%d (G=1 in %d, teams in %d) windows=%.0f entries/window=%.1f "
-                     "workgroups/window=%.2f evals=%.0f slots/eval=%.0f kernel=%.2f ms\n",
+                     "workgroups/window=%.2f evals=%.0f slots/eval=%.0f proposal slots/eval=%.0f kernel=%.2f ms\n",
                      B, n, tr_chunks, tr_g1, tr_team, tr_windows, tr_windows > 0 ? n / tr_windows : 0.0,
                      tr_windows > 0 ? tr_k / tr_windows : 0.0, tr_evals, tr_evals > 0 ? tr_scan / tr_evals : 0.0,
+                     tr_evals > 0 ? tr_prop / tr_evals : 0.0,
                      tr_ms);
         (void)hipEventDestroy(tr_e0);
         (void)hipEventDestroy(tr_e1);

@@ -73,20 +73,22 @@ __device__ void write_result(const EvalParams& P, const MoveResult& R, int v, in
     }
 }
 
-template <int U = 2, int W = WAVE>
+// LRN: the learned mixture's draws and probabilities (P.lw >= 0), in instantiations of their own so
+// that the plain ones carry none of the mixture's registers.
+template <int U = 2, int W = WAVE, bool LRN = false>
 __device__ __forceinline__ void eval_vertex(const EvalParams& P, const int* list, int* tab, int cap, int idx) {
     const int v = list[idx];
     int gs = P.given_s ? P.given_s[idx] : -1;
     double gu = P.given_u ? P.given_u[idx] : 0.0;
     double lqf = 0.0, lqb = 0.0;
-    if (P.lw >= 0.0f) {
+    if constexpr (LRN) {
         gs = P.lps[v];
         gu = P.lpu[v];
         lqf = P.lqf[v];
         lqb = P.lqb[v];
     }
     MoveResult R = eval_move<CsrView, U, W>(P.G, P.S, P.M, v, tab, cap, P.seed, P.call, P.tag, gs, gu,
-                                            P.propose_only != 0, (double)P.lw, lqf, lqb);
+                                            P.propose_only != 0, LRN ? (double)P.lw : -1.0, lqf, lqb);
     if (group_lane<W>() == 0) write_result(P, R, v, idx);
 }
 
@@ -103,48 +105,49 @@ __device__ __forceinline__ int next_entries(int* queue, int take) {
 }
 
 // One wavefront per vertex, its neighbour-block table in LDS (CAP slots); WAVES wavefronts per
-// workgroup share nothing but the launch. U: row slots per lane per trip of the scans. In graph
-// mode the workgroups stay resident and each wavefront takes vertices from the work counter
-// until the list is done.
-template <int CAP, int WAVES, int U>
+// workgroup share nothing but the launch. U: row slots per lane per trip of the scans. PERS (graph
+// mode): the workgroups stay resident and each wavefront takes vertices from the work counter
+// until the list is done; a separate instantiation, since the loop's live state would cost the
+// one-vertex launches registers (and occupancy).
+template <int CAP, int WAVES, int U, bool PERS = false, bool LRN = false>
 __global__ void __launch_bounds__(WAVES * WAVE) eval_bin_kernel(EvalParams P) {
     __shared__ int tab[WAVES][7 * CAP];
     const int wave = threadIdx.x / WAVE;
-    const int n = P.n.get();
-    const int* list = eval_list(P);
-    if (P.queue) {
+    if constexpr (PERS) {
+        const int n = P.n.get();
+        const int* list = eval_list(P);
         for (int idx = next_entries(P.queue, 1); idx < n; idx = next_entries(P.queue, 1))
-            eval_vertex<U>(P, list, tab[wave], CAP, idx);
-        return;
+            eval_vertex<U, WAVE, LRN>(P, list, tab[wave], CAP, idx);
+    } else {
+        const int idx = blockIdx.x * WAVES + wave;
+        if (idx >= P.n.v) return;
+        eval_vertex<U, WAVE, LRN>(P, P.list, tab[wave], CAP, idx);
     }
-    const int idx = blockIdx.x * WAVES + wave;
-    if (idx >= n) return;
-    eval_vertex<U>(P, list, tab[wave], CAP, idx);
 }
 
 // Low-degree vertices packed WAVE / W to a wavefront: a group of W lanes per vertex, each with its
 // own CAP-slot table in LDS. The evaluation is a chain of dependent gathers, so more vertices in
 // flight per CU (not more lanes per vertex) is what shortens the launch.
-template <int CAP, int WAVES, int U, int W>
+template <int CAP, int WAVES, int U, int W, bool PERS = false, bool LRN = false>
 __global__ void __launch_bounds__(WAVES * WAVE) eval_sub_kernel(EvalParams P) {
     constexpr int PER = WAVES * (WAVE / W);
     constexpr int SUBS = WAVE / W;  // vertices per wavefront
     __shared__ int tab[PER][7 * CAP];
     const int sub = threadIdx.x / W;
-    const int n = P.n.get();
-    const int* list = eval_list(P);
-    if (P.queue) {
+    if constexpr (PERS) {
+        const int n = P.n.get();
+        const int* list = eval_list(P);
         // the wavefront takes SUBS entries at a time, one per lane group; the groups run the same
         // trip count (the evaluation's wave-level steps need every lane)
         for (int base = next_entries(P.queue, SUBS); base < n; base = next_entries(P.queue, SUBS)) {
             const int idx = base + (lane_id() / W);
-            if (idx < n) eval_vertex<U, W>(P, list, tab[sub], CAP, idx);
+            if (idx < n) eval_vertex<U, W, LRN>(P, list, tab[sub], CAP, idx);
         }
-        return;
+    } else {
+        const int idx = blockIdx.x * PER + sub;
+        if (idx >= P.n.v) return;
+        eval_vertex<U, W, LRN>(P, P.list, tab[sub], CAP, idx);
     }
-    const int idx = blockIdx.x * PER + sub;
-    if (idx >= n) return;
-    eval_vertex<U, W>(P, list, tab[sub], CAP, idx);
 }
 
 // Vertices of the highest degrees: G wavefronts (the whole workgroup) share one vertex, its
@@ -211,19 +214,32 @@ static int eval_grid(const Dyn& n, int per) {
     return (int)std::max<long>(1, n.p ? std::min<long>(g, PERSIST_GRID) : g);
 }
 
-template <int CAP, int WAVES>
-static void launch_bin(const EvalParams& P, int variant, hipStream_t st) {
+// PERS: graph mode (a work counter); LRN: the learned mixture. Graph mode runs batches of graphs,
+// which the learned proposal does not, so the two are never combined.
+template <int CAP, int WAVES, bool PERS, bool LRN>
+static void launch_bin_as(const EvalParams& P, int variant, hipStream_t st) {
     if constexpr (CAP == BIN_CAP[0]) {
         if (variant == 2) {
             constexpr int W = 16, SW = 4, PER = SW * (WAVE / W);
-            eval_sub_kernel<CAP, SW, 2, W><<<eval_grid(P.n, PER), SW * WAVE, 0, st>>>(P);
+            eval_sub_kernel<CAP, SW, 2, W, PERS, LRN><<<eval_grid(P.n, PER), SW * WAVE, 0, st>>>(P);
             return;
         }
     }
     if (variant == 1)
-        eval_bin_kernel<CAP, WAVES, 4><<<eval_grid(P.n, WAVES), WAVES * WAVE, 0, st>>>(P);
+        eval_bin_kernel<CAP, WAVES, 4, PERS, LRN><<<eval_grid(P.n, WAVES), WAVES * WAVE, 0, st>>>(P);
     else
-        eval_bin_kernel<CAP, WAVES, 2><<<eval_grid(P.n, WAVES), WAVES * WAVE, 0, st>>>(P);
+        eval_bin_kernel<CAP, WAVES, 2, PERS, LRN><<<eval_grid(P.n, WAVES), WAVES * WAVE, 0, st>>>(P);
+}
+template <int CAP, int WAVES>
+static void launch_bin(const EvalParams& P, int variant, hipStream_t st) {
+    if (P.queue) {
+        if (P.lw >= 0.0f) throw StateError("graph-mode evaluation with the learned proposal");
+        launch_bin_as<CAP, WAVES, true, false>(P, variant, st);
+    } else if (P.lw >= 0.0f) {
+        launch_bin_as<CAP, WAVES, false, true>(P, variant, st);
+    } else {
+        launch_bin_as<CAP, WAVES, false, false>(P, variant, st);
+    }
 }
 
 static EvalParams make_params(Ctx& c) {

@@ -179,19 +179,19 @@ __device__ void eval_group_entry(const EvalParams& P, const int* list, int idx, 
     if (threadIdx.x == 0) write_result(P, R, v, idx);
 }
 
-// In graph mode the workgroup stays resident and takes vertices from the work counter; the two
-// barriers around the shared index keep the previous vertex's LDS in use until every wavefront
-// has finished it.
-template <int G, int CAP, bool LRN = false>
+// PERS (graph mode, an instantiation of its own): the workgroup stays resident and takes vertices
+// from the work counter; the two barriers around the shared index keep the previous vertex's LDS
+// in use until every wavefront has finished it.
+template <int G, int CAP, bool LRN = false, bool PERS = false>
 __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
     __shared__ int tab_lds[CAP > 0 ? 7 * CAP : 1];
     __shared__ int hist[256];
     __shared__ double redd[G][RED_D];
     __shared__ int redi[G][RED_I];
-    __shared__ int s_idx;
-    const int n = P.n.get();
-    const int* list = eval_list(P);
-    if (P.queue) {
+    if constexpr (PERS) {
+        __shared__ int s_idx;
+        const int n = P.n.get();
+        const int* list = eval_list(P);
         for (;;) {
             if (threadIdx.x == 0) s_idx = atomicAdd(P.queue, 1);
             __syncthreads();
@@ -200,10 +200,20 @@ __global__ void __launch_bounds__(G * WAVE) eval_group_kernel(EvalParams P) {
             if (idx >= n) return;
             eval_group_entry<G, CAP, LRN>(P, list, idx, tab_lds, hist, redd, redi);
         }
+    } else {
+        const int idx = blockIdx.x;
+        if (idx >= P.n.v) return;
+        eval_group_entry<G, CAP, LRN>(P, P.list, idx, tab_lds, hist, redd, redi);
     }
-    const int idx = blockIdx.x;
-    if (idx >= n) return;
-    eval_group_entry<G, CAP, LRN>(P, list, idx, tab_lds, hist, redd, redi);
+}
+
+// The group kernel for one launch: graph mode (a work counter) or one workgroup per vertex.
+template <int G, int CAP, bool LRN = false>
+static void launch_group_eval(const EvalParams& P, int grid, hipStream_t st) {
+    if (P.queue)
+        eval_group_kernel<G, CAP, LRN, true><<<grid, G * WAVE, 0, st>>>(P);
+    else
+        eval_group_kernel<G, CAP, LRN, false><<<grid, G * WAVE, 0, st>>>(P);
 }
 
 // Workgroups of an evaluation launch that covers `per` vertices each: every vertex its own
@@ -289,6 +299,7 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
     P.pool_off = L.huge_offsets;
     if (L.learned) {
         if (L.bin >= NUM_LDS_BINS) throw StateError("learned proposals are evaluated by the LDS-table bins only");
+        if (L.queue) throw StateError("graph-mode evaluation with the learned proposal");
         P.lps = c.mlp.ps;
         P.lpu = L.learned_u ? L.learned_u : c.ev_u;
         P.lqf = c.mlp.qf;
@@ -305,16 +316,16 @@ void launch_eval(Ctx& c, const EvalLaunch& L) {
         case 2: launch_bin<256, 4>(P, L.variant, c.stream); break;
         case 3:
             if (L.variant == 1 && L.learned) eval_group_kernel<4, 512, true><<<g1, 4 * WAVE, 0, c.stream>>>(P);
-            else if (L.variant == 1) eval_group_kernel<4, 512><<<g1, 4 * WAVE, 0, c.stream>>>(P);
+            else if (L.variant == 1) launch_group_eval<4, 512>(P, g1, c.stream);
             else launch_bin<512, 2>(P, 0, c.stream);
             break;
         case 4:
             if (L.variant == 1 && L.learned) eval_group_kernel<4, 1024, true><<<g1, 4 * WAVE, 0, c.stream>>>(P);
-            else if (L.variant == 1) eval_group_kernel<4, 1024><<<g1, 4 * WAVE, 0, c.stream>>>(P);
+            else if (L.variant == 1) launch_group_eval<4, 1024>(P, g1, c.stream);
             else launch_bin<1024, 1>(P, 0, c.stream);
             break;
-        case 5: eval_group_kernel<4, 4096><<<g1, 4 * WAVE, 0, c.stream>>>(P); break;
-        default: eval_group_kernel<16, 0><<<g1, 16 * WAVE, 0, c.stream>>>(P); break;
+        case 5: launch_group_eval<4, 4096>(P, g1, c.stream); break;
+        default: launch_group_eval<16, 0>(P, g1, c.stream); break;
     }
     SBPX_HIP(hipGetLastError());
 }

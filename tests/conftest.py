@@ -25,5 +25,5 @@ def toy():
 
 @pytest.fixture(scope="session")
 def crosschecks():
-    with open(os.path.join(HERE, "golden", "survey_crosschecks.json")) as f:
+    with open(os.path.join(HERE, "golden", "crosscheck_moves.json")) as f:
         return json.load(f)

@@ -1,6 +1,6 @@
 """The C++ layer called as the reference's code and tests call it (tests/cpp/toy_caller.cpp, built
 by build.py): on the reference's toy graph, the known answers of test/entropy_test.cpp,
-test/finetune_test.cpp, test/block_merge_test.cpp and test/blockmodel_test.cpp, the survey's
+test/finetune_test.cpp, test/block_merge_test.cpp and test/blockmodel_test.cpp, independent
 cross-check values for delta MDL and the Hastings correction, move_vertex's degrees with the
 total-degree drift, and a whole sbp::stochastic_block_partition against the oracle's."""
 import os

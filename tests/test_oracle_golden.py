@@ -1,6 +1,6 @@
 """Pins the CPU oracle against the reference's own known-answer tests (test/*.cpp, transcribed in
 tests/golden/toy_example.json), against closed forms and invariants, and against the
-independent cross-check values in tests/golden/survey_crosschecks.json."""
+independent cross-check values in tests/golden/crosscheck_moves.json."""
 import itertools
 import math
 

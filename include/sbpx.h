@@ -72,8 +72,7 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            in its last bits between them.
  * option 4 = proposal: 0 the reference's, 1 the learned mixture in the batched evaluation
  *            (see sbpx_learned_init), 2 also in the sequential Metropolis-Hastings pass, where
- *            each evaluation scores the vertex's candidates on the current state (needs option
- *            6 != 1).
+ *            each window's entries are scored on the current state (needs option 6 != 1).
  * option 5 = block matrix after a batch of moves: 1 (default) updated from the moved vertices'
  *            edges when they are at most a quarter of the edges, else rebuilt by a full sort;
  *            0 always rebuilt. Both give the same arrays.
@@ -297,8 +296,9 @@ sbpx_status sbpx_batch_merge(sbpx_ctx* ctx, int32_t n, const int32_t* graphs, co
  * out[i] = log of the float whose bit pattern is lo + i, for i < n. */
 sbpx_status sbpx_selftest_logf(sbpx_ctx* ctx, uint32_t lo, uint32_t n, float* out);
 
-/* Synthetic degree-corrected SBM graph (host only; no GPU needed). Returns CSR/CSC sized by
- * the caller after a first call with NULL outputs reports N and E. */
+/* Synthetic degree-corrected SBM graph (host only; no GPU needed; no counterpart in the
+ * reference, whose graphs come from graph-tool files). Returns CSR/CSC sized by the caller after
+ * a first call with NULL outputs reports N and E. */
 typedef struct sbpx_dcsbm_params {
     int32_t num_vertices;
     int32_t num_blocks;

@@ -179,6 +179,7 @@ struct MhState {
     int max_cap = 0;        // slots of the longest row or column when the tables were built
     double mean_scan = -1.0;  // mean row/column slots scanned per evaluation in the last chunk
     double mean_prop = 0.0;   // mean slots of the proposal's row per evaluation (window kernel)
+    double mean_wmax = 0.0;   // mean over windows of the most row slots one evaluation visited
     int last_B = -1;          // number of blocks when mean_scan was measured
     int g_yield = 1;  // wavefronts per vertex the last chunk's window yield calls for
     double y_est = 16.0;  // queue entries a window advanced in the last chunk

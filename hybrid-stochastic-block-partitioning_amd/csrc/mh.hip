@@ -1173,7 +1173,9 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
     const HashView M{P.rbase, P.rcap, P.rkeys, P.rvals, P.cbase, P.ccap, P.ckeys, P.cvals, P.B, lds_hist[g]};
     const bool lead = blockIdx.x == 0;
     long long att = 0, evs = 0, mvs = 0, bytes_sum = 0;
-    double dsum = 0.0, scan_sum = 0.0, scan_count = 0.0, scan_prop = 0.0, windows = 0.0;
+    double dsum = 0.0, scan_sum = 0.0, scan_count = 0.0, scan_prop = 0.0, scan_wmax = 0.0, windows = 0.0;
+    int radix_passes = 1;  // passes of the proposal's radix selection (8 bits each)
+    while (radix_passes < 4 && (1LL << (8 * radix_passes)) < (long long)P.B) ++radix_passes;
     int pos = P.start;
     int stopped = 0;
     int serial = P.status[4];
@@ -1423,6 +1425,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
             const int end = s_end;
             long long a = 0, e = 0, by = 0;
             double sc = 0.0, scn = 0.0, sp = 0.0;
+            long long wm = 0;  // the most row slots an evaluation of the window visited
             for (int i = threadIdx.x; i < end; i += MH_WIDE_WAVES * WAVE) {
                 const WideSlot& o = P.wide[i];
                 if (o.st == 3) continue;
@@ -1433,11 +1436,17 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                     sc += (double)o.nnz_rs;
                     sp += (double)o.tslots;
                     scn += 1.0;
+                    const long long visits = o.nnz_rs + (long long)radix_passes * o.tslots;
+                    wm = visits > wm ? visits : wm;
                 }
             }
             a = wave_sum_ll(a);
             e = wave_sum_ll(e);
             by = wave_sum_ll(by);
+            for (int off = WAVE / 2; off > 0; off >>= 1) {
+                const long long y = __shfl_xor(wm, off, WAVE);
+                wm = y > wm ? y : wm;
+            }
             sc = wave_sum(sc);
             sp = wave_sum(sp);
             scn = wave_sum(scn);
@@ -1445,6 +1454,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                 s_red[wave][0] = a;
                 s_red[wave][1] = e;
                 s_red[wave][2] = by;
+                s_red[wave][3] = wm;
                 s_redd[wave][0] = sc;
                 s_redd[wave][1] = scn;
                 s_redd[wave][2] = sp;
@@ -1485,7 +1495,9 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
                     scan_sum += s_redd[w][0];
                     scan_count += s_redd[w][1];
                     scan_prop += s_redd[w][2];
+                    wm = s_red[w][3] > wm ? s_red[w][3] : wm;
                 }
+                scan_wmax += (double)wm;
                 P.ctrl[1] = INT32_MAX;
                 P.ctrl[2] = pos + end;
                 P.ctrl[3] = s_stop;
@@ -1507,6 +1519,7 @@ __global__ void __launch_bounds__(MH_WIDE_WAVES * WAVE) mh_wide_kernel(MhParams 
         atomicAdd(&P.scan[0], scan_sum);
         atomicAdd(&P.scan[1], scan_count);
         atomicAdd(&P.scan[3], scan_prop);
+        atomicAdd(&P.scan[4], scan_wmax);
         atomicAdd(&P.scan[2], windows);
         P.status[0] = pos;
         P.status[1] = stopped;
@@ -2013,9 +2026,9 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
     SBPX_HIP(hipMemcpyAsync(h.assign0, c.bm.assign, sizeof(int) * c.g.N, hipMemcpyDeviceToDevice, c.stream));
     int pos = 0;
     int rounds = 0;
-    // [4] delta-MDL row/column slots scanned, evaluations, windows, and proposal row slots of the
-    // current chunk
-    double* scan = c.red + 2052;
+    // [5] delta-MDL row/column slots scanned, evaluations, windows, proposal row slots, and the sum
+    // over windows of the most row slots one of its evaluations visited, in the current chunk
+    double* scan = c.red + 2064;
     // SBPX_TRACE_MH=1: one line per pass on stderr (windows, entries per window, workgroups and
     // wavefronts per vertex chosen, kernel time)
     static const bool trace = std::getenv("SBPX_TRACE_MH") != nullptr;
@@ -2085,7 +2098,8 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
         // run instead (16 / G entries per window).
         // Scans of tens of thousands of slots (rows of a block that holds a large share of the
         // graph, as a merge can leave behind) take teams: TS workgroups of 8 wavefronts share each
-        // entry, about 4096 slots per member, up to 32 (option 10 fixes TS).
+        // entry, about 4096 slots per member of the largest evaluation a window held in the
+        // previous chunk (on average over its windows), up to 32 (option 10 fixes TS).
         while (pos < n) {
             int G = c.mh_group;
             const int k_cap = learned ? h.k_max_l : h.k_max;
@@ -2111,11 +2125,20 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (wide && c.mh_team >= 2) {
                 TS = c.mh_team;
             } else if (wide && c.mh_team == 0 && c.mh_group == 0) {
-                const double sc = fresh ? (double)h.max_cap : visits;
+                // a window waits for its evaluation with the most row slots: its typical size in
+                // the previous chunk sets the team (from the window kernel's counts; else the mean)
+                const double sc = fresh ? (double)h.max_cap : (h.mean_wmax > 0.0 ? h.mean_wmax : visits);
                 if (sc >= MH_TEAM_SCAN) {
                     TS = 2;
                     while (TS < 32 && TS * (MH_TEAM_SCAN / 2) < sc) TS *= 2;
                 }
+                // a team shrinks the window to k_cap / TS entries: only as far as the entries the
+                // windows advance still fit
+                const double want = std::max(1.0, 1.5 * h.y_est);
+                int ts_cap = 1;
+                while (2.0 * ts_cap * want <= (double)k_cap) ts_cap *= 2;
+                TS = std::min(TS, ts_cap);
+                if (TS < 2) TS = 1;
             }
             TS = std::min(TS, k_cap);
             if (TS > 1) G = 8;
@@ -2143,7 +2166,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             P.start = pos;
             // a launch covers at least MH_CHUNK entries and four full windows
             P.stop_at = std::min(n, pos + std::max(MH_CHUNK, 4 * (MH_WIDE_WAVES / std::min(G, 8)) * K));
-            SBPX_HIP(hipMemsetAsync(scan, 0, 4 * sizeof(double), c.stream));
+            SBPX_HIP(hipMemsetAsync(scan, 0, 5 * sizeof(double), c.stream));
             if (wk) wide_init_kernel<<<1, 256, 0, c.stream>>>(h.ctrl, h.team_arrivals, TS > 1 ? K / TS : 0);
             if (timed) c.timer_begin();
             if (trace) SBPX_HIP(hipEventRecord(tr_e0, c.stream));
@@ -2184,7 +2207,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (timed) c.timer_end(KERNEL_MH);
             if (trace) SBPX_HIP(hipEventRecord(tr_e1, c.stream));
             int stat[10];
-            double sc[4];
+            double sc[5];
             SBPX_HIP(hipMemcpyAsync(stat, h.status, sizeof(stat), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipMemcpyAsync(sc, scan, sizeof(sc), hipMemcpyDeviceToHost, c.stream));
             SBPX_HIP(hipStreamSynchronize(c.stream));
@@ -2207,6 +2230,7 @@ void mh_sequential(Ctx& c, const int* d_list, int n, int max_deg, uint64_t seed,
             if (sc[1] > 0.0) {
                 h.mean_scan = sc[0] / sc[1];
                 h.mean_prop = sc[3] / sc[1];
+                h.mean_wmax = sc[2] > 0.0 ? sc[4] / sc[2] : 0.0;
                 h.last_B = B;
             }
             // Entries a window advanced. For the one-workgroup kernels (16 / G vertices per

@@ -98,9 +98,10 @@ const char* sbpx_last_error(const sbpx_ctx* ctx);
  *            either way. Rounds replayed from the graph report no per-kernel times.
  * option 10 = workgroups sharing one entry of the sequential Metropolis-Hastings pass's windows
  *            (a team of 8-wavefront workgroups, each scanning a slice of the rows): 0 to follow
- *            the mean row/column slots an evaluation scans (teams from 8192 slots, about 4096
- *            per member, up to 32; default), 1 never, or a power of two up to 32 (then 8
- *            wavefronts per member). Every choice applies the same moves in the same order. */
+ *            the row slots of the largest evaluation per window (teams from 8192 slots, about
+ *            4096 per member, up to 32, as long as the entries the windows advance still fit;
+ *            default), 1 never, or a power of two up to 32 (then 8 wavefronts per member).
+ *            Every choice applies the same moves in the same order. */
 sbpx_status sbpx_set_option(sbpx_ctx* ctx, int option, int64_t value);
 
 /* Graph: deduplicated directed adjacency as int32 CSR (out) and CSC (in), neighbour order as

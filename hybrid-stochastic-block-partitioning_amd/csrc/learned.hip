@@ -145,26 +145,73 @@ struct MixParams {
     float* qb;            // [N] learned reverse probability
 };
 
-// One thread per vertex: draws the mixture's proposal (the branch, then on the learned branch a
-// candidate from the softmax over the candidates other than r, in slot order), or takes the given
-// target, and the net's forward and reverse probabilities of the move from the same scores.
-__global__ void mixture_kernel(MixParams P) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const int v = P.list[i];
-    const int r = P.S.assign[v];
-    const int* cand = P.cand + (long)i * MLP_CAND;
-    const float* z = P.z + (long)i * MLP_OUT;
-    int s = P.s_given ? P.s_given[i] : P.s_plain[v];
-    int any = 0;
-    for (int j = 0; j < MLP_CAND; ++j) any |= cand[j] >= 0 && cand[j] != r;
-    if (any && P.s_given == nullptr) {
+// 32 lanes per vertex, one per candidate slot (two vertices per wavefront): draws the mixture's
+// proposal (the branch, then on the learned branch a candidate from the softmax over the
+// candidates other than r), or takes the given target, and the net's forward and reverse
+// probabilities of the move from the same scores. The maxima are order-free; the softmax sums and
+// the draw's running sum are folded in slot order by every lane of the half, so each value has
+// the bits of cand_prob and learned_pick (which the sequential pass uses).
+__global__ void __launch_bounds__(256) mixture_kernel(MixParams P) {
+    const int lane = lane_id();
+    const int half = lane & 32;  // first lane of this vertex's half-wavefront
+    const int j = lane & 31;     // candidate slot
+    const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 5);
+    const bool on = i < P.n;
+    const int v = on ? P.list[i] : 0;
+    const int r = on ? P.S.assign[v] : -1;
+    const int c = on ? P.cand[(long)i * MLP_CAND + j] : -1;
+    const float zj = on ? P.z[(long)i * MLP_OUT + j] : 0.0f;
+    auto half_max = [&](float x) {
+        for (int off = 16; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, WAVE));
+        return x;
+    };
+    auto half_any = [&](bool b) { return ((__ballot(b) >> half) & 0xFFFFFFFFull) != 0ull; };
+    // forward: the candidates other than r
+    const bool qf_ok = c >= 0 && c != r;
+    const bool any = half_any(qf_ok);
+    const float mx_f = half_max(qf_ok ? zj : -1e30f);
+    const float e_f = qf_ok ? expf(zj - mx_f) : 0.0f;
+    float den_f = 0.0f;
+    for (int k = 0; k < MLP_CAND; ++k) den_f += __shfl(e_f, half + k, WAVE);  // slot order; + 0 is exact
+    int s = on ? (P.s_given ? P.s_given[i] : P.s_plain[v]) : -1;
+    if (any && P.s_given == nullptr && on) {
         const Draw4 dr = philox4x32_10(P.seed, (uint32_t)v, P.call, 0u, TAG_LEARN);
-        if (unit53(dr.x0, dr.x1) >= (double)P.w_plain) s = learned_pick(z, cand, r, unit53(dr.x2, dr.x3));
+        if (unit53(dr.x0, dr.x1) >= (double)P.w_plain) {
+            const float target = (float)unit53(dr.x2, dr.x3) * den_f;
+            float run = 0.0f;
+            int pick = -1;
+            bool done = false;
+            for (int k = 0; k < MLP_CAND; ++k) {
+                const float ek = __shfl(e_f, half + k, WAVE);
+                const int ck = __shfl(c, half + k, WAVE);
+                if (!done && ck >= 0 && ck != r) {
+                    run += ek;
+                    pick = ck;
+                    done = run > target;
+                }
+            }
+            s = pick;
+        }
     }
-    P.ps[v] = s;
-    P.qf[v] = cand_prob(z, cand, r, s);
-    P.qb[v] = s == r ? -1.0f : cand_prob(z, cand, s, r);
+    // the forward probability of s, and the reverse one of r over the candidates other than s
+    const float num_f = (qf_ok && c == s) ? e_f : 0.0f;
+    const bool qb_ok = c >= 0 && c != s;
+    const bool any_b = half_any(qb_ok);
+    const float mx_b = half_max(qb_ok ? zj : -1e30f);
+    const float e_b = qb_ok ? expf(zj - mx_b) : 0.0f;
+    float den_b = 0.0f, nf = 0.0f, nb = 0.0f;
+    for (int k = 0; k < MLP_CAND; ++k) {
+        den_b += __shfl(e_b, half + k, WAVE);
+        nf = fmaxf(nf, __shfl(num_f, half + k, WAVE));
+        const int ck = __shfl(c, half + k, WAVE);
+        const float ebk = __shfl(e_b, half + k, WAVE);
+        if (ck == r && ck >= 0 && ck != s) nb = ebk;
+    }
+    if (on && j == 0) {
+        P.ps[v] = s;
+        P.qf[v] = any ? nf / den_f : -1.0f;
+        P.qb[v] = s == r ? -1.0f : (any_b ? nb / den_b : -1.0f);
+    }
 }
 
 // Appends the accepted moves whose target is a candidate to the training set.
@@ -287,7 +334,7 @@ void learned_mixture(Ctx& c, const int* list, int n, uint64_t seed, uint32_t cal
     P.ps = m.ps;
     P.qf = m.qf;
     P.qb = m.qb;
-    mixture_kernel<<<(n + 255) / 256, 256, 0, c.stream>>>(P);
+    mixture_kernel<<<(n + 7) / 8, 256, 0, c.stream>>>(P);  // 8 vertices per workgroup
     SBPX_HIP(hipGetLastError());
 }
 

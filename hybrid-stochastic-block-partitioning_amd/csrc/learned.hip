@@ -106,6 +106,14 @@ __global__ void __launch_bounds__(FEAT_WAVES * WAVE) features_kernel(FeatParams 
             const long long o = __shfl_xor(best, off, WAVE);
             best = o > best ? o : best;
         }
+        if (best < 0) {  // no block left: the remaining candidates are empty
+            if (lane == 0)
+                for (int u = t; u < MLP_CAND; ++u) {
+                    cand[u] = -1;
+                    cnt[u] = 0;
+                }
+            break;
+        }
         int b = -1, c = 0;
         if (best >= 0) {
             b = 0x7FFFFFFF - (int)(best & 0xFFFFFFFF);

@@ -498,6 +498,15 @@ __device__ bool learned_features_wave(const GraphView& G, const BlockState& S, i
             const long long o = __shfl_xor(wb, off, WAVE);
             wb = o > wb ? o : wb;
         }
+        if (wb < 0) {  // no block left: the remaining candidates are empty
+            if (lane == 0)
+                for (int u = t; u < MLP_CAND; ++u) {
+                    cand[u] = -1;
+                    cnt[u] = 0;
+                }
+            fence_wave();
+            break;
+        }
         int b = -1, c = 0;
         if (wb >= 0) {
             b = 0x7FFFFFFF - (int)(wb & 0xFFFFFFFF);
